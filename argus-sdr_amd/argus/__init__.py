@@ -1,0 +1,6 @@
+"""argus -- Python binding of libargus_hip.so (MI355X per-signal extraction).
+
+Add the package directory (argus-sdr_amd/) to sys.path and `import argus`.
+"""
+from ._lib import ErrUnavailable, GpudError, EXPORTS, LIB_PATH, lib  # noqa: F401
+from . import gpudemod  # noqa: F401

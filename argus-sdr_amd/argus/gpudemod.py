@@ -1,0 +1,229 @@
+"""Host mirror of the reference's gpudemod streaming surface over the C ABI.
+
+Names, argument meaning and error behaviour follow
+internal/demod/gpudemod (batch_runner.go, streaming_types.go,
+stream_state.go, streaming_gpu_stub.go, polyphase.go, cpu_oracle.go):
+
+    r = NewBatchRunner(max_samples, sample_rate)
+    results = r.StreamingExtractGPU(iq_new, jobs)    # list[StreamingExtractResult]
+    r.ResetSignalState(signal_id); r.ResetAllSignalStates(); r.Close()
+
+All work runs in libargus_hip.so; this module only marshals arguments.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from ._lib import (ErrUnavailable, GpudError, check, gpud_streaming_job, gpud_streaming_result, lib, ptr)
+
+EXEC_BATCHED = 0
+EXEC_PER_SIGNAL = 1
+WINDOW_HANN = 0
+WINDOW_HAMMING = 1
+
+
+@dataclass
+class StreamingExtractJob:
+    """streaming_types.go:8-15"""
+    SignalID: int
+    OffsetHz: float
+    Bandwidth: float
+    OutRate: int
+    NumTaps: int
+    ConfigHash: int
+
+
+@dataclass
+class StreamingExtractResult:
+    """streaming_types.go:17-24 (+ the NCO phase the runner carries)."""
+    SignalID: int
+    IQ: np.ndarray
+    Rate: int
+    NOut: int
+    PhaseCount: int
+    HistoryLen: int
+    NCOPhase: float = 0.0
+
+
+@dataclass
+class ExtractStreamState:
+    """streaming_types.go:26-44 (the fields a test inspects)."""
+    SignalID: int
+    ConfigHash: int
+    NCOPhase: float
+    Decim: int
+    PhaseCount: int
+    NumTaps: int
+    ShiftedHistory: np.ndarray = field(default_factory=lambda: np.zeros(0, np.complex64))
+
+
+def Available() -> bool:
+    """gpudemod.go:112-118"""
+    try:
+        n = C.c_int(0)
+        return lib().gpud_device_count(C.byref(n)) == 0 and n.value > 0
+    except ErrUnavailable:
+        return False
+
+
+def StreamingConfigHash(signal_id, offset_hz, bandwidth, out_rate, num_taps, sample_rate) -> int:
+    """streaming_types.go:58-72"""
+    return int(lib().gpud_streaming_config_hash(signal_id, offset_hz, bandwidth, out_rate, num_taps, sample_rate))
+
+
+def ExactIntegerDecimation(sample_rate: int, out_rate: int) -> int:
+    """cpu_oracle.go:155-163; raises ValueError like the Go error return."""
+    d = lib().gpud_exact_integer_decimation(sample_rate, out_rate)
+    if d < 0:
+        raise ValueError(f"streaming polyphase extractor requires integer decimation: "
+                         f"sampleRate={sample_rate} outRate={out_rate}")
+    return d
+
+
+def LowpassFIR(cutoff_hz: float, sample_rate: int, taps: int) -> np.ndarray:
+    """dsp/fir.go:6-28"""
+    out = np.zeros(taps + 1, np.float64)
+    n = lib().gpud_lowpass_fir(cutoff_hz, sample_rate, taps, ptr(out))
+    return out[:n]
+
+
+def PolyphasePhaseLen(base_len: int, decim: int) -> int:
+    """polyphase.go:24-28"""
+    if decim <= 0 or base_len <= 0:
+        return 0
+    return (base_len + decim - 1) // decim
+
+
+def BuildPolyphaseTapsPhaseMajor(base, decim: int) -> np.ndarray:
+    """polyphase.go:6-22"""
+    base = np.ascontiguousarray(base, np.float32)
+    if decim <= 0 or len(base) == 0:
+        return np.zeros(0, np.float32)
+    out = np.zeros(decim * PolyphasePhaseLen(len(base), decim), np.float32)
+    n = lib().gpud_build_polyphase_taps(ptr(base), len(base), decim, ptr(out))
+    return out[:n]
+
+
+def _jobs_array(jobs):
+    arr = (gpud_streaming_job * max(1, len(jobs)))()
+    for i, j in enumerate(jobs):
+        arr[i].signal_id = int(j.SignalID)
+        arr[i].offset_hz = float(j.OffsetHz)
+        arr[i].bandwidth = float(j.Bandwidth)
+        arr[i].out_rate = int(j.OutRate)
+        arr[i].num_taps = int(j.NumTaps)
+        arr[i].config_hash = int(j.ConfigHash) & 0xFFFFFFFFFFFFFFFF
+    return arr
+
+
+class BatchRunner:
+    """batch_runner.go:12-50 (streaming surface)."""
+
+    def __init__(self, max_samples: int, sample_rate: int, exec_mode: int = EXEC_BATCHED,
+                 window: int = WINDOW_HANN):
+        h = C.c_void_p()
+        rc = lib().gpud_runner_create(max_samples, sample_rate, C.byref(h))
+        if rc != 0:
+            raise ErrUnavailable(f"gpud_runner_create failed ({rc})")
+        self._h = h
+        self.sample_rate = sample_rate
+        self.max_samples = max_samples
+        self.SetExecMode(exec_mode)
+        check("gpud_runner_set_window", lib().gpud_runner_set_window(self._h, window))
+
+    # -- lifecycle --------------------------------------------------------
+    def Close(self) -> None:
+        if self._h:
+            lib().gpud_runner_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.Close()
+        except Exception:
+            pass
+
+    def SetExecMode(self, mode: int) -> None:
+        check("gpud_runner_set_exec_mode", lib().gpud_runner_set_exec_mode(self._h, mode))
+
+    def stream(self) -> int:
+        return lib().gpud_runner_stream(self._h)
+
+    # -- streaming extraction ----------------------------------------------
+    def StreamingExtractGPU(self, iq_new, jobs) -> list[StreamingExtractResult]:
+        """streaming_gpu_stub.go:21-26"""
+        if not self._h:
+            raise ErrUnavailable("runner closed")
+        iq = np.ascontiguousarray(iq_new if iq_new is not None else np.zeros(0, np.complex64), np.complex64)
+        n = len(iq)
+        jarr = _jobs_array(jobs)
+        res = (gpud_streaming_result * max(1, len(jobs)))()
+        cap = sum(n // max(1, 1) for _ in jobs) + 2 * len(jobs) + 1  # upper bound (D >= 1)
+        out = np.zeros(max(cap, 1), np.complex64)
+        rc = lib().gpud_runner_streaming_extract(self._h, ptr(iq) if n else None, n, jarr, len(jobs), ptr(out),
+                                                 len(out), res)
+        if rc == -1:
+            raise ValueError("streaming extract rejected the jobs (bad arguments or non-integer decimation)")
+        check("gpud_runner_streaming_extract", rc)
+        results = []
+        for i in range(len(jobs)):
+            r = res[i]
+            iqo = out[r.out_offset:r.out_offset + r.n_out].copy()
+            results.append(StreamingExtractResult(r.signal_id, iqo, r.rate, r.n_out, r.phase_count, r.history_len,
+                                                  r.nco_phase))
+        return results
+
+    StreamingExtractGPUExec = StreamingExtractGPU
+
+    def StreamingExtractDevice(self, iq_dev_ptr: int, n_new: int, jobs):
+        """Device-resident input; returns (device output pointer, raw results).
+        Completion is ordered on self.stream()."""
+        jarr = _jobs_array(jobs)
+        res = (gpud_streaming_result * max(1, len(jobs)))()
+        out = C.c_void_p()
+        check("gpud_runner_streaming_extract_device",
+              lib().gpud_runner_streaming_extract_device(self._h, iq_dev_ptr, n_new, jarr, len(jobs),
+                                                         C.byref(out), res))
+        return out.value, res
+
+    # -- state ------------------------------------------------------------
+    def SignalState(self, signal_id: int) -> ExtractStreamState | None:
+        h = C.c_uint64()
+        ph = C.c_double()
+        dec, pc, nt = C.c_int(), C.c_int(), C.c_int()
+        hist = np.zeros(4096, np.complex64)
+        n = lib().gpud_runner_get_signal_state(self._h, signal_id, C.byref(h), C.byref(ph), C.byref(dec),
+                                               C.byref(pc), C.byref(nt), ptr(hist), len(hist))
+        if n < 0:
+            return None
+        return ExtractStreamState(signal_id, h.value, ph.value, dec.value, pc.value, nt.value, hist[:n].copy())
+
+    def StateIDs(self) -> list[int]:
+        n = lib().gpud_runner_num_states(self._h)
+        ids = np.zeros(max(n, 1), np.int64)
+        lib().gpud_runner_state_ids(self._h, ptr(ids), len(ids))
+        return [int(x) for x in ids[:n]]
+
+    def ResetSignalState(self, signal_id: int) -> None:
+        """stream_state.go:9-16"""
+        check("gpud_runner_reset_signal_state", lib().gpud_runner_reset_signal_state(self._h, signal_id))
+
+    def ResetAllSignalStates(self) -> None:
+        """stream_state.go:18-24"""
+        check("gpud_runner_reset_all_signal_states", lib().gpud_runner_reset_all_signal_states(self._h))
+
+
+def NewBatchRunner(max_samples: int, sample_rate: int, **kw) -> BatchRunner:
+    """batch_runner.go:23-34"""
+    return BatchRunner(max_samples, sample_rate, **kw)
+
+
+__all__ = [
+    "Available", "BatchRunner", "NewBatchRunner", "StreamingExtractJob", "StreamingExtractResult",
+    "ExtractStreamState", "StreamingConfigHash", "ExactIntegerDecimation", "BuildPolyphaseTapsPhaseMajor",
+    "PolyphasePhaseLen", "LowpassFIR", "ErrUnavailable", "GpudError", "EXEC_BATCHED", "EXEC_PER_SIGNAL",
+    "WINDOW_HANN", "WINDOW_HAMMING",
+]

@@ -1,0 +1,208 @@
+/*
+ * argus_gpu.h -- C ABI of libargus_hip.so, the MI355X (gfx950) implementation
+ * of Argus's per-signal extraction and surveillance hot path.
+ *
+ * Conventions (kept from the reference's native export surface,
+ * internal/demod/gpudemod/native/exports.cu:4-12):
+ *   - every function returns int: 0 = OK, negative = argument/capacity error
+ *     (same numbering as the entry point it succeeds), positive = hipError_t;
+ *   - a launch with n <= 0 is a no-op returning 0;
+ *   - complex samples are gpud_float2 {float x, y} == Go complex64;
+ *   - device pointers are plain pointers from gpud_malloc; the library never
+ *     aborts the process.
+ * Names: a successor of a reference export keeps its name with the "_cuda"
+ * suffix replaced by "_hip"; entry points with no reference counterpart use
+ * the gpud_runner_ / gpud_surv_ / gpud_serve_ / gpud_wfm_ prefixes.
+ */
+#ifndef ARGUS_GPU_H
+#define ARGUS_GPU_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GPUD_API __attribute__((visibility("default")))
+
+typedef void* gpud_stream_handle;             /* exports.cu:12 */
+typedef struct gpud_float2 { float x, y; } gpud_float2; /* windows_bridge.go:12 */
+
+/* ------------------------------------------------------------------ */
+/* Device, memory and stream helpers.  The reference's cgo preamble     */
+/* calls cudart for these directly (gpudemod.go:13-31,                  */
+/* windows_bridge.go:50-54); here they sit behind the C ABI so Go never */
+/* includes a HIP header.                                               */
+/* ------------------------------------------------------------------ */
+GPUD_API int gpud_device_count(int* count);                        /* gpudemod.go:112-118 Available() */
+GPUD_API int gpud_malloc(void** ptr, size_t bytes);                /* gpudemod.go:13-16 gpud_cuda_malloc */
+GPUD_API int gpud_free(void* ptr);                                 /* gpudemod.go:17-19 gpud_cuda_free */
+GPUD_API int gpud_host_alloc(void** ptr, size_t bytes);            /* pinned host memory (no reference counterpart) */
+GPUD_API int gpud_host_free(void* ptr);
+GPUD_API int gpud_memcpy_h2d(void* dst, const void* src, size_t bytes); /* gpudemod.go:20-22 */
+GPUD_API int gpud_memcpy_d2h(void* dst, const void* src, size_t bytes); /* gpudemod.go:23-25 */
+GPUD_API int gpud_memcpy_d2d(void* dst, const void* src, size_t bytes);
+GPUD_API int gpud_memset_zero(void* dst, size_t bytes);
+GPUD_API int gpud_device_sync(void);                               /* gpudemod.go:26-28 */
+GPUD_API int gpud_stream_create(gpud_stream_handle* out);          /* exports.cu:18-25 */
+GPUD_API int gpud_stream_destroy(gpud_stream_handle stream);       /* exports.cu:27-30 */
+GPUD_API int gpud_stream_sync(gpud_stream_handle stream);          /* exports.cu:32-35 */
+GPUD_API const char* gpud_version(void);
+/* HIP events for timing on a given stream (no reference counterpart: the
+ * reference has no GPU timing). */
+typedef void* gpud_event_handle;
+GPUD_API int gpud_event_create(gpud_event_handle* ev);
+GPUD_API int gpud_event_destroy(gpud_event_handle ev);
+GPUD_API int gpud_event_record(gpud_event_handle ev, gpud_stream_handle stream);
+GPUD_API int gpud_event_elapsed_ms(gpud_event_handle start, gpud_event_handle stop, float* ms);
+
+/* ------------------------------------------------------------------ */
+/* Primitive launchers: successors of exports.cu's kernels.             */
+/* ------------------------------------------------------------------ */
+/* exports.cu:62-75  out[i] = in[i] * e^{j(phase_start + phase_inc*i)} */
+GPUD_API int gpud_launch_freq_shift_stream_hip(const gpud_float2* in, gpud_float2* out, int n, double phase_inc,
+                                               double phase_start, gpud_stream_handle stream);
+/* exports.cu:77-85 */
+GPUD_API int gpud_launch_freq_shift_hip(const gpud_float2* in, gpud_float2* out, int n, double phase_inc,
+                                        double phase_start);
+/* exports.cu:102-112  n-1 outputs: atan2(Im(conj(x_i) x_{i+1}), Re(...)) */
+GPUD_API int gpud_launch_fm_discrim_hip(const gpud_float2* in, float* out, int n);
+/* exports.cu:163-167  taps for gpud_launch_fir[_stream]_hip, n <= 256, else -1 */
+GPUD_API int gpud_upload_fir_taps_hip(const float* taps, int n);
+/* exports.cu:169-181  causal FIR with the uploaded taps, zero history */
+GPUD_API int gpud_launch_fir_hip(const gpud_float2* in, gpud_float2* out, int n, int num_taps);
+/* exports.cu:183-196 */
+GPUD_API int gpud_launch_fir_stream_hip(const gpud_float2* in, gpud_float2* out, int n, int num_taps,
+                                        gpud_stream_handle stream);
+/* exports.cu:230-244  causal FIR, taps from a device pointer */
+GPUD_API int gpud_launch_fir_v2_stream_hip(const gpud_float2* in, gpud_float2* out, const float* taps, int n,
+                                           int num_taps, gpud_stream_handle stream);
+/* exports.cu:246-257  out[i] = in[i*factor] */
+GPUD_API int gpud_launch_decimate_hip(const gpud_float2* in, gpud_float2* out, int n_out, int factor);
+/* exports.cu:259-271 */
+GPUD_API int gpud_launch_decimate_stream_hip(const gpud_float2* in, gpud_float2* out, int n_out, int factor,
+                                             gpud_stream_handle stream);
+/* exports.cu:284-294  |x| */
+GPUD_API int gpud_launch_am_envelope_hip(const gpud_float2* in, float* out, int n);
+/* exports.cu:314-326  Re(x * e^{j(phase_start + phase_inc*i)}) */
+GPUD_API int gpud_launch_ssb_product_hip(const gpud_float2* in, float* out, int n, double phase_inc,
+                                         double phase_start);
+
+/* exports.cu:356-460  legacy one-shot streaming polyphase (history in/out
+ * arrays).  Errors: -1 bad args, -2 polyphase_len < decim*phase_len,
+ * -3 null in_new, -4 null out, -5 null history_in.  Unlike the reference it
+ * allocates nothing per call. */
+GPUD_API int gpud_launch_streaming_polyphase_prepare_hip(
+    const gpud_float2* in_new, int n_new, const gpud_float2* history_in, int history_len,
+    const float* polyphase_taps, int polyphase_len, int decim, int num_taps, int phase_count_in,
+    double phase_start, double phase_inc, gpud_float2* out, int* n_out, int* phase_count_out,
+    double* phase_end_out, gpud_float2* history_out);
+
+/* exports.cu:533-631  production stateful streaming extractor for one signal:
+ * NCO shift + phase-major polyphase FIR/decimate + history update, one fused
+ * kernel.  history_state holds the shifted history (<= history_cap samples);
+ * the new history is written to history_scratch and copied back on the same
+ * stream.  Errors: -10 missing pointer/arg, -11 negative size, -12
+ * polyphase_len < decim*ceil(num_taps/decim), -13 null in_new or
+ * shifted_new_tmp with n_new > 0, -14 out_cap too small, -15 null out,
+ * -16 null history buffers.  shifted_new_tmp is accepted for ABI
+ * compatibility and not written (the shift is fused).  Runs on the default
+ * stream and returns after the history copy has been queued. */
+GPUD_API int gpud_launch_streaming_polyphase_stateful_hip(
+    const gpud_float2* in_new, int n_new, gpud_float2* shifted_new_tmp, const float* polyphase_taps,
+    int polyphase_len, int decim, int num_taps, gpud_float2* history_state, gpud_float2* history_scratch,
+    int history_cap, int* history_len_io, int* phase_count_state, double* phase_state, double phase_inc,
+    gpud_float2* out, int out_cap, int* n_out);
+
+/* ------------------------------------------------------------------ */
+/* Streaming batch runner: successor of gpudemod.BatchRunner's          */
+/* streaming surface (batch_runner.go:23-34, streaming_gpu_stub.go:21,  */
+/* stream_state.go:9-24).  State (NCO phase, decimator phase, shifted   */
+/* history, taps) stays on the device between calls.                    */
+/* ------------------------------------------------------------------ */
+typedef struct gpud_runner gpud_runner;
+
+/* streaming_types.go:8-15  StreamingExtractJob */
+typedef struct gpud_streaming_job {
+    int64_t  signal_id;
+    double   offset_hz;
+    double   bandwidth;
+    int32_t  out_rate;
+    int32_t  num_taps;     /* <= 0 -> 101 (stream_state.go:50-53) */
+    uint64_t config_hash;  /* a change resets the signal's state (streaming_types.go:46-56) */
+} gpud_streaming_job;
+
+/* streaming_types.go:17-24  StreamingExtractResult (+ where its IQ landed) */
+typedef struct gpud_streaming_result {
+    int64_t signal_id;
+    int32_t rate;
+    int32_t n_out;
+    int32_t phase_count;
+    int32_t history_len;
+    int64_t out_offset;   /* complex-sample offset of this signal's IQ in the output buffer */
+    double  nco_phase;    /* NCO phase carried to the next call, radians in [-pi, pi) */
+} gpud_streaming_result;
+
+enum {
+    GPUD_EXEC_BATCHED = 0,     /* one fused launch for all signals per chunk (default) */
+    GPUD_EXEC_PER_SIGNAL = 1   /* plain form: per-signal upload, launch and sync copies,
+                                  the reference's call pattern (streaming_gpu_native_prepare.go:17-108) */
+};
+
+/* batch_runner.go:23-34 NewBatchRunner(maxSamples, sampleRate).  -1 bad args. */
+GPUD_API int gpud_runner_create(int max_samples, int sample_rate, gpud_runner** out);
+/* batch_runner.go:36-50 Close */
+GPUD_API int gpud_runner_destroy(gpud_runner* r);
+GPUD_API int gpud_runner_set_exec_mode(gpud_runner* r, int mode);
+/* Tap window for the low-pass design of the runner's signals: 0 = Hann
+ * (LowpassFIR, dsp/fir.go:6-28, the default), 1 = Hamming (same sinc). */
+GPUD_API int gpud_runner_set_window(gpud_runner* r, int window);
+/* Kernel timing: while enabled, the runner brackets every fused extraction
+ * launch with HIP events on its stream; gpud_runner_kernel_time waits for
+ * them and returns the summed kernel milliseconds and the launch count since
+ * it was enabled. */
+GPUD_API int gpud_runner_enable_kernel_timing(gpud_runner* r, int enable);
+GPUD_API int gpud_runner_kernel_time(gpud_runner* r, double* total_ms, int* launches);
+/* Stream the runner launches on (created by the runner, default priority). */
+GPUD_API gpud_stream_handle gpud_runner_stream(gpud_runner* r);
+
+/* streaming_gpu_stub.go:21-26 StreamingExtractGPU.  iq is host memory.
+ * Outputs of job i are written to out + results[i].out_offset (assigned in
+ * job order).  Signals absent from jobs are pruned (streaming_gpu_prepare.go:
+ * 40-44).  Errors: -1 bad args / non-integer decimation
+ * (cpu_oracle.go:155-163), -2 n_new > max_samples, -14 out_cap too small. */
+GPUD_API int gpud_runner_streaming_extract(gpud_runner* r, const gpud_float2* iq, int n_new,
+                                           const gpud_streaming_job* jobs, int n_jobs, gpud_float2* out,
+                                           int64_t out_cap, gpud_streaming_result* results);
+/* Same with device-resident input; on return *out_dev points at the packed
+ * device outputs (valid until the next call on this runner).  No host
+ * copies, no synchronisation: completion is ordered on gpud_runner_stream. */
+GPUD_API int gpud_runner_streaming_extract_device(gpud_runner* r, const gpud_float2* iq_dev, int n_new,
+                                                  const gpud_streaming_job* jobs, int n_jobs,
+                                                  gpud_float2** out_dev, gpud_streaming_result* results);
+/* Inspect a signal's carried state (ExtractStreamState, streaming_types.go:26-44).
+ * Returns the history length, -1 if the signal has no state. */
+GPUD_API int gpud_runner_get_signal_state(gpud_runner* r, int64_t signal_id, uint64_t* config_hash,
+                                          double* nco_phase, int* decim, int* phase_count, int* num_taps,
+                                          gpud_float2* history, int hist_cap);
+GPUD_API int gpud_runner_num_states(gpud_runner* r);
+GPUD_API int gpud_runner_state_ids(gpud_runner* r, int64_t* ids, int cap);
+/* stream_state.go:9-16 ResetSignalState */
+GPUD_API int gpud_runner_reset_signal_state(gpud_runner* r, int64_t signal_id);
+/* stream_state.go:18-24 ResetAllSignalStates */
+GPUD_API int gpud_runner_reset_all_signal_states(gpud_runner* r);
+/* streaming_types.go:58-72 StreamingConfigHash (FNV-1a-64) */
+GPUD_API uint64_t gpud_streaming_config_hash(int64_t signal_id, double offset_hz, double bandwidth, int out_rate,
+                                             int num_taps, int sample_rate);
+/* cpu_oracle.go:155-163 ExactIntegerDecimation: decimation or -1 */
+GPUD_API int gpud_exact_integer_decimation(int sample_rate, int out_rate);
+/* dsp/fir.go:6-28 LowpassFIR (Hann); returns the (odd) tap count */
+GPUD_API int gpud_lowpass_fir(double cutoff_hz, int sample_rate, int taps, double* out);
+/* polyphase.go:6-22 BuildPolyphaseTapsPhaseMajor; returns decim*ceil(n/decim) */
+GPUD_API int gpud_build_polyphase_taps(const float* base, int n, int decim, float* out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* ARGUS_GPU_H */

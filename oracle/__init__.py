@@ -1,0 +1,411 @@
+"""oracle -- ctypes wrapper of the CPU restatement of the reference path.
+
+TEST INFRASTRUCTURE.  Only tests/, __graft_entry__.smoke() and bench.py's
+cpu_baseline leg import this; the product (libargus_hip.so and the argus
+Python binding) never does.  Function names follow the reference Go code so
+tests read like the reference's own tests.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass, field
+from pathlib import Path
+
+import numpy as np
+
+_DIR = Path(__file__).resolve().parent
+_LIB_PATH = _DIR / "liboracle.so"
+_lib = None
+
+
+class orc_job(C.Structure):
+    _fields_ = [("signal_id", C.c_int64), ("offset_hz", C.c_double), ("bandwidth", C.c_double),
+                ("out_rate", C.c_int32), ("num_taps", C.c_int32), ("config_hash", C.c_uint64)]
+
+
+class orc_result(C.Structure):
+    _fields_ = [("signal_id", C.c_int64), ("rate", C.c_int32), ("n_out", C.c_int32),
+                ("phase_count", C.c_int32), ("history_len", C.c_int32), ("out_offset", C.c_int64),
+                ("nco_phase", C.c_double)]
+
+
+P, I, D = C.c_void_p, C.c_int, C.c_double
+PI_, PD_ = C.POINTER(C.c_int), C.POINTER(C.c_double)
+_SIGS = {
+    "orc_lowpass_fir": (I, [D, I, I, P]),
+    "orc_lowpass_fir_hamming": (I, [D, I, I, P]),
+    "orc_apply_fir": (None, [P, I, P, I, P]),
+    "orc_decimate": (I, [P, I, I, P]),
+    "orc_freq_shift": (None, [P, I, I, D, P]),
+    "orc_stateful_fir_real": (None, [P, I, P, P, I, P]),
+    "orc_stateful_decim_fir_complex": (I, [P, I, I, P, P, PI_, P, I, P]),
+    "orc_polyphase_phase_len": (I, [I, I]),
+    "orc_build_polyphase": (I, [P, I, I, P]),
+    "orc_exact_decim": (I, [I, I]),
+    "orc_config_hash": (C.c_uint64, [C.c_int64, D, D, I, I, I]),
+    "orc_host_core": (I, [P, I, I, D, PD_, PI_, I, I, P, PI_, P, I, P, I]),
+    "orc_extract_direct": (I, [P, I, D, PD_, PI_, I, I, P, P, PI_, P, I]),
+    "orc_extract_polyphase": (I, [P, I, D, PD_, PI_, I, I, P, P, P, PI_, P, I]),
+    "orc_runner_new": (P, [I]),
+    "orc_runner_free": (None, [P]),
+    "orc_runner_streaming_extract": (I, [P, P, I, C.POINTER(orc_job), I, P, C.c_int64, C.POINTER(orc_result)]),
+    "orc_runner_num_states": (I, [P]),
+    "orc_runner_state_ids": (I, [P, P, I]),
+    "orc_runner_get_state": (I, [P, C.c_int64, C.POINTER(C.c_uint64), PD_, PI_, PI_, PI_, P, I]),
+    "orc_runner_reset_signal": (None, [P, C.c_int64]),
+    "orc_runner_reset_all": (None, [P]),
+    "orc_runner_set_threads": (I, [P, I]),
+    "orc_fm_discrim": (I, [P, I, P]),
+    "orc_fm_discrim_bridged": (I, [P, I, P, PI_, P]),
+    "orc_am_demod": (I, [P, I, P]),
+    "orc_ssb_demod": (I, [P, I, I, I, PD_, P]),
+    "orc_cw_demod": (I, [P, I, I, PD_, P]),
+    "orc_rds_baseband": (I, [P, I, I, P, I, PI_]),
+}
+
+
+def register(name, restype, argtypes):
+    _SIGS[name] = (restype, argtypes)
+    if _lib is not None:
+        f = getattr(_lib, name)
+        f.restype, f.argtypes = restype, argtypes
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        h = C.CDLL(str(_LIB_PATH))
+        for name, (res, args) in _SIGS.items():
+            f = getattr(h, name)
+            f.restype, f.argtypes = res, args
+        _lib = h
+    return _lib
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data
+
+
+def c64(x) -> np.ndarray:
+    return np.ascontiguousarray(x, dtype=np.complex64)
+
+
+# ---------------------------------------------------------------- dsp
+def LowpassFIR(cutoff_hz, sample_rate, taps):
+    out = np.zeros(taps + 1)
+    n = lib().orc_lowpass_fir(cutoff_hz, sample_rate, taps, _p(out))
+    return out[:n]
+
+
+def LowpassFIRHamming(cutoff_hz, sample_rate, taps):
+    out = np.zeros(taps + 1)
+    n = lib().orc_lowpass_fir_hamming(cutoff_hz, sample_rate, taps, _p(out))
+    return out[:n]
+
+
+def ApplyFIR(iq, taps):
+    iq = c64(iq)
+    taps = np.ascontiguousarray(taps, np.float64)
+    out = np.zeros(len(iq), np.complex64)
+    if len(iq) and len(taps):
+        lib().orc_apply_fir(_p(iq), len(iq), _p(taps), len(taps), _p(out))
+    return out
+
+
+def Decimate(iq, factor):
+    iq = c64(iq)
+    out = np.zeros(len(iq) + 1, np.complex64)
+    m = lib().orc_decimate(_p(iq), len(iq), factor, _p(out))
+    return out[:m]
+
+
+def FreqShift(iq, sample_rate, offset_hz):
+    iq = c64(iq)
+    out = np.zeros(len(iq), np.complex64)
+    lib().orc_freq_shift(_p(iq), len(iq), sample_rate, offset_hz, _p(out))
+    return out
+
+
+class StatefulDecimatingFIRComplex:
+    """dsp/decimating_fir.go:6-95"""
+
+    def __init__(self, taps, factor):
+        self.taps = np.ascontiguousarray(taps, np.float64)
+        self.factor = max(1, factor)
+        self.dr = np.zeros(len(self.taps))
+        self.di = np.zeros(len(self.taps))
+        self.phase = C.c_int(0)
+
+    def Process(self, iq):
+        iq = c64(iq)
+        out = np.zeros(len(iq) + 1, np.complex64)
+        m = lib().orc_stateful_decim_fir_complex(_p(self.taps), len(self.taps), self.factor, _p(self.dr),
+                                                 _p(self.di), C.byref(self.phase), _p(iq), len(iq), _p(out))
+        return out[:m]
+
+
+class StatefulFIRReal:
+    """dsp/fir_stateful.go:6-54"""
+
+    def __init__(self, taps):
+        self.taps = np.ascontiguousarray(taps, np.float64)
+        self.delay = np.zeros(len(self.taps))
+
+    def Process(self, x):
+        x = np.ascontiguousarray(x, np.float32)
+        out = np.zeros(len(x), np.float32)
+        if len(x):
+            lib().orc_stateful_fir_real(_p(self.taps), len(self.taps), _p(self.delay), _p(x), len(x), _p(out))
+        return out
+
+
+# ---------------------------------------------------------------- extractor
+def PolyphasePhaseLen(base_len, decim):
+    return lib().orc_polyphase_phase_len(base_len, decim)
+
+
+def BuildPolyphaseTapsPhaseMajor(base, decim):
+    base = np.ascontiguousarray(base, np.float32)
+    if decim <= 0 or len(base) == 0:
+        return np.zeros(0, np.float32)
+    out = np.zeros(decim * PolyphasePhaseLen(len(base), decim), np.float32)
+    n = lib().orc_build_polyphase(_p(base), len(base), decim, _p(out))
+    return out[:n]
+
+
+def ExactIntegerDecimation(sample_rate, out_rate):
+    d = lib().orc_exact_decim(sample_rate, out_rate)
+    if d < 0:
+        raise ValueError("integer decimation required")
+    return d
+
+
+def StreamingConfigHash(signal_id, offset_hz, bandwidth, out_rate, num_taps, sample_rate):
+    return int(lib().orc_config_hash(signal_id, offset_hz, bandwidth, out_rate, num_taps, sample_rate))
+
+
+@dataclass
+class CPUOracleState:
+    """cpu_oracle.go:8-18"""
+    SignalID: int = 0
+    ConfigHash: int = 0
+    NCOPhase: float = 0.0
+    Decim: int = 1
+    PhaseCount: int = 0
+    NumTaps: int = 0
+    ShiftedHistory: np.ndarray = field(default_factory=lambda: np.zeros(0, np.complex64))
+    BaseTaps: np.ndarray = field(default_factory=lambda: np.zeros(0, np.float32))
+    PolyphaseTaps: np.ndarray = field(default_factory=lambda: np.zeros(0, np.float32))
+
+
+def _hist_buf(state):
+    cap = max(1, state.NumTaps - 1) + 1
+    h = np.zeros(max(cap, len(state.ShiftedHistory) + 1), np.complex64)
+    h[:len(state.ShiftedHistory)] = state.ShiftedHistory
+    return h
+
+
+def CPUOracleExtract(iq_new, state: CPUOracleState, phase_inc: float):
+    """cpu_oracle.go:32-75 (mutates state)."""
+    iq = c64(iq_new)
+    if state.NumTaps <= 0 or state.Decim <= 0 or len(state.BaseTaps) < state.NumTaps:
+        return np.zeros(0, np.complex64)
+    h = _hist_buf(state)
+    hl, pc, ph = C.c_int(len(state.ShiftedHistory)), C.c_int(state.PhaseCount), C.c_double(state.NCOPhase)
+    out = np.zeros(len(iq) // state.Decim + 2, np.complex64)
+    base = np.ascontiguousarray(state.BaseTaps, np.float32)
+    n = lib().orc_extract_direct(_p(iq), len(iq), phase_inc, C.byref(ph), C.byref(pc), state.NumTaps, state.Decim,
+                                 _p(base), _p(h), C.byref(hl), _p(out), len(out))
+    state.NCOPhase, state.PhaseCount = ph.value, pc.value
+    state.ShiftedHistory = h[:hl.value].copy()
+    return out[:n]
+
+
+def CPUOracleExtractPolyphase(iq_new, state: CPUOracleState, phase_inc: float):
+    """cpu_oracle.go:79-132 (mutates state)."""
+    iq = c64(iq_new)
+    if state.NumTaps <= 0 or state.Decim <= 0 or len(state.BaseTaps) < state.NumTaps:
+        return np.zeros(0, np.complex64)
+    if len(state.PolyphaseTaps) == 0:
+        state.PolyphaseTaps = BuildPolyphaseTapsPhaseMajor(state.BaseTaps, state.Decim)
+    h = _hist_buf(state)
+    hl, pc, ph = C.c_int(len(state.ShiftedHistory)), C.c_int(state.PhaseCount), C.c_double(state.NCOPhase)
+    out = np.zeros(len(iq) // state.Decim + 2, np.complex64)
+    base = np.ascontiguousarray(state.BaseTaps, np.float32)
+    poly = np.ascontiguousarray(state.PolyphaseTaps, np.float32)
+    n = lib().orc_extract_polyphase(_p(iq), len(iq), phase_inc, C.byref(ph), C.byref(pc), state.NumTaps,
+                                    state.Decim, _p(base), _p(poly), _p(h), C.byref(hl), _p(out), len(out))
+    state.NCOPhase, state.PhaseCount = ph.value, pc.value
+    state.ShiftedHistory = h[:hl.value].copy()
+    return out[:n]
+
+
+def runStreamingPolyphaseHostCore(iq_new, sample_rate, offset_hz, nco_phase, phase_count, num_taps, decim,
+                                  history, poly_taps):
+    """streaming_host_core.go:5-64 -> (out, phase, phaseCount, history)."""
+    iq = c64(iq_new)
+    cap = max(1, num_taps - 1) + 1
+    h = np.zeros(max(cap, len(history) + 1), np.complex64)
+    h[:len(history)] = history
+    hl, pc, ph = C.c_int(len(history)), C.c_int(phase_count), C.c_double(nco_phase)
+    poly = np.ascontiguousarray(poly_taps, np.float32)
+    out = np.zeros(len(iq) // max(1, decim) + 2, np.complex64)
+    n = lib().orc_host_core(_p(iq), len(iq), sample_rate, offset_hz, C.byref(ph), C.byref(pc), num_taps, decim,
+                            _p(h), C.byref(hl), _p(poly), len(poly), _p(out), len(out))
+    return out[:n], ph.value, pc.value, h[:hl.value].copy()
+
+
+@dataclass
+class StreamingExtractResult:
+    SignalID: int
+    IQ: np.ndarray
+    Rate: int
+    NOut: int
+    PhaseCount: int
+    HistoryLen: int
+    NCOPhase: float = 0.0
+
+
+class CPUOracleRunner:
+    """streaming_oracle_extract.go:9-111"""
+
+    def __init__(self, sample_rate: int, threads: int = 1):
+        self.SampleRate = sample_rate
+        self._h = lib().orc_runner_new(sample_rate)
+        lib().orc_runner_set_threads(self._h, threads)
+
+    def __del__(self):
+        try:
+            if self._h:
+                lib().orc_runner_free(self._h)
+                self._h = None
+        except Exception:
+            pass
+
+    def StreamingExtract(self, iq_new, jobs):
+        iq = c64(iq_new if iq_new is not None else np.zeros(0, np.complex64))
+        jarr = (orc_job * max(1, len(jobs)))()
+        for i, j in enumerate(jobs):
+            jarr[i].signal_id, jarr[i].offset_hz, jarr[i].bandwidth = j.SignalID, j.OffsetHz, j.Bandwidth
+            jarr[i].out_rate, jarr[i].num_taps = j.OutRate, j.NumTaps
+            jarr[i].config_hash = int(j.ConfigHash) & 0xFFFFFFFFFFFFFFFF
+        res = (orc_result * max(1, len(jobs)))()
+        out = np.zeros(len(iq) * max(1, len(jobs)) + 2 * len(jobs) + 1, np.complex64)
+        rc = lib().orc_runner_streaming_extract(self._h, _p(iq), len(iq), jarr, len(jobs), _p(out), len(out), res)
+        if rc != 0:
+            raise ValueError(f"oracle StreamingExtract failed ({rc})")
+        return [StreamingExtractResult(r.signal_id, out[r.out_offset:r.out_offset + r.n_out].copy(), r.rate,
+                                       r.n_out, r.phase_count, r.history_len, r.nco_phase)
+                for r in (res[i] for i in range(len(jobs)))]
+
+    def StreamingExtractRaw(self, iq, jobs_arr, n_jobs, out, results):
+        """Raw call for the CPU baseline timing (preallocated ctypes arrays)."""
+        return lib().orc_runner_streaming_extract(self._h, _p(iq), len(iq), jobs_arr, n_jobs, _p(out), len(out),
+                                                  results)
+
+    @property
+    def States(self) -> dict:
+        n = lib().orc_runner_num_states(self._h)
+        ids = np.zeros(max(n, 1), np.int64)
+        lib().orc_runner_state_ids(self._h, _p(ids), len(ids))
+        out = {}
+        for sid in ids[:n]:
+            h = C.c_uint64()
+            ph = C.c_double()
+            dec, pc, nt = C.c_int(), C.c_int(), C.c_int()
+            hist = np.zeros(8192, np.complex64)
+            hl = lib().orc_runner_get_state(self._h, int(sid), C.byref(h), C.byref(ph), C.byref(dec), C.byref(pc),
+                                            C.byref(nt), _p(hist), len(hist))
+            out[int(sid)] = CPUOracleState(int(sid), h.value, ph.value, dec.value, pc.value, nt.value,
+                                           hist[:hl].copy())
+        return out
+
+    def ResetSignalState(self, sid):
+        lib().orc_runner_reset_signal(self._h, sid)
+
+    def ResetAllStates(self):
+        lib().orc_runner_reset_all(self._h)
+
+
+# ---------------------------------------------------------------- demods
+def fmDiscrim(iq):
+    iq = c64(iq)
+    out = np.zeros(max(len(iq) - 1, 0), np.float32)
+    n = lib().orc_fm_discrim(_p(iq), len(iq), _p(out))
+    return out[:n]
+
+
+class BridgedFMDiscrim:
+    """streamer.go:1287-1302 block-bridged discriminator."""
+
+    def __init__(self):
+        self.last = np.zeros(2, np.float32)
+        self.have = C.c_int(0)
+
+    def Demod(self, iq):
+        iq = c64(iq)
+        out = np.zeros(len(iq) + 1, np.float32)
+        n = lib().orc_fm_discrim_bridged(_p(iq), len(iq), _p(self.last), C.byref(self.have), _p(out))
+        return out[:n]
+
+
+def AMDemod(iq):
+    iq = c64(iq)
+    out = np.zeros(len(iq), np.float32)
+    n = lib().orc_am_demod(_p(iq), len(iq), _p(out))
+    return out[:n]
+
+
+def SSBDemod(iq, sample_rate, usb=True, phase=0.0):
+    """Returns (audio, carried phase)."""
+    iq = c64(iq)
+    out = np.zeros(len(iq), np.float32)
+    ph = C.c_double(phase)
+    n = lib().orc_ssb_demod(_p(iq), len(iq), sample_rate, 1 if usb else 0, C.byref(ph), _p(out))
+    return out[:n], ph.value
+
+
+def CWDemod(iq, sample_rate, phase=0.0):
+    iq = c64(iq)
+    out = np.zeros(len(iq), np.float32)
+    ph = C.c_double(phase)
+    n = lib().orc_cw_demod(_p(iq), len(iq), sample_rate, C.byref(ph), _p(out))
+    return out[:n], ph.value
+
+
+def RDSBasebandComplex(iq, sample_rate):
+    """fm.go:143-163 / 234-271 -> (samples, rate)."""
+    iq = c64(iq)
+    out = np.zeros(len(iq) + 1, np.complex64)
+    rate = C.c_int(0)
+    n = lib().orc_rds_baseband(_p(iq), len(iq), sample_rate, _p(out), len(out), C.byref(rate))
+    return out[:n], rate.value
+
+
+# ---------------------------------------------------------------- reference test inputs
+def MakeDeterministicIQ(n):
+    """test_harness.go:16-24"""
+    i = np.arange(n, dtype=np.float64)
+    a, b = 0.017 * i, 0.031 * i
+    return (np.cos(a) + 0.2 * np.cos(b) + 1j * (np.sin(a) + 0.15 * np.sin(b))).astype(np.complex64)
+
+
+def MakeToneIQ(n, phase_inc):
+    """test_harness.go:26-34 (sequential accumulation of the phase)."""
+    ph = np.cumsum(np.concatenate([[0.0], np.full(max(n - 1, 0), phase_inc)]))[:n]
+    return (np.cos(ph) + 1j * np.sin(ph)).astype(np.complex64)
+
+
+def MakeLowpassTaps(n):
+    """test_harness.go:36-42"""
+    return np.full(n, np.float32(1.0) / np.float32(n), np.float32)
+
+
+def makeToneNoiseIQ(n, phase_inc):
+    """streaming_gpu_validation_helpers_test.go:16-27"""
+    i = np.arange(n, dtype=np.float64)
+    ph = np.cumsum(np.concatenate([[0.0], np.full(max(n - 1, 0), phase_inc)]))[:n]
+    tone = np.cos(ph) + 1j * np.sin(ph)
+    ni = 0.17 * np.cos(0.113 * i + 0.31) + 0.07 * np.sin(0.071 * i)
+    nq = 0.13 * np.sin(0.097 * i + 0.11) - 0.05 * np.cos(0.043 * i)
+    return (0.85 * tone + 0.15 * (ni + 1j * nq)).astype(np.complex64)
