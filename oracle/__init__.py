@@ -409,3 +409,108 @@ def makeToneNoiseIQ(n, phase_inc):
     ni = 0.17 * np.cos(0.113 * i + 0.31) + 0.07 * np.sin(0.071 * i)
     nq = 0.13 * np.sin(0.097 * i + 0.11) - 0.05 * np.cos(0.043 * i)
     return (0.85 * tone + 0.15 * (ni + 1j * nq)).astype(np.complex64)
+
+
+# ---------------------------------------------------------------- surveillance
+CFAR_CA, CFAR_GOSCA, CFAR_CASO, CFAR_OS = 0, 1, 2, 3
+_CFAR_MODES = {"CA": CFAR_CA, "GOSCA": CFAR_GOSCA, "CASO": CFAR_CASO, "OS": CFAR_OS}
+
+
+class orc_detector_cfg(C.Structure):
+    _fields_ = [("threshold_db", C.c_double), ("hysteresis_db", C.c_double), ("cfar_scale_db", C.c_double),
+                ("edge_margin_db", C.c_double), ("max_signal_bw_hz", C.c_double), ("merge_gap_hz", C.c_double),
+                ("bin_width", C.c_double), ("center_hz", C.c_double), ("nbins", C.c_int), ("pad", C.c_int)]
+
+
+class orc_signal(C.Structure):
+    _fields_ = [("first_bin", C.c_int), ("last_bin", C.c_int), ("peak_bin", C.c_int), ("pad", C.c_int),
+                ("center_hz", C.c_double), ("bw_hz", C.c_double), ("peak_db", C.c_double), ("snr_db", C.c_double),
+                ("noise_db", C.c_double)]
+
+
+register("orc_hann", None, [I, P])
+register("orc_fft", None, [P, I])
+register("orc_spectrum", None, [P, I, P, I, P])
+register("orc_spectrum_from_fft", None, [P, I, P])
+register("orc_welch_psd", I, [P, I, I, D, P, I, P])
+register("orc_cfar_thresholds", I, [I, I, I, I, D, I, P, I, P])
+register("orc_median", D, [P, I])
+register("orc_detect_signals", I, [C.POINTER(orc_detector_cfg), P, P, I, C.POINTER(orc_signal), I, PD_])
+
+
+def Hann(n):
+    w = np.zeros(max(n, 0))
+    lib().orc_hann(n, _p(w))
+    return w
+
+
+def FFT(x):
+    a = np.ascontiguousarray(x, np.complex128).copy()
+    lib().orc_fft(_p(a), len(a))
+    return a
+
+
+def Spectrum(iq, window=None):
+    iq = c64(iq)
+    w = np.ascontiguousarray(window if window is not None else np.zeros(0), np.float64)
+    out = np.zeros(len(iq))
+    lib().orc_spectrum(_p(iq), len(iq), _p(w), len(w), _p(out))
+    return out
+
+
+def SpectrumFromFFT(X):
+    X = c64(X)
+    out = np.zeros(len(X))
+    lib().orc_spectrum_from_fft(_p(X), len(X), _p(out))
+    return out
+
+
+def WelchPSD(iq, seg, overlap, window=None):
+    iq = c64(iq)
+    w = np.ascontiguousarray(window if window is not None else np.zeros(0), np.float64)
+    out = np.zeros(max(seg, len(iq)))
+    segs = lib().orc_welch_psd(_p(iq), len(iq), seg, overlap, _p(w), len(w), _p(out))
+    return out[:seg] if segs > 0 else out[:len(iq)]
+
+
+def CFARThresholds(mode, spectrum, guard, train, scale_db, wrap=True, rank=0):
+    s = np.ascontiguousarray(spectrum, np.float64)
+    out = np.zeros(len(s))
+    m = _CFAR_MODES[mode] if isinstance(mode, str) else mode
+    rc = lib().orc_cfar_thresholds(m, guard, train, rank, scale_db, 1 if wrap else 0, _p(s), len(s), _p(out))
+    if rc != 0:
+        return None
+    return out
+
+
+def median(v):
+    v = np.ascontiguousarray(v, np.float64)
+    return lib().orc_median(_p(v), len(v))
+
+
+@dataclass
+class Signal:
+    FirstBin: int
+    LastBin: int
+    PeakBin: int
+    CenterHz: float
+    BWHz: float
+    PeakDb: float
+    SNRDb: float
+    NoiseDb: float
+
+
+def DetectSignals(spectrum, thresholds, *, sample_rate, center_hz=0.0, threshold_db=-20.0, hysteresis_db=3.0,
+                  cfar_scale_db=6.0, edge_margin_db=3.0, max_signal_bw_hz=150000.0, merge_gap_hz=5000.0):
+    """detector.go:408-481 threshold path on an already smoothed spectrum."""
+    s = np.ascontiguousarray(spectrum, np.float64)
+    n = len(s)
+    cfg = orc_detector_cfg(threshold_db, hysteresis_db, cfar_scale_db, edge_margin_db, max_signal_bw_hz,
+                           merge_gap_hz, sample_rate / n, center_hz, n, 0)
+    thr = None if thresholds is None else np.ascontiguousarray(thresholds, np.float64)
+    cap = 4096
+    out = (orc_signal * cap)()
+    ng = C.c_double()
+    m = lib().orc_detect_signals(C.byref(cfg), _p(s), _p(thr), n, out, cap, C.byref(ng))
+    return [Signal(o.first_bin, o.last_bin, o.peak_bin, o.center_hz, o.bw_hz, o.peak_db, o.snr_db, o.noise_db)
+            for o in out[:min(m, cap)]]

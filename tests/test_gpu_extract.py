@@ -186,7 +186,7 @@ def test_stateful_entry_point_direct(gpu):
         assert f(*args(in_=None)) == -13
         assert f(*args(tmp=None)) == -13
         assert f(*args(n=5000, cap=1)) == -14
-        assert f(*args(n=5000, out=None)) == -15
+        assert f(*args(n=5000, out=None, cap=1000)) == -15
     finally:
         for p in ptrs.values():
             L.gpud_free(p)
