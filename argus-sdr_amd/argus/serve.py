@@ -1,0 +1,133 @@
+"""Streaming service binding (gpud_serve_* in include/argus_gpu.h).
+
+    s = Serve(sample_rate, block_samples, hbm_budget_bytes)
+    s.Open(ChannelSpec(...))            # False if the HBM budget refuses it
+    t = s.Submit(iq_block)              # async; returns a ticket
+    info = s.Wait(t); iq, audio, rate = s.Output(t, channel_id)
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+
+import numpy as np
+
+from ._lib import check, lib, register
+
+DEMOD_NONE, DEMOD_NFM, DEMOD_WFM, DEMOD_AM, DEMOD_USB, DEMOD_LSB, DEMOD_CW = range(7)
+ERR_BUDGET = -20
+ERR_DUPLICATE = -21
+
+
+class gpud_channel_spec(C.Structure):
+    _fields_ = [("channel_id", C.c_int64), ("offset_hz", C.c_double), ("bandwidth", C.c_double),
+                ("out_rate", C.c_int32), ("num_taps", C.c_int32), ("demod", C.c_int32), ("window", C.c_int32)]
+
+
+class gpud_serve_block_info(C.Structure):
+    _fields_ = [("ticket", C.c_int64), ("n_channels", C.c_int32), ("n_samples", C.c_int32),
+                ("iq_total", C.c_int64), ("audio_total", C.c_int64), ("gpu_ms", C.c_float),
+                ("kernel_ms", C.c_float)]
+
+
+P, I = C.c_void_p, C.c_int
+register("gpud_serve_create", I, [I, I, C.c_uint64, C.POINTER(P)])
+register("gpud_serve_destroy", I, [P])
+register("gpud_serve_open", I, [P, C.POINTER(gpud_channel_spec)])
+register("gpud_serve_close", I, [P, C.c_int64])
+register("gpud_serve_info", I, [P, C.POINTER(I), C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)])
+register("gpud_serve_submit", I, [P, P, I, C.POINTER(C.c_int64)])
+register("gpud_serve_submit_device", I, [P, P, I, C.POINTER(C.c_int64)])
+register("gpud_serve_set_download", I, [P, I])
+register("gpud_serve_wait", I, [P, C.c_int64, C.POINTER(gpud_serve_block_info)])
+register("gpud_serve_channel_output", I, [P, C.c_int64, C.c_int64, C.POINTER(P), C.POINTER(I), C.POINTER(P),
+                                          C.POINTER(I), C.POINTER(I)])
+
+
+@dataclass
+class ChannelSpec:
+    ChannelID: int
+    OffsetHz: float
+    Bandwidth: float
+    OutRate: int
+    NumTaps: int = 101
+    Demod: int = DEMOD_NONE
+    Window: int = 0
+
+    def c(self) -> gpud_channel_spec:
+        return gpud_channel_spec(self.ChannelID, self.OffsetHz, self.Bandwidth, self.OutRate, self.NumTaps,
+                                 self.Demod, self.Window)
+
+
+class Serve:
+    def __init__(self, sample_rate: int, block_samples: int, hbm_budget_bytes: int):
+        h = C.c_void_p()
+        check("gpud_serve_create", lib().gpud_serve_create(sample_rate, block_samples, hbm_budget_bytes, C.byref(h)))
+        self._h = h
+        self.sample_rate = sample_rate
+        self.block_samples = block_samples
+
+    def Close(self):
+        if self._h:
+            lib().gpud_serve_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.Close()
+        except Exception:
+            pass
+
+    def Open(self, spec: ChannelSpec) -> bool:
+        """True if admitted, False if the HBM budget refuses it."""
+        cs = spec.c()
+        rc = lib().gpud_serve_open(self._h, C.byref(cs))
+        if rc == ERR_BUDGET:
+            return False
+        check("gpud_serve_open", rc)
+        return True
+
+    def Retire(self, channel_id: int) -> None:
+        check("gpud_serve_close", lib().gpud_serve_close(self._h, channel_id))
+
+    def Info(self):
+        n = C.c_int()
+        used = C.c_uint64()
+        bud = C.c_uint64()
+        check("gpud_serve_info", lib().gpud_serve_info(self._h, C.byref(n), C.byref(used), C.byref(bud)))
+        return n.value, used.value, bud.value
+
+    def SetDownload(self, enable: bool) -> None:
+        check("gpud_serve_set_download", lib().gpud_serve_set_download(self._h, 1 if enable else 0))
+
+    def Submit(self, iq) -> int:
+        iq = np.ascontiguousarray(iq, np.complex64)
+        t = C.c_int64()
+        check("gpud_serve_submit", lib().gpud_serve_submit(self._h, iq.ctypes.data if len(iq) else None, len(iq),
+                                                           C.byref(t)))
+        return t.value
+
+    def SubmitDevice(self, dev_ptr: int, n: int) -> int:
+        t = C.c_int64()
+        check("gpud_serve_submit_device", lib().gpud_serve_submit_device(self._h, dev_ptr, n, C.byref(t)))
+        return t.value
+
+    def Wait(self, ticket: int) -> gpud_serve_block_info:
+        info = gpud_serve_block_info()
+        check("gpud_serve_wait", lib().gpud_serve_wait(self._h, ticket, C.byref(info)))
+        return info
+
+    def Output(self, ticket: int, channel_id: int):
+        """(iq complex64 copy, audio float32 copy, rate) of one channel, or None."""
+        iq_p, aud_p = C.c_void_p(), C.c_void_p()
+        n_iq, n_a, rate = C.c_int(), C.c_int(), C.c_int()
+        rc = lib().gpud_serve_channel_output(self._h, ticket, channel_id, C.byref(iq_p), C.byref(n_iq),
+                                             C.byref(aud_p), C.byref(n_a), C.byref(rate))
+        if rc == -1:
+            return None
+        check("gpud_serve_channel_output", rc)
+        iq = np.ctypeslib.as_array((C.c_float * (2 * n_iq.value)).from_address(iq_p.value)).view(np.complex64).copy() \
+            if n_iq.value else np.zeros(0, np.complex64)
+        au = np.ctypeslib.as_array((C.c_float * n_a.value).from_address(aud_p.value)).copy() \
+            if n_a.value else np.zeros(0, np.float32)
+        return iq, au, rate.value

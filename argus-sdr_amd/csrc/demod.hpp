@@ -1,0 +1,48 @@
+// demod.hpp -- batched demodulation of extracted baseband (FM discriminator
+// with block bridging, AM envelope minus block mean, SSB/CW product
+// detector with a carried BFO phase).
+#pragma once
+#include "common.hpp"
+
+enum {
+    ARGUS_DEMOD_NONE = 0,
+    ARGUS_DEMOD_NFM = 1,
+    ARGUS_DEMOD_WFM = 2,
+    ARGUS_DEMOD_AM = 3,
+    ARGUS_DEMOD_USB = 4,
+    ARGUS_DEMOD_LSB = 5,
+    ARGUS_DEMOD_CW = 6,
+};
+
+struct ArgusDemodDesc {
+    const float2* iq;     // this block's baseband, n samples
+    float* audio;         // n - skip outputs
+    const float2* carry_rd;  // previous block's last baseband sample (FM)
+    float2* carry_wr;        // this block's last baseband sample (FM)
+    uint64_t bfo0;        // SSB/CW: NCO phase of sample 0, pre-increment applied (turns)
+    uint64_t bfo_inc;     // SSB/CW: increment per sample (turns)
+    int n;                // baseband samples this block
+    int mode;             // ARGUS_DEMOD_*
+    int have_carry;       // FM: carry_rd holds a sample
+    int skip;             // outputs not produced at the head (FM without carry: 1)
+};
+
+struct ArgusDemodItem {
+    int chan;
+    int start;
+    int count;
+    int pad;
+};
+
+#define ARGUS_DEMOD_TILE 4096
+
+// Audio samples a channel produces from n baseband samples.
+static inline int argus_demod_count(int mode, int n, int have_carry)
+{
+    if (mode == ARGUS_DEMOD_NONE) return 0;
+    if (mode == ARGUS_DEMOD_NFM || mode == ARGUS_DEMOD_WFM) return have_carry ? n : (n > 0 ? n - 1 : 0);
+    return n;
+}
+
+int argus_launch_demod(const ArgusDemodDesc* d_desc, const ArgusDemodItem* d_items, int n_items,
+                       hipStream_t stream);

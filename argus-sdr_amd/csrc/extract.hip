@@ -16,47 +16,126 @@ __device__ __forceinline__ float2 argus_shifted(const ArgusChanDesc& ch, const f
     return h >= 0 ? ch.hist_rd[h] : make_float2(0.0f, 0.0f);
 }
 
-__device__ __forceinline__ void argus_extract_tile(const ArgusChanDesc& ch, const float2* __restrict__ in, int m0,
-                                                   float2* lds)
+__device__ __forceinline__ float argus_wave_sum(float v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    return v;
+}
+
+// Dense tile: M outputs starting at m0, samples staged in LDS by residue.
+__device__ __forceinline__ void argus_dense_tile(const ArgusChanDesc& ch, const float2* __restrict__ in, int m0,
+                                                 float2* lds_s)
 {
     const int D = ch.D, T = ch.T, L = ch.L, P = ch.P, PS = ch.PS;
     const int M = min(ch.M, ch.n_out - m0);
     const int em0 = ch.e0 + m0 * D;
-    if (D < T) {
-        const int n_lo = em0 - (T - 1);
-        const int span = (M - 1) * D + T;
-        for (int i = threadIdx.x; i < span; i += blockDim.x) {
+    float* lds_t = (float*)(lds_s + P * PS);
+    for (int i = threadIdx.x; i < P * L; i += blockDim.x) lds_t[i] = ch.taps[i];
+
+    // Staging: sample n = n_lo + i for i in [0, span); d = em0 - n = T-1-i,
+    // residue p = d mod D, plane index j = (p - d) / D.  One division per
+    // thread, then stepped by blockDim without dividing again.
+    const int span = (M - 1) * D + T;
+    const int n_lo = em0 - (T - 1);
+    int i = threadIdx.x;
+    if (i < span) {
+        const int B = blockDim.x;
+        const int stepq = B / D, stepr = B - stepq * D;
+        int d = (T - 1) - i;
+        int p = d % D;
+        if (p < 0) p += D;
+        int j = (p - d) / D;
+        uint64_t ph = ch.phase0 + ch.inc * (uint64_t)(int64_t)(n_lo + i);
+        const uint64_t phstep = ch.inc * (uint64_t)B;
+        for (; i < span; i += B) {
             const int n = n_lo + i;
-            const int d = em0 - n;               // in [-(M-1)D, T-1]
-            int p = d % D;
-            if (p < 0) p += D;
-            const int j = (p - d) / D;           // exact: p - d is a multiple of D
-            lds[p * PS + j + L - 1] = argus_shifted(ch, in, n);
-        }
-    } else {
-        const int tot = M * T;
-        for (int i = threadIdx.x; i < tot; i += blockDim.x) {
-            const int mr = i / T;
-            const int q = i - mr * T;
-            const int n = em0 + mr * D - (T - 1) + q;
-            lds[(T - 1 - q) * PS + mr] = argus_shifted(ch, in, n);
+            float2 v;
+            if (n >= 0) {
+                float sn, cs;
+                argus_nco(ph, sn, cs);
+                v = argus_rot(in[n], sn, cs);
+            } else {
+                const int h = ch.H + n;
+                v = h >= 0 ? ch.hist_rd[h] : make_float2(0.0f, 0.0f);
+            }
+            lds_s[p * PS + j + L - 1] = v;
+            ph += phstep;
+            p -= stepr;
+            j += stepq;
+            if (p < 0) {
+                p += D;
+                j += 1;
+            }
         }
     }
     __syncthreads();
-    for (int t = threadIdx.x; t < M; t += blockDim.x) {
-        float ar = 0.0f, ai = 0.0f;
-        for (int p = 0; p < P; ++p) {
-            const float* __restrict__ tp = ch.taps + p * L;
-            const float2* sp = lds + p * PS + t + L - 1;
-            const int kmax = (T - p + D - 1) / D;   // taps with p + k*D < T
-            for (int k = 0; k < kmax; ++k) {
-                const float tap = tp[k];
+
+    // FIR: output m = t / S, lanes g = t % S share its planes p = g, g+S, ...
+    const int S = ch.S;
+    const int t = threadIdx.x;
+    const int m = t / S;
+    const int g = t & (S - 1);
+    float ar0 = 0.0f, ai0 = 0.0f, ar1 = 0.0f, ai1 = 0.0f;
+    if (m < M) {
+        const int pfull = T - (L - 1) * D;  // planes p < pfull carry L taps, the rest L-1
+        for (int p = g; p < P; p += S) {
+            const int kmax = p < pfull ? L : L - 1;
+            const float* tp = lds_t + p * L;
+            const float2* sp = lds_s + p * PS + m + L - 1;
+            int k = 0;
+            for (; k + 4 <= kmax; k += 4) {
+                const float h0 = tp[k], h1 = tp[k + 1], h2 = tp[k + 2], h3 = tp[k + 3];
+                const float2 v0 = sp[-k], v1 = sp[-k - 1], v2 = sp[-k - 2], v3 = sp[-k - 3];
+                ar0 = fmaf(h0, v0.x, ar0);
+                ai0 = fmaf(h0, v0.y, ai0);
+                ar1 = fmaf(h1, v1.x, ar1);
+                ai1 = fmaf(h1, v1.y, ai1);
+                ar0 = fmaf(h2, v2.x, ar0);
+                ai0 = fmaf(h2, v2.y, ai0);
+                ar1 = fmaf(h3, v3.x, ar1);
+                ai1 = fmaf(h3, v3.y, ai1);
+            }
+            for (; k < kmax; ++k) {
+                const float h = tp[k];
                 const float2 v = sp[-k];
-                ar = fmaf(tap, v.x, ar);
-                ai = fmaf(tap, v.y, ai);
+                ar0 = fmaf(h, v.x, ar0);
+                ai0 = fmaf(h, v.y, ai0);
             }
         }
-        ch.out[m0 + t] = make_float2(ar, ai);
+    }
+    float ar = ar0 + ar1, ai = ai0 + ai1;
+    for (int o = S >> 1; o > 0; o >>= 1) {
+        ar += __shfl_xor(ar, o);
+        ai += __shfl_xor(ai, o);
+    }
+    if (m < M && g == 0) ch.out[m0 + m] = make_float2(ar, ai);
+    __syncthreads();
+}
+
+// Narrow group: one wavefront per output, lanes over taps, no staging.
+__device__ __forceinline__ void argus_narrow_group(const ArgusChanDesc& ch, const float2* __restrict__ in, int m0,
+                                                   float* lds_t)
+{
+    const int T = ch.T, D = ch.D;
+    for (int i = threadIdx.x; i < T; i += blockDim.x) lds_t[i] = ch.taps[i];  // L == 1: taps[k] = h[k]
+    __syncthreads();
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const int nw = blockDim.x >> 6;
+    const int mend = min(m0 + ch.M, ch.n_out);
+    for (int m = m0 + wave; m < mend; m += nw) {
+        const int e = ch.e0 + m * D;
+        float ar = 0.0f, ai = 0.0f;
+        for (int k = lane; k < T; k += 64) {
+            const float2 s = argus_shifted(ch, in, e - k);
+            const float h = lds_t[k];
+            ar = fmaf(h, s.x, ar);
+            ai = fmaf(h, s.y, ai);
+        }
+        ar = argus_wave_sum(ar);
+        ai = argus_wave_sum(ai);
+        if (lane == 0) ch.out[m] = make_float2(ar, ai);
     }
     __syncthreads();
 }
@@ -65,6 +144,17 @@ __device__ __forceinline__ void argus_history_tail(const ArgusChanDesc& ch, cons
 {
     for (int i = threadIdx.x; i < ch.keep; i += blockDim.x)
         ch.hist_wr[i] = argus_shifted(ch, in, n_new - ch.keep + i);
+}
+
+__device__ __forceinline__ void argus_run_item(const ArgusChanDesc& ch, int kind, int m0,
+                                               const float2* __restrict__ in, int n_new, float2* lds)
+{
+    if (kind == ARGUS_ITEM_DENSE)
+        argus_dense_tile(ch, in, m0, lds);
+    else if (kind == ARGUS_ITEM_NARROW)
+        argus_narrow_group(ch, in, m0, (float*)lds);
+    else
+        argus_history_tail(ch, in, n_new);
 }
 
 // Batched, persistent: every work item of every channel of one chunk.
@@ -76,35 +166,36 @@ __global__ __launch_bounds__(ARGUS_EXTRACT_THREADS) void argus_extract_batch_ker
     for (int w = blockIdx.x; w < n_items; w += gridDim.x) {
         const ArgusWorkItem it = items[w];
         const ArgusChanDesc ch = chans[it.chan];
-        if (it.kind == 0)
-            argus_extract_tile(ch, in, it.m0, lds);
-        else
-            argus_history_tail(ch, in, n_new);
+        argus_run_item(ch, it.kind, it.m0, in, n_new, lds);
     }
 }
 
-// One signal: blocks [0, n_tiles) are output tiles, block n_tiles (if
+// One signal: blocks [0, n_items) are output items, block n_items (if
 // launched) writes the new history.
 __global__ __launch_bounds__(ARGUS_EXTRACT_THREADS) void argus_extract_one_kernel(ArgusChanDesc ch,
                                                                                    const float2* __restrict__ in,
-                                                                                   int n_new, int n_tiles)
+                                                                                   int n_new, int n_items)
 {
     extern __shared__ float2 lds[];
-    if ((int)blockIdx.x < n_tiles)
-        argus_extract_tile(ch, in, blockIdx.x * ch.M, lds);
-    else
-        argus_history_tail(ch, in, n_new);
+    const int kind = (int)blockIdx.x < n_items ? (ch.narrow ? ARGUS_ITEM_NARROW : ARGUS_ITEM_DENSE)
+                                               : ARGUS_ITEM_HISTORY;
+    argus_run_item(ch, kind, blockIdx.x * ch.M, in, n_new, lds);
+}
+
+size_t argus_item_lds_bytes(const ArgusChanDesc& ch)
+{
+    return ch.narrow ? (size_t)ch.T * sizeof(float) : argus_dense_lds_bytes(ch);
 }
 
 // Host: launch the single-signal kernel for a fully described channel.
 int argus_launch_extract_one(const ArgusChanDesc& ch, const float2* in, int n_new, hipStream_t stream)
 {
-    const int n_tiles = ch.n_out > 0 ? (ch.n_out + ch.M - 1) / ch.M : 0;
-    const int blocks = n_tiles + (ch.keep > 0 ? 1 : 0);
+    const int n_items = ch.n_out > 0 ? (ch.n_out + ch.M - 1) / ch.M : 0;
+    const int blocks = n_items + (ch.keep > 0 ? 1 : 0);
     if (blocks == 0) return 0;
-    const size_t lds = n_tiles > 0 ? (size_t)ch.P * ch.PS * sizeof(float2) : 0;
+    const size_t lds = n_items > 0 ? argus_item_lds_bytes(ch) : 0;
     hipLaunchKernelGGL(argus_extract_one_kernel, dim3(blocks), dim3(ARGUS_EXTRACT_THREADS), lds, stream, ch, in,
-                       n_new, n_tiles);
+                       n_new, n_items);
     return (int)hipGetLastError();
 }
 
@@ -121,10 +212,12 @@ int argus_launch_extract_batch(const ArgusChanDesc* d_chans, const ArgusWorkItem
     return (int)hipGetLastError();
 }
 
-// Describe one signal for the kernels.  Returns 0 or -1 if T is too large.
+// Describe one signal for the kernels.  Returns 0 or -1 if the geometry
+// cannot be staged (T too large).
 int argus_fill_chan(ArgusChanDesc* ch, const float2* hist_rd, int H, float2* hist_wr, int keep, const float* taps,
                     float2* out, int D, int T, int phase_count, int n_new, double phase_start, double phase_inc)
 {
+    if (T > ARGUS_MAX_TAPS) return -1;
     ch->hist_rd = hist_rd;
     ch->hist_wr = hist_wr;
     ch->taps = taps;
@@ -139,10 +232,7 @@ int argus_fill_chan(ArgusChanDesc* ch, const float2* hist_rd, int H, float2* his
     ch->e0 = D - phase_count - 1;
     ch->n_out = (phase_count + n_new) / D;
     ch->keep = keep;
-    int ps = 0;
-    ch->M = argus_plan_tile(D, T, ch->n_out, &ps);
-    ch->PS = ps;
-    return ch->M > 0 ? 0 : -1;
+    return argus_plan_chan(ch);
 }
 
 extern "C" {

@@ -17,6 +17,7 @@ int argus_launch_extract_batch(const ArgusChanDesc* d_chans, const ArgusWorkItem
                                size_t lds_bytes, const float2* in, int n_new, hipStream_t stream);
 int argus_fill_chan(ArgusChanDesc* ch, const float2* hist_rd, int H, float2* hist_wr, int keep, const float* taps,
                     float2* out, int D, int T, int phase_count, int n_new, double phase_start, double phase_inc);
+size_t argus_item_lds_bytes(const ArgusChanDesc& ch);
 
 // ---------------------------------------------------------------------------
 // Tap design (dsp/fir.go:6-28) and polyphase layout (polyphase.go:6-22).
@@ -385,11 +386,14 @@ static int run_batched(gpud_runner* r, const float2* d_in, int n_new, const gpud
                             inc) != 0)
             return -1;
         const int tiles = ch.n_out > 0 ? (ch.n_out + ch.M - 1) / ch.M : 0;
-        if (tiles > 0) lds = std::max(lds, (size_t)ch.P * ch.PS * sizeof(float2));
-        const int span = D < T ? (ch.M - 1) * D + T : ch.M * T;
-        const long cost = (long)span + (long)ch.M * T / 2;
-        for (int t = 0; t < tiles; t++) r->items.push_back({i, t * ch.M, 0, (int)std::min<long>(cost, 1L << 30)});
-        if (keep > 0) r->items.push_back({i, 0, 1, keep});
+        if (tiles > 0) lds = std::max(lds, argus_item_lds_bytes(ch));
+        // cost in lane-operations: NCO rotations (~24 each) plus taps (~2 each)
+        const long rot = ch.narrow ? (long)ch.M * T : (long)(ch.M - 1) * D + T;
+        const long cost = rot * 24 + (long)ch.M * T * 2;
+        const int kind = ch.narrow ? ARGUS_ITEM_NARROW : ARGUS_ITEM_DENSE;
+        for (int t = 0; t < tiles; t++)
+            r->items.push_back({i, t * ch.M, kind, (int)std::min<long>(cost, 1L << 30)});
+        if (keep > 0) r->items.push_back({i, 0, ARGUS_ITEM_HISTORY, keep * 24});
     }
     // heaviest first (LPT order) so the persistent grid drains evenly
     std::stable_sort(r->items.begin(), r->items.end(),

@@ -1,0 +1,76 @@
+// serve.hpp -- state of the streaming service.
+#pragma once
+#include "extract.hpp"
+#include "demod.hpp"
+#include <map>
+#include <vector>
+
+#define ARGUS_SERVE_SLOTS 3
+
+struct ServeChannel {
+    gpud_channel_spec spec{};
+    int decim = 0, T = 0;
+    size_t footprint = 0;
+    float* d_taps = nullptr;
+    float2* d_hist[2] = {nullptr, nullptr};
+    int hist_sel = 0, hist_len = 0;
+    float2* d_carry = nullptr;  // 2 slots, ping-pong
+    int carry_sel = 0;
+    bool have_carry = false;
+    double nco_phase = 0.0;
+    int phase_count = 0;
+    double bfo_phase = 0.0;
+    int64_t last_ticket = 0;  // last block that used the channel
+};
+
+struct ServeOut {
+    int64_t channel_id;
+    int64_t iq_off, audio_off;
+    int n_iq, n_audio, rate;
+};
+
+struct ServeSlot {
+    int64_t ticket = 0;
+    bool busy = false;
+    int n = 0;
+    hipEvent_t ev_start = nullptr, ev_in = nullptr, ev_kstart = nullptr, ev_kern = nullptr, ev_done = nullptr;
+    float2* d_in = nullptr;
+    float2* h_in = nullptr;
+    float2* d_iq = nullptr;
+    float* d_audio = nullptr;
+    size_t iq_cap = 0, audio_cap = 0;
+    float2* h_iq = nullptr;
+    float* h_audio = nullptr;
+    size_t h_iq_cap = 0, h_audio_cap = 0;
+    ArgusChanDesc* h_chans = nullptr;
+    ArgusChanDesc* d_chans = nullptr;
+    size_t h_chans_cap = 0, d_chans_cap = 0;
+    ArgusWorkItem* h_items = nullptr;
+    ArgusWorkItem* d_items = nullptr;
+    size_t h_items_cap = 0, d_items_cap = 0;
+    ArgusDemodDesc* h_ddesc = nullptr;
+    ArgusDemodDesc* d_ddesc = nullptr;
+    size_t h_ddesc_cap = 0, d_ddesc_cap = 0;
+    ArgusDemodItem* h_ditems = nullptr;
+    ArgusDemodItem* d_ditems = nullptr;
+    size_t h_ditems_cap = 0, d_ditems_cap = 0;
+    int64_t iq_total = 0, audio_total = 0;
+    std::vector<ServeOut> outs;
+};
+
+struct gpud_serve {
+    int sample_rate = 0;
+    int block_samples = 0;
+    uint64_t budget = 0;
+    uint64_t bytes_in_use = 0;
+    bool download = true;
+    hipStream_t compute = nullptr, copy = nullptr;
+    std::map<int64_t, ServeChannel> live;  // ordered by id: a deterministic channel order per block
+    std::vector<ServeChannel> closed;     // awaiting their last block
+    ServeSlot slots[ARGUS_SERVE_SLOTS];
+    int next_slot = 0;
+    int64_t last_ticket = 0;
+    int64_t completed_ticket = 0;
+    std::vector<ArgusWorkItem> items;
+    std::vector<ArgusDemodItem> ditems;
+};

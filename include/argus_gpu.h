@@ -202,6 +202,69 @@ GPUD_API int gpud_lowpass_fir(double cutoff_hz, int sample_rate, int taps, doubl
 /* polyphase.go:6-22 BuildPolyphaseTapsPhaseMajor; returns decim*ceil(n/decim) */
 GPUD_API int gpud_build_polyphase_taps(const float* base, int n, int decim, float* out);
 
+/* ------------------------------------------------------------------ */
+/* Streaming service (no reference counterpart: the reference drives one */
+/* runner per DSP tick from a goroutine, cmd/sdrd/dsp_loop.go:54-172).  */
+/* Channels are admitted against a fixed HBM budget and retired while   */
+/* blocks flow; each block is one fused extraction launch plus one      */
+/* demodulation launch over every live channel, pipelined against the   */
+/* block's upload and the previous block's download.                    */
+/* ------------------------------------------------------------------ */
+typedef struct gpud_serve gpud_serve;
+
+enum {
+    GPUD_DEMOD_NONE = 0, /* baseband IQ only */
+    GPUD_DEMOD_NFM = 1,  /* fm.go:179-196, bridged (streamer.go:1287-1302) */
+    GPUD_DEMOD_WFM = 2,  /* same discriminator */
+    GPUD_DEMOD_AM = 3,   /* am.go:11-27 */
+    GPUD_DEMOD_USB = 4,  /* ssb.go:19-39, BFO +700 Hz, carried phase */
+    GPUD_DEMOD_LSB = 5,  /* ssb.go:19-39, BFO -700 Hz */
+    GPUD_DEMOD_CW = 6    /* cw.go:11-26 */
+};
+
+#define GPUD_ERR_BUDGET (-20)    /* admission refused: HBM budget exhausted */
+#define GPUD_ERR_DUPLICATE (-21) /* channel id already live */
+
+typedef struct gpud_channel_spec {
+    int64_t channel_id;
+    double  offset_hz;
+    double  bandwidth;    /* low-pass cutoff = max(bandwidth/2, 200) (stream_state.go:54-57) */
+    int32_t out_rate;     /* must divide the sample rate */
+    int32_t num_taps;     /* <= 0 -> 101 */
+    int32_t demod;        /* GPUD_DEMOD_* */
+    int32_t window;       /* 0 = Hann (LowpassFIR), 1 = Hamming */
+} gpud_channel_spec;
+
+typedef struct gpud_serve_block_info {
+    int64_t ticket;
+    int32_t n_channels;
+    int32_t n_samples;
+    int64_t iq_total;
+    int64_t audio_total;
+    float   gpu_ms;       /* event time: upload start -> download end */
+    float   kernel_ms;    /* event time: extraction + demodulation kernels */
+} gpud_serve_block_info;
+
+GPUD_API int gpud_serve_create(int sample_rate, int block_samples, uint64_t hbm_budget_bytes, gpud_serve** out);
+GPUD_API int gpud_serve_destroy(gpud_serve* s);
+/* 0, -1 bad args / non-integer decimation, GPUD_ERR_DUPLICATE, GPUD_ERR_BUDGET */
+GPUD_API int gpud_serve_open(gpud_serve* s, const gpud_channel_spec* spec);
+GPUD_API int gpud_serve_close(gpud_serve* s, int64_t channel_id);
+GPUD_API int gpud_serve_info(gpud_serve* s, int* live_channels, uint64_t* bytes_in_use, uint64_t* budget);
+/* Enqueue a block (host memory, copied before return) and return at once. */
+GPUD_API int gpud_serve_submit(gpud_serve* s, const gpud_float2* iq_host, int n, int64_t* ticket);
+/* Enqueue a block already resident in device memory (must stay valid until
+ * the block completes). */
+GPUD_API int gpud_serve_submit_device(gpud_serve* s, const gpud_float2* iq_dev, int n, int64_t* ticket);
+/* Disable (0) / enable (1, default) the download of outputs to host memory. */
+GPUD_API int gpud_serve_set_download(gpud_serve* s, int enable);
+/* Wait for a block.  -2 if the ticket is unknown or its slot was reused. */
+GPUD_API int gpud_serve_wait(gpud_serve* s, int64_t ticket, gpud_serve_block_info* info);
+/* A channel's outputs in a completed block (pinned host memory, valid until
+ * its slot is reused, i.e. for the next two submits). */
+GPUD_API int gpud_serve_channel_output(gpud_serve* s, int64_t ticket, int64_t channel_id, const gpud_float2** iq,
+                                       int* n_iq, const float** audio, int* n_audio, int* rate);
+
 #ifdef __cplusplus
 }
 #endif

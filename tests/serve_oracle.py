@@ -1,0 +1,63 @@
+"""Oracle model of one served channel: the reference's streaming extractor
+(streaming_host_core.go) followed by the reference demodulators, with the
+block-to-block state the service carries.  Test infrastructure only."""
+import numpy as np
+
+import oracle as O
+
+NONE, NFM, WFM, AM, USB, LSB, CW = range(7)
+
+
+class OracleChannel:
+    def __init__(self, sample_rate, offset_hz, bandwidth, out_rate, num_taps=101, demod=NONE, window=0):
+        self.sr = sample_rate
+        self.off = offset_hz
+        self.decim = O.ExactIntegerDecimation(sample_rate, out_rate)
+        self.T = num_taps if num_taps > 0 else 101
+        cutoff = max(bandwidth / 2, 200)
+        design = O.LowpassFIRHamming if window == 1 else O.LowpassFIR
+        base = design(cutoff, sample_rate, self.T).astype(np.float32)[:self.T]
+        self.poly = O.BuildPolyphaseTapsPhaseMajor(base, self.decim)
+        self.phase, self.pc, self.hist = 0.0, 0, np.zeros(0, np.complex64)
+        self.demod = demod
+        self.out_rate = out_rate
+        self.fm = O.BridgedFMDiscrim()
+        self.bfo = 0.0
+
+    def block(self, iq):
+        out, self.phase, self.pc, self.hist = O.runStreamingPolyphaseHostCore(
+            iq, self.sr, self.off, self.phase, self.pc, self.T, self.decim, self.hist, self.poly)
+        if self.demod in (NFM, WFM):
+            audio = self.fm.Demod(out)
+        elif self.demod == AM:
+            audio = O.AMDemod(out)
+        elif self.demod in (USB, LSB):
+            audio, self.bfo = O.SSBDemod(out, self.out_rate, self.demod == USB, self.bfo)
+        elif self.demod == CW:
+            audio, self.bfo = O.CWDemod(out, self.out_rate, self.bfo)
+        else:
+            audio = np.zeros(0, np.float32)
+        return out, audio
+
+
+def synth_scene(n, sr, carriers, seed=0, noise=0.05):
+    """Sum of modulated carriers + complex AWGN.  carriers: (kind, offset_hz, level)."""
+    rng = np.random.default_rng(seed)
+    t = np.arange(n) / sr
+    x = noise * (rng.standard_normal(n) + 1j * rng.standard_normal(n)) / np.sqrt(2)
+    for kind, f0, lvl in carriers:
+        if kind == "fm":
+            dev, fm = 5000.0, 1000.0
+            ph = 2 * np.pi * f0 * t + dev / fm * np.sin(2 * np.pi * fm * t)
+            x += lvl * np.exp(1j * ph)
+        elif kind == "wfm":
+            dev, fm = 75000.0, 1000.0
+            ph = 2 * np.pi * f0 * t + dev / fm * np.sin(2 * np.pi * fm * t)
+            x += lvl * np.exp(1j * ph)
+        elif kind == "am":
+            x += lvl * (1 + 0.5 * np.cos(2 * np.pi * 700 * t)) * np.exp(2j * np.pi * f0 * t)
+        elif kind == "ssb":
+            x += lvl * np.exp(2j * np.pi * (f0 + 900) * t)
+        else:
+            x += lvl * np.exp(2j * np.pi * f0 * t)
+    return x.astype(np.complex64)
