@@ -1,0 +1,186 @@
+"""Surveillance binding: Welch PSD, FFT, CFAR runs and the detection list
+(gpud_surv_* in include/argus_gpu.h).  Mirrors internal/fft (WelchPSD,
+Spectrum), internal/fft/gpufft (Engine.Exec) and the threshold path of
+internal/detector."""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+
+import numpy as np
+
+from ._lib import check, lib, register
+
+CFAR_CA, CFAR_GOSCA, CFAR_CASO = 0, 1, 2
+_MODES = {"CA": CFAR_CA, "GOSCA": CFAR_GOSCA, "CASO": CFAR_CASO}
+
+
+class gpud_cfar_params(C.Structure):
+    _fields_ = [("mode", C.c_int32), ("guard", C.c_int32), ("train", C.c_int32), ("wrap", C.c_int32),
+                ("scale_db", C.c_double), ("hysteresis_db", C.c_double)]
+
+
+class gpud_surv_run(C.Structure):
+    _fields_ = [("first_bin", C.c_int32), ("last_bin", C.c_int32), ("peak_bin", C.c_int32), ("pad", C.c_int32),
+                ("peak_db", C.c_double), ("noise_db", C.c_double)]
+
+
+class gpud_surv_signal(C.Structure):
+    _fields_ = [("first_bin", C.c_int32), ("last_bin", C.c_int32), ("peak_bin", C.c_int32), ("pad", C.c_int32),
+                ("center_hz", C.c_double), ("bw_hz", C.c_double), ("peak_db", C.c_double), ("snr_db", C.c_double),
+                ("noise_db", C.c_double)]
+
+
+class gpud_detector_params(C.Structure):
+    _fields_ = [("threshold_db", C.c_double), ("hysteresis_db", C.c_double), ("cfar_scale_db", C.c_double),
+                ("edge_margin_db", C.c_double), ("max_signal_bw_hz", C.c_double), ("merge_gap_hz", C.c_double),
+                ("center_hz", C.c_double)]
+
+
+P, I, L64 = C.c_void_p, C.c_int, C.c_int64
+register("gpud_surv_create", I, [I, I, C.POINTER(P)])
+register("gpud_surv_destroy", I, [P])
+register("gpud_surv_stream", P, [P])
+register("gpud_surv_welch_device", I, [P, P, L64, I, I, I, P])
+register("gpud_surv_fft_device", I, [P, P, L64, I, I, P])
+register("gpud_surv_cfar_runs_device", I, [P, P, I, C.POINTER(gpud_cfar_params), P, P, I])
+register("gpud_surv_finish_frame", I, [P, I, I, P, I, C.POINTER(gpud_detector_params), P, I])
+register("gpud_surv_detect", I, [P, P, L64, I, I, I, C.POINTER(gpud_cfar_params), C.POINTER(gpud_detector_params),
+                                 P, I, P, P])
+
+
+@dataclass
+class Signal:
+    FirstBin: int
+    LastBin: int
+    PeakBin: int
+    CenterHz: float
+    BWHz: float
+    PeakDb: float
+    SNRDb: float
+    NoiseDb: float
+
+
+class DeviceBuffer:
+    """Device allocation through the C ABI (no torch)."""
+
+    def __init__(self, nbytes: int):
+        p = C.c_void_p()
+        check("gpud_malloc", lib().gpud_malloc(C.byref(p), max(nbytes, 1)))
+        self.ptr = p.value
+        self.nbytes = nbytes
+
+    def upload(self, a: np.ndarray) -> "DeviceBuffer":
+        a = np.ascontiguousarray(a)
+        assert a.nbytes <= self.nbytes
+        check("gpud_memcpy_h2d", lib().gpud_memcpy_h2d(self.ptr, a.ctypes.data, a.nbytes))
+        return self
+
+    def download(self, dtype, count) -> np.ndarray:
+        out = np.zeros(count, dtype)
+        check("gpud_memcpy_d2h", lib().gpud_memcpy_d2h(out.ctypes.data, self.ptr, out.nbytes))
+        return out
+
+    def free(self):
+        if self.ptr:
+            lib().gpud_free(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def cfar_params(mode="CA", guard=4, train=16, scale_db=12.3748, wrap=True, hysteresis_db=3.0):
+    return gpud_cfar_params(_MODES[mode] if isinstance(mode, str) else mode, guard, train, 1 if wrap else 0,
+                            scale_db, hysteresis_db)
+
+
+def detector_params(threshold_db=-20.0, hysteresis_db=3.0, cfar_scale_db=12.3748, edge_margin_db=3.0,
+                    max_signal_bw_hz=150000.0, merge_gap_hz=5000.0, center_hz=0.0):
+    return gpud_detector_params(threshold_db, hysteresis_db, cfar_scale_db, edge_margin_db, max_signal_bw_hz,
+                                merge_gap_hz, center_hz)
+
+
+class Surveillance:
+    """gpufft.Engine successor plus the Welch / CFAR / detection-list pass."""
+
+    def __init__(self, n_fft: int, sample_rate: int):
+        h = C.c_void_p()
+        rc = lib().gpud_surv_create(n_fft, sample_rate, C.byref(h))
+        check("gpud_surv_create", rc)
+        self._h = h
+        self.n = n_fft
+        self.sample_rate = sample_rate
+
+    def Close(self):
+        if self._h:
+            lib().gpud_surv_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.Close()
+        except Exception:
+            pass
+
+    def sync(self):
+        check("gpud_stream_sync", lib().gpud_stream_sync(lib().gpud_surv_stream(self._h)))
+
+    def Exec(self, iq, window=False) -> np.ndarray:
+        """gpu.go:67-91: forward unnormalised FFT of one or more frames."""
+        iq = np.ascontiguousarray(iq, np.complex64)
+        frames = len(iq) // self.n
+        din = DeviceBuffer(iq.nbytes).upload(iq)
+        dout = DeviceBuffer(iq.nbytes)
+        check("gpud_surv_fft_device", lib().gpud_surv_fft_device(self._h, din.ptr, self.n, frames, 1 if window else 0,
+                                                                 dout.ptr))
+        self.sync()
+        return dout.download(np.complex64, frames * self.n)
+
+    def WelchDevice(self, iq_dev: int, frame_step: int, seg_hop: int, n_segs: int, n_frames: int, psd_dev: int):
+        check("gpud_surv_welch_device", lib().gpud_surv_welch_device(self._h, iq_dev, frame_step, seg_hop, n_segs,
+                                                                     n_frames, psd_dev))
+
+    def WelchPSD(self, iq, n_segs: int, seg_hop: int, frame_step: int, n_frames: int) -> np.ndarray:
+        iq = np.ascontiguousarray(iq, np.complex64)
+        din = DeviceBuffer(iq.nbytes).upload(iq)
+        dpsd = DeviceBuffer(4 * self.n * n_frames)
+        self.WelchDevice(din.ptr, frame_step, seg_hop, n_segs, n_frames, dpsd.ptr)
+        self.sync()
+        return dpsd.download(np.float32, self.n * n_frames).reshape(n_frames, self.n)
+
+    def CFARRuns(self, psd, cfar: gpud_cfar_params, max_runs: int = 2048):
+        psd = np.ascontiguousarray(psd, np.float32).reshape(-1, self.n)
+        nf = psd.shape[0]
+        dpsd = DeviceBuffer(psd.nbytes).upload(psd)
+        druns = DeviceBuffer(C.sizeof(gpud_surv_run) * max_runs * nf)
+        dcnt = DeviceBuffer(4 * nf)
+        check("gpud_surv_cfar_runs_device", lib().gpud_surv_cfar_runs_device(self._h, dpsd.ptr, nf, C.byref(cfar),
+                                                                             druns.ptr, dcnt.ptr, max_runs))
+        self.sync()
+        counts = dcnt.download(np.int32, nf)
+        raw = (gpud_surv_run * (max_runs * nf))()
+        check("gpud_memcpy_d2h", lib().gpud_memcpy_d2h(C.addressof(raw), druns.ptr, C.sizeof(raw)))
+        out = []
+        for f in range(nf):
+            out.append([raw[f * max_runs + i] for i in range(min(int(counts[f]), max_runs))])
+        return counts, out
+
+    def Detect(self, iq, n_segs, seg_hop, frame_step, n_frames, cfar, det, cap=1024, want_psd=True):
+        iq = np.ascontiguousarray(iq, np.complex64)
+        din = DeviceBuffer(iq.nbytes).upload(iq)
+        sig = (gpud_surv_signal * (cap * n_frames))()
+        counts = np.zeros(n_frames, np.int32)
+        psd = np.zeros(n_frames * self.n, np.float32) if want_psd else None
+        check("gpud_surv_detect", lib().gpud_surv_detect(self._h, din.ptr, frame_step, seg_hop, n_segs, n_frames,
+                                                         C.byref(cfar), C.byref(det), C.addressof(sig), cap,
+                                                         counts.ctypes.data,
+                                                         psd.ctypes.data if psd is not None else None))
+        lists = []
+        for f in range(n_frames):
+            lists.append([Signal(s.first_bin, s.last_bin, s.peak_bin, s.center_hz, s.bw_hz, s.peak_db, s.snr_db,
+                                 s.noise_db) for s in sig[f * cap:f * cap + min(int(counts[f]), cap)]])
+        return lists, (psd.reshape(n_frames, self.n) if psd is not None else None)

@@ -1,0 +1,535 @@
+// surv.hip -- wideband surveillance: windowed FFT / Welch PSD, CFAR and the
+// detection list (successors of internal/fft, internal/cfar and the
+// threshold path of internal/detector).
+//
+// FFT of N = R^3 points (R = 32: 32768; R = 16: 4096) by one workgroup of
+// R^2 threads.  Index n = a + R b + R^2 c; thread (a, b) holds the R samples
+// c = 0..R-1 in registers:
+//   pass 1  DFT_R over c, twiddle W_N^{(a + R b) kc}
+//   LDS transpose -> thread (a, kc) holds b = 0..R-1
+//   pass 2  DFT_R over b, twiddle W_{R^2}^{a kb}
+//   LDS transpose (XOR-swizzled) -> thread (kb, kc) holds a = 0..R-1
+//   pass 3  DFT_R over a  ->  X[kc + R kb + R^2 ka]
+// Each transpose moves the real parts, then the imaginary parts, through a
+// 4N-byte LDS array, so a 32768-point transform needs 128 KiB of LDS and no
+// HBM round trip.  Welch segments of one output frame are transformed back to
+// back by the same workgroup and their |X|^2 accumulates in registers.
+#include "common.hpp"
+#include "fft_twiddles.hpp"
+#include "surv.hpp"
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+namespace {
+
+template <int R>
+__host__ __device__ constexpr int brev(int i)
+{
+    int r = 0;
+    for (int b = 1; b < R; b <<= 1) {
+        r <<= 1;
+        if (i & b) r |= 1;
+    }
+    return r;
+}
+
+// In-register forward DFT of R points (radix-2, decimation in time).
+template <int R>
+__device__ __forceinline__ void dft(float2 (&v)[R])
+{
+#pragma unroll
+    for (int i = 0; i < R; ++i) {
+        const int j = brev<R>(i);
+        if (i < j) {
+            const float2 t = v[i];
+            v[i] = v[j];
+            v[j] = t;
+        }
+    }
+#pragma unroll
+    for (int len = 2; len <= R; len <<= 1) {
+        const int half = len >> 1;
+#pragma unroll
+        for (int i = 0; i < R; i += len) {
+#pragma unroll
+            for (int k = 0; k < half; ++k) {
+                const float2 a = v[i + k], b = v[i + k + half];
+                float2 tb;
+                if (k == 0) {
+                    tb = b;
+                } else if (4 * k == len) {  // W = -i
+                    tb = make_float2(b.y, -b.x);
+                } else {
+                    const int wi = k * (32 / len);
+                    const float cr = argus_fft::W32_RE[wi], ci = argus_fft::W32_IM[wi];
+                    tb = make_float2(b.x * cr - b.y * ci, b.x * ci + b.y * cr);
+                }
+                v[i + k] = make_float2(a.x + tb.x, a.y + tb.y);
+                v[i + k + half] = make_float2(a.x - tb.x, a.y - tb.y);
+            }
+        }
+    }
+}
+
+// e^{-2 pi i e / M}, e in [0, M), M a power of two: exact turn argument.
+__device__ __forceinline__ float2 twiddle(int e, float inv_m)
+{
+    const float x = -(float)e * inv_m;
+    return make_float2(__builtin_amdgcn_cosf(x), __builtin_amdgcn_sinf(x));
+}
+
+__device__ __forceinline__ float2 cmul(float2 a, float2 w)
+{
+    return make_float2(a.x * w.x - a.y * w.y, a.x * w.y + a.y * w.x);
+}
+
+// Transpose through LDS: thread writes its R values to wr(k), barrier, reads
+// R values from rd(k); real and imaginary parts go in two rounds.
+template <int R, typename WR, typename RD>
+__device__ __forceinline__ void exchange(float2 (&v)[R], float* xs, WR wr, RD rd)
+{
+    // each half is read back into the registers it was written from, so the
+    // transpose needs no extra registers
+#pragma unroll
+    for (int k = 0; k < R; ++k) xs[wr(k)] = v[k].x;
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < R; ++k) v[k].x = xs[rd(k)];
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < R; ++k) xs[wr(k)] = v[k].y;
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < R; ++k) v[k].y = xs[rd(k)];
+    __syncthreads();
+}
+
+// Full N = R^3 forward transform of the values each thread holds
+// (thread t = a + R b holds c = 0..R-1).  On return thread t = kb + R kc
+// holds X[kc + R kb + R^2 ka] in v[ka].
+template <int R>
+__device__ __forceinline__ void fft_r3(float2 (&v)[R], float* xs)
+{
+    constexpr int TH = R * R;
+    constexpr float invN = 1.0f / (float)(R * R * R);
+    constexpr float invRR = 1.0f / (float)(R * R);
+    int t = threadIdx.x;
+    // opaque to the optimiser: keeps the twiddles (a function of t only)
+    // from being hoisted out of a caller's segment loop into scratch memory
+    asm volatile("" : "+v"(t));
+    dft<R>(v);
+#pragma unroll
+    for (int k = 1; k < R; ++k) v[k] = cmul(v[k], twiddle(t * k, invN));
+    {
+        const int a = t % R, kc = t / R;
+        exchange<R>(v, xs, [&](int k) { return t + TH * k; }, [&](int b) { return a + R * b + TH * kc; });
+    }
+    dft<R>(v);
+    {
+        const int a = t % R, kc = t / R;
+#pragma unroll
+        for (int k = 1; k < R; ++k) v[k] = cmul(v[k], twiddle(a * k, invRR));
+        // padded layout a + (R+1) kb + R(R+1) kc: odd strides both ways, so
+        // neither the writes (lanes vary a) nor the reads (lanes vary kb)
+        // conflict, and every address is affine in the unrolled index
+        const int kb2 = t % R, kc2 = t / R;
+        exchange<R>(
+            v, xs, [&](int kb) { return a + (R + 1) * kb + R * (R + 1) * kc; },
+            [&](int aa) { return aa + (R + 1) * kb2 + R * (R + 1) * kc2; });
+    }
+    dft<R>(v);
+}
+
+// LDS floats the transforms need: the padded R x (R+1) x R image.
+template <int R>
+constexpr int fft_lds_floats()
+{
+    return R * R * (R + 1);
+}
+
+template <int R>
+__global__ __launch_bounds__(R* R) void argus_welch_kernel(const float2* __restrict__ in, int64_t frame_step,
+                                                           int seg_hop, int n_segs, const float* __restrict__ win,
+                                                           float* __restrict__ out_db)
+{
+    constexpr int N = R * R * R, TH = R * R, PL = R * (R + 1);
+    __shared__ float xs[fft_lds_floats<R>()];
+    const int t = threadIdx.x;
+    const int64_t base = (int64_t)blockIdx.x * frame_step;
+    const float invN = 1.0f / (float)N;
+    float acc[R];
+#pragma unroll
+    for (int k = 0; k < R; ++k) acc[k] = 0.0f;
+    for (int seg = 0; seg < n_segs; ++seg) {
+        const float2* x = in + base + (int64_t)seg * seg_hop;
+        float2 v[R];
+        int tl = t;
+        asm volatile("" : "+v"(tl));  // per segment, so window loads are not hoisted into scratch
+#pragma unroll
+        for (int c = 0; c < R; ++c) {
+            const int idx = tl + TH * c;
+            const float2 s = x[idx];
+            const float w = win[idx];
+            v[c] = make_float2(s.x * w, s.y * w);
+        }
+        fft_r3<R>(v, xs);
+#pragma unroll
+        for (int k = 0; k < R; ++k) {
+            const float mr = v[k].x * invN, mi = v[k].y * invN;
+            acc[k] = fmaf(mr, mr, fmaf(mi, mi, acc[k]));
+        }
+    }
+    // thread t = kb + R kc holds bin k = kc + R kb + R^2 ka; fftshift puts it
+    // at i = kc + R kb + R^2 ((ka + R/2) mod R).  Stage through the padded
+    // LDS image (odd stride R+1 between lanes) for coalesced stores.
+    const int kb = t % R, kc = t / R;
+    const float inv_segs = 1.0f / (float)(n_segs > 0 ? n_segs : 1);
+#pragma unroll
+    for (int ka = 0; ka < R; ++ka) {
+        const int q = (ka + R / 2) % R;
+        xs[kc + (R + 1) * kb + PL * q] = 10.0f * log10f(acc[ka] * inv_segs + 1e-24f);
+    }
+    __syncthreads();
+    float* o = out_db + (int64_t)blockIdx.x * N;
+    const int jc = t % R, jb = t / R;
+#pragma unroll
+    for (int q = 0; q < R; ++q) o[t + TH * q] = xs[jc + (R + 1) * jb + PL * q];
+}
+
+// Plain forward transform of one frame, natural-order complex output
+// (the successor of cufftExecC2C(CUFFT_FORWARD), gpufft/gpu.go:67-91),
+// optionally windowed.
+template <int R>
+__global__ __launch_bounds__(R* R) void argus_fft_kernel(const float2* __restrict__ in, const float* __restrict__ win,
+                                                         float2* __restrict__ out, int64_t frame_step)
+{
+    constexpr int N = R * R * R, TH = R * R, PL = R * (R + 1);
+    __shared__ float xs[fft_lds_floats<R>()];
+    const int t = threadIdx.x;
+    const float2* x = in + (int64_t)blockIdx.x * frame_step;
+    float2 v[R];
+#pragma unroll
+    for (int c = 0; c < R; ++c) {
+        const int idx = t + TH * c;
+        float2 s = x[idx];
+        if (win) {
+            const float w = win[idx];
+            s = make_float2(s.x * w, s.y * w);
+        }
+        v[c] = s;
+    }
+    fft_r3<R>(v, xs);
+    const int kb = t % R, kc = t / R;
+    float2* o = out + (int64_t)blockIdx.x * N;
+    const int jc = t % R, jb = t / R;
+    // real parts then imaginary parts through the padded LDS image, each
+    // read back into the registers it came from
+#pragma unroll
+    for (int ka = 0; ka < R; ++ka) xs[kc + (R + 1) * kb + PL * ka] = v[ka].x;
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < R; ++q) v[q].x = xs[jc + (R + 1) * jb + PL * q];
+    __syncthreads();
+#pragma unroll
+    for (int ka = 0; ka < R; ++ka) xs[kc + (R + 1) * kb + PL * ka] = v[ka].y;
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < R; ++q) o[t + TH * q] = make_float2(v[q].x, xs[jc + (R + 1) * jb + PL * q]);
+}
+
+// ---------------------------------------------------------------------------
+// CFAR + run extraction, one workgroup of 1024 threads per frame.
+// ---------------------------------------------------------------------------
+constexpr int kCfarThreads = 1024;
+constexpr int kCfarTile = 16384;  // bins of linear power staged in LDS at a time
+
+__device__ __forceinline__ int wrap_bin(int i, int n, int wrap)
+{
+    if (wrap) return ((i % n) + n) % n;
+    return i < 0 ? 0 : (i >= n ? n - 1 : i);
+}
+
+__device__ __forceinline__ double cfar_combine(int mode, double left, double right, int train)
+{
+    if (mode == GPUD_CFAR_CA) return (left + right) / (double)(2 * train);
+    const double lm = left / (double)train, rm = right / (double)train;
+    if (mode == GPUD_CFAR_GOSCA) return lm > rm ? lm : rm;
+    return lm < rm ? lm : rm;  // CASO
+}
+
+// Threshold of one bin straight from the dB spectrum (used for peak bins).
+__device__ double cfar_threshold_at(const float* __restrict__ db, int n, int i, const gpud_cfar_params& p)
+{
+    double left = 0.0, right = 0.0;
+    for (int k = 1; k <= p.train; ++k) {
+        left += pow(10.0, (double)db[wrap_bin(i - p.guard - k, n, p.wrap)] / 10.0);
+        right += pow(10.0, (double)db[wrap_bin(i + p.guard + k, n, p.wrap)] / 10.0);
+    }
+    return 10.0 * log10(cfar_combine(p.mode, left, right, p.train)) + p.scale_db;
+}
+
+__global__ __launch_bounds__(kCfarThreads) void argus_cfar_runs_kernel(const float* __restrict__ psd, int n,
+                                                                       gpud_cfar_params p,
+                                                                       gpud_surv_run* __restrict__ runs,
+                                                                       int* __restrict__ run_counts, int max_runs)
+{
+    extern __shared__ double lin[];  // kCfarTile + 2 (guard + train) doubles, reused below
+    const int t = threadIdx.x;
+    const float* db = psd + (int64_t)blockIdx.x * n;
+    const int halo = p.guard + p.train;
+    const int tile = n < kCfarTile ? n : kCfarTile;
+    uint32_t* status = (uint32_t*)(lin + tile + 2 * halo);  // 2 bits per bin
+
+    // ---- CFAR status per bin: 2 = v >= thr, 0 = v < thr - hysteresis, 1 = between
+    for (int w = t; w < n / 16; w += kCfarThreads) status[w] = 0u;
+    for (int t0 = 0; t0 < n; t0 += tile) {
+        __syncthreads();  // previous tile's readers are done with lin
+        for (int i = t; i < tile + 2 * halo; i += kCfarThreads) {
+            const int b = wrap_bin(t0 - halo + i, n, p.wrap);
+            lin[i] = pow(10.0, (double)db[b] / 10.0);
+        }
+        __syncthreads();
+        // consecutive lanes take consecutive bins: conflict-free LDS reads.
+        // Window sums are direct (k = 1..train in order), the threshold of
+        // ca.go:50-59 / gosca.go / caso.go without the running-sum drift.
+        for (int j = t; j < tile; j += kCfarThreads) {
+            double left = 0.0, right = 0.0;
+            for (int k = 1; k <= p.train; ++k) {
+                left += lin[j - p.guard - k + halo];
+                right += lin[j + p.guard + k + halo];
+            }
+            const double thr = 10.0 * log10(cfar_combine(p.mode, left, right, p.train)) + p.scale_db;
+            const int b = t0 + j;
+            const double v = (double)db[b];
+            const uint32_t code = v >= thr ? 2u : (v < thr - p.hysteresis_db ? 0u : 1u);
+            if (code) atomicOr(&status[b >> 4], code << ((b & 15) * 2));
+        }
+    }
+    __syncthreads();
+
+    // ---- run extraction: scan of the "in a run" state over bins -----------
+    const int bpt = n / kCfarThreads;  // bins owned by each thread
+    const int first = t * bpt;
+    auto code_at = [&](int b) { return (int)((status[b >> 4] >> ((b & 15) * 2)) & 3u); };
+    int f = -1;  // transfer of this chunk: -1 identity, 0/1 constant
+    for (int j = 0; j < bpt; ++j) {
+        const int c = code_at(first + j);
+        if (c == 2) f = 1;
+        else if (c == 0) f = 0;
+    }
+    int* scan = (int*)lin;  // kCfarThreads ints (the linear tile is no longer needed)
+    scan[t] = f;
+    __syncthreads();
+    for (int off = 1; off < kCfarThreads; off <<= 1) {
+        const int mine = scan[t];
+        const int prev = t >= off ? scan[t - off] : -1;
+        __syncthreads();
+        scan[t] = mine != -1 ? mine : prev;  // the last constant wins
+        __syncthreads();
+    }
+    int in_state = t > 0 ? scan[t - 1] : -1;
+    if (in_state < 0) in_state = 0;
+    // events: starts and ends in this chunk
+    int ns = 0, ne = 0;
+    {
+        int st = in_state;
+        for (int j = 0; j < bpt; ++j) {
+            const int c = code_at(first + j);
+            const int nx = c == 2 ? 1 : (c == 0 ? 0 : st);
+            if (!st && nx) ns++;
+            if (st && !nx) ne++;
+            st = nx;
+        }
+        if (t == kCfarThreads - 1 && st) ne++;  // a run still open at the last bin ends there
+    }
+    __syncthreads();
+    int* cs = scan;                  // start counts
+    int* ce = scan + kCfarThreads;   // end counts
+    cs[t] = ns;
+    ce[t] = ne;
+    __syncthreads();
+    for (int off = 1; off < kCfarThreads; off <<= 1) {
+        const int a = cs[t], b = ce[t];
+        const int pa = t >= off ? cs[t - off] : 0, pb = t >= off ? ce[t - off] : 0;
+        __syncthreads();
+        cs[t] = a + pa;
+        ce[t] = b + pb;
+        __syncthreads();
+    }
+    const int total = cs[kCfarThreads - 1];
+    int* starts = scan + 2 * kCfarThreads;
+    int* ends = starts + max_runs;
+    {
+        int ps = cs[t] - ns, pe = ce[t] - ne;
+        int st = in_state;
+        for (int j = 0; j < bpt; ++j) {
+            const int b = first + j;
+            const int c = code_at(b);
+            const int nx = c == 2 ? 1 : (c == 0 ? 0 : st);
+            if (!st && nx) {
+                if (ps < max_runs) starts[ps] = b;
+                ps++;
+            }
+            if (st && !nx) {
+                if (pe < max_runs) ends[pe] = b - 1;
+                pe++;
+            }
+            st = nx;
+        }
+        if (t == kCfarThreads - 1 && st) {
+            if (pe < max_runs) ends[pe] = n - 1;
+        }
+    }
+    __syncthreads();
+    const int nr = total < max_runs ? total : max_runs;
+    gpud_surv_run* out = runs + (int64_t)blockIdx.x * max_runs;
+    for (int r = t; r < nr; r += kCfarThreads) {
+        const int s0 = starts[r], e0 = ends[r];
+        int pk = s0;
+        double pv = -1e300;
+        for (int b = s0; b <= e0; ++b)
+            if (code_at(b) == 2) {
+                const double v = (double)db[b];
+                if (v > pv) {
+                    pv = v;
+                    pk = b;
+                }
+            }
+        gpud_surv_run rr;
+        rr.first_bin = s0;
+        rr.last_bin = e0;
+        rr.peak_bin = pk;
+        rr.pad = 0;
+        rr.peak_db = pv;
+        rr.noise_db = cfar_threshold_at(db, n, pk, p) - p.scale_db;
+        out[r] = rr;
+    }
+    if (t == 0) run_counts[blockIdx.x] = total;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// Host side
+// ---------------------------------------------------------------------------
+static int fft_radix(int n)
+{
+    if (n == 32768) return 32;
+    if (n == 4096) return 16;
+    return 0;
+}
+
+size_t argus_cfar_lds_bytes(int n, const gpud_cfar_params& p)
+{
+    const int tile = n < kCfarTile ? n : kCfarTile;
+    const size_t a = (size_t)(tile + 2 * (p.guard + p.train)) * sizeof(double) + (size_t)(n / 16) * 4;
+    const size_t b = (size_t)(2 * kCfarThreads) * 4 + 0;  // scans (runs come after, see below)
+    (void)b;
+    return a;
+}
+
+extern "C" {
+
+GPUD_API int gpud_surv_create(int n_fft, int sample_rate, gpud_surv** out)
+{
+    if (!out || fft_radix(n_fft) == 0 || sample_rate <= 0) return -1;
+    gpud_surv* s = new gpud_surv();
+    s->n = n_fft;
+    s->sample_rate = sample_rate;
+    ARGUS_CHECK(hipStreamCreate(&s->stream));
+    // symmetric Hann (internal/fft/fft.go:5-17), FP64 then rounded
+    std::vector<float> w((size_t)n_fft);
+    for (int i = 0; i < n_fft; i++) w[(size_t)i] = (float)(0.5 * (1 - std::cos(2 * M_PI * (double)i / (double)(n_fft - 1))));
+    ARGUS_CHECK(hipMalloc((void**)&s->d_win, sizeof(float) * (size_t)n_fft));
+    ARGUS_CHECK(hipMemcpy(s->d_win, w.data(), sizeof(float) * (size_t)n_fft, hipMemcpyHostToDevice));
+    *out = s;
+    return 0;
+}
+
+GPUD_API int gpud_surv_destroy(gpud_surv* s)
+{
+    if (!s) return 0;
+    hipStreamSynchronize(s->stream);
+    hipFree(s->d_win);
+    hipStreamDestroy(s->stream);
+    delete s;
+    return 0;
+}
+
+GPUD_API gpud_stream_handle gpud_surv_stream(gpud_surv* s) { return s ? (gpud_stream_handle)s->stream : nullptr; }
+
+// Welch PSD (internal/fft/welch.go:15-66) of n_frames output frames: frame f
+// averages n_segs Hann-windowed segments starting at f*frame_step +
+// s*seg_hop.  Output: dB, fftshifted, float32 [n_frames][n_fft] on the
+// device.  The caller guarantees (n_frames-1)*frame_step +
+// (n_segs-1)*seg_hop + n_fft samples are readable.
+GPUD_API int gpud_surv_welch_device(gpud_surv* s, const gpud_float2* iq_dev, int64_t frame_step, int seg_hop,
+                                    int n_segs, int n_frames, float* psd_dev)
+{
+    if (!s || !iq_dev || !psd_dev || n_segs <= 0 || seg_hop < 0 || frame_step < 0) return -1;
+    if (n_frames <= 0) return 0;
+    const int R = fft_radix(s->n);
+    if (R == 32)
+        hipLaunchKernelGGL(argus_welch_kernel<32>, dim3(n_frames), dim3(1024), 0, s->stream, (const float2*)iq_dev,
+                           frame_step, seg_hop, n_segs, s->d_win, psd_dev);
+    else
+        hipLaunchKernelGGL(argus_welch_kernel<16>, dim3(n_frames), dim3(256), 0, s->stream, (const float2*)iq_dev,
+                           frame_step, seg_hop, n_segs, s->d_win, psd_dev);
+    return (int)hipGetLastError();
+}
+
+// Forward unnormalised C2C FFT of n_frames frames (frame f at f*frame_step),
+// natural-order output [n_frames][n_fft]; window 1 applies the Hann window
+// (the host multiply of pipeline_runtime.go:315-341 fused in).
+GPUD_API int gpud_surv_fft_device(gpud_surv* s, const gpud_float2* in_dev, int64_t frame_step, int n_frames,
+                                  int window, gpud_float2* out_dev)
+{
+    if (!s || !in_dev || !out_dev) return -1;
+    if (n_frames <= 0) return 0;
+    const float* w = window ? s->d_win : nullptr;
+    if (fft_radix(s->n) == 32)
+        hipLaunchKernelGGL(argus_fft_kernel<32>, dim3(n_frames), dim3(1024), 0, s->stream, (const float2*)in_dev, w,
+                           (float2*)out_dev, frame_step);
+    else
+        hipLaunchKernelGGL(argus_fft_kernel<16>, dim3(n_frames), dim3(256), 0, s->stream, (const float2*)in_dev, w,
+                           (float2*)out_dev, frame_step);
+    return (int)hipGetLastError();
+}
+
+// CA / GOSCA / CASO CFAR (internal/cfar/ca.go:22-68, gosca.go:22-78,
+// caso.go:16-70) on each frame of a dB spectrum, followed by the threshold
+// crossing with hysteresis of detector.go:436-480: runs (first, last, peak
+// bin, peak dB, noise = threshold[peak] - scale).  runs_dev holds n_frames *
+// max_runs records; run_counts_dev the true run count per frame (may exceed
+// max_runs, in which case the list is truncated).
+GPUD_API int gpud_surv_cfar_runs_device(gpud_surv* s, const float* psd_dev, int n_frames,
+                                        const gpud_cfar_params* params, gpud_surv_run* runs_dev,
+                                        int* run_counts_dev, int max_runs)
+{
+    if (!s || !psd_dev || !params || !runs_dev || !run_counts_dev || max_runs <= 0) return -1;
+    const gpud_cfar_params p = *params;
+    if (p.train <= 0 || p.guard < 0 || p.guard + p.train > 2048) return -1;
+    if (p.mode != GPUD_CFAR_CA && p.mode != GPUD_CFAR_GOSCA && p.mode != GPUD_CFAR_CASO) return -1;
+    if (s->n % kCfarThreads != 0) return -1;
+    if (n_frames <= 0) return 0;
+    size_t lds = argus_cfar_lds_bytes(s->n, p);
+    const size_t need_runs = (size_t)(2 * kCfarThreads + 2 * max_runs) * 4 + (size_t)(s->n / 16) * 4;
+    // status words live after the linear tile; the scan/run region reuses the
+    // tile, so it must fit below the status words
+    const int tile = s->n < kCfarTile ? s->n : kCfarTile;
+    if ((size_t)(2 * kCfarThreads + 2 * max_runs) * 4 > (size_t)(tile + 2 * (p.guard + p.train)) * 8) return -1;
+    (void)need_runs;
+    if (lds > 160 * 1024) return -1;
+    gpud_cfar_params pp = p;
+    if (pp.scale_db <= 0) pp.scale_db = 6;  // cfar.go:13-15
+    if (lds > 64 * 1024)
+        ARGUS_CHECK(hipFuncSetAttribute((const void*)argus_cfar_runs_kernel,
+                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    hipLaunchKernelGGL(argus_cfar_runs_kernel, dim3(n_frames), dim3(kCfarThreads), lds, s->stream, psd_dev, s->n, pp,
+                       runs_dev, run_counts_dev, max_runs);
+    return (int)hipGetLastError();
+}
+
+}  // extern "C"

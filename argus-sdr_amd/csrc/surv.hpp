@@ -1,0 +1,10 @@
+// surv.hpp -- surveillance engine state.
+#pragma once
+#include "common.hpp"
+
+struct gpud_surv {
+    int n = 0;             // FFT size
+    int sample_rate = 0;
+    hipStream_t stream = nullptr;
+    float* d_win = nullptr;  // symmetric Hann, n floats
+};

@@ -265,6 +265,87 @@ GPUD_API int gpud_serve_wait(gpud_serve* s, int64_t ticket, gpud_serve_block_inf
 GPUD_API int gpud_serve_channel_output(gpud_serve* s, int64_t ticket, int64_t channel_id, const gpud_float2** iq,
                                        int* n_iq, const float** audio, int* n_audio, int* rate);
 
+/* ------------------------------------------------------------------ */
+/* Surveillance: Welch PSD, FFT, CFAR and the detection list.           */
+/* Successors of gpufft.Engine (internal/fft/gpufft/gpu.go:33-91), the  */
+/* CPU WelchPSD (internal/fft/welch.go:15-66), internal/cfar and the    */
+/* threshold path of detector.detectSignals (detector.go:408-713).      */
+/* ------------------------------------------------------------------ */
+typedef struct gpud_surv gpud_surv;
+
+enum { GPUD_CFAR_CA = 0, GPUD_CFAR_GOSCA = 1, GPUD_CFAR_CASO = 2 };
+
+typedef struct gpud_cfar_params {
+    int32_t mode;          /* GPUD_CFAR_* (cfar/types.go:6-14) */
+    int32_t guard;         /* guard cells per side */
+    int32_t train;         /* training cells per side */
+    int32_t wrap;          /* 1 = wrap-around indexing, 0 = clamp to edge */
+    double  scale_db;      /* <= 0 -> 6 dB (cfar.go:13-15) */
+    double  hysteresis_db; /* run closes below threshold - hysteresis (detector.go:445) */
+} gpud_cfar_params;
+
+/* One threshold run of a frame (detector.go:436-480, before edge expansion). */
+typedef struct gpud_surv_run {
+    int32_t first_bin;
+    int32_t last_bin;
+    int32_t peak_bin;
+    int32_t pad;
+    double  peak_db;
+    double  noise_db;      /* threshold[peak] - scale_db */
+} gpud_surv_run;
+
+/* detector.go:82-94 Signal fields of the detection list. */
+typedef struct gpud_surv_signal {
+    int32_t first_bin;
+    int32_t last_bin;
+    int32_t peak_bin;
+    int32_t pad;
+    double  center_hz;
+    double  bw_hz;
+    double  peak_db;
+    double  snr_db;
+    double  noise_db;
+} gpud_surv_signal;
+
+/* detector.go:101-261 Detector fields the detection list reads. */
+typedef struct gpud_detector_params {
+    double threshold_db;      /* used where a threshold is NaN */
+    double hysteresis_db;
+    double cfar_scale_db;
+    double edge_margin_db;    /* <= 0 -> 3 */
+    double max_signal_bw_hz;
+    double merge_gap_hz;
+    double center_hz;
+} gpud_detector_params;
+
+/* gpu.go:33-51 New(n): n_fft in {4096, 32768}.  -1 otherwise. */
+GPUD_API int gpud_surv_create(int n_fft, int sample_rate, gpud_surv** out);
+GPUD_API int gpud_surv_destroy(gpud_surv* s);
+GPUD_API gpud_stream_handle gpud_surv_stream(gpud_surv* s);
+/* welch.go:15-66 per output frame: dB, fftshifted, float32 [n_frames][n_fft]. */
+GPUD_API int gpud_surv_welch_device(gpud_surv* s, const gpud_float2* iq_dev, int64_t frame_step, int seg_hop,
+                                    int n_segs, int n_frames, float* psd_dev);
+/* gpu.go:67-91 Exec: forward unnormalised C2C, natural order; window 1 =
+ * Hann applied on the device. */
+GPUD_API int gpud_surv_fft_device(gpud_surv* s, const gpud_float2* in_dev, int64_t frame_step, int n_frames,
+                                  int window, gpud_float2* out_dev);
+/* ca.go / gosca.go / caso.go + detector.go:436-480 per frame. */
+GPUD_API int gpud_surv_cfar_runs_device(gpud_surv* s, const float* psd_dev, int n_frames,
+                                        const gpud_cfar_params* params, gpud_surv_run* runs_dev,
+                                        int* run_counts_dev, int max_runs);
+/* detector.go:483-639 expandSignalEdges + mergeOverlapping +
+ * powerWeightedCenter on one frame's runs (host memory).  Returns the signal
+ * count (<= cap) or -1. */
+GPUD_API int gpud_surv_finish_frame(const float* psd, int n_bins, int sample_rate, const gpud_surv_run* runs,
+                                    int n_runs, const gpud_detector_params* params, gpud_surv_signal* out, int cap);
+/* Whole detection list for n_frames frames of device-resident IQ: Welch,
+ * CFAR, runs, expansion.  signals: n_frames * cap_per_frame records; counts
+ * per frame.  psd_host (optional, n_frames * n_fft floats) receives the
+ * spectra. */
+GPUD_API int gpud_surv_detect(gpud_surv* s, const gpud_float2* iq_dev, int64_t frame_step, int seg_hop, int n_segs,
+                              int n_frames, const gpud_cfar_params* cfar, const gpud_detector_params* det,
+                              gpud_surv_signal* signals, int cap_per_frame, int* counts, float* psd_host);
+
 #ifdef __cplusplus
 }
 #endif
