@@ -55,3 +55,50 @@ def surveillance_scene(n: int, fs: float, n_carriers: int = 40, seed: int = 0, s
             x += amp * multitone_fm(t, f0, bw, np.random.default_rng(cs))
         out[s0:s0 + m] = x.astype(np.complex64)
     return out, cars
+
+
+RDS_BAUD = 1187.5
+
+
+def rds_symbols(bits: np.ndarray) -> np.ndarray:
+    """Differentially encode data bits (s_i = s_{i-1} xor d_i) and map to NRZ
+    levels +-1; the decoder recovers d_i = h_i xor h_{i-1}
+    (rds.go:205-224)."""
+    s = np.bitwise_xor.accumulate(bits.astype(np.uint8)) if len(bits) else np.zeros(0, np.uint8)
+    return np.where(s > 0, 1.0, -1.0)
+
+
+class WFMStation:
+    """Broadcast stereo WFM station after genWFMDynamic (synth.go:269-356):
+    multitone L/R program, 19 kHz pilot (0.09), L-R DSB-SC at 38 kHz, 57 kHz
+    RDS NRZ BPSK (0.04) from a given symbol stream, +-75 kHz deviation.
+    Generated block by block with continuous phase."""
+
+    def __init__(self, fs: float, f0: float, symbols: np.ndarray, seed: int = 0, amp: float = 1.0):
+        rng = np.random.default_rng(seed)
+        self.fs, self.f0, self.amp = fs, f0, amp
+        self.sym = symbols
+        k = 12
+        self.loud = 0.25 + 0.75 * rng.random()
+        bw = 4000.0 + 11000.0 * rng.random()
+        self.lf = bw * np.arange(1, k + 1) / k
+        self.la, self.lp = rng.random(k), rng.random(k) * 2 * np.pi
+        self.ra, self.rp = rng.random(k), rng.random(k) * 2 * np.pi
+        peak = np.sum(self.la) if np.sum(self.la) > np.sum(self.ra) else np.sum(self.ra)
+        self.g = self.loud / max(peak, 1e-9)
+        self.phase = 0.0
+        self.n = 0
+
+    def block(self, m: int) -> np.ndarray:
+        t = (np.arange(m) + self.n) / self.fs
+        l = (self.la[None, :] * np.cos(2 * np.pi * self.lf[None, :] * t[:, None] + self.lp[None, :])).sum(1) * self.g
+        r = (self.ra[None, :] * np.cos(2 * np.pi * self.lf[None, :] * t[:, None] + self.rp[None, :])).sum(1) * self.g
+        mono, diff = 0.5 * (l + r), 0.5 * (l - r)
+        bi = np.minimum((t * RDS_BAUD).astype(np.int64), len(self.sym) - 1)
+        comp = (mono + diff * np.sin(2 * np.pi * 38000 * t) + 0.09 * np.sin(2 * np.pi * 19000 * t)
+                + 0.04 * self.sym[bi] * np.sin(2 * np.pi * 57000 * t))
+        inst = self.f0 + 75000.0 * comp
+        ph = self.phase + np.cumsum(2 * np.pi * inst / self.fs)
+        self.phase = float(np.mod(ph[-1], 2 * np.pi)) if m else self.phase
+        self.n += m
+        return (self.amp * np.exp(1j * ph)).astype(np.complex64)

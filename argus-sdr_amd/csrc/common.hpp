@@ -54,6 +54,12 @@ __device__ __forceinline__ float2 argus_rot(float2 v, float s, float c)
     return make_float2(v.x * c - v.y * s, v.x * s + v.y * c);
 }
 
+// Every launch site clears the runtime's last-error slot before launching
+// (argus_clear_error) and reads it right after, so it reports that launch's
+// own configuration error and not an earlier failed call the caller has
+// already seen.
+static inline void argus_clear_error() { (void)hipGetLastError(); }
+
 #define ARGUS_CHECK(expr)                                    \
     do {                                                     \
         hipError_t _e = (expr);                              \

@@ -89,6 +89,7 @@ int argus_launch_demod(const ArgusDemodDesc* d_desc, const ArgusDemodItem* d_ite
 {
     if (n_items <= 0) return 0;
     const int grid = n_items < 2048 ? n_items : 2048;
+    argus_clear_error();
     hipLaunchKernelGGL(argus_demod_kernel, dim3(grid), dim3(256), 0, stream, d_desc, d_items, n_items);
     return (int)hipGetLastError();
 }

@@ -194,6 +194,7 @@ int argus_launch_extract_one(const ArgusChanDesc& ch, const float2* in, int n_ne
     const int blocks = n_items + (ch.keep > 0 ? 1 : 0);
     if (blocks == 0) return 0;
     const size_t lds = n_items > 0 ? argus_item_lds_bytes(ch) : 0;
+    argus_clear_error();
     hipLaunchKernelGGL(argus_extract_one_kernel, dim3(blocks), dim3(ARGUS_EXTRACT_THREADS), lds, stream, ch, in,
                        n_new, n_items);
     return (int)hipGetLastError();
@@ -207,6 +208,7 @@ int argus_launch_extract_batch(const ArgusChanDesc* d_chans, const ArgusWorkItem
     // item list is ordered heaviest first so the tail is short.
     const int max_grid = 256 * 8;
     const int grid = n_items < max_grid ? n_items : max_grid;
+    argus_clear_error();
     hipLaunchKernelGGL(argus_extract_batch_kernel, dim3(grid), dim3(ARGUS_EXTRACT_THREADS), lds_bytes, stream,
                        d_chans, d_items, n_items, in, n_new);
     return (int)hipGetLastError();

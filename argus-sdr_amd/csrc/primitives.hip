@@ -90,6 +90,7 @@ int launch_fir(bool const_taps, const float2* in, float2* out, const float* taps
     if (n <= 0 || T <= 0 || T > 256) return 0;
     const size_t lds = (size_t)(kBlock + T - 1) * sizeof(float2);
     const int grid = (n + kBlock - 1) / kBlock;
+    argus_clear_error();
     if (const_taps)
         hipLaunchKernelGGL(k_fir<true>, dim3(grid), dim3(kBlock), lds, st, in, out, taps, n, T);
     else
@@ -188,6 +189,7 @@ GPUD_API int gpud_launch_freq_shift_stream_hip(const gpud_float2* in, gpud_float
                                                double phase_start, gpud_stream_handle stream)
 {
     if (n <= 0) return 0;
+    argus_clear_error();
     hipLaunchKernelGGL(k_freq_shift, dim3(grid_for(n)), dim3(kBlock), 0, (hipStream_t)stream, (const float2*)in,
                        (float2*)out, n, argus_rad_to_fix(phase_inc), argus_rad_to_fix(phase_start));
     return (int)hipGetLastError();
@@ -200,6 +202,7 @@ GPUD_API int gpud_launch_freq_shift_hip(const gpud_float2* in, gpud_float2* out,
 GPUD_API int gpud_launch_fm_discrim_hip(const gpud_float2* in, float* out, int n)
 {
     if (n <= 1) return 0;
+    argus_clear_error();
     hipLaunchKernelGGL(k_fm_discrim, dim3(grid_for(n)), dim3(kBlock), 0, 0, (const float2*)in, out, n);
     return (int)hipGetLastError();
 }
@@ -227,6 +230,7 @@ GPUD_API int gpud_launch_decimate_stream_hip(const gpud_float2* in, gpud_float2*
                                              gpud_stream_handle stream)
 {
     if (n_out <= 0 || factor <= 0) return 0;
+    argus_clear_error();
     hipLaunchKernelGGL(k_decimate, dim3(grid_for(n_out)), dim3(kBlock), 0, (hipStream_t)stream,
                        (const float2*)in, (float2*)out, n_out, factor);
     return (int)hipGetLastError();
@@ -238,6 +242,7 @@ GPUD_API int gpud_launch_decimate_hip(const gpud_float2* in, gpud_float2* out, i
 GPUD_API int gpud_launch_am_envelope_hip(const gpud_float2* in, float* out, int n)
 {
     if (n <= 0) return 0;
+    argus_clear_error();
     hipLaunchKernelGGL(k_am_envelope, dim3(grid_for(n)), dim3(kBlock), 0, 0, (const float2*)in, out, n);
     return (int)hipGetLastError();
 }
@@ -245,6 +250,7 @@ GPUD_API int gpud_launch_ssb_product_hip(const gpud_float2* in, float* out, int 
                                          double phase_start)
 {
     if (n <= 0) return 0;
+    argus_clear_error();
     hipLaunchKernelGGL(k_ssb_product, dim3(grid_for(n)), dim3(kBlock), 0, 0, (const float2*)in, out, n,
                        argus_rad_to_fix(phase_inc), argus_rad_to_fix(phase_start));
     return (int)hipGetLastError();

@@ -138,7 +138,7 @@ GPUD_API int gpud_serve_create(int sample_rate, int block_samples, uint64_t hbm_
     for (int i = 0; i < ARGUS_SERVE_SLOTS; i++) {
         ServeSlot& sl = s->slots[i];
         ARGUS_CHECK(hipEventCreateWithFlags(&sl.ev_in, hipEventDisableTiming));
-        ARGUS_CHECK(hipEventCreateWithFlags(&sl.ev_kern, hipEventDisableTiming));
+        ARGUS_CHECK(hipEventCreate(&sl.ev_kern));  // timed: kernel_ms in gpud_serve_wait
         ARGUS_CHECK(hipEventCreate(&sl.ev_done));
         ARGUS_CHECK(hipEventCreate(&sl.ev_start));
         ARGUS_CHECK(hipEventCreate(&sl.ev_kstart));

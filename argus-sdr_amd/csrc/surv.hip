@@ -471,6 +471,7 @@ GPUD_API int gpud_surv_welch_device(gpud_surv* s, const gpud_float2* iq_dev, int
     if (!s || !iq_dev || !psd_dev || n_segs <= 0 || seg_hop < 0 || frame_step < 0) return -1;
     if (n_frames <= 0) return 0;
     const int R = fft_radix(s->n);
+    argus_clear_error();
     if (R == 32)
         hipLaunchKernelGGL(argus_welch_kernel<32>, dim3(n_frames), dim3(1024), 0, s->stream, (const float2*)iq_dev,
                            frame_step, seg_hop, n_segs, s->d_win, psd_dev);
@@ -489,6 +490,7 @@ GPUD_API int gpud_surv_fft_device(gpud_surv* s, const gpud_float2* in_dev, int64
     if (!s || !in_dev || !out_dev) return -1;
     if (n_frames <= 0) return 0;
     const float* w = window ? s->d_win : nullptr;
+    argus_clear_error();
     if (fft_radix(s->n) == 32)
         hipLaunchKernelGGL(argus_fft_kernel<32>, dim3(n_frames), dim3(1024), 0, s->stream, (const float2*)in_dev, w,
                            (float2*)out_dev, frame_step);
@@ -527,6 +529,7 @@ GPUD_API int gpud_surv_cfar_runs_device(gpud_surv* s, const float* psd_dev, int 
     if (lds > 64 * 1024)
         ARGUS_CHECK(hipFuncSetAttribute((const void*)argus_cfar_runs_kernel,
                                         hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    argus_clear_error();
     hipLaunchKernelGGL(argus_cfar_runs_kernel, dim3(n_frames), dim3(kCfarThreads), lds, s->stream, psd_dev, s->n, pp,
                        runs_dev, run_counts_dev, max_runs);
     return (int)hipGetLastError();

@@ -514,3 +514,140 @@ def DetectSignals(spectrum, thresholds, *, sample_rate, center_hz=0.0, threshold
     m = lib().orc_detect_signals(C.byref(cfg), _p(s), _p(thr), n, out, cap, C.byref(ng))
     return [Signal(o.first_bin, o.last_bin, o.peak_bin, o.center_hz, o.bw_hz, o.peak_db, o.snr_db, o.noise_db)
             for o in out[:min(m, cap)]]
+
+
+# ---------------------------------------------------------------- WFM stereo / RDS
+class orc_fir_state(C.Structure):
+    _fields_ = [("taps", C.c_double * 103), ("delay", C.c_double * 103), ("nt", C.c_int), ("pad", C.c_int)]
+
+
+class orc_stereo_state(C.Structure):
+    _fields_ = [("lpf", orc_fir_state), ("bp_hi", orc_fir_state), ("bp_lo", orc_fir_state),
+                ("lr_lpf", orc_fir_state), ("pilot_hi", orc_fir_state), ("pilot_lo", orc_fir_state),
+                ("phase", C.c_double), ("freq", C.c_double), ("alpha", C.c_double), ("beta", C.c_double),
+                ("err_avg", C.c_double), ("i_state", C.c_double), ("q_state", C.c_double),
+                ("lp_alpha", C.c_double), ("rate", C.c_int), ("on_count", C.c_int), ("off_count", C.c_int),
+                ("enabled", C.c_int)]
+
+
+RDS_MAX_GROUPS = 4096
+
+
+class orc_rds_decoder(C.Structure):
+    _fields_ = [("ps", C.c_char * 8), ("rt", C.c_char * 64), ("pi", C.c_uint16), ("pad0", C.c_uint16),
+                ("costas_phase", C.c_double), ("costas_freq", C.c_double), ("mu", C.c_double),
+                ("prev", C.c_float * 2), ("prev_dec", C.c_float), ("has_prev", C.c_int), ("last_rate", C.c_int),
+                ("last_hard", C.c_int), ("has_last_hard", C.c_int), ("total_decodes", C.c_int),
+                ("groups_found", C.c_int), ("n_groups", C.c_int),
+                ("groups", (C.c_uint16 * 4) * RDS_MAX_GROUPS)]
+
+
+register("orc_pll_coefficients", None, [D, D, I, PD_, PD_])
+register("orc_stereo_init", None, [C.POINTER(orc_stereo_state), I])
+register("orc_stereo_decode", I, [C.POINTER(orc_stereo_state), P, I, P])
+register("orc_stereo_pilot_present", I, [P, I, I, PD_])
+register("orc_rds_syndrome", C.c_uint16, [C.c_uint32])
+register("orc_rds_init", None, [C.POINTER(orc_rds_decoder)])
+register("orc_rds_decode", I, [C.POINTER(orc_rds_decoder), P, I, I])
+register("orc_rds_block", C.c_uint32, [C.c_uint16, C.c_uint16])
+register("orc_rds_encode_groups", I, [C.c_uint16, C.c_char_p, C.c_char_p, I, P])
+
+
+def pllCoefficients(loop_bw, damping, sample_rate):
+    a, b = C.c_double(), C.c_double()
+    lib().orc_pll_coefficients(loop_bw, damping, sample_rate, C.byref(a), C.byref(b))
+    return a.value, b.value
+
+
+class StereoDecoder:
+    """streamer.go:1525-1643 stereoDecodeStateful + lock hysteresis."""
+
+    def __init__(self, sample_rate):
+        self.st = orc_stereo_state()
+        lib().orc_stereo_init(C.byref(self.st), sample_rate)
+
+    def Decode(self, mono):
+        mono = np.ascontiguousarray(mono, np.float32)
+        out = np.zeros(2 * len(mono), np.float32)
+        locked = lib().orc_stereo_decode(C.byref(self.st), _p(mono), len(mono), _p(out))
+        return out, bool(locked)
+
+    @property
+    def Enabled(self):
+        return bool(self.st.enabled)
+
+
+def StereoPilotPresent(iq, sample_rate):
+    iq = c64(iq)
+    r = C.c_double()
+    ok = lib().orc_stereo_pilot_present(_p(iq), len(iq), sample_rate, C.byref(r))
+    return bool(ok), r.value
+
+
+def crcSyndrome(raw):
+    return int(lib().orc_rds_syndrome(raw))
+
+
+def RDSBlock(data, offset):
+    return int(lib().orc_rds_block(data, offset))
+
+
+def RDSEncodeGroups(pi, ps, rt, n_groups):
+    bits = np.zeros(104 * n_groups, np.uint8)
+    ps8 = ps.encode().ljust(8)[:8]
+    rt64 = rt.encode().ljust(64)[:64]
+    n = lib().orc_rds_encode_groups(pi, ps8, rt64, n_groups, _p(bits))
+    return bits[:n]
+
+
+class RDSDecoder:
+    """rds.go:10-264 Decoder."""
+
+    def __init__(self):
+        self.d = orc_rds_decoder()
+        lib().orc_rds_init(C.byref(self.d))
+
+    def Decode(self, samples, sample_rate):
+        s = c64(samples)
+        return lib().orc_rds_decode(C.byref(self.d), _p(s), len(s), sample_rate)
+
+    @property
+    def PI(self):
+        return int(self.d.pi)
+
+    @property
+    def PS(self):
+        return bytes(self.d.ps).decode("latin-1").rstrip(" ")
+
+    @property
+    def RT(self):
+        return bytes(self.d.rt).decode("latin-1").rstrip(" ")
+
+    def Groups(self):
+        n = min(self.d.n_groups, RDS_MAX_GROUPS)
+        return [tuple(self.d.groups[i]) for i in range(n)]
+
+
+class orc_rds_stream(C.Structure):
+    _fields_ = [("taps", C.c_double * 103), ("hr", C.c_double * 103), ("hi", C.c_double * 103),
+                ("phase", C.c_double), ("inc", C.c_double), ("count", C.c_longlong), ("nt", C.c_int),
+                ("decim", C.c_int), ("rate", C.c_int), ("pad", C.c_int)]
+
+
+register("orc_rds_stream_init", None, [C.POINTER(orc_rds_stream), I])
+register("orc_rds_stream_process", I, [C.POINTER(orc_rds_stream), P, I, P])
+
+
+class RDSBasebandStream:
+    """fm.go:234-271 carried across blocks (mixer phase, FIR history, decimation phase)."""
+
+    def __init__(self, sample_rate):
+        self.s = orc_rds_stream()
+        lib().orc_rds_stream_init(C.byref(self.s), sample_rate)
+        self.rate = sample_rate // self.s.decim
+
+    def Process(self, disc):
+        d = np.ascontiguousarray(disc, np.float32)
+        out = np.zeros(len(d) // max(self.s.decim, 1) + 2, np.complex64)
+        m = lib().orc_rds_stream_process(C.byref(self.s), _p(d), len(d), _p(out))
+        return out[:m]
