@@ -3,18 +3,20 @@
 
 Workload (the serving shape at 20 Msps): complex64 IQ at 20 Msps in 20 ms
 blocks (400 000 samples); CH channels drawn from the uneven mode mix
-{NFM 12.5 kHz, AM 10 kHz, USB 3 kHz, WFM 200 kHz} with decimations 8..1600
-and odd FIR lengths 63..511 (seed 0).  One step = one block through the fused
-batched extractor (NCO shift + polyphase FIR/decimate for every channel, one
-launch) with the block already resident in HBM.
+{NFM 12.5 kHz, AM 10 kHz, USB 3 kHz, WFM 200 kHz} with decimations 8..1600,
+odd FIR lengths 63..511 and the matching demodulator (seed 0).  One step =
+one block through the streaming service: one fused extraction launch (NCO
+shift + polyphase FIR/decimate for every channel) and one demodulation
+launch, with the block already resident in HBM and blocks pipelined.
 
-    value = CH * block_duration / ms_per_step   (real-time channels)
+    value = CH * block_duration / ms_per_step      (real-time channels)
 
-Reported beside it, in the same run: per-block latency p50/p99 (event-timed
-device work, and host-to-host with the block's upload and the outputs'
-download), the plain one-kernel-per-signal GPU form (the reference's call
-pattern: per-signal upload, launch, synchronous copies), the roofline of the
-fused kernel, and the CPU oracle timed on a bounded sample on the host cores.
+Reported beside it in the same run: per-block latency p50/p99 (GPU event
+time, and host-to-host with the block's upload and the outputs' download),
+the plain one-kernel-per-signal GPU form (the reference's call pattern), the
+roofline of the fused extraction kernel, the surveillance pass (Welch +
+CA-CFAR + detection list at 10 Msps, N = 32768) and the 2.048 Msps 4-station
+WFM case, and the CPU oracle timed on a bounded sample on the host cores.
 """
 from __future__ import annotations
 
@@ -35,156 +37,312 @@ sys.path.insert(0, str(ROOT / "argus-sdr_amd"))
 FS = 20_000_000
 BLOCK = 400_000            # 20 ms at 20 Msps
 BLOCK_S = BLOCK / FS
-MODES = [("NFM", 12_500.0, [400, 800]), ("AM", 10_000.0, [500, 1000]), ("USB", 3_000.0, [1250, 1600]),
-         ("WFM", 200_000.0, [8, 16, 40, 80])]
+# mode name, bandwidth, decimations, demod id (argus.serve DEMOD_*)
+MODES = [("NFM", 12_500.0, [400, 800], 1), ("AM", 10_000.0, [500, 1000], 3), ("USB", 3_000.0, [1250, 1600], 4),
+         ("WFM", 200_000.0, [8, 16, 40, 80], 2)]
 HBM_PEAK_GBS = 8000.0
 FP32_PEAK_TFLOPS = 157.3
 METRIC = "real-time channels sustained @20 Msps span (1 GPU); p99 block latency; HBM GB/s"
 
 
 def make_channels(n: int, seed: int = 0):
-    from argus.gpudemod import StreamingExtractJob
+    """[(id, offset, bw, out_rate, taps, demod)] of the uneven mode mix."""
     rng = np.random.default_rng(seed)
-    jobs = []
+    out = []
     for i in range(n):
-        name, bw, ds = MODES[int(rng.integers(0, 4))]
+        name, bw, ds, dm = MODES[int(rng.integers(0, 4))]
         d = int(rng.choice(ds))
         t = 2 * int(rng.integers(31, 256)) + 1
         off = float(rng.uniform(-9.5e6, 9.5e6))
-        jobs.append(StreamingExtractJob(SignalID=i + 1, OffsetHz=off, Bandwidth=bw, OutRate=FS // d, NumTaps=t,
-                                        ConfigHash=0x9E3779B97F4A7C15 ^ (i + 1)))
-    return jobs
+        out.append((i + 1, off, bw, FS // d, t, dm))
+    return out
 
 
 def make_block(seed: int) -> np.ndarray:
     rng = np.random.default_rng(seed)
-    x = (rng.standard_normal(2 * BLOCK, dtype=np.float32) * np.float32(0.5)).view(np.complex64)
-    return x
+    return (rng.standard_normal(2 * BLOCK, dtype=np.float32) * np.float32(0.5)).view(np.complex64)
 
 
-def alg_figures(jobs, n_in):
-    """Algorithmic bytes and flops of one fused launch (unique bytes only)."""
+def alg_figures(chans, n_in):
+    """Algorithmic bytes and flops of one fused extraction launch (unique bytes)."""
     b = 8 * n_in
     f = 0
-    for j in jobs:
-        d = FS // j.OutRate
-        n_out = n_in // d
-        b += n_out * 8 + 16 * (j.NumTaps - 1) + 4 * j.NumTaps
-        f += 6 * n_in + 4 * j.NumTaps * n_out
+    for (_, _, _, rate, taps, _) in chans:
+        n_out = n_in // (FS // rate)
+        b += n_out * 8 + 16 * (taps - 1) + 4 * taps
+        f += 6 * n_in + 4 * taps * n_out
     return b, f
 
 
-def cpu_baseline(jobs, block, threads):
+def load_traffic():
+    """Measured HBM bytes per fused launch (rocprofv3 FETCH_SIZE + WRITE_SIZE,
+    scaled by a copy of known size), if the profile summary is present."""
+    p = ROOT / "profiles" / "extract_traffic.json"
+    if p.exists():
+        try:
+            return json.loads(p.read_text()).get("bytes_per_launch")
+        except (ValueError, OSError):
+            return None
+    return None
+
+
+def percentile(v, q):
+    return float(np.percentile(np.asarray(v, np.float64), q)) if len(v) else None
+
+
+def bench_serve(a, chans, blocks, lib, S):
+    """The fused, pipelined streaming service (the value figure)."""
+    from argus._lib import check
+    s = S.Serve(FS, BLOCK, 64 << 30)
+    for (cid, off, bw, rate, taps, dm) in chans:
+        assert s.Open(S.ChannelSpec(cid, off, bw, rate, taps, dm, 0))
+    d_blocks = []
+    for blk in blocks:
+        p = C.c_void_p()
+        check("gpud_malloc", lib.gpud_malloc(C.byref(p), blk.nbytes))
+        check("gpud_memcpy_h2d", lib.gpud_memcpy_h2d(p, blk.ctypes.data, blk.nbytes))
+        d_blocks.append(p.value)
+    s.SetDownload(False)
+    pending = []
+    for i in range(a.warmup):
+        pending.append(s.SubmitDevice(d_blocks[i % len(d_blocks)], BLOCK))
+        if len(pending) == 2:
+            s.Wait(pending.pop(0))
+    for t in pending:
+        s.Wait(t)
+    check("gpud_device_sync", lib.gpud_device_sync())
+    kern, gpu = [], []
+    pending = []
+    t0 = time.perf_counter()
+    for i in range(a.steps):
+        pending.append(s.SubmitDevice(d_blocks[i % len(d_blocks)], BLOCK))
+        if len(pending) == 2:  # two blocks in flight
+            info = s.Wait(pending.pop(0))
+            kern.append(info.kernel_ms)
+            gpu.append(info.gpu_ms)
+    for t in pending:
+        info = s.Wait(t)
+        kern.append(info.kernel_ms)
+        gpu.append(info.gpu_ms)
+    check("gpud_device_sync", lib.gpud_device_sync())
+    wall = time.perf_counter() - t0
+    # host-to-host latency: upload from host memory, kernels, download
+    s.SetDownload(True)
+    lat = []
+    for i in range(min(a.steps, 100)):
+        t1 = time.perf_counter()
+        tk = s.Submit(blocks[i % len(blocks)])
+        s.Wait(tk)
+        lat.append((time.perf_counter() - t1) * 1e3)
+    s.Close()
+    for p in d_blocks:
+        lib.gpud_free(p)
+    return wall * 1e3 / a.steps, kern, gpu, lat
+
+
+def bench_extract_kernel(a, chans, blocks, lib, G):
+    """Event-timed fused extraction kernel (runner API, device input)."""
+    from argus._lib import check
+    jobs = [G.StreamingExtractJob(c[0], c[1], c[2], c[3], c[4], 0x9E3779B97F4A7C15 ^ c[0]) for c in chans]
+    r = G.NewBatchRunner(BLOCK, FS)
+    p = C.c_void_p()
+    check("gpud_malloc", lib.gpud_malloc(C.byref(p), blocks[0].nbytes))
+    check("gpud_memcpy_h2d", lib.gpud_memcpy_h2d(p, blocks[0].ctypes.data, blocks[0].nbytes))
+    for _ in range(5):
+        r.StreamingExtractDevice(p.value, BLOCK, jobs)
+    check("sync", lib.gpud_stream_sync(r.stream()))
+    check("timing", lib.gpud_runner_enable_kernel_timing(r._h, 1))
+    reps = max(20, a.steps // 4)
+    for _ in range(reps):
+        r.StreamingExtractDevice(p.value, BLOCK, jobs)
+    tot, nl = C.c_double(), C.c_int()
+    check("kernel time", lib.gpud_runner_kernel_time(r._h, C.byref(tot), C.byref(nl)))
+    r.Close()
+    lib.gpud_free(p.value)
+    return tot.value / max(nl.value, 1), jobs
+
+
+def bench_plain(a, jobs, blocks, G):
+    rp = G.NewBatchRunner(BLOCK, FS, exec_mode=G.EXEC_PER_SIGNAL)
+    rp.StreamingExtractGPU(blocks[0], jobs)
+    t1 = time.perf_counter()
+    for i in range(a.plain_blocks):
+        rp.StreamingExtractGPU(blocks[(i + 1) % len(blocks)], jobs)
+    ms = (time.perf_counter() - t1) * 1e3 / a.plain_blocks
+    rp.Close()
+    return ms
+
+
+def bench_surveillance(a, lib):
+    """10 Msps, N = 32768 Hann, 50 % overlap, 8-segment Welch frames,
+    CA-CFAR t=16 g=4 at Pfa 1e-6 (12.3748 dB), detection list."""
+    from argus import surveillance as SV
+    from argus.synth import surveillance_scene
+    from argus._lib import check
+    fs, n, hop, segs = 10_000_000, 32768, 16384, 8
+    frames = a.surv_frames
+    total = (frames - 1) * segs * hop + (segs - 1) * hop + n
+    iq, _ = surveillance_scene(min(total, 4 * segs * hop + n), fs, n_carriers=40, seed=0)
+    reps = int(np.ceil(total / len(iq)))
+    iq = np.tile(iq, reps)[:total]
+    sv = SV.Surveillance(n, fs)
+    din = SV.DeviceBuffer(iq.nbytes).upload(iq)
+    dpsd = SV.DeviceBuffer(4 * n * frames)
+    max_runs = 2048
+    druns = SV.DeviceBuffer(C.sizeof(SV.gpud_surv_run) * max_runs * frames)
+    dcnt = SV.DeviceBuffer(4 * frames)
+    cfar = SV.cfar_params("CA", 4, 16, 12.3748, True, 3.0)
+    st = lib.gpud_surv_stream(sv._h)
+    e = [C.c_void_p() for _ in range(3)]
+    for x in e:
+        check("ev", lib.gpud_event_create(C.byref(x)))
+    for _ in range(2):
+        sv.WelchDevice(din.ptr, segs * hop, hop, segs, frames, dpsd.ptr)
+        check("cfar", lib.gpud_surv_cfar_runs_device(sv._h, dpsd.ptr, frames, C.byref(cfar), druns.ptr, dcnt.ptr,
+                                                     max_runs))
+    sv.sync()
+    reps = 5
+    welch_ms = cfar_ms = 0.0
+    for _ in range(reps):
+        lib.gpud_event_record(e[0], st)
+        sv.WelchDevice(din.ptr, segs * hop, hop, segs, frames, dpsd.ptr)
+        lib.gpud_event_record(e[1], st)
+        lib.gpud_surv_cfar_runs_device(sv._h, dpsd.ptr, frames, C.byref(cfar), druns.ptr, dcnt.ptr, max_runs)
+        lib.gpud_event_record(e[2], st)
+        m1, m2 = C.c_float(), C.c_float()
+        check("el", lib.gpud_event_elapsed_ms(e[0], e[1], C.byref(m1)))
+        check("el", lib.gpud_event_elapsed_ms(e[1], e[2], C.byref(m2)))
+        welch_ms += m1.value / reps
+        cfar_ms += m2.value / reps
+    # whole detection list (kernels + download + host expansion) once, timed
+    det = SV.detector_params(cfar_scale_db=12.3748)
+    t0 = time.perf_counter()
+    lists, _ = sv.Detect(iq, segs, hop, segs * hop, frames, cfar, det, want_psd=False)
+    detect_ms = (time.perf_counter() - t0) * 1e3
+    for x in e:
+        lib.gpud_event_destroy(x)
+    sv.Close()
+    b_alg = 8 * total + 4 * n * frames
+    flops = 5 * n * np.log2(n) * segs * frames
+    return {"frames": frames, "fft": n, "segments_per_frame": segs, "welch_ms": round(welch_ms, 4),
+            "cfar_runs_ms": round(cfar_ms, 4), "detect_list_ms_host": round(detect_ms, 2),
+            "frames_per_s": round(frames / ((welch_ms + cfar_ms) * 1e-3), 1),
+            "msps_sustained": round(total / ((welch_ms + cfar_ms) * 1e-3) / 1e6, 1),
+            "welch_hbm_gbps": round(b_alg / (welch_ms * 1e-3) / 1e9, 1),
+            "welch_tflops": round(flops / (welch_ms * 1e-3) / 1e12, 2),
+            "signals_first_frame": len(lists[0])}, iq[:(segs - 1) * hop + n]
+
+
+def bench_wfm_2048k(a, lib, S):
+    """2.048 Msps, 1 s, 4 WFM stations, 127-tap Hamming fc 100 kHz, /8 -> 256 ksps,
+    FM discriminator, 16384-sample chunks."""
+    from argus.synth import WFMStation
+    from argus._lib import check
+    fs, chunk = 2_048_000, 16384
+    rng = np.random.default_rng(0)
+    offs = []
+    while len(offs) < 4:
+        f = float(rng.uniform(-900e3, 900e3))
+        if all(abs(f - g) >= 200e3 for g in offs):
+            offs.append(f)
+    n = fs
+    x = np.zeros(n, np.complex64)
+    for k, f in enumerate(offs):
+        st = WFMStation(fs, f, np.ones(4000), seed=k)
+        for i in range(0, n, 1 << 17):
+            x[i:i + (1 << 17)] += st.block(min(1 << 17, n - i))
+    sig_p = np.mean(np.abs(x) ** 2) / 4
+    sigma = np.sqrt(sig_p * 200e3 / fs / 10 ** 3.0 / 2)
+    x += (sigma * (rng.standard_normal(n) + 1j * rng.standard_normal(n))).astype(np.complex64)
+    s = S.Serve(fs, chunk, 1 << 30)
+    for k, f in enumerate(offs):
+        assert s.Open(S.ChannelSpec(k + 1, f, 200e3, 256_000, 127, S.DEMOD_WFM, 1))
+    s.SetDownload(True)
+    t0 = time.perf_counter()
+    tickets = []
+    for i in range(0, n, chunk):
+        tickets.append(s.Submit(x[i:i + chunk]))
+        if len(tickets) == 2:
+            s.Wait(tickets.pop(0))
+    for t in tickets:
+        s.Wait(t)
+    wall = time.perf_counter() - t0
+    s.Close()
+    return {"stations": 4, "seconds_of_signal": 1.0, "wall_s": round(wall, 4),
+            "realtime_factor": round(1.0 / wall, 1)}, x[:chunk * 8], offs
+
+
+def cpu_baseline_extract(chans, block, threads):
     import oracle as O
+    jobs = [O.orc_job(c[0], c[1], c[2], c[3], c[4], 0x9E3779B97F4A7C15 ^ c[0]) for c in chans]
+    arr = (O.orc_job * len(jobs))(*jobs)
+    res = (O.orc_result * len(jobs))()
+    out = np.zeros(len(block) * 2, np.complex64)
     r = O.CPUOracleRunner(FS, threads=threads)
     t0 = time.perf_counter()
-    r.StreamingExtract(block, jobs)
+    rc = r.StreamingExtractRaw(block, arr, len(jobs), out, res)
     dt = time.perf_counter() - t0
+    assert rc == 0
     return dt
 
 
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--steps", type=int, default=300)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--channels", type=int, default=256)
     ap.add_argument("--plain-blocks", type=int, default=2)
     ap.add_argument("--cpu-channels", type=int, default=32)
+    ap.add_argument("--surv-frames", type=int, default=500)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-extras", action="store_true", help="skip surveillance and the 2.048 Msps case")
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: the extraction path shards by channel and this build runs on a "
               "single MI355X; run with --gpus 1.", file=sys.stderr)
         return 2
 
-    import argus
     from argus import gpudemod as G
-    from argus._lib import lib, check
+    from argus import serve as S
+    from argus._lib import lib
     L = lib()
     if not G.Available():
         print("no HIP device", file=sys.stderr)
         return 1
 
-    jobs = make_channels(a.channels)
+    chans = make_channels(a.channels)
     blocks = [make_block(s) for s in range(4)]
-    print(f"[bench] {a.channels} channels, block {BLOCK} samples @ {FS/1e6:.0f} Msps", file=sys.stderr, flush=True)
-
-    # ---- fused batched form, block resident in HBM --------------------------
-    r = G.NewBatchRunner(BLOCK, FS)
-    d_blocks = []
-    for blk in blocks:
-        p = C.c_void_p()
-        check("gpud_malloc", L.gpud_malloc(C.byref(p), blk.nbytes))
-        check("gpud_memcpy_h2d", L.gpud_memcpy_h2d(p, blk.ctypes.data, blk.nbytes))
-        d_blocks.append(p.value)
-    stream = r.stream()
-    for i in range(a.warmup):
-        r.StreamingExtractDevice(d_blocks[i % 4], BLOCK, jobs)
-    check("gpud_stream_sync", L.gpud_stream_sync(stream))
-    evs = []
-    for _ in range(2 * a.steps):
-        e = C.c_void_p()
-        check("gpud_event_create", L.gpud_event_create(C.byref(e)))
-        evs.append(e.value)
-    check("enable timing", L.gpud_runner_enable_kernel_timing(r._h, 1))
-    check("gpud_device_sync", L.gpud_device_sync())
-    t0 = time.perf_counter()
-    for i in range(a.steps):
-        L.gpud_event_record(evs[2 * i], stream)
-        r.StreamingExtractDevice(d_blocks[i % 4], BLOCK, jobs)
-        L.gpud_event_record(evs[2 * i + 1], stream)
-    check("gpud_stream_sync", L.gpud_stream_sync(stream))
-    check("gpud_device_sync", L.gpud_device_sync())
-    wall = time.perf_counter() - t0
-    tot = C.c_double()
-    nl = C.c_int()
-    check("kernel time", L.gpud_runner_kernel_time(r._h, C.byref(tot), C.byref(nl)))
-    L.gpud_runner_enable_kernel_timing(r._h, 0)
-    step_ms = []
-    for i in range(a.steps):
-        ms = C.c_float()
-        check("elapsed", L.gpud_event_elapsed_ms(evs[2 * i], evs[2 * i + 1], C.byref(ms)))
-        step_ms.append(ms.value)
-    ms_per_step = wall * 1e3 / a.steps
-    kern_ms = tot.value / max(nl.value, 1)
+    print(f"[bench] serve: {a.channels} channels, block {BLOCK} samples @ {FS / 1e6:.0f} Msps", file=sys.stderr,
+          flush=True)
+    ms_per_step, kern, gpu, lat = bench_serve(a, chans, blocks, L, S)
     value = a.channels * BLOCK_S * 1e3 / ms_per_step
-
-    # ---- host-to-host per-block latency (upload + extract + download) ------
-    lat = []
-    for i in range(min(50, a.steps)):
-        t1 = time.perf_counter()
-        r.StreamingExtractGPU(blocks[i % 4], jobs)
-        lat.append((time.perf_counter() - t1) * 1e3)
-    r.Close()
-
-    # ---- plain per-signal GPU form ----------------------------------------
-    rp = G.NewBatchRunner(BLOCK, FS, exec_mode=G.EXEC_PER_SIGNAL)
-    rp.StreamingExtractGPU(blocks[0], jobs)
-    t1 = time.perf_counter()
-    for i in range(a.plain_blocks):
-        rp.StreamingExtractGPU(blocks[(i + 1) % 4], jobs)
-    plain_ms = (time.perf_counter() - t1) * 1e3 / a.plain_blocks
-    rp.Close()
+    print("[bench] fused extraction kernel", file=sys.stderr, flush=True)
+    kern_ms, jobs = bench_extract_kernel(a, chans, blocks, L, G)
+    print("[bench] plain per-signal form", file=sys.stderr, flush=True)
+    plain_ms = bench_plain(a, jobs, blocks, G)
     plain_channels = a.channels * BLOCK_S * 1e3 / plain_ms
 
-    for p in d_blocks:
-        L.gpud_free(p)
-    for e in evs:
-        L.gpud_event_destroy(e)
-
-    b_alg, f_alg = alg_figures(jobs, BLOCK)
+    b_alg, f_alg = alg_figures(chans, BLOCK)
     achieved = b_alg / (kern_ms * 1e-3) / 1e9
     valu = f_alg / (kern_ms * 1e-3) / 1e12
 
+    extras = {}
+    if not a.no_extras:
+        print("[bench] surveillance", file=sys.stderr, flush=True)
+        extras["surveillance_10msps_n32768"], surv_sample = bench_surveillance(a, L)
+        print("[bench] 2.048 Msps WFM", file=sys.stderr, flush=True)
+        extras["wfm_2048ksps_4stations"], _, _ = bench_wfm_2048k(a, L, S)
+
     cpu = None
     if not a.no_cpu:
-        sub = jobs[:a.cpu_channels]
-        dt = cpu_baseline(sub, blocks[0], 1)
+        print("[bench] cpu baseline", file=sys.stderr, flush=True)
+        sub = chans[:a.cpu_channels]
+        dt = cpu_baseline_extract(sub, blocks[0], 1)
         cpu = {"value": round(len(sub) * BLOCK_S / dt, 3), "unit": "real-time channels", "cores": 1,
                "kind": "port",
-               "sample": f"oracle streaming extractor, {len(sub)} channels of the same mix x one 20 ms block "
-                         f"({dt:.2f} s of CPU work, single thread)"}
+               "sample": f"oracle streaming extractor (no demod), {len(sub)} channels of the same mix x one "
+                         f"20 ms block ({dt:.2f} s of CPU work, one thread); host has {os.cpu_count()} cores"}
 
     out = {
         "metric": METRIC,
@@ -199,22 +357,26 @@ def main() -> int:
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
-        "config": {"workload": "uneven-mode extraction at 20 Msps, 20 ms blocks", "sample_rate": FS,
-                   "block_samples": BLOCK, "channels": a.channels, "modes": "NFM/AM/USB/WFM",
-                   "decim": "8-1600", "taps": "63-511", "seed": 0},
-        "p50_block_ms": round(float(np.percentile(step_ms, 50)), 4),
-        "p99_block_ms": round(float(np.percentile(step_ms, 99)), 4),
-        "p50_block_ms_host": round(float(np.percentile(lat, 50)), 3),
-        "p99_block_ms_host": round(float(np.percentile(lat, 99)), 3),
+        "config": {"workload": "serving loop: uneven-mode extraction + demod at 20 Msps, 20 ms blocks",
+                   "sample_rate": FS, "block_samples": BLOCK, "channels": a.channels,
+                   "modes": "NFM/AM/USB/WFM (+ demod)", "decim": "8-1600", "taps": "63-511", "seed": 0},
+        "p50_block_ms": round(percentile(gpu, 50), 4),
+        "p99_block_ms": round(percentile(gpu, 99), 4),
+        "p50_kernels_ms": round(percentile(kern, 50), 4),
+        "p99_kernels_ms": round(percentile(kern, 99), 4),
+        "p50_block_ms_host_to_host": round(percentile(lat, 50), 3),
+        "p99_block_ms_host_to_host": round(percentile(lat, 99), 3),
         "hbm_gbps": round(achieved, 1),
         "plain_per_signal": {"ms_per_block": round(plain_ms, 2), "real_time_channels": round(plain_channels, 1),
                              "fused_over_plain": round(value / plain_channels, 1)},
         "roofline": {"kernel": "argus_extract_batch_kernel", "bound": "hbm", "achieved": round(achieved, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": None, "kernel_ms": round(kern_ms, 4), "alg_bytes": b_alg, "alg_flops": f_alg,
-                     "valu_tflops": round(valu, 2), "valu_frac": round(valu / FP32_PEAK_TFLOPS, 4)},
+                     "traffic": load_traffic(), "kernel_ms": round(kern_ms, 4), "alg_bytes": b_alg,
+                     "alg_flops": f_alg, "valu_tflops": round(valu, 2),
+                     "valu_frac": round(valu / FP32_PEAK_TFLOPS, 4)},
         "cpu_baseline": cpu,
     }
+    out.update(extras)
     print(json.dumps(out), flush=True)
     return 0
 
