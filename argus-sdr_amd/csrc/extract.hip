@@ -153,7 +153,7 @@ __device__ __forceinline__ void argus_run_item(const ArgusChanDesc& ch, int kind
         argus_dense_tile(ch, in, m0, lds);
     else if (kind == ARGUS_ITEM_NARROW)
         argus_narrow_group(ch, in, m0, (float*)lds);
-    else
+    else if (kind == ARGUS_ITEM_HISTORY)
         argus_history_tail(ch, in, n_new);
 }
 
@@ -212,6 +212,54 @@ int argus_launch_extract_batch(const ArgusChanDesc* d_chans, const ArgusWorkItem
     hipLaunchKernelGGL(argus_extract_batch_kernel, dim3(grid), dim3(ARGUS_EXTRACT_THREADS), lds_bytes, stream,
                        d_chans, d_items, n_items, in, n_new);
     return (int)hipGetLastError();
+}
+
+// Append the work items of one described channel; pos receives each item's
+// first input sample (history reads count as sample 0).
+void argus_append_items(const ArgusChanDesc& ch, int chan_index, int n_new, std::vector<ArgusWorkItem>& items,
+                        std::vector<int>& pos)
+{
+    const int tiles = ch.n_out > 0 ? (ch.n_out + ch.M - 1) / ch.M : 0;
+    // cost in lane-operations: NCO rotations (~24 each) plus taps (~2 each)
+    const long rot = ch.narrow ? (long)ch.M * ch.T : (long)(ch.M - 1) * ch.D + ch.T;
+    const long cost = rot * 24 + (long)ch.M * ch.T * 2;
+    const int kind = ch.narrow ? ARGUS_ITEM_NARROW : ARGUS_ITEM_DENSE;
+    for (int t = 0; t < tiles; t++) {
+        items.push_back({chan_index, t * ch.M, kind, (int)std::min<long>(cost, 1L << 30)});
+        pos.push_back(std::max(0, ch.e0 + t * ch.M * ch.D - (ch.T - 1)));
+    }
+    if (ch.keep > 0) {
+        items.push_back({chan_index, 0, ARGUS_ITEM_HISTORY, ch.keep * 24});
+        pos.push_back(std::max(0, n_new - ch.keep));
+    }
+}
+
+// Order the items of one launch for the persistent grid.  Workgroups are
+// dealt to the XCDs round-robin, and the grid is a multiple of ARGUS_XCDS, so
+// item w always runs on XCD w % ARGUS_XCDS.  Items are bucketed by where
+// their input lies in the block (one eighth per XCD), heaviest first within a
+// bucket, and interleaved: each XCD's L2 then holds one eighth of the block
+// instead of all of it.  Short buckets are padded with no-op items.
+void argus_order_items(std::vector<ArgusWorkItem>& items, const std::vector<int>& pos, int n_new)
+{
+    std::vector<std::vector<ArgusWorkItem>> bucket(ARGUS_XCDS);
+    const long span = std::max(1, n_new);
+    for (size_t i = 0; i < items.size(); i++) {
+        int b = (int)((long)pos[i] * ARGUS_XCDS / span);
+        b = std::min(std::max(b, 0), ARGUS_XCDS - 1);
+        bucket[(size_t)b].push_back(items[i]);
+    }
+    size_t longest = 0;
+    for (auto& v : bucket) {
+        std::stable_sort(v.begin(), v.end(),
+                         [](const ArgusWorkItem& a, const ArgusWorkItem& b) { return a.pad > b.pad; });
+        longest = std::max(longest, v.size());
+    }
+    items.clear();
+    for (size_t j = 0; j < longest; j++)
+        for (int x = 0; x < ARGUS_XCDS; x++)
+            items.push_back(j < bucket[(size_t)x].size() ? bucket[(size_t)x][j]
+                                                        : ArgusWorkItem{0, 0, ARGUS_ITEM_NOP, 0});
 }
 
 // Describe one signal for the kernels.  Returns 0 or -1 if the geometry

@@ -49,7 +49,10 @@ struct ArgusWorkItem {
     int pad;   // host: cost estimate for ordering
 };
 
-enum { ARGUS_ITEM_DENSE = 0, ARGUS_ITEM_HISTORY = 1, ARGUS_ITEM_NARROW = 2 };
+enum { ARGUS_ITEM_DENSE = 0, ARGUS_ITEM_HISTORY = 1, ARGUS_ITEM_NARROW = 2, ARGUS_ITEM_NOP = 3 };
+
+// Number of XCDs workgroups are dealt to round-robin (MI355X: 8).
+#define ARGUS_XCDS 8
 
 #define ARGUS_EXTRACT_THREADS 256
 #define ARGUS_LDS_ENTRIES 6144     // complex samples of LDS per dense tile (48 KiB)

@@ -374,6 +374,7 @@ static int run_batched(gpud_runner* r, const float2* d_in, int n_new, const gpud
     ARGUS_CHECK((hipError_t)ensure_dev(&r->d_chans[slot], &r->d_chans_cap[slot], (size_t)std::max(n_jobs, 1)));
 
     r->items.clear();
+    r->item_pos.clear();
     size_t lds = 0;
     for (int i = 0; i < n_jobs; i++) {
         ArgusSignal* s = sig[(size_t)i];
@@ -385,19 +386,10 @@ static int run_batched(gpud_runner* r, const float2* d_in, int n_new, const gpud
                             r->d_out + results[i].out_offset, D, T, s->phase_count, n_new, s->nco_phase,
                             inc) != 0)
             return -1;
-        const int tiles = ch.n_out > 0 ? (ch.n_out + ch.M - 1) / ch.M : 0;
-        if (tiles > 0) lds = std::max(lds, argus_item_lds_bytes(ch));
-        // cost in lane-operations: NCO rotations (~24 each) plus taps (~2 each)
-        const long rot = ch.narrow ? (long)ch.M * T : (long)(ch.M - 1) * D + T;
-        const long cost = rot * 24 + (long)ch.M * T * 2;
-        const int kind = ch.narrow ? ARGUS_ITEM_NARROW : ARGUS_ITEM_DENSE;
-        for (int t = 0; t < tiles; t++)
-            r->items.push_back({i, t * ch.M, kind, (int)std::min<long>(cost, 1L << 30)});
-        if (keep > 0) r->items.push_back({i, 0, ARGUS_ITEM_HISTORY, keep * 24});
+        if (ch.n_out > 0) lds = std::max(lds, argus_item_lds_bytes(ch));
+        argus_append_items(ch, i, n_new, r->items, r->item_pos);
     }
-    // heaviest first (LPT order) so the persistent grid drains evenly
-    std::stable_sort(r->items.begin(), r->items.end(),
-                     [](const ArgusWorkItem& a, const ArgusWorkItem& b) { return a.pad > b.pad; });
+    argus_order_items(r->items, r->item_pos, n_new);
     const int n_items = (int)r->items.size();
     ARGUS_CHECK((hipError_t)ensure_host(&r->h_items[slot], &r->h_items_cap[slot], (size_t)std::max(n_items, 1)));
     ARGUS_CHECK((hipError_t)ensure_dev(&r->d_items[slot], &r->d_items_cap[slot], (size_t)std::max(n_items, 1)));

@@ -53,6 +53,7 @@ struct gpud_runner {
     ArgusWorkItem* d_items[2] = {nullptr, nullptr};
     size_t h_items_cap[2] = {0, 0}, d_items_cap[2] = {0, 0};
     std::vector<ArgusWorkItem> items;
+    std::vector<int> item_pos;
     // kernel timing (event pair per fused launch)
     bool timing = false;
     std::vector<hipEvent_t> tev;  // 2 per launch
@@ -63,5 +64,8 @@ struct gpud_runner {
 };
 
 int argus_lowpass_fir(double cutoff_hz, int sample_rate, int taps, double* out, bool hamming);
+void argus_append_items(const ArgusChanDesc& ch, int chan_index, int n_new, std::vector<ArgusWorkItem>& items,
+                        std::vector<int>& pos);
+void argus_order_items(std::vector<ArgusWorkItem>& items, const std::vector<int>& pos, int n_new);
 ArgusSignal* argus_runner_find(gpud_runner* r, int64_t id);
 void argus_signal_release(ArgusSignal& s);

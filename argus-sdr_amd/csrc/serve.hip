@@ -25,6 +25,9 @@ int argus_launch_extract_batch(const ArgusChanDesc* d_chans, const ArgusWorkItem
 int argus_fill_chan(ArgusChanDesc* ch, const float2* hist_rd, int H, float2* hist_wr, int keep, const float* taps,
                     float2* out, int D, int T, int phase_count, int n_new, double phase_start, double phase_inc);
 size_t argus_item_lds_bytes(const ArgusChanDesc& ch);
+void argus_append_items(const ArgusChanDesc& ch, int chan_index, int n_new, std::vector<ArgusWorkItem>& items,
+                        std::vector<int>& pos);
+void argus_order_items(std::vector<ArgusWorkItem>& items, const std::vector<int>& pos, int n_new);
 int argus_lowpass_fir(double cutoff_hz, int sample_rate, int taps, double* out, bool hamming);
 
 namespace {
@@ -293,6 +296,7 @@ static int serve_submit(gpud_serve* s, const gpud_float2* iq, int n, bool on_dev
     ARGUS_CHECK((hipError_t)grow_dev(&sl.d_ddesc, &sl.d_ddesc_cap, (size_t)std::max(nch, 1)));
 
     s->items.clear();
+    s->item_pos.clear();
     s->ditems.clear();
     size_t lds = 0;
     int ci = 0;
@@ -306,13 +310,8 @@ static int serve_submit(gpud_serve* s, const gpud_float2* iq, int n, bool on_dev
         if (argus_fill_chan(&ch, c.d_hist[c.hist_sel], c.hist_len, c.d_hist[c.hist_sel ^ 1], keep, c.d_taps,
                             sl.d_iq + o.iq_off, c.decim, c.T, c.phase_count, n, c.nco_phase, inc) != 0)
             return -1;
-        const int tiles = ch.n_out > 0 ? (ch.n_out + ch.M - 1) / ch.M : 0;
-        if (tiles > 0) lds = std::max(lds, argus_item_lds_bytes(ch));
-        const long rot = ch.narrow ? (long)ch.M * c.T : (long)(ch.M - 1) * c.decim + c.T;
-        const long cost = rot * 24 + (long)ch.M * c.T * 2;
-        const int kind = ch.narrow ? ARGUS_ITEM_NARROW : ARGUS_ITEM_DENSE;
-        for (int t = 0; t < tiles; t++) s->items.push_back({ci, t * ch.M, kind, (int)std::min<long>(cost, 1L << 30)});
-        if (keep > 0) s->items.push_back({ci, 0, ARGUS_ITEM_HISTORY, keep * 24});
+        if (ch.n_out > 0) lds = std::max(lds, argus_item_lds_bytes(ch));
+        argus_append_items(ch, ci, n, s->items, s->item_pos);
 
         ArgusDemodDesc& dd = sl.h_ddesc[ci];
         dd.iq = sl.d_iq + o.iq_off;
@@ -348,8 +347,7 @@ static int serve_submit(gpud_serve* s, const gpud_float2* iq, int n, bool on_dev
         c.last_ticket = sl.ticket;
         ci++;
     }
-    std::stable_sort(s->items.begin(), s->items.end(),
-                     [](const ArgusWorkItem& a, const ArgusWorkItem& b) { return a.pad > b.pad; });
+    argus_order_items(s->items, s->item_pos, n);
     const int n_items = (int)s->items.size(), n_ditems = (int)s->ditems.size();
     ARGUS_CHECK((hipError_t)grow_host(&sl.h_items, &sl.h_items_cap, (size_t)std::max(n_items, 1)));
     ARGUS_CHECK((hipError_t)grow_dev(&sl.d_items, &sl.d_items_cap, (size_t)std::max(n_items, 1)));

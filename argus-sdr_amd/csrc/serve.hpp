@@ -72,5 +72,6 @@ struct gpud_serve {
     int64_t last_ticket = 0;
     int64_t completed_ticket = 0;
     std::vector<ArgusWorkItem> items;
+    std::vector<int> item_pos;
     std::vector<ArgusDemodItem> ditems;
 };

@@ -297,6 +297,8 @@ def main() -> int:
     ap.add_argument("--surv-frames", type=int, default=500)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extras", action="store_true", help="skip surveillance and the 2.048 Msps case")
+    ap.add_argument("--calibrate-copy-mb", type=int, default=0,
+                    help="run one device-to-device copy of this many MiB (counter calibration)")
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: the extraction path shards by channel and this build runs on a "
@@ -313,6 +315,18 @@ def main() -> int:
 
     chans = make_channels(a.channels)
     blocks = [make_block(s) for s in range(4)]
+    if a.calibrate_copy_mb > 0:
+        # a device-to-device copy of known size: its FETCH_SIZE / WRITE_SIZE
+        # under rocprofv3 scale the counters read for the extraction kernel
+        nb = a.calibrate_copy_mb << 20
+        src, dst = C.c_void_p(), C.c_void_p()
+        L.gpud_malloc(C.byref(src), nb)
+        L.gpud_malloc(C.byref(dst), nb)
+        L.gpud_memset_zero(src, nb)
+        L.gpud_memcpy_d2d(dst, src, nb)
+        L.gpud_device_sync()
+        L.gpud_free(src)
+        L.gpud_free(dst)
     print(f"[bench] serve: {a.channels} channels, block {BLOCK} samples @ {FS / 1e6:.0f} Msps", file=sys.stderr,
           flush=True)
     ms_per_step, kern, gpu, lat = bench_serve(a, chans, blocks, L, S)
