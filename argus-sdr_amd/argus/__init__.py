@@ -6,3 +6,4 @@ from ._lib import ErrUnavailable, GpudError, EXPORTS, LIB_PATH, lib  # noqa: F40
 from . import gpudemod  # noqa: F401
 from . import serve  # noqa: F401
 from . import surveillance  # noqa: F401
+from . import wfm  # noqa: F401

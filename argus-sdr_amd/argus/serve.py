@@ -14,7 +14,7 @@ import numpy as np
 
 from ._lib import check, lib, register
 
-DEMOD_NONE, DEMOD_NFM, DEMOD_WFM, DEMOD_AM, DEMOD_USB, DEMOD_LSB, DEMOD_CW = range(7)
+DEMOD_NONE, DEMOD_NFM, DEMOD_WFM, DEMOD_AM, DEMOD_USB, DEMOD_LSB, DEMOD_CW, DEMOD_WFM_STEREO = range(8)
 ERR_BUDGET = -20
 ERR_DUPLICATE = -21
 
@@ -42,6 +42,8 @@ register("gpud_serve_set_download", I, [P, I])
 register("gpud_serve_wait", I, [P, C.c_int64, C.POINTER(gpud_serve_block_info)])
 register("gpud_serve_channel_output", I, [P, C.c_int64, C.c_int64, C.POINTER(P), C.POINTER(I), C.POINTER(P),
                                           C.POINTER(I), C.POINTER(I)])
+register("gpud_serve_channel_wfm", I, [P, C.c_int64, C.c_int64, C.POINTER(P), C.POINTER(I), C.POINTER(I),
+                                       C.POINTER(I), C.POINTER(I)])
 
 
 @dataclass
@@ -131,3 +133,17 @@ class Serve:
         au = np.ctypeslib.as_array((C.c_float * n_a.value).from_address(aud_p.value)).copy() \
             if n_a.value else np.zeros(0, np.float32)
         return iq, au, rate.value
+
+    def OutputWFM(self, ticket: int, channel_id: int):
+        """(rds complex64 copy, rds_rate, locked, stereo) of a WFM stereo
+        channel in a completed block, or None if it was not in the block."""
+        p = C.c_void_p()
+        n, rate, locked, stereo = C.c_int(), C.c_int(), C.c_int(), C.c_int()
+        rc = lib().gpud_serve_channel_wfm(self._h, ticket, channel_id, C.byref(p), C.byref(n), C.byref(rate),
+                                          C.byref(locked), C.byref(stereo))
+        if rc == -1:
+            return None
+        check("gpud_serve_channel_wfm", rc)
+        rds = np.ctypeslib.as_array((C.c_float * (2 * n.value)).from_address(p.value)).view(np.complex64).copy() \
+            if n.value else np.zeros(0, np.complex64)
+        return rds, rate.value, bool(locked.value), bool(stereo.value)

@@ -72,20 +72,44 @@ int grow_host(T** p, size_t* cap, size_t need)
     return 0;
 }
 
-bool valid_mode(int m) { return m >= GPUD_DEMOD_NONE && m <= GPUD_DEMOD_CW; }
+bool valid_mode(int m) { return m >= GPUD_DEMOD_NONE && m <= GPUD_DEMOD_WFM_STEREO; }
+
+// Device block of a stereo channel: ArgusWfmState, then three ping-pong pairs
+// of delay lines (mono, L-R, fallback), then the tap sets.
+constexpr size_t kWfmState = 64;
+constexpr size_t kWfmHistPair = 2 * ARGUS_WFM_HIST * sizeof(float);
+constexpr size_t kWfmMonoHist = kWfmState;
+constexpr size_t kWfmLrHist = kWfmMonoHist + kWfmHistPair;
+constexpr size_t kWfmFbHist = kWfmLrHist + kWfmHistPair;
+constexpr size_t kWfmTaps = kWfmFbHist + kWfmHistPair;
+constexpr size_t kWfmBytes = kWfmTaps + ARGUS_WFM_NTAPSETS * ARGUS_WFM_TAPS * sizeof(float);
+static_assert(sizeof(ArgusWfmState) <= kWfmState, "state block");
+
+// RDS mixer phase as a fixed-point turn: num / den of a turn, num < den.
+uint64_t frac_to_fix(uint64_t num, uint64_t den)
+{
+    return (uint64_t)(((unsigned __int128)num << 64) / den);
+}
 
 }  // namespace
 
 // Device bytes a channel holds while open (taps, ping-pong history, FM carry)
 // plus its share of the per-slot output buffers.
-static size_t channel_footprint(const gpud_serve* s, int D, int T)
+static size_t channel_footprint(const gpud_serve* s, int D, int T, bool stereo)
 {
     const size_t L = (size_t)(T + D - 1) / D;
     const size_t taps = (size_t)D * L * sizeof(float);
     const size_t hist = 2 * (size_t)std::max(1, T - 1) * sizeof(float2);
     const size_t carry = 2 * sizeof(float2);
-    const size_t per_block = ((size_t)s->block_samples / D + 2) * (sizeof(float2) + sizeof(float));
-    return taps + hist + carry + per_block * ARGUS_SERVE_SLOTS;
+    const size_t n_out = (size_t)s->block_samples / D + 2;
+    size_t per_block = n_out * (sizeof(float2) + sizeof(float));
+    size_t fixed = taps + hist + carry;
+    if (stereo) {
+        // mono + 4 scratch floats + a second audio float + RDS samples (<= 1 per 2 outputs)
+        per_block += n_out * (6 * sizeof(float)) + (n_out / 2 + 1) * sizeof(float2) + sizeof(ArgusWfmBlock);
+        fixed += kWfmBytes;
+    }
+    return fixed + per_block * ARGUS_SERVE_SLOTS;
 }
 
 static void release_channel(ServeChannel& c)
@@ -94,9 +118,11 @@ static void release_channel(ServeChannel& c)
     if (c.d_hist[0]) hipFree(c.d_hist[0]);
     if (c.d_hist[1]) hipFree(c.d_hist[1]);
     if (c.d_carry) hipFree(c.d_carry);
+    if (c.d_wfm) hipFree(c.d_wfm);
     c.d_taps = nullptr;
     c.d_hist[0] = c.d_hist[1] = nullptr;
     c.d_carry = nullptr;
+    c.d_wfm = nullptr;
 }
 
 // Release channels whose last block has completed.
@@ -113,6 +139,32 @@ static void reap(gpud_serve* s)
             i++;
         }
     }
+}
+
+// Device block of a new stereo channel: PLL at 19 kHz (streamer.go:1532-1549),
+// zero delay lines, the tap sets at the channel's output rate.
+static hipError_t wfm_channel_init(ServeChannel& c, int rate)
+{
+    std::vector<char> blob(kWfmBytes, 0);
+    ArgusWfmState st{};
+    st.freq = 2.0 * M_PI * 19000.0 / (double)rate;
+    std::memcpy(blob.data(), &st, sizeof st);
+    const double cut[6] = {15000.0, 53000.0, 23000.0, 21000.0, 17000.0, 7500.0};
+    double h[6][ARGUS_WFM_TAPS + 1];
+    for (int i = 0; i < 6; i++) argus_lowpass_fir(cut[i], rate, ARGUS_WFM_TAPS, h[i], false);
+    float* taps = (float*)(blob.data() + kWfmTaps);
+    for (int k = 0; k < ARGUS_WFM_TAPS; k++) {
+        taps[ARGUS_WFM_LPF * ARGUS_WFM_TAPS + k] = (float)h[0][k];
+        taps[ARGUS_WFM_BPF * ARGUS_WFM_TAPS + k] = (float)(h[1][k] - h[2][k]);
+        taps[ARGUS_WFM_PILOT * ARGUS_WFM_TAPS + k] = (float)(h[3][k] - h[4][k]);
+        taps[ARGUS_WFM_RDS * ARGUS_WFM_TAPS + k] = (float)h[5][k];
+    }
+    c.rds_decim = std::max(1, rate / 19000);
+    c.rds_count = 0;
+    c.wfm_sel = 0;
+    hipError_t e = dev_alloc(&c.d_wfm, kWfmBytes);
+    if (e != hipSuccess) return e;
+    return hipMemcpy(c.d_wfm, blob.data(), kWfmBytes, hipMemcpyHostToDevice);
 }
 
 static int slot_wait(gpud_serve* s, ServeSlot& sl)
@@ -179,6 +231,16 @@ GPUD_API int gpud_serve_destroy(gpud_serve* s)
         hipHostFree(sl.h_items);
         hipHostFree(sl.h_ddesc);
         hipHostFree(sl.h_ditems);
+        hipFree(sl.d_mono);
+        hipFree(sl.d_wscratch);
+        hipFree(sl.d_rds);
+        hipFree(sl.d_wblk);
+        hipFree(sl.d_wdesc);
+        hipFree(sl.d_witems);
+        hipHostFree(sl.h_rds);
+        hipHostFree(sl.h_wblk);
+        hipHostFree(sl.h_wdesc);
+        hipHostFree(sl.h_witems);
     }
     hipStreamDestroy(s->compute);
     hipStreamDestroy(s->copy);
@@ -197,7 +259,8 @@ GPUD_API int gpud_serve_open(gpud_serve* s, const gpud_channel_spec* spec)
     const int D = s->sample_rate / spec->out_rate;
     const int T = spec->num_taps > 0 ? spec->num_taps : 101;
     if (T > ARGUS_MAX_TAPS) return -1;
-    const size_t fp = channel_footprint(s, D, T);
+    const bool stereo = spec->demod == GPUD_DEMOD_WFM_STEREO;
+    const size_t fp = channel_footprint(s, D, T, stereo);
     reap(s);
     if (s->bytes_in_use + fp > s->budget) return GPUD_ERR_BUDGET;
 
@@ -206,6 +269,7 @@ GPUD_API int gpud_serve_open(gpud_serve* s, const gpud_channel_spec* spec)
     c.decim = D;
     c.T = T;
     c.footprint = fp;
+    c.stereo = stereo;
     double cutoff = spec->bandwidth / 2;
     if (cutoff < 200) cutoff = 200;
     std::vector<double> b64((size_t)T + 1);
@@ -223,6 +287,7 @@ GPUD_API int gpud_serve_open(gpud_serve* s, const gpud_channel_spec* spec)
         return (int)hipErrorOutOfMemory;
     }
     hipError_t e = hipMemcpy(c.d_taps, poly.data(), poly.size() * sizeof(float), hipMemcpyHostToDevice);
+    if (e == hipSuccess && stereo) e = wfm_channel_init(c, spec->out_rate);
     if (e != hipSuccess) {
         release_channel(c);
         return (int)e;
@@ -256,6 +321,59 @@ GPUD_API int gpud_serve_info(gpud_serve* s, int* live_channels, uint64_t* bytes_
 
 }  // extern "C"
 
+// streamer.go:1496-1506 pllCoefficients: type-II loop gains.
+static void pll_coefficients(double loop_bw, double damping, int rate, double* alpha, double* beta)
+{
+    const double bl = loop_bw / (double)rate;
+    const double theta = bl / (damping + 0.25 / damping);
+    const double d = 1 + 2 * damping * theta + theta * theta;
+    *alpha = (4 * damping * theta) / d;
+    *beta = (4 * theta * theta) / d;
+}
+
+// Stereo-stage descriptor and work items of one channel for this block; the
+// channel's host-side carry (delay-line half, RDS sample count) advances.
+static void wfm_describe(ServeChannel& c, const ServeOut& o, ServeSlot& sl, std::vector<ArgusWfmItem>& items)
+{
+    const int rate = c.spec.out_rate;
+    const int D = c.rds_decim;
+    ArgusWfmDesc& wd = sl.h_wdesc[o.wfm_index];
+    wd.mono = sl.d_mono + o.mono_off;
+    wd.audio = sl.d_audio + o.audio_off;
+    wd.scratch = sl.d_wscratch + 4 * o.mono_off;
+    wd.rds = sl.d_rds + o.rds_off;
+    float* mh = (float*)(c.d_wfm + kWfmMonoHist);
+    float* lh = (float*)(c.d_wfm + kWfmLrHist);
+    wd.mono_hist_rd = mh + c.wfm_sel * ARGUS_WFM_HIST;
+    wd.mono_hist_wr = mh + (c.wfm_sel ^ 1) * ARGUS_WFM_HIST;
+    wd.lr_hist_rd = lh + c.wfm_sel * ARGUS_WFM_HIST;
+    wd.lr_hist_wr = lh + (c.wfm_sel ^ 1) * ARGUS_WFM_HIST;
+    wd.fb_hist = (float*)(c.d_wfm + kWfmFbHist);
+    wd.taps = (const float*)(c.d_wfm + kWfmTaps);
+    wd.st = (ArgusWfmState*)c.d_wfm;
+    wd.blk = sl.d_wblk + o.wfm_index;
+    // stream sample c0 + i mixes with 2 pi ((c0 + i + 1) 57000 mod rate) / rate
+    const uint64_t r = (uint64_t)rate;
+    const uint64_t num0 = ((uint64_t)((c.rds_count + 1) % rate) * 57000u) % r;
+    wd.mix0 = frac_to_fix(num0, r);
+    wd.mix_inc = frac_to_fix(57000u % r, r);
+    pll_coefficients(50.0, 0.707, rate, &wd.alpha, &wd.beta);
+    wd.lpa = 1.0 - std::exp(-2.0 * M_PI * 200.0 / (double)rate);
+    wd.min_f = 2.0 * M_PI * 17000.0 / (double)rate;
+    wd.max_f = 2.0 * M_PI * 21000.0 / (double)rate;
+    wd.rate = (double)rate;
+    wd.n = o.n_mono;
+    wd.rds_first = (int)((D - c.rds_count % D) % D);
+    wd.rds_decim = D;
+    wd.n_rds = o.n_rds;
+    for (int st = 0; st < o.n_mono; st += ARGUS_WFM_TILE) items.push_back({o.wfm_index, st, ARGUS_WFM_ITEM_TILE, 0});
+    if (o.n_mono > 0) {
+        items.push_back({o.wfm_index, 0, ARGUS_WFM_ITEM_TAIL, 0});
+        c.wfm_sel ^= 1;
+    }
+    c.rds_count += o.n_mono;
+}
+
 // Enqueue one block: upload (unless device-resident), extract, demodulate,
 // download.  Returns immediately; *ticket identifies the block.
 static int serve_submit(gpud_serve* s, const gpud_float2* iq, int n, bool on_device, int64_t* ticket)
@@ -272,7 +390,8 @@ static int serve_submit(gpud_serve* s, const gpud_float2* iq, int n, bool on_dev
     const int nch = (int)s->live.size();
     sl.outs.clear();
     sl.outs.reserve((size_t)nch);
-    int64_t iq_total = 0, audio_total = 0;
+    int64_t iq_total = 0, audio_total = 0, mono_total = 0, rds_total = 0;
+    int n_wfm = 0;
     for (auto& kv : s->live) {
         ServeChannel& c = kv.second;
         ServeOut o;
@@ -282,6 +401,21 @@ static int serve_submit(gpud_serve* s, const gpud_float2* iq, int n, bool on_dev
         o.n_audio = argus_demod_count(c.spec.demod, o.n_iq, c.have_carry);
         o.audio_off = audio_total;
         o.rate = c.spec.out_rate;
+        o.wfm_index = -1;
+        o.mono_off = o.rds_off = 0;
+        o.n_mono = o.n_rds = o.rds_rate = 0;
+        if (c.stereo) {
+            o.n_mono = argus_demod_count(ARGUS_DEMOD_WFM, o.n_iq, c.have_carry);
+            o.n_audio = 2 * o.n_mono;
+            o.wfm_index = n_wfm++;
+            o.mono_off = mono_total;
+            const int first = (int)((c.rds_decim - c.rds_count % c.rds_decim) % c.rds_decim);
+            o.n_rds = o.n_mono > first ? (o.n_mono - 1 - first) / c.rds_decim + 1 : 0;
+            o.rds_off = rds_total;
+            o.rds_rate = c.spec.out_rate / c.rds_decim;
+            mono_total += o.n_mono;
+            rds_total += o.n_rds;
+        }
         iq_total += o.n_iq;
         audio_total += o.n_audio;
         sl.outs.push_back(o);
@@ -294,10 +428,23 @@ static int serve_submit(gpud_serve* s, const gpud_float2* iq, int n, bool on_dev
     ARGUS_CHECK((hipError_t)grow_dev(&sl.d_chans, &sl.d_chans_cap, (size_t)std::max(nch, 1)));
     ARGUS_CHECK((hipError_t)grow_host(&sl.h_ddesc, &sl.h_ddesc_cap, (size_t)std::max(nch, 1)));
     ARGUS_CHECK((hipError_t)grow_dev(&sl.d_ddesc, &sl.d_ddesc_cap, (size_t)std::max(nch, 1)));
+    if (n_wfm > 0) {
+        ARGUS_CHECK((hipError_t)grow_dev(&sl.d_mono, &sl.mono_cap, (size_t)std::max<int64_t>(mono_total, 1)));
+        ARGUS_CHECK((hipError_t)grow_dev(&sl.d_wscratch, &sl.wscratch_cap, (size_t)std::max<int64_t>(4 * mono_total, 1)));
+        ARGUS_CHECK((hipError_t)grow_dev(&sl.d_rds, &sl.rds_cap, (size_t)std::max<int64_t>(rds_total, 1)));
+        ARGUS_CHECK((hipError_t)grow_host(&sl.h_rds, &sl.h_rds_cap, (size_t)std::max<int64_t>(rds_total, 1)));
+        ARGUS_CHECK((hipError_t)grow_dev(&sl.d_wblk, &sl.wblk_cap, (size_t)n_wfm));
+        ARGUS_CHECK((hipError_t)grow_host(&sl.h_wblk, &sl.h_wblk_cap, (size_t)n_wfm));
+        ARGUS_CHECK((hipError_t)grow_host(&sl.h_wdesc, &sl.h_wdesc_cap, (size_t)n_wfm));
+        ARGUS_CHECK((hipError_t)grow_dev(&sl.d_wdesc, &sl.d_wdesc_cap, (size_t)n_wfm));
+    }
+    sl.n_wfm = n_wfm;
+    sl.rds_total = rds_total;
 
     s->items.clear();
     s->item_pos.clear();
     s->ditems.clear();
+    s->witems.clear();
     size_t lds = 0;
     int ci = 0;
     const float2* in = on_device ? (const float2*)iq : sl.d_in;
@@ -315,11 +462,11 @@ static int serve_submit(gpud_serve* s, const gpud_float2* iq, int n, bool on_dev
 
         ArgusDemodDesc& dd = sl.h_ddesc[ci];
         dd.iq = sl.d_iq + o.iq_off;
-        dd.audio = sl.d_audio + o.audio_off;
+        dd.audio = c.stereo ? sl.d_mono + o.mono_off : sl.d_audio + o.audio_off;
         dd.carry_rd = c.d_carry + c.carry_sel;
         dd.carry_wr = c.d_carry + (c.carry_sel ^ 1);
         dd.n = o.n_iq;
-        dd.mode = c.spec.demod;
+        dd.mode = c.stereo ? ARGUS_DEMOD_WFM : c.spec.demod;
         dd.have_carry = c.have_carry ? 1 : 0;
         dd.skip = (dd.mode == ARGUS_DEMOD_NFM || dd.mode == ARGUS_DEMOD_WFM) && !c.have_carry ? 1 : 0;
         const double bfo_hz = dd.mode == ARGUS_DEMOD_LSB ? -700.0 : 700.0;
@@ -334,6 +481,7 @@ static int serve_submit(gpud_serve* s, const gpud_float2* iq, int n, bool on_dev
                     s->ditems.push_back({ci, st, std::min(ARGUS_DEMOD_TILE, o.n_iq - st), 0});
             }
         }
+        if (c.stereo) wfm_describe(c, o, sl, s->witems);
         // carry state forward (the kernels compute with the values above)
         c.phase_count = (c.phase_count + n) % c.decim;
         c.nco_phase = argus_reduce_phase(c.nco_phase + inc * (double)n);
@@ -349,6 +497,12 @@ static int serve_submit(gpud_serve* s, const gpud_float2* iq, int n, bool on_dev
     }
     argus_order_items(s->items, s->item_pos, n);
     const int n_items = (int)s->items.size(), n_ditems = (int)s->ditems.size();
+    const int n_witems = (int)s->witems.size();
+    if (n_witems > 0) {
+        ARGUS_CHECK((hipError_t)grow_host(&sl.h_witems, &sl.h_witems_cap, (size_t)n_witems));
+        ARGUS_CHECK((hipError_t)grow_dev(&sl.d_witems, &sl.d_witems_cap, (size_t)n_witems));
+        std::memcpy(sl.h_witems, s->witems.data(), (size_t)n_witems * sizeof(ArgusWfmItem));
+    }
     ARGUS_CHECK((hipError_t)grow_host(&sl.h_items, &sl.h_items_cap, (size_t)std::max(n_items, 1)));
     ARGUS_CHECK((hipError_t)grow_dev(&sl.d_items, &sl.d_items_cap, (size_t)std::max(n_items, 1)));
     ARGUS_CHECK((hipError_t)grow_host(&sl.h_ditems, &sl.h_ditems_cap, (size_t)std::max(n_ditems, 1)));
@@ -375,10 +529,19 @@ static int serve_submit(gpud_serve* s, const gpud_float2* iq, int n, bool on_dev
                                s->compute));
     ARGUS_CHECK(hipMemcpyAsync(sl.d_ditems, sl.h_ditems, (size_t)n_ditems * sizeof(ArgusDemodItem),
                                hipMemcpyHostToDevice, s->compute));
+    if (n_wfm > 0) {
+        ARGUS_CHECK(hipMemcpyAsync(sl.d_wdesc, sl.h_wdesc, (size_t)n_wfm * sizeof(ArgusWfmDesc),
+                                   hipMemcpyHostToDevice, s->compute));
+        if (n_witems > 0)
+            ARGUS_CHECK(hipMemcpyAsync(sl.d_witems, sl.h_witems, (size_t)n_witems * sizeof(ArgusWfmItem),
+                                       hipMemcpyHostToDevice, s->compute));
+    }
     ARGUS_CHECK(hipEventRecord(sl.ev_kstart, s->compute));
     err = argus_launch_extract_batch(sl.d_chans, sl.d_items, n_items, lds, in, n, s->compute);
     if (err) return err;
     err = argus_launch_demod(sl.d_ddesc, sl.d_ditems, n_ditems, s->compute);
+    if (err) return err;
+    err = argus_launch_wfm(sl.d_wdesc, n_wfm, sl.d_witems, n_witems, s->compute);
     if (err) return err;
     ARGUS_CHECK(hipEventRecord(sl.ev_kern, s->compute));
     // ---- copy stream: download ------------------------------------------
@@ -389,6 +552,12 @@ static int serve_submit(gpud_serve* s, const gpud_float2* iq, int n, bool on_dev
                                        s->copy));
         if (audio_total > 0)
             ARGUS_CHECK(hipMemcpyAsync(sl.h_audio, sl.d_audio, (size_t)audio_total * sizeof(float),
+                                       hipMemcpyDeviceToHost, s->copy));
+        if (rds_total > 0)
+            ARGUS_CHECK(hipMemcpyAsync(sl.h_rds, sl.d_rds, (size_t)rds_total * sizeof(float2), hipMemcpyDeviceToHost,
+                                       s->copy));
+        if (n_wfm > 0)
+            ARGUS_CHECK(hipMemcpyAsync(sl.h_wblk, sl.d_wblk, (size_t)n_wfm * sizeof(ArgusWfmBlock),
                                        hipMemcpyDeviceToHost, s->copy));
     }
     ARGUS_CHECK(hipEventRecord(sl.ev_done, s->copy));
@@ -463,6 +632,29 @@ GPUD_API int gpud_serve_channel_output(gpud_serve* s, int64_t ticket, int64_t ch
             if (audio) *audio = sl->h_audio + o.audio_off;
             if (n_audio) *n_audio = o.n_audio;
             if (rate) *rate = o.rate;
+            return 0;
+        }
+    return -1;
+}
+
+// RDS baseband and stereo lock of a WFM stereo channel in a completed block.
+// -1 if the channel was not part of the block, -3 if it is not a stereo
+// channel.
+GPUD_API int gpud_serve_channel_wfm(gpud_serve* s, int64_t ticket, int64_t channel_id, const gpud_float2** rds,
+                                    int* n_rds, int* rds_rate, int* locked, int* stereo)
+{
+    if (!s) return -1;
+    ServeSlot* sl = find_slot(s, ticket);
+    if (!sl || sl->busy) return -2;
+    for (const ServeOut& o : sl->outs)
+        if (o.channel_id == channel_id) {
+            if (o.wfm_index < 0) return -3;
+            if (rds) *rds = (const gpud_float2*)(sl->h_rds + o.rds_off);
+            if (n_rds) *n_rds = o.n_rds;
+            if (rds_rate) *rds_rate = o.rds_rate;
+            const ArgusWfmBlock& b = sl->h_wblk[o.wfm_index];
+            if (locked) *locked = o.n_mono > 0 ? b.locked : 0;
+            if (stereo) *stereo = b.stereo;
             return 0;
         }
     return -1;

@@ -2,6 +2,7 @@
 #pragma once
 #include "extract.hpp"
 #include "demod.hpp"
+#include "wfm.hpp"
 #include <map>
 #include <vector>
 
@@ -21,12 +22,23 @@ struct ServeChannel {
     int phase_count = 0;
     double bfo_phase = 0.0;
     int64_t last_ticket = 0;  // last block that used the channel
+    // WFM stereo + RDS (demod GPUD_DEMOD_WFM_STEREO): one device block of
+    // ArgusWfmState | mono delay lines [2] | L-R delay lines [2] |
+    // fallback delay lines [2] | tap sets
+    bool stereo = false;
+    char* d_wfm = nullptr;
+    int wfm_sel = 0;         // ping-pong half of the mono and L-R delay lines
+    int64_t rds_count = 0;   // discriminator samples seen so far
+    int rds_decim = 1;
 };
 
 struct ServeOut {
     int64_t channel_id;
     int64_t iq_off, audio_off;
     int n_iq, n_audio, rate;
+    int wfm_index;           // index among the block's stereo channels, or -1
+    int64_t mono_off, rds_off;
+    int n_mono, n_rds, rds_rate;
 };
 
 struct ServeSlot {
@@ -54,6 +66,27 @@ struct ServeSlot {
     ArgusDemodItem* h_ditems = nullptr;
     ArgusDemodItem* d_ditems = nullptr;
     size_t h_ditems_cap = 0, d_ditems_cap = 0;
+    // WFM stereo stage
+    float* d_mono = nullptr;
+    size_t mono_cap = 0;
+    float* d_wscratch = nullptr;
+    size_t wscratch_cap = 0;
+    float2* d_rds = nullptr;
+    size_t rds_cap = 0;
+    float2* h_rds = nullptr;
+    size_t h_rds_cap = 0;
+    ArgusWfmBlock* d_wblk = nullptr;
+    size_t wblk_cap = 0;
+    ArgusWfmBlock* h_wblk = nullptr;
+    size_t h_wblk_cap = 0;
+    ArgusWfmDesc* h_wdesc = nullptr;
+    ArgusWfmDesc* d_wdesc = nullptr;
+    size_t h_wdesc_cap = 0, d_wdesc_cap = 0;
+    ArgusWfmItem* h_witems = nullptr;
+    ArgusWfmItem* d_witems = nullptr;
+    size_t h_witems_cap = 0, d_witems_cap = 0;
+    int n_wfm = 0;
+    int64_t rds_total = 0;
     int64_t iq_total = 0, audio_total = 0;
     std::vector<ServeOut> outs;
 };
@@ -74,4 +107,5 @@ struct gpud_serve {
     std::vector<ArgusWorkItem> items;
     std::vector<int> item_pos;
     std::vector<ArgusDemodItem> ditems;
+    std::vector<ArgusWfmItem> witems;
 };

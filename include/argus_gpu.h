@@ -219,7 +219,12 @@ enum {
     GPUD_DEMOD_AM = 3,   /* am.go:11-27 */
     GPUD_DEMOD_USB = 4,  /* ssb.go:19-39, BFO +700 Hz, carried phase */
     GPUD_DEMOD_LSB = 5,  /* ssb.go:19-39, BFO -700 Hz */
-    GPUD_DEMOD_CW = 6    /* cw.go:11-26 */
+    GPUD_DEMOD_CW = 6,   /* cw.go:11-26 */
+    /* WFM multiplex: the discriminator as WFM, then the stateful stereo
+     * decoder with lock hysteresis and mono fallback (streamer.go:1367-1395,
+     * 1496-1643) and the RDS baseband (fm.go:234-271) carried across
+     * blocks.  audio = 2 x samples, L/R interleaved. */
+    GPUD_DEMOD_WFM_STEREO = 7
 };
 
 #define GPUD_ERR_BUDGET (-20)    /* admission refused: HBM budget exhausted */
@@ -264,6 +269,33 @@ GPUD_API int gpud_serve_wait(gpud_serve* s, int64_t ticket, gpud_serve_block_inf
  * its slot is reused, i.e. for the next two submits). */
 GPUD_API int gpud_serve_channel_output(gpud_serve* s, int64_t ticket, int64_t channel_id, const gpud_float2** iq,
                                        int* n_iq, const float** audio, int* n_audio, int* rate);
+/* RDS baseband samples (rate = out_rate / floor(out_rate / 19000)) and the
+ * stereo lock of a GPUD_DEMOD_WFM_STEREO channel in a completed block:
+ * *locked = this block's lock test (streamer.go:1634), *stereo = 1 when the
+ * block's audio is the L/R matrix, 0 when it is the mono fallback.  Valid
+ * while downloads are enabled.  -3 if the channel is not a stereo channel. */
+GPUD_API int gpud_serve_channel_wfm(gpud_serve* s, int64_t ticket, int64_t channel_id, const gpud_float2** rds,
+                                    int* n_rds, int* rds_rate, int* locked, int* stereo);
+
+/* ------------------------------------------------------------------ */
+/* RDS group decoder (host): successor of rds.Decoder.Decode             */
+/* (internal/rds/rds.go:78-264, tryDecode 365-430, decodeGroupAt 432-473, */
+/* applyGroup 475-502, decodeBlockExpected 549-579, crcSyndrome 590-599): */
+/* Mueller & Muller timing, Costas loop, differential bits carried across  */
+/* calls, CRC block sync with 1-bit FEC and the +-3-bit group flywheel.   */
+/* ------------------------------------------------------------------ */
+typedef struct gpud_rds_decoder gpud_rds_decoder;
+GPUD_API int gpud_rds_decoder_create(gpud_rds_decoder** out);
+GPUD_API int gpud_rds_decoder_destroy(gpud_rds_decoder* d);
+/* One Decode call on n complex baseband samples.  Returns the groups found in
+ * this call (>= 0), -1 on bad arguments. */
+GPUD_API int gpud_rds_decode(gpud_rds_decoder* d, const gpud_float2* samples, int n, int sample_rate);
+/* PI, PS (8 chars + NUL) and RT (64 chars + NUL) as the decoder holds them
+ * (rds.go:604-636: unset characters read as spaces, trailing spaces trimmed). */
+GPUD_API int gpud_rds_decoder_state(gpud_rds_decoder* d, uint16_t* pi, char* ps9, char* rt65);
+/* Up to cap groups applied so far (4 x 16-bit blocks each, in order);
+ * returns the total number applied. */
+GPUD_API int gpud_rds_decoder_groups(gpud_rds_decoder* d, uint16_t* groups4, int cap);
 
 /* ------------------------------------------------------------------ */
 /* Surveillance: Welch PSD, FFT, CFAR and the detection list.           */

@@ -630,7 +630,7 @@ class RDSDecoder:
 
 class orc_rds_stream(C.Structure):
     _fields_ = [("taps", C.c_double * 103), ("hr", C.c_double * 103), ("hi", C.c_double * 103),
-                ("phase", C.c_double), ("inc", C.c_double), ("count", C.c_longlong), ("nt", C.c_int),
+                ("mix_num", C.c_longlong), ("count", C.c_longlong), ("nt", C.c_int),
                 ("decim", C.c_int), ("rate", C.c_int), ("pad", C.c_int)]
 
 

@@ -81,7 +81,7 @@ int orc_ssb_demod(const float* iq, int n, int sample_rate, int usb, double* bfo_
         double c = cos(phase), s = sin(phase);
         out[i] = (float)((double)iq[2 * i] * c - (double)iq[2 * i + 1] * s);
     }
-    *bfo_phase_io = phase;
+    *bfo_phase_io = fmod(phase, 2 * ORC_PI); /* carried modulo one turn */
     return n;
 }
 
@@ -95,7 +95,7 @@ int orc_cw_demod(const float* iq, int n, int sample_rate, double* bfo_phase_io, 
         phase += inc;
         out[i] = (float)((double)iq[2 * i] * cos(phase));
     }
-    *bfo_phase_io = phase;
+    *bfo_phase_io = fmod(phase, 2 * ORC_PI); /* carried modulo one turn */
     return n;
 }
 

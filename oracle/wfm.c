@@ -478,7 +478,8 @@ int orc_rds_encode_groups(uint16_t pi, const char* ps8, const char* rt64, int n_
 /*
  * The RDS baseband of internal/demod/fm.go:234-271 (RDSBasebandComplexInto)
  * carried across blocks instead of restarted per call: the +57 kHz mixer
- * phase (pre-incremented, as FreqShift(-57000) does), the 101-tap
+ * phase (pre-incremented, as FreqShift(-57000) does, but computed exactly
+ * from the sample count instead of by repeated FP64 addition), the 101-tap
  * LowpassFIR(7500) complex FIR history (FP64 accumulation as ApplyFIR) and
  * the decimation phase (keep stream indices 0, D, 2D, ...; D = fs/19000).
  * Input is the FM discriminator output.  Returns the output count.
@@ -490,16 +491,19 @@ void orc_rds_stream_init(orc_rds_stream* s, int sample_rate)
     s->nt = orc_lowpass_fir(7500, sample_rate, 101, s->taps);
     s->decim = sample_rate / 19000;
     if (s->decim < 1) s->decim = 1;
-    s->inc = -2 * ORC_PI * (-57000.0) / (double)sample_rate;
+    s->mix_num = 0;
 }
 
 int orc_rds_stream_process(orc_rds_stream* s, const float* disc, int n, float* out)
 {
+    if (s->rate <= 0) return 0;
     int m = 0;
     for (int i = 0; i < n; i++) {
-        s->phase += s->inc;
+        /* phase += -2 pi (-57000) / fs, pre-incremented as FreqShift does */
+        s->mix_num = (s->mix_num + 57000) % s->rate;
+        const double phase = 2 * ORC_PI * (double)s->mix_num / (double)s->rate;
         double b = disc[i];
-        float cr = (float)(b * cos(s->phase)), ci = (float)(b * sin(s->phase));
+        float cr = (float)(b * cos(phase)), ci = (float)(b * sin(phase));
         /* delay line, newest first */
         memmove(s->hr + 1, s->hr, sizeof(double) * (size_t)(s->nt - 1));
         memmove(s->hi + 1, s->hi, sizeof(double) * (size_t)(s->nt - 1));

@@ -41,11 +41,13 @@ typedef struct orc_rds_decoder {
     uint16_t groups[ORC_RDS_MAX_GROUPS][4];
 } orc_rds_decoder;
 
-/* streaming RDS baseband state (fm.go:234-271 carried across blocks) */
+/* streaming RDS baseband state (fm.go:234-271 carried across blocks).  The
+ * +57 kHz mixer phase of input sample c is 2 pi ((c + 1) * 57000 mod fs) / fs,
+ * carried as the exact integer numerator, so it does not drift. */
 typedef struct orc_rds_stream {
     double taps[103];
     double hr[103], hi[103];
-    double phase, inc;
+    long long mix_num;  /* (count * 57000) mod rate */
     long long count;
     int nt, decim, rate, pad;
 } orc_rds_stream;

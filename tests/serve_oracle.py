@@ -5,7 +5,7 @@ import numpy as np
 
 import oracle as O
 
-NONE, NFM, WFM, AM, USB, LSB, CW = range(7)
+NONE, NFM, WFM, AM, USB, LSB, CW, WFM_STEREO = range(8)
 
 
 class OracleChannel:
@@ -38,6 +38,30 @@ class OracleChannel:
         else:
             audio = np.zeros(0, np.float32)
         return out, audio
+
+
+class OracleStereoChannel:
+    """A WFM stereo channel: extraction and the bridged discriminator
+    (OracleChannel), then stereoDecodeStateful with the lock hysteresis and
+    the mono fallback filter (streamer.go:1367-1395, 1511-1643), and the
+    streaming RDS baseband of the discriminator output (fm.go:234-271)."""
+
+    def __init__(self, sample_rate, offset_hz, bandwidth, out_rate, num_taps=101, window=0):
+        self.ch = OracleChannel(sample_rate, offset_hz, bandwidth, out_rate, num_taps, WFM, window)
+        self.stereo = O.StereoDecoder(out_rate)
+        self.fallback = O.StatefulFIRReal(O.LowpassFIR(15000, out_rate, 101))
+        self.rds = O.RDSBasebandStream(out_rate)
+
+    def block(self, iq):
+        """(baseband, audio L/R interleaved, rds baseband, locked, stereo)."""
+        out, mono = self.ch.block(iq)
+        locked, stereo = False, self.stereo.Enabled
+        audio = np.zeros(0, np.float32)
+        if len(mono):
+            lr, locked = self.stereo.Decode(mono)
+            stereo = self.stereo.Enabled
+            audio = lr if stereo else np.repeat(self.fallback.Process(mono), 2)
+        return out, audio, self.rds.Process(mono), locked, stereo
 
 
 def synth_scene(n, sr, carriers, seed=0, noise=0.05):
