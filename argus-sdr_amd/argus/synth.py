@@ -60,6 +60,40 @@ def surveillance_scene(n: int, fs: float, n_carriers: int = 40, seed: int = 0, s
 RDS_BAUD = 1187.5
 
 
+_RDS_OFFSETS = (0x0FC, 0x198, 0x168, 0x1B4)  # A, B, C, D offset words (IEC 62106)
+
+
+def _rds_block(data: int, offset: int) -> int:
+    """26-bit block: 16 information bits, then the 10-bit checkword (the
+    remainder of data * x^10 modulo x^10+x^8+x^7+x^5+x^4+x^3+1) xor the
+    offset word."""
+    reg = data << 10
+    for bit in range(25, 9, -1):
+        if reg >> bit & 1:
+            reg ^= 0x5B9 << (bit - 10)
+    return (data << 10) | ((reg & 0x3FF) ^ offset)
+
+
+def rds_group_bits(pi: int, ps: str, rt: str, n_groups: int) -> np.ndarray:
+    """Data bits (MSB first) of n_groups RDS groups alternating 0A (two PS
+    characters per group) and 2A (four RT characters per group)."""
+    ps8 = ps.encode("latin-1").ljust(8)[:8]
+    rt64 = rt.encode("latin-1").ljust(64)[:64]
+    out = np.zeros(104 * n_groups, np.uint8)
+    for g in range(n_groups):
+        if g % 2 == 0:
+            a = (g // 2) % 4
+            words = (pi, a, 0xE0CD, ps8[2 * a] << 8 | ps8[2 * a + 1])
+        else:
+            a = (g // 2) % 16
+            words = (pi, 2 << 12 | a, rt64[4 * a] << 8 | rt64[4 * a + 1], rt64[4 * a + 2] << 8 | rt64[4 * a + 3])
+        for k, (w, off) in enumerate(zip(words, _RDS_OFFSETS)):
+            blk = _rds_block(w, off)
+            base = 104 * g + 26 * k
+            out[base:base + 26] = [(blk >> (25 - i)) & 1 for i in range(26)]
+    return out
+
+
 def rds_symbols(bits: np.ndarray) -> np.ndarray:
     """Differentially encode data bits (s_i = s_{i-1} xor d_i) and map to NRZ
     levels +-1; the decoder recovers d_i = h_i xor h_{i-1}
@@ -102,3 +136,50 @@ class WFMStation:
         self.phase = float(np.mod(ph[-1], 2 * np.pi)) if m else self.phase
         self.n += m
         return (self.amp * np.exp(1j * ph)).astype(np.complex64)
+
+
+class StereoScene:
+    """Wideband span of stereo WFM stations with RDS (the 20-station
+    multiplex shape), generated block by block.  Each station's composite is
+    synthesised at a low rate (WFMStation at `low_rate`, whose 200 kHz
+    occupied band it represents), its phase is interpolated to the span rate
+    and the carrier offset added; AWGN sets each station's in-band SNR
+    (synth.go:84-90 definition over 200 kHz).  Returns complex64 blocks."""
+
+    def __init__(self, fs: float, offsets, snr_db, stations_rds, seed: int = 0, sigma: float = 0.01,
+                 low_rate: float = 500_000.0):
+        self.fs, self.low = fs, low_rate
+        self.r = int(round(fs / low_rate))
+        assert self.r * low_rate == fs
+        self.offsets = list(offsets)
+        self.sigma = sigma
+        self.amps = [np.sqrt(carrier_power(s, sigma, 200e3, fs)) for s in snr_db]
+        self.st = [WFMStation(low_rate, 0.0, rds_symbols(bits), seed=seed + 1 + k)
+                   for k, bits in enumerate(stations_rds)]
+        self.last_ph = [0.0] * len(self.st)
+        self.n = 0
+        self.rng = np.random.default_rng(seed)
+
+    def block(self, m: int) -> np.ndarray:
+        assert m % self.r == 0
+        ml = m // self.r
+        t = (np.arange(m) + self.n) / self.fs
+        x = awgn(m, self.sigma, self.rng).astype(np.complex64)
+        xi = np.arange(m, dtype=np.float64) / self.r + (1.0 - 1.0 / self.r)
+        lows = [st.block(ml) for st in self.st]
+
+        def station(k):
+            ph = np.unwrap(np.concatenate([[self.last_ph[k]], np.angle(lows[k])]))
+            # low-rate sample j sits at span index (j + 1) * r - 1 of this block
+            p = np.interp(xi, np.arange(ml + 1), ph)
+            p += 2 * np.pi * self.offsets[k] * t
+            return ph[-1], (self.amps[k] * np.exp(1j * p)).astype(np.complex64)
+
+        from concurrent.futures import ThreadPoolExecutor
+        with ThreadPoolExecutor(max_workers=min(8, len(self.st) or 1)) as ex:
+            parts = list(ex.map(station, range(len(self.st))))
+        for k, (last, v) in enumerate(parts):
+            self.last_ph[k] = last
+            x += v
+        self.n += m
+        return x

@@ -2,21 +2,29 @@
 """Benchmark: real-time channels sustained at a 20 Msps span on one MI355X.
 
 Workload (the serving shape at 20 Msps): complex64 IQ at 20 Msps in 20 ms
-blocks (400 000 samples); CH channels drawn from the uneven mode mix
+blocks (400 000 samples); N channels drawn from the uneven mode mix
 {NFM 12.5 kHz, AM 10 kHz, USB 3 kHz, WFM 200 kHz} with decimations 8..1600,
 odd FIR lengths 63..511 and the matching demodulator (seed 0).  One step =
 one block through the streaming service: one fused extraction launch (NCO
 shift + polyphase FIR/decimate for every channel) and one demodulation
-launch, with the block already resident in HBM and blocks pipelined.
+launch, with the block already resident in HBM.
 
-    value = CH * block_duration / ms_per_step      (real-time channels)
+    value = the largest N whose p99 block time is <= 5 ms  (real-time channels)
 
-Reported beside it in the same run: per-block latency p50/p99 (GPU event
-time, and host-to-host with the block's upload and the outputs' download),
-the plain one-kernel-per-signal GPU form (the reference's call pattern), the
-roofline of the fused extraction kernel, the surveillance pass (Welch +
-CA-CFAR + detection list at 10 Msps, N = 32768) and the 2.048 Msps 4-station
-WFM case, and the CPU oracle timed on a bounded sample on the host cores.
+found by a measured sweep (N doubling from 256, then bisection to 256
+channels; block time = wall time from the block's submit call, host-side
+descriptor build included, to the return of its wait, one block in flight,
+input resident in HBM, outputs left in HBM).  At that N the K timed steps run with two
+blocks in flight (ms_per_step), and the same sweep is run for the plain
+one-kernel-per-signal GPU form (the reference's call pattern, with its
+synchronous copies) for the fused/plain ratio.
+
+Reported beside it in the same run: p50/p99 block latency (GPU events, and
+host-to-host with the block's upload and the outputs' download), the
+roofline of the fused extraction kernel at N, the surveillance pass (Welch +
+CA-CFAR + detection list at 10 Msps, N = 32768), the 2.048 Msps 4-station
+WFM case, the 20 Msps 20-station stereo WFM + RDS case, and the CPU oracle
+timed on a bounded sample on the host cores.
 """
 from __future__ import annotations
 
@@ -41,6 +49,7 @@ BLOCK_S = BLOCK / FS
 MODES = [("NFM", 12_500.0, [400, 800], 1), ("AM", 10_000.0, [500, 1000], 3), ("USB", 3_000.0, [1250, 1600], 4),
          ("WFM", 200_000.0, [8, 16, 40, 80], 2)]
 HBM_PEAK_GBS = 8000.0
+P99_BOUND_MS = 5.0
 FP32_PEAK_TFLOPS = 157.3
 METRIC = "real-time channels sustained @20 Msps span (1 GPU); p99 block latency; HBM GB/s"
 
@@ -74,15 +83,18 @@ def alg_figures(chans, n_in):
     return b, f
 
 
-def load_traffic():
+def load_traffic(channels):
     """Measured HBM bytes per fused launch (rocprofv3 FETCH_SIZE + WRITE_SIZE,
-    scaled by a copy of known size), if the profile summary is present."""
+    scaled by a copy of known size) for this channel count, if the profile
+    summary holds it."""
     p = ROOT / "profiles" / "extract_traffic.json"
     if p.exists():
         try:
-            return json.loads(p.read_text()).get("bytes_per_launch")
+            d = json.loads(p.read_text())
         except (ValueError, OSError):
             return None
+        if d.get("channels") == channels:
+            return d.get("bytes_per_launch")
     return None
 
 
@@ -140,6 +152,76 @@ def bench_serve(a, chans, blocks, lib, S):
     return wall * 1e3 / a.steps, kern, gpu, lat
 
 
+def serve_block_times(chans, d_block, lib, S, blocks=50, warmup=5):
+    """Latency of each block through the service, one block in flight: wall
+    time from the submit call (host-side descriptor build included) to the
+    return of its wait, device-resident input, outputs left in HBM."""
+    s = S.Serve(FS, BLOCK, 256 << 30)
+    for (cid, off, bw, rate, taps, dm) in chans:
+        assert s.Open(S.ChannelSpec(cid, off, bw, rate, taps, dm, 0))
+    s.SetDownload(False)
+    out = []
+    for i in range(warmup + blocks):
+        t1 = time.perf_counter()
+        s.Wait(s.SubmitDevice(d_block, BLOCK))
+        if i >= warmup:
+            out.append((time.perf_counter() - t1) * 1e3)
+    s.Close()
+    return out
+
+
+def capacity_sweep(times_fn, start, bound_ms, cap, grain):
+    """Largest N with p99(times_fn(N)) <= bound_ms: doubling from `start`,
+    then bisection down to `grain` channels.  Returns (N, table)."""
+    table = {}
+
+    def p99(n):
+        if n not in table:
+            t = times_fn(n)
+            table[n] = (percentile(t, 50), percentile(t, 99))
+            print(f"[bench]   N={n}: p50 {table[n][0]:.3f} ms, p99 {table[n][1]:.3f} ms", file=sys.stderr,
+                  flush=True)
+        return table[n][1]
+
+    lo, hi = 0, None
+    n = start
+    while n <= cap:
+        if p99(n) <= bound_ms:
+            lo = n
+            n *= 2
+        else:
+            hi = n
+            break
+    if hi is None:
+        return lo, table
+    if lo == 0:  # even the first size misses the bound: bisect below it
+        lo = 0
+    while hi - lo > grain:
+        mid = (lo + hi) // 2 // grain * grain
+        if mid <= lo:
+            break
+        if p99(mid) <= bound_ms:
+            lo = mid
+        else:
+            hi = mid
+    return lo, table
+
+
+def plain_block_times(chans, block, G, blocks=6):
+    """Wall time per block of the plain per-signal form (synchronous copies
+    per signal, the reference's call pattern)."""
+    jobs = [G.StreamingExtractJob(c[0], c[1], c[2], c[3], c[4], 0x9E3779B97F4A7C15 ^ c[0]) for c in chans]
+    rp = G.NewBatchRunner(BLOCK, FS, exec_mode=G.EXEC_PER_SIGNAL)
+    rp.StreamingExtractGPU(block, jobs)
+    out = []
+    for _ in range(blocks):
+        t1 = time.perf_counter()
+        rp.StreamingExtractGPU(block, jobs)
+        out.append((time.perf_counter() - t1) * 1e3)
+    rp.Close()
+    return out
+
+
 def bench_extract_kernel(a, chans, blocks, lib, G):
     """Event-timed fused extraction kernel (runner API, device input)."""
     from argus._lib import check
@@ -160,17 +242,6 @@ def bench_extract_kernel(a, chans, blocks, lib, G):
     r.Close()
     lib.gpud_free(p.value)
     return tot.value / max(nl.value, 1), jobs
-
-
-def bench_plain(a, jobs, blocks, G):
-    rp = G.NewBatchRunner(BLOCK, FS, exec_mode=G.EXEC_PER_SIGNAL)
-    rp.StreamingExtractGPU(blocks[0], jobs)
-    t1 = time.perf_counter()
-    for i in range(a.plain_blocks):
-        rp.StreamingExtractGPU(blocks[(i + 1) % len(blocks)], jobs)
-    ms = (time.perf_counter() - t1) * 1e3 / a.plain_blocks
-    rp.Close()
-    return ms
 
 
 def bench_surveillance(a, lib):
@@ -272,6 +343,70 @@ def bench_wfm_2048k(a, lib, S):
             "realtime_factor": round(1.0 / wall, 1)}, x[:chunk * 8], offs
 
 
+def bench_wfm_stereo_20m(a, lib, S):
+    """20 Msps span, 20 stereo WFM stations with RDS (pilot 19 kHz, L-R at
+    38 kHz, RDS at 57 kHz with seeded PI/PS/RT groups, SNR 15-45 dB); per
+    station: shift, /80 to 250 ksps, FM discriminator, pilot PLL stereo
+    decoder, RDS baseband; RDS groups decoded on the host from the downloaded
+    baseband.  Blocks are generated on the fly and submitted pipelined."""
+    from argus.synth import StereoScene, rds_group_bits
+    from argus.wfm import RDSDecoder
+    rng = np.random.default_rng(5)
+    n_st = 20
+    offs = [-9.0e6 + k * 0.9e6 + float(rng.uniform(-100e3, 100e3)) for k in range(n_st)]
+    snr = rng.uniform(15.0, 45.0, n_st)
+    pis = [int(v) for v in rng.integers(0x1000, 0xFFFF, n_st)]
+    bits = [rds_group_bits(pis[k], f"ST{k:02d} FM", f"radiotext of station {k}", 64) for k in range(n_st)]
+    scene = StereoScene(FS, offs, snr, bits, seed=5, low_rate=1_000_000.0)
+    n_blocks = a.stereo_blocks
+    print(f"[bench]   generating {n_blocks} blocks", file=sys.stderr, flush=True)
+    blocks = [scene.block(BLOCK) for _ in range(n_blocks)]
+    s = S.Serve(FS, BLOCK, 1 << 30)
+    for k in range(n_st):
+        assert s.Open(S.ChannelSpec(k + 1, offs[k], 200_000.0, 250_000, 101, S.DEMOD_WFM_STEREO, 0))
+    s.SetDownload(True)
+    rds = {k: [] for k in range(n_st)}
+    stereo = np.zeros(n_st, np.int64)
+    gpu, kern = [], []
+
+    def collect(t):
+        info = s.Wait(t)
+        gpu.append(info.gpu_ms)
+        kern.append(info.kernel_ms)
+        for k in range(n_st):
+            x, rate, _, st = s.OutputWFM(t, k + 1)
+            rds[k].append(x)
+            stereo[k] += st
+        return rate
+
+    pending = []
+    t0 = time.perf_counter()
+    rate = 0
+    for blk in blocks:
+        pending.append(s.Submit(blk))
+        if len(pending) == 2:
+            rate = collect(pending.pop(0))
+    for t in pending:
+        rate = collect(t)
+    wall = time.perf_counter() - t0
+    s.Close()
+    groups, pi_ok = 0, 0
+    for k in range(n_st):
+        d = RDSDecoder()
+        x = np.concatenate(rds[k])
+        for i in range(0, len(x), rate // 2):
+            d.Decode(x[i:i + rate // 2], rate)
+        groups += len(d.Groups())
+        pi_ok += int(d.PI == pis[k])
+    ms_block = wall * 1e3 / n_blocks
+    return {"stations": n_st, "blocks": n_blocks, "block_ms_host_to_host": round(ms_block, 3),
+            "p50_gpu_ms": round(percentile(gpu, 50), 3), "p99_gpu_ms": round(percentile(gpu, 99), 3),
+            "p50_kernels_ms": round(percentile(kern, 50), 3), "p99_kernels_ms": round(percentile(kern, 99), 3),
+            "realtime_factor": round(BLOCK_S * 1e3 / ms_block, 1),
+            "stereo_blocks_per_station": round(float(stereo.mean()), 1),
+            "rds_groups_decoded": groups, "rds_pi_correct_stations": pi_ok}
+
+
 def cpu_baseline_extract(chans, block, threads):
     import oracle as O
     jobs = [O.orc_job(c[0], c[1], c[2], c[3], c[4], 0x9E3779B97F4A7C15 ^ c[0]) for c in chans]
@@ -291,12 +426,16 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=300)
     ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--channels", type=int, default=256)
-    ap.add_argument("--plain-blocks", type=int, default=2)
+    ap.add_argument("--channels", type=int, default=0,
+                    help="fixed channel count (0: the p99 <= 5 ms capacity found by the sweep)")
+    ap.add_argument("--sweep-cap", type=int, default=32768)
+    ap.add_argument("--plain-cap", type=int, default=512)
     ap.add_argument("--cpu-channels", type=int, default=32)
     ap.add_argument("--surv-frames", type=int, default=500)
+    ap.add_argument("--stereo-blocks", type=int, default=30)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--no-extras", action="store_true", help="skip surveillance and the 2.048 Msps case")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="skip surveillance, the 2.048 Msps case and the stereo multiplex case")
     ap.add_argument("--calibrate-copy-mb", type=int, default=0,
                     help="run one device-to-device copy of this many MiB (counter calibration)")
     a = ap.parse_args()
@@ -307,13 +446,12 @@ def main() -> int:
 
     from argus import gpudemod as G
     from argus import serve as S
-    from argus._lib import lib
+    from argus._lib import check, lib
     L = lib()
     if not G.Available():
         print("no HIP device", file=sys.stderr)
         return 1
 
-    chans = make_channels(a.channels)
     blocks = [make_block(s) for s in range(4)]
     if a.calibrate_copy_mb > 0:
         # a device-to-device copy of known size: its FETCH_SIZE / WRITE_SIZE
@@ -327,26 +465,41 @@ def main() -> int:
         L.gpud_device_sync()
         L.gpud_free(src)
         L.gpud_free(dst)
-    print(f"[bench] serve: {a.channels} channels, block {BLOCK} samples @ {FS / 1e6:.0f} Msps", file=sys.stderr,
+    d0 = C.c_void_p()
+    check("gpud_malloc", L.gpud_malloc(C.byref(d0), blocks[0].nbytes))
+    check("gpud_memcpy_h2d", L.gpud_memcpy_h2d(d0, blocks[0].ctypes.data, blocks[0].nbytes))
+
+    # ---- capacity: largest N with p99 block time <= 5 ms ------------------
+    sweep = {}
+    if a.channels > 0:
+        n_rt = a.channels
+    else:
+        print("[bench] fused capacity sweep (p99 block time <= 5 ms)", file=sys.stderr, flush=True)
+        n_rt, sweep = capacity_sweep(lambda n: serve_block_times(make_channels(n), d0.value, L, S),
+                                     256, P99_BOUND_MS, a.sweep_cap, 256)
+    print("[bench] plain per-signal capacity sweep", file=sys.stderr, flush=True)
+    n_plain, plain_sweep = capacity_sweep(lambda n: plain_block_times(make_channels(n), blocks[0], G),
+                                          4, P99_BOUND_MS, a.plain_cap, 1)
+    chans = make_channels(n_rt)
+
+    print(f"[bench] serve: {n_rt} channels, block {BLOCK} samples @ {FS / 1e6:.0f} Msps", file=sys.stderr,
           flush=True)
     ms_per_step, kern, gpu, lat = bench_serve(a, chans, blocks, L, S)
-    value = a.channels * BLOCK_S * 1e3 / ms_per_step
     print("[bench] fused extraction kernel", file=sys.stderr, flush=True)
     kern_ms, jobs = bench_extract_kernel(a, chans, blocks, L, G)
-    print("[bench] plain per-signal form", file=sys.stderr, flush=True)
-    plain_ms = bench_plain(a, jobs, blocks, G)
-    plain_channels = a.channels * BLOCK_S * 1e3 / plain_ms
-
     b_alg, f_alg = alg_figures(chans, BLOCK)
     achieved = b_alg / (kern_ms * 1e-3) / 1e9
     valu = f_alg / (kern_ms * 1e-3) / 1e12
+    L.gpud_free(d0.value)
 
     extras = {}
     if not a.no_extras:
         print("[bench] surveillance", file=sys.stderr, flush=True)
-        extras["surveillance_10msps_n32768"], surv_sample = bench_surveillance(a, L)
+        extras["surveillance_10msps_n32768"], _ = bench_surveillance(a, L)
         print("[bench] 2.048 Msps WFM", file=sys.stderr, flush=True)
         extras["wfm_2048ksps_4stations"], _, _ = bench_wfm_2048k(a, L, S)
+        print("[bench] 20 Msps stereo WFM + RDS, 20 stations", file=sys.stderr, flush=True)
+        extras["wfm_stereo_rds_20msps_20stations"] = bench_wfm_stereo_20m(a, L, S)
 
     cpu = None
     if not a.no_cpu:
@@ -358,9 +511,12 @@ def main() -> int:
                "sample": f"oracle streaming extractor (no demod), {len(sub)} channels of the same mix x one "
                          f"20 ms block ({dt:.2f} s of CPU work, one thread); host has {os.cpu_count()} cores"}
 
+    def table(t):
+        return {str(k): {"p50_ms": round(v[0], 3), "p99_ms": round(v[1], 3)} for k, v in sorted(t.items())}
+
     out = {
         "metric": METRIC,
-        "value": round(value, 1),
+        "value": n_rt,
         "unit": "real-time channels",
         "n_gpus": 1,
         "steps": a.steps,
@@ -372,20 +528,23 @@ def main() -> int:
         "dtype": "f32",
         "data": "synthetic",
         "config": {"workload": "serving loop: uneven-mode extraction + demod at 20 Msps, 20 ms blocks",
-                   "sample_rate": FS, "block_samples": BLOCK, "channels": a.channels,
-                   "modes": "NFM/AM/USB/WFM (+ demod)", "decim": "8-1600", "taps": "63-511", "seed": 0},
+                   "sample_rate": FS, "block_samples": BLOCK, "channels": n_rt,
+                   "modes": "NFM/AM/USB/WFM (+ demod)", "decim": "8-1600", "taps": "63-511", "seed": 0,
+                   "p99_bound_ms": P99_BOUND_MS},
         "p50_block_ms": round(percentile(gpu, 50), 4),
         "p99_block_ms": round(percentile(gpu, 99), 4),
         "p50_kernels_ms": round(percentile(kern, 50), 4),
         "p99_kernels_ms": round(percentile(kern, 99), 4),
         "p50_block_ms_host_to_host": round(percentile(lat, 50), 3),
         "p99_block_ms_host_to_host": round(percentile(lat, 99), 3),
+        "throughput_channels": round(n_rt * BLOCK_S * 1e3 / ms_per_step, 1),
         "hbm_gbps": round(achieved, 1),
-        "plain_per_signal": {"ms_per_block": round(plain_ms, 2), "real_time_channels": round(plain_channels, 1),
-                             "fused_over_plain": round(value / plain_channels, 1)},
+        "capacity_sweep": table(sweep),
+        "plain_per_signal": {"real_time_channels": n_plain, "sweep": table(plain_sweep),
+                             "fused_over_plain": round(n_rt / max(n_plain, 1), 1)},
         "roofline": {"kernel": "argus_extract_batch_kernel", "bound": "hbm", "achieved": round(achieved, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": load_traffic(), "kernel_ms": round(kern_ms, 4), "alg_bytes": b_alg,
+                     "traffic": load_traffic(n_rt), "kernel_ms": round(kern_ms, 4), "alg_bytes": b_alg,
                      "alg_flops": f_alg, "valu_tflops": round(valu, 2),
                      "valu_frac": round(valu / FP32_PEAK_TFLOPS, 4)},
         "cpu_baseline": cpu,
