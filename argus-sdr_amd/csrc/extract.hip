@@ -16,126 +16,372 @@ __device__ __forceinline__ float2 argus_shifted(const ArgusChanDesc& ch, const f
     return h >= 0 ? ch.hist_rd[h] : make_float2(0.0f, 0.0f);
 }
 
-__device__ __forceinline__ float argus_wave_sum(float v)
+// v + (v moved by a DPP lane pattern); rows outside row_mask add 0.
+template <int CTRL, int ROW_MASK = 0xF>
+__device__ __forceinline__ float argus_dpp_add(float v)
 {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    return v + __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, ROW_MASK, 0xF, false));
+}
+
+// Sum over aligned groups of S lanes (S = 1, 2, 4, 8, 16) with DPP
+// (quad_perm [1,0,3,2], quad_perm [2,3,0,1], row_half_mirror, row_mirror);
+// every lane of a group receives the group's sum.
+template <int S>
+__device__ __forceinline__ float argus_group_sum(float v)
+{
+    if constexpr (S >= 2) v = argus_dpp_add<0xB1>(v);
+    if constexpr (S >= 4) v = argus_dpp_add<0x4E>(v);
+    if constexpr (S >= 8) v = argus_dpp_add<0x141>(v);
+    if constexpr (S >= 16) v = argus_dpp_add<0x140>(v);
     return v;
 }
 
-// Dense tile: M outputs starting at m0, samples staged in LDS by residue.
+// Sum over the wavefront; the total is valid in lane 63 (row_bcast15 into
+// rows 1 and 3, then row_bcast31 into rows 2 and 3).
+__device__ __forceinline__ float argus_wave_sum63(float v)
+{
+    v = argus_group_sum<16>(v);
+    v = argus_dpp_add<0x142, 0xA>(v);
+    v = argus_dpp_add<0x143, 0xC>(v);
+    return v;
+}
+
+// Sum over aligned groups of S lanes (power of two <= 64); the sum is valid
+// in the first lane of each group for S <= 16 (every lane), and in lane 63 of
+// the wave when S == 64.
+__device__ __forceinline__ float argus_lanes_sum(float v, int S)
+{
+    switch (S) {
+    case 1: return v;
+    case 2: return argus_group_sum<2>(v);
+    case 4: return argus_group_sum<4>(v);
+    case 8: return argus_group_sum<8>(v);
+    case 16: return argus_group_sum<16>(v);
+    default:
+        for (int o = S >> 1; o > 0; o >>= 1) v += __shfl_xor(v, o);
+        return v;
+    }
+}
+
+// Dense tile: M outputs starting at m0, samples staged in LDS by residue,
+// each plane as R interleaved sub-planes; each thread computes R consecutive
+// outputs over its share of the planes.
+template <int R>
 __device__ __forceinline__ void argus_dense_tile(const ArgusChanDesc& ch, const float2* __restrict__ in, int m0,
                                                  float2* lds_s)
 {
-    const int D = ch.D, T = ch.T, L = ch.L, P = ch.P, PS = ch.PS;
+    const int D = ch.D, T = ch.T, L = ch.L, P = ch.P, PS = ch.PS, SP = ch.SP;
     const int M = min(ch.M, ch.n_out - m0);
     const int em0 = ch.e0 + m0 * D;
     float* lds_t = (float*)(lds_s + P * PS);
     for (int i = threadIdx.x; i < P * L; i += blockDim.x) lds_t[i] = ch.taps[i];
 
     // Staging: sample n = n_lo + i for i in [0, span); d = em0 - n = T-1-i,
-    // residue p = d mod D, plane index j = (p - d) / D.  One division per
-    // thread, then stepped by blockDim without dividing again.
+    // residue p = d mod D, plane index j = (p - d) / D, plane position
+    // x = j + L - 1.  One division per thread, then stepped by blockDim
+    // without dividing again.  Loads are issued kBatch at a time; the NCO is
+    // evaluated exactly at the first sample of each batch and advanced across
+    // the batch by complex multiplication with e^{j inc B} (at most kBatch-1
+    // steps, so the rounding drift stays below 1e-6).  Tiles that lie wholly
+    // in the new input (all but the first of a chunk) take a path without
+    // history checks.
     const int span = (M - 1) * D + T;
     const int n_lo = em0 - (T - 1);
+    const int B = blockDim.x;
     int i = threadIdx.x;
     if (i < span) {
-        const int B = blockDim.x;
+        constexpr int kBatch = 8;
         const int stepq = B / D, stepr = B - stepq * D;
         int d = (T - 1) - i;
         int p = d % D;
         if (p < 0) p += D;
         int j = (p - d) / D;
+        int pbase = p * PS;  // p * PS, stepped with p
+        const int step_base = stepr * PS, wrap_base = D * PS;
         uint64_t ph = ch.phase0 + ch.inc * (uint64_t)(int64_t)(n_lo + i);
         const uint64_t phstep = ch.inc * (uint64_t)B;
-        for (; i < span; i += B) {
-            const int n = n_lo + i;
-            float2 v;
-            if (n >= 0) {
-                float sn, cs;
-                argus_nco(ph, sn, cs);
-                v = argus_rot(in[n], sn, cs);
-            } else {
-                const int h = ch.H + n;
-                v = h >= 0 ? ch.hist_rd[h] : make_float2(0.0f, 0.0f);
-            }
-            lds_s[p * PS + j + L - 1] = v;
+        float st_s, st_c;
+        argus_nco(phstep, st_s, st_c);
+        const bool all_new = n_lo >= 0;
+        auto put = [&](float2 x) {
+            const int xp = j + L - 1;
+            lds_s[pbase + (xp % R) * SP + xp / R] = x;
             ph += phstep;
             p -= stepr;
+            pbase -= step_base;
             j += stepq;
             if (p < 0) {
                 p += D;
+                pbase += wrap_base;
                 j += 1;
+            }
+            i += B;
+        };
+        if (all_new && D <= B) {
+            // Threads [0, Bp) with Bp a multiple of D: each thread stays in
+            // one plane p and its position x advances by Bp / D per sample.
+            const int Bp = (B / D) * D;
+            const int sq = Bp / D;
+            const float2* src = in + n_lo;
+            float2* plane = lds_s + pbase;
+            int x = j + L - 1;
+            uint64_t phb = ph;
+            const uint64_t phstep_b = ch.inc * (uint64_t)Bp;
+            float sb_s, sb_c;
+            argus_nco(phstep_b, sb_s, sb_c);
+            if (threadIdx.x >= Bp) i = span;
+            while (i + (kBatch - 1) * Bp < span) {
+                float2 v[kBatch];
+#pragma unroll
+                for (int u = 0; u < kBatch; u++) v[u] = src[i + u * Bp];
+                float sn, cs;
+                argus_nco(phb, sn, cs);
+#pragma unroll
+                for (int u = 0; u < kBatch; u++) {
+                    plane[__umul24((unsigned)(x % R), (unsigned)SP) + x / R] = argus_rot(v[u], sn, cs);
+                    x += sq;
+                    const float c2 = cs * sb_c - sn * sb_s;
+                    sn = fmaf(sn, sb_c, cs * sb_s);
+                    cs = c2;
+                }
+                i += kBatch * Bp;
+                phb += (uint64_t)kBatch * phstep_b;
+            }
+            for (; i < span; i += Bp) {
+                float sn, cs;
+                argus_nco(phb, sn, cs);
+                plane[__umul24((unsigned)(x % R), (unsigned)SP) + x / R] = argus_rot(src[i], sn, cs);
+                x += sq;
+                phb += phstep_b;
+            }
+        } else if (all_new) {
+            const float2* src = in + n_lo;
+            while (i + (kBatch - 1) * B < span) {
+                float2 v[kBatch];
+#pragma unroll
+                for (int u = 0; u < kBatch; u++) v[u] = src[i + u * B];
+                float sn, cs;
+                argus_nco(ph, sn, cs);
+#pragma unroll
+                for (int u = 0; u < kBatch; u++) {
+                    put(argus_rot(v[u], sn, cs));
+                    const float c2 = cs * st_c - sn * st_s;
+                    sn = fmaf(sn, st_c, cs * st_s);
+                    cs = c2;
+                }
+            }
+            for (; i < span;) {
+                float sn, cs;
+                argus_nco(ph, sn, cs);
+                put(argus_rot(src[i], sn, cs));
+            }
+        } else {
+            for (; i < span;) {
+                const int n = n_lo + i;
+                float2 x = make_float2(0.0f, 0.0f);
+                if (n >= 0) {
+                    float sn, cs;
+                    argus_nco(ph, sn, cs);
+                    x = argus_rot(in[n], sn, cs);
+                } else if (ch.H + n >= 0) {
+                    x = ch.hist_rd[ch.H + n];  // history samples are stored already shifted
+                }
+                put(x);
             }
         }
     }
     __syncthreads();
 
-    // FIR: output m = t / S, lanes g = t % S share its planes p = g, g+S, ...
+    // FIR: output group q = t / S (outputs qR .. qR+R-1), lanes g = t % S
+    // share its planes p = g, g+S, ...  The sample output qR+r needs at tap k
+    // sits at plane position x = qR + r - k + L - 1; walking v_i = position
+    // qR + (L + R - 2) - i, output r at tap k uses v_{R-1-r+k}.
     const int S = ch.S;
     const int t = threadIdx.x;
-    const int m = t / S;
+    const int q = t / S;
     const int g = t & (S - 1);
-    float ar0 = 0.0f, ai0 = 0.0f, ar1 = 0.0f, ai1 = 0.0f;
-    if (m < M) {
+    const int G = (M + R - 1) / R;
+    float ar[R], ai[R];
+#pragma unroll
+    for (int r = 0; r < R; r++) ar[r] = ai[r] = 0.0f;
+    if (q < G) {
         const int pfull = T - (L - 1) * D;  // planes p < pfull carry L taps, the rest L-1
+        const int c = L + R - 2;
         for (int p = g; p < P; p += S) {
             const int kmax = p < pfull ? L : L - 1;
             const float* tp = lds_t + p * L;
-            const float2* sp = lds_s + p * PS + m + L - 1;
-            int k = 0;
-            for (; k + 4 <= kmax; k += 4) {
-                const float h0 = tp[k], h1 = tp[k + 1], h2 = tp[k + 2], h3 = tp[k + 3];
-                const float2 v0 = sp[-k], v1 = sp[-k - 1], v2 = sp[-k - 2], v3 = sp[-k - 3];
-                ar0 = fmaf(h0, v0.x, ar0);
-                ai0 = fmaf(h0, v0.y, ai0);
-                ar1 = fmaf(h1, v1.x, ar1);
-                ai1 = fmaf(h1, v1.y, ai1);
-                ar0 = fmaf(h2, v2.x, ar0);
-                ai0 = fmaf(h2, v2.y, ai0);
-                ar1 = fmaf(h3, v3.x, ar1);
-                ai1 = fmaf(h3, v3.y, ai1);
+            const float2* pl = lds_s + p * PS + q;
+#define ARGUS_V(idx) pl[((c - (idx)) % R) * SP + (c - (idx)) / R]
+            if constexpr (R == 4) {
+                float2 a0 = ARGUS_V(0), a1 = ARGUS_V(1), a2 = ARGUS_V(2);
+                // v_{k+3+u} for k = 0 mod 4 sits at sub-plane (c-3-u) & 3,
+                // index ((c-3-u) >> 2) - k/4: one base pointer per u, so the
+                // loop carries no per-tap index arithmetic.  Blocks of 4 taps
+                // go two per iteration, each block's samples and taps loaded
+                // one block ahead (the prefetch index is clamped to the last
+                // block, so no load leaves the plane).
+                const float2* q0 = pl + ((c - 3) & 3) * SP + ((c - 3) >> 2);
+                const float2* q1 = pl + ((c - 4) & 3) * SP + ((c - 4) >> 2);
+                const float2* q2 = pl + ((c - 5) & 3) * SP + ((c - 5) >> 2);
+                const float2* q3 = pl + ((c - 6) & 3) * SP + ((c - 6) >> 2);
+                const int n4 = kmax >> 2;
+#define ARGUS_BLOCK4(A0, A1, A2, B0, B1, B2, B3, H)                                      \
+    ar[3] = fmaf(H.x, A0.x, ar[3]); ai[3] = fmaf(H.x, A0.y, ai[3]);                      \
+    ar[2] = fmaf(H.x, A1.x, ar[2]); ai[2] = fmaf(H.x, A1.y, ai[2]);                      \
+    ar[1] = fmaf(H.x, A2.x, ar[1]); ai[1] = fmaf(H.x, A2.y, ai[1]);                      \
+    ar[0] = fmaf(H.x, B0.x, ar[0]); ai[0] = fmaf(H.x, B0.y, ai[0]);                      \
+    ar[3] = fmaf(H.y, A1.x, ar[3]); ai[3] = fmaf(H.y, A1.y, ai[3]);                      \
+    ar[2] = fmaf(H.y, A2.x, ar[2]); ai[2] = fmaf(H.y, A2.y, ai[2]);                      \
+    ar[1] = fmaf(H.y, B0.x, ar[1]); ai[1] = fmaf(H.y, B0.y, ai[1]);                      \
+    ar[0] = fmaf(H.y, B1.x, ar[0]); ai[0] = fmaf(H.y, B1.y, ai[0]);                      \
+    ar[3] = fmaf(H.z, A2.x, ar[3]); ai[3] = fmaf(H.z, A2.y, ai[3]);                      \
+    ar[2] = fmaf(H.z, B0.x, ar[2]); ai[2] = fmaf(H.z, B0.y, ai[2]);                      \
+    ar[1] = fmaf(H.z, B1.x, ar[1]); ai[1] = fmaf(H.z, B1.y, ai[1]);                      \
+    ar[0] = fmaf(H.z, B2.x, ar[0]); ai[0] = fmaf(H.z, B2.y, ai[0]);                      \
+    ar[3] = fmaf(H.w, B0.x, ar[3]); ai[3] = fmaf(H.w, B0.y, ai[3]);                      \
+    ar[2] = fmaf(H.w, B1.x, ar[2]); ai[2] = fmaf(H.w, B1.y, ai[2]);                      \
+    ar[1] = fmaf(H.w, B2.x, ar[1]); ai[1] = fmaf(H.w, B2.y, ai[1]);                      \
+    ar[0] = fmaf(H.w, B3.x, ar[0]); ai[0] = fmaf(H.w, B3.y, ai[0]);
+                int k = 0;
+                if (n4 > 0) {
+                    float2 b0 = q0[0], b1 = q1[0], b2 = q2[0], b3 = q3[0];
+                    float4 hb = make_float4(tp[0], tp[1], tp[2], tp[3]);
+                    int k4 = 0;
+                    for (; k4 + 2 <= n4; k4 += 2) {
+                        const int kc = k4 + 1;  // block k4+1 exists
+                        const float2 c0 = q0[-kc], c1 = q1[-kc], c2 = q2[-kc], c3 = q3[-kc];
+                        const float4 hc = make_float4(tp[4 * kc], tp[4 * kc + 1], tp[4 * kc + 2], tp[4 * kc + 3]);
+                        ARGUS_BLOCK4(a0, a1, a2, b0, b1, b2, b3, hb)
+                        const int kn = min(k4 + 2, n4 - 1);  // next pair's first block (clamped)
+                        b0 = q0[-kn];
+                        const float2 n1 = q1[-kn], n2 = q2[-kn], n3 = q3[-kn];
+                        hb = make_float4(tp[4 * kn], tp[4 * kn + 1], tp[4 * kn + 2], tp[4 * kn + 3]);
+                        ARGUS_BLOCK4(b1, b2, b3, c0, c1, c2, c3, hc)
+                        a0 = c1;
+                        a1 = c2;
+                        a2 = c3;
+                        b1 = n1;
+                        b2 = n2;
+                        b3 = n3;
+                    }
+                    if (k4 < n4) {  // odd block count: the last block is loaded in b, hb
+                        ARGUS_BLOCK4(a0, a1, a2, b0, b1, b2, b3, hb)
+                        a0 = b1;
+                        a1 = b2;
+                        a2 = b3;
+                    }
+                    k = 4 * n4;
+                }
+#undef ARGUS_BLOCK4
+                for (; k < kmax; k++) {
+                    const float2 b0 = ARGUS_V(k + 3);
+                    const float h = tp[k];
+                    ar[3] = fmaf(h, a0.x, ar[3]); ai[3] = fmaf(h, a0.y, ai[3]);
+                    ar[2] = fmaf(h, a1.x, ar[2]); ai[2] = fmaf(h, a1.y, ai[2]);
+                    ar[1] = fmaf(h, a2.x, ar[1]); ai[1] = fmaf(h, a2.y, ai[1]);
+                    ar[0] = fmaf(h, b0.x, ar[0]); ai[0] = fmaf(h, b0.y, ai[0]);
+                    a0 = a1;
+                    a1 = a2;
+                    a2 = b0;
+                }
+            } else {
+                int k = 0;
+                for (; k + 2 <= kmax; k += 2) {
+                    const float2 v0 = ARGUS_V(k), v1 = ARGUS_V(k + 1);
+                    const float h0 = tp[k], h1 = tp[k + 1];
+                    ar[0] = fmaf(h0, v0.x, ar[0]);
+                    ai[0] = fmaf(h0, v0.y, ai[0]);
+                    ar[0] = fmaf(h1, v1.x, ar[0]);
+                    ai[0] = fmaf(h1, v1.y, ai[0]);
+                }
+                if (k < kmax) {
+                    const float2 v0 = ARGUS_V(k);
+                    ar[0] = fmaf(tp[k], v0.x, ar[0]);
+                    ai[0] = fmaf(tp[k], v0.y, ai[0]);
+                }
             }
-            for (; k < kmax; ++k) {
-                const float h = tp[k];
-                const float2 v = sp[-k];
-                ar0 = fmaf(h, v.x, ar0);
-                ai0 = fmaf(h, v.y, ai0);
-            }
+#undef ARGUS_V
         }
     }
-    float ar = ar0 + ar1, ai = ai0 + ai1;
-    for (int o = S >> 1; o > 0; o >>= 1) {
-        ar += __shfl_xor(ar, o);
-        ai += __shfl_xor(ai, o);
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+        ar[r] = argus_lanes_sum(ar[r], S);
+        ai[r] = argus_lanes_sum(ai[r], S);
     }
-    if (m < M && g == 0) ch.out[m0 + m] = make_float2(ar, ai);
+    if (q < G && g == 0) {
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+            const int m = q * R + r;
+            if (m < M) ch.out[m0 + m] = make_float2(ar[r], ai[r]);
+        }
+    }
     __syncthreads();
 }
 
-// Narrow group: one wavefront per output, lanes over taps, no staging.
+// Narrow group: one wavefront per output, lanes over taps, no staging.  Each
+// lane issues its loads four at a time (taps k, k+64, k+128, k+192) before
+// rotating them; items whose windows lie wholly in the new input skip the
+// history checks.  The wave sum lands in lane 63, which writes the output.
 __device__ __forceinline__ void argus_narrow_group(const ArgusChanDesc& ch, const float2* __restrict__ in, int m0,
                                                    float* lds_t)
 {
-    const int T = ch.T, D = ch.D;
-    for (int i = threadIdx.x; i < T; i += blockDim.x) lds_t[i] = ch.taps[i];  // L == 1: taps[k] = h[k]
+    const int T = ch.T, D = ch.D, L = ch.L;
+    // natural tap order from the phase-major layout: h[k] = taps[(k mod D) L + k div D]
+    for (int i = threadIdx.x; i < T; i += blockDim.x) lds_t[i] = L == 1 ? ch.taps[i] : ch.taps[(i % D) * L + i / D];
     __syncthreads();
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
     const int nw = blockDim.x >> 6;
     const int mend = min(m0 + ch.M, ch.n_out);
+    const uint64_t inc64 = ch.inc * 64u;
+    const bool all_new = ch.e0 + m0 * D - (T - 1) >= 0;
     for (int m = m0 + wave; m < mend; m += nw) {
         const int e = ch.e0 + m * D;
         float ar = 0.0f, ai = 0.0f;
-        for (int k = lane; k < T; k += 64) {
-            const float2 s = argus_shifted(ch, in, e - k);
-            const float h = lds_t[k];
-            ar = fmaf(h, s.x, ar);
-            ai = fmaf(h, s.y, ai);
+        for (int k0 = 0; k0 < T; k0 += 256) {
+            float2 x[4];
+            if (all_new) {
+                const float2* src = in + (e - k0 - lane);
+#pragma unroll
+                for (int u = 0; u < 4; u++) x[u] = k0 + lane + 64 * u < T ? src[-64 * u] : make_float2(0.0f, 0.0f);
+            } else {
+#pragma unroll
+                for (int u = 0; u < 4; u++) {
+                    const int k = k0 + lane + 64 * u;
+                    const int n = e - k;
+                    float2 v = make_float2(0.0f, 0.0f);
+                    if (k < T) {
+                        if (n >= 0) {
+                            v = in[n];
+                        } else {
+                            const int h = ch.H + n;
+                            if (h >= 0) v = ch.hist_rd[h];
+                        }
+                    }
+                    x[u] = v;
+                }
+            }
+            uint64_t ph = ch.phase0 + ch.inc * (uint64_t)(int64_t)(e - k0 - lane);
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                const int k = k0 + lane + 64 * u;
+                if (k < T) {
+                    float2 v = x[u];
+                    if (all_new || e - k >= 0) {  // history samples are stored already shifted
+                        float sn, cs;
+                        argus_nco(ph, sn, cs);
+                        v = argus_rot(v, sn, cs);
+                    }
+                    const float h = lds_t[k];
+                    ar = fmaf(h, v.x, ar);
+                    ai = fmaf(h, v.y, ai);
+                }
+                ph -= inc64;
+            }
         }
-        ar = argus_wave_sum(ar);
-        ai = argus_wave_sum(ai);
-        if (lane == 0) ch.out[m] = make_float2(ar, ai);
+        ar = argus_wave_sum63(ar);
+        ai = argus_wave_sum63(ai);
+        if (lane == 63) ch.out[m] = make_float2(ar, ai);
     }
     __syncthreads();
 }
@@ -149,8 +395,12 @@ __device__ __forceinline__ void argus_history_tail(const ArgusChanDesc& ch, cons
 __device__ __forceinline__ void argus_run_item(const ArgusChanDesc& ch, int kind, int m0,
                                                const float2* __restrict__ in, int n_new, float2* lds)
 {
-    if (kind == ARGUS_ITEM_DENSE)
-        argus_dense_tile(ch, in, m0, lds);
+    if (kind == ARGUS_ITEM_DENSE) {
+        if (ch.R == 4)
+            argus_dense_tile<4>(ch, in, m0, lds);
+        else
+            argus_dense_tile<1>(ch, in, m0, lds);
+    }
     else if (kind == ARGUS_ITEM_NARROW)
         argus_narrow_group(ch, in, m0, (float*)lds);
     else if (kind == ARGUS_ITEM_HISTORY)

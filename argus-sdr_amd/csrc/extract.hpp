@@ -11,12 +11,19 @@
 //    (the reference's phase-major order, streaming_host_core.go:29-52).
 //    When M < 256 the planes of one output are split over S lanes and the
 //    partial sums are combined with lane shuffles, so every thread works.
+//    With register blocking (R = 4) a thread computes R consecutive outputs
+//    from a sliding window of R samples, so each LDS read feeds R complex
+//    MACs; each plane is then stored as R interleaved sub-planes (position x
+//    at sub-plane x mod R, index x div R) so the lanes of a read still touch
+//    consecutive words.
 //
 //  narrow group (D >= T): each output needs only the T samples before its
 //    emission point and windows do not overlap, so a wavefront computes one
 //    output at a time: lanes take taps k = lane, lane+64, ..., shift their
 //    sample with the NCO straight from global memory (coalesced) and the
-//    wave reduces.  No LDS staging, no redundant rotations.
+//    wave reduces.  No LDS staging, no redundant rotations.  Also the
+//    fallback when a dense tile does not fit the LDS budget (very large D
+//    with T > D), at T/D redundant rotations.
 //
 // A third kind writes the channel's new shifted history (the newest `keep`
 // samples of history ++ shifted input) into the other half of a ping-pong
@@ -38,7 +45,9 @@ struct ArgusChanDesc {
     int keep;               // new history length
     int M;                  // outputs per item
     int PS;                 // LDS plane stride (complex samples), dense only
-    int S;                  // lanes per output in the dense FIR phase (power of 2)
+    int SP;                 // LDS sub-plane stride (complex samples), dense only
+    int R;                  // outputs per thread in the dense FIR phase (1 or 4)
+    int S;                  // lanes per output group in the dense FIR phase (power of 2)
     int narrow;             // 1 = narrow (wave-per-output) items
 };
 
@@ -59,29 +68,70 @@ enum { ARGUS_ITEM_DENSE = 0, ARGUS_ITEM_HISTORY = 1, ARGUS_ITEM_NARROW = 2, ARGU
 #define ARGUS_MAX_TAPS 4095
 #define ARGUS_NARROW_OUTPUTS 32    // outputs per narrow item (8 per wave)
 
-// Host-side planning for one channel.  Fills M, PS, S, narrow.  Returns 0,
-// or -1 if the geometry cannot be staged.
+#define ARGUS_DENSE_R 4             // register blocking of the dense FIR
+
+// Plane stride: at least R*SP, and congruent mod 16 (complex words) to 16/S
+// so the 16 lanes of one LDS read phase (S planes x 16/S output groups)
+// touch distinct bank pairs.
+static inline int argus_plane_stride(int R, int SP, int S)
+{
+    const int want = S >= 16 || S == 1 ? 1 : 16 / S;
+    int PS = R * SP;
+    while ((PS & 15) != want) PS++;
+    return PS;
+}
+
+// Dense geometry with R outputs per thread within `entries` complex words of
+// LDS.  Returns 0, or -1 if not even one output group fits.
+static inline int argus_plan_dense(ArgusChanDesc* ch, int R, int entries)
+{
+    const int P = ch->P, L = ch->L;
+    const int tail = (L + R - 2) / R;  // sub-plane words beyond the G groups
+    int G = (entries / P) / R - tail;
+    const int need = ch->n_out > 0 ? (ch->n_out + R - 1) / R : 1;
+    if (G > ARGUS_EXTRACT_THREADS) G = ARGUS_EXTRACT_THREADS;
+    if (G > need) G = need;
+    for (; G > 0; G--) {
+        int S = 1;
+        while (S * 2 * G <= ARGUS_EXTRACT_THREADS && S * 2 <= P && S < 64) S *= 2;
+        int PS = argus_plane_stride(R, G + tail, S);
+        if (P * PS > entries) PS = R * (G + tail);  // no room for bank padding
+        if (P * PS <= entries) {
+            ch->narrow = 0;
+            ch->R = R;
+            ch->M = G * R;
+            ch->SP = G + tail;
+            ch->PS = PS;
+            ch->S = S;
+            return 0;
+        }
+    }
+    return -1;
+}
+
+// Host-side planning for one channel.  Fills M, PS, SP, R, S, narrow.
+// Dense tiles when the windows of consecutive outputs overlap (D < T) and a
+// tile fits the LDS budget, else narrow items (any D and T).
 static inline int argus_plan_chan(ArgusChanDesc* ch)
 {
-    const int D = ch->D, T = ch->T, L = ch->L, P = ch->P;
-    if (D >= T) {
-        ch->narrow = 1;
-        ch->M = ARGUS_NARROW_OUTPUTS;
-        ch->PS = 0;
-        ch->S = 1;
-        return 0;
+    const int D = ch->D, T = ch->T;
+    if (D < T) {
+        ArgusChanDesc c = *ch;
+        if (argus_plan_dense(&c, ARGUS_DENSE_R, ARGUS_LDS_ENTRIES) == 0 && c.M / ARGUS_DENSE_R >= 4) {
+            *ch = c;
+            return 0;
+        }
+        c = *ch;
+        if (argus_plan_dense(&c, 1, ARGUS_LDS_ENTRIES) == 0) {
+            *ch = c;
+            return 0;
+        }
     }
-    ch->narrow = 0;
-    int M = ARGUS_LDS_ENTRIES / P - (L - 1);
-    if (M > ARGUS_EXTRACT_THREADS) M = ARGUS_EXTRACT_THREADS;
-    if (ch->n_out > 0 && M > ch->n_out) M = ch->n_out;
-    while (M > 0 && P * ((M + L - 1) | 1) > ARGUS_LDS_ENTRIES) M--;
-    if (M <= 0) return -1;
-    ch->M = M;
-    ch->PS = (M + L - 1) | 1;  // odd stride spreads planes over banks
-    int S = 1;
-    while (S * 2 * M <= ARGUS_EXTRACT_THREADS && S * 2 <= P && S < 64) S *= 2;
-    ch->S = S;
+    ch->narrow = 1;
+    ch->M = ARGUS_NARROW_OUTPUTS;
+    ch->PS = ch->SP = 0;
+    ch->R = 1;
+    ch->S = 1;
     return 0;
 }
 

@@ -131,6 +131,35 @@ def test_uneven_mix_long_stream(gpu):
     r.Close()
 
 
+def test_tile_geometries(gpu):
+    """Channels that take each tile shape: register-blocked dense (D=2, 80),
+    many-plane dense with one output per thread (D=400 < T=511), the narrow
+    fallback for a dense tile that does not fit the LDS budget (D=3000 <
+    T=3301), no decimation (D=1) and narrow windows (D >= T)."""
+    sr = 12_000_000
+    shapes = [(2, 63), (80, 301), (400, 511), (3000, 3301), (1, 101), (6000, 1001), (8, 9)]
+    jobs = [J(SignalID=200 + i, OffsetHz=-1.1e6 + 0.4e6 * i, Bandwidth=sr / d / 2, OutRate=sr // d, NumTaps=t,
+              ConfigHash=5000 + i) for i, (d, t) in enumerate(shapes)]
+    r = gpu.NewBatchRunner(70000, sr)
+    orc = O.CPUOracleRunner(sr, threads=8)
+    rng = np.random.default_rng(3)
+    acc_g = {j.SignalID: [] for j in jobs}
+    acc_w = {j.SignalID: [] for j in jobs}
+    for k, n in enumerate([5000, 65536, 12345, 1]):
+        iq = (rng.standard_normal(n) + 1j * rng.standard_normal(n)).astype(np.complex64)
+        print(f"geometry step {k} n={n}", flush=True)
+        got = r.StreamingExtractGPU(iq, jobs)
+        want = orc.StreamingExtract(iq, jobs)
+        for g, w in zip(got, want):
+            assert g.NOut == w.NOut and g.PhaseCount == w.PhaseCount and g.HistoryLen == w.HistoryLen
+            acc_g[g.SignalID].append(g.IQ)
+            acc_w[w.SignalID].append(w.IQ)
+    for (d, t), sid in zip(shapes, acc_g):
+        e = rel_rms(np.concatenate(acc_g[sid]), np.concatenate(acc_w[sid]))
+        assert e <= 1e-4, (d, t, e)
+    r.Close()
+
+
 def test_stateful_entry_point_direct(gpu):
     """gpud_launch_streaming_polyphase_stateful_hip against the host core,
     including its argument-error contract (exports.cu:552-606)."""
