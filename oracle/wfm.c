@@ -128,19 +128,20 @@ int orc_stereo_decode(orc_stereo_state* st, const float* mono, int n, float* out
 
 /* ------------------------------------------------------------ pilot presence */
 
-/* internal/classifier/pll.go:178-193 goertzelMagnitude */
-static double goertzel(const float* x, int n, double freq, int sr)
+/* internal/classifier/pll.go:178-193 goertzelMagnitude (FP64 samples) */
+static double goertzel(const double* x, int n, double freq, int sr)
 {
-    double w = 2 * ORC_PI * freq / (double)sr;
+    if (n == 0) return 0;
+    double k = freq / ((double)sr / (double)n);
+    double w = 2 * ORC_PI * k / (double)n;
     double c = 2 * cos(w);
     double s1 = 0, s2 = 0;
     for (int i = 0; i < n; i++) {
-        double s0 = (double)x[i] + c * s1 - s2;
+        double s0 = x[i] + c * s1 - s2;
         s2 = s1;
         s1 = s0;
     }
-    double p = s1 * s1 + s2 * s2 - c * s1 * s2;
-    return p < 0 ? 0 : sqrt(p);
+    return sqrt(s1 * s1 + s2 * s2 - c * s1 * s2);
 }
 
 static int cmp_d(const void* a, const void* b)
@@ -149,33 +150,39 @@ static int cmp_d(const void* a, const void* b)
     return (x > y) - (x < y);
 }
 
-/* internal/classifier/pll.go:82-131 StereoPilotPresent/estimateWFMPilot on
- * the FM discriminator of the last <= 0.5 s of iq: Goertzel scan of
- * 19 kHz +- 1 kHz in 10 Hz steps; present if max/median > 4. */
+/* internal/classifier/pll.go:82-131 StereoPilotPresent/estimateWFMPilot: the
+ * FP64 FM discriminator (fmDemod, pll.go:160-176) of the last <= fs/2
+ * samples, Goertzel magnitudes at 19 kHz +- 1 kHz in 10 Hz steps, present if
+ * max / median > 4 (pilotLockRatio); rates below 40 kHz are never stereo. */
 int orc_stereo_pilot_present(const float* iq, int n, int sample_rate, double* ratio_out)
 {
     if (ratio_out) *ratio_out = 0;
-    if (n < 2 || sample_rate <= 0) return 0;
+    if (sample_rate <= 0) return 0;
     int maxn = sample_rate / 2;
-    if (n > maxn) {
+    if (maxn > 0 && n > maxn) {
         iq += 2 * (size_t)(n - maxn);
         n = maxn;
     }
-    float* d = (float*)malloc(sizeof(float) * (size_t)n);
-    int m = orc_fm_discrim(iq, n, d);
+    if (sample_rate < 40000 || n < 2) return 0;
+    double* d = (double*)malloc(sizeof(double) * (size_t)(n - 1));
+    for (int i = 1; i < n; i++) {
+        double pr = iq[2 * (i - 1)], pi = iq[2 * (i - 1) + 1];
+        double cr = iq[2 * i], ci = iq[2 * i + 1];
+        d[i - 1] = atan2(pr * ci - pi * cr, pr * cr + pi * ci);
+    }
     double mags[201];
     double best = 0;
-    for (int k = 0; k < 201; k++) {
-        double f = 18000 + 10.0 * k;
-        mags[k] = goertzel(d, m, f, sample_rate);
+    int k = 0;
+    for (double off = -1000.0; off <= 1000.0; off += 10.0, k++) {
+        mags[k] = goertzel(d, n - 1, 19000.0 + off, sample_rate);
         if (mags[k] > best) best = mags[k];
     }
-    qsort(mags, 201, sizeof(double), cmp_d);
-    double med = mags[100];
+    qsort(mags, (size_t)k, sizeof(double), cmp_d);
+    double med = mags[k / 2];
     free(d);
     double ratio = med > 0 ? best / med : 0;
     if (ratio_out) *ratio_out = ratio;
-    return ratio > 4;
+    return med > 0 && ratio > 4;
 }
 
 /* ------------------------------------------------------------ RDS decode */

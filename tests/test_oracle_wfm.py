@@ -209,3 +209,33 @@ def test_station_chain_recovers_rds():
             d.Decode(np.concatenate(win), s.rate)
             win = []
     assert d.PI == 0x2F1A and d.PS == "STATION7"
+
+
+def test_stereo_pilot_present_on_station_absent_on_mono():
+    """classifier/pll.go:82-131: a stereo station's discriminator shows the
+    19 kHz pilot (max/median of the 201 Goertzel bins far above 4); a mono FM
+    carrier (1 kHz tone, 75 kHz deviation) and noise do not."""
+    from argus.synth import WFMStation, rds_symbols
+    fs = 250_000
+    st = WFMStation(fs, 0.0, rds_symbols(np.zeros(800, np.uint8)), seed=3)
+    ok, ratio = O.StereoPilotPresent(st.block(fs // 2), fs)
+    assert ok and ratio > 20, ratio
+    t = np.arange(fs // 2) / fs
+    mono = np.exp(1j * (75000 / 1000) * np.sin(2 * np.pi * 1000 * t)).astype(np.complex64)
+    ok, ratio = O.StereoPilotPresent(mono, fs)
+    assert not ok, ratio
+    rng = np.random.default_rng(4)
+    noise = (rng.standard_normal(fs // 2) + 1j * rng.standard_normal(fs // 2)).astype(np.complex64)
+    assert not O.StereoPilotPresent(noise, fs)[0]
+
+
+def test_stereo_pilot_uses_the_last_half_second():
+    """Only the last 0.5 s of IQ is examined (pll.go:90-96): a pilot in the
+    first second followed by 0.5 s of mono is not detected."""
+    from argus.synth import WFMStation, rds_symbols
+    fs = 250_000
+    st = WFMStation(fs, 0.0, rds_symbols(np.zeros(2000, np.uint8)), seed=5)
+    t = np.arange(fs // 2) / fs
+    mono = np.exp(1j * 75 * np.sin(2 * np.pi * 1000 * t)).astype(np.complex64)
+    assert O.StereoPilotPresent(st.block(fs), fs)[0]
+    assert not O.StereoPilotPresent(np.concatenate([st.block(fs), mono]), fs)[0]
