@@ -42,6 +42,17 @@ register("gpud_serve_set_download", I, [P, I])
 register("gpud_serve_wait", I, [P, C.c_int64, C.POINTER(gpud_serve_block_info)])
 register("gpud_serve_channel_output", I, [P, C.c_int64, C.c_int64, C.POINTER(P), C.POINTER(I), C.POINTER(P),
                                           C.POINTER(I), C.POINTER(I)])
+class gpud_tune(C.Structure):
+    _fields_ = [("lds_entries", C.c_int32), ("narrow_outputs", C.c_int32), ("grid", C.c_int32), ("pad", C.c_int32),
+                ("kernel_ms", C.c_float), ("pad2", C.c_float)]
+
+
+register("gpud_serve_set_graph", I, [P, I])
+register("gpud_serve_get_tune", I, [P, C.POINTER(gpud_tune)])
+register("gpud_serve_set_tune", I, [P, C.POINTER(gpud_tune)])
+register("gpud_serve_autotune", I, [P, P, I, I, C.POINTER(gpud_tune)])
+register("gpud_runner_get_tune", I, [P, C.POINTER(gpud_tune)])
+register("gpud_runner_set_tune", I, [P, C.POINTER(gpud_tune)])
 register("gpud_serve_channel_wfm", I, [P, C.c_int64, C.c_int64, C.POINTER(P), C.POINTER(I), C.POINTER(I),
                                        C.POINTER(I), C.POINTER(I)])
 
@@ -147,3 +158,20 @@ class Serve:
         rds = np.ctypeslib.as_array((C.c_float * (2 * n.value)).from_address(p.value)).view(np.complex64).copy() \
             if n.value else np.zeros(0, np.complex64)
         return rds, rate.value, bool(locked.value), bool(stereo.value)
+
+    def SetGraph(self, enable: bool) -> None:
+        check("gpud_serve_set_graph", lib().gpud_serve_set_graph(self._h, 1 if enable else 0))
+
+    def Tune(self) -> gpud_tune:
+        t = gpud_tune()
+        check("gpud_serve_get_tune", lib().gpud_serve_get_tune(self._h, C.byref(t)))
+        return t
+
+    def SetTune(self, lds_entries: int, narrow_outputs: int, grid: int) -> None:
+        t = gpud_tune(lds_entries, narrow_outputs, grid, 0, 0.0, 0.0)
+        check("gpud_serve_set_tune", lib().gpud_serve_set_tune(self._h, C.byref(t)))
+
+    def Autotune(self, dev_ptr: int, n: int, reps: int = 3) -> gpud_tune:
+        t = gpud_tune()
+        check("gpud_serve_autotune", lib().gpud_serve_autotune(self._h, dev_ptr, n, reps, C.byref(t)))
+        return t

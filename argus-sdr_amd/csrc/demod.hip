@@ -42,8 +42,8 @@ __device__ void am_block(const ArgusDemodDesc& d)
 
 }  // namespace
 
-__global__ __launch_bounds__(256) void argus_demod_kernel(const ArgusDemodDesc* __restrict__ descs,
-                                                          const ArgusDemodItem* __restrict__ items, int n_items)
+__device__ __forceinline__ void argus_demod_items(const ArgusDemodDesc* __restrict__ descs,
+                                                  const ArgusDemodItem* __restrict__ items, int n_items)
 {
     for (int w = blockIdx.x; w < n_items; w += gridDim.x) {
         const ArgusDemodItem it = items[w];
@@ -84,6 +84,18 @@ __global__ __launch_bounds__(256) void argus_demod_kernel(const ArgusDemodDesc* 
     }
 }
 
+__global__ __launch_bounds__(256) void argus_demod_kernel(const ArgusDemodDesc* __restrict__ descs,
+                                                          const ArgusDemodItem* __restrict__ items, int n_items)
+{
+    argus_demod_items(descs, items, n_items);
+}
+
+__global__ __launch_bounds__(256) void argus_demod_hdr_kernel(const ArgusDemodHdr* __restrict__ hdr)
+{
+    const ArgusDemodHdr h = *hdr;
+    argus_demod_items(h.descs, h.items, h.n_items);
+}
+
 int argus_launch_demod(const ArgusDemodDesc* d_desc, const ArgusDemodItem* d_items, int n_items,
                        hipStream_t stream)
 {
@@ -91,5 +103,12 @@ int argus_launch_demod(const ArgusDemodDesc* d_desc, const ArgusDemodItem* d_ite
     const int grid = n_items < 2048 ? n_items : 2048;
     argus_clear_error();
     hipLaunchKernelGGL(argus_demod_kernel, dim3(grid), dim3(256), 0, stream, d_desc, d_items, n_items);
+    return (int)hipGetLastError();
+}
+
+int argus_launch_demod_hdr(const ArgusDemodHdr* d_hdr, hipStream_t stream)
+{
+    argus_clear_error();
+    hipLaunchKernelGGL(argus_demod_hdr_kernel, dim3(ARGUS_DEMOD_GRID), dim3(256), 0, stream, d_hdr);
     return (int)hipGetLastError();
 }

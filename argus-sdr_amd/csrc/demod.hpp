@@ -44,5 +44,16 @@ static inline int argus_demod_count(int mode, int n, int have_carry)
     return n;
 }
 
+struct ArgusDemodHdr {
+    const ArgusDemodDesc* descs;
+    const ArgusDemodItem* items;
+    int n_items;
+    int pad;
+};
+
+#define ARGUS_DEMOD_GRID 2048
+
 int argus_launch_demod(const ArgusDemodDesc* d_desc, const ArgusDemodItem* d_items, int n_items,
                        hipStream_t stream);
+// Graph-mode launch: counts and pointers come from *hdr (device memory).
+int argus_launch_demod_hdr(const ArgusDemodHdr* d_hdr, hipStream_t stream);

@@ -408,16 +408,32 @@ __device__ __forceinline__ void argus_run_item(const ArgusChanDesc& ch, int kind
 }
 
 // Batched, persistent: every work item of every channel of one chunk.
-__global__ __launch_bounds__(ARGUS_EXTRACT_THREADS) void argus_extract_batch_kernel(
-    const ArgusChanDesc* __restrict__ chans, const ArgusWorkItem* __restrict__ items, int n_items,
-    const float2* __restrict__ in, int n_new)
+__device__ __forceinline__ void argus_extract_items(const ArgusChanDesc* __restrict__ chans,
+                                                    const ArgusWorkItem* __restrict__ items, int n_items,
+                                                    const float2* __restrict__ in, int n_new, float2* lds)
 {
-    extern __shared__ float2 lds[];
     for (int w = blockIdx.x; w < n_items; w += gridDim.x) {
         const ArgusWorkItem it = items[w];
         const ArgusChanDesc ch = chans[it.chan];
         argus_run_item(ch, it.kind, it.m0, in, n_new, lds);
     }
+}
+
+__global__ __launch_bounds__(ARGUS_EXTRACT_THREADS) void argus_extract_batch_kernel(
+    const ArgusChanDesc* __restrict__ chans, const ArgusWorkItem* __restrict__ items, int n_items,
+    const float2* __restrict__ in, int n_new)
+{
+    extern __shared__ float2 lds[];
+    argus_extract_items(chans, items, n_items, in, n_new, lds);
+}
+
+// The same launch with its arguments read from device memory, so a captured
+// graph replays it unchanged while the block's tables change.
+__global__ __launch_bounds__(ARGUS_EXTRACT_THREADS) void argus_extract_hdr_kernel(const ArgusExtractHdr* __restrict__ hdr)
+{
+    extern __shared__ float2 lds[];
+    const ArgusExtractHdr h = *hdr;
+    argus_extract_items(h.chans, h.items, h.n_items, h.in, h.n_new, lds);
 }
 
 // One signal: blocks [0, n_items) are output items, block n_items (if
@@ -437,6 +453,25 @@ size_t argus_item_lds_bytes(const ArgusChanDesc& ch)
     return ch.narrow ? (size_t)ch.T * sizeof(float) : argus_dense_lds_bytes(ch);
 }
 
+// Dynamic LDS above 64 KiB must be allowed per kernel once.
+static int argus_allow_big_lds()
+{
+    static int done = 0;
+    if (done) return 0;
+    const int bytes = 160 * 1024;
+    hipError_t e = hipFuncSetAttribute((const void*)argus_extract_batch_kernel,
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+    if (e == hipSuccess)
+        e = hipFuncSetAttribute((const void*)argus_extract_hdr_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                bytes);
+    if (e == hipSuccess)
+        e = hipFuncSetAttribute((const void*)argus_extract_one_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                bytes);
+    if (e != hipSuccess) return (int)e;
+    done = 1;
+    return 0;
+}
+
 // Host: launch the single-signal kernel for a fully described channel.
 int argus_launch_extract_one(const ArgusChanDesc& ch, const float2* in, int n_new, hipStream_t stream)
 {
@@ -444,20 +479,47 @@ int argus_launch_extract_one(const ArgusChanDesc& ch, const float2* in, int n_ne
     const int blocks = n_items + (ch.keep > 0 ? 1 : 0);
     if (blocks == 0) return 0;
     const size_t lds = n_items > 0 ? argus_item_lds_bytes(ch) : 0;
+    if (lds > 64 * 1024) {
+        int e = argus_allow_big_lds();
+        if (e) return e;
+    }
     argus_clear_error();
     hipLaunchKernelGGL(argus_extract_one_kernel, dim3(blocks), dim3(ARGUS_EXTRACT_THREADS), lds, stream, ch, in,
                        n_new, n_items);
     return (int)hipGetLastError();
 }
 
+int argus_extract_allow_lds(size_t bytes) { return bytes > 64 * 1024 ? argus_allow_big_lds() : 0; }
+
+size_t argus_tune_max_lds(const ArgusTune& tune, int max_taps)
+{
+    // dense: P*PS <= lds_entries words of samples plus P*L <= T + P - 1 < 2*T taps;
+    // narrow: T taps
+    return (size_t)tune.lds_entries * sizeof(float2) + (size_t)2 * max_taps * sizeof(float);
+}
+
+int argus_launch_extract_hdr(const ArgusExtractHdr* d_hdr, size_t lds_bytes, int grid, hipStream_t stream)
+{
+    if (lds_bytes > 64 * 1024) {
+        int e = argus_allow_big_lds();
+        if (e) return e;
+    }
+    argus_clear_error();
+    hipLaunchKernelGGL(argus_extract_hdr_kernel, dim3(grid), dim3(ARGUS_EXTRACT_THREADS), lds_bytes, stream, d_hdr);
+    return (int)hipGetLastError();
+}
+
 int argus_launch_extract_batch(const ArgusChanDesc* d_chans, const ArgusWorkItem* d_items, int n_items,
-                               size_t lds_bytes, const float2* in, int n_new, hipStream_t stream)
+                               size_t lds_bytes, const float2* in, int n_new, int grid_max, hipStream_t stream)
 {
     if (n_items <= 0) return 0;
     // Persistent grid: enough blocks to fill every CU several times over; the
     // item list is ordered heaviest first so the tail is short.
-    const int max_grid = 256 * 8;
-    const int grid = n_items < max_grid ? n_items : max_grid;
+    const int grid = n_items < grid_max ? n_items : grid_max;
+    if (lds_bytes > 64 * 1024) {
+        int e = argus_allow_big_lds();
+        if (e) return e;
+    }
     argus_clear_error();
     hipLaunchKernelGGL(argus_extract_batch_kernel, dim3(grid), dim3(ARGUS_EXTRACT_THREADS), lds_bytes, stream,
                        d_chans, d_items, n_items, in, n_new);
@@ -531,7 +593,8 @@ void argus_order_items(std::vector<ArgusWorkItem>& items, const std::vector<int>
 // Describe one signal for the kernels.  Returns 0 or -1 if the geometry
 // cannot be staged (T too large).
 int argus_fill_chan(ArgusChanDesc* ch, const float2* hist_rd, int H, float2* hist_wr, int keep, const float* taps,
-                    float2* out, int D, int T, int phase_count, int n_new, double phase_start, double phase_inc)
+                    float2* out, int D, int T, int phase_count, int n_new, double phase_start, double phase_inc,
+                    const ArgusTune& tune)
 {
     if (T > ARGUS_MAX_TAPS) return -1;
     ch->hist_rd = hist_rd;
@@ -548,7 +611,7 @@ int argus_fill_chan(ArgusChanDesc* ch, const float2* hist_rd, int H, float2* his
     ch->e0 = D - phase_count - 1;
     ch->n_out = (phase_count + n_new) / D;
     ch->keep = keep;
-    return argus_plan_chan(ch);
+    return argus_plan_chan(ch, tune);
 }
 
 extern "C" {
@@ -583,7 +646,7 @@ GPUD_API int gpud_launch_streaming_polyphase_stateful_hip(
 
     ArgusChanDesc ch;
     if (argus_fill_chan(&ch, (const float2*)history_state, H, (float2*)history_scratch, new_h, polyphase_taps,
-                        (float2*)out, decim, num_taps, pc, n_new, phase_start, phase_inc) != 0)
+                        (float2*)out, decim, num_taps, pc, n_new, phase_start, phase_inc, argus_default_tune()) != 0)
         return -12;
     int err = argus_launch_extract_one(ch, (const float2*)in_new, n_new, 0);
     if (err) return err;
@@ -622,7 +685,7 @@ GPUD_API int gpud_launch_streaming_polyphase_prepare_hip(
     const int new_h = (history_out && keep > 0) ? std::min(H + n_new, keep) : 0;
     ArgusChanDesc ch;
     if (argus_fill_chan(&ch, (const float2*)history_in, H, (float2*)history_out, new_h, polyphase_taps,
-                        (float2*)out, decim, num_taps, pc, n_new, phase_start, phase_inc) != 0)
+                        (float2*)out, decim, num_taps, pc, n_new, phase_start, phase_inc, argus_default_tune()) != 0)
         return -2;
     int err = argus_launch_extract_one(ch, (const float2*)in_new, n_new, 0);
     if (err) return err;

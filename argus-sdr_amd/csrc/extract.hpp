@@ -30,6 +30,7 @@
 // pair, so no item reads history another item is overwriting.
 #pragma once
 #include "common.hpp"
+#include <vector>
 
 struct ArgusChanDesc {
     const float2* hist_rd;  // old shifted history, H samples, oldest first
@@ -51,6 +52,17 @@ struct ArgusChanDesc {
     int narrow;             // 1 = narrow (wave-per-output) items
 };
 
+// Everything the graph-mode extraction kernel reads for one block (lives in
+// device memory so a captured launch can be replayed with new contents).
+struct ArgusWorkItem;
+struct ArgusExtractHdr {
+    const ArgusChanDesc* chans;
+    const ArgusWorkItem* items;
+    const float2* in;
+    int n_items;
+    int n_new;
+};
+
 struct ArgusWorkItem {
     int chan;
     int m0;    // first output of the item
@@ -64,9 +76,21 @@ enum { ARGUS_ITEM_DENSE = 0, ARGUS_ITEM_HISTORY = 1, ARGUS_ITEM_NARROW = 2, ARGU
 #define ARGUS_XCDS 8
 
 #define ARGUS_EXTRACT_THREADS 256
-#define ARGUS_LDS_ENTRIES 6144     // complex samples of LDS per dense tile (48 KiB)
+#define ARGUS_LDS_ENTRIES 6144     // default complex samples of LDS per dense tile (48 KiB)
+#define ARGUS_MAX_LDS_ENTRIES 16384
 #define ARGUS_MAX_TAPS 4095
-#define ARGUS_NARROW_OUTPUTS 32    // outputs per narrow item (8 per wave)
+#define ARGUS_NARROW_OUTPUTS 32    // default outputs per narrow item (8 per wave)
+#define ARGUS_GRID 2048            // default persistent workgroups (a multiple of ARGUS_XCDS)
+
+// Tile and launch shape of the fused extraction launch (picked by the
+// autotuner, gpud_*_autotune).
+struct ArgusTune {
+    int lds_entries;     // complex samples of LDS a dense tile may stage
+    int narrow_outputs;  // outputs per narrow item
+    int grid;            // persistent workgroups (multiple of ARGUS_XCDS)
+};
+
+static inline ArgusTune argus_default_tune() { return ArgusTune{ARGUS_LDS_ENTRIES, ARGUS_NARROW_OUTPUTS, ARGUS_GRID}; }
 
 #define ARGUS_DENSE_R 4             // register blocking of the dense FIR
 
@@ -112,23 +136,23 @@ static inline int argus_plan_dense(ArgusChanDesc* ch, int R, int entries)
 // Host-side planning for one channel.  Fills M, PS, SP, R, S, narrow.
 // Dense tiles when the windows of consecutive outputs overlap (D < T) and a
 // tile fits the LDS budget, else narrow items (any D and T).
-static inline int argus_plan_chan(ArgusChanDesc* ch)
+static inline int argus_plan_chan(ArgusChanDesc* ch, const ArgusTune& tune)
 {
     const int D = ch->D, T = ch->T;
     if (D < T) {
         ArgusChanDesc c = *ch;
-        if (argus_plan_dense(&c, ARGUS_DENSE_R, ARGUS_LDS_ENTRIES) == 0 && c.M / ARGUS_DENSE_R >= 4) {
+        if (argus_plan_dense(&c, ARGUS_DENSE_R, tune.lds_entries) == 0 && c.M / ARGUS_DENSE_R >= 4) {
             *ch = c;
             return 0;
         }
         c = *ch;
-        if (argus_plan_dense(&c, 1, ARGUS_LDS_ENTRIES) == 0) {
+        if (argus_plan_dense(&c, 1, tune.lds_entries) == 0) {
             *ch = c;
             return 0;
         }
     }
     ch->narrow = 1;
-    ch->M = ARGUS_NARROW_OUTPUTS;
+    ch->M = tune.narrow_outputs;
     ch->PS = ch->SP = 0;
     ch->R = 1;
     ch->S = 1;
@@ -140,3 +164,20 @@ static inline size_t argus_dense_lds_bytes(const ArgusChanDesc& ch)
 {
     return (size_t)ch.P * ch.PS * 8 + (size_t)ch.P * ch.L * 4;
 }
+
+int argus_fill_chan(ArgusChanDesc* ch, const float2* hist_rd, int H, float2* hist_wr, int keep, const float* taps,
+                    float2* out, int D, int T, int phase_count, int n_new, double phase_start, double phase_inc,
+                    const ArgusTune& tune);
+size_t argus_item_lds_bytes(const ArgusChanDesc& ch);
+int argus_launch_extract_batch(const ArgusChanDesc* d_chans, const ArgusWorkItem* d_items, int n_items,
+                               size_t lds_bytes, const float2* in, int n_new, int grid, hipStream_t stream);
+// Allow the extraction kernels dynamic LDS of `bytes` (needed above 64 KiB;
+// not a stream operation, so call it outside graph capture).
+int argus_extract_allow_lds(size_t bytes);
+// Graph-mode launch: counts and pointers come from *hdr (device memory).
+int argus_launch_extract_hdr(const ArgusExtractHdr* d_hdr, size_t lds_bytes, int grid, hipStream_t stream);
+void argus_append_items(const ArgusChanDesc& ch, int chan_index, int n_new, std::vector<ArgusWorkItem>& items,
+                        std::vector<int>& pos);
+void argus_order_items(std::vector<ArgusWorkItem>& items, const std::vector<int>& pos, int n_new);
+// LDS bytes of the largest dense tile or narrow item a tune allows for taps <= max_taps.
+size_t argus_tune_max_lds(const ArgusTune& tune, int max_taps);

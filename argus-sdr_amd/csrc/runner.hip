@@ -13,11 +13,6 @@
 #include <vector>
 
 int argus_launch_extract_one(const ArgusChanDesc& ch, const float2* in, int n_new, hipStream_t stream);
-int argus_launch_extract_batch(const ArgusChanDesc* d_chans, const ArgusWorkItem* d_items, int n_items,
-                               size_t lds_bytes, const float2* in, int n_new, hipStream_t stream);
-int argus_fill_chan(ArgusChanDesc* ch, const float2* hist_rd, int H, float2* hist_wr, int keep, const float* taps,
-                    float2* out, int D, int T, int phase_count, int n_new, double phase_start, double phase_inc);
-size_t argus_item_lds_bytes(const ArgusChanDesc& ch);
 
 // ---------------------------------------------------------------------------
 // Tap design (dsp/fir.go:6-28) and polyphase layout (polyphase.go:6-22).
@@ -49,6 +44,22 @@ static int build_poly(const float* base, int n, int decim, float* out)
             out[p * L + k] = src < n ? base[src] : 0.0f;
         }
     return decim * L;
+}
+
+extern "C" GPUD_API int gpud_runner_get_tune(gpud_runner* r, gpud_tune* t)
+{
+    if (!r || !t) return -1;
+    *t = gpud_tune{r->tune.lds_entries, r->tune.narrow_outputs, r->tune.grid, 0, 0.0f, 0.0f};
+    return 0;
+}
+
+extern "C" GPUD_API int gpud_runner_set_tune(gpud_runner* r, const gpud_tune* t)
+{
+    if (!r || !t || t->lds_entries < 1024 || t->lds_entries > ARGUS_MAX_LDS_ENTRIES || t->narrow_outputs < 1 ||
+        t->narrow_outputs > 1024 || t->grid < ARGUS_XCDS || t->grid > 65536 || t->grid % ARGUS_XCDS != 0)
+        return -1;
+    r->tune = ArgusTune{t->lds_entries, t->narrow_outputs, t->grid};
+    return 0;
 }
 
 extern "C" GPUD_API int gpud_lowpass_fir(double cutoff_hz, int sample_rate, int taps, double* out)
@@ -384,7 +395,7 @@ static int run_batched(gpud_runner* r, const float2* d_in, int n_new, const gpud
         const double inc = -2.0 * M_PI * jobs[i].offset_hz / (double)r->sample_rate;
         if (argus_fill_chan(&ch, s->d_hist[s->hist_sel], s->hist_len, s->d_hist[s->hist_sel ^ 1], keep, s->d_taps,
                             r->d_out + results[i].out_offset, D, T, s->phase_count, n_new, s->nco_phase,
-                            inc) != 0)
+                            inc, r->tune) != 0)
             return -1;
         if (ch.n_out > 0) lds = std::max(lds, argus_item_lds_bytes(ch));
         argus_append_items(ch, i, n_new, r->items, r->item_pos);
@@ -401,7 +412,8 @@ static int run_batched(gpud_runner* r, const float2* d_in, int n_new, const gpud
     ARGUS_CHECK(hipEventRecord(r->stage_ev[slot], r->stream));
     r->stage_recorded[slot] = true;
     if (r->timing) ARGUS_CHECK((hipError_t)argus_timing_begin(r));
-    int err = argus_launch_extract_batch(r->d_chans[slot], r->d_items[slot], n_items, lds, d_in, n_new, r->stream);
+    int err = argus_launch_extract_batch(r->d_chans[slot], r->d_items[slot], n_items, lds, d_in, n_new, r->tune.grid,
+                                         r->stream);
     if (err) return err;
     if (r->timing) ARGUS_CHECK(hipEventRecord(r->tev[(size_t)r->tev_used - 1], r->stream));
     // carry state forward (the kernel computes the same quantities)

@@ -30,6 +30,7 @@ struct ArgusSignal {
 };
 
 struct gpud_runner {
+    ArgusTune tune = argus_default_tune();
     int max_samples = 0;
     int sample_rate = 0;
     int exec_mode = GPUD_EXEC_BATCHED;
@@ -64,8 +65,5 @@ struct gpud_runner {
 };
 
 int argus_lowpass_fir(double cutoff_hz, int sample_rate, int taps, double* out, bool hamming);
-void argus_append_items(const ArgusChanDesc& ch, int chan_index, int n_new, std::vector<ArgusWorkItem>& items,
-                        std::vector<int>& pos);
-void argus_order_items(std::vector<ArgusWorkItem>& items, const std::vector<int>& pos, int n_new);
 ArgusSignal* argus_runner_find(gpud_runner* r, int64_t id);
 void argus_signal_release(ArgusSignal& s);

@@ -20,14 +20,6 @@
 #include <map>
 #include <vector>
 
-int argus_launch_extract_batch(const ArgusChanDesc* d_chans, const ArgusWorkItem* d_items, int n_items,
-                               size_t lds_bytes, const float2* in, int n_new, hipStream_t stream);
-int argus_fill_chan(ArgusChanDesc* ch, const float2* hist_rd, int H, float2* hist_wr, int keep, const float* taps,
-                    float2* out, int D, int T, int phase_count, int n_new, double phase_start, double phase_inc);
-size_t argus_item_lds_bytes(const ArgusChanDesc& ch);
-void argus_append_items(const ArgusChanDesc& ch, int chan_index, int n_new, std::vector<ArgusWorkItem>& items,
-                        std::vector<int>& pos);
-void argus_order_items(std::vector<ArgusWorkItem>& items, const std::vector<int>& pos, int n_new);
 int argus_lowpass_fir(double cutoff_hz, int sample_rate, int taps, double* out, bool hamming);
 
 namespace {
@@ -167,6 +159,41 @@ static hipError_t wfm_channel_init(ServeChannel& c, int rate)
     return hipMemcpy(c.d_wfm, blob.data(), kWfmBytes, hipMemcpyHostToDevice);
 }
 
+// Free a slot's output buffers, tables and captured graph (not its input
+// staging buffer or events).
+static void slot_release(ServeSlot& sl)
+{
+    hipFree(sl.d_iq);
+    hipFree(sl.d_audio);
+    hipHostFree(sl.h_iq);
+    hipHostFree(sl.h_audio);
+    hipFree(sl.d_mono);
+    hipFree(sl.d_wscratch);
+    hipFree(sl.d_rds);
+    hipFree(sl.d_wblk);
+    hipHostFree(sl.h_rds);
+    hipHostFree(sl.h_wblk);
+    hipHostFree(sl.h_tab);
+    hipFree(sl.d_tab);
+    if (sl.gexec) hipGraphExecDestroy(sl.gexec);
+    sl.d_iq = nullptr;
+    sl.d_audio = nullptr;
+    sl.h_iq = nullptr;
+    sl.h_audio = nullptr;
+    sl.d_mono = nullptr;
+    sl.d_wscratch = nullptr;
+    sl.d_rds = nullptr;
+    sl.d_wblk = nullptr;
+    sl.h_rds = nullptr;
+    sl.h_wblk = nullptr;
+    sl.h_tab = nullptr;
+    sl.d_tab = nullptr;
+    sl.gexec = nullptr;
+    sl.iq_cap = sl.audio_cap = sl.h_iq_cap = sl.h_audio_cap = 0;
+    sl.mono_cap = sl.wscratch_cap = sl.rds_cap = sl.h_rds_cap = sl.wblk_cap = sl.h_wblk_cap = 0;
+    sl.tab_cap = 0;
+}
+
 static int slot_wait(gpud_serve* s, ServeSlot& sl)
 {
     if (!sl.busy) return 0;
@@ -219,28 +246,7 @@ GPUD_API int gpud_serve_destroy(gpud_serve* s)
         hipEventDestroy(sl.ev_kstart);
         hipFree(sl.d_in);
         hipHostFree(sl.h_in);
-        hipFree(sl.d_iq);
-        hipFree(sl.d_audio);
-        hipHostFree(sl.h_iq);
-        hipHostFree(sl.h_audio);
-        hipFree(sl.d_chans);
-        hipFree(sl.d_items);
-        hipFree(sl.d_ddesc);
-        hipFree(sl.d_ditems);
-        hipHostFree(sl.h_chans);
-        hipHostFree(sl.h_items);
-        hipHostFree(sl.h_ddesc);
-        hipHostFree(sl.h_ditems);
-        hipFree(sl.d_mono);
-        hipFree(sl.d_wscratch);
-        hipFree(sl.d_rds);
-        hipFree(sl.d_wblk);
-        hipFree(sl.d_wdesc);
-        hipFree(sl.d_witems);
-        hipHostFree(sl.h_rds);
-        hipHostFree(sl.h_wblk);
-        hipHostFree(sl.h_wdesc);
-        hipHostFree(sl.h_witems);
+        slot_release(sl);
     }
     hipStreamDestroy(s->compute);
     hipStreamDestroy(s->copy);
@@ -331,13 +337,14 @@ static void pll_coefficients(double loop_bw, double damping, int rate, double* a
     *beta = (4 * theta * theta) / d;
 }
 
-// Stereo-stage descriptor and work items of one channel for this block; the
-// channel's host-side carry (delay-line half, RDS sample count) advances.
-static void wfm_describe(ServeChannel& c, const ServeOut& o, ServeSlot& sl, std::vector<ArgusWfmItem>& items)
+// Stereo-stage descriptor and work items of one channel for this block; with
+// commit the channel's host-side carry (delay-line half, RDS sample count)
+// advances.
+static void wfm_describe(ServeChannel& c, const ServeOut& o, ServeSlot& sl, ArgusWfmDesc& wd,
+                         std::vector<ArgusWfmItem>& items, bool commit)
 {
     const int rate = c.spec.out_rate;
     const int D = c.rds_decim;
-    ArgusWfmDesc& wd = sl.h_wdesc[o.wfm_index];
     wd.mono = sl.d_mono + o.mono_off;
     wd.audio = sl.d_audio + o.audio_off;
     wd.scratch = sl.d_wscratch + 4 * o.mono_off;
@@ -367,26 +374,45 @@ static void wfm_describe(ServeChannel& c, const ServeOut& o, ServeSlot& sl, std:
     wd.rds_decim = D;
     wd.n_rds = o.n_rds;
     for (int st = 0; st < o.n_mono; st += ARGUS_WFM_TILE) items.push_back({o.wfm_index, st, ARGUS_WFM_ITEM_TILE, 0});
-    if (o.n_mono > 0) {
-        items.push_back({o.wfm_index, 0, ARGUS_WFM_ITEM_TAIL, 0});
-        c.wfm_sel ^= 1;
+    if (o.n_mono > 0) items.push_back({o.wfm_index, 0, ARGUS_WFM_ITEM_TAIL, 0});
+    if (commit) {
+        if (o.n_mono > 0) c.wfm_sel ^= 1;
+        c.rds_count += o.n_mono;
     }
-    c.rds_count += o.n_mono;
 }
 
-// Enqueue one block: upload (unless device-resident), extract, demodulate,
-// download.  Returns immediately; *ticket identifies the block.
-static int serve_submit(gpud_serve* s, const gpud_float2* iq, int n, bool on_device, int64_t* ticket)
-{
-    if (!s || n < 0 || n > s->block_samples || (n > 0 && !iq)) return -1;
-    ServeSlot& sl = s->slots[s->next_slot];
-    int err = slot_wait(s, sl);  // the slot's previous block must be done
-    if (err) return err;
-    s->next_slot = (s->next_slot + 1) % ARGUS_SERVE_SLOTS;
-    sl.ticket = ++s->last_ticket;
-    sl.n = n;
+static size_t align_up(size_t v) { return (v + 255) & ~(size_t)255; }
 
-    // ---- per-channel descriptors (host) --------------------------------
+// Grow a slot's table to at least `need` bytes, keeping its first `keep` bytes.
+static int grow_table(ServeSlot& sl, size_t need, size_t keep)
+{
+    if (sl.h_tab && sl.tab_cap >= need) return 0;
+    const size_t cap = std::max<size_t>(need + need / 4, 1 << 16);
+    char* h = nullptr;
+    char* d = nullptr;
+    ARGUS_CHECK(hipHostMalloc((void**)&h, cap, hipHostMallocDefault));
+    hipError_t e = hipMalloc((void**)&d, cap);
+    if (e != hipSuccess) {
+        hipHostFree(h);
+        return (int)e;
+    }
+    if (keep && sl.h_tab) std::memcpy(h, sl.h_tab, keep);
+    if (sl.h_tab) hipHostFree(sl.h_tab);
+    if (sl.d_tab) hipFree(sl.d_tab);
+    sl.h_tab = h;
+    sl.d_tab = d;
+    sl.tab_cap = cap;
+    return 0;
+}
+
+// Build one block's outputs layout and tables in slot sl for input `in` (a
+// device pointer) of n samples.  With commit the channels' carried state
+// (phases, delay-line halves, carries) advances to after this block; without
+// it (autotuning) the state is left as it is -- the kernels then only write
+// outputs and the inactive history halves.
+static int build_block(gpud_serve* s, ServeSlot& sl, int n, const float2* in, bool commit)
+{
+    // ---- outputs layout ---------------------------------------------------
     const int nch = (int)s->live.size();
     sl.outs.clear();
     sl.outs.reserve((size_t)nch);
@@ -424,10 +450,6 @@ static int serve_submit(gpud_serve* s, const gpud_float2* iq, int n, bool on_dev
     ARGUS_CHECK((hipError_t)grow_dev(&sl.d_audio, &sl.audio_cap, (size_t)std::max<int64_t>(audio_total, 1)));
     ARGUS_CHECK((hipError_t)grow_host(&sl.h_iq, &sl.h_iq_cap, (size_t)std::max<int64_t>(iq_total, 1)));
     ARGUS_CHECK((hipError_t)grow_host(&sl.h_audio, &sl.h_audio_cap, (size_t)std::max<int64_t>(audio_total, 1)));
-    ARGUS_CHECK((hipError_t)grow_host(&sl.h_chans, &sl.h_chans_cap, (size_t)std::max(nch, 1)));
-    ARGUS_CHECK((hipError_t)grow_dev(&sl.d_chans, &sl.d_chans_cap, (size_t)std::max(nch, 1)));
-    ARGUS_CHECK((hipError_t)grow_host(&sl.h_ddesc, &sl.h_ddesc_cap, (size_t)std::max(nch, 1)));
-    ARGUS_CHECK((hipError_t)grow_dev(&sl.d_ddesc, &sl.d_ddesc_cap, (size_t)std::max(nch, 1)));
     if (n_wfm > 0) {
         ARGUS_CHECK((hipError_t)grow_dev(&sl.d_mono, &sl.mono_cap, (size_t)std::max<int64_t>(mono_total, 1)));
         ARGUS_CHECK((hipError_t)grow_dev(&sl.d_wscratch, &sl.wscratch_cap, (size_t)std::max<int64_t>(4 * mono_total, 1)));
@@ -435,11 +457,21 @@ static int serve_submit(gpud_serve* s, const gpud_float2* iq, int n, bool on_dev
         ARGUS_CHECK((hipError_t)grow_host(&sl.h_rds, &sl.h_rds_cap, (size_t)std::max<int64_t>(rds_total, 1)));
         ARGUS_CHECK((hipError_t)grow_dev(&sl.d_wblk, &sl.wblk_cap, (size_t)n_wfm));
         ARGUS_CHECK((hipError_t)grow_host(&sl.h_wblk, &sl.h_wblk_cap, (size_t)n_wfm));
-        ARGUS_CHECK((hipError_t)grow_host(&sl.h_wdesc, &sl.h_wdesc_cap, (size_t)n_wfm));
-        ARGUS_CHECK((hipError_t)grow_dev(&sl.d_wdesc, &sl.d_wdesc_cap, (size_t)n_wfm));
     }
     sl.n_wfm = n_wfm;
     sl.rds_total = rds_total;
+    sl.iq_total = iq_total;
+    sl.audio_total = audio_total;
+
+    // ---- tables: descriptors first (counts known), lists after ------------
+    const size_t off_chans = align_up(sizeof(ArgusServeHdr));
+    const size_t off_ddesc = align_up(off_chans + (size_t)nch * sizeof(ArgusChanDesc));
+    const size_t off_wdesc = align_up(off_ddesc + (size_t)nch * sizeof(ArgusDemodDesc));
+    const size_t prefix = off_wdesc + (size_t)n_wfm * sizeof(ArgusWfmDesc);
+    ARGUS_CHECK((hipError_t)grow_table(sl, prefix, 0));
+    ArgusChanDesc* hc = (ArgusChanDesc*)(sl.h_tab + off_chans);
+    ArgusDemodDesc* hd = (ArgusDemodDesc*)(sl.h_tab + off_ddesc);
+    ArgusWfmDesc* hw = (ArgusWfmDesc*)(sl.h_tab + off_wdesc);
 
     s->items.clear();
     s->item_pos.clear();
@@ -447,20 +479,19 @@ static int serve_submit(gpud_serve* s, const gpud_float2* iq, int n, bool on_dev
     s->witems.clear();
     size_t lds = 0;
     int ci = 0;
-    const float2* in = on_device ? (const float2*)iq : sl.d_in;
     for (auto& kv : s->live) {
         ServeChannel& c = kv.second;
         const ServeOut& o = sl.outs[(size_t)ci];
         const int keep = std::min(c.hist_len + n, c.T - 1);
         const double inc = -2.0 * M_PI * c.spec.offset_hz / (double)s->sample_rate;
-        ArgusChanDesc& ch = sl.h_chans[ci];
+        ArgusChanDesc& ch = hc[ci];
         if (argus_fill_chan(&ch, c.d_hist[c.hist_sel], c.hist_len, c.d_hist[c.hist_sel ^ 1], keep, c.d_taps,
-                            sl.d_iq + o.iq_off, c.decim, c.T, c.phase_count, n, c.nco_phase, inc) != 0)
+                            sl.d_iq + o.iq_off, c.decim, c.T, c.phase_count, n, c.nco_phase, inc, s->tune) != 0)
             return -1;
         if (ch.n_out > 0) lds = std::max(lds, argus_item_lds_bytes(ch));
         argus_append_items(ch, ci, n, s->items, s->item_pos);
 
-        ArgusDemodDesc& dd = sl.h_ddesc[ci];
+        ArgusDemodDesc& dd = hd[ci];
         dd.iq = sl.d_iq + o.iq_off;
         dd.audio = c.stereo ? sl.d_mono + o.mono_off : sl.d_audio + o.audio_off;
         dd.carry_rd = c.d_carry + c.carry_sel;
@@ -481,36 +512,106 @@ static int serve_submit(gpud_serve* s, const gpud_float2* iq, int n, bool on_dev
                     s->ditems.push_back({ci, st, std::min(ARGUS_DEMOD_TILE, o.n_iq - st), 0});
             }
         }
-        if (c.stereo) wfm_describe(c, o, sl, s->witems);
-        // carry state forward (the kernels compute with the values above)
-        c.phase_count = (c.phase_count + n) % c.decim;
-        c.nco_phase = argus_reduce_phase(c.nco_phase + inc * (double)n);
-        if (keep > 0) c.hist_sel ^= 1;
-        c.hist_len = keep;
-        if ((dd.mode == ARGUS_DEMOD_NFM || dd.mode == ARGUS_DEMOD_WFM) && o.n_iq > 0) {
-            c.carry_sel ^= 1;
-            c.have_carry = true;
+        if (c.stereo) wfm_describe(c, o, sl, hw[o.wfm_index], s->witems, commit);
+        if (commit) {
+            // carry state forward (the kernels compute with the values above)
+            c.phase_count = (c.phase_count + n) % c.decim;
+            c.nco_phase = argus_reduce_phase(c.nco_phase + inc * (double)n);
+            if (keep > 0) c.hist_sel ^= 1;
+            c.hist_len = keep;
+            if ((dd.mode == ARGUS_DEMOD_NFM || dd.mode == ARGUS_DEMOD_WFM) && o.n_iq > 0) {
+                c.carry_sel ^= 1;
+                c.have_carry = true;
+            }
+            c.bfo_phase = argus_reduce_phase(c.bfo_phase + bfo_inc * (double)o.n_iq);
+            c.last_ticket = sl.ticket;
         }
-        c.bfo_phase = argus_reduce_phase(c.bfo_phase + bfo_inc * (double)o.n_iq);
-        c.last_ticket = sl.ticket;
         ci++;
     }
     argus_order_items(s->items, s->item_pos, n);
-    const int n_items = (int)s->items.size(), n_ditems = (int)s->ditems.size();
-    const int n_witems = (int)s->witems.size();
-    if (n_witems > 0) {
-        ARGUS_CHECK((hipError_t)grow_host(&sl.h_witems, &sl.h_witems_cap, (size_t)n_witems));
-        ARGUS_CHECK((hipError_t)grow_dev(&sl.d_witems, &sl.d_witems_cap, (size_t)n_witems));
-        std::memcpy(sl.h_witems, s->witems.data(), (size_t)n_witems * sizeof(ArgusWfmItem));
+    const int n_items = (int)s->items.size(), n_ditems = (int)s->ditems.size(), n_witems = (int)s->witems.size();
+    const size_t off_ditems = align_up(prefix);
+    const size_t off_witems = align_up(off_ditems + (size_t)n_ditems * sizeof(ArgusDemodItem));
+    const size_t off_items = align_up(off_witems + (size_t)n_witems * sizeof(ArgusWfmItem));
+    const size_t total = off_items + (size_t)n_items * sizeof(ArgusWorkItem);
+    ARGUS_CHECK((hipError_t)grow_table(sl, total, prefix));
+    if (n_ditems) std::memcpy(sl.h_tab + off_ditems, s->ditems.data(), (size_t)n_ditems * sizeof(ArgusDemodItem));
+    if (n_witems) std::memcpy(sl.h_tab + off_witems, s->witems.data(), (size_t)n_witems * sizeof(ArgusWfmItem));
+    if (n_items) std::memcpy(sl.h_tab + off_items, s->items.data(), (size_t)n_items * sizeof(ArgusWorkItem));
+    ArgusServeHdr& hdr = *(ArgusServeHdr*)sl.h_tab;
+    const char* d = sl.d_tab;
+    hdr.x = ArgusExtractHdr{(const ArgusChanDesc*)(d + off_chans), (const ArgusWorkItem*)(d + off_items), in, n_items,
+                            n};
+    hdr.d = ArgusDemodHdr{(const ArgusDemodDesc*)(d + off_ddesc), (const ArgusDemodItem*)(d + off_ditems), n_ditems, 0};
+    hdr.w = ArgusWfmHdr{(const ArgusWfmDesc*)(d + off_wdesc), (const ArgusWfmItem*)(d + off_witems), n_wfm, n_witems};
+    sl.tab_bytes = total;
+    sl.lds = lds;
+    return 0;
+}
+
+// The block's kernels in order, each reading its counts and pointers from the
+// table header on the device.
+static int launch_eager(gpud_serve* s, ServeSlot& sl, bool wfm)
+{
+    const ArgusServeHdr* dh = (const ArgusServeHdr*)sl.d_tab;
+    int err = argus_launch_extract_hdr(&dh->x, sl.lds, s->tune.grid, s->compute);
+    if (err) return err;
+    err = argus_launch_demod_hdr(&dh->d, s->compute);
+    if (err) return err;
+    return wfm ? argus_launch_wfm_hdr(&dh->w, s->compute) : 0;
+}
+
+// Replay the slot's captured graph of the block's launches, capturing it
+// again when what it depends on (table address, LDS size, grid, whether a
+// stereo stage runs) has changed.
+static int launch_block(gpud_serve* s, ServeSlot& sl)
+{
+    const bool wfm = sl.n_wfm > 0;
+    if (!s->graph) return launch_eager(s, sl, wfm);
+    if (!sl.gexec || sl.g_tab != sl.d_tab || sl.g_lds != sl.lds || sl.g_grid != s->tune.grid ||
+        sl.g_wfm != (int)wfm) {
+        if (sl.gexec) {
+            hipGraphExecDestroy(sl.gexec);
+            sl.gexec = nullptr;
+        }
+        int lerr = argus_extract_allow_lds(sl.lds);  // outside capture
+        if (lerr) return lerr;
+        ARGUS_CHECK(hipStreamBeginCapture(s->compute, hipStreamCaptureModeThreadLocal));
+        int err = launch_eager(s, sl, wfm);
+        hipGraph_t g = nullptr;
+        hipError_t e = hipStreamEndCapture(s->compute, &g);
+        if (err || e != hipSuccess) {
+            if (g) hipGraphDestroy(g);
+            return err ? err : (int)e;
+        }
+        e = hipGraphInstantiate(&sl.gexec, g, nullptr, nullptr, 0);
+        hipGraphDestroy(g);
+        if (e != hipSuccess) {
+            sl.gexec = nullptr;
+            return (int)e;
+        }
+        sl.g_tab = sl.d_tab;
+        sl.g_lds = sl.lds;
+        sl.g_grid = s->tune.grid;
+        sl.g_wfm = (int)wfm;
     }
-    ARGUS_CHECK((hipError_t)grow_host(&sl.h_items, &sl.h_items_cap, (size_t)std::max(n_items, 1)));
-    ARGUS_CHECK((hipError_t)grow_dev(&sl.d_items, &sl.d_items_cap, (size_t)std::max(n_items, 1)));
-    ARGUS_CHECK((hipError_t)grow_host(&sl.h_ditems, &sl.h_ditems_cap, (size_t)std::max(n_ditems, 1)));
-    ARGUS_CHECK((hipError_t)grow_dev(&sl.d_ditems, &sl.d_ditems_cap, (size_t)std::max(n_ditems, 1)));
-    if (n_items) std::memcpy(sl.h_items, s->items.data(), (size_t)n_items * sizeof(ArgusWorkItem));
-    if (n_ditems) std::memcpy(sl.h_ditems, s->ditems.data(), (size_t)n_ditems * sizeof(ArgusDemodItem));
-    sl.iq_total = iq_total;
-    sl.audio_total = audio_total;
+    return (int)hipGraphLaunch(sl.gexec, s->compute);
+}
+
+// Enqueue one block: upload (unless device-resident), extract, demodulate,
+// download.  Returns immediately; *ticket identifies the block.
+static int serve_submit(gpud_serve* s, const gpud_float2* iq, int n, bool on_device, int64_t* ticket)
+{
+    if (!s || n < 0 || n > s->block_samples || (n > 0 && !iq)) return -1;
+    ServeSlot& sl = s->slots[s->next_slot];
+    int err = slot_wait(s, sl);  // the slot's previous block must be done
+    if (err) return err;
+    s->next_slot = (s->next_slot + 1) % ARGUS_SERVE_SLOTS;
+    sl.ticket = ++s->last_ticket;
+    sl.n = n;
+    const float2* in = on_device ? (const float2*)iq : sl.d_in;
+    err = build_block(s, sl, n, in, true);
+    if (err) return err;
 
     // ---- copy stream: block upload -------------------------------------
     ARGUS_CHECK(hipEventRecord(sl.ev_start, s->copy));
@@ -519,45 +620,27 @@ static int serve_submit(gpud_serve* s, const gpud_float2* iq, int n, bool on_dev
         ARGUS_CHECK(hipMemcpyAsync(sl.d_in, sl.h_in, (size_t)n * sizeof(float2), hipMemcpyHostToDevice, s->copy));
     }
     ARGUS_CHECK(hipEventRecord(sl.ev_in, s->copy));
-    // ---- compute stream: tables, extraction, demodulation ----------------
+    // ---- compute stream: one table copy, then the block's kernels ----------
     ARGUS_CHECK(hipStreamWaitEvent(s->compute, sl.ev_in, 0));
-    ARGUS_CHECK(hipMemcpyAsync(sl.d_chans, sl.h_chans, (size_t)nch * sizeof(ArgusChanDesc), hipMemcpyHostToDevice,
-                               s->compute));
-    ARGUS_CHECK(hipMemcpyAsync(sl.d_items, sl.h_items, (size_t)n_items * sizeof(ArgusWorkItem),
-                               hipMemcpyHostToDevice, s->compute));
-    ARGUS_CHECK(hipMemcpyAsync(sl.d_ddesc, sl.h_ddesc, (size_t)nch * sizeof(ArgusDemodDesc), hipMemcpyHostToDevice,
-                               s->compute));
-    ARGUS_CHECK(hipMemcpyAsync(sl.d_ditems, sl.h_ditems, (size_t)n_ditems * sizeof(ArgusDemodItem),
-                               hipMemcpyHostToDevice, s->compute));
-    if (n_wfm > 0) {
-        ARGUS_CHECK(hipMemcpyAsync(sl.d_wdesc, sl.h_wdesc, (size_t)n_wfm * sizeof(ArgusWfmDesc),
-                                   hipMemcpyHostToDevice, s->compute));
-        if (n_witems > 0)
-            ARGUS_CHECK(hipMemcpyAsync(sl.d_witems, sl.h_witems, (size_t)n_witems * sizeof(ArgusWfmItem),
-                                       hipMemcpyHostToDevice, s->compute));
-    }
+    ARGUS_CHECK(hipMemcpyAsync(sl.d_tab, sl.h_tab, sl.tab_bytes, hipMemcpyHostToDevice, s->compute));
     ARGUS_CHECK(hipEventRecord(sl.ev_kstart, s->compute));
-    err = argus_launch_extract_batch(sl.d_chans, sl.d_items, n_items, lds, in, n, s->compute);
-    if (err) return err;
-    err = argus_launch_demod(sl.d_ddesc, sl.d_ditems, n_ditems, s->compute);
-    if (err) return err;
-    err = argus_launch_wfm(sl.d_wdesc, n_wfm, sl.d_witems, n_witems, s->compute);
+    err = launch_block(s, sl);
     if (err) return err;
     ARGUS_CHECK(hipEventRecord(sl.ev_kern, s->compute));
     // ---- copy stream: download ------------------------------------------
     ARGUS_CHECK(hipStreamWaitEvent(s->copy, sl.ev_kern, 0));
     if (s->download) {
-        if (iq_total > 0)
-            ARGUS_CHECK(hipMemcpyAsync(sl.h_iq, sl.d_iq, (size_t)iq_total * sizeof(float2), hipMemcpyDeviceToHost,
+        if (sl.iq_total > 0)
+            ARGUS_CHECK(hipMemcpyAsync(sl.h_iq, sl.d_iq, (size_t)sl.iq_total * sizeof(float2), hipMemcpyDeviceToHost,
                                        s->copy));
-        if (audio_total > 0)
-            ARGUS_CHECK(hipMemcpyAsync(sl.h_audio, sl.d_audio, (size_t)audio_total * sizeof(float),
+        if (sl.audio_total > 0)
+            ARGUS_CHECK(hipMemcpyAsync(sl.h_audio, sl.d_audio, (size_t)sl.audio_total * sizeof(float),
                                        hipMemcpyDeviceToHost, s->copy));
-        if (rds_total > 0)
-            ARGUS_CHECK(hipMemcpyAsync(sl.h_rds, sl.d_rds, (size_t)rds_total * sizeof(float2), hipMemcpyDeviceToHost,
-                                       s->copy));
-        if (n_wfm > 0)
-            ARGUS_CHECK(hipMemcpyAsync(sl.h_wblk, sl.d_wblk, (size_t)n_wfm * sizeof(ArgusWfmBlock),
+        if (sl.rds_total > 0)
+            ARGUS_CHECK(hipMemcpyAsync(sl.h_rds, sl.d_rds, (size_t)sl.rds_total * sizeof(float2),
+                                       hipMemcpyDeviceToHost, s->copy));
+        if (sl.n_wfm > 0)
+            ARGUS_CHECK(hipMemcpyAsync(sl.h_wblk, sl.d_wblk, (size_t)sl.n_wfm * sizeof(ArgusWfmBlock),
                                        hipMemcpyDeviceToHost, s->copy));
     }
     ARGUS_CHECK(hipEventRecord(sl.ev_done, s->copy));
@@ -635,6 +718,108 @@ GPUD_API int gpud_serve_channel_output(gpud_serve* s, int64_t ticket, int64_t ch
             return 0;
         }
     return -1;
+}
+
+GPUD_API int gpud_serve_set_graph(gpud_serve* s, int enable)
+{
+    if (!s) return -1;
+    for (auto& sl : s->slots) slot_wait(s, sl);
+    s->graph = enable != 0;
+    return 0;
+}
+
+GPUD_API int gpud_serve_get_tune(gpud_serve* s, gpud_tune* t)
+{
+    if (!s || !t) return -1;
+    *t = gpud_tune{s->tune.lds_entries, s->tune.narrow_outputs, s->tune.grid, 0, 0.0f, 0.0f};
+    return 0;
+}
+
+static bool valid_tune(const gpud_tune& t)
+{
+    return t.lds_entries >= 1024 && t.lds_entries <= ARGUS_MAX_LDS_ENTRIES && t.narrow_outputs >= 1 &&
+           t.narrow_outputs <= 1024 && t.grid >= ARGUS_XCDS && t.grid <= 65536 && t.grid % ARGUS_XCDS == 0;
+}
+
+GPUD_API int gpud_serve_set_tune(gpud_serve* s, const gpud_tune* t)
+{
+    if (!s || !t || !valid_tune(*t)) return -1;
+    s->tune = ArgusTune{t->lds_entries, t->narrow_outputs, t->grid};
+    return 0;
+}
+
+// Time the fused extraction launch of the live channel set on the block at
+// iq_dev (n samples, device memory) for tile and launch shapes around the
+// current one, one parameter at a time (grid, then LDS budget per dense tile,
+// then outputs per narrow item), and keep the fastest.  Channel state does
+// not advance: the launches write a scratch slot's outputs and the inactive
+// history halves only.  Blocks in flight are completed first.
+GPUD_API int gpud_serve_autotune(gpud_serve* s, const gpud_float2* iq_dev, int n, int reps, gpud_tune* best)
+{
+    if (!s || !iq_dev || n <= 0 || n > s->block_samples || reps <= 0) return -1;
+    for (auto& sl : s->slots) {
+        int err = slot_wait(s, sl);
+        if (err) return err;
+    }
+    ServeSlot scratch;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    int err = (int)hipEventCreate(&e0);
+    if (!err) err = (int)hipEventCreate(&e1);
+    auto eval = [&](const ArgusTune& t, float* ms) -> int {
+        s->tune = t;
+        int rc = build_block(s, scratch, n, (const float2*)iq_dev, false);
+        if (rc) return rc;
+        rc = argus_extract_allow_lds(scratch.lds);
+        if (rc) return rc;
+        ARGUS_CHECK(hipMemcpyAsync(scratch.d_tab, scratch.h_tab, scratch.tab_bytes, hipMemcpyHostToDevice,
+                                   s->compute));
+        const ArgusServeHdr* dh = (const ArgusServeHdr*)scratch.d_tab;
+        rc = argus_launch_extract_hdr(&dh->x, scratch.lds, t.grid, s->compute);  // warm-up
+        if (rc) return rc;
+        float bestm = 1e30f;
+        for (int r = 0; r < reps; r++) {
+            ARGUS_CHECK(hipEventRecord(e0, s->compute));
+            rc = argus_launch_extract_hdr(&dh->x, scratch.lds, t.grid, s->compute);
+            if (rc) return rc;
+            ARGUS_CHECK(hipEventRecord(e1, s->compute));
+            ARGUS_CHECK(hipEventSynchronize(e1));
+            float m = 0.0f;
+            ARGUS_CHECK(hipEventElapsedTime(&m, e0, e1));
+            bestm = std::min(bestm, m);
+        }
+        *ms = bestm;
+        return 0;
+    };
+    ArgusTune cur = s->tune;
+    float cur_ms = 0.0f;
+    if (!err) err = eval(cur, &cur_ms);
+    const int grids[] = {1024, 2048, 3072, 4096, 6144};
+    const int ldss[] = {4096, 6144, 8192, 10240, 12288};
+    const int narrows[] = {16, 32, 64};
+    for (int pass = 0; pass < 3 && !err; pass++) {
+        const int* vals = pass == 0 ? grids : pass == 1 ? ldss : narrows;
+        const int nv = pass == 0 ? 5 : pass == 1 ? 5 : 3;
+        for (int i = 0; i < nv && !err; i++) {
+            ArgusTune t = cur;
+            int& field = pass == 0 ? t.grid : pass == 1 ? t.lds_entries : t.narrow_outputs;
+            if (field == vals[i]) continue;
+            field = vals[i];
+            float ms = 0.0f;
+            err = eval(t, &ms);
+            if (!err && ms < cur_ms) {
+                cur = t;
+                cur_ms = ms;
+            }
+        }
+    }
+    s->tune = cur;
+    hipStreamSynchronize(s->compute);
+    slot_release(scratch);
+    if (e0) hipEventDestroy(e0);
+    if (e1) hipEventDestroy(e1);
+    if (err) return err;
+    if (best) *best = gpud_tune{cur.lds_entries, cur.narrow_outputs, cur.grid, 0, cur_ms, 0.0f};
+    return 0;
 }
 
 // RDS baseband and stereo lock of a WFM stereo channel in a completed block.

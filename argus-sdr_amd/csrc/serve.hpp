@@ -54,18 +54,21 @@ struct ServeSlot {
     float2* h_iq = nullptr;
     float* h_audio = nullptr;
     size_t h_iq_cap = 0, h_audio_cap = 0;
-    ArgusChanDesc* h_chans = nullptr;
-    ArgusChanDesc* d_chans = nullptr;
-    size_t h_chans_cap = 0, d_chans_cap = 0;
-    ArgusWorkItem* h_items = nullptr;
-    ArgusWorkItem* d_items = nullptr;
-    size_t h_items_cap = 0, d_items_cap = 0;
-    ArgusDemodDesc* h_ddesc = nullptr;
-    ArgusDemodDesc* d_ddesc = nullptr;
-    size_t h_ddesc_cap = 0, d_ddesc_cap = 0;
-    ArgusDemodItem* h_ditems = nullptr;
-    ArgusDemodItem* d_ditems = nullptr;
-    size_t h_ditems_cap = 0, d_ditems_cap = 0;
+    // The block's tables, one pinned host buffer mirrored by one device
+    // buffer and uploaded with one copy: ArgusServeHdr | channel descriptors |
+    // demod descriptors | stereo descriptors | demod items | stereo items |
+    // extraction work items.
+    char* h_tab = nullptr;
+    char* d_tab = nullptr;
+    size_t tab_cap = 0;
+    size_t tab_bytes = 0;
+    size_t lds = 0;
+    // captured launch sequence of this slot and what it was captured for
+    hipGraphExec_t gexec = nullptr;
+    const char* g_tab = nullptr;
+    size_t g_lds = 0;
+    int g_grid = 0;
+    int g_wfm = -1;
     // WFM stereo stage
     float* d_mono = nullptr;
     size_t mono_cap = 0;
@@ -79,19 +82,22 @@ struct ServeSlot {
     size_t wblk_cap = 0;
     ArgusWfmBlock* h_wblk = nullptr;
     size_t h_wblk_cap = 0;
-    ArgusWfmDesc* h_wdesc = nullptr;
-    ArgusWfmDesc* d_wdesc = nullptr;
-    size_t h_wdesc_cap = 0, d_wdesc_cap = 0;
-    ArgusWfmItem* h_witems = nullptr;
-    ArgusWfmItem* d_witems = nullptr;
-    size_t h_witems_cap = 0, d_witems_cap = 0;
     int n_wfm = 0;
     int64_t rds_total = 0;
     int64_t iq_total = 0, audio_total = 0;
     std::vector<ServeOut> outs;
 };
 
+// What the graph-mode kernels of one block read (first bytes of the table).
+struct ArgusServeHdr {
+    ArgusExtractHdr x;
+    ArgusDemodHdr d;
+    ArgusWfmHdr w;
+};
+
 struct gpud_serve {
+    ArgusTune tune = argus_default_tune();
+    bool graph = true;  // replay a captured per-slot graph of the block's launches
     int sample_rate = 0;
     int block_samples = 0;
     uint64_t budget = 0;

@@ -51,8 +51,8 @@ __device__ __forceinline__ float fb_at(const ArgusWfmDesc& d, int fb_rd, int idx
 
 }  // namespace
 
-__global__ __launch_bounds__(256) void argus_wfm_pre_kernel(const ArgusWfmDesc* __restrict__ descs,
-                                                            const ArgusWfmItem* __restrict__ items, int n_items)
+__device__ __forceinline__ void wfm_pre(const ArgusWfmDesc* __restrict__ descs, const ArgusWfmItem* __restrict__ items,
+                                        int n_items)
 {
     __shared__ float xs[ARGUS_WFM_TILE + ARGUS_WFM_HIST];
     __shared__ float2 ms[ARGUS_WFM_TILE + ARGUS_WFM_HIST];
@@ -124,12 +124,10 @@ __global__ __launch_bounds__(256) void argus_wfm_pre_kernel(const ArgusWfmDesc* 
 
 // One wavefront per channel: the wave stages the pilot in LDS, lane 0 runs
 // the recurrence, the wave writes the per-sample oscillator phase back.
-__global__ __launch_bounds__(64) void argus_wfm_pll_kernel(const ArgusWfmDesc* __restrict__ descs, int n_chans)
+__device__ __forceinline__ void wfm_pll_one(const ArgusWfmDesc& d)
 {
     __shared__ float pl[ARGUS_WFM_TILE];
     __shared__ float tt[ARGUS_WFM_TILE];
-    if ((int)blockIdx.x >= n_chans) return;
-    const ArgusWfmDesc& d = descs[blockIdx.x];
     const int n = d.n;
     ArgusWfmState* sp = d.st;
     if (n <= 0) {  // no samples: state and lock counters unchanged (stereoDecodeStateful returns early)
@@ -209,8 +207,8 @@ __global__ __launch_bounds__(64) void argus_wfm_pll_kernel(const ArgusWfmDesc* _
     }
 }
 
-__global__ __launch_bounds__(256) void argus_wfm_post_kernel(const ArgusWfmDesc* __restrict__ descs,
-                                                             const ArgusWfmItem* __restrict__ items, int n_items)
+__device__ __forceinline__ void wfm_post(const ArgusWfmDesc* __restrict__ descs, const ArgusWfmItem* __restrict__ items,
+                                         int n_items)
 {
     __shared__ float xs[ARGUS_WFM_TILE + ARGUS_WFM_HIST];
     __shared__ float hs[ARGUS_WFM_TAPS];
@@ -254,6 +252,60 @@ __global__ __launch_bounds__(256) void argus_wfm_post_kernel(const ArgusWfmDesc*
         }
         __syncthreads();
     }
+}
+
+__global__ __launch_bounds__(256) void argus_wfm_pre_kernel(const ArgusWfmDesc* __restrict__ descs,
+                                                            const ArgusWfmItem* __restrict__ items, int n_items)
+{
+    wfm_pre(descs, items, n_items);
+}
+
+__global__ __launch_bounds__(64) void argus_wfm_pll_kernel(const ArgusWfmDesc* __restrict__ descs, int n_chans)
+{
+    for (int c = blockIdx.x; c < n_chans; c += gridDim.x) {
+        wfm_pll_one(descs[c]);
+        __syncthreads();
+    }
+}
+
+__global__ __launch_bounds__(256) void argus_wfm_post_kernel(const ArgusWfmDesc* __restrict__ descs,
+                                                             const ArgusWfmItem* __restrict__ items, int n_items)
+{
+    wfm_post(descs, items, n_items);
+}
+
+__global__ __launch_bounds__(256) void argus_wfm_pre_hdr_kernel(const ArgusWfmHdr* __restrict__ hdr)
+{
+    const ArgusWfmHdr h = *hdr;
+    wfm_pre(h.descs, h.items, h.n_items);
+}
+
+__global__ __launch_bounds__(64) void argus_wfm_pll_hdr_kernel(const ArgusWfmHdr* __restrict__ hdr)
+{
+    const ArgusWfmHdr h = *hdr;
+    for (int c = blockIdx.x; c < h.n_chans; c += gridDim.x) {
+        wfm_pll_one(h.descs[c]);
+        __syncthreads();
+    }
+}
+
+__global__ __launch_bounds__(256) void argus_wfm_post_hdr_kernel(const ArgusWfmHdr* __restrict__ hdr)
+{
+    const ArgusWfmHdr h = *hdr;
+    wfm_post(h.descs, h.items, h.n_items);
+}
+
+int argus_launch_wfm_hdr(const ArgusWfmHdr* d_hdr, hipStream_t stream)
+{
+    argus_clear_error();
+    hipLaunchKernelGGL(argus_wfm_pre_hdr_kernel, dim3(ARGUS_WFM_GRID), dim3(256), 0, stream, d_hdr);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return (int)e;
+    hipLaunchKernelGGL(argus_wfm_pll_hdr_kernel, dim3(ARGUS_WFM_GRID), dim3(64), 0, stream, d_hdr);
+    e = hipGetLastError();
+    if (e != hipSuccess) return (int)e;
+    hipLaunchKernelGGL(argus_wfm_post_hdr_kernel, dim3(ARGUS_WFM_GRID), dim3(256), 0, stream, d_hdr);
+    return (int)hipGetLastError();
 }
 
 int argus_launch_wfm(const ArgusWfmDesc* d_desc, int n_chans, const ArgusWfmItem* d_items, int n_items,

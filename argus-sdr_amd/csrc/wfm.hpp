@@ -73,5 +73,16 @@ struct ArgusWfmItem {
     int pad;
 };
 
+struct ArgusWfmHdr {
+    const ArgusWfmDesc* descs;
+    const ArgusWfmItem* items;
+    int n_chans;
+    int n_items;
+};
+
+#define ARGUS_WFM_GRID 256
+
 int argus_launch_wfm(const ArgusWfmDesc* d_desc, int n_chans, const ArgusWfmItem* d_items, int n_items,
                      hipStream_t stream);
+// Graph-mode launch of the three stages: counts and pointers from *hdr.
+int argus_launch_wfm_hdr(const ArgusWfmHdr* d_hdr, hipStream_t stream);

@@ -150,6 +150,15 @@ enum {
                                   the reference's call pattern (streaming_gpu_native_prepare.go:17-108) */
 };
 
+/* Tile and launch shape of the fused extraction launch. */
+typedef struct gpud_tune {
+    int32_t lds_entries;     /* complex samples of LDS one dense tile may stage (1024..16384; default 6144) */
+    int32_t narrow_outputs;  /* outputs per narrow work item (default 32) */
+    int32_t grid;            /* persistent workgroups, a multiple of 8 (default 2048) */
+    int32_t pad;
+    float   kernel_ms;       /* autotune: best launch time found */
+    float   pad2;
+} gpud_tune;
 /* batch_runner.go:23-34 NewBatchRunner(maxSamples, sampleRate).  -1 bad args. */
 GPUD_API int gpud_runner_create(int max_samples, int sample_rate, gpud_runner** out);
 /* batch_runner.go:36-50 Close */
@@ -164,6 +173,10 @@ GPUD_API int gpud_runner_set_window(gpud_runner* r, int window);
  * it was enabled. */
 GPUD_API int gpud_runner_enable_kernel_timing(gpud_runner* r, int enable);
 GPUD_API int gpud_runner_kernel_time(gpud_runner* r, double* total_ms, int* launches);
+/* Tile and launch shape of the runner's fused launches (no reference
+ * counterpart; see gpud_serve_autotune).  -1 if out of range. */
+GPUD_API int gpud_runner_get_tune(gpud_runner* r, gpud_tune* t);
+GPUD_API int gpud_runner_set_tune(gpud_runner* r, const gpud_tune* t);
 /* Stream the runner launches on (created by the runner, default priority). */
 GPUD_API gpud_stream_handle gpud_runner_stream(gpud_runner* r);
 
@@ -269,6 +282,16 @@ GPUD_API int gpud_serve_wait(gpud_serve* s, int64_t ticket, gpud_serve_block_inf
  * its slot is reused, i.e. for the next two submits). */
 GPUD_API int gpud_serve_channel_output(gpud_serve* s, int64_t ticket, int64_t channel_id, const gpud_float2** iq,
                                        int* n_iq, const float** audio, int* n_audio, int* rate);
+/* Replay (1, default) or not (0) a captured per-slot hipGraph of each
+ * block's launches; waits for blocks in flight. */
+GPUD_API int gpud_serve_set_graph(gpud_serve* s, int enable);
+GPUD_API int gpud_serve_get_tune(gpud_serve* s, gpud_tune* t);
+GPUD_API int gpud_serve_set_tune(gpud_serve* s, const gpud_tune* t);
+/* Autotune the tile and launch shape on the live channel set with a block
+ * at iq_dev (device memory, n samples): times the fused extraction launch
+ * (best of reps) over grid, LDS budget and narrow item size, one parameter at
+ * a time, keeps the fastest and returns it.  Channel state does not advance. */
+GPUD_API int gpud_serve_autotune(gpud_serve* s, const gpud_float2* iq_dev, int n, int reps, gpud_tune* best);
 /* RDS baseband samples (rate = out_rate / floor(out_rate / 19000)) and the
  * stereo lock of a GPUD_DEMOD_WFM_STEREO channel in a completed block:
  * *locked = this block's lock test (streamer.go:1634), *stereo = 1 when the
