@@ -395,6 +395,9 @@ __device__ __forceinline__ void argus_history_tail(const ArgusChanDesc& ch, cons
 __device__ __forceinline__ void argus_run_item(const ArgusChanDesc& ch, int kind, int m0,
                                                const float2* __restrict__ in, int n_new, float2* lds)
 {
+    // a work list built for the most outputs a chunk can give may hold one
+    // tile past this chunk's last output
+    if ((kind == ARGUS_ITEM_DENSE || kind == ARGUS_ITEM_NARROW) && m0 >= ch.n_out) return;
     if (kind == ARGUS_ITEM_DENSE) {
         if (ch.R == 4)
             argus_dense_tile<4>(ch, in, m0, lds);
@@ -609,9 +612,14 @@ int argus_fill_chan(ArgusChanDesc* ch, const float2* hist_rd, int H, float2* his
     ch->P = D < T ? D : T;
     ch->H = H;
     ch->e0 = D - phase_count - 1;
-    ch->n_out = (phase_count + n_new) / D;
     ch->keep = keep;
-    return argus_plan_chan(ch, tune);
+    // the tile shape depends on the chunk length only (planned for the most
+    // outputs a chunk of n_new samples can give), so a work list built for
+    // one chunk stays valid for the next of the same length
+    ch->n_out = (D - 1 + n_new) / D;
+    const int rc = argus_plan_chan(ch, tune);
+    ch->n_out = (phase_count + n_new) / D;
+    return rc;
 }
 
 extern "C" {

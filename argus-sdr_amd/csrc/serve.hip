@@ -300,6 +300,7 @@ GPUD_API int gpud_serve_open(gpud_serve* s, const gpud_channel_spec* spec)
     }
     s->bytes_in_use += fp;
     s->live.emplace(spec->channel_id, c);
+    s->epoch++;
     return 0;
 }
 
@@ -312,6 +313,7 @@ GPUD_API int gpud_serve_close(gpud_serve* s, int64_t channel_id)
     if (it == s->live.end()) return -1;
     s->closed.push_back(it->second);
     s->live.erase(it);
+    s->epoch++;
     reap(s);
     return 0;
 }
@@ -463,18 +465,21 @@ static int build_block(gpud_serve* s, ServeSlot& sl, int n, const float2* in, bo
     sl.iq_total = iq_total;
     sl.audio_total = audio_total;
 
-    // ---- tables: descriptors first (counts known), lists after ------------
-    const size_t off_chans = align_up(sizeof(ArgusServeHdr));
-    const size_t off_ddesc = align_up(off_chans + (size_t)nch * sizeof(ArgusChanDesc));
-    const size_t off_wdesc = align_up(off_ddesc + (size_t)nch * sizeof(ArgusDemodDesc));
-    const size_t prefix = off_wdesc + (size_t)n_wfm * sizeof(ArgusWfmDesc);
-    ARGUS_CHECK((hipError_t)grow_table(sl, prefix, 0));
-    ArgusChanDesc* hc = (ArgusChanDesc*)(sl.h_tab + off_chans);
-    ArgusDemodDesc* hd = (ArgusDemodDesc*)(sl.h_tab + off_ddesc);
-    ArgusWfmDesc* hw = (ArgusWfmDesc*)(sl.h_tab + off_wdesc);
-
-    s->items.clear();
-    s->item_pos.clear();
+    // ---- tables ------------------------------------------------------------
+    // Work items: reused while the channel set, block length and tune are
+    // unchanged (tiles are planned for the most outputs a block of n samples
+    // gives; the kernels skip tiles past a block's last output).
+    const bool reuse = sl.items_epoch == s->epoch && sl.items_n == n && sl.h_tab;
+    if (!reuse) {
+        s->items.clear();
+        s->item_pos.clear();
+    }
+    size_t items_bytes = reuse ? (size_t)sl.n_items * sizeof(ArgusWorkItem) : 0;
+    // descriptors are written straight into the table, after the items
+    // region, once its size is known; stage them in the channel order first
+    s->chans.resize((size_t)nch);
+    s->ddescs.resize((size_t)nch);
+    s->wdescs.resize((size_t)n_wfm);
     s->ditems.clear();
     s->witems.clear();
     size_t lds = 0;
@@ -484,14 +489,18 @@ static int build_block(gpud_serve* s, ServeSlot& sl, int n, const float2* in, bo
         const ServeOut& o = sl.outs[(size_t)ci];
         const int keep = std::min(c.hist_len + n, c.T - 1);
         const double inc = -2.0 * M_PI * c.spec.offset_hz / (double)s->sample_rate;
-        ArgusChanDesc& ch = hc[ci];
+        ArgusChanDesc& ch = s->chans[(size_t)ci];
         if (argus_fill_chan(&ch, c.d_hist[c.hist_sel], c.hist_len, c.d_hist[c.hist_sel ^ 1], keep, c.d_taps,
                             sl.d_iq + o.iq_off, c.decim, c.T, c.phase_count, n, c.nco_phase, inc, s->tune) != 0)
             return -1;
-        if (ch.n_out > 0) lds = std::max(lds, argus_item_lds_bytes(ch));
-        argus_append_items(ch, ci, n, s->items, s->item_pos);
+        if ((c.decim - 1 + n) / c.decim > 0) lds = std::max(lds, argus_item_lds_bytes(ch));
+        if (!reuse) {
+            ArgusChanDesc full = ch;
+            full.n_out = (c.decim - 1 + n) / c.decim;
+            argus_append_items(full, ci, n, s->items, s->item_pos);
+        }
 
-        ArgusDemodDesc& dd = hd[ci];
+        ArgusDemodDesc& dd = s->ddescs[(size_t)ci];
         dd.iq = sl.d_iq + o.iq_off;
         dd.audio = c.stereo ? sl.d_mono + o.mono_off : sl.d_audio + o.audio_off;
         dd.carry_rd = c.d_carry + c.carry_sel;
@@ -512,7 +521,7 @@ static int build_block(gpud_serve* s, ServeSlot& sl, int n, const float2* in, bo
                     s->ditems.push_back({ci, st, std::min(ARGUS_DEMOD_TILE, o.n_iq - st), 0});
             }
         }
-        if (c.stereo) wfm_describe(c, o, sl, hw[o.wfm_index], s->witems, commit);
+        if (c.stereo) wfm_describe(c, o, sl, s->wdescs[(size_t)o.wfm_index], s->witems, commit);
         if (commit) {
             // carry state forward (the kernels compute with the values above)
             c.phase_count = (c.phase_count + n) % c.decim;
@@ -528,24 +537,57 @@ static int build_block(gpud_serve* s, ServeSlot& sl, int n, const float2* in, bo
         }
         ci++;
     }
-    argus_order_items(s->items, s->item_pos, n);
-    const int n_items = (int)s->items.size(), n_ditems = (int)s->ditems.size(), n_witems = (int)s->witems.size();
-    const size_t off_ditems = align_up(prefix);
+    if (!reuse) {
+        argus_order_items(s->items, s->item_pos, n);
+        items_bytes = s->items.size() * sizeof(ArgusWorkItem);
+    }
+    const int n_items = reuse ? sl.n_items : (int)s->items.size();
+    const int n_ditems = (int)s->ditems.size(), n_witems = (int)s->witems.size();
+    const size_t off_hdr = align_up(items_bytes);
+    const size_t off_chans = align_up(off_hdr + sizeof(ArgusServeHdr));
+    const size_t off_ddesc = align_up(off_chans + (size_t)nch * sizeof(ArgusChanDesc));
+    const size_t off_wdesc = align_up(off_ddesc + (size_t)nch * sizeof(ArgusDemodDesc));
+    const size_t off_ditems = align_up(off_wdesc + (size_t)n_wfm * sizeof(ArgusWfmDesc));
     const size_t off_witems = align_up(off_ditems + (size_t)n_ditems * sizeof(ArgusDemodItem));
-    const size_t off_items = align_up(off_witems + (size_t)n_witems * sizeof(ArgusWfmItem));
-    const size_t total = off_items + (size_t)n_items * sizeof(ArgusWorkItem);
-    ARGUS_CHECK((hipError_t)grow_table(sl, total, prefix));
+    const size_t total = off_witems + (size_t)n_witems * sizeof(ArgusWfmItem);
+    const char* d_before = sl.d_tab;
+    ARGUS_CHECK((hipError_t)grow_table(sl, total, reuse ? items_bytes : 0));
+    if (sl.d_tab != d_before) sl.items_dirty = true;  // a new device table holds no items yet
+    if (!reuse) {
+        if (n_items) std::memcpy(sl.h_tab, s->items.data(), items_bytes);
+        sl.items_epoch = commit ? s->epoch : 0;
+        sl.items_n = n;
+        sl.n_items = n_items;
+        sl.items_dirty = true;
+    }
+    if (nch) {
+        std::memcpy(sl.h_tab + off_chans, s->chans.data(), (size_t)nch * sizeof(ArgusChanDesc));
+        std::memcpy(sl.h_tab + off_ddesc, s->ddescs.data(), (size_t)nch * sizeof(ArgusDemodDesc));
+    }
+    if (n_wfm) std::memcpy(sl.h_tab + off_wdesc, s->wdescs.data(), (size_t)n_wfm * sizeof(ArgusWfmDesc));
     if (n_ditems) std::memcpy(sl.h_tab + off_ditems, s->ditems.data(), (size_t)n_ditems * sizeof(ArgusDemodItem));
     if (n_witems) std::memcpy(sl.h_tab + off_witems, s->witems.data(), (size_t)n_witems * sizeof(ArgusWfmItem));
-    if (n_items) std::memcpy(sl.h_tab + off_items, s->items.data(), (size_t)n_items * sizeof(ArgusWorkItem));
-    ArgusServeHdr& hdr = *(ArgusServeHdr*)sl.h_tab;
+    ArgusServeHdr& hdr = *(ArgusServeHdr*)(sl.h_tab + off_hdr);
     const char* d = sl.d_tab;
-    hdr.x = ArgusExtractHdr{(const ArgusChanDesc*)(d + off_chans), (const ArgusWorkItem*)(d + off_items), in, n_items,
-                            n};
+    hdr.x = ArgusExtractHdr{(const ArgusChanDesc*)(d + off_chans), (const ArgusWorkItem*)d, in, n_items, n};
     hdr.d = ArgusDemodHdr{(const ArgusDemodDesc*)(d + off_ddesc), (const ArgusDemodItem*)(d + off_ditems), n_ditems, 0};
     hdr.w = ArgusWfmHdr{(const ArgusWfmDesc*)(d + off_wdesc), (const ArgusWfmItem*)(d + off_witems), n_wfm, n_witems};
     sl.tab_bytes = total;
+    sl.off_hdr = off_hdr;
     sl.lds = lds;
+    return 0;
+}
+
+// Upload the block's table: the work items only when they were rebuilt.
+static int upload_table(gpud_serve* s, ServeSlot& sl)
+{
+    if (sl.items_dirty) {
+        ARGUS_CHECK(hipMemcpyAsync(sl.d_tab, sl.h_tab, sl.tab_bytes, hipMemcpyHostToDevice, s->compute));
+        sl.items_dirty = false;
+    } else {
+        ARGUS_CHECK(hipMemcpyAsync(sl.d_tab + sl.off_hdr, sl.h_tab + sl.off_hdr, sl.tab_bytes - sl.off_hdr,
+                                   hipMemcpyHostToDevice, s->compute));
+    }
     return 0;
 }
 
@@ -553,7 +595,7 @@ static int build_block(gpud_serve* s, ServeSlot& sl, int n, const float2* in, bo
 // table header on the device.
 static int launch_eager(gpud_serve* s, ServeSlot& sl, bool wfm)
 {
-    const ArgusServeHdr* dh = (const ArgusServeHdr*)sl.d_tab;
+    const ArgusServeHdr* dh = (const ArgusServeHdr*)(sl.d_tab + sl.off_hdr);
     int err = argus_launch_extract_hdr(&dh->x, sl.lds, s->tune.grid, s->compute);
     if (err) return err;
     err = argus_launch_demod_hdr(&dh->d, s->compute);
@@ -568,7 +610,8 @@ static int launch_block(gpud_serve* s, ServeSlot& sl)
 {
     const bool wfm = sl.n_wfm > 0;
     if (!s->graph) return launch_eager(s, sl, wfm);
-    if (!sl.gexec || sl.g_tab != sl.d_tab || sl.g_lds != sl.lds || sl.g_grid != s->tune.grid ||
+    const char* hdr = sl.d_tab + sl.off_hdr;
+    if (!sl.gexec || sl.g_hdr != hdr || sl.g_lds != sl.lds || sl.g_grid != s->tune.grid ||
         sl.g_wfm != (int)wfm) {
         if (sl.gexec) {
             hipGraphExecDestroy(sl.gexec);
@@ -590,7 +633,7 @@ static int launch_block(gpud_serve* s, ServeSlot& sl)
             sl.gexec = nullptr;
             return (int)e;
         }
-        sl.g_tab = sl.d_tab;
+        sl.g_hdr = hdr;
         sl.g_lds = sl.lds;
         sl.g_grid = s->tune.grid;
         sl.g_wfm = (int)wfm;
@@ -622,7 +665,8 @@ static int serve_submit(gpud_serve* s, const gpud_float2* iq, int n, bool on_dev
     ARGUS_CHECK(hipEventRecord(sl.ev_in, s->copy));
     // ---- compute stream: one table copy, then the block's kernels ----------
     ARGUS_CHECK(hipStreamWaitEvent(s->compute, sl.ev_in, 0));
-    ARGUS_CHECK(hipMemcpyAsync(sl.d_tab, sl.h_tab, sl.tab_bytes, hipMemcpyHostToDevice, s->compute));
+    err = upload_table(s, sl);
+    if (err) return err;
     ARGUS_CHECK(hipEventRecord(sl.ev_kstart, s->compute));
     err = launch_block(s, sl);
     if (err) return err;
@@ -745,6 +789,7 @@ GPUD_API int gpud_serve_set_tune(gpud_serve* s, const gpud_tune* t)
 {
     if (!s || !t || !valid_tune(*t)) return -1;
     s->tune = ArgusTune{t->lds_entries, t->narrow_outputs, t->grid};
+    s->epoch++;
     return 0;
 }
 
@@ -767,13 +812,14 @@ GPUD_API int gpud_serve_autotune(gpud_serve* s, const gpud_float2* iq_dev, int n
     if (!err) err = (int)hipEventCreate(&e1);
     auto eval = [&](const ArgusTune& t, float* ms) -> int {
         s->tune = t;
+        s->epoch++;
         int rc = build_block(s, scratch, n, (const float2*)iq_dev, false);
         if (rc) return rc;
         rc = argus_extract_allow_lds(scratch.lds);
         if (rc) return rc;
         ARGUS_CHECK(hipMemcpyAsync(scratch.d_tab, scratch.h_tab, scratch.tab_bytes, hipMemcpyHostToDevice,
                                    s->compute));
-        const ArgusServeHdr* dh = (const ArgusServeHdr*)scratch.d_tab;
+        const ArgusServeHdr* dh = (const ArgusServeHdr*)(scratch.d_tab + scratch.off_hdr);
         rc = argus_launch_extract_hdr(&dh->x, scratch.lds, t.grid, s->compute);  // warm-up
         if (rc) return rc;
         float bestm = 1e30f;
@@ -793,12 +839,12 @@ GPUD_API int gpud_serve_autotune(gpud_serve* s, const gpud_float2* iq_dev, int n
     ArgusTune cur = s->tune;
     float cur_ms = 0.0f;
     if (!err) err = eval(cur, &cur_ms);
-    const int grids[] = {1024, 2048, 3072, 4096, 6144};
+    const int grids[] = {1024, 2048, 4096, 6144, 8192, 12288};
     const int ldss[] = {4096, 6144, 8192, 10240, 12288};
     const int narrows[] = {16, 32, 64};
     for (int pass = 0; pass < 3 && !err; pass++) {
         const int* vals = pass == 0 ? grids : pass == 1 ? ldss : narrows;
-        const int nv = pass == 0 ? 5 : pass == 1 ? 5 : 3;
+        const int nv = pass == 0 ? 6 : pass == 1 ? 5 : 3;
         for (int i = 0; i < nv && !err; i++) {
             ArgusTune t = cur;
             int& field = pass == 0 ? t.grid : pass == 1 ? t.lds_entries : t.narrow_outputs;
@@ -813,6 +859,7 @@ GPUD_API int gpud_serve_autotune(gpud_serve* s, const gpud_float2* iq_dev, int n
         }
     }
     s->tune = cur;
+    s->epoch++;
     hipStreamSynchronize(s->compute);
     slot_release(scratch);
     if (e0) hipEventDestroy(e0);

@@ -55,17 +55,25 @@ struct ServeSlot {
     float* h_audio = nullptr;
     size_t h_iq_cap = 0, h_audio_cap = 0;
     // The block's tables, one pinned host buffer mirrored by one device
-    // buffer and uploaded with one copy: ArgusServeHdr | channel descriptors |
-    // demod descriptors | stereo descriptors | demod items | stereo items |
-    // extraction work items.
+    // buffer: extraction work items | ArgusServeHdr | channel descriptors |
+    // demod descriptors | stereo descriptors | demod items | stereo items.
+    // The work items depend only on the channel set, the block length and
+    // the tune, so they are rebuilt and uploaded only when those change
+    // (items_epoch, items_n); everything from the header on is uploaded with
+    // every block.
     char* h_tab = nullptr;
     char* d_tab = nullptr;
     size_t tab_cap = 0;
     size_t tab_bytes = 0;
+    size_t off_hdr = 0;
     size_t lds = 0;
+    uint64_t items_epoch = 0;
+    int items_n = -1;
+    int n_items = 0;
+    bool items_dirty = true;
     // captured launch sequence of this slot and what it was captured for
     hipGraphExec_t gexec = nullptr;
-    const char* g_tab = nullptr;
+    const char* g_hdr = nullptr;
     size_t g_lds = 0;
     int g_grid = 0;
     int g_wfm = -1;
@@ -97,6 +105,7 @@ struct ArgusServeHdr {
 
 struct gpud_serve {
     ArgusTune tune = argus_default_tune();
+    uint64_t epoch = 1;  // bumped when the channel set or the tune changes
     bool graph = true;  // replay a captured per-slot graph of the block's launches
     int sample_rate = 0;
     int block_samples = 0;
@@ -112,6 +121,9 @@ struct gpud_serve {
     int64_t completed_ticket = 0;
     std::vector<ArgusWorkItem> items;
     std::vector<int> item_pos;
+    std::vector<ArgusChanDesc> chans;
+    std::vector<ArgusDemodDesc> ddescs;
+    std::vector<ArgusWfmDesc> wdescs;
     std::vector<ArgusDemodItem> ditems;
     std::vector<ArgusWfmItem> witems;
 };

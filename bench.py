@@ -114,6 +114,9 @@ def bench_serve(a, chans, blocks, lib, S):
         check("gpud_malloc", lib.gpud_malloc(C.byref(p), blk.nbytes))
         check("gpud_memcpy_h2d", lib.gpud_memcpy_h2d(p, blk.ctypes.data, blk.nbytes))
         d_blocks.append(p.value)
+    tune = s.Autotune(d_blocks[0], BLOCK, reps=3)
+    a.tune = {"lds_entries": tune.lds_entries, "narrow_outputs": tune.narrow_outputs, "grid": tune.grid,
+              "extract_ms_at_tune": round(tune.kernel_ms, 4)}
     s.SetDownload(False)
     pending = []
     for i in range(a.warmup):
@@ -159,6 +162,7 @@ def serve_block_times(chans, d_block, lib, S, blocks=50, warmup=5):
     s = S.Serve(FS, BLOCK, 256 << 30)
     for (cid, off, bw, rate, taps, dm) in chans:
         assert s.Open(S.ChannelSpec(cid, off, bw, rate, taps, dm, 0))
+    s.Autotune(d_block, BLOCK, reps=2)
     s.SetDownload(False)
     out = []
     for i in range(warmup + blocks):
@@ -223,10 +227,14 @@ def plain_block_times(chans, block, G, blocks=6):
 
 
 def bench_extract_kernel(a, chans, blocks, lib, G):
-    """Event-timed fused extraction kernel (runner API, device input)."""
+    """Event-timed fused extraction kernel (runner API, device input), with
+    the tile shape the service's autotuner picked."""
     from argus._lib import check
+    from argus.serve import gpud_tune
     jobs = [G.StreamingExtractJob(c[0], c[1], c[2], c[3], c[4], 0x9E3779B97F4A7C15 ^ c[0]) for c in chans]
     r = G.NewBatchRunner(BLOCK, FS)
+    t = gpud_tune(a.tune["lds_entries"], a.tune["narrow_outputs"], a.tune["grid"], 0, 0.0, 0.0)
+    check("gpud_runner_set_tune", lib.gpud_runner_set_tune(r._h, C.byref(t)))
     p = C.c_void_p()
     check("gpud_malloc", lib.gpud_malloc(C.byref(p), blocks[0].nbytes))
     check("gpud_memcpy_h2d", lib.gpud_memcpy_h2d(p, blocks[0].ctypes.data, blocks[0].nbytes))
@@ -538,6 +546,8 @@ def main() -> int:
         "p50_block_ms_host_to_host": round(percentile(lat, 50), 3),
         "p99_block_ms_host_to_host": round(percentile(lat, 99), 3),
         "throughput_channels": round(n_rt * BLOCK_S * 1e3 / ms_per_step, 1),
+        "tune": a.tune,
+        "graph": True,
         "hbm_gbps": round(achieved, 1),
         "capacity_sweep": table(sweep),
         "plain_per_signal": {"real_time_channels": n_plain, "sweep": table(plain_sweep),
