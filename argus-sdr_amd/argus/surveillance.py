@@ -5,19 +5,20 @@ internal/detector."""
 from __future__ import annotations
 
 import ctypes as C
+import math
 from dataclasses import dataclass
 
 import numpy as np
 
 from ._lib import check, lib, register
 
-CFAR_CA, CFAR_GOSCA, CFAR_CASO = 0, 1, 2
-_MODES = {"CA": CFAR_CA, "GOSCA": CFAR_GOSCA, "CASO": CFAR_CASO}
+CFAR_CA, CFAR_GOSCA, CFAR_CASO, CFAR_OS = 0, 1, 2, 3
+_MODES = {"CA": CFAR_CA, "GOSCA": CFAR_GOSCA, "CASO": CFAR_CASO, "OS": CFAR_OS}
 
 
 class gpud_cfar_params(C.Structure):
     _fields_ = [("mode", C.c_int32), ("guard", C.c_int32), ("train", C.c_int32), ("wrap", C.c_int32),
-                ("scale_db", C.c_double), ("hysteresis_db", C.c_double)]
+                ("scale_db", C.c_double), ("hysteresis_db", C.c_double), ("rank", C.c_int32), ("pad", C.c_int32)]
 
 
 class gpud_surv_run(C.Structure):
@@ -43,6 +44,7 @@ register("gpud_surv_destroy", I, [P])
 register("gpud_surv_stream", P, [P])
 register("gpud_surv_welch_device", I, [P, P, L64, I, I, I, P])
 register("gpud_surv_fft_device", I, [P, P, L64, I, I, P])
+register("gpud_surv_spectrum_device", I, [P, P, L64, I, I, P])
 register("gpud_surv_cfar_runs_device", I, [P, P, I, C.POINTER(gpud_cfar_params), P, P, I])
 register("gpud_surv_finish_frame", I, [P, I, I, P, I, C.POINTER(gpud_detector_params), P, I])
 register("gpud_surv_detect", I, [P, P, L64, I, I, I, C.POINTER(gpud_cfar_params), C.POINTER(gpud_detector_params),
@@ -93,9 +95,13 @@ class DeviceBuffer:
             pass
 
 
-def cfar_params(mode="CA", guard=4, train=16, scale_db=12.3748, wrap=True, hysteresis_db=3.0):
+def cfar_params(mode="CA", guard=4, train=16, scale_db=12.3748, wrap=True, hysteresis_db=3.0, rank=0):
+    """rank (OS only): 1-based rank in the 2*train cells; <= 0 takes the
+    detector's default round(0.75 * 2 * train) (detector.go:203-207)."""
+    if rank <= 0:
+        rank = max(1, int(math.floor(0.75 * 2 * train + 0.5)))
     return gpud_cfar_params(_MODES[mode] if isinstance(mode, str) else mode, guard, train, 1 if wrap else 0,
-                            scale_db, hysteresis_db)
+                            scale_db, hysteresis_db, rank, 0)
 
 
 def detector_params(threshold_db=-20.0, hysteresis_db=3.0, cfar_scale_db=12.3748, edge_margin_db=3.0,
@@ -139,6 +145,18 @@ class Surveillance:
                                                                  dout.ptr))
         self.sync()
         return dout.download(np.complex64, frames * self.n)
+
+    def SpectrumDB(self, iq, window=True) -> np.ndarray:
+        """spectrumFromIQWithPlan's GPU branch: window, FFT, SpectrumFromFFT
+        (dB, fftshifted) and NaN/Inf sanitising, for each n-sample frame."""
+        iq = np.ascontiguousarray(iq, np.complex64)
+        frames = len(iq) // self.n
+        din = DeviceBuffer(iq.nbytes).upload(iq)
+        dout = DeviceBuffer(4 * self.n * frames)
+        check("gpud_surv_spectrum_device", lib().gpud_surv_spectrum_device(self._h, din.ptr, self.n, frames,
+                                                                           1 if window else 0, dout.ptr))
+        self.sync()
+        return dout.download(np.float32, frames * self.n).reshape(frames, self.n)
 
     def WelchDevice(self, iq_dev: int, frame_step: int, seg_hop: int, n_segs: int, n_frames: int, psd_dev: int):
         check("gpud_surv_welch_device", lib().gpud_surv_welch_device(self._h, iq_dev, frame_step, seg_hop, n_segs,

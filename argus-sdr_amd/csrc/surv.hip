@@ -189,7 +189,9 @@ __global__ __launch_bounds__(R* R) void argus_welch_kernel(const float2* __restr
 #pragma unroll
     for (int ka = 0; ka < R; ++ka) {
         const int q = (ka + R / 2) % R;
-        xs[kc + (R + 1) * kb + PL * q] = 10.0f * log10f(acc[ka] * inv_segs + 1e-24f);
+        const float db = 10.0f * log10f(acc[ka] * inv_segs + 1e-24f);
+        // pipeline_runtime.go:391-397 sanitizeSpectrum: NaN / Inf -> -200 dB
+        xs[kc + (R + 1) * kb + PL * q] = isfinite(db) ? db : -200.0f;
     }
     __syncthreads();
     float* o = out_db + (int64_t)blockIdx.x * N;
@@ -239,6 +241,23 @@ __global__ __launch_bounds__(R* R) void argus_fft_kernel(const float2* __restric
     for (int q = 0; q < R; ++q) o[t + TH * q] = make_float2(v[q].x, xs[jc + (R + 1) * jb + PL * q]);
 }
 
+// Single-FFT spectrum in dB (fft.go:58-73 SpectrumFromFFT): bin i of the
+// output is FFT bin (i + n/2) mod n, 20 log10(|X| / n + 1e-12), with NaN /
+// Inf mapped to -200 dB (pipeline_runtime.go:391-397).
+__global__ __launch_bounds__(256) void argus_spectrum_db_kernel(const float2* __restrict__ X, int n, int64_t total,
+                                                                float* __restrict__ out)
+{
+    const double inv_n = 1.0 / (double)n;
+    for (int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; g < total; g += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t f = g / n;
+        const int i = (int)(g - f * n);
+        const float2 v = X[f * n + (i + n / 2) % n];
+        const double mag = hypot((double)v.x, (double)v.y) * inv_n;
+        const double db = 20.0 * log10(mag + 1e-12);
+        out[g] = isfinite(db) ? (float)db : -200.0f;
+    }
+}
+
 // ---------------------------------------------------------------------------
 // CFAR + run extraction, one workgroup of 1024 threads per frame.
 // ---------------------------------------------------------------------------
@@ -259,9 +278,46 @@ __device__ __forceinline__ double cfar_combine(int mode, double left, double rig
     return lm < rm ? lm : rm;  // CASO
 }
 
+// OS-CFAR (os.go:31-80): the rank-th smallest (0-based rk) of the 2*train
+// training cells in dB.  The window is a multiset: cell c is the answer when
+// fewer than rk+1 cells are below it and at least rk+1 are at or below it,
+// which is the sorted window's element rk whatever the order of ties.
+template <typename Cell>
+__device__ __forceinline__ double os_select(Cell cell, int train, int rk)
+{
+    const int total = 2 * train;
+    double best = 0.0;
+    for (int c = 0; c < total; ++c) {
+        const double v = cell(c);
+        int less = 0, leq = 0;
+        for (int m = 0; m < total; ++m) {
+            const double u = cell(m);
+            less += u < v;
+            leq += u <= v;
+        }
+        if (less <= rk && rk < leq) best = v;
+    }
+    return best;
+}
+
+__device__ __forceinline__ int os_rank(const gpud_cfar_params& p)
+{
+    int rk = p.rank - 1;
+    if (rk < 0) rk = 0;
+    if (rk >= 2 * p.train) rk = 2 * p.train - 1;
+    return rk;
+}
+
 // Threshold of one bin straight from the dB spectrum (used for peak bins).
 __device__ double cfar_threshold_at(const float* __restrict__ db, int n, int i, const gpud_cfar_params& p)
 {
+    if (p.mode == GPUD_CFAR_OS) {
+        auto cell = [&](int m) {
+            const int k = m / 2 + 1;
+            return (double)db[wrap_bin((m & 1) ? i + p.guard + k : i - p.guard - k, n, p.wrap)];
+        };
+        return os_select(cell, p.train, os_rank(p)) + p.scale_db;
+    }
     double left = 0.0, right = 0.0;
     for (int k = 1; k <= p.train; ++k) {
         left += pow(10.0, (double)db[wrap_bin(i - p.guard - k, n, p.wrap)] / 10.0);
@@ -286,21 +342,31 @@ __global__ __launch_bounds__(kCfarThreads) void argus_cfar_runs_kernel(const flo
     for (int w = t; w < n / 16; w += kCfarThreads) status[w] = 0u;
     for (int t0 = 0; t0 < n; t0 += tile) {
         __syncthreads();  // previous tile's readers are done with lin
+        const bool os = p.mode == GPUD_CFAR_OS;  // OS ranks the dB values themselves
         for (int i = t; i < tile + 2 * halo; i += kCfarThreads) {
             const int b = wrap_bin(t0 - halo + i, n, p.wrap);
-            lin[i] = pow(10.0, (double)db[b] / 10.0);
+            lin[i] = os ? (double)db[b] : pow(10.0, (double)db[b] / 10.0);
         }
         __syncthreads();
         // consecutive lanes take consecutive bins: conflict-free LDS reads.
         // Window sums are direct (k = 1..train in order), the threshold of
         // ca.go:50-59 / gosca.go / caso.go without the running-sum drift.
         for (int j = t; j < tile; j += kCfarThreads) {
-            double left = 0.0, right = 0.0;
-            for (int k = 1; k <= p.train; ++k) {
-                left += lin[j - p.guard - k + halo];
-                right += lin[j + p.guard + k + halo];
+            double thr;
+            if (os) {
+                auto cell = [&](int m) {
+                    const int k = m / 2 + 1;
+                    return lin[((m & 1) ? j + p.guard + k : j - p.guard - k) + halo];
+                };
+                thr = os_select(cell, p.train, os_rank(p)) + p.scale_db;
+            } else {
+                double left = 0.0, right = 0.0;
+                for (int k = 1; k <= p.train; ++k) {
+                    left += lin[j - p.guard - k + halo];
+                    right += lin[j + p.guard + k + halo];
+                }
+                thr = 10.0 * log10(cfar_combine(p.mode, left, right, p.train)) + p.scale_db;
             }
-            const double thr = 10.0 * log10(cfar_combine(p.mode, left, right, p.train)) + p.scale_db;
             const int b = t0 + j;
             const double v = (double)db[b];
             const uint32_t code = v >= thr ? 2u : (v < thr - p.hysteresis_db ? 0u : 1u);
@@ -453,6 +519,7 @@ GPUD_API int gpud_surv_destroy(gpud_surv* s)
     if (!s) return 0;
     hipStreamSynchronize(s->stream);
     hipFree(s->d_win);
+    if (s->d_scratch) hipFree(s->d_scratch);
     hipStreamDestroy(s->stream);
     delete s;
     return 0;
@@ -500,6 +567,33 @@ GPUD_API int gpud_surv_fft_device(gpud_surv* s, const gpud_float2* in_dev, int64
     return (int)hipGetLastError();
 }
 
+// Single-FFT dB spectra of n_frames frames (frame f at f*frame_step): the
+// GPU path of spectrumFromIQWithPlan (pipeline_runtime.go:315-341) --
+// window (1 = Hann on the device), forward FFT, SpectrumFromFFT and the
+// NaN/Inf sanitising -- as float32 [n_frames][n_fft], fftshifted.
+GPUD_API int gpud_surv_spectrum_device(gpud_surv* s, const gpud_float2* in_dev, int64_t frame_step, int n_frames,
+                                       int window, float* db_dev)
+{
+    if (!s || !in_dev || !db_dev) return -1;
+    if (n_frames <= 0) return 0;
+    const size_t need = (size_t)n_frames * (size_t)s->n;
+    if (s->scratch_cap < need) {
+        if (s->d_scratch) hipFree(s->d_scratch);
+        s->d_scratch = nullptr;
+        s->scratch_cap = 0;
+        ARGUS_CHECK(hipMalloc((void**)&s->d_scratch, need * sizeof(float2)));
+        s->scratch_cap = need;
+    }
+    int err = gpud_surv_fft_device(s, in_dev, frame_step, n_frames, window, (gpud_float2*)s->d_scratch);
+    if (err) return err;
+    const int64_t total = (int64_t)need;
+    const int grid = (int)std::min<int64_t>((total + 255) / 256, 4096);
+    argus_clear_error();
+    hipLaunchKernelGGL(argus_spectrum_db_kernel, dim3(grid), dim3(256), 0, s->stream, (const float2*)s->d_scratch,
+                       s->n, total, db_dev);
+    return (int)hipGetLastError();
+}
+
 // CA / GOSCA / CASO CFAR (internal/cfar/ca.go:22-68, gosca.go:22-78,
 // caso.go:16-70) on each frame of a dB spectrum, followed by the threshold
 // crossing with hysteresis of detector.go:436-480: runs (first, last, peak
@@ -513,7 +607,9 @@ GPUD_API int gpud_surv_cfar_runs_device(gpud_surv* s, const float* psd_dev, int 
     if (!s || !psd_dev || !params || !runs_dev || !run_counts_dev || max_runs <= 0) return -1;
     const gpud_cfar_params p = *params;
     if (p.train <= 0 || p.guard < 0 || p.guard + p.train > 2048) return -1;
-    if (p.mode != GPUD_CFAR_CA && p.mode != GPUD_CFAR_GOSCA && p.mode != GPUD_CFAR_CASO) return -1;
+    if (p.mode != GPUD_CFAR_CA && p.mode != GPUD_CFAR_GOSCA && p.mode != GPUD_CFAR_CASO && p.mode != GPUD_CFAR_OS)
+        return -1;
+    if (p.mode == GPUD_CFAR_OS && p.train > 128) return -1;
     if (s->n % kCfarThreads != 0) return -1;
     if (n_frames <= 0) return 0;
     size_t lds = argus_cfar_lds_bytes(s->n, p);

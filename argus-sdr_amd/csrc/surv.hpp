@@ -7,4 +7,6 @@ struct gpud_surv {
     int sample_rate = 0;
     hipStream_t stream = nullptr;
     float* d_win = nullptr;  // symmetric Hann, n floats
+    float2* d_scratch = nullptr;  // complex spectra of gpud_surv_spectrum_device
+    size_t scratch_cap = 0;       // complex samples
 };

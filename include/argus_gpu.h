@@ -328,7 +328,7 @@ GPUD_API int gpud_rds_decoder_groups(gpud_rds_decoder* d, uint16_t* groups4, int
 /* ------------------------------------------------------------------ */
 typedef struct gpud_surv gpud_surv;
 
-enum { GPUD_CFAR_CA = 0, GPUD_CFAR_GOSCA = 1, GPUD_CFAR_CASO = 2 };
+enum { GPUD_CFAR_CA = 0, GPUD_CFAR_GOSCA = 1, GPUD_CFAR_CASO = 2, GPUD_CFAR_OS = 3 /* os.go:31-80 */ };
 
 typedef struct gpud_cfar_params {
     int32_t mode;          /* GPUD_CFAR_* (cfar/types.go:6-14) */
@@ -337,6 +337,8 @@ typedef struct gpud_cfar_params {
     int32_t wrap;          /* 1 = wrap-around indexing, 0 = clamp to edge */
     double  scale_db;      /* <= 0 -> 6 dB (cfar.go:13-15) */
     double  hysteresis_db; /* run closes below threshold - hysteresis (detector.go:445) */
+    int32_t rank;          /* OS only: 1-based rank in the 2*train cells, clamped (os.go:14-22); train <= 128 */
+    int32_t pad;
 } gpud_cfar_params;
 
 /* One threshold run of a frame (detector.go:436-480, before edge expansion). */
@@ -384,7 +386,14 @@ GPUD_API int gpud_surv_welch_device(gpud_surv* s, const gpud_float2* iq_dev, int
  * Hann applied on the device. */
 GPUD_API int gpud_surv_fft_device(gpud_surv* s, const gpud_float2* in_dev, int64_t frame_step, int n_frames,
                                   int window, gpud_float2* out_dev);
-/* ca.go / gosca.go / caso.go + detector.go:436-480 per frame. */
+/* pipeline_runtime.go:315-341 spectrumFromIQWithPlan (GPU branch) +
+ * fft.go:58-73 SpectrumFromFFT + pipeline_runtime.go:391-397 sanitizeSpectrum:
+ * window (1 = Hann), FFT, 20 log10(|X|/N + 1e-12), fftshift, NaN/Inf ->
+ * -200 dB; float32 [n_frames][n_fft].  (The Welch output is sanitised the
+ * same way.) */
+GPUD_API int gpud_surv_spectrum_device(gpud_surv* s, const gpud_float2* in_dev, int64_t frame_step, int n_frames,
+                                       int window, float* db_dev);
+/* ca.go / gosca.go / caso.go / os.go + detector.go:436-480 per frame. */
 GPUD_API int gpud_surv_cfar_runs_device(gpud_surv* s, const float* psd_dev, int n_frames,
                                         const gpud_cfar_params* params, gpud_surv_run* runs_dev,
                                         int* run_counts_dev, int max_runs);

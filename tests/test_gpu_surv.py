@@ -63,20 +63,20 @@ def _same_lists(gpu_sigs, orc_sigs):
     return a == b
 
 
-@pytest.mark.parametrize("mode", ["CA", "GOSCA", "CASO"])
+@pytest.mark.parametrize("mode", ["CA", "GOSCA", "CASO", "OS"])
 def test_detection_list_identical_on_same_spectrum(surv, mode):
     from argus.synth import surveillance_scene
     n, hop, segs, frames = 32768, 16384, 8, 4
     total = (frames - 1) * segs * hop + (segs - 1) * hop + n
     iq, _ = surveillance_scene(total, FS, n_carriers=40, seed=11)
     s = surv.Surveillance(n, FS)
-    cfar = surv.cfar_params(mode, guard=4, train=16, scale_db=12.3748, wrap=True, hysteresis_db=3.0)
+    cfar = surv.cfar_params(mode, guard=4, train=16, scale_db=12.3748, wrap=True, hysteresis_db=3.0, rank=24)
     det = surv.detector_params(cfar_scale_db=12.3748, center_hz=100e6)
     lists, psd = s.Detect(iq, segs, hop, segs * hop, frames, cfar, det)
     near = 0
     for f in range(frames):
         spec = psd[f].astype(np.float64)
-        thr = O.CFARThresholds(mode, spec, 4, 16, 12.3748, wrap=True)
+        thr = O.CFARThresholds(mode, spec, 4, 16, 12.3748, wrap=True, rank=24)
         near += int(np.sum(np.abs(spec - thr) < 1e-9) + np.sum(np.abs(spec - (thr - 3.0)) < 1e-9))
         want = O.DetectSignals(spec, thr, sample_rate=FS, center_hz=100e6, cfar_scale_db=12.3748)
         print(f"frame {f}: gpu {len(lists[f])} signals, oracle {len(want)}", flush=True)
@@ -97,4 +97,43 @@ def test_cfar_runs_known_plateau(surv):
     counts, runs = s.CFARRuns(spec, surv.cfar_params("CA", guard=2, train=16, scale_db=6.0, wrap=True))
     got = [(r.first_bin, r.last_bin, r.peak_bin) for r in runs[0]]
     assert got == [(1000, 1010, 1000), (3000, 3001, 3000)], got
+    s.Close()
+
+
+@pytest.mark.parametrize("n", [4096, 32768])
+def test_single_fft_spectrum_db_matches_fp64_oracle(surv, n):
+    """fft.go:24-73 + pipeline_runtime.go:315-341: Hann, FFT, 20 log10(|X|/N +
+    1e-12), fftshift; against the oracle's FP64 SpectrumWithPlan."""
+    rng = np.random.default_rng(12)
+    t = np.arange(2 * n)
+    iq = (0.5 * np.exp(2j * np.pi * 0.123 * t) + 0.01 * (rng.standard_normal(2 * n)
+                                                          + 1j * rng.standard_normal(2 * n))).astype(np.complex64)
+    s = surv.Surveillance(n, FS)
+    got = s.SpectrumDB(iq, window=True)
+    w = O.Hann(n)
+    for f in range(2):
+        want = O.Spectrum(iq[f * n:(f + 1) * n], w)
+        # FP32 FFT: error bounded relative to the peak magnitude (about log2 N
+        # roundings of 6e-8), so dB error grows as a bin falls below the peak
+        lg, lw = 10 ** (got[f] / 20.0), 10 ** (want / 20.0)
+        assert np.max(np.abs(lg - lw)) <= 2e-6 * lw.max(), float(np.max(np.abs(lg - lw)) / lw.max())
+        near = want > want.max() - 40.0
+        assert np.max(np.abs(got[f][near] - want[near])) < 1e-3
+        assert int(np.argmax(got[f])) == int(np.argmax(want))
+    s.Close()
+
+
+def test_spectra_sanitise_non_finite_input(surv):
+    """pipeline_runtime.go:391-397: NaN / Inf bins read -200 dB."""
+    n = 4096
+    s = surv.Surveillance(n, FS)
+    iq = np.zeros(n, np.complex64)
+    iq[7] = np.nan
+    db = s.SpectrumDB(iq, window=False)
+    assert np.all(db == -200.0)
+    hop = n // 2
+    iq2 = np.zeros(3 * hop + n, np.complex64)
+    iq2[5] = np.inf
+    psd = s.WelchPSD(iq2, 2, hop, 2 * hop, 1)
+    assert np.all(np.isfinite(psd)) and np.all(psd[0] == -200.0)
     s.Close()
