@@ -545,6 +545,7 @@ class orc_rds_decoder(C.Structure):
 register("orc_pll_coefficients", None, [D, D, I, PD_, PD_])
 register("orc_stereo_init", None, [C.POINTER(orc_stereo_state), I])
 register("orc_stereo_decode", I, [C.POINTER(orc_stereo_state), P, I, P])
+register("orc_stereo_decode_batch", I, [P, I, I, P])
 register("orc_stereo_pilot_present", I, [P, I, I, PD_])
 register("orc_rds_syndrome", C.c_uint16, [C.c_uint32])
 register("orc_rds_init", None, [C.POINTER(orc_rds_decoder)])
@@ -575,6 +576,14 @@ class StereoDecoder:
     @property
     def Enabled(self):
         return bool(self.st.enabled)
+
+
+def StereoDecodeBatch(mono, sample_rate):
+    """wfm_batch_stereo.go:76-157 wfmStereoDecodeBatch -> (L/R interleaved, locked)."""
+    mono = np.ascontiguousarray(mono, np.float32)
+    out = np.zeros(2 * len(mono), np.float32)
+    locked = lib().orc_stereo_decode_batch(_p(mono), len(mono), sample_rate, _p(out))
+    return out, bool(locked)
 
 
 def StereoPilotPresent(iq, sample_rate):

@@ -126,6 +126,64 @@ int orc_stereo_decode(orc_stereo_state* st, const float* mono, int n, float* out
     return locked;
 }
 
+/*
+ * internal/recorder/wfm_batch_stereo.go:76-157 wfmStereoDecodeBatch: the
+ * stateless decoder of event recordings.  Every filter starts from zero and
+ * the PLL from 19 kHz; lock = pilotPower/totalPower > 0.003 and |f - 19 kHz|
+ * < 250 Hz and the block's mean |err| < 0.35.  out receives 2n interleaved
+ * L, R (the matrix output, whatever the lock); returns the lock flag.
+ */
+int orc_stereo_decode_batch(const float* mono, int n, int sample_rate, float* out)
+{
+    if (n <= 0 || sample_rate <= 0) return 0;
+    orc_stereo_state st;
+    orc_stereo_init(&st, sample_rate);
+    float* lpr = (float*)malloc(sizeof(float) * (size_t)n * 5);
+    float* bpf = lpr + n;
+    float* lr = lpr + 2 * n;
+    float* w1 = lpr + 3 * n;
+    float* w2 = lpr + 4 * n;
+    orc_stateful_fir_real(st.lpf.taps, st.lpf.nt, st.lpf.delay, mono, n, lpr);
+    orc_stateful_fir_real(st.bp_hi.taps, st.bp_hi.nt, st.bp_hi.delay, mono, n, w1);
+    orc_stateful_fir_real(st.bp_lo.taps, st.bp_lo.nt, st.bp_lo.delay, mono, n, w2);
+    for (int i = 0; i < n; i++) bpf[i] = w1[i] - w2[i];
+    orc_stateful_fir_real(st.pilot_hi.taps, st.pilot_hi.nt, st.pilot_hi.delay, mono, n, w1);
+    orc_stateful_fir_real(st.pilot_lo.taps, st.pilot_lo.nt, st.pilot_lo.delay, mono, n, w2);
+    for (int i = 0; i < n; i++) w1[i] = w1[i] - w2[i];
+    double phase = 0, freq = st.freq, is = 0, qs = 0;
+    const double alpha = st.alpha, beta = st.beta, lpa = st.lp_alpha;
+    const double min_f = 2 * ORC_PI * 17000 / (double)sample_rate;
+    const double max_f = 2 * ORC_PI * 21000 / (double)sample_rate;
+    double pilot_power = 0, total_power = 0, err_sum = 0;
+    for (int i = 0; i < n; i++) {
+        double p = (double)w1[i];
+        double sp = sin(phase), cp = cos(phase);
+        is += lpa * (p * cp - is);
+        qs += lpa * (p * -sp - qs);
+        double err = atan2(qs, is);
+        freq += beta * err;
+        if (freq < min_f) freq = min_f;
+        else if (freq > max_f) freq = max_f;
+        phase += freq + alpha * err;
+        if (phase > 2 * ORC_PI) phase -= 2 * ORC_PI;
+        else if (phase < 0) phase += 2 * ORC_PI;
+        total_power += (double)mono[i] * (double)mono[i];
+        pilot_power += p * p;
+        err_sum += fabs(err);
+        lr[i] = bpf[i] * (float)(2 * sin(2 * phase));
+    }
+    orc_stateful_fir_real(st.lr_lpf.taps, st.lr_lpf.nt, st.lr_lpf.delay, lr, n, w2);
+    double ratio = total_power > 0 ? pilot_power / total_power : 0;
+    double freq_hz = freq * (double)sample_rate / (2 * ORC_PI);
+    int locked = ratio > 0.003 && fabs(freq_hz - 19000) < 250 && err_sum / (double)n < 0.35;
+    for (int i = 0; i < n; i++) {
+        out[2 * i] = 0.5f * (lpr[i] + w2[i]);
+        out[2 * i + 1] = 0.5f * (lpr[i] - w2[i]);
+    }
+    free(lpr);
+    return locked;
+}
+
 /* ------------------------------------------------------------ pilot presence */
 
 /* internal/classifier/pll.go:178-193 goertzelMagnitude (FP64 samples) */

@@ -57,6 +57,7 @@ int orc_rds_stream_process(orc_rds_stream* s, const float* disc, int n, float* o
 void orc_pll_coefficients(double loop_bw, double damping, int sample_rate, double* alpha, double* beta);
 void orc_stereo_init(orc_stereo_state* st, int sample_rate);
 int orc_stereo_decode(orc_stereo_state* st, const float* mono, int n, float* out);
+int orc_stereo_decode_batch(const float* mono, int n, int sample_rate, float* out);
 int orc_stereo_pilot_present(const float* iq, int n, int sample_rate, double* ratio_out);
 uint16_t orc_rds_syndrome(uint32_t raw);
 void orc_rds_init(orc_rds_decoder* d);

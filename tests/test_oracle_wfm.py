@@ -239,3 +239,32 @@ def test_stereo_pilot_uses_the_last_half_second():
     mono = np.exp(1j * 75 * np.sin(2 * np.pi * 1000 * t)).astype(np.complex64)
     assert O.StereoPilotPresent(st.block(fs), fs)[0]
     assert not O.StereoPilotPresent(np.concatenate([st.block(fs), mono]), fs)[0]
+
+
+def test_batch_stereo_decoder_locks_and_separates():
+    """wfm_batch_stereo.go:76-157: a station's MPX locks in one call (pilot
+    ratio > 0.003) and an L-only program comes out on one channel (the
+    separation criterion of streamer_test.go:8-67); noise does not lock."""
+    fs = 250_000
+    n = int(0.4 * fs)
+    t = np.arange(n) / fs
+    left = 0.5 * np.sin(2 * np.pi * 1000 * t)
+    mpx = (0.5 * left + 0.5 * left * np.sin(2 * np.pi * 38000 * t) + 0.09 * np.sin(2 * np.pi * 19000 * t)).astype(np.float32)
+    out, locked = O.StereoDecodeBatch(mpx, fs)
+    assert locked
+    lr = out.reshape(-1, 2)[n // 2:]
+    L, R = np.std(lr[:, 0]), np.std(lr[:, 1])
+    assert max(L, R) >= 1.3 * min(L, R) and max(L, R) >= 0.05
+    rng = np.random.default_rng(8)
+    _, locked = O.StereoDecodeBatch((rng.standard_normal(n) * 0.3).astype(np.float32), fs)
+    assert not locked
+
+
+def test_batch_stereo_equals_stateful_decoder_on_first_block():
+    """Both decoders start from the same zero state: on one block their L/R
+    outputs agree exactly (only the lock rule differs)."""
+    fs = 250_000
+    mono = _station_mono(fs, 0.1, seed=9)
+    a, _ = O.StereoDecodeBatch(mono, fs)
+    b, _ = O.StereoDecoder(fs).Decode(mono)
+    np.testing.assert_array_equal(a, b)
