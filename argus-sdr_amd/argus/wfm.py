@@ -73,3 +73,65 @@ class RDSDecoder:
         if total > 0:
             lib().gpud_rds_decoder_groups(self._h, out.ctypes.data, total)
         return out
+
+
+register("gpud_wfm_stereo_batch", I, [P, P, P, I, I, P, P, P])
+
+
+def StereoDecodeBatch(monos, rates, mono_fallback: bool = False):
+    """wfmStereoDecodeBatch (internal/recorder/wfm_batch_stereo.go:76-157) of
+    several recordings in one call: returns ([L/R interleaved float32 per
+    recording], [locked per recording]).  With mono_fallback an unlocked
+    recording comes back as its raw discriminator on both channels, as
+    demodWFMStereoBatchAudio (lines 42-49) does."""
+    from .surveillance import DeviceBuffer
+    monos = [np.ascontiguousarray(m, np.float32) for m in monos]
+    rates = np.ascontiguousarray(rates, np.int32)
+    if len(rates) != len(monos):
+        raise ValueError("one rate per recording")
+    offs = np.zeros(len(monos) + 1, np.int64)
+    offs[1:] = np.cumsum([len(m) for m in monos])
+    total = int(offs[-1])
+    flat = np.concatenate(monos) if total else np.zeros(1, np.float32)
+    din = DeviceBuffer(4 * max(total, 1)).upload(flat)
+    dout = DeviceBuffer(8 * max(total, 1))
+    locked = np.zeros(len(monos), np.int32)
+    check("gpud_wfm_stereo_batch",
+          lib().gpud_wfm_stereo_batch(din.ptr, offs.ctypes.data, rates.ctypes.data, len(monos),
+                                      1 if mono_fallback else 0, dout.ptr, locked.ctypes.data, None))
+    out = dout.download(np.float32, 2 * total) if total else np.zeros(0, np.float32)
+    audio = []
+    for i, m in enumerate(monos):
+        if len(m) == 0 or rates[i] <= 0:
+            audio.append(np.zeros(0, np.float32))
+        else:
+            audio.append(out[2 * offs[i]:2 * offs[i + 1]].copy())
+    return audio, [bool(x) for x in locked]
+
+
+register("gpud_stereo_pilot_present", I, [P, P, P, I, P, P, P, P, P])
+
+
+def StereoPilotPresent(iqs, rates):
+    """StereoPilotPresent / estimateWFMPilot (internal/classifier/pll.go:82-131)
+    of several baseband IQ slices in one call: returns (present, ratio,
+    offset_hz, mags[n, 201]) per slice."""
+    from .surveillance import DeviceBuffer
+    iqs = [np.ascontiguousarray(x, np.complex64) for x in iqs]
+    rates = np.ascontiguousarray(rates, np.int32)
+    if len(rates) != len(iqs):
+        raise ValueError("one rate per slice")
+    offs = np.zeros(len(iqs) + 1, np.int64)
+    offs[1:] = np.cumsum([len(x) for x in iqs])
+    total = int(offs[-1])
+    flat = np.concatenate(iqs) if total else np.zeros(1, np.complex64)
+    din = DeviceBuffer(8 * max(total, 1)).upload(flat)
+    k = len(iqs)
+    present = np.zeros(k, np.int32)
+    ratio = np.zeros(k, np.float64)
+    off = np.zeros(k, np.float64)
+    mags = np.zeros((k, 201), np.float64)
+    check("gpud_stereo_pilot_present",
+          lib().gpud_stereo_pilot_present(din.ptr, offs.ctypes.data, rates.ctypes.data, k, present.ctypes.data,
+                                          ratio.ctypes.data, off.ctypes.data, mags.ctypes.data, None))
+    return present.astype(bool), ratio, off, mags

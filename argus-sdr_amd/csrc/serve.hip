@@ -141,16 +141,7 @@ static hipError_t wfm_channel_init(ServeChannel& c, int rate)
     ArgusWfmState st{};
     st.freq = 2.0 * M_PI * 19000.0 / (double)rate;
     std::memcpy(blob.data(), &st, sizeof st);
-    const double cut[6] = {15000.0, 53000.0, 23000.0, 21000.0, 17000.0, 7500.0};
-    double h[6][ARGUS_WFM_TAPS + 1];
-    for (int i = 0; i < 6; i++) argus_lowpass_fir(cut[i], rate, ARGUS_WFM_TAPS, h[i], false);
-    float* taps = (float*)(blob.data() + kWfmTaps);
-    for (int k = 0; k < ARGUS_WFM_TAPS; k++) {
-        taps[ARGUS_WFM_LPF * ARGUS_WFM_TAPS + k] = (float)h[0][k];
-        taps[ARGUS_WFM_BPF * ARGUS_WFM_TAPS + k] = (float)(h[1][k] - h[2][k]);
-        taps[ARGUS_WFM_PILOT * ARGUS_WFM_TAPS + k] = (float)(h[3][k] - h[4][k]);
-        taps[ARGUS_WFM_RDS * ARGUS_WFM_TAPS + k] = (float)h[5][k];
-    }
+    argus_wfm_design_taps(rate, (float*)(blob.data() + kWfmTaps));
     c.rds_decim = std::max(1, rate / 19000);
     c.rds_count = 0;
     c.wfm_sel = 0;
@@ -329,16 +320,6 @@ GPUD_API int gpud_serve_info(gpud_serve* s, int* live_channels, uint64_t* bytes_
 
 }  // extern "C"
 
-// streamer.go:1496-1506 pllCoefficients: type-II loop gains.
-static void pll_coefficients(double loop_bw, double damping, int rate, double* alpha, double* beta)
-{
-    const double bl = loop_bw / (double)rate;
-    const double theta = bl / (damping + 0.25 / damping);
-    const double d = 1 + 2 * damping * theta + theta * theta;
-    *alpha = (4 * damping * theta) / d;
-    *beta = (4 * theta * theta) / d;
-}
-
 // Stereo-stage descriptor and work items of one channel for this block; with
 // commit the channel's host-side carry (delay-line half, RDS sample count)
 // advances.
@@ -366,15 +347,13 @@ static void wfm_describe(ServeChannel& c, const ServeOut& o, ServeSlot& sl, Argu
     const uint64_t num0 = ((uint64_t)((c.rds_count + 1) % rate) * 57000u) % r;
     wd.mix0 = frac_to_fix(num0, r);
     wd.mix_inc = frac_to_fix(57000u % r, r);
-    pll_coefficients(50.0, 0.707, rate, &wd.alpha, &wd.beta);
-    wd.lpa = 1.0 - std::exp(-2.0 * M_PI * 200.0 / (double)rate);
-    wd.min_f = 2.0 * M_PI * 17000.0 / (double)rate;
-    wd.max_f = 2.0 * M_PI * 21000.0 / (double)rate;
-    wd.rate = (double)rate;
+    argus_wfm_loop_constants(rate, wd);
     wd.n = o.n_mono;
     wd.rds_first = (int)((D - c.rds_count % D) % D);
     wd.rds_decim = D;
     wd.n_rds = o.n_rds;
+    wd.batch = ARGUS_WFM_STREAM;
+    wd.pad = 0;
     for (int st = 0; st < o.n_mono; st += ARGUS_WFM_TILE) items.push_back({o.wfm_index, st, ARGUS_WFM_ITEM_TILE, 0});
     if (o.n_mono > 0) items.push_back({o.wfm_index, 0, ARGUS_WFM_ITEM_TAIL, 0});
     if (commit) {

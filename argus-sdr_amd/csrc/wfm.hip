@@ -22,6 +22,7 @@
 // the PLL state and loop arithmetic are FP64 as in the reference, and the
 // oscillators use the hardware turn-based sine/cosine.
 #include "wfm.hpp"
+#include "runner.hpp"
 
 namespace {
 
@@ -128,6 +129,7 @@ __device__ __forceinline__ void wfm_pll_one(const ArgusWfmDesc& d)
 {
     __shared__ float pl[ARGUS_WFM_TILE];
     __shared__ float tt[ARGUS_WFM_TILE];
+    __shared__ double tp[64];
     const int n = d.n;
     ArgusWfmState* sp = d.st;
     if (n <= 0) {  // no samples: state and lock counters unchanged (stereoDecodeStateful returns early)
@@ -144,6 +146,14 @@ __device__ __forceinline__ void wfm_pll_one(const ArgusWfmDesc& d)
     double phase = sp->phase, freq = sp->freq, is = sp->i_state, qs = sp->q_state;
     const double alpha = d.alpha, beta = d.beta, lpa = d.lpa, min_f = d.min_f, max_f = d.max_f;
     double pp = 0.0, es = 0.0;
+    if (d.batch) {  // total power of the discriminator output, for the batch lock test
+        double acc = 0.0;
+        for (int j = threadIdx.x; j < n; j += blockDim.x) {
+            const double m = (double)d.mono[j];
+            acc += m * m;
+        }
+        tp[threadIdx.x] = acc;
+    }
     const float* pilot = d.scratch + 2 * n;
     float* t2 = d.scratch + 3 * n;
     for (int i0 = 0; i0 < n; i0 += ARGUS_WFM_TILE) {
@@ -177,7 +187,20 @@ __device__ __forceinline__ void wfm_pll_one(const ArgusWfmDesc& d)
         for (int j = threadIdx.x; j < cnt; j += blockDim.x) t2[i0 + j] = tt[j];
         __syncthreads();
     }
-    if (threadIdx.x == 0) {
+    if (threadIdx.x == 0 && d.batch) {
+        // wfm_batch_stereo.go:139-145: one decision for the whole recording
+        double total = 0.0;
+        for (int j = 0; j < (int)blockDim.x; j++) total += tp[j];
+        const double ratio = total > 0.0 ? pp / total : 0.0;
+        const double f_hz = freq * d.rate / kTwoPi;
+        const int locked = ratio > 0.003 && fabs(f_hz - 19000.0) < 250.0 && es / (double)n < 0.35;
+        ArgusWfmBlock b;
+        b.locked = locked;
+        b.stereo = d.batch == ARGUS_WFM_BATCH_MATRIX ? 1 : locked;
+        b.fb_rd = 0;
+        b.fb_wr = 0;
+        *d.blk = b;
+    } else if (threadIdx.x == 0) {
         ArgusWfmState st = *sp;
         st.phase = phase;
         st.freq = freq;
@@ -227,6 +250,14 @@ __device__ __forceinline__ void wfm_post(const ArgusWfmDesc* __restrict__ descs,
         }
         const int i0 = it.start;
         const int cnt = min(ARGUS_WFM_TILE, n - i0);
+        if (d.batch && !b.stereo) {  // unlocked recording: dual mono of the raw discriminator
+            for (int i = threadIdx.x; i < cnt; i += blockDim.x) {
+                const float m = d.mono[i0 + i];
+                d.audio[2 * (i0 + i)] = m;
+                d.audio[2 * (i0 + i) + 1] = m;
+            }
+            continue;
+        }
         const int span = cnt + ARGUS_WFM_HIST;
         // the L-R low-pass and the fallback low-pass are both LowpassFIR(15000)
         for (int j = threadIdx.x; j < ARGUS_WFM_TAPS; j += blockDim.x) hs[j] = d.taps[ARGUS_WFM_LPF * ARGUS_WFM_TAPS + j];
@@ -293,6 +324,39 @@ __global__ __launch_bounds__(256) void argus_wfm_post_hdr_kernel(const ArgusWfmH
 {
     const ArgusWfmHdr h = *hdr;
     wfm_post(h.descs, h.items, h.n_items);
+}
+
+// L+R low-pass LowpassFIR(15k); L-R band-pass LowpassFIR(53k) - LowpassFIR(23k);
+// pilot band-pass LowpassFIR(21k) - LowpassFIR(17k) (streamer.go:1532-1549,
+// wfm_batch_stereo.go:81-98); RDS low-pass LowpassFIR(7.5k) (fm.go:254).
+void argus_wfm_design_taps(int rate, float* taps)
+{
+    const double cut[6] = {15000.0, 53000.0, 23000.0, 21000.0, 17000.0, 7500.0};
+    double h[6][ARGUS_WFM_TAPS + 1];
+    for (int i = 0; i < 6; i++) argus_lowpass_fir(cut[i], rate, ARGUS_WFM_TAPS, h[i], false);
+    for (int k = 0; k < ARGUS_WFM_TAPS; k++) {
+        taps[ARGUS_WFM_LPF * ARGUS_WFM_TAPS + k] = (float)h[0][k];
+        taps[ARGUS_WFM_BPF * ARGUS_WFM_TAPS + k] = (float)(h[1][k] - h[2][k]);
+        taps[ARGUS_WFM_PILOT * ARGUS_WFM_TAPS + k] = (float)(h[3][k] - h[4][k]);
+        taps[ARGUS_WFM_RDS * ARGUS_WFM_TAPS + k] = (float)h[5][k];
+    }
+}
+
+// pllCoefficients(50, 0.707) (streamer.go:1496-1506; the batch decoder's
+// pllCoefficientsBatch, wfm_batch_stereo.go:159-169, is the same), the 200 Hz
+// arm low-pass and the 17-21 kHz frequency clamp.
+void argus_wfm_loop_constants(int rate, ArgusWfmDesc& wd)
+{
+    const double loop_bw = 50.0, damping = 0.707;
+    const double bl = loop_bw / (double)rate;
+    const double theta = bl / (damping + 0.25 / damping);
+    const double dd = 1 + 2 * damping * theta + theta * theta;
+    wd.alpha = (4 * damping * theta) / dd;
+    wd.beta = (4 * theta * theta) / dd;
+    wd.lpa = 1.0 - std::exp(-2.0 * M_PI * 200.0 / (double)rate);
+    wd.min_f = 2.0 * M_PI * 17000.0 / (double)rate;
+    wd.max_f = 2.0 * M_PI * 21000.0 / (double)rate;
+    wd.rate = (double)rate;
 }
 
 int argus_launch_wfm_hdr(const ArgusWfmHdr* d_hdr, hipStream_t stream)

@@ -17,6 +17,13 @@
 //        or the mono fallback low-pass when stereo is not enabled.
 // Filter delay lines are carried as the last ARGUS_WFM_HIST inputs of each
 // filter in ping-pong pairs; a tail item per channel writes the new ones.
+//
+// Batch mode (ArgusWfmDesc::batch != 0) is the one-shot decoder of a whole
+// recording (internal/recorder/wfm_batch_stereo.go:76-157): zero delay lines,
+// a fresh PLL, no tail items, the lock test pilot power / total power >
+// 0.003 (no hysteresis), and always the L/R matrix output -- or, with
+// ARGUS_WFM_BATCH_FALLBACK, the raw discriminator duplicated on both channels
+// when the block does not lock (demodWFMStereoBatchAudio, 42-49).
 #pragma once
 #include "common.hpp"
 
@@ -64,7 +71,11 @@ struct ArgusWfmDesc {
     int rds_first;             // first sample index of this block the RDS decimator keeps
     int rds_decim;
     int n_rds;
+    int batch;                 // 0: streaming; ARGUS_WFM_BATCH_MATRIX or ARGUS_WFM_BATCH_FALLBACK
+    int pad;
 };
+
+enum { ARGUS_WFM_STREAM = 0, ARGUS_WFM_BATCH_MATRIX = 1, ARGUS_WFM_BATCH_FALLBACK = 2 };
 
 struct ArgusWfmItem {
     int chan;
@@ -81,6 +92,11 @@ struct ArgusWfmHdr {
 };
 
 #define ARGUS_WFM_GRID 256
+
+// Host: the four tap sets at `rate` (ARGUS_WFM_NTAPSETS x ARGUS_WFM_TAPS
+// floats) and the PLL loop constants and clamps of a descriptor.
+void argus_wfm_design_taps(int rate, float* taps);
+void argus_wfm_loop_constants(int rate, ArgusWfmDesc& wd);
 
 int argus_launch_wfm(const ArgusWfmDesc* d_desc, int n_chans, const ArgusWfmItem* d_items, int n_items,
                      hipStream_t stream);

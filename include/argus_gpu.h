@@ -321,6 +321,45 @@ GPUD_API int gpud_rds_decoder_state(gpud_rds_decoder* d, uint16_t* pi, char* ps9
 GPUD_API int gpud_rds_decoder_groups(gpud_rds_decoder* d, uint16_t* groups4, int cap);
 
 /* ------------------------------------------------------------------ */
+/* Batch WFM stereo: wfmStereoDecodeBatch (internal/recorder/           */
+/* wfm_batch_stereo.go:76-157) over n_stations whole recordings at once. */
+/* offsets holds n_stations + 1 sample offsets: station i's             */
+/* discriminator output is mono_dev[offsets[i] .. offsets[i+1]) at       */
+/* rates[i]; its L/R interleaved audio goes to                           */
+/* out_dev[2*offsets[i] .. 2*offsets[i+1]).  Each station is decoded     */
+/* from zero filter state and a fresh PLL, and locked[i] is the          */
+/* recording's lock test (pilot/total power > 0.003, |f - 19 kHz| <      */
+/* 250 Hz, mean |err| < 0.35).  mono_fallback = 0 writes the L/R matrix  */
+/* whatever the lock, as wfmStereoDecodeBatch does; mono_fallback = 1    */
+/* writes the raw discriminator on both channels for unlocked stations,  */
+/* as its caller demodWFMStereoBatchAudio does (lines 42-49).  A station */
+/* with no samples or rate <= 0 gets locked = 0 and no output.  Device   */
+/* pointers; workspace is taken from the stream-ordered pool; returns    */
+/* after the stream has finished.  -1 bad arguments.                     */
+/* ------------------------------------------------------------------ */
+GPUD_API int gpud_wfm_stereo_batch(const float* mono_dev, const int64_t* offsets, const int32_t* rates,
+                                   int n_stations, int mono_fallback, float* out_dev, int32_t* locked,
+                                   gpud_stream_handle stream);
+
+/* ------------------------------------------------------------------ */
+/* Stereo pilot presence: StereoPilotPresent / estimateWFMPilot         */
+/* (internal/classifier/pll.go:82-131) for n_signals WFM baseband IQ    */
+/* slices at once.  offsets holds n_signals + 1 sample offsets into      */
+/* iq_dev (device); signal i is analysed on its last rates[i]/2 samples: */
+/* FP64 FM discriminator, |DFT| at 19 kHz +- 1 kHz in 10 Hz steps (the   */
+/* 201 goertzelMagnitude bins), present[i] = median > 0 and max/median > */
+/* 4.  ratio[i] = max/median (0 if the median is 0), offset_hz[i] = the  */
+/* first maximum's frequency - 19 kHz, mags201 the 201 magnitudes per    */
+/* signal in frequency order; each of the three may be NULL.  Signals    */
+/* below 40 kHz or with fewer than 2 samples report 0s.  Workspace from  */
+/* the stream-ordered pool; returns after the stream has finished.       */
+/* -1 bad arguments.                                                     */
+/* ------------------------------------------------------------------ */
+GPUD_API int gpud_stereo_pilot_present(const gpud_float2* iq_dev, const int64_t* offsets, const int32_t* rates,
+                                       int n_signals, int32_t* present, double* ratio, double* offset_hz,
+                                       double* mags201, gpud_stream_handle stream);
+
+/* ------------------------------------------------------------------ */
 /* Surveillance: Welch PSD, FFT, CFAR and the detection list.           */
 /* Successors of gpufft.Engine (internal/fft/gpufft/gpu.go:33-91), the  */
 /* CPU WelchPSD (internal/fft/welch.go:15-66), internal/cfar and the    */

@@ -547,6 +547,7 @@ register("orc_stereo_init", None, [C.POINTER(orc_stereo_state), I])
 register("orc_stereo_decode", I, [C.POINTER(orc_stereo_state), P, I, P])
 register("orc_stereo_decode_batch", I, [P, I, I, P])
 register("orc_stereo_pilot_present", I, [P, I, I, PD_])
+register("orc_stereo_pilot_estimate", I, [P, I, I, PD_, PD_, P])
 register("orc_rds_syndrome", C.c_uint16, [C.c_uint32])
 register("orc_rds_init", None, [C.POINTER(orc_rds_decoder)])
 register("orc_rds_decode", I, [C.POINTER(orc_rds_decoder), P, I, I])
@@ -591,6 +592,17 @@ def StereoPilotPresent(iq, sample_rate):
     r = C.c_double()
     ok = lib().orc_stereo_pilot_present(_p(iq), len(iq), sample_rate, C.byref(r))
     return bool(ok), r.value
+
+
+def StereoPilotEstimate(iq, sample_rate):
+    """(present, ratio, offset_hz, 201 Goertzel magnitudes) of estimateWFMPilot
+    on the last sample_rate/2 samples (pll.go:82-131)."""
+    iq = c64(iq)
+    r, o = C.c_double(), C.c_double()
+    mags = np.zeros(201, np.float64)
+    ok = lib().orc_stereo_pilot_estimate(_p(iq), len(iq), sample_rate, C.byref(r), C.byref(o),
+                                         mags.ctypes.data)
+    return bool(ok), r.value, o.value, mags
 
 
 def crcSyndrome(raw):

@@ -211,10 +211,15 @@ static int cmp_d(const void* a, const void* b)
 /* internal/classifier/pll.go:82-131 StereoPilotPresent/estimateWFMPilot: the
  * FP64 FM discriminator (fmDemod, pll.go:160-176) of the last <= fs/2
  * samples, Goertzel magnitudes at 19 kHz +- 1 kHz in 10 Hz steps, present if
- * max / median > 4 (pilotLockRatio); rates below 40 kHz are never stereo. */
-int orc_stereo_pilot_present(const float* iq, int n, int sample_rate, double* ratio_out)
+ * max / median > 4 (pilotLockRatio); rates below 40 kHz are never stereo.
+ * offset_out = best frequency - 19 kHz (the first maximum, pll.go:107-114);
+ * mags_out receives the 201 magnitudes in frequency order. */
+int orc_stereo_pilot_estimate(const float* iq, int n, int sample_rate, double* ratio_out, double* offset_out,
+                              double* mags_out)
 {
     if (ratio_out) *ratio_out = 0;
+    if (offset_out) *offset_out = 0;
+    if (mags_out) memset(mags_out, 0, sizeof(double) * 201);
     if (sample_rate <= 0) return 0;
     int maxn = sample_rate / 2;
     if (maxn > 0 && n > maxn) {
@@ -229,18 +234,28 @@ int orc_stereo_pilot_present(const float* iq, int n, int sample_rate, double* ra
         d[i - 1] = atan2(pr * ci - pi * cr, pr * cr + pi * ci);
     }
     double mags[201];
-    double best = 0;
+    double best = 0, best_f = 19000.0;
     int k = 0;
     for (double off = -1000.0; off <= 1000.0; off += 10.0, k++) {
         mags[k] = goertzel(d, n - 1, 19000.0 + off, sample_rate);
-        if (mags[k] > best) best = mags[k];
+        if (mags[k] > best) {
+            best = mags[k];
+            best_f = 19000.0 + off;
+        }
     }
+    if (mags_out) memcpy(mags_out, mags, sizeof mags);
     qsort(mags, (size_t)k, sizeof(double), cmp_d);
     double med = mags[k / 2];
     free(d);
     double ratio = med > 0 ? best / med : 0;
     if (ratio_out) *ratio_out = ratio;
+    if (offset_out) *offset_out = best_f - 19000.0;
     return med > 0 && ratio > 4;
+}
+
+int orc_stereo_pilot_present(const float* iq, int n, int sample_rate, double* ratio_out)
+{
+    return orc_stereo_pilot_estimate(iq, n, sample_rate, ratio_out, NULL, NULL);
 }
 
 /* ------------------------------------------------------------ RDS decode */

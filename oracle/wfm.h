@@ -59,6 +59,8 @@ void orc_stereo_init(orc_stereo_state* st, int sample_rate);
 int orc_stereo_decode(orc_stereo_state* st, const float* mono, int n, float* out);
 int orc_stereo_decode_batch(const float* mono, int n, int sample_rate, float* out);
 int orc_stereo_pilot_present(const float* iq, int n, int sample_rate, double* ratio_out);
+int orc_stereo_pilot_estimate(const float* iq, int n, int sample_rate, double* ratio_out, double* offset_out,
+                              double* mags_out);
 uint16_t orc_rds_syndrome(uint32_t raw);
 void orc_rds_init(orc_rds_decoder* d);
 int orc_rds_decode(orc_rds_decoder* d, const float* samples, int n, int sample_rate);

@@ -241,6 +241,27 @@ def test_stereo_pilot_uses_the_last_half_second():
     assert not O.StereoPilotPresent(np.concatenate([st.block(fs), mono]), fs)[0]
 
 
+def test_pilot_goertzel_bins_are_dft_magnitudes():
+    """The 201 Goertzel magnitudes (pll.go:178-193) equal |sum d[i] e^{-j 2pi f i / fs}|
+    of the FP64 discriminator, computed here as a direct DFT; the reported
+    offset is the bin of a pilot placed 300 Hz above 19 kHz."""
+    fs = 200_000
+    n = fs // 2
+    t = np.arange(n) / fs
+    rng = np.random.default_rng(12)
+    mpx = 0.3 * np.sin(2 * np.pi * 1200 * t) + 0.08 * np.sin(2 * np.pi * 19300 * t) + 0.01 * rng.standard_normal(n)
+    iq = np.exp(1j * 2 * np.pi * 75000 * np.cumsum(mpx) / fs).astype(np.complex64)
+    ok, ratio, off, mags = O.StereoPilotEstimate(iq, fs)
+    x = iq.astype(np.complex128)
+    d = np.angle(np.conj(x[:-1]) * x[1:])
+    freqs = 19000.0 + 10.0 * np.arange(-100, 101)
+    ph = np.exp(-2j * np.pi * np.outer(freqs, np.arange(n - 1)) / fs)
+    want = np.abs(ph @ d)
+    np.testing.assert_allclose(mags, want, rtol=1e-7, atol=1e-9 * want.max())
+    assert ok and off == 300.0
+    assert ratio == pytest.approx(want.max() / np.sort(want)[100], rel=1e-7)
+
+
 def test_batch_stereo_decoder_locks_and_separates():
     """wfm_batch_stereo.go:76-157: a station's MPX locks in one call (pilot
     ratio > 0.003) and an L-only program comes out on one channel (the
