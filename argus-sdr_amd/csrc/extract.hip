@@ -386,6 +386,80 @@ __device__ __forceinline__ void argus_narrow_group(const ArgusChanDesc& ch, cons
     __syncthreads();
 }
 
+// Narrow group with modulated taps: four outputs per wavefront, one per
+// 16-lane row; lane r of a row takes taps k = r + 16 i.  Waves whose padded
+// windows (T rounded up to 64, zero taps beyond T) lie wholly in the new
+// input run a branch-free loop with four loads in flight per lane; otherwise
+// each tap checks its sample, and samples before the chunk (the history,
+// stored already shifted) are filtered with the real taps and added after
+// the rotation.  The row sum lands in every lane of the row; lane 15 writes.
+__device__ __forceinline__ void argus_narrow_mod(const ArgusChanDesc& ch, const float2* __restrict__ in, int m0,
+                                                 float2* lds)
+{
+    const int T = ch.T, D = ch.D, L = ch.L;
+    const int TP = (T + 63) & ~63;
+    float2* gs = lds;
+    float* hs = (float*)(lds + TP);
+    const bool hist = ch.e0 + m0 * D - (T - 1) < 0;  // a window of this item reaches into the history
+    for (int i = threadIdx.x; i < TP; i += blockDim.x) gs[i] = i < T ? ch.ctaps[i] : make_float2(0.0f, 0.0f);
+    if (hist)
+        for (int i = threadIdx.x; i < T; i += blockDim.x) hs[i] = L == 1 ? ch.taps[i] : ch.taps[(i % D) * L + i / D];
+    __syncthreads();
+    const int lane = threadIdx.x & 63;
+    const int row = lane >> 4, rl = lane & 15;
+    const int nw = blockDim.x >> 6;
+    const int mend = min(m0 + ch.M, ch.n_out);
+    for (int mb = m0 + 4 * (int)(threadIdx.x >> 6); mb < mend; mb += 4 * nw) {
+        const int m = mb + row;
+        const int e = ch.e0 + m * D;
+        float ar = 0.0f, ai = 0.0f, br = 0.0f, bi = 0.0f;
+        if (ch.e0 + mb * D - (TP - 1) >= 0) {  // same for the whole wave: its lowest output decides
+            if (m < mend) {
+                const float2* src = in + (e - rl);
+                const float2* gp = gs + rl;
+                for (int k = 0; k < TP; k += 64) {
+                    const float2 x0 = src[-k], x1 = src[-k - 16], x2 = src[-k - 32], x3 = src[-k - 48];
+                    const float2 g0 = gp[k], g1 = gp[k + 16], g2 = gp[k + 32], g3 = gp[k + 48];
+#define ARGUS_CMAC(G, X)                                                        \
+    ar = fmaf(G.x, X.x, ar); ar = fmaf(-G.y, X.y, ar);                          \
+    ai = fmaf(G.x, X.y, ai); ai = fmaf(G.y, X.x, ai);
+                    ARGUS_CMAC(g0, x0)
+                    ARGUS_CMAC(g1, x1)
+                    ARGUS_CMAC(g2, x2)
+                    ARGUS_CMAC(g3, x3)
+                }
+            }
+        } else if (m < mend) {
+            for (int k = rl; k < T; k += 16) {
+                const int n = e - k;
+                if (n >= 0) {
+                    const float2 x = in[n];
+                    const float2 g = gs[k];
+                    ARGUS_CMAC(g, x)
+                } else if (ch.H + n >= 0) {
+                    const float2 v = ch.hist_rd[ch.H + n];
+                    br = fmaf(hs[k], v.x, br);
+                    bi = fmaf(hs[k], v.y, bi);
+                }
+            }
+#undef ARGUS_CMAC
+        }
+        ar = argus_group_sum<16>(ar);
+        ai = argus_group_sum<16>(ai);
+        if (hist) {
+            br = argus_group_sum<16>(br);
+            bi = argus_group_sum<16>(bi);
+        }
+        if (rl == 15 && m < mend) {
+            float sn, cs;
+            argus_nco(ch.phase0 + ch.inc * (uint64_t)(int64_t)e, sn, cs);
+            const float2 y = argus_rot(make_float2(ar, ai), sn, cs);
+            ch.out[m] = make_float2(y.x + br, y.y + bi);
+        }
+    }
+    __syncthreads();
+}
+
 __device__ __forceinline__ void argus_history_tail(const ArgusChanDesc& ch, const float2* __restrict__ in, int n_new)
 {
     for (int i = threadIdx.x; i < ch.keep; i += blockDim.x)
@@ -404,8 +478,12 @@ __device__ __forceinline__ void argus_run_item(const ArgusChanDesc& ch, int kind
         else
             argus_dense_tile<1>(ch, in, m0, lds);
     }
-    else if (kind == ARGUS_ITEM_NARROW)
-        argus_narrow_group(ch, in, m0, (float*)lds);
+    else if (kind == ARGUS_ITEM_NARROW) {
+        if (ch.ctaps)
+            argus_narrow_mod(ch, in, m0, lds);
+        else
+            argus_narrow_group(ch, in, m0, (float*)lds);
+    }
     else if (kind == ARGUS_ITEM_HISTORY)
         argus_history_tail(ch, in, n_new);
 }
@@ -453,7 +531,9 @@ __global__ __launch_bounds__(ARGUS_EXTRACT_THREADS) void argus_extract_one_kerne
 
 size_t argus_item_lds_bytes(const ArgusChanDesc& ch)
 {
-    return ch.narrow ? (size_t)ch.T * sizeof(float) : argus_dense_lds_bytes(ch);
+    if (!ch.narrow) return argus_dense_lds_bytes(ch);
+    return ch.ctaps ? (size_t)((ch.T + 63) & ~63) * sizeof(float2) + (size_t)ch.T * sizeof(float)
+                    : (size_t)ch.T * sizeof(float);
 }
 
 // Dynamic LDS above 64 KiB must be allowed per kernel once.
@@ -497,8 +577,10 @@ int argus_extract_allow_lds(size_t bytes) { return bytes > 64 * 1024 ? argus_all
 size_t argus_tune_max_lds(const ArgusTune& tune, int max_taps)
 {
     // dense: P*PS <= lds_entries words of samples plus P*L <= T + P - 1 < 2*T taps;
-    // narrow: T taps
-    return (size_t)tune.lds_entries * sizeof(float2) + (size_t)2 * max_taps * sizeof(float);
+    // narrow: T (+ 63 padding) complex and T real taps
+    const size_t dense = (size_t)tune.lds_entries * sizeof(float2) + (size_t)2 * max_taps * sizeof(float);
+    const size_t narrow = (size_t)(max_taps + 63) * sizeof(float2) + (size_t)max_taps * sizeof(float);
+    return std::max(dense, narrow);
 }
 
 int argus_launch_extract_hdr(const ArgusExtractHdr* d_hdr, size_t lds_bytes, int grid, hipStream_t stream)
@@ -535,9 +617,13 @@ void argus_append_items(const ArgusChanDesc& ch, int chan_index, int n_new, std:
                         std::vector<int>& pos)
 {
     const int tiles = ch.n_out > 0 ? (ch.n_out + ch.M - 1) / ch.M : 0;
-    // cost in lane-operations: NCO rotations (~24 each) plus taps (~2 each)
-    const long rot = ch.narrow ? (long)ch.M * ch.T : (long)(ch.M - 1) * ch.D + ch.T;
-    const long cost = rot * 24 + (long)ch.M * ch.T * 2;
+    // cost in lane-operations: NCO rotations (~24 each) plus taps (~2 each,
+    // ~6 for a complex tap with its load)
+    long cost;
+    if (ch.narrow && ch.ctaps)
+        cost = (long)ch.M * (((ch.T + 63) & ~63) * 6 + 64);
+    else
+        cost = (ch.narrow ? (long)ch.M * ch.T : (long)(ch.M - 1) * ch.D + ch.T) * 24 + (long)ch.M * ch.T * 2;
     const int kind = ch.narrow ? ARGUS_ITEM_NARROW : ARGUS_ITEM_DENSE;
     for (int t = 0; t < tiles; t++) {
         items.push_back({chan_index, t * ch.M, kind, (int)std::min<long>(cost, 1L << 30)});
@@ -603,6 +689,7 @@ int argus_fill_chan(ArgusChanDesc* ch, const float2* hist_rd, int H, float2* his
     ch->hist_rd = hist_rd;
     ch->hist_wr = hist_wr;
     ch->taps = taps;
+    ch->ctaps = nullptr;
     ch->out = out;
     ch->phase0 = argus_rad_to_fix(phase_start);
     ch->inc = argus_rad_to_fix(phase_inc);
@@ -620,6 +707,16 @@ int argus_fill_chan(ArgusChanDesc* ch, const float2* hist_rd, int H, float2* his
     const int rc = argus_plan_chan(ch, tune);
     ch->n_out = (phase_count + n_new) / D;
     return rc;
+}
+
+void argus_modulated_taps(const float* poly, int D, int T, uint64_t inc, float2* out)
+{
+    const int L = (T + D - 1) / D;
+    for (int k = 0; k < T; k++) {
+        const double h = poly[(size_t)(k % D) * L + k / D];
+        const double a = argus_fix_to_rad(inc * (uint64_t)k);
+        out[k] = make_float2((float)(h * std::cos(a)), (float)(-h * std::sin(a)));
+    }
 }
 
 extern "C" {

@@ -18,12 +18,17 @@
 //    consecutive words.
 //
 //  narrow group (D >= T): each output needs only the T samples before its
-//    emission point and windows do not overlap, so a wavefront computes one
-//    output at a time: lanes take taps k = lane, lane+64, ..., shift their
-//    sample with the NCO straight from global memory (coalesced) and the
-//    wave reduces.  No LDS staging, no redundant rotations.  Also the
-//    fallback when a dense tile does not fit the LDS budget (very large D
-//    with T > D), at T/D redundant rotations.
+//    emission point and windows do not overlap, so no input is staged.
+//    With modulated taps (ctaps, g[k] = h[k] e^{-j 2pi inc k}) the raw input
+//    is filtered directly, since
+//        y[m] = sum_k h[k] s[e - k] = e^{j 2pi (phase0 + inc e)} sum_k g[k] x[e - k]
+//    for s[n] = x[n] e^{j 2pi (phase0 + inc n)}: one rotation per output
+//    instead of one per tap.  A wavefront computes four outputs, one per
+//    16-lane row (lanes over taps, coalesced loads, row reduction by DPP).
+//    Without ctaps (the single-signal entry point, whose caller supplies only
+//    the polyphase taps) a wavefront computes one output and rotates every
+//    sample.  Also the fallback when a dense tile does not fit the LDS budget
+//    (very large D with T > D).
 //
 // A third kind writes the channel's new shifted history (the newest `keep`
 // samples of history ++ shifted input) into the other half of a ping-pong
@@ -36,6 +41,7 @@ struct ArgusChanDesc {
     const float2* hist_rd;  // old shifted history, H samples, oldest first
     float2* hist_wr;        // new shifted history, keep samples
     const float* taps;      // phase-major polyphase taps (D planes of L floats)
+    const float2* ctaps;    // modulated taps, natural order (T), or null
     float2* out;            // n_out baseband outputs of this chunk
     uint64_t phase0;        // NCO phase of new sample 0 (turns, 2^64 == 1)
     uint64_t inc;           // NCO increment per sample (turns, 2^64 == 1)
@@ -165,6 +171,12 @@ static inline size_t argus_dense_lds_bytes(const ArgusChanDesc& ch)
     return (size_t)ch.P * ch.PS * 8 + (size_t)ch.P * ch.L * 4;
 }
 
+// Modulated taps g[k] = h[k] e^{-j 2pi inc k} (inc in turns, 2^64 == 1) in
+// natural tap order, from the phase-major layout; FP64, rounded once.
+void argus_modulated_taps(const float* poly, int D, int T, uint64_t inc, float2* out);
+
+// argus_fill_chan leaves ctaps null; callers that keep modulated taps for
+// the channel set it afterwards.
 int argus_fill_chan(ArgusChanDesc* ch, const float2* hist_rd, int H, float2* hist_wr, int keep, const float* taps,
                     float2* out, int D, int T, int phase_count, int n_new, double phase_start, double phase_inc,
                     const ArgusTune& tune);

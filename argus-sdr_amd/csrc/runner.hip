@@ -134,10 +134,14 @@ int ensure_host(T** p, size_t* cap, size_t need)
 void argus_signal_release(ArgusSignal& s)
 {
     if (s.d_taps) hipFree(s.d_taps);
+    if (s.d_ctaps) hipFree(s.d_ctaps);
     if (s.d_hist[0]) hipFree(s.d_hist[0]);
     if (s.d_hist[1]) hipFree(s.d_hist[1]);
     if (s.d_out) hipFree(s.d_out);
     s.d_taps = nullptr;
+    s.d_ctaps = nullptr;
+    s.d_ctaps_len = 0;
+    s.ctaps_valid = false;
     s.d_hist[0] = s.d_hist[1] = nullptr;
     s.d_out = nullptr;
     s.out_cap = 0;
@@ -207,6 +211,26 @@ int argus_runner_prepare_signal(gpud_runner* r, const gpud_streaming_job& job, A
         ARGUS_CHECK(hipMalloc((void**)&s->d_hist[1], cap * sizeof(float2)));
         s->hist_sel = 0;
         if (s->hist_len > T - 1) s->hist_len = 0;
+    }
+    // modulated taps follow the taps and the NCO increment (the increment the
+    // batched launch derives from the same offset)
+    const uint64_t inc = argus_rad_to_fix(-2.0 * M_PI * job.offset_hz / (double)r->sample_rate);
+    if (s->taps_dirty || !s->ctaps_valid || s->ctaps_inc != inc) {
+        std::vector<float2> ct((size_t)T);
+        argus_modulated_taps(s->poly.data(), decim, T, inc, ct.data());
+        if (s->d_ctaps && s->d_ctaps_len != ct.size()) {
+            hipFree(s->d_ctaps);
+            s->d_ctaps = nullptr;
+        }
+        if (!s->d_ctaps) {
+            ARGUS_CHECK(hipMalloc((void**)&s->d_ctaps, ct.size() * sizeof(float2)));
+            s->d_ctaps_len = ct.size();
+        }
+        ARGUS_CHECK(hipMemcpyAsync(s->d_ctaps, ct.data(), ct.size() * sizeof(float2), hipMemcpyHostToDevice,
+                                   r->stream));
+        ARGUS_CHECK(hipStreamSynchronize(r->stream));  // ct is pageable host memory
+        s->ctaps_inc = inc;
+        s->ctaps_valid = true;
     }
     if (s->taps_dirty) {
         if (s->d_taps && s->d_taps_len != s->poly.size()) {
@@ -397,6 +421,7 @@ static int run_batched(gpud_runner* r, const float2* d_in, int n_new, const gpud
                             r->d_out + results[i].out_offset, D, T, s->phase_count, n_new, s->nco_phase,
                             inc, r->tune) != 0)
             return -1;
+        ch.ctaps = s->d_ctaps;
         if (ch.n_out > 0) lds = std::max(lds, argus_item_lds_bytes(ch));
         argus_append_items(ch, i, n_new, r->items, r->item_pos);
     }

@@ -22,6 +22,10 @@ struct ArgusSignal {
     bool taps_dirty = false;
     float* d_taps = nullptr;
     size_t d_taps_len = 0;
+    float2* d_ctaps = nullptr;  // modulated taps for the NCO increment ctaps_inc (T)
+    size_t d_ctaps_len = 0;
+    uint64_t ctaps_inc = 0;
+    bool ctaps_valid = false;
     float2* d_hist[2] = {nullptr, nullptr};  // ping-pong shifted history, cap num_taps-1
     int hist_sel = 0;
     int hist_len = 0;

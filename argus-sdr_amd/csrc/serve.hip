@@ -85,12 +85,12 @@ uint64_t frac_to_fix(uint64_t num, uint64_t den)
 
 }  // namespace
 
-// Device bytes a channel holds while open (taps, ping-pong history, FM carry)
-// plus its share of the per-slot output buffers.
+// Device bytes a channel holds while open (taps, modulated taps, ping-pong
+// history, FM carry) plus its share of the per-slot output buffers.
 static size_t channel_footprint(const gpud_serve* s, int D, int T, bool stereo)
 {
     const size_t L = (size_t)(T + D - 1) / D;
-    const size_t taps = (size_t)D * L * sizeof(float);
+    const size_t taps = (size_t)D * L * sizeof(float) + (size_t)T * sizeof(float2);
     const size_t hist = 2 * (size_t)std::max(1, T - 1) * sizeof(float2);
     const size_t carry = 2 * sizeof(float2);
     const size_t n_out = (size_t)s->block_samples / D + 2;
@@ -107,11 +107,13 @@ static size_t channel_footprint(const gpud_serve* s, int D, int T, bool stereo)
 static void release_channel(ServeChannel& c)
 {
     if (c.d_taps) hipFree(c.d_taps);
+    if (c.d_ctaps) hipFree(c.d_ctaps);
     if (c.d_hist[0]) hipFree(c.d_hist[0]);
     if (c.d_hist[1]) hipFree(c.d_hist[1]);
     if (c.d_carry) hipFree(c.d_carry);
     if (c.d_wfm) hipFree(c.d_wfm);
     c.d_taps = nullptr;
+    c.d_ctaps = nullptr;
     c.d_hist[0] = c.d_hist[1] = nullptr;
     c.d_carry = nullptr;
     c.d_wfm = nullptr;
@@ -278,12 +280,19 @@ GPUD_API int gpud_serve_open(gpud_serve* s, const gpud_channel_spec* spec)
             const int src = p + k * D;
             if (src < std::min(nb, T)) poly[(size_t)p * L + k] = (float)b64[(size_t)src];
         }
-    if (dev_alloc(&c.d_taps, poly.size()) != hipSuccess || dev_alloc(&c.d_hist[0], (size_t)std::max(1, T - 1)) ||
-        dev_alloc(&c.d_hist[1], (size_t)std::max(1, T - 1)) || dev_alloc(&c.d_carry, 2)) {
+    // the increment the block builder derives from the same offset
+    const uint64_t inc = argus_rad_to_fix(-2.0 * M_PI * spec->offset_hz / (double)s->sample_rate);
+    std::vector<float2> ctaps((size_t)T);
+    argus_modulated_taps(poly.data(), D, T, inc, ctaps.data());
+    if (dev_alloc(&c.d_taps, poly.size()) != hipSuccess || dev_alloc(&c.d_ctaps, ctaps.size()) != hipSuccess ||
+        dev_alloc(&c.d_hist[0], (size_t)std::max(1, T - 1)) || dev_alloc(&c.d_hist[1], (size_t)std::max(1, T - 1)) ||
+        dev_alloc(&c.d_carry, 2)) {
         release_channel(c);
         return (int)hipErrorOutOfMemory;
     }
     hipError_t e = hipMemcpy(c.d_taps, poly.data(), poly.size() * sizeof(float), hipMemcpyHostToDevice);
+    if (e == hipSuccess)
+        e = hipMemcpy(c.d_ctaps, ctaps.data(), ctaps.size() * sizeof(float2), hipMemcpyHostToDevice);
     if (e == hipSuccess && stereo) e = wfm_channel_init(c, spec->out_rate);
     if (e != hipSuccess) {
         release_channel(c);
@@ -472,6 +481,7 @@ static int build_block(gpud_serve* s, ServeSlot& sl, int n, const float2* in, bo
         if (argus_fill_chan(&ch, c.d_hist[c.hist_sel], c.hist_len, c.d_hist[c.hist_sel ^ 1], keep, c.d_taps,
                             sl.d_iq + o.iq_off, c.decim, c.T, c.phase_count, n, c.nco_phase, inc, s->tune) != 0)
             return -1;
+        ch.ctaps = c.d_ctaps;
         if ((c.decim - 1 + n) / c.decim > 0) lds = std::max(lds, argus_item_lds_bytes(ch));
         if (!reuse) {
             ArgusChanDesc full = ch;

@@ -13,6 +13,7 @@ struct ServeChannel {
     int decim = 0, T = 0;
     size_t footprint = 0;
     float* d_taps = nullptr;
+    float2* d_ctaps = nullptr;  // modulated taps for the channel's NCO increment (T)
     float2* d_hist[2] = {nullptr, nullptr};
     int hist_sel = 0, hist_len = 0;
     float2* d_carry = nullptr;  // 2 slots, ping-pong
