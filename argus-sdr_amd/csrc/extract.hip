@@ -2,6 +2,7 @@
 // single-signal entry points (successors of exports.cu:356-631).
 #include "extract.hpp"
 #include <algorithm>
+#include <type_traits>
 
 // NCO-shifted sample n of the chunk (n < 0 reads the old shifted history).
 __device__ __forceinline__ float2 argus_shifted(const ArgusChanDesc& ch, const float2* __restrict__ in, int n)
@@ -63,98 +64,152 @@ __device__ __forceinline__ float argus_lanes_sum(float v, int S)
     }
 }
 
-// Dense tile: M outputs starting at m0, samples staged in LDS by residue,
-// each plane as R interleaved sub-planes; each thread computes R consecutive
-// outputs over its share of the planes.
-template <int R>
+// Dense tile: the outputs m0 .. m0+M-1 of one channel, staged and filtered
+// in LDS (layout in extract.hpp).
 __device__ __forceinline__ void argus_dense_tile(const ArgusChanDesc& ch, const float2* __restrict__ in, int m0,
                                                  float2* lds_s)
 {
-    const int D = ch.D, T = ch.T, L = ch.L, P = ch.P, PS = ch.PS, SP = ch.SP;
-    const int M = min(ch.M, ch.n_out - m0);
+    const int D = ch.D, L = ch.L, PS = ch.PS;
+    const int Lp = (L + 3) & ~3;
+    const int Gt = ch.M >> 2;                // output groups of a full tile: fixes the layout
+    const int M = min(ch.M, ch.n_out - m0);  // outputs of this tile
+    const int ytop = 4 * Gt + Lp - 2;        // reversed position of position 0
     const int em0 = ch.e0 + m0 * D;
-    float* lds_t = (float*)(lds_s + P * PS);
-    for (int i = threadIdx.x; i < P * L; i += blockDim.x) lds_t[i] = ch.taps[i];
+    float* lds_t = (float*)(lds_s + ch.SP);
+    // Padded taps: plane p, slot k < Lp holds taps[p*L + k] (k < L) or 0.
+    // With Lp <= blockDim each thread keeps one slot k and steps over planes
+    // (one division per thread); otherwise threads step over the slots of
+    // each plane.
+    auto load_taps = [&]() {
+        const int B = blockDim.x;
+        if (Lp <= B) {
+            const int per = B / Lp;
+            const int p0 = (int)threadIdx.x / Lp, k = (int)threadIdx.x - p0 * Lp;
+            if (p0 < per)
+                for (int p = p0; p < D; p += per) lds_t[p * Lp + k] = k < L ? ch.taps[p * L + k] : 0.0f;
+        } else {
+            for (int p = 0; p < D; p++)
+                for (int k = threadIdx.x; k < Lp; k += B) lds_t[p * Lp + k] = k < L ? ch.taps[p * L + k] : 0.0f;
+        }
+    };
 
-    // Staging: sample n = n_lo + i for i in [0, span); d = em0 - n = T-1-i,
-    // residue p = d mod D, plane index j = (p - d) / D, plane position
-    // x = j + L - 1.  One division per thread, then stepped by blockDim
-    // without dividing again.  Loads are issued kBatch at a time; the NCO is
-    // evaluated exactly at the first sample of each batch and advanced across
-    // the batch by complex multiplication with e^{j inc B} (at most kBatch-1
-    // steps, so the rounding drift stays below 1e-6).  Tiles that lie wholly
-    // in the new input (all but the first of a chunk) take a path without
-    // history checks.
-    const int span = (M - 1) * D + T;
-    const int n_lo = em0 - (T - 1);
+    // Staging: positions 0 .. M+Lp-2 of every plane, i.e. the span of
+    // (M + Lp - 1) * D samples from n_lo; sample i goes to plane D-1-(i mod D)
+    // at reversed position ytop - i div D, word y + y div 4.  Loads are
+    // issued kBatch at a time; the NCO is evaluated exactly at the first
+    // sample of each batch and advanced across the batch by complex
+    // multiplication (at most kBatch-1 steps, rounding drift below 1e-6).
+    // Tiles wholly in the new input skip the history checks.
+    const int span = (M + Lp - 1) * D;
+    const int n_lo = em0 - (Lp * D - 1);
     const int B = blockDim.x;
-    int i = threadIdx.x;
-    if (i < span) {
-        constexpr int kBatch = 8;
-        const int stepq = B / D, stepr = B - stepq * D;
-        int d = (T - 1) - i;
-        int p = d % D;
-        if (p < 0) p += D;
-        int j = (p - d) / D;
-        int pbase = p * PS;  // p * PS, stepped with p
-        const int step_base = stepr * PS, wrap_base = D * PS;
-        uint64_t ph = ch.phase0 + ch.inc * (uint64_t)(int64_t)(n_lo + i);
-        const uint64_t phstep = ch.inc * (uint64_t)B;
-        float st_s, st_c;
-        argus_nco(phstep, st_s, st_c);
-        const bool all_new = n_lo >= 0;
-        auto put = [&](float2 x) {
-            const int xp = j + L - 1;
-            lds_s[pbase + (xp % R) * SP + xp / R] = x;
-            ph += phstep;
-            p -= stepr;
-            pbase -= step_base;
-            j += stepq;
-            if (p < 0) {
-                p += D;
-                pbase += wrap_base;
-                j += 1;
-            }
-            i += B;
-        };
-        if (all_new && D <= B) {
-            // Threads [0, Bp) with Bp a multiple of D: each thread stays in
-            // one plane p and its position x advances by Bp / D per sample.
-            const int Bp = (B / D) * D;
-            const int sq = Bp / D;
-            const float2* src = in + n_lo;
-            float2* plane = lds_s + pbase;
-            int x = j + L - 1;
-            uint64_t phb = ph;
-            const uint64_t phstep_b = ch.inc * (uint64_t)Bp;
-            float sb_s, sb_c;
-            argus_nco(phstep_b, sb_s, sb_c);
-            if (threadIdx.x >= Bp) i = span;
-            while (i + (kBatch - 1) * Bp < span) {
-                float2 v[kBatch];
+    constexpr int kBatch = 8;
+    const bool all_new = n_lo >= 0;
+    if (all_new && D <= B) {
+        // Threads [0, Bp) with Bp a multiple of D: each stays in one plane
+        // and its reversed position drops by sq = Bp / D per sample, by
+        // kBatch*sq (a multiple of 4) per batch -- so y mod 4, and with it the
+        // word offsets of a batch's samples from its first, are the same for
+        // every batch, and the first word drops by 10*sq per batch.  The input
+        // is read through a buffer resource (lane offset in a vector register,
+        // batch and sample offsets in scalar registers; reads past the span
+        // return 0 and are not stored).
+        static_assert(kBatch == 8, "word step per batch assumes 8 samples");
+        const int Bp = (B / D) * D;
+        if ((int)threadIdx.x < Bp) {
+            const int t = threadIdx.x;
+            const int sq = B / D;
+            const int x0 = t / D;
+            const int y0 = ytop - x0;
+            const int w0 = y0 + (y0 >> 2);
+            int off[kBatch];
 #pragma unroll
-                for (int u = 0; u < kBatch; u++) v[u] = src[i + u * Bp];
+            for (int u = 0; u < kBatch; u++) {
+                const int y = y0 - u * sq;
+                off[u] = y + (y >> 2) - w0;
+            }
+            float2* w = lds_s + (D - 1 - (t - x0 * D)) * PS + w0;
+            const __amdgpu_buffer_rsrc_t rs =
+                __builtin_amdgcn_make_buffer_rsrc((void*)(in + n_lo), (short)0, span * (int)sizeof(float2), 0x00020000);
+            const int voff = t * (int)sizeof(float2);
+            uint64_t ph = ch.phase0 + ch.inc * (uint64_t)(int64_t)(n_lo + t);
+            const uint64_t phstep = ch.inc * (uint64_t)Bp;
+            float sb_s, sb_c;
+            argus_nco(phstep, sb_s, sb_c);
+            auto load = [&](float2* v, int first) {  // first: span index of the batch's sample 0 (thread 0)
+#pragma unroll
+                for (int u = 0; u < kBatch; u++)
+                    v[u] = __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(
+                                                          rs, voff, (first + u * Bp) * (int)sizeof(float2), 0));
+            };
+            auto store = [&](const float2* v, int first, auto checked) {
                 float sn, cs;
-                argus_nco(phb, sn, cs);
+                argus_nco(ph, sn, cs);
 #pragma unroll
                 for (int u = 0; u < kBatch; u++) {
-                    plane[__umul24((unsigned)(x % R), (unsigned)SP) + x / R] = argus_rot(v[u], sn, cs);
-                    x += sq;
+                    if (!decltype(checked)::value || t + first + u * Bp < span) w[off[u]] = argus_rot(v[u], sn, cs);
                     const float c2 = cs * sb_c - sn * sb_s;
                     sn = fmaf(sn, sb_c, cs * sb_s);
                     cs = c2;
                 }
-                i += kBatch * Bp;
-                phb += (uint64_t)kBatch * phstep_b;
+                w -= 10 * sq;
+                ph += (uint64_t)kBatch * phstep;
+            };
+            // The first two batches and the taps are in flight together, so a
+            // typical tile (<= 16 samples per thread) waits on memory once.
+            const int nfull = span / (kBatch * Bp);  // batches wholly inside the span for every thread
+            const int nb_all = (span + kBatch * Bp - 1) / (kBatch * Bp);
+            float2 va[kBatch], vb[kBatch];
+            load(va, 0);
+            if (nb_all > 1) load(vb, kBatch * Bp);
+            load_taps();
+            if (nfull > 0)
+                store(va, 0, std::false_type{});
+            else
+                store(va, 0, std::true_type{});
+            if (nb_all > 1) {
+                if (nfull > 1)
+                    store(vb, kBatch * Bp, std::false_type{});
+                else
+                    store(vb, kBatch * Bp, std::true_type{});
             }
-            for (; i < span; i += Bp) {
-                float sn, cs;
-                argus_nco(phb, sn, cs);
-                plane[__umul24((unsigned)(x % R), (unsigned)SP) + x / R] = argus_rot(src[i], sn, cs);
-                x += sq;
-                phb += phstep_b;
+            for (int bt = 2; bt < nb_all; bt++) {
+                load(va, bt * kBatch * Bp);
+                if (bt < nfull)
+                    store(va, bt * kBatch * Bp, std::false_type{});
+                else
+                    store(va, bt * kBatch * Bp, std::true_type{});
             }
-        } else if (all_new) {
+        } else {
+            load_taps();
+        }
+    } else {
+        load_taps();
+    }
+    if (!(all_new && D <= B) && (int)threadIdx.x < span) {
+        // General order: thread t takes samples t, t+B, ...; plane residue r
+        // and reversed position y are stepped without dividing again.
+        int i = threadIdx.x;
+        const int stepq = B / D, stepr = B - stepq * D;
+        const int x0 = i / D;
+        int r = i - x0 * D;
+        int y = ytop - x0;
+        uint64_t ph = ch.phase0 + ch.inc * (uint64_t)(int64_t)(n_lo + i);
+        const uint64_t phstep = ch.inc * (uint64_t)B;
+        auto put = [&](float2 v) {
+            lds_s[(D - 1 - r) * PS + y + (y >> 2)] = v;
+            ph += phstep;
+            r += stepr;
+            y -= stepq;
+            if (r >= D) {
+                r -= D;
+                y -= 1;
+            }
+            i += B;
+        };
+        if (all_new) {
+            float st_s, st_c;
+            argus_nco(phstep, st_s, st_c);
             const float2* src = in + n_lo;
             while (i + (kBatch - 1) * B < span) {
                 float2 v[kBatch];
@@ -170,13 +225,13 @@ __device__ __forceinline__ void argus_dense_tile(const ArgusChanDesc& ch, const 
                     cs = c2;
                 }
             }
-            for (; i < span;) {
+            while (i < span) {
                 float sn, cs;
                 argus_nco(ph, sn, cs);
                 put(argus_rot(src[i], sn, cs));
             }
         } else {
-            for (; i < span;) {
+            while (i < span) {
                 const int n = n_lo + i;
                 float2 x = make_float2(0.0f, 0.0f);
                 if (n >= 0) {
@@ -192,126 +247,51 @@ __device__ __forceinline__ void argus_dense_tile(const ArgusChanDesc& ch, const 
     }
     __syncthreads();
 
-    // FIR: output group q = t / S (outputs qR .. qR+R-1), lanes g = t % S
-    // share its planes p = g, g+S, ...  The sample output qR+r needs at tap k
-    // sits at plane position x = qR + r - k + L - 1; walking v_i = position
-    // qR + (L + R - 2) - i, output r at tap k uses v_{R-1-r+k}.
+    // FIR: output group q = t / S (outputs 4q .. 4q+3), lanes g = t % S
+    // share its planes p = g, g+S, ...  Block kc (taps 4kc .. 4kc+3) reads
+    // the samples v_t at words 5(Gt-1-q+kc) + {0,1,2,3,5,6,7}; output r at
+    // tap 4kc+i uses v_{3-r+i}.
     const int S = ch.S;
     const int t = threadIdx.x;
-    const int q = t / S;
+    const int q = t >> (__ffs(S) - 1);
     const int g = t & (S - 1);
-    const int G = (M + R - 1) / R;
-    float ar[R], ai[R];
+    const int Gm = (M + 3) >> 2;
+    float ar[4], ai[4];
 #pragma unroll
-    for (int r = 0; r < R; r++) ar[r] = ai[r] = 0.0f;
-    if (q < G) {
-        const int pfull = T - (L - 1) * D;  // planes p < pfull carry L taps, the rest L-1
-        const int c = L + R - 2;
-        for (int p = g; p < P; p += S) {
-            const int kmax = p < pfull ? L : L - 1;
-            const float* tp = lds_t + p * L;
-            const float2* pl = lds_s + p * PS + q;
-#define ARGUS_V(idx) pl[((c - (idx)) % R) * SP + (c - (idx)) / R]
-            if constexpr (R == 4) {
-                float2 a0 = ARGUS_V(0), a1 = ARGUS_V(1), a2 = ARGUS_V(2);
-                // v_{k+3+u} for k = 0 mod 4 sits at sub-plane (c-3-u) & 3,
-                // index ((c-3-u) >> 2) - k/4: one base pointer per u, so the
-                // loop carries no per-tap index arithmetic.  Blocks of 4 taps
-                // go two per iteration, each block's samples and taps loaded
-                // one block ahead (the prefetch index is clamped to the last
-                // block, so no load leaves the plane).
-                const float2* q0 = pl + ((c - 3) & 3) * SP + ((c - 3) >> 2);
-                const float2* q1 = pl + ((c - 4) & 3) * SP + ((c - 4) >> 2);
-                const float2* q2 = pl + ((c - 5) & 3) * SP + ((c - 5) >> 2);
-                const float2* q3 = pl + ((c - 6) & 3) * SP + ((c - 6) >> 2);
-                const int n4 = kmax >> 2;
-#define ARGUS_BLOCK4(A0, A1, A2, B0, B1, B2, B3, H)                                      \
-    ar[3] = fmaf(H.x, A0.x, ar[3]); ai[3] = fmaf(H.x, A0.y, ai[3]);                      \
-    ar[2] = fmaf(H.x, A1.x, ar[2]); ai[2] = fmaf(H.x, A1.y, ai[2]);                      \
-    ar[1] = fmaf(H.x, A2.x, ar[1]); ai[1] = fmaf(H.x, A2.y, ai[1]);                      \
-    ar[0] = fmaf(H.x, B0.x, ar[0]); ai[0] = fmaf(H.x, B0.y, ai[0]);                      \
-    ar[3] = fmaf(H.y, A1.x, ar[3]); ai[3] = fmaf(H.y, A1.y, ai[3]);                      \
-    ar[2] = fmaf(H.y, A2.x, ar[2]); ai[2] = fmaf(H.y, A2.y, ai[2]);                      \
-    ar[1] = fmaf(H.y, B0.x, ar[1]); ai[1] = fmaf(H.y, B0.y, ai[1]);                      \
-    ar[0] = fmaf(H.y, B1.x, ar[0]); ai[0] = fmaf(H.y, B1.y, ai[0]);                      \
-    ar[3] = fmaf(H.z, A2.x, ar[3]); ai[3] = fmaf(H.z, A2.y, ai[3]);                      \
-    ar[2] = fmaf(H.z, B0.x, ar[2]); ai[2] = fmaf(H.z, B0.y, ai[2]);                      \
-    ar[1] = fmaf(H.z, B1.x, ar[1]); ai[1] = fmaf(H.z, B1.y, ai[1]);                      \
-    ar[0] = fmaf(H.z, B2.x, ar[0]); ai[0] = fmaf(H.z, B2.y, ai[0]);                      \
-    ar[3] = fmaf(H.w, B0.x, ar[3]); ai[3] = fmaf(H.w, B0.y, ai[3]);                      \
-    ar[2] = fmaf(H.w, B1.x, ar[2]); ai[2] = fmaf(H.w, B1.y, ai[2]);                      \
-    ar[1] = fmaf(H.w, B2.x, ar[1]); ai[1] = fmaf(H.w, B2.y, ai[1]);                      \
-    ar[0] = fmaf(H.w, B3.x, ar[0]); ai[0] = fmaf(H.w, B3.y, ai[0]);
-                int k = 0;
-                if (n4 > 0) {
-                    float2 b0 = q0[0], b1 = q1[0], b2 = q2[0], b3 = q3[0];
-                    float4 hb = make_float4(tp[0], tp[1], tp[2], tp[3]);
-                    int k4 = 0;
-                    for (; k4 + 2 <= n4; k4 += 2) {
-                        const int kc = k4 + 1;  // block k4+1 exists
-                        const float2 c0 = q0[-kc], c1 = q1[-kc], c2 = q2[-kc], c3 = q3[-kc];
-                        const float4 hc = make_float4(tp[4 * kc], tp[4 * kc + 1], tp[4 * kc + 2], tp[4 * kc + 3]);
-                        ARGUS_BLOCK4(a0, a1, a2, b0, b1, b2, b3, hb)
-                        const int kn = min(k4 + 2, n4 - 1);  // next pair's first block (clamped)
-                        b0 = q0[-kn];
-                        const float2 n1 = q1[-kn], n2 = q2[-kn], n3 = q3[-kn];
-                        hb = make_float4(tp[4 * kn], tp[4 * kn + 1], tp[4 * kn + 2], tp[4 * kn + 3]);
-                        ARGUS_BLOCK4(b1, b2, b3, c0, c1, c2, c3, hc)
-                        a0 = c1;
-                        a1 = c2;
-                        a2 = c3;
-                        b1 = n1;
-                        b2 = n2;
-                        b3 = n3;
-                    }
-                    if (k4 < n4) {  // odd block count: the last block is loaded in b, hb
-                        ARGUS_BLOCK4(a0, a1, a2, b0, b1, b2, b3, hb)
-                        a0 = b1;
-                        a1 = b2;
-                        a2 = b3;
-                    }
-                    k = 4 * n4;
-                }
-#undef ARGUS_BLOCK4
-                for (; k < kmax; k++) {
-                    const float2 b0 = ARGUS_V(k + 3);
-                    const float h = tp[k];
-                    ar[3] = fmaf(h, a0.x, ar[3]); ai[3] = fmaf(h, a0.y, ai[3]);
-                    ar[2] = fmaf(h, a1.x, ar[2]); ai[2] = fmaf(h, a1.y, ai[2]);
-                    ar[1] = fmaf(h, a2.x, ar[1]); ai[1] = fmaf(h, a2.y, ai[1]);
-                    ar[0] = fmaf(h, b0.x, ar[0]); ai[0] = fmaf(h, b0.y, ai[0]);
-                    a0 = a1;
-                    a1 = a2;
-                    a2 = b0;
-                }
-            } else {
-                int k = 0;
-                for (; k + 2 <= kmax; k += 2) {
-                    const float2 v0 = ARGUS_V(k), v1 = ARGUS_V(k + 1);
-                    const float h0 = tp[k], h1 = tp[k + 1];
-                    ar[0] = fmaf(h0, v0.x, ar[0]);
-                    ai[0] = fmaf(h0, v0.y, ai[0]);
-                    ar[0] = fmaf(h1, v1.x, ar[0]);
-                    ai[0] = fmaf(h1, v1.y, ai[0]);
-                }
-                if (k < kmax) {
-                    const float2 v0 = ARGUS_V(k);
-                    ar[0] = fmaf(tp[k], v0.x, ar[0]);
-                    ai[0] = fmaf(tp[k], v0.y, ai[0]);
-                }
+    for (int r = 0; r < 4; r++) ar[r] = ai[r] = 0.0f;
+    if (q < Gm) {
+        const int nb = Lp >> 2;
+        const float2* lane = lds_s + 5 * (Gt - 1 - q);
+        for (int p = g; p < D; p += S) {
+            const float2* b = lane + p * PS;
+            const float4* h4 = (const float4*)(lds_t + p * Lp);
+#pragma unroll 2
+            for (int kc = 0; kc < nb; kc++) {
+                const float2 v0 = b[0], v1 = b[1], v2 = b[2], v3 = b[3], v4 = b[5], v5 = b[6], v6 = b[7];
+                const float4 H = h4[kc];
+#define ARGUS_TAP(HC, A, B_, C, E)                                                    \
+    ar[3] = fmaf(HC, A.x, ar[3]); ai[3] = fmaf(HC, A.y, ai[3]);                       \
+    ar[2] = fmaf(HC, B_.x, ar[2]); ai[2] = fmaf(HC, B_.y, ai[2]);                     \
+    ar[1] = fmaf(HC, C.x, ar[1]); ai[1] = fmaf(HC, C.y, ai[1]);                       \
+    ar[0] = fmaf(HC, E.x, ar[0]); ai[0] = fmaf(HC, E.y, ai[0]);
+                ARGUS_TAP(H.x, v0, v1, v2, v3)
+                ARGUS_TAP(H.y, v1, v2, v3, v4)
+                ARGUS_TAP(H.z, v2, v3, v4, v5)
+                ARGUS_TAP(H.w, v3, v4, v5, v6)
+#undef ARGUS_TAP
+                b += 5;
             }
-#undef ARGUS_V
         }
     }
 #pragma unroll
-    for (int r = 0; r < R; r++) {
+    for (int r = 0; r < 4; r++) {
         ar[r] = argus_lanes_sum(ar[r], S);
         ai[r] = argus_lanes_sum(ai[r], S);
     }
-    if (q < G && g == 0) {
+    if (q < Gm && g == 0) {
 #pragma unroll
-        for (int r = 0; r < R; r++) {
-            const int m = q * R + r;
+        for (int r = 0; r < 4; r++) {
+            const int m = 4 * q + r;
             if (m < M) ch.out[m0 + m] = make_float2(ar[r], ai[r]);
         }
     }
@@ -472,12 +452,8 @@ __device__ __forceinline__ void argus_run_item(const ArgusChanDesc& ch, int kind
     // a work list built for the most outputs a chunk can give may hold one
     // tile past this chunk's last output
     if ((kind == ARGUS_ITEM_DENSE || kind == ARGUS_ITEM_NARROW) && m0 >= ch.n_out) return;
-    if (kind == ARGUS_ITEM_DENSE) {
-        if (ch.R == 4)
-            argus_dense_tile<4>(ch, in, m0, lds);
-        else
-            argus_dense_tile<1>(ch, in, m0, lds);
-    }
+    if (kind == ARGUS_ITEM_DENSE)
+        argus_dense_tile(ch, in, m0, lds);
     else if (kind == ARGUS_ITEM_NARROW) {
         if (ch.ctaps)
             argus_narrow_mod(ch, in, m0, lds);
@@ -493,9 +469,21 @@ __device__ __forceinline__ void argus_extract_items(const ArgusChanDesc* __restr
                                                     const ArgusWorkItem* __restrict__ items, int n_items,
                                                     const float2* __restrict__ in, int n_new, float2* lds)
 {
+    // The tables are written before the launch and only read during it, so
+    // they are read through the constant address space: scalar loads even
+    // when the table pointers themselves come from device memory.
+#if defined(__HIP_DEVICE_COMPILE__)
+    typedef const __attribute__((address_space(4))) ArgusChanDesc CChan;
+    typedef const __attribute__((address_space(4))) ArgusWorkItem CItem;
+    CChan* cchans = (CChan*)chans;
+    CItem* citems = (CItem*)items;
+#else  // the host pass only needs the function to compile
+    const ArgusChanDesc* cchans = chans;
+    const ArgusWorkItem* citems = items;
+#endif
     for (int w = blockIdx.x; w < n_items; w += gridDim.x) {
-        const ArgusWorkItem it = items[w];
-        const ArgusChanDesc ch = chans[it.chan];
+        const ArgusWorkItem it = citems[w];
+        const ArgusChanDesc ch = cchans[it.chan];
         argus_run_item(ch, it.kind, it.m0, in, n_new, lds);
     }
 }
@@ -620,10 +608,14 @@ void argus_append_items(const ArgusChanDesc& ch, int chan_index, int n_new, std:
     // cost in lane-operations: NCO rotations (~24 each) plus taps (~2 each,
     // ~6 for a complex tap with its load)
     long cost;
-    if (ch.narrow && ch.ctaps)
+    if (!ch.narrow) {
+        const int Lp = argus_dense_lp(ch.L);
+        cost = (long)(ch.M + Lp - 1) * ch.D * 10 + (long)ch.M * Lp * ch.D * 2;
+    } else if (ch.ctaps) {
         cost = (long)ch.M * (((ch.T + 63) & ~63) * 6 + 64);
-    else
-        cost = (ch.narrow ? (long)ch.M * ch.T : (long)(ch.M - 1) * ch.D + ch.T) * 24 + (long)ch.M * ch.T * 2;
+    } else {
+        cost = (long)ch.M * ch.T * 26;
+    }
     const int kind = ch.narrow ? ARGUS_ITEM_NARROW : ARGUS_ITEM_DENSE;
     for (int t = 0; t < tiles; t++) {
         items.push_back({chan_index, t * ch.M, kind, (int)std::min<long>(cost, 1L << 30)});
@@ -682,14 +674,14 @@ void argus_order_items(std::vector<ArgusWorkItem>& items, const std::vector<int>
 // Describe one signal for the kernels.  Returns 0 or -1 if the geometry
 // cannot be staged (T too large).
 int argus_fill_chan(ArgusChanDesc* ch, const float2* hist_rd, int H, float2* hist_wr, int keep, const float* taps,
-                    float2* out, int D, int T, int phase_count, int n_new, double phase_start, double phase_inc,
-                    const ArgusTune& tune)
+                    const float2* ctaps, float2* out, int D, int T, int phase_count, int n_new, double phase_start,
+                    double phase_inc, const ArgusTune& tune)
 {
     if (T > ARGUS_MAX_TAPS) return -1;
     ch->hist_rd = hist_rd;
     ch->hist_wr = hist_wr;
     ch->taps = taps;
-    ch->ctaps = nullptr;
+    ch->ctaps = ctaps;
     ch->out = out;
     ch->phase0 = argus_rad_to_fix(phase_start);
     ch->inc = argus_rad_to_fix(phase_inc);
@@ -704,7 +696,7 @@ int argus_fill_chan(ArgusChanDesc* ch, const float2* hist_rd, int H, float2* his
     // outputs a chunk of n_new samples can give), so a work list built for
     // one chunk stays valid for the next of the same length
     ch->n_out = (D - 1 + n_new) / D;
-    const int rc = argus_plan_chan(ch, tune);
+    const int rc = argus_plan_chan(ch, tune, ctaps != nullptr);
     ch->n_out = (phase_count + n_new) / D;
     return rc;
 }
@@ -751,7 +743,7 @@ GPUD_API int gpud_launch_streaming_polyphase_stateful_hip(
 
     ArgusChanDesc ch;
     if (argus_fill_chan(&ch, (const float2*)history_state, H, (float2*)history_scratch, new_h, polyphase_taps,
-                        (float2*)out, decim, num_taps, pc, n_new, phase_start, phase_inc, argus_default_tune()) != 0)
+                        nullptr, (float2*)out, decim, num_taps, pc, n_new, phase_start, phase_inc, argus_default_tune()) != 0)
         return -12;
     int err = argus_launch_extract_one(ch, (const float2*)in_new, n_new, 0);
     if (err) return err;
@@ -790,7 +782,7 @@ GPUD_API int gpud_launch_streaming_polyphase_prepare_hip(
     const int new_h = (history_out && keep > 0) ? std::min(H + n_new, keep) : 0;
     ArgusChanDesc ch;
     if (argus_fill_chan(&ch, (const float2*)history_in, H, (float2*)history_out, new_h, polyphase_taps,
-                        (float2*)out, decim, num_taps, pc, n_new, phase_start, phase_inc, argus_default_tune()) != 0)
+                        nullptr, (float2*)out, decim, num_taps, pc, n_new, phase_start, phase_inc, argus_default_tune()) != 0)
         return -2;
     int err = argus_launch_extract_one(ch, (const float2*)in_new, n_new, 0);
     if (err) return err;

@@ -2,20 +2,23 @@
 //
 // Two shapes of work item, picked per channel by the host:
 //
-//  dense tile (D < T): a tile of M consecutive outputs.  The NCO-shifted
-//    input span the tile needs is staged in LDS grouped by polyphase residue
-//    ("planes"): plane p holds s[e_m0 - p + j*D], j = -(L-1) .. M-1, so for
-//    a fixed (p, k) consecutive outputs read consecutive LDS words -- no bank
-//    conflicts whatever D is.  Each output is
-//        y[m] = sum_{p < P} sum_{k < L_p} h[p + k*D] * s[e_m - p - k*D]
-//    (the reference's phase-major order, streaming_host_core.go:29-52).
-//    When M < 256 the planes of one output are split over S lanes and the
-//    partial sums are combined with lane shuffles, so every thread works.
-//    With register blocking (R = 4) a thread computes R consecutive outputs
-//    from a sliding window of R samples, so each LDS read feeds R complex
-//    MACs; each plane is then stored as R interleaved sub-planes (position x
-//    at sub-plane x mod R, index x div R) so the lanes of a read still touch
-//    consecutive words.
+//  dense tile (D < T): a tile of M = 4G consecutive outputs.  The
+//    NCO-shifted input span the tile needs is staged in LDS grouped by
+//    polyphase residue ("planes"): plane p holds s[e_m0 - p + j*D] for
+//    j = -(Lp-1) .. M-1 (position x = j + Lp - 1), so each output is
+//        y[m] = sum_{p < D} sum_{k < Lp} h[p + k*D] * s[e_m - p - k*D]
+//    (the reference's phase-major order, streaming_host_core.go:29-52) with
+//    the L = ceil(T/D) taps of a plane zero-padded to Lp = roundup(L, 4).
+//    Sample i of the span (input n_lo + i) lands in plane D-1-(i mod D) at
+//    position i div D.  A thread computes the 4 consecutive outputs of one
+//    output group over its share of the planes (S lanes per group, combined
+//    by lane shuffles), 4 taps at a time: a block needs 7 consecutive
+//    positions.  Positions are stored reversed, 5 words per 4 positions
+//    (word of reversed position y = y + y div 4), so the 7 samples of a
+//    block sit at fixed offsets {0,1,2,3,5,6,7} from one pointer that
+//    advances by 5 words per block, and the output groups of a read are 5
+//    words apart (odd: distinct banks).  The plane stride is chosen so the S
+//    planes of one read land in disjoint bank ranges.
 //
 //  narrow group (D >= T): each output needs only the T samples before its
 //    emission point and windows do not overlap, so no input is staged.
@@ -51,9 +54,9 @@ struct ArgusChanDesc {
     int n_out;              // outputs this chunk
     int keep;               // new history length
     int M;                  // outputs per item
-    int PS;                 // LDS plane stride (complex samples), dense only
-    int SP;                 // LDS sub-plane stride (complex samples), dense only
-    int R;                  // outputs per thread in the dense FIR phase (1 or 4)
+    int PS;                 // LDS plane stride (complex words), dense only
+    int SP;                 // LDS word offset of the padded taps, dense only
+    int R;                  // outputs per thread in the dense FIR phase (4; 1 for narrow)
     int S;                  // lanes per output group in the dense FIR phase (power of 2)
     int narrow;             // 1 = narrow (wave-per-output) items
 };
@@ -98,39 +101,63 @@ struct ArgusTune {
 
 static inline ArgusTune argus_default_tune() { return ArgusTune{ARGUS_LDS_ENTRIES, ARGUS_NARROW_OUTPUTS, ARGUS_GRID}; }
 
-#define ARGUS_DENSE_R 4             // register blocking of the dense FIR
+#define ARGUS_DENSE_R 4             // outputs per thread in the dense FIR
 
-// Plane stride: at least R*SP, and congruent mod 16 (complex words) to 16/S
-// so the 16 lanes of one LDS read phase (S planes x 16/S output groups)
-// touch distinct bank pairs.
-static inline int argus_plane_stride(int R, int SP, int S)
+// Dense tile geometry: taps per plane padded to a multiple of 4, positions
+// per plane for G output groups, and words per plane (5 per 4 positions).
+static inline int argus_dense_lp(int L) { return (L + 3) & ~3; }
+static inline int argus_dense_words(int G, int Lp)
 {
-    const int want = S >= 16 || S == 1 ? 1 : 16 / S;
-    int PS = R * SP;
-    while ((PS & 15) != want) PS++;
+    const int y = 4 * G + Lp - 2;  // largest reversed position
+    return y + (y >> 2) + 1;
+}
+
+// Plane stride for S lanes per output group: at least `words`, and
+// congruent mod 32 to -5*32/S, so the S planes x 32/S output groups of a
+// 32-lane read (addresses g*PS - 5q) cover 32 distinct banks.  S = 1 or
+// S >= 32: odd (then consecutive planes start in distinct banks, which
+// keeps the staging writes apart too).
+static inline int argus_plane_stride(int words, int S)
+{
+    int PS = words;
+    if (S >= 2 && S < 32) {
+        const int want = (32 - (5 * 32 / S) % 32) & 31;
+        while ((PS & 31) != want) PS++;
+    } else if (!(PS & 1)) {
+        PS++;
+    }
     return PS;
 }
 
-// Dense geometry with R outputs per thread within `entries` complex words of
-// LDS.  Returns 0, or -1 if not even one output group fits.
-static inline int argus_plan_dense(ArgusChanDesc* ch, int R, int entries)
+// Complex words of LDS a dense tile takes: planes, then the padded taps
+// (P*Lp floats) from an even word so they can be read 16 bytes at a time.
+static inline int argus_dense_entries(int P, int PS, int Lp)
 {
-    const int P = ch->P, L = ch->L;
-    const int tail = (L + R - 2) / R;  // sub-plane words beyond the G groups
-    int G = (entries / P) / R - tail;
-    const int need = ch->n_out > 0 ? (ch->n_out + R - 1) / R : 1;
+    const int samples = (P * PS + 1) & ~1;
+    return samples + (P * Lp + 1) / 2;
+}
+
+// Dense geometry within `entries` complex words of LDS.  Returns 0, or -1 if
+// fewer than 4 output groups fit.
+static inline int argus_plan_dense(ArgusChanDesc* ch, int entries)
+{
+    const int P = ch->P, Lp = argus_dense_lp(ch->L);
+    const int need = ch->n_out > 0 ? (ch->n_out + 3) / 4 : 1;
+    int G = entries / P / 5;  // upper bound: a plane holds 4G + Lp - 1 positions at 5 words per 4
     if (G > ARGUS_EXTRACT_THREADS) G = ARGUS_EXTRACT_THREADS;
     if (G > need) G = need;
-    for (; G > 0; G--) {
+    // at least 4 groups per tile, unless the chunk has fewer outputs
+    for (; G > 0 && (G >= 4 || G == need); G--) {
         int S = 1;
         while (S * 2 * G <= ARGUS_EXTRACT_THREADS && S * 2 <= P && S < 64) S *= 2;
-        int PS = argus_plane_stride(R, G + tail, S);
-        if (P * PS > entries) PS = R * (G + tail);  // no room for bank padding
-        if (P * PS <= entries) {
+        const int words = argus_dense_words(G, Lp);
+        int PS = argus_plane_stride(words, S);
+        if (argus_dense_entries(P, PS, Lp) > entries) PS = words;  // no room for bank padding
+        if (argus_dense_entries(P, PS, Lp) <= entries) {
             ch->narrow = 0;
-            ch->R = R;
-            ch->M = G * R;
-            ch->SP = G + tail;
+            ch->R = ARGUS_DENSE_R;
+            ch->M = 4 * G;
+            ch->SP = (P * PS + 1) & ~1;  // word offset of the taps (even)
             ch->PS = PS;
             ch->S = S;
             return 0;
@@ -140,19 +167,23 @@ static inline int argus_plan_dense(ArgusChanDesc* ch, int R, int entries)
 }
 
 // Host-side planning for one channel.  Fills M, PS, SP, R, S, narrow.
-// Dense tiles when the windows of consecutive outputs overlap (D < T) and a
-// tile fits the LDS budget, else narrow items (any D and T).
-static inline int argus_plan_chan(ArgusChanDesc* ch, const ArgusTune& tune)
+// Dense tiles when the windows of consecutive outputs overlap (D < T), a
+// tile fits the LDS budget and, with modulated taps available, the dense
+// form is the cheaper one (staging costs ~10 lane operations per input
+// sample, a dense tap ~2.2 per output, a narrow complex tap ~5.5 per
+// output); else narrow items (any D and T).
+static inline int argus_plan_chan(ArgusChanDesc* ch, const ArgusTune& tune, bool mod_taps)
 {
     const int D = ch->D, T = ch->T;
-    if (D < T) {
+    bool dense = D < T;
+    if (dense && mod_taps) {
+        const long dense_cost = 22L * argus_dense_lp(ch->L) * D + 100L * D;
+        const long narrow_cost = 55L * ((T + 63) & ~63);
+        dense = dense_cost < narrow_cost;
+    }
+    if (dense) {
         ArgusChanDesc c = *ch;
-        if (argus_plan_dense(&c, ARGUS_DENSE_R, tune.lds_entries) == 0 && c.M / ARGUS_DENSE_R >= 4) {
-            *ch = c;
-            return 0;
-        }
-        c = *ch;
-        if (argus_plan_dense(&c, 1, tune.lds_entries) == 0) {
+        if (argus_plan_dense(&c, tune.lds_entries) == 0) {
             *ch = c;
             return 0;
         }
@@ -168,18 +199,18 @@ static inline int argus_plan_chan(ArgusChanDesc* ch, const ArgusTune& tune)
 // LDS bytes a dense tile of this channel needs (samples + taps).
 static inline size_t argus_dense_lds_bytes(const ArgusChanDesc& ch)
 {
-    return (size_t)ch.P * ch.PS * 8 + (size_t)ch.P * ch.L * 4;
+    return (size_t)argus_dense_entries(ch.P, ch.PS, argus_dense_lp(ch.L)) * 8;
 }
 
 // Modulated taps g[k] = h[k] e^{-j 2pi inc k} (inc in turns, 2^64 == 1) in
 // natural tap order, from the phase-major layout; FP64, rounded once.
 void argus_modulated_taps(const float* poly, int D, int T, uint64_t inc, float2* out);
 
-// argus_fill_chan leaves ctaps null; callers that keep modulated taps for
-// the channel set it afterwards.
+// ctaps: the channel's modulated taps (argus_modulated_taps for the same
+// increment), or null when the caller keeps none.
 int argus_fill_chan(ArgusChanDesc* ch, const float2* hist_rd, int H, float2* hist_wr, int keep, const float* taps,
-                    float2* out, int D, int T, int phase_count, int n_new, double phase_start, double phase_inc,
-                    const ArgusTune& tune);
+                    const float2* ctaps, float2* out, int D, int T, int phase_count, int n_new, double phase_start,
+                    double phase_inc, const ArgusTune& tune);
 size_t argus_item_lds_bytes(const ArgusChanDesc& ch);
 int argus_launch_extract_batch(const ArgusChanDesc* d_chans, const ArgusWorkItem* d_items, int n_items,
                                size_t lds_bytes, const float2* in, int n_new, int grid, hipStream_t stream);

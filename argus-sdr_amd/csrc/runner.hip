@@ -418,10 +418,9 @@ static int run_batched(gpud_runner* r, const float2* d_in, int n_new, const gpud
         ArgusChanDesc& ch = r->h_chans[slot][i];
         const double inc = -2.0 * M_PI * jobs[i].offset_hz / (double)r->sample_rate;
         if (argus_fill_chan(&ch, s->d_hist[s->hist_sel], s->hist_len, s->d_hist[s->hist_sel ^ 1], keep, s->d_taps,
-                            r->d_out + results[i].out_offset, D, T, s->phase_count, n_new, s->nco_phase,
+                            s->d_ctaps, r->d_out + results[i].out_offset, D, T, s->phase_count, n_new, s->nco_phase,
                             inc, r->tune) != 0)
             return -1;
-        ch.ctaps = s->d_ctaps;
         if (ch.n_out > 0) lds = std::max(lds, argus_item_lds_bytes(ch));
         argus_append_items(ch, i, n_new, r->items, r->item_pos);
     }

@@ -329,11 +329,10 @@ GPUD_API int gpud_serve_info(gpud_serve* s, int* live_channels, uint64_t* bytes_
 
 }  // extern "C"
 
-// Stereo-stage descriptor and work items of one channel for this block; with
-// commit the channel's host-side carry (delay-line half, RDS sample count)
-// advances.
-static void wfm_describe(ServeChannel& c, const ServeOut& o, ServeSlot& sl, ArgusWfmDesc& wd,
-                         std::vector<ArgusWfmItem>& items, bool commit)
+// Stereo-stage descriptor and work items of one channel for this block (the
+// channel's carry advances in commit_block).
+static void wfm_describe(const ServeChannel& c, const ServeOut& o, ServeSlot& sl, ArgusWfmDesc& wd,
+                         std::vector<ArgusWfmItem>& items)
 {
     const int rate = c.spec.out_rate;
     const int D = c.rds_decim;
@@ -365,10 +364,6 @@ static void wfm_describe(ServeChannel& c, const ServeOut& o, ServeSlot& sl, Argu
     wd.pad = 0;
     for (int st = 0; st < o.n_mono; st += ARGUS_WFM_TILE) items.push_back({o.wfm_index, st, ARGUS_WFM_ITEM_TILE, 0});
     if (o.n_mono > 0) items.push_back({o.wfm_index, 0, ARGUS_WFM_ITEM_TAIL, 0});
-    if (commit) {
-        if (o.n_mono > 0) c.wfm_sel ^= 1;
-        c.rds_count += o.n_mono;
-    }
 }
 
 static size_t align_up(size_t v) { return (v + 255) & ~(size_t)255; }
@@ -395,27 +390,22 @@ static int grow_table(ServeSlot& sl, size_t need, size_t keep)
     return 0;
 }
 
-// Build one block's outputs layout and tables in slot sl for input `in` (a
-// device pointer) of n samples.  With commit the channels' carried state
-// (phases, delay-line halves, carries) advances to after this block; without
-// it (autotuning) the state is left as it is -- the kernels then only write
-// outputs and the inactive history halves.
-static int build_block(gpud_serve* s, ServeSlot& sl, int n, const float2* in, bool commit)
+// Output layout of one block of n samples over the live channels (in the
+// order of `live`): per-channel counts and offsets, and the totals.
+static void layout_block(const gpud_serve* s, int n, BlockLayout& L)
 {
-    // ---- outputs layout ---------------------------------------------------
-    const int nch = (int)s->live.size();
-    sl.outs.clear();
-    sl.outs.reserve((size_t)nch);
-    int64_t iq_total = 0, audio_total = 0, mono_total = 0, rds_total = 0;
-    int n_wfm = 0;
-    for (auto& kv : s->live) {
-        ServeChannel& c = kv.second;
+    L.outs.clear();
+    L.outs.reserve(s->live.size());
+    L.iq_total = L.audio_total = L.mono_total = L.rds_total = 0;
+    L.n_wfm = 0;
+    for (const auto& kv : s->live) {
+        const ServeChannel& c = kv.second;
         ServeOut o;
         o.channel_id = kv.first;
         o.n_iq = (c.phase_count + n) / c.decim;
-        o.iq_off = iq_total;
+        o.iq_off = L.iq_total;
         o.n_audio = argus_demod_count(c.spec.demod, o.n_iq, c.have_carry);
-        o.audio_off = audio_total;
+        o.audio_off = L.audio_total;
         o.rate = c.spec.out_rate;
         o.wfm_index = -1;
         o.mono_off = o.rds_off = 0;
@@ -423,37 +413,72 @@ static int build_block(gpud_serve* s, ServeSlot& sl, int n, const float2* in, bo
         if (c.stereo) {
             o.n_mono = argus_demod_count(ARGUS_DEMOD_WFM, o.n_iq, c.have_carry);
             o.n_audio = 2 * o.n_mono;
-            o.wfm_index = n_wfm++;
-            o.mono_off = mono_total;
+            o.wfm_index = L.n_wfm++;
+            o.mono_off = L.mono_total;
             const int first = (int)((c.rds_decim - c.rds_count % c.rds_decim) % c.rds_decim);
             o.n_rds = o.n_mono > first ? (o.n_mono - 1 - first) / c.rds_decim + 1 : 0;
-            o.rds_off = rds_total;
+            o.rds_off = L.rds_total;
             o.rds_rate = c.spec.out_rate / c.rds_decim;
-            mono_total += o.n_mono;
-            rds_total += o.n_rds;
+            L.mono_total += o.n_mono;
+            L.rds_total += o.n_rds;
         }
-        iq_total += o.n_iq;
-        audio_total += o.n_audio;
-        sl.outs.push_back(o);
+        L.iq_total += o.n_iq;
+        L.audio_total += o.n_audio;
+        L.outs.push_back(o);
     }
-    ARGUS_CHECK((hipError_t)grow_dev(&sl.d_iq, &sl.iq_cap, (size_t)std::max<int64_t>(iq_total, 1)));
-    ARGUS_CHECK((hipError_t)grow_dev(&sl.d_audio, &sl.audio_cap, (size_t)std::max<int64_t>(audio_total, 1)));
-    ARGUS_CHECK((hipError_t)grow_host(&sl.h_iq, &sl.h_iq_cap, (size_t)std::max<int64_t>(iq_total, 1)));
-    ARGUS_CHECK((hipError_t)grow_host(&sl.h_audio, &sl.h_audio_cap, (size_t)std::max<int64_t>(audio_total, 1)));
-    if (n_wfm > 0) {
-        ARGUS_CHECK((hipError_t)grow_dev(&sl.d_mono, &sl.mono_cap, (size_t)std::max<int64_t>(mono_total, 1)));
-        ARGUS_CHECK((hipError_t)grow_dev(&sl.d_wscratch, &sl.wscratch_cap, (size_t)std::max<int64_t>(4 * mono_total, 1)));
-        ARGUS_CHECK((hipError_t)grow_dev(&sl.d_rds, &sl.rds_cap, (size_t)std::max<int64_t>(rds_total, 1)));
-        ARGUS_CHECK((hipError_t)grow_host(&sl.h_rds, &sl.h_rds_cap, (size_t)std::max<int64_t>(rds_total, 1)));
-        ARGUS_CHECK((hipError_t)grow_dev(&sl.d_wblk, &sl.wblk_cap, (size_t)n_wfm));
-        ARGUS_CHECK((hipError_t)grow_host(&sl.h_wblk, &sl.h_wblk_cap, (size_t)n_wfm));
-    }
-    sl.n_wfm = n_wfm;
-    sl.rds_total = rds_total;
-    sl.iq_total = iq_total;
-    sl.audio_total = audio_total;
+}
 
-    // ---- tables ------------------------------------------------------------
+// Whether the slot's output buffers already hold a block of this layout.
+static bool slot_fits(const ServeSlot& sl, const BlockLayout& L)
+{
+    auto fits = [](size_t cap, int64_t need) { return cap >= (size_t)std::max<int64_t>(need, 1); };
+    bool ok = fits(sl.iq_cap, L.iq_total) && fits(sl.audio_cap, L.audio_total) && fits(sl.h_iq_cap, L.iq_total) &&
+              fits(sl.h_audio_cap, L.audio_total);
+    if (L.n_wfm > 0)
+        ok = ok && fits(sl.mono_cap, L.mono_total) && fits(sl.wscratch_cap, 4 * L.mono_total) &&
+             fits(sl.rds_cap, L.rds_total) && fits(sl.h_rds_cap, L.rds_total) && fits(sl.wblk_cap, L.n_wfm) &&
+             fits(sl.h_wblk_cap, L.n_wfm);
+    return ok;
+}
+
+// Grow the slot's output buffers for a layout (invalidates the outputs of
+// the block the slot held).
+static int slot_grow(ServeSlot& sl, const BlockLayout& L)
+{
+    ARGUS_CHECK((hipError_t)grow_dev(&sl.d_iq, &sl.iq_cap, (size_t)std::max<int64_t>(L.iq_total, 1)));
+    ARGUS_CHECK((hipError_t)grow_dev(&sl.d_audio, &sl.audio_cap, (size_t)std::max<int64_t>(L.audio_total, 1)));
+    ARGUS_CHECK((hipError_t)grow_host(&sl.h_iq, &sl.h_iq_cap, (size_t)std::max<int64_t>(L.iq_total, 1)));
+    ARGUS_CHECK((hipError_t)grow_host(&sl.h_audio, &sl.h_audio_cap, (size_t)std::max<int64_t>(L.audio_total, 1)));
+    if (L.n_wfm > 0) {
+        ARGUS_CHECK((hipError_t)grow_dev(&sl.d_mono, &sl.mono_cap, (size_t)std::max<int64_t>(L.mono_total, 1)));
+        ARGUS_CHECK((hipError_t)grow_dev(&sl.d_wscratch, &sl.wscratch_cap, (size_t)std::max<int64_t>(4 * L.mono_total, 1)));
+        ARGUS_CHECK((hipError_t)grow_dev(&sl.d_rds, &sl.rds_cap, (size_t)std::max<int64_t>(L.rds_total, 1)));
+        ARGUS_CHECK((hipError_t)grow_host(&sl.h_rds, &sl.h_rds_cap, (size_t)std::max<int64_t>(L.rds_total, 1)));
+        ARGUS_CHECK((hipError_t)grow_dev(&sl.d_wblk, &sl.wblk_cap, (size_t)L.n_wfm));
+        ARGUS_CHECK((hipError_t)grow_host(&sl.h_wblk, &sl.h_wblk_cap, (size_t)L.n_wfm));
+    }
+    return 0;
+}
+
+// Make a layout the slot's current block (its totals and per-channel outputs).
+static void slot_adopt(ServeSlot& sl, BlockLayout& L)
+{
+    sl.outs.swap(L.outs);
+    sl.n_wfm = L.n_wfm;
+    sl.rds_total = L.rds_total;
+    sl.iq_total = L.iq_total;
+    sl.audio_total = L.audio_total;
+}
+
+// The block's tables for the live channels' current state, written into the
+// slot's host table for input `in` (a device pointer) of n samples: outs is
+// the block's layout (the slot's buffers must hold it).  Channel state is
+// not changed (commit_block advances it).  With cache_items the work items
+// are kept for later blocks of the same channel set, length and tune.
+static int describe_block(gpud_serve* s, ServeSlot& sl, const std::vector<ServeOut>& outs, int n_wfm, int n,
+                          const float2* in, bool cache_items)
+{
+    const int nch = (int)s->live.size();
     // Work items: reused while the channel set, block length and tune are
     // unchanged (tiles are planned for the most outputs a block of n samples
     // gives; the kernels skip tiles past a block's last output).
@@ -472,16 +497,16 @@ static int build_block(gpud_serve* s, ServeSlot& sl, int n, const float2* in, bo
     s->witems.clear();
     size_t lds = 0;
     int ci = 0;
-    for (auto& kv : s->live) {
-        ServeChannel& c = kv.second;
-        const ServeOut& o = sl.outs[(size_t)ci];
+    for (const auto& kv : s->live) {
+        const ServeChannel& c = kv.second;
+        const ServeOut& o = outs[(size_t)ci];
         const int keep = std::min(c.hist_len + n, c.T - 1);
         const double inc = -2.0 * M_PI * c.spec.offset_hz / (double)s->sample_rate;
         ArgusChanDesc& ch = s->chans[(size_t)ci];
         if (argus_fill_chan(&ch, c.d_hist[c.hist_sel], c.hist_len, c.d_hist[c.hist_sel ^ 1], keep, c.d_taps,
-                            sl.d_iq + o.iq_off, c.decim, c.T, c.phase_count, n, c.nco_phase, inc, s->tune) != 0)
+                            c.d_ctaps, sl.d_iq + o.iq_off, c.decim, c.T, c.phase_count, n, c.nco_phase, inc,
+                            s->tune) != 0)
             return -1;
-        ch.ctaps = c.d_ctaps;
         if ((c.decim - 1 + n) / c.decim > 0) lds = std::max(lds, argus_item_lds_bytes(ch));
         if (!reuse) {
             ArgusChanDesc full = ch;
@@ -510,20 +535,7 @@ static int build_block(gpud_serve* s, ServeSlot& sl, int n, const float2* in, bo
                     s->ditems.push_back({ci, st, std::min(ARGUS_DEMOD_TILE, o.n_iq - st), 0});
             }
         }
-        if (c.stereo) wfm_describe(c, o, sl, s->wdescs[(size_t)o.wfm_index], s->witems, commit);
-        if (commit) {
-            // carry state forward (the kernels compute with the values above)
-            c.phase_count = (c.phase_count + n) % c.decim;
-            c.nco_phase = argus_reduce_phase(c.nco_phase + inc * (double)n);
-            if (keep > 0) c.hist_sel ^= 1;
-            c.hist_len = keep;
-            if ((dd.mode == ARGUS_DEMOD_NFM || dd.mode == ARGUS_DEMOD_WFM) && o.n_iq > 0) {
-                c.carry_sel ^= 1;
-                c.have_carry = true;
-            }
-            c.bfo_phase = argus_reduce_phase(c.bfo_phase + bfo_inc * (double)o.n_iq);
-            c.last_ticket = sl.ticket;
-        }
+        if (c.stereo) wfm_describe(c, o, sl, s->wdescs[(size_t)o.wfm_index], s->witems);
         ci++;
     }
     if (!reuse) {
@@ -544,7 +556,7 @@ static int build_block(gpud_serve* s, ServeSlot& sl, int n, const float2* in, bo
     if (sl.d_tab != d_before) sl.items_dirty = true;  // a new device table holds no items yet
     if (!reuse) {
         if (n_items) std::memcpy(sl.h_tab, s->items.data(), items_bytes);
-        sl.items_epoch = commit ? s->epoch : 0;
+        sl.items_epoch = cache_items ? s->epoch : 0;
         sl.items_n = n;
         sl.n_items = n_items;
         sl.items_dirty = true;
@@ -565,6 +577,48 @@ static int build_block(gpud_serve* s, ServeSlot& sl, int n, const float2* in, bo
     sl.off_hdr = off_hdr;
     sl.lds = lds;
     return 0;
+}
+
+// Advance every live channel's carried state past a block of n samples with
+// the given layout (the values describe_block used become the previous
+// block's): NCO and decimator phase, history and FM-carry halves, BFO phase,
+// stereo delay-line half and RDS sample count.
+static void commit_block(gpud_serve* s, const std::vector<ServeOut>& outs, int n, int64_t ticket)
+{
+    int ci = 0;
+    for (auto& kv : s->live) {
+        ServeChannel& c = kv.second;
+        const ServeOut& o = outs[(size_t)ci++];
+        const int keep = std::min(c.hist_len + n, c.T - 1);
+        const double inc = -2.0 * M_PI * c.spec.offset_hz / (double)s->sample_rate;
+        const int mode = c.stereo ? ARGUS_DEMOD_WFM : c.spec.demod;
+        const double bfo_inc = 2.0 * M_PI * (mode == ARGUS_DEMOD_LSB ? -700.0 : 700.0) / (double)c.spec.out_rate;
+        c.phase_count = (c.phase_count + n) % c.decim;
+        c.nco_phase = argus_reduce_phase(c.nco_phase + inc * (double)n);
+        if (keep > 0) c.hist_sel ^= 1;
+        c.hist_len = keep;
+        if ((mode == ARGUS_DEMOD_NFM || mode == ARGUS_DEMOD_WFM) && o.n_iq > 0) {
+            c.carry_sel ^= 1;
+            c.have_carry = true;
+        }
+        c.bfo_phase = argus_reduce_phase(c.bfo_phase + bfo_inc * (double)o.n_iq);
+        c.last_ticket = ticket;
+        if (c.stereo) {
+            if (o.n_mono > 0) c.wfm_sel ^= 1;
+            c.rds_count += o.n_mono;
+        }
+    }
+}
+
+// Layout, buffers and tables of a block in slot sl, built at submit time.
+static int build_block(gpud_serve* s, ServeSlot& sl, int n, const float2* in, bool cache_items)
+{
+    BlockLayout L;
+    layout_block(s, n, L);
+    int err = slot_grow(sl, L);
+    if (err) return err;
+    slot_adopt(sl, L);
+    return describe_block(s, sl, sl.outs, sl.n_wfm, n, in, cache_items);
 }
 
 // Upload the block's table: the work items only when they were rebuilt.
@@ -632,18 +686,55 @@ static int launch_block(gpud_serve* s, ServeSlot& sl)
 
 // Enqueue one block: upload (unless device-resident), extract, demodulate,
 // download.  Returns immediately; *ticket identifies the block.
+// Build and upload the next block's tables ahead, into the next slot, while
+// the GPU works on the block just launched (ServePrep).  Skipped when that
+// slot's previous block is still running or its buffers would have to grow
+// (the outputs it holds stay valid until the next submit).
+static void prepare_next(gpud_serve* s, int n, const float2* in_dev)
+{
+    const int idx = s->next_slot;
+    ServeSlot& sl = s->slots[idx];
+    if (sl.busy) {
+        if (hipEventQuery(sl.ev_done) != hipSuccess) return;
+        if (slot_wait(s, sl) != 0) return;  // already complete: only marks it so
+    }
+    ServePrep& pp = s->prep;
+    layout_block(s, n, pp.layout);
+    if (!slot_fits(sl, pp.layout)) return;
+    const float2* in = in_dev ? in_dev : sl.d_in;
+    if (describe_block(s, sl, pp.layout.outs, pp.layout.n_wfm, n, in, true) != 0) return;
+    if (upload_table(s, sl) != 0) return;
+    pp.slot = idx;
+    pp.ticket = s->last_ticket + 1;
+    pp.epoch = s->epoch;
+    pp.n = n;
+    pp.in = in;
+    pp.valid = true;
+}
+
 static int serve_submit(gpud_serve* s, const gpud_float2* iq, int n, bool on_device, int64_t* ticket)
 {
     if (!s || n < 0 || n > s->block_samples || (n > 0 && !iq)) return -1;
-    ServeSlot& sl = s->slots[s->next_slot];
+    const int idx = s->next_slot;
+    ServeSlot& sl = s->slots[idx];
     int err = slot_wait(s, sl);  // the slot's previous block must be done
     if (err) return err;
-    s->next_slot = (s->next_slot + 1) % ARGUS_SERVE_SLOTS;
+    s->next_slot = (idx + 1) % ARGUS_SERVE_SLOTS;
     sl.ticket = ++s->last_ticket;
     sl.n = n;
     const float2* in = on_device ? (const float2*)iq : sl.d_in;
-    err = build_block(s, sl, n, in, true);
-    if (err) return err;
+    // tables prepared while the previous block ran, if nothing they depend on
+    // has changed since; else build them now
+    ServePrep& pp = s->prep;
+    const bool prepared =
+        pp.valid && pp.slot == idx && pp.ticket == sl.ticket && pp.epoch == s->epoch && pp.n == n && pp.in == in;
+    pp.valid = false;
+    if (prepared) {
+        slot_adopt(sl, pp.layout);
+    } else {
+        err = build_block(s, sl, n, in, true);
+        if (err) return err;
+    }
 
     // ---- copy stream: block upload -------------------------------------
     ARGUS_CHECK(hipEventRecord(sl.ev_start, s->copy));
@@ -654,12 +745,15 @@ static int serve_submit(gpud_serve* s, const gpud_float2* iq, int n, bool on_dev
     ARGUS_CHECK(hipEventRecord(sl.ev_in, s->copy));
     // ---- compute stream: one table copy, then the block's kernels ----------
     ARGUS_CHECK(hipStreamWaitEvent(s->compute, sl.ev_in, 0));
-    err = upload_table(s, sl);
-    if (err) return err;
+    if (!prepared) {  // a prepared table was uploaded when it was built
+        err = upload_table(s, sl);
+        if (err) return err;
+    }
     ARGUS_CHECK(hipEventRecord(sl.ev_kstart, s->compute));
     err = launch_block(s, sl);
     if (err) return err;
     ARGUS_CHECK(hipEventRecord(sl.ev_kern, s->compute));
+    commit_block(s, sl.outs, n, sl.ticket);
     // ---- copy stream: download ------------------------------------------
     ARGUS_CHECK(hipStreamWaitEvent(s->copy, sl.ev_kern, 0));
     if (s->download) {
@@ -679,6 +773,7 @@ static int serve_submit(gpud_serve* s, const gpud_float2* iq, int n, bool on_dev
     ARGUS_CHECK(hipEventRecord(sl.ev_done, s->copy));
     sl.busy = true;
     if (ticket) *ticket = sl.ticket;
+    prepare_next(s, n, on_device ? in : nullptr);
     return 0;
 }
 
@@ -785,7 +880,7 @@ GPUD_API int gpud_serve_set_tune(gpud_serve* s, const gpud_tune* t)
 // Time the fused extraction launch of the live channel set on the block at
 // iq_dev (n samples, device memory) for tile and launch shapes around the
 // current one, one parameter at a time (grid, then LDS budget per dense tile,
-// then outputs per narrow item), and keep the fastest.  Channel state does
+// then outputs per narrow item; two rounds), and keep the fastest.  Channel state does
 // not advance: the launches write a scratch slot's outputs and the inactive
 // history halves only.  Blocks in flight are completed first.
 GPUD_API int gpud_serve_autotune(gpud_serve* s, const gpud_float2* iq_dev, int n, int reps, gpud_tune* best)
@@ -828,15 +923,17 @@ GPUD_API int gpud_serve_autotune(gpud_serve* s, const gpud_float2* iq_dev, int n
     ArgusTune cur = s->tune;
     float cur_ms = 0.0f;
     if (!err) err = eval(cur, &cur_ms);
+    // coordinate descent, twice over (grid and LDS budget interact: smaller
+    // tiles mean more workgroups resident per CU)
     const int grids[] = {1024, 2048, 4096, 6144, 8192, 12288};
-    const int ldss[] = {4096, 6144, 8192, 10240, 12288};
+    const int ldss[] = {2048, 3072, 4096, 6144, 8192, 12288};
     const int narrows[] = {16, 32, 64};
-    for (int pass = 0; pass < 3 && !err; pass++) {
-        const int* vals = pass == 0 ? grids : pass == 1 ? ldss : narrows;
-        const int nv = pass == 0 ? 6 : pass == 1 ? 5 : 3;
+    for (int pass = 0; pass < 6 && !err; pass++) {
+        const int* vals = pass % 3 == 0 ? grids : pass % 3 == 1 ? ldss : narrows;
+        const int nv = pass % 3 == 2 ? 3 : 6;
         for (int i = 0; i < nv && !err; i++) {
             ArgusTune t = cur;
-            int& field = pass == 0 ? t.grid : pass == 1 ? t.lds_entries : t.narrow_outputs;
+            int& field = pass % 3 == 0 ? t.grid : pass % 3 == 1 ? t.lds_entries : t.narrow_outputs;
             if (field == vals[i]) continue;
             field = vals[i];
             float ms = 0.0f;

@@ -97,6 +97,28 @@ struct ServeSlot {
     std::vector<ServeOut> outs;
 };
 
+// Output layout of one block over the live channels.
+struct BlockLayout {
+    std::vector<ServeOut> outs;
+    int n_wfm = 0;
+    int64_t iq_total = 0, audio_total = 0, mono_total = 0, rds_total = 0;
+};
+
+// A block prepared ahead: after a block is launched, the next block's
+// tables are built and uploaded into the next slot while the GPU works, on
+// the assumption that it comes with the same channel set, length and input
+// buffer.  Its layout waits here (the slot still holds the outputs of its
+// previous block until the next submit).
+struct ServePrep {
+    bool valid = false;
+    int slot = -1;
+    int64_t ticket = 0;
+    uint64_t epoch = 0;
+    int n = 0;
+    const float2* in = nullptr;
+    BlockLayout layout;
+};
+
 // What the graph-mode kernels of one block read (first bytes of the table).
 struct ArgusServeHdr {
     ArgusExtractHdr x;
@@ -120,6 +142,7 @@ struct gpud_serve {
     int next_slot = 0;
     int64_t last_ticket = 0;
     int64_t completed_ticket = 0;
+    ServePrep prep;
     std::vector<ArgusWorkItem> items;
     std::vector<int> item_pos;
     std::vector<ArgusChanDesc> chans;
