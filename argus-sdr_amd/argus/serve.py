@@ -27,7 +27,7 @@ class gpud_channel_spec(C.Structure):
 class gpud_serve_block_info(C.Structure):
     _fields_ = [("ticket", C.c_int64), ("n_channels", C.c_int32), ("n_samples", C.c_int32),
                 ("iq_total", C.c_int64), ("audio_total", C.c_int64), ("gpu_ms", C.c_float),
-                ("kernel_ms", C.c_float)]
+                ("kernel_ms", C.c_float), ("extract_ms", C.c_float), ("pad", C.c_float)]
 
 
 P, I = C.c_void_p, C.c_int
@@ -51,6 +51,7 @@ register("gpud_serve_set_graph", I, [P, I])
 register("gpud_serve_get_tune", I, [P, C.POINTER(gpud_tune)])
 register("gpud_serve_set_tune", I, [P, C.POINTER(gpud_tune)])
 register("gpud_serve_autotune", I, [P, P, I, I, C.POINTER(gpud_tune)])
+register("gpud_serve_extract_time", I, [P, P, I, I, C.POINTER(C.c_float), C.POINTER(C.c_float)])
 register("gpud_runner_get_tune", I, [P, C.POINTER(gpud_tune)])
 register("gpud_runner_set_tune", I, [P, C.POINTER(gpud_tune)])
 register("gpud_serve_channel_wfm", I, [P, C.c_int64, C.c_int64, C.POINTER(P), C.POINTER(I), C.POINTER(I),
@@ -175,3 +176,11 @@ class Serve:
         t = gpud_tune()
         check("gpud_serve_autotune", lib().gpud_serve_autotune(self._h, dev_ptr, n, reps, C.byref(t)))
         return t
+
+    def ExtractTime(self, dev_ptr: int, n: int, reps: int = 20):
+        """(average, minimum) ms of the fused extraction launch of the live
+        channel set at the current tune (scratch slot, state unchanged)."""
+        avg, mn = C.c_float(), C.c_float()
+        check("gpud_serve_extract_time",
+              lib().gpud_serve_extract_time(self._h, dev_ptr, n, reps, C.byref(avg), C.byref(mn)))
+        return avg.value, mn.value

@@ -168,7 +168,8 @@ static void slot_release(ServeSlot& sl)
     hipHostFree(sl.h_wblk);
     hipHostFree(sl.h_tab);
     hipFree(sl.d_tab);
-    if (sl.gexec) hipGraphExecDestroy(sl.gexec);
+    for (auto& ge : sl.gexec)
+        if (ge) hipGraphExecDestroy(ge);
     sl.d_iq = nullptr;
     sl.d_audio = nullptr;
     sl.h_iq = nullptr;
@@ -181,7 +182,7 @@ static void slot_release(ServeSlot& sl)
     sl.h_wblk = nullptr;
     sl.h_tab = nullptr;
     sl.d_tab = nullptr;
-    sl.gexec = nullptr;
+    sl.gexec[0] = sl.gexec[1] = nullptr;
     sl.iq_cap = sl.audio_cap = sl.h_iq_cap = sl.h_audio_cap = 0;
     sl.mono_cap = sl.wscratch_cap = sl.rds_cap = sl.h_rds_cap = sl.wblk_cap = sl.h_wblk_cap = 0;
     sl.tab_cap = 0;
@@ -217,6 +218,7 @@ GPUD_API int gpud_serve_create(int sample_rate, int block_samples, uint64_t hbm_
         ARGUS_CHECK(hipEventCreate(&sl.ev_done));
         ARGUS_CHECK(hipEventCreate(&sl.ev_start));
         ARGUS_CHECK(hipEventCreate(&sl.ev_kstart));
+        ARGUS_CHECK(hipEventCreate(&sl.ev_xend));  // timed: extract_ms (recorded inside the block's graph)
         ARGUS_CHECK(dev_alloc(&sl.d_in, (size_t)block_samples));
         ARGUS_CHECK(host_alloc(&sl.h_in, (size_t)block_samples));
     }
@@ -237,6 +239,7 @@ GPUD_API int gpud_serve_destroy(gpud_serve* s)
         hipEventDestroy(sl.ev_done);
         hipEventDestroy(sl.ev_start);
         hipEventDestroy(sl.ev_kstart);
+        hipEventDestroy(sl.ev_xend);
         hipFree(sl.d_in);
         hipHostFree(sl.h_in);
         slot_release(sl);
@@ -634,54 +637,66 @@ static int upload_table(gpud_serve* s, ServeSlot& sl)
     return 0;
 }
 
-// The block's kernels in order, each reading its counts and pointers from the
-// table header on the device.
-static int launch_eager(gpud_serve* s, ServeSlot& sl, bool wfm)
+// The block's launches in two stages, each reading its counts and pointers
+// from the table header on the device: the fused extraction, then the
+// demodulation and (with stereo channels) the stereo stage.
+static int launch_stage(gpud_serve* s, ServeSlot& sl, int stage, bool wfm)
 {
     const ArgusServeHdr* dh = (const ArgusServeHdr*)(sl.d_tab + sl.off_hdr);
-    int err = argus_launch_extract_hdr(&dh->x, sl.lds, s->tune.grid, s->compute);
-    if (err) return err;
-    err = argus_launch_demod_hdr(&dh->d, s->compute);
+    if (stage == 0) return argus_launch_extract_hdr(&dh->x, sl.lds, s->tune.grid, s->compute);
+    int err = argus_launch_demod_hdr(&dh->d, s->compute);
     if (err) return err;
     return wfm ? argus_launch_wfm_hdr(&dh->w, s->compute) : 0;
 }
 
-// Replay the slot's captured graph of the block's launches, capturing it
-// again when what it depends on (table address, LDS size, grid, whether a
-// stereo stage runs) has changed.
+// Run the block's two stages, recording ev_xend between them (the
+// extraction's end, for extract_ms).  In graph mode each stage is the
+// replay of a captured per-slot graph, captured again when what it depends
+// on (table address, LDS size, grid, whether a stereo stage runs) has
+// changed; the event stays outside the graphs so its time can be read.
 static int launch_block(gpud_serve* s, ServeSlot& sl)
 {
     const bool wfm = sl.n_wfm > 0;
-    if (!s->graph) return launch_eager(s, sl, wfm);
+    if (!s->graph) {
+        int err = launch_stage(s, sl, 0, wfm);
+        if (err) return err;
+        ARGUS_CHECK(hipEventRecord(sl.ev_xend, s->compute));
+        return launch_stage(s, sl, 1, wfm);
+    }
     const char* hdr = sl.d_tab + sl.off_hdr;
-    if (!sl.gexec || sl.g_hdr != hdr || sl.g_lds != sl.lds || sl.g_grid != s->tune.grid ||
+    if (!sl.gexec[0] || sl.g_hdr != hdr || sl.g_lds != sl.lds || sl.g_grid != s->tune.grid ||
         sl.g_wfm != (int)wfm) {
-        if (sl.gexec) {
-            hipGraphExecDestroy(sl.gexec);
-            sl.gexec = nullptr;
-        }
+        for (auto& ge : sl.gexec)
+            if (ge) {
+                hipGraphExecDestroy(ge);
+                ge = nullptr;
+            }
         int lerr = argus_extract_allow_lds(sl.lds);  // outside capture
         if (lerr) return lerr;
-        ARGUS_CHECK(hipStreamBeginCapture(s->compute, hipStreamCaptureModeThreadLocal));
-        int err = launch_eager(s, sl, wfm);
-        hipGraph_t g = nullptr;
-        hipError_t e = hipStreamEndCapture(s->compute, &g);
-        if (err || e != hipSuccess) {
-            if (g) hipGraphDestroy(g);
-            return err ? err : (int)e;
-        }
-        e = hipGraphInstantiate(&sl.gexec, g, nullptr, nullptr, 0);
-        hipGraphDestroy(g);
-        if (e != hipSuccess) {
-            sl.gexec = nullptr;
-            return (int)e;
+        for (int stage = 0; stage < 2; stage++) {
+            ARGUS_CHECK(hipStreamBeginCapture(s->compute, hipStreamCaptureModeThreadLocal));
+            int err = launch_stage(s, sl, stage, wfm);
+            hipGraph_t g = nullptr;
+            hipError_t e = hipStreamEndCapture(s->compute, &g);
+            if (err || e != hipSuccess) {
+                if (g) hipGraphDestroy(g);
+                return err ? err : (int)e;
+            }
+            e = hipGraphInstantiate(&sl.gexec[stage], g, nullptr, nullptr, 0);
+            hipGraphDestroy(g);
+            if (e != hipSuccess) {
+                sl.gexec[stage] = nullptr;
+                return (int)e;
+            }
         }
         sl.g_hdr = hdr;
         sl.g_lds = sl.lds;
         sl.g_grid = s->tune.grid;
         sl.g_wfm = (int)wfm;
     }
-    return (int)hipGraphLaunch(sl.gexec, s->compute);
+    ARGUS_CHECK(hipGraphLaunch(sl.gexec[0], s->compute));
+    ARGUS_CHECK(hipEventRecord(sl.ev_xend, s->compute));
+    return (int)hipGraphLaunch(sl.gexec[1], s->compute);
 }
 
 // Enqueue one block: upload (unless device-resident), extract, demodulate,
@@ -823,6 +838,8 @@ GPUD_API int gpud_serve_wait(gpud_serve* s, int64_t ticket, gpud_serve_block_inf
         info->gpu_ms = ms;
         ARGUS_CHECK(hipEventElapsedTime(&ms, sl->ev_kstart, sl->ev_kern));
         info->kernel_ms = ms;
+        ARGUS_CHECK(hipEventElapsedTime(&ms, sl->ev_kstart, sl->ev_xend));
+        info->extract_ms = ms;
     }
     return 0;
 }
@@ -883,6 +900,59 @@ GPUD_API int gpud_serve_set_tune(gpud_serve* s, const gpud_tune* t)
 // then outputs per narrow item; two rounds), and keep the fastest.  Channel state does
 // not advance: the launches write a scratch slot's outputs and the inactive
 // history halves only.  Blocks in flight are completed first.
+// Event-timed fused extraction launches of the live channel set at the
+// current tune on the block at iq_dev, using a scratch slot (outputs and the
+// inactive history halves only; channel state does not advance): one
+// warm-up launch, then reps timed ones.
+static int time_extract(gpud_serve* s, ServeSlot& scratch, const float2* iq_dev, int n, int reps, hipEvent_t e0,
+                        hipEvent_t e1, float* min_ms, float* avg_ms)
+{
+    s->epoch++;  // the scratch slot's table is built for this call only
+    int rc = build_block(s, scratch, n, iq_dev, false);
+    if (rc) return rc;
+    rc = argus_extract_allow_lds(scratch.lds);
+    if (rc) return rc;
+    ARGUS_CHECK(hipMemcpyAsync(scratch.d_tab, scratch.h_tab, scratch.tab_bytes, hipMemcpyHostToDevice, s->compute));
+    const ArgusServeHdr* dh = (const ArgusServeHdr*)(scratch.d_tab + scratch.off_hdr);
+    rc = argus_launch_extract_hdr(&dh->x, scratch.lds, s->tune.grid, s->compute);  // warm-up
+    if (rc) return rc;
+    float bestm = 1e30f, sum = 0.0f;
+    for (int r = 0; r < reps; r++) {
+        ARGUS_CHECK(hipEventRecord(e0, s->compute));
+        rc = argus_launch_extract_hdr(&dh->x, scratch.lds, s->tune.grid, s->compute);
+        if (rc) return rc;
+        ARGUS_CHECK(hipEventRecord(e1, s->compute));
+        ARGUS_CHECK(hipEventSynchronize(e1));
+        float m = 0.0f;
+        ARGUS_CHECK(hipEventElapsedTime(&m, e0, e1));
+        bestm = std::min(bestm, m);
+        sum += m;
+    }
+    if (min_ms) *min_ms = bestm;
+    if (avg_ms) *avg_ms = sum / (float)reps;
+    return 0;
+}
+
+GPUD_API int gpud_serve_extract_time(gpud_serve* s, const gpud_float2* iq_dev, int n, int reps, float* avg_ms,
+                                     float* min_ms)
+{
+    if (!s || !iq_dev || n <= 0 || n > s->block_samples || reps <= 0) return -1;
+    for (auto& sl : s->slots) {
+        int err = slot_wait(s, sl);
+        if (err) return err;
+    }
+    ServeSlot scratch;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    int err = (int)hipEventCreate(&e0);
+    if (!err) err = (int)hipEventCreate(&e1);
+    if (!err) err = time_extract(s, scratch, (const float2*)iq_dev, n, reps, e0, e1, min_ms, avg_ms);
+    hipStreamSynchronize(s->compute);
+    slot_release(scratch);
+    if (e0) hipEventDestroy(e0);
+    if (e1) hipEventDestroy(e1);
+    return err;
+}
+
 GPUD_API int gpud_serve_autotune(gpud_serve* s, const gpud_float2* iq_dev, int n, int reps, gpud_tune* best)
 {
     if (!s || !iq_dev || n <= 0 || n > s->block_samples || reps <= 0) return -1;
@@ -896,29 +966,7 @@ GPUD_API int gpud_serve_autotune(gpud_serve* s, const gpud_float2* iq_dev, int n
     if (!err) err = (int)hipEventCreate(&e1);
     auto eval = [&](const ArgusTune& t, float* ms) -> int {
         s->tune = t;
-        s->epoch++;
-        int rc = build_block(s, scratch, n, (const float2*)iq_dev, false);
-        if (rc) return rc;
-        rc = argus_extract_allow_lds(scratch.lds);
-        if (rc) return rc;
-        ARGUS_CHECK(hipMemcpyAsync(scratch.d_tab, scratch.h_tab, scratch.tab_bytes, hipMemcpyHostToDevice,
-                                   s->compute));
-        const ArgusServeHdr* dh = (const ArgusServeHdr*)(scratch.d_tab + scratch.off_hdr);
-        rc = argus_launch_extract_hdr(&dh->x, scratch.lds, t.grid, s->compute);  // warm-up
-        if (rc) return rc;
-        float bestm = 1e30f;
-        for (int r = 0; r < reps; r++) {
-            ARGUS_CHECK(hipEventRecord(e0, s->compute));
-            rc = argus_launch_extract_hdr(&dh->x, scratch.lds, t.grid, s->compute);
-            if (rc) return rc;
-            ARGUS_CHECK(hipEventRecord(e1, s->compute));
-            ARGUS_CHECK(hipEventSynchronize(e1));
-            float m = 0.0f;
-            ARGUS_CHECK(hipEventElapsedTime(&m, e0, e1));
-            bestm = std::min(bestm, m);
-        }
-        *ms = bestm;
-        return 0;
+        return time_extract(s, scratch, (const float2*)iq_dev, n, reps, e0, e1, ms, nullptr);
     };
     ArgusTune cur = s->tune;
     float cur_ms = 0.0f;

@@ -47,6 +47,7 @@ struct ServeSlot {
     bool busy = false;
     int n = 0;
     hipEvent_t ev_start = nullptr, ev_in = nullptr, ev_kstart = nullptr, ev_kern = nullptr, ev_done = nullptr;
+    hipEvent_t ev_xend = nullptr;  // after the extraction launch
     float2* d_in = nullptr;
     float2* h_in = nullptr;
     float2* d_iq = nullptr;
@@ -72,8 +73,9 @@ struct ServeSlot {
     int items_n = -1;
     int n_items = 0;
     bool items_dirty = true;
-    // captured launch sequence of this slot and what it was captured for
-    hipGraphExec_t gexec = nullptr;
+    // captured launches of this slot (extraction; demodulation + stereo
+    // stage) and what they were captured for
+    hipGraphExec_t gexec[2] = {nullptr, nullptr};
     const char* g_hdr = nullptr;
     size_t g_lds = 0;
     int g_grid = 0;

@@ -84,18 +84,22 @@ def alg_figures(chans, n_in):
 
 
 def load_traffic(channels):
-    """Measured HBM bytes per fused launch (rocprofv3 FETCH_SIZE + WRITE_SIZE,
-    scaled by a copy of known size) for this channel count, if the profile
-    summary holds it."""
+    """(bytes, profile): measured HBM bytes per fused launch (rocprofv3
+    FETCH_SIZE + WRITE_SIZE, scaled by a copy of known size) if the profile
+    summary was taken at this channel count, else None; and what the profile
+    holds (its channel count, bytes, and their ratio to the algorithmic
+    bytes), or None."""
     p = ROOT / "profiles" / "extract_traffic.json"
-    if p.exists():
-        try:
-            d = json.loads(p.read_text())
-        except (ValueError, OSError):
-            return None
-        if d.get("channels") == channels:
-            return d.get("bytes_per_launch")
-    return None
+    if not p.exists():
+        return None, None
+    try:
+        d = json.loads(p.read_text())
+    except (ValueError, OSError):
+        return None, None
+    prof = {"channels": d.get("channels"), "bytes_per_launch": d.get("bytes_per_launch"),
+            "over_algorithmic": round(d["bytes_per_launch"] / d["algorithmic_bytes_per_launch"], 3)
+            if d.get("algorithmic_bytes_per_launch") else None}
+    return (d.get("bytes_per_launch") if d.get("channels") == channels else None), prof
 
 
 def percentile(v, q):
@@ -114,9 +118,14 @@ def bench_serve(a, chans, blocks, lib, S):
         check("gpud_malloc", lib.gpud_malloc(C.byref(p), blk.nbytes))
         check("gpud_memcpy_h2d", lib.gpud_memcpy_h2d(p, blk.ctypes.data, blk.nbytes))
         d_blocks.append(p.value)
-    tune = s.Autotune(d_blocks[0], BLOCK, reps=3)
-    a.tune = {"lds_entries": tune.lds_entries, "narrow_outputs": tune.narrow_outputs, "grid": tune.grid,
-              "extract_ms_at_tune": round(tune.kernel_ms, 4)}
+    if a.tune_fixed:
+        s.SetTune(*a.tune_fixed)
+        a.tune = {"lds_entries": a.tune_fixed[0], "narrow_outputs": a.tune_fixed[1], "grid": a.tune_fixed[2],
+                  "autotuned": False}
+    else:
+        tune = s.Autotune(d_blocks[0], BLOCK, reps=3)
+        a.tune = {"lds_entries": tune.lds_entries, "narrow_outputs": tune.narrow_outputs, "grid": tune.grid,
+                  "extract_ms_at_tune": round(tune.kernel_ms, 4), "autotuned": True}
     s.SetDownload(False)
     pending = []
     for i in range(a.warmup):
@@ -126,7 +135,7 @@ def bench_serve(a, chans, blocks, lib, S):
     for t in pending:
         s.Wait(t)
     check("gpud_device_sync", lib.gpud_device_sync())
-    kern, gpu = [], []
+    kern, gpu, xk = [], [], []
     pending = []
     t0 = time.perf_counter()
     for i in range(a.steps):
@@ -135,10 +144,15 @@ def bench_serve(a, chans, blocks, lib, S):
             info = s.Wait(pending.pop(0))
             kern.append(info.kernel_ms)
             gpu.append(info.gpu_ms)
+            xk.append(info.extract_ms)
     for t in pending:
         info = s.Wait(t)
         kern.append(info.kernel_ms)
         gpu.append(info.gpu_ms)
+        xk.append(info.extract_ms)
+    # the fused extraction launch alone, event-timed inside each block's
+    # graph on the compute stream over the timed steps
+    a.extract_ms = float(np.mean(xk))
     check("gpud_device_sync", lib.gpud_device_sync())
     wall = time.perf_counter() - t0
     # host-to-host latency: upload from host memory, kernels, download
@@ -155,14 +169,17 @@ def bench_serve(a, chans, blocks, lib, S):
     return wall * 1e3 / a.steps, kern, gpu, lat
 
 
-def serve_block_times(chans, d_block, lib, S, blocks=50, warmup=5):
+def serve_block_times(chans, d_block, lib, S, blocks=50, warmup=5, tune_fixed=None):
     """Latency of each block through the service, one block in flight: wall
     time from the submit call (host-side descriptor build included) to the
     return of its wait, device-resident input, outputs left in HBM."""
     s = S.Serve(FS, BLOCK, 256 << 30)
     for (cid, off, bw, rate, taps, dm) in chans:
         assert s.Open(S.ChannelSpec(cid, off, bw, rate, taps, dm, 0))
-    s.Autotune(d_block, BLOCK, reps=2)
+    if tune_fixed:
+        s.SetTune(*tune_fixed)
+    else:
+        s.Autotune(d_block, BLOCK, reps=2)
     s.SetDownload(False)
     out = []
     for i in range(warmup + blocks):
@@ -446,7 +463,10 @@ def main() -> int:
                     help="skip surveillance, the 2.048 Msps case and the stereo multiplex case")
     ap.add_argument("--calibrate-copy-mb", type=int, default=0,
                     help="run one device-to-device copy of this many MiB (counter calibration)")
+    ap.add_argument("--tune", default="",
+                    help="LDS_ENTRIES,NARROW_OUTPUTS,GRID: use this tile/launch shape instead of autotuning")
     a = ap.parse_args()
+    a.tune_fixed = tuple(int(x) for x in a.tune.split(",")) if a.tune else None
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: the extraction path shards by channel and this build runs on a "
               "single MI355X; run with --gpus 1.", file=sys.stderr)
@@ -483,8 +503,9 @@ def main() -> int:
         n_rt = a.channels
     else:
         print("[bench] fused capacity sweep (p99 block time <= 5 ms)", file=sys.stderr, flush=True)
-        n_rt, sweep = capacity_sweep(lambda n: serve_block_times(make_channels(n), d0.value, L, S),
-                                     256, P99_BOUND_MS, a.sweep_cap, 256)
+        n_rt, sweep = capacity_sweep(
+            lambda n: serve_block_times(make_channels(n), d0.value, L, S, tune_fixed=a.tune_fixed),
+            256, P99_BOUND_MS, a.sweep_cap, 256)
     print("[bench] plain per-signal capacity sweep", file=sys.stderr, flush=True)
     n_plain, plain_sweep = capacity_sweep(lambda n: plain_block_times(make_channels(n), blocks[0], G),
                                           4, P99_BOUND_MS, a.plain_cap, 1)
@@ -493,11 +514,13 @@ def main() -> int:
     print(f"[bench] serve: {n_rt} channels, block {BLOCK} samples @ {FS / 1e6:.0f} Msps", file=sys.stderr,
           flush=True)
     ms_per_step, kern, gpu, lat = bench_serve(a, chans, blocks, L, S)
-    print("[bench] fused extraction kernel", file=sys.stderr, flush=True)
-    kern_ms, jobs = bench_extract_kernel(a, chans, blocks, L, G)
+    kern_ms = a.extract_ms
+    print("[bench] fused extraction kernel through the batch runner", file=sys.stderr, flush=True)
+    runner_ms, jobs = bench_extract_kernel(a, chans, blocks, L, G)
     b_alg, f_alg = alg_figures(chans, BLOCK)
     achieved = b_alg / (kern_ms * 1e-3) / 1e9
     valu = f_alg / (kern_ms * 1e-3) / 1e12
+    traffic, traffic_prof = load_traffic(n_rt)
     L.gpud_free(d0.value)
 
     extras = {}
@@ -552,11 +575,16 @@ def main() -> int:
         "capacity_sweep": table(sweep),
         "plain_per_signal": {"real_time_channels": n_plain, "sweep": table(plain_sweep),
                              "fused_over_plain": round(n_rt / max(n_plain, 1), 1)},
-        "roofline": {"kernel": "argus_extract_batch_kernel", "bound": "hbm", "achieved": round(achieved, 1),
-                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": load_traffic(n_rt), "kernel_ms": round(kern_ms, 4), "alg_bytes": b_alg,
-                     "alg_flops": f_alg, "valu_tflops": round(valu, 2),
-                     "valu_frac": round(valu / FP32_PEAK_TFLOPS, 4)},
+        # The extraction kernel is compute-bound (alg_flops / alg_bytes ~ 170
+        # flop/byte against a machine balance of ~20): its roof is FP32
+        # arithmetic, computed on the vector ALUs (no matrix cores), against
+        # MI355X's published FP32 peak.  The HBM figures sit beside it.
+        "roofline": {"kernel": "argus_extract_hdr_kernel", "bound": "mfma", "compute": "fp32 VALU",
+                     "achieved": round(valu, 2), "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": round(valu / FP32_PEAK_TFLOPS, 4), "traffic": traffic, "traffic_profile": traffic_prof,
+                     "kernel_ms": round(kern_ms, 4), "alg_flops": f_alg, "alg_bytes": b_alg,
+                     "hbm_gbps": round(achieved, 1), "hbm_frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "runner_batch_kernel_ms": round(runner_ms, 4)},
         "cpu_baseline": cpu,
     }
     out.update(extras)

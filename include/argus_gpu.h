@@ -261,6 +261,8 @@ typedef struct gpud_serve_block_info {
     int64_t audio_total;
     float   gpu_ms;       /* event time: upload start -> download end */
     float   kernel_ms;    /* event time: extraction + demodulation kernels */
+    float   extract_ms;   /* event time: the fused extraction launch alone */
+    float   pad;
 } gpud_serve_block_info;
 
 GPUD_API int gpud_serve_create(int sample_rate, int block_samples, uint64_t hbm_budget_bytes, gpud_serve** out);
@@ -291,6 +293,14 @@ GPUD_API int gpud_serve_set_tune(gpud_serve* s, const gpud_tune* t);
  * at iq_dev (device memory, n samples): times the fused extraction launch
  * (best of reps) over grid, LDS budget and narrow item size, one parameter at
  * a time, keeps the fastest and returns it.  Channel state does not advance. */
+/* Event-timed fused extraction launch of the live channel set at the
+ * current tune on the block at iq_dev (n samples, device memory): reps
+ * launches on a scratch slot after one warm-up (blocks in flight are
+ * completed first; channel state does not advance); average and minimum
+ * milliseconds (either pointer may be NULL).  No reference counterpart:
+ * the measurement behind a kernel roofline. */
+GPUD_API int gpud_serve_extract_time(gpud_serve* s, const gpud_float2* iq_dev, int n, int reps, float* avg_ms,
+                                     float* min_ms);
 GPUD_API int gpud_serve_autotune(gpud_serve* s, const gpud_float2* iq_dev, int n, int reps, gpud_tune* best);
 /* RDS baseband samples (rate = out_rate / floor(out_rate / 19000)) and the
  * stereo lock of a GPUD_DEMOD_WFM_STEREO channel in a completed block:

@@ -75,6 +75,9 @@ def test_graph_replays_equal_eager(S):
 
 
 def test_autotune_keeps_state_and_parity(S, gpu):
+    """Autotuning and timing the extraction launch between blocks leave the
+    channels' streams exactly where the oracle expects them; each block's
+    wait reports its extraction time within its kernel time."""
     from argus._lib import check, lib
     L = lib()
     x = scene(50_000 * 8, seed=2)
@@ -95,9 +98,13 @@ def test_autotune_keeps_state_and_parity(S, gpu):
             print("tuned", t.lds_entries, t.narrow_outputs, t.grid, t.kernel_ms, "from", before.lds_entries,
                   before.narrow_outputs, before.grid, flush=True)
             assert t.grid % 8 == 0 and t.kernel_ms > 0
+        if k == 5:  # timing the extraction launch also leaves the stream untouched
+            avg, mn = s.ExtractTime(d.value, 50_000, reps=3)
+            assert 0 < mn <= avg
         blk = x[50_000 * k:50_000 * (k + 1)]
         tk = s.Submit(blk)
-        s.Wait(tk)
+        info = s.Wait(tk)
+        assert 0 < info.extract_ms <= info.kernel_ms
         for cid, oc in orc.items():
             want = oc.block(blk)
             got = s.Output(tk, cid)
