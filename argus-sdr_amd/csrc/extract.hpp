@@ -138,32 +138,40 @@ static inline int argus_dense_entries(int P, int PS, int Lp)
 }
 
 // Dense geometry within `entries` complex words of LDS.  Returns 0, or -1 if
-// fewer than 4 output groups fit.
+// fewer than 4 output groups fit.  The bank-padded plane stride is taken if
+// it keeps at least 7/8 of the largest tile the budget holds unpadded.
 static inline int argus_plan_dense(ArgusChanDesc* ch, int entries)
 {
     const int P = ch->P, Lp = argus_dense_lp(ch->L);
     const int need = ch->n_out > 0 ? (ch->n_out + 3) / 4 : 1;
-    int G = entries / P / 5;  // upper bound: a plane holds 4G + Lp - 1 positions at 5 words per 4
-    if (G > ARGUS_EXTRACT_THREADS) G = ARGUS_EXTRACT_THREADS;
-    if (G > need) G = need;
-    // at least 4 groups per tile, unless the chunk has fewer outputs
-    for (; G > 0 && (G >= 4 || G == need); G--) {
+    int Gmax = entries / P / 5;  // upper bound: a plane holds 4G + Lp - 1 positions at 5 words per 4
+    if (Gmax > ARGUS_EXTRACT_THREADS) Gmax = ARGUS_EXTRACT_THREADS;
+    if (Gmax > need) Gmax = need;
+    auto lanes = [&](int G) {
         int S = 1;
         while (S * 2 * G <= ARGUS_EXTRACT_THREADS && S * 2 <= P && S < 64) S *= 2;
+        return S;
+    };
+    // largest G with the padded stride, and with the plain one
+    int Gpad = 0, Graw = 0;
+    for (int G = Gmax; G > 0 && (G >= 4 || G == need) && (!Gpad || !Graw); G--) {
         const int words = argus_dense_words(G, Lp);
-        int PS = argus_plane_stride(words, S);
-        if (argus_dense_entries(P, PS, Lp) > entries) PS = words;  // no room for bank padding
-        if (argus_dense_entries(P, PS, Lp) <= entries) {
-            ch->narrow = 0;
-            ch->R = ARGUS_DENSE_R;
-            ch->M = 4 * G;
-            ch->SP = (P * PS + 1) & ~1;  // word offset of the taps (even)
-            ch->PS = PS;
-            ch->S = S;
-            return 0;
-        }
+        if (!Gpad && argus_dense_entries(P, argus_plane_stride(words, lanes(G)), Lp) <= entries) Gpad = G;
+        if (!Graw && argus_dense_entries(P, words, Lp) <= entries) Graw = G;
     }
-    return -1;
+    if (!Gpad && !Graw) return -1;
+    const bool pad = Gpad && 8 * Gpad >= 7 * Graw;
+    const int G = pad ? Gpad : Graw;
+    const int S = lanes(G);
+    const int words = argus_dense_words(G, Lp);
+    const int PS = pad ? argus_plane_stride(words, S) : words;
+    ch->narrow = 0;
+    ch->R = ARGUS_DENSE_R;
+    ch->M = 4 * G;
+    ch->SP = (P * PS + 1) & ~1;  // word offset of the taps (even)
+    ch->PS = PS;
+    ch->S = S;
+    return 0;
 }
 
 // Host-side planning for one channel.  Fills M, PS, SP, R, S, narrow.
