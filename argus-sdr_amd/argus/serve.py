@@ -4,6 +4,7 @@
     s.Open(ChannelSpec(...))            # False if the HBM budget refuses it
     t = s.Submit(iq_block)              # async; returns a ticket
     info = s.Wait(t); iq, audio, rate = s.Output(t, channel_id)
+    s.SetAudio(channel_id, 48000, 50.0); frames, rate = s.OutputAudio(t, channel_id)
 """
 from __future__ import annotations
 
@@ -56,6 +57,9 @@ register("gpud_runner_get_tune", I, [P, C.POINTER(gpud_tune)])
 register("gpud_runner_set_tune", I, [P, C.POINTER(gpud_tune)])
 register("gpud_serve_channel_wfm", I, [P, C.c_int64, C.c_int64, C.POINTER(P), C.POINTER(I), C.POINTER(I),
                                        C.POINTER(I), C.POINTER(I)])
+register("gpud_serve_set_audio", I, [P, C.c_int64, I, C.c_double])
+register("gpud_serve_channel_audio", I, [P, C.c_int64, C.c_int64, C.POINTER(P), C.POINTER(I), C.POINTER(I),
+                                         C.POINTER(I)])
 
 
 @dataclass
@@ -159,6 +163,31 @@ class Serve:
         rds = np.ctypeslib.as_array((C.c_float * (2 * n.value)).from_address(p.value)).view(np.complex64).copy() \
             if n.value else np.zeros(0, np.complex64)
         return rds, rate.value, bool(locked.value), bool(stereo.value)
+
+    def SetAudio(self, channel_id: int, audio_rate: int = 48000, deemphasis_us: float = 50.0) -> bool:
+        """Turn a live channel's audio stage on (audio_rate > 0: low-pass for
+        plain WFM, resampler to audio_rate, WFM de-emphasis and gain) or off
+        (0).  False if the HBM budget refuses it."""
+        rc = lib().gpud_serve_set_audio(self._h, channel_id, audio_rate, deemphasis_us)
+        if rc == ERR_BUDGET:
+            return False
+        check("gpud_serve_set_audio", rc)
+        return True
+
+    def OutputAudio(self, ticket: int, channel_id: int):
+        """(frames float32 copy shaped (n, channels), rate) of a channel's
+        audio stage in a completed block; None if the channel was not in the
+        block or its stage was off."""
+        p = C.c_void_p()
+        n, ch, rate = C.c_int(), C.c_int(), C.c_int()
+        rc = lib().gpud_serve_channel_audio(self._h, ticket, channel_id, C.byref(p), C.byref(n), C.byref(ch),
+                                            C.byref(rate))
+        if rc in (-1, -3):
+            return None
+        check("gpud_serve_channel_audio", rc)
+        cnt = n.value * ch.value
+        a = np.ctypeslib.as_array((C.c_float * cnt).from_address(p.value)).copy() if cnt else np.zeros(0, np.float32)
+        return a.reshape(n.value, ch.value), rate.value
 
     def SetGraph(self, enable: bool) -> None:
         check("gpud_serve_set_graph", lib().gpud_serve_set_graph(self._h, 1 if enable else 0))

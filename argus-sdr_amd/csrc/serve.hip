@@ -75,6 +75,13 @@ constexpr size_t kWfmLrHist = kWfmMonoHist + kWfmHistPair;
 constexpr size_t kWfmFbHist = kWfmLrHist + kWfmHistPair;
 constexpr size_t kWfmTaps = kWfmFbHist + kWfmHistPair;
 constexpr size_t kWfmBytes = kWfmTaps + ARGUS_WFM_NTAPSETS * ARGUS_WFM_TAPS * sizeof(float);
+// audio-stage block of a channel (see ServeChannel::d_aud)
+constexpr size_t kAudHist = 0;  // [2 halves][2 sides][ARGUS_AUDIO_ARM] floats
+constexpr size_t kAudDeemph = kAudHist + 4 * ARGUS_AUDIO_ARM * sizeof(float);  // 2 doubles
+constexpr size_t kAudLpfHist = kAudDeemph + 2 * sizeof(double);               // [2][100] floats
+constexpr size_t kAudLpfTaps = kAudLpfHist + 2 * ARGUS_AUDIO_LPF_HIST * sizeof(float);
+constexpr size_t kAudBytes = kAudLpfTaps + ARGUS_AUDIO_LPF_TAPS * sizeof(float);
+static_assert(kAudDeemph % 8 == 0, "de-emphasis states are doubles");
 static_assert(sizeof(ArgusWfmState) <= kWfmState, "state block");
 
 // RDS mixer phase as a fixed-point turn: num / den of a turn, num < den.
@@ -112,6 +119,8 @@ static void release_channel(ServeChannel& c)
     if (c.d_hist[1]) hipFree(c.d_hist[1]);
     if (c.d_carry) hipFree(c.d_carry);
     if (c.d_wfm) hipFree(c.d_wfm);
+    if (c.d_aud) hipFree(c.d_aud);
+    c.d_aud = nullptr;
     c.d_taps = nullptr;
     c.d_ctaps = nullptr;
     c.d_hist[0] = c.d_hist[1] = nullptr;
@@ -168,6 +177,11 @@ static void slot_release(ServeSlot& sl)
     hipHostFree(sl.h_wblk);
     hipHostFree(sl.h_tab);
     hipFree(sl.d_tab);
+    hipFree(sl.d_aout);
+    hipHostFree(sl.h_aout);
+    hipFree(sl.d_alpf);
+    sl.d_aout = sl.h_aout = sl.d_alpf = nullptr;
+    sl.aout_cap = sl.h_aout_cap = sl.alpf_cap = 0;
     for (auto& ge : sl.gexec)
         if (ge) hipGraphExecDestroy(ge);
     sl.d_iq = nullptr;
@@ -244,6 +258,7 @@ GPUD_API int gpud_serve_destroy(gpud_serve* s)
         hipHostFree(sl.h_in);
         slot_release(sl);
     }
+    for (auto& kv : s->arms) hipFree(kv.second);
     hipStreamDestroy(s->compute);
     hipStreamDestroy(s->copy);
     delete s;
@@ -369,6 +384,43 @@ static void wfm_describe(const ServeChannel& c, const ServeOut& o, ServeSlot& sl
     if (o.n_mono > 0) items.push_back({o.wfm_index, 0, ARGUS_WFM_ITEM_TAIL, 0});
 }
 
+// Audio-stage descriptor and low-pass items of one channel for this block
+// (the resampler phase and the delay-line halves advance in commit_block).
+static void audio_describe(const ServeChannel& c, const ServeOut& o, ServeSlot& sl, ArgusAudioDesc& ad,
+                           std::vector<ArgusAudioItem>& items)
+{
+    const bool wfm = c.stereo || c.spec.demod == GPUD_DEMOD_WFM;
+    const int n_in = c.stereo ? o.n_mono : o.n_audio;
+    float* lh = (float*)(c.d_aud + kAudLpfHist);
+    float* hh = (float*)(c.d_aud + kAudHist);
+    ad.lpf_in = sl.d_audio + o.audio_off;
+    ad.lpf_out = c.a_lpf ? sl.d_alpf + o.alpf_off : nullptr;
+    ad.lpf_taps = (const float*)(c.d_aud + kAudLpfTaps);
+    ad.lpf_hist_rd = lh + c.a_lpf_sel * ARGUS_AUDIO_LPF_HIST;
+    ad.lpf_hist_wr = lh + (c.a_lpf_sel ^ 1) * ARGUS_AUDIO_LPF_HIST;
+    ad.in = c.a_lpf ? ad.lpf_out : sl.d_audio + o.audio_off;
+    ad.out = sl.d_aout + o.aout_off;
+    ad.arms = c.d_arms;
+    ad.hist_rd = hh + c.a_sel * 2 * ARGUS_AUDIO_ARM;
+    ad.hist_wr = hh + (c.a_sel ^ 1) * 2 * ARGUS_AUDIO_ARM;
+    ad.deemph = (double*)(c.d_aud + kAudDeemph);
+    ad.deemph_on = wfm && c.deemph_us > 0.0 ? 1 : 0;
+    ad.alpha = ad.deemph_on ? std::exp(-1.0 / ((double)c.audio_rate * c.deemph_us * 1e-6)) : 0.0;
+    ad.L = c.aL;
+    ad.M = c.aM;
+    ad.t0 = c.at0;
+    ad.n_in = n_in;
+    ad.n_out = o.n_aout;
+    ad.channels = o.a_channels;
+    ad.gain = wfm ? 0.35f : 1.0f;
+    ad.lpf = c.a_lpf ? 1 : 0;
+    ad.pad = 0;
+    if (c.a_lpf && n_in > 0) {
+        for (int st = 0; st < n_in; st += ARGUS_AUDIO_LPF_TILE) items.push_back({o.a_index, st, 0, 0});
+        items.push_back({o.a_index, 0, 1, 0});
+    }
+}
+
 static size_t align_up(size_t v) { return (v + 255) & ~(size_t)255; }
 
 // Grow a slot's table to at least `need` bytes, keeping its first `keep` bytes.
@@ -413,6 +465,11 @@ static void layout_block(const gpud_serve* s, int n, BlockLayout& L)
         o.wfm_index = -1;
         o.mono_off = o.rds_off = 0;
         o.n_mono = o.n_rds = o.rds_rate = 0;
+        o.a_index = -1;
+        o.n_aout = o.a_channels = 0;
+        o.a_tnext = c.at0;
+        o.a_rate = c.audio_rate;
+        o.aout_off = o.alpf_off = 0;
         if (c.stereo) {
             o.n_mono = argus_demod_count(ARGUS_DEMOD_WFM, o.n_iq, c.have_carry);
             o.n_audio = 2 * o.n_mono;
@@ -424,6 +481,20 @@ static void layout_block(const gpud_serve* s, int n, BlockLayout& L)
             o.rds_rate = c.spec.out_rate / c.rds_decim;
             L.mono_total += o.n_mono;
             L.rds_total += o.n_rds;
+        }
+        if (c.audio_rate > 0) {
+            // the stage's input: the stereo stage's L/R frames, or the mono
+            // demodulator output (low-passed first for plain WFM)
+            const int n_in = c.stereo ? o.n_mono : o.n_audio;
+            o.a_index = L.n_audio_chans++;
+            o.a_channels = c.stereo ? 2 : 1;
+            argus_audio_block(c.aL, c.aM, c.at0, n_in, &o.n_aout, &o.a_tnext);
+            o.aout_off = L.aout_total;
+            L.aout_total += (int64_t)o.n_aout * o.a_channels;
+            if (c.a_lpf) {
+                o.alpf_off = L.alpf_total;
+                L.alpf_total += n_in;
+            }
         }
         L.iq_total += o.n_iq;
         L.audio_total += o.n_audio;
@@ -441,6 +512,9 @@ static bool slot_fits(const ServeSlot& sl, const BlockLayout& L)
         ok = ok && fits(sl.mono_cap, L.mono_total) && fits(sl.wscratch_cap, 4 * L.mono_total) &&
              fits(sl.rds_cap, L.rds_total) && fits(sl.h_rds_cap, L.rds_total) && fits(sl.wblk_cap, L.n_wfm) &&
              fits(sl.h_wblk_cap, L.n_wfm);
+    if (L.n_audio_chans > 0)
+        ok = ok && fits(sl.aout_cap, L.aout_total) && fits(sl.h_aout_cap, L.aout_total) &&
+             fits(sl.alpf_cap, L.alpf_total);
     return ok;
 }
 
@@ -460,6 +534,11 @@ static int slot_grow(ServeSlot& sl, const BlockLayout& L)
         ARGUS_CHECK((hipError_t)grow_dev(&sl.d_wblk, &sl.wblk_cap, (size_t)L.n_wfm));
         ARGUS_CHECK((hipError_t)grow_host(&sl.h_wblk, &sl.h_wblk_cap, (size_t)L.n_wfm));
     }
+    if (L.n_audio_chans > 0) {
+        ARGUS_CHECK((hipError_t)grow_dev(&sl.d_aout, &sl.aout_cap, (size_t)std::max<int64_t>(L.aout_total, 1)));
+        ARGUS_CHECK((hipError_t)grow_host(&sl.h_aout, &sl.h_aout_cap, (size_t)std::max<int64_t>(L.aout_total, 1)));
+        ARGUS_CHECK((hipError_t)grow_dev(&sl.d_alpf, &sl.alpf_cap, (size_t)std::max<int64_t>(L.alpf_total, 1)));
+    }
     return 0;
 }
 
@@ -471,6 +550,8 @@ static void slot_adopt(ServeSlot& sl, BlockLayout& L)
     sl.rds_total = L.rds_total;
     sl.iq_total = L.iq_total;
     sl.audio_total = L.audio_total;
+    sl.n_audio_chans = L.n_audio_chans;
+    sl.aout_total = L.aout_total;
 }
 
 // The block's tables for the live channels' current state, written into the
@@ -498,6 +579,11 @@ static int describe_block(gpud_serve* s, ServeSlot& sl, const std::vector<ServeO
     s->wdescs.resize((size_t)n_wfm);
     s->ditems.clear();
     s->witems.clear();
+    int n_audio = 0;
+    for (const ServeOut& o : outs) n_audio += o.a_index >= 0 ? 1 : 0;
+    s->adescs.resize((size_t)n_audio);
+    s->aitems.clear();
+    int a_mode = n_audio > 0 ? 1 : 0;
     size_t lds = 0;
     int ci = 0;
     for (const auto& kv : s->live) {
@@ -539,6 +625,10 @@ static int describe_block(gpud_serve* s, ServeSlot& sl, const std::vector<ServeO
             }
         }
         if (c.stereo) wfm_describe(c, o, sl, s->wdescs[(size_t)o.wfm_index], s->witems);
+        if (o.a_index >= 0) {
+            audio_describe(c, o, sl, s->adescs[(size_t)o.a_index], s->aitems);
+            if (c.a_lpf) a_mode = 2;
+        }
         ci++;
     }
     if (!reuse) {
@@ -553,7 +643,10 @@ static int describe_block(gpud_serve* s, ServeSlot& sl, const std::vector<ServeO
     const size_t off_wdesc = align_up(off_ddesc + (size_t)nch * sizeof(ArgusDemodDesc));
     const size_t off_ditems = align_up(off_wdesc + (size_t)n_wfm * sizeof(ArgusWfmDesc));
     const size_t off_witems = align_up(off_ditems + (size_t)n_ditems * sizeof(ArgusDemodItem));
-    const size_t total = off_witems + (size_t)n_witems * sizeof(ArgusWfmItem);
+    const int n_aitems = (int)s->aitems.size();
+    const size_t off_adesc = align_up(off_witems + (size_t)n_witems * sizeof(ArgusWfmItem));
+    const size_t off_aitems = align_up(off_adesc + (size_t)n_audio * sizeof(ArgusAudioDesc));
+    const size_t total = off_aitems + (size_t)n_aitems * sizeof(ArgusAudioItem);
     const char* d_before = sl.d_tab;
     ARGUS_CHECK((hipError_t)grow_table(sl, total, reuse ? items_bytes : 0));
     if (sl.d_tab != d_before) sl.items_dirty = true;  // a new device table holds no items yet
@@ -571,11 +664,16 @@ static int describe_block(gpud_serve* s, ServeSlot& sl, const std::vector<ServeO
     if (n_wfm) std::memcpy(sl.h_tab + off_wdesc, s->wdescs.data(), (size_t)n_wfm * sizeof(ArgusWfmDesc));
     if (n_ditems) std::memcpy(sl.h_tab + off_ditems, s->ditems.data(), (size_t)n_ditems * sizeof(ArgusDemodItem));
     if (n_witems) std::memcpy(sl.h_tab + off_witems, s->witems.data(), (size_t)n_witems * sizeof(ArgusWfmItem));
+    if (n_audio) std::memcpy(sl.h_tab + off_adesc, s->adescs.data(), (size_t)n_audio * sizeof(ArgusAudioDesc));
+    if (n_aitems) std::memcpy(sl.h_tab + off_aitems, s->aitems.data(), (size_t)n_aitems * sizeof(ArgusAudioItem));
     ArgusServeHdr& hdr = *(ArgusServeHdr*)(sl.h_tab + off_hdr);
     const char* d = sl.d_tab;
     hdr.x = ArgusExtractHdr{(const ArgusChanDesc*)(d + off_chans), (const ArgusWorkItem*)d, in, n_items, n};
     hdr.d = ArgusDemodHdr{(const ArgusDemodDesc*)(d + off_ddesc), (const ArgusDemodItem*)(d + off_ditems), n_ditems, 0};
     hdr.w = ArgusWfmHdr{(const ArgusWfmDesc*)(d + off_wdesc), (const ArgusWfmItem*)(d + off_witems), n_wfm, n_witems};
+    hdr.a = ArgusAudioHdr{(const ArgusAudioDesc*)(d + off_adesc), (const ArgusAudioItem*)(d + off_aitems), n_audio,
+                          n_aitems};
+    sl.a_mode = a_mode;
     sl.tab_bytes = total;
     sl.off_hdr = off_hdr;
     sl.lds = lds;
@@ -609,6 +707,13 @@ static void commit_block(gpud_serve* s, const std::vector<ServeOut>& outs, int n
         if (c.stereo) {
             if (o.n_mono > 0) c.wfm_sel ^= 1;
             c.rds_count += o.n_mono;
+        }
+        if (o.a_index >= 0) {
+            c.at0 = o.a_tnext;
+            if ((c.stereo ? o.n_mono : o.n_audio) > 0) {
+                c.a_sel ^= 1;
+                if (c.a_lpf) c.a_lpf_sel ^= 1;
+            }
         }
     }
 }
@@ -646,7 +751,11 @@ static int launch_stage(gpud_serve* s, ServeSlot& sl, int stage, bool wfm)
     if (stage == 0) return argus_launch_extract_hdr(&dh->x, sl.lds, s->tune.grid, s->compute);
     int err = argus_launch_demod_hdr(&dh->d, s->compute);
     if (err) return err;
-    return wfm ? argus_launch_wfm_hdr(&dh->w, s->compute) : 0;
+    if (wfm) {
+        err = argus_launch_wfm_hdr(&dh->w, s->compute);
+        if (err) return err;
+    }
+    return sl.a_mode ? argus_launch_audio_hdr(&dh->a, sl.a_mode == 2, s->compute) : 0;
 }
 
 // Run the block's two stages, recording ev_xend between them (the
@@ -665,7 +774,7 @@ static int launch_block(gpud_serve* s, ServeSlot& sl)
     }
     const char* hdr = sl.d_tab + sl.off_hdr;
     if (!sl.gexec[0] || sl.g_hdr != hdr || sl.g_lds != sl.lds || sl.g_grid != s->tune.grid ||
-        sl.g_wfm != (int)wfm) {
+        sl.g_wfm != (int)wfm || sl.g_audio != sl.a_mode) {
         for (auto& ge : sl.gexec)
             if (ge) {
                 hipGraphExecDestroy(ge);
@@ -693,6 +802,7 @@ static int launch_block(gpud_serve* s, ServeSlot& sl)
         sl.g_lds = sl.lds;
         sl.g_grid = s->tune.grid;
         sl.g_wfm = (int)wfm;
+        sl.g_audio = sl.a_mode;
     }
     ARGUS_CHECK(hipGraphLaunch(sl.gexec[0], s->compute));
     ARGUS_CHECK(hipEventRecord(sl.ev_xend, s->compute));
@@ -784,6 +894,9 @@ static int serve_submit(gpud_serve* s, const gpud_float2* iq, int n, bool on_dev
         if (sl.n_wfm > 0)
             ARGUS_CHECK(hipMemcpyAsync(sl.h_wblk, sl.d_wblk, (size_t)sl.n_wfm * sizeof(ArgusWfmBlock),
                                        hipMemcpyDeviceToHost, s->copy));
+        if (sl.aout_total > 0)
+            ARGUS_CHECK(hipMemcpyAsync(sl.h_aout, sl.d_aout, (size_t)sl.aout_total * sizeof(float),
+                                       hipMemcpyDeviceToHost, s->copy));
     }
     ARGUS_CHECK(hipEventRecord(sl.ev_done, s->copy));
     sl.busy = true;
@@ -863,6 +976,103 @@ GPUD_API int gpud_serve_channel_output(gpud_serve* s, int64_t ticket, int64_t ch
             return 0;
         }
     return -1;
+}
+
+GPUD_API int gpud_serve_channel_audio(gpud_serve* s, int64_t ticket, int64_t channel_id, const float** audio,
+                                      int* n_frames, int* channels, int* rate)
+{
+    if (!s) return -1;
+    ServeSlot* sl = find_slot(s, ticket);
+    if (!sl || sl->busy) return -2;
+    for (const ServeOut& o : sl->outs)
+        if (o.channel_id == channel_id) {
+            if (o.a_index < 0) return -3;
+            if (audio) *audio = sl->h_aout + o.aout_off;
+            if (n_frames) *n_frames = o.n_aout;
+            if (channels) *channels = o.a_channels;
+            if (rate) *rate = o.a_rate;
+            return 0;
+        }
+    return -1;
+}
+
+// Device bytes of a channel's audio stage: its state block plus its share of
+// the per-slot audio outputs and low-pass scratch.
+static size_t audio_footprint(const gpud_serve* s, const ServeChannel& c, int L, int M)
+{
+    const int64_t n_in = (int64_t)s->block_samples / c.decim + 2;
+    const int64_t n_out = n_in * L / M + 4;
+    const int ch = c.stereo ? 2 : 1;
+    size_t per_block = (size_t)(n_out * ch) * sizeof(float);
+    if (c.spec.demod == GPUD_DEMOD_WFM) per_block += (size_t)n_in * sizeof(float);
+    return kAudBytes + per_block * ARGUS_SERVE_SLOTS;
+}
+
+GPUD_API int gpud_serve_set_audio(gpud_serve* s, int64_t channel_id, int audio_rate, double deemphasis_us)
+{
+    if (!s || audio_rate < 0 || !(deemphasis_us >= 0.0)) return -1;
+    auto it = s->live.find(channel_id);
+    if (it == s->live.end()) return -1;
+    ServeChannel& c = it->second;
+    for (auto& sl : s->slots) {  // blocks in flight read the stage's current state
+        int err = slot_wait(s, sl);
+        if (err) return err;
+    }
+    const size_t old_fp = c.audio_rate > 0 ? audio_footprint(s, c, c.aL, c.aM) : 0;
+    if (audio_rate == 0) {
+        if (c.d_aud) hipFree(c.d_aud);
+        c.d_aud = nullptr;
+        c.audio_rate = 0;
+        c.footprint -= old_fp;
+        s->bytes_in_use -= old_fp;
+        s->epoch++;
+        return 0;
+    }
+    int L = 1, M = 1;
+    if (argus_audio_arms(c.spec.out_rate, audio_rate, &L, &M, nullptr) != 0) return -1;
+    const size_t fp = audio_footprint(s, c, L, M);
+    if (s->bytes_in_use - old_fp + fp > s->budget) return GPUD_ERR_BUDGET;
+    const double* arms = nullptr;
+    if (L != M) {
+        auto key = std::make_pair(c.spec.out_rate, audio_rate);
+        auto ai = s->arms.find(key);
+        if (ai == s->arms.end()) {
+            std::vector<double> h((size_t)L * ARGUS_AUDIO_ARM);
+            if (argus_audio_arms(c.spec.out_rate, audio_rate, nullptr, nullptr, h.data()) != 0) return -1;
+            double* d = nullptr;
+            ARGUS_CHECK(dev_alloc(&d, h.size()));
+            hipError_t e = hipMemcpy(d, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice);
+            if (e != hipSuccess) {
+                hipFree(d);
+                return (int)e;
+            }
+            ai = s->arms.emplace(key, d).first;
+        }
+        arms = ai->second;
+    }
+    // fresh state: zero delay lines and de-emphasis, the low-pass taps
+    std::vector<char> blob(kAudBytes, 0);
+    const bool lpf = c.spec.demod == GPUD_DEMOD_WFM;
+    if (lpf) {
+        std::vector<double> b((size_t)ARGUS_AUDIO_LPF_TAPS + 1);
+        const int nb = argus_lowpass_fir(15000.0, c.spec.out_rate, ARGUS_AUDIO_LPF_TAPS, b.data(), false);
+        float* taps = (float*)(blob.data() + kAudLpfTaps);
+        for (int k = 0; k < ARGUS_AUDIO_LPF_TAPS && k < nb; k++) taps[k] = (float)b[(size_t)k];
+    }
+    if (!c.d_aud) ARGUS_CHECK(dev_alloc(&c.d_aud, kAudBytes));
+    ARGUS_CHECK(hipMemcpy(c.d_aud, blob.data(), kAudBytes, hipMemcpyHostToDevice));
+    c.audio_rate = audio_rate;
+    c.deemph_us = deemphasis_us;
+    c.aL = L;
+    c.aM = M;
+    c.at0 = 0;
+    c.a_sel = c.a_lpf_sel = 0;
+    c.a_lpf = lpf;
+    c.d_arms = arms;
+    c.footprint += fp - old_fp;
+    s->bytes_in_use += fp - old_fp;
+    s->epoch++;
+    return 0;
 }
 
 GPUD_API int gpud_serve_set_graph(gpud_serve* s, int enable)

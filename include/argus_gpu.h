@@ -310,6 +310,25 @@ GPUD_API int gpud_serve_autotune(gpud_serve* s, const gpud_float2* iq_dev, int n
 GPUD_API int gpud_serve_channel_wfm(gpud_serve* s, int64_t ticket, int64_t channel_id, const gpud_float2** rds,
                                     int* n_rds, int* rds_rate, int* locked, int* stereo);
 
+/* Post-demodulation audio stage of one live channel (streamer.go:1382-1470
+ * with internal/dsp/resample.go:28-170): plain WFM is first low-passed at
+ * 15 kHz (wfmAudioFilter, 101 taps, streamer.go:1511-1520); every mode is
+ * resampled from its out_rate to audio_rate by the rational polyphase
+ * resampler (NewResampler(in, out, 32); two of them for the L/R frames of a
+ * GPUD_DEMOD_WFM_STEREO channel); WFM and WFM stereo are then de-emphasised
+ * (time constant deemphasis_us, 0 = off; FP64 state) and scaled by 0.35.
+ * audio_rate 0 turns the stage off.  Resets the stage's state (a new
+ * resampler, filter and de-emphasis state) and waits for blocks in flight;
+ * applies from the next submitted block.  0, -1 bad arguments or unknown
+ * channel, GPUD_ERR_BUDGET if the stage's device memory does not fit. */
+GPUD_API int gpud_serve_set_audio(gpud_serve* s, int64_t channel_id, int audio_rate, double deemphasis_us);
+/* The audio stage's output of a channel in a completed block: n_frames
+ * frames of `channels` (1, or 2 interleaved L/R) float samples at `rate`
+ * (pinned host memory, valid as gpud_serve_channel_output's).  -1 if the
+ * channel was not part of the block, -3 if its audio stage was off. */
+GPUD_API int gpud_serve_channel_audio(gpud_serve* s, int64_t ticket, int64_t channel_id, const float** audio,
+                                      int* n_frames, int* channels, int* rate);
+
 /* ------------------------------------------------------------------ */
 /* RDS group decoder (host): successor of rds.Decoder.Decode             */
 /* (internal/rds/rds.go:78-264, tryDecode 365-430, decodeGroupAt 432-473, */

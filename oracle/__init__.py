@@ -672,3 +672,90 @@ class RDSBasebandStream:
         out = np.zeros(len(d) // max(self.s.decim, 1) + 2, np.complex64)
         m = lib().orc_rds_stream_process(C.byref(self.s), _p(d), len(d), _p(out))
         return out[:m]
+
+
+# ---- post-demodulation audio (internal/dsp/resample.go, streamer.go:1408-1470) ----
+class orc_resampler(C.Structure):
+    _fields_ = [("l", I), ("m", I), ("taps", I), ("out_time", I), ("bank", P), ("delay", P)]
+
+
+register("orc_gcd", I, [I, I])
+register("orc_windowed_sinc", None, [I, D, D, P])
+register("orc_resampler_init", I, [C.POINTER(orc_resampler), I, I, I])
+register("orc_resampler_free", None, [C.POINTER(orc_resampler)])
+register("orc_resampler_process", I, [C.POINTER(orc_resampler), P, I, P])
+register("orc_deemphasis", D, [P, I, I, D, D])
+
+
+def gcd(a, b):
+    return int(lib().orc_gcd(a, b))
+
+
+def WindowedSinc(length, fc, gain):
+    out = np.zeros(length, np.float64)
+    lib().orc_windowed_sinc(length, fc, gain, _p(out))
+    return out
+
+
+class Resampler:
+    """dsp.Resampler (resample.go:28-170): stateful rational polyphase L/M."""
+
+    def __init__(self, in_rate, out_rate, taps_per_phase=32):
+        self._r = orc_resampler()
+        lib().orc_resampler_init(C.byref(self._r), in_rate, out_rate, taps_per_phase)
+
+    def __del__(self):
+        try:
+            lib().orc_resampler_free(C.byref(self._r))
+        except Exception:
+            pass
+
+    def Ratio(self):
+        return self._r.l, self._r.m
+
+    @property
+    def taps(self):
+        return self._r.taps
+
+    def Bank(self):
+        """polyBank[phase][tap] as an (l, taps) array."""
+        n = self._r.l * self._r.taps
+        return np.ctypeslib.as_array(C.cast(self._r.bank, C.POINTER(C.c_double)), (n,)).reshape(self._r.l, -1).copy()
+
+    def Process(self, x):
+        x = np.ascontiguousarray(x, np.float32)
+        out = np.zeros(len(x) * self._r.l // self._r.m + 8, np.float32)
+        k = lib().orc_resampler_process(C.byref(self._r), _p(x), len(x), _p(out))
+        return out[:k].copy()
+
+
+class StereoResampler:
+    """dsp.StereoResampler (resample.go:174-233): two resamplers, interleaved
+    in and out, the shorter length kept."""
+
+    def __init__(self, in_rate, out_rate, taps_per_phase=32):
+        self.left = Resampler(in_rate, out_rate, taps_per_phase)
+        self.right = Resampler(in_rate, out_rate, taps_per_phase)
+
+    def Process(self, x):
+        x = np.ascontiguousarray(x, np.float32)
+        n = len(x) // 2
+        if n == 0:
+            return np.zeros(0, np.float32)
+        a = self.left.Process(x[0:2 * n:2])
+        b = self.right.Process(x[1:2 * n:2])
+        k = min(len(a), len(b))
+        out = np.empty(2 * k, np.float32)
+        out[0::2] = a[:k]
+        out[1::2] = b[:k]
+        return out
+
+
+def Deemphasis(x, alpha, y, stride=1, offset=0):
+    """streamer.go:1443-1467 one channel (in place on a float32 copy); returns
+    (samples, new state)."""
+    x = np.ascontiguousarray(x, np.float32).copy()
+    if len(x) == 0:
+        return x, y
+    y = lib().orc_deemphasis(x[offset:].ctypes.data, (len(x) - offset + stride - 1) // stride, stride, alpha, y)
+    return x, y

@@ -106,6 +106,19 @@ int  orc_ssb_demod(const float* iq, int n, int sample_rate, int usb, double* bfo
 int  orc_cw_demod(const float* iq, int n, int sample_rate, double* bfo_phase_io, float* out);
 int  orc_rds_baseband(const float* iq, int n, int sample_rate, float* out, int out_cap, int* out_rate);
 
+/* ---- post-demodulation audio (internal/dsp/resample.go, streamer.go:1408-1470) ---- */
+typedef struct orc_resampler {
+    int l, m, taps, out_time;
+    double* bank;   /* l x taps */
+    double* delay;  /* taps, newest first */
+} orc_resampler;
+int    orc_gcd(int a, int b);
+void   orc_windowed_sinc(int length, double fc, double gain, double* out);
+int    orc_resampler_init(orc_resampler* r, int in_rate, int out_rate, int taps_per_phase);
+void   orc_resampler_free(orc_resampler* r);
+int    orc_resampler_process(orc_resampler* r, const float* in, int n, float* out);
+double orc_deemphasis(float* x, int n, int stride, double alpha, double y);
+
 #ifdef __cplusplus
 }
 #endif

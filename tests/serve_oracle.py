@@ -85,3 +85,37 @@ def synth_scene(n, sr, carriers, seed=0, noise=0.05):
         else:
             x += lvl * np.exp(2j * np.pi * f0 * t)
     return x.astype(np.complex64)
+
+
+class OracleAudioStage:
+    """The post-demodulation audio stage of one channel
+    (streamer.go:1382-1470): plain WFM low-passed at 15 kHz
+    (wfmAudioFilter, streamer.go:1511-1520), resampled from the demod rate to
+    audio_rate (NewResampler / NewStereoResampler with 32 taps per phase,
+    skipped when the rates are equal), then for WFM and WFM stereo the
+    de-emphasis y = a y + (1 - a) x with a = exp(-1 / (audio_rate tau)) and
+    the gain 0.35.  process() takes the demodulator's block output (L/R
+    interleaved for stereo) and returns (frames, channels)."""
+
+    def __init__(self, demod, in_rate, audio_rate=48000, deemph_us=50.0):
+        self.wfm = demod in (WFM, WFM_STEREO)
+        self.ch = 2 if demod == WFM_STEREO else 1
+        self.lpf = O.StatefulFIRReal(O.LowpassFIR(15000, in_rate, 101)) if demod == WFM else None
+        self.same = in_rate == audio_rate
+        self.rs = O.StereoResampler(in_rate, audio_rate, 32) if self.ch == 2 else O.Resampler(in_rate, audio_rate, 32)
+        self.deemph = self.wfm and deemph_us > 0
+        self.alpha = np.exp(-1.0 / (audio_rate * deemph_us * 1e-6)) if self.deemph else 0.0
+        self.y = [0.0, 0.0]
+
+    def process(self, audio):
+        a = np.ascontiguousarray(audio, np.float32)
+        if self.lpf is not None and len(a):
+            a = self.lpf.Process(a)
+        if not self.same:
+            a = self.rs.Process(a)
+        if self.deemph:
+            for s in range(self.ch):
+                a, self.y[s] = O.Deemphasis(a, self.alpha, self.y[s], stride=self.ch, offset=s)
+        if self.wfm:
+            a = a * np.float32(0.35)
+        return a.reshape(-1, self.ch)
