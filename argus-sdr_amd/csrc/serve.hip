@@ -76,11 +76,9 @@ constexpr size_t kWfmFbHist = kWfmLrHist + kWfmHistPair;
 constexpr size_t kWfmTaps = kWfmFbHist + kWfmHistPair;
 constexpr size_t kWfmBytes = kWfmTaps + ARGUS_WFM_NTAPSETS * ARGUS_WFM_TAPS * sizeof(float);
 // audio-stage block of a channel (see ServeChannel::d_aud)
-constexpr size_t kAudHist = 0;  // [2 halves][2 sides][ARGUS_AUDIO_ARM] floats
-constexpr size_t kAudDeemph = kAudHist + 4 * ARGUS_AUDIO_ARM * sizeof(float);  // 2 doubles
-constexpr size_t kAudLpfHist = kAudDeemph + 2 * sizeof(double);               // [2][100] floats
-constexpr size_t kAudLpfTaps = kAudLpfHist + 2 * ARGUS_AUDIO_LPF_HIST * sizeof(float);
-constexpr size_t kAudBytes = kAudLpfTaps + ARGUS_AUDIO_LPF_TAPS * sizeof(float);
+constexpr size_t kAudHist = 0;  // [2 halves][2 sides][ARGUS_AUDIO_TMAX] floats
+constexpr size_t kAudDeemph = kAudHist + 4 * ARGUS_AUDIO_TMAX * sizeof(float);  // 2 doubles
+constexpr size_t kAudBytes = kAudDeemph + 2 * sizeof(double);
 static_assert(kAudDeemph % 8 == 0, "de-emphasis states are doubles");
 static_assert(sizeof(ArgusWfmState) <= kWfmState, "state block");
 
@@ -179,9 +177,8 @@ static void slot_release(ServeSlot& sl)
     hipFree(sl.d_tab);
     hipFree(sl.d_aout);
     hipHostFree(sl.h_aout);
-    hipFree(sl.d_alpf);
-    sl.d_aout = sl.h_aout = sl.d_alpf = nullptr;
-    sl.aout_cap = sl.h_aout_cap = sl.alpf_cap = 0;
+    sl.d_aout = sl.h_aout = nullptr;
+    sl.aout_cap = sl.h_aout_cap = 0;
     for (auto& ge : sl.gexec)
         if (ge) hipGraphExecDestroy(ge);
     sl.d_iq = nullptr;
@@ -258,7 +255,7 @@ GPUD_API int gpud_serve_destroy(gpud_serve* s)
         hipHostFree(sl.h_in);
         slot_release(sl);
     }
-    for (auto& kv : s->arms) hipFree(kv.second);
+    for (auto& kv : s->filters) hipFree(kv.second);
     hipStreamDestroy(s->compute);
     hipStreamDestroy(s->copy);
     delete s;
@@ -384,41 +381,29 @@ static void wfm_describe(const ServeChannel& c, const ServeOut& o, ServeSlot& sl
     if (o.n_mono > 0) items.push_back({o.wfm_index, 0, ARGUS_WFM_ITEM_TAIL, 0});
 }
 
-// Audio-stage descriptor and low-pass items of one channel for this block
-// (the resampler phase and the delay-line halves advance in commit_block).
-static void audio_describe(const ServeChannel& c, const ServeOut& o, ServeSlot& sl, ArgusAudioDesc& ad,
-                           std::vector<ArgusAudioItem>& items)
+// Audio-stage descriptor of one channel for this block (the resampler phase
+// and the delay-line half advance in commit_block).
+static void audio_describe(const ServeChannel& c, const ServeOut& o, ServeSlot& sl, ArgusAudioDesc& ad)
 {
     const bool wfm = c.stereo || c.spec.demod == GPUD_DEMOD_WFM;
-    const int n_in = c.stereo ? o.n_mono : o.n_audio;
-    float* lh = (float*)(c.d_aud + kAudLpfHist);
     float* hh = (float*)(c.d_aud + kAudHist);
-    ad.lpf_in = sl.d_audio + o.audio_off;
-    ad.lpf_out = c.a_lpf ? sl.d_alpf + o.alpf_off : nullptr;
-    ad.lpf_taps = (const float*)(c.d_aud + kAudLpfTaps);
-    ad.lpf_hist_rd = lh + c.a_lpf_sel * ARGUS_AUDIO_LPF_HIST;
-    ad.lpf_hist_wr = lh + (c.a_lpf_sel ^ 1) * ARGUS_AUDIO_LPF_HIST;
-    ad.in = c.a_lpf ? ad.lpf_out : sl.d_audio + o.audio_off;
+    ad.in = sl.d_audio + o.audio_off;
     ad.out = sl.d_aout + o.aout_off;
     ad.arms = c.d_arms;
-    ad.hist_rd = hh + c.a_sel * 2 * ARGUS_AUDIO_ARM;
-    ad.hist_wr = hh + (c.a_sel ^ 1) * 2 * ARGUS_AUDIO_ARM;
+    ad.hist_rd = hh + c.a_sel * 2 * ARGUS_AUDIO_TMAX;
+    ad.hist_wr = hh + (c.a_sel ^ 1) * 2 * ARGUS_AUDIO_TMAX;
     ad.deemph = (double*)(c.d_aud + kAudDeemph);
     ad.deemph_on = wfm && c.deemph_us > 0.0 ? 1 : 0;
     ad.alpha = ad.deemph_on ? std::exp(-1.0 / ((double)c.audio_rate * c.deemph_us * 1e-6)) : 0.0;
     ad.L = c.aL;
     ad.M = c.aM;
     ad.t0 = c.at0;
-    ad.n_in = n_in;
+    ad.T = c.aT;
+    ad.ident = c.a_ident ? 1 : 0;
+    ad.n_in = c.stereo ? o.n_mono : o.n_audio;
     ad.n_out = o.n_aout;
     ad.channels = o.a_channels;
     ad.gain = wfm ? 0.35f : 1.0f;
-    ad.lpf = c.a_lpf ? 1 : 0;
-    ad.pad = 0;
-    if (c.a_lpf && n_in > 0) {
-        for (int st = 0; st < n_in; st += ARGUS_AUDIO_LPF_TILE) items.push_back({o.a_index, st, 0, 0});
-        items.push_back({o.a_index, 0, 1, 0});
-    }
 }
 
 static size_t align_up(size_t v) { return (v + 255) & ~(size_t)255; }
@@ -451,8 +436,8 @@ static void layout_block(const gpud_serve* s, int n, BlockLayout& L)
 {
     L.outs.clear();
     L.outs.reserve(s->live.size());
-    L.iq_total = L.audio_total = L.mono_total = L.rds_total = 0;
-    L.n_wfm = 0;
+    L.iq_total = L.audio_total = L.mono_total = L.rds_total = L.aout_total = 0;
+    L.n_wfm = L.n_audio_chans = 0;
     for (const auto& kv : s->live) {
         const ServeChannel& c = kv.second;
         ServeOut o;
@@ -469,7 +454,7 @@ static void layout_block(const gpud_serve* s, int n, BlockLayout& L)
         o.n_aout = o.a_channels = 0;
         o.a_tnext = c.at0;
         o.a_rate = c.audio_rate;
-        o.aout_off = o.alpf_off = 0;
+        o.aout_off = 0;
         if (c.stereo) {
             o.n_mono = argus_demod_count(ARGUS_DEMOD_WFM, o.n_iq, c.have_carry);
             o.n_audio = 2 * o.n_mono;
@@ -484,17 +469,13 @@ static void layout_block(const gpud_serve* s, int n, BlockLayout& L)
         }
         if (c.audio_rate > 0) {
             // the stage's input: the stereo stage's L/R frames, or the mono
-            // demodulator output (low-passed first for plain WFM)
+            // demodulator output
             const int n_in = c.stereo ? o.n_mono : o.n_audio;
             o.a_index = L.n_audio_chans++;
             o.a_channels = c.stereo ? 2 : 1;
             argus_audio_block(c.aL, c.aM, c.at0, n_in, &o.n_aout, &o.a_tnext);
             o.aout_off = L.aout_total;
             L.aout_total += (int64_t)o.n_aout * o.a_channels;
-            if (c.a_lpf) {
-                o.alpf_off = L.alpf_total;
-                L.alpf_total += n_in;
-            }
         }
         L.iq_total += o.n_iq;
         L.audio_total += o.n_audio;
@@ -513,8 +494,7 @@ static bool slot_fits(const ServeSlot& sl, const BlockLayout& L)
              fits(sl.rds_cap, L.rds_total) && fits(sl.h_rds_cap, L.rds_total) && fits(sl.wblk_cap, L.n_wfm) &&
              fits(sl.h_wblk_cap, L.n_wfm);
     if (L.n_audio_chans > 0)
-        ok = ok && fits(sl.aout_cap, L.aout_total) && fits(sl.h_aout_cap, L.aout_total) &&
-             fits(sl.alpf_cap, L.alpf_total);
+        ok = ok && fits(sl.aout_cap, L.aout_total) && fits(sl.h_aout_cap, L.aout_total);
     return ok;
 }
 
@@ -537,7 +517,6 @@ static int slot_grow(ServeSlot& sl, const BlockLayout& L)
     if (L.n_audio_chans > 0) {
         ARGUS_CHECK((hipError_t)grow_dev(&sl.d_aout, &sl.aout_cap, (size_t)std::max<int64_t>(L.aout_total, 1)));
         ARGUS_CHECK((hipError_t)grow_host(&sl.h_aout, &sl.h_aout_cap, (size_t)std::max<int64_t>(L.aout_total, 1)));
-        ARGUS_CHECK((hipError_t)grow_dev(&sl.d_alpf, &sl.alpf_cap, (size_t)std::max<int64_t>(L.alpf_total, 1)));
     }
     return 0;
 }
@@ -582,8 +561,6 @@ static int describe_block(gpud_serve* s, ServeSlot& sl, const std::vector<ServeO
     int n_audio = 0;
     for (const ServeOut& o : outs) n_audio += o.a_index >= 0 ? 1 : 0;
     s->adescs.resize((size_t)n_audio);
-    s->aitems.clear();
-    int a_mode = n_audio > 0 ? 1 : 0;
     size_t lds = 0;
     int ci = 0;
     for (const auto& kv : s->live) {
@@ -624,10 +601,13 @@ static int describe_block(gpud_serve* s, ServeSlot& sl, const std::vector<ServeO
                     s->ditems.push_back({ci, st, std::min(ARGUS_DEMOD_TILE, o.n_iq - st), 0});
             }
         }
-        if (c.stereo) wfm_describe(c, o, sl, s->wdescs[(size_t)o.wfm_index], s->witems);
+        if (c.stereo) {
+            if (o.wfm_index < 0 || o.wfm_index >= n_wfm) return -1;
+            wfm_describe(c, o, sl, s->wdescs[(size_t)o.wfm_index], s->witems);
+        }
         if (o.a_index >= 0) {
-            audio_describe(c, o, sl, s->adescs[(size_t)o.a_index], s->aitems);
-            if (c.a_lpf) a_mode = 2;
+            if (o.a_index >= n_audio) return -1;  // a layout that is not this channel set's
+            audio_describe(c, o, sl, s->adescs[(size_t)o.a_index]);
         }
         ci++;
     }
@@ -643,10 +623,8 @@ static int describe_block(gpud_serve* s, ServeSlot& sl, const std::vector<ServeO
     const size_t off_wdesc = align_up(off_ddesc + (size_t)nch * sizeof(ArgusDemodDesc));
     const size_t off_ditems = align_up(off_wdesc + (size_t)n_wfm * sizeof(ArgusWfmDesc));
     const size_t off_witems = align_up(off_ditems + (size_t)n_ditems * sizeof(ArgusDemodItem));
-    const int n_aitems = (int)s->aitems.size();
     const size_t off_adesc = align_up(off_witems + (size_t)n_witems * sizeof(ArgusWfmItem));
-    const size_t off_aitems = align_up(off_adesc + (size_t)n_audio * sizeof(ArgusAudioDesc));
-    const size_t total = off_aitems + (size_t)n_aitems * sizeof(ArgusAudioItem);
+    const size_t total = off_adesc + (size_t)n_audio * sizeof(ArgusAudioDesc);
     const char* d_before = sl.d_tab;
     ARGUS_CHECK((hipError_t)grow_table(sl, total, reuse ? items_bytes : 0));
     if (sl.d_tab != d_before) sl.items_dirty = true;  // a new device table holds no items yet
@@ -665,15 +643,13 @@ static int describe_block(gpud_serve* s, ServeSlot& sl, const std::vector<ServeO
     if (n_ditems) std::memcpy(sl.h_tab + off_ditems, s->ditems.data(), (size_t)n_ditems * sizeof(ArgusDemodItem));
     if (n_witems) std::memcpy(sl.h_tab + off_witems, s->witems.data(), (size_t)n_witems * sizeof(ArgusWfmItem));
     if (n_audio) std::memcpy(sl.h_tab + off_adesc, s->adescs.data(), (size_t)n_audio * sizeof(ArgusAudioDesc));
-    if (n_aitems) std::memcpy(sl.h_tab + off_aitems, s->aitems.data(), (size_t)n_aitems * sizeof(ArgusAudioItem));
     ArgusServeHdr& hdr = *(ArgusServeHdr*)(sl.h_tab + off_hdr);
     const char* d = sl.d_tab;
     hdr.x = ArgusExtractHdr{(const ArgusChanDesc*)(d + off_chans), (const ArgusWorkItem*)d, in, n_items, n};
     hdr.d = ArgusDemodHdr{(const ArgusDemodDesc*)(d + off_ddesc), (const ArgusDemodItem*)(d + off_ditems), n_ditems, 0};
     hdr.w = ArgusWfmHdr{(const ArgusWfmDesc*)(d + off_wdesc), (const ArgusWfmItem*)(d + off_witems), n_wfm, n_witems};
-    hdr.a = ArgusAudioHdr{(const ArgusAudioDesc*)(d + off_adesc), (const ArgusAudioItem*)(d + off_aitems), n_audio,
-                          n_aitems};
-    sl.a_mode = a_mode;
+    hdr.a = ArgusAudioHdr{(const ArgusAudioDesc*)(d + off_adesc), n_audio, 0};
+    sl.a_mode = n_audio > 0 ? 1 : 0;
     sl.tab_bytes = total;
     sl.off_hdr = off_hdr;
     sl.lds = lds;
@@ -710,10 +686,7 @@ static void commit_block(gpud_serve* s, const std::vector<ServeOut>& outs, int n
         }
         if (o.a_index >= 0) {
             c.at0 = o.a_tnext;
-            if ((c.stereo ? o.n_mono : o.n_audio) > 0) {
-                c.a_sel ^= 1;
-                if (c.a_lpf) c.a_lpf_sel ^= 1;
-            }
+            if ((c.stereo ? o.n_mono : o.n_audio) > 0) c.a_sel ^= 1;
         }
     }
 }
@@ -755,7 +728,7 @@ static int launch_stage(gpud_serve* s, ServeSlot& sl, int stage, bool wfm)
         err = argus_launch_wfm_hdr(&dh->w, s->compute);
         if (err) return err;
     }
-    return sl.a_mode ? argus_launch_audio_hdr(&dh->a, sl.a_mode == 2, s->compute) : 0;
+    return sl.a_mode ? argus_launch_audio_hdr(&dh->a, s->compute) : 0;
 }
 
 // Run the block's two stages, recording ev_xend between them (the
@@ -997,15 +970,13 @@ GPUD_API int gpud_serve_channel_audio(gpud_serve* s, int64_t ticket, int64_t cha
 }
 
 // Device bytes of a channel's audio stage: its state block plus its share of
-// the per-slot audio outputs and low-pass scratch.
+// the per-slot audio outputs.
 static size_t audio_footprint(const gpud_serve* s, const ServeChannel& c, int L, int M)
 {
     const int64_t n_in = (int64_t)s->block_samples / c.decim + 2;
     const int64_t n_out = n_in * L / M + 4;
     const int ch = c.stereo ? 2 : 1;
-    size_t per_block = (size_t)(n_out * ch) * sizeof(float);
-    if (c.spec.demod == GPUD_DEMOD_WFM) per_block += (size_t)n_in * sizeof(float);
-    return kAudBytes + per_block * ARGUS_SERVE_SLOTS;
+    return kAudBytes + (size_t)(n_out * ch) * sizeof(float) * ARGUS_SERVE_SLOTS;
 }
 
 GPUD_API int gpud_serve_set_audio(gpud_serve* s, int64_t channel_id, int audio_rate, double deemphasis_us)
@@ -1028,47 +999,41 @@ GPUD_API int gpud_serve_set_audio(gpud_serve* s, int64_t channel_id, int audio_r
         s->epoch++;
         return 0;
     }
-    int L = 1, M = 1;
-    if (argus_audio_arms(c.spec.out_rate, audio_rate, &L, &M, nullptr) != 0) return -1;
+    // plain WFM is low-passed at 15 kHz first (folded into the filter); the
+    // stereo stage's frames already are
+    const bool lowpass = c.spec.demod == GPUD_DEMOD_WFM;
+    int L = 1, M = 1, T = 1;
+    if (argus_audio_filter(c.spec.out_rate, audio_rate, lowpass, &L, &M, &T, nullptr) != 0) return -1;
     const size_t fp = audio_footprint(s, c, L, M);
     if (s->bytes_in_use - old_fp + fp > s->budget) return GPUD_ERR_BUDGET;
-    const double* arms = nullptr;
-    if (L != M) {
-        auto key = std::make_pair(c.spec.out_rate, audio_rate);
-        auto ai = s->arms.find(key);
-        if (ai == s->arms.end()) {
-            std::vector<double> h((size_t)L * ARGUS_AUDIO_ARM);
-            if (argus_audio_arms(c.spec.out_rate, audio_rate, nullptr, nullptr, h.data()) != 0) return -1;
-            double* d = nullptr;
-            ARGUS_CHECK(dev_alloc(&d, h.size()));
-            hipError_t e = hipMemcpy(d, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice);
-            if (e != hipSuccess) {
-                hipFree(d);
-                return (int)e;
-            }
-            ai = s->arms.emplace(key, d).first;
+    const auto key = std::make_tuple(c.spec.out_rate, audio_rate, lowpass);
+    auto fi = s->filters.find(key);
+    if (fi == s->filters.end()) {
+        std::vector<float> h((size_t)L * T);
+        if (argus_audio_filter(c.spec.out_rate, audio_rate, lowpass, nullptr, nullptr, nullptr, h.data()) != 0)
+            return -1;
+        float* d = nullptr;
+        ARGUS_CHECK(dev_alloc(&d, h.size()));
+        hipError_t e = hipMemcpy(d, h.data(), h.size() * sizeof(float), hipMemcpyHostToDevice);
+        if (e != hipSuccess) {
+            hipFree(d);
+            return (int)e;
         }
-        arms = ai->second;
+        fi = s->filters.emplace(key, d).first;
     }
-    // fresh state: zero delay lines and de-emphasis, the low-pass taps
-    std::vector<char> blob(kAudBytes, 0);
-    const bool lpf = c.spec.demod == GPUD_DEMOD_WFM;
-    if (lpf) {
-        std::vector<double> b((size_t)ARGUS_AUDIO_LPF_TAPS + 1);
-        const int nb = argus_lowpass_fir(15000.0, c.spec.out_rate, ARGUS_AUDIO_LPF_TAPS, b.data(), false);
-        float* taps = (float*)(blob.data() + kAudLpfTaps);
-        for (int k = 0; k < ARGUS_AUDIO_LPF_TAPS && k < nb; k++) taps[k] = (float)b[(size_t)k];
-    }
+    // fresh state: zero delay lines and de-emphasis
     if (!c.d_aud) ARGUS_CHECK(dev_alloc(&c.d_aud, kAudBytes));
-    ARGUS_CHECK(hipMemcpy(c.d_aud, blob.data(), kAudBytes, hipMemcpyHostToDevice));
+    const std::vector<char> zero(kAudBytes, 0);
+    ARGUS_CHECK(hipMemcpy(c.d_aud, zero.data(), kAudBytes, hipMemcpyHostToDevice));
     c.audio_rate = audio_rate;
     c.deemph_us = deemphasis_us;
     c.aL = L;
     c.aM = M;
+    c.aT = T;
+    c.a_ident = L == M;
     c.at0 = 0;
-    c.a_sel = c.a_lpf_sel = 0;
-    c.a_lpf = lpf;
-    c.d_arms = arms;
+    c.a_sel = 0;
+    c.d_arms = fi->second;
     c.footprint += fp - old_fp;
     s->bytes_in_use += fp - old_fp;
     s->epoch++;

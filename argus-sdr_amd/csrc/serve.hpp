@@ -5,6 +5,7 @@
 #include "wfm.hpp"
 #include "audio.hpp"
 #include <map>
+#include <tuple>
 #include <vector>
 
 #define ARGUS_SERVE_SLOTS 3
@@ -33,14 +34,13 @@ struct ServeChannel {
     int64_t rds_count = 0;   // discriminator samples seen so far
     int rds_decim = 1;
     // post-demodulation audio stage (gpud_serve_set_audio): one device block of
-    // resampler delay lines [2 halves][2 sides][33] | de-emphasis states [2] |
-    // low-pass delay lines [2][100] | low-pass taps [101]
+    // delay lines [2 halves][2 sides][ARGUS_AUDIO_TMAX] | de-emphasis states [2]
     int audio_rate = 0;      // 0: stage off
     double deemph_us = 0.0;
-    int aL = 1, aM = 1, at0 = 0;
-    int a_sel = 0, a_lpf_sel = 0;
-    bool a_lpf = false;      // plain WFM: 15 kHz low-pass first
-    const double* d_arms = nullptr;  // shared per ratio (gpud_serve::arms)
+    int aL = 1, aM = 1, aT = 1, at0 = 0;
+    bool a_ident = false;    // equal rates
+    int a_sel = 0;           // delay-line half read by the next block
+    const float* d_arms = nullptr;  // shared per filter (gpud_serve::filters)
     char* d_aud = nullptr;
 };
 
@@ -54,7 +54,7 @@ struct ServeOut {
     int a_index;             // index among the block's audio-stage channels, or -1
     int n_aout, a_channels, a_tnext;  // audio frames, channels per frame, next resampler phase
     int a_rate;                       // audio rate (0: no audio stage)
-    int64_t aout_off, alpf_off;       // audio output / low-pass scratch offsets (floats)
+    int64_t aout_off;                 // audio output offset (floats)
 };
 
 struct ServeSlot {
@@ -74,7 +74,7 @@ struct ServeSlot {
     // The block's tables, one pinned host buffer mirrored by one device
     // buffer: extraction work items | ArgusServeHdr | channel descriptors |
     // demod descriptors | stereo descriptors | demod items | stereo items |
-    // audio descriptors | audio low-pass items.
+    // audio descriptors.
     // The work items depend only on the channel set, the block length and
     // the tune, so they are rebuilt and uploaded only when those change
     // (items_epoch, items_n); everything from the header on is uploaded with
@@ -117,11 +117,9 @@ struct ServeSlot {
     size_t aout_cap = 0;
     float* h_aout = nullptr;
     size_t h_aout_cap = 0;
-    float* d_alpf = nullptr;
-    size_t alpf_cap = 0;
     int n_audio_chans = 0;
     int64_t aout_total = 0;
-    int a_mode = 0;          // audio stage of the described block: 0 none, 1 resampler, 2 low-pass + resampler
+    int a_mode = 0;          // the described block runs the audio stage (1) or not (0)
     int g_audio = -1;        // a_mode the stage graph was captured for
     std::vector<ServeOut> outs;
 };
@@ -131,7 +129,7 @@ struct BlockLayout {
     std::vector<ServeOut> outs;
     int n_wfm = 0;
     int n_audio_chans = 0;
-    int64_t iq_total = 0, audio_total = 0, mono_total = 0, rds_total = 0, aout_total = 0, alpf_total = 0;
+    int64_t iq_total = 0, audio_total = 0, mono_total = 0, rds_total = 0, aout_total = 0;
 };
 
 // A block prepared ahead: after a block is launched, the next block's
@@ -174,9 +172,8 @@ struct gpud_serve {
     int64_t last_ticket = 0;
     int64_t completed_ticket = 0;
     ServePrep prep;
-    std::map<std::pair<int, int>, double*> arms;  // resampler arms per (in rate, out rate)
+    std::map<std::tuple<int, int, bool>, float*> filters;  // audio filters per (in rate, out rate, low-pass)
     std::vector<ArgusAudioDesc> adescs;
-    std::vector<ArgusAudioItem> aitems;
     std::vector<ArgusWorkItem> items;
     std::vector<int> item_pos;
     std::vector<ArgusChanDesc> chans;

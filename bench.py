@@ -51,6 +51,7 @@ MODES = [("NFM", 12_500.0, [400, 800], 1), ("AM", 10_000.0, [500, 1000], 3), ("U
 HBM_PEAK_GBS = 8000.0
 P99_BOUND_MS = 5.0
 FP32_PEAK_TFLOPS = 157.3
+DEEMPH_US = 50.0           # de-emphasis time constant of the audio stage (Europe)
 METRIC = "real-time channels sustained @20 Msps span (1 GPU); p99 block latency; HBM GB/s"
 
 
@@ -112,6 +113,8 @@ def bench_serve(a, chans, blocks, lib, S):
     s = S.Serve(FS, BLOCK, 64 << 30)
     for (cid, off, bw, rate, taps, dm) in chans:
         assert s.Open(S.ChannelSpec(cid, off, bw, rate, taps, dm, 0))
+        if a.audio:
+            assert s.SetAudio(cid, a.audio, DEEMPH_US)
     d_blocks = []
     for blk in blocks:
         p = C.c_void_p()
@@ -169,15 +172,19 @@ def bench_serve(a, chans, blocks, lib, S):
     return wall * 1e3 / a.steps, kern, gpu, lat
 
 
-def serve_block_times(chans, d_block, lib, S, blocks=200, warmup=10, tune_fixed=None):
+def serve_block_times(chans, d_block, lib, S, blocks=200, warmup=10, tune_fixed=None, audio=0, infos=None):
     """Latency of each block through the service, one block in flight: wall
     time from the submit call (host-side descriptor build included) to the
     return of its wait, device-resident input, outputs left in HBM.  200
     blocks, so the p99 is a percentile (the second-largest time) rather than
-    the single largest."""
+    the single largest.  audio > 0 turns every channel's audio stage on at
+    that rate; infos, if given, collects each timed block's
+    (kernels ms, extraction ms)."""
     s = S.Serve(FS, BLOCK, 256 << 30)
     for (cid, off, bw, rate, taps, dm) in chans:
         assert s.Open(S.ChannelSpec(cid, off, bw, rate, taps, dm, 0))
+        if audio:
+            assert s.SetAudio(cid, audio, DEEMPH_US)
     if tune_fixed:
         s.SetTune(*tune_fixed)
     else:
@@ -186,11 +193,57 @@ def serve_block_times(chans, d_block, lib, S, blocks=200, warmup=10, tune_fixed=
     out = []
     for i in range(warmup + blocks):
         t1 = time.perf_counter()
-        s.Wait(s.SubmitDevice(d_block, BLOCK))
+        info = s.Wait(s.SubmitDevice(d_block, BLOCK))
         if i >= warmup:
             out.append((time.perf_counter() - t1) * 1e3)
+            if infos is not None:
+                infos.append((info.kernel_ms, info.extract_ms))
     s.Close()
     return out
+
+
+def audio_flops(chans, audio_rate):
+    """Algorithmic work of the audio stage per block: (FP64 flops of the
+    resampler, FP32 flops of the plain-WFM low-pass).  33-tap arms per output
+    (two sides for stereo), 101 taps per low-passed input."""
+    f64 = f32 = 0
+    for (_, _, _, rate, _, dm) in chans:
+        n_in = BLOCK // (FS // rate)
+        n_out = n_in * audio_rate // rate
+        f64 += 2 * 33 * n_out
+        if dm == 2:  # plain WFM
+            f32 += 2 * 101 * n_in
+    return f64, f32
+
+
+def bench_audio_stage(a, n_rt, d_block, lib, S):
+    """The audio stage (plain-WFM low-pass, resampling to 48 kHz, WFM
+    de-emphasis and gain) on every channel of the headline mix: block and
+    kernel times at the headline N with the stage off and on (same tune),
+    and the largest N whose p99 block time stays <= 5 ms with it on."""
+    tune = (a.tune["lds_entries"], a.tune["narrow_outputs"], a.tune["grid"])
+    chans = make_channels(n_rt)
+    base, aud = [], []
+    t_base = serve_block_times(chans, d_block, lib, S, blocks=a.audio_blocks, tune_fixed=tune, infos=base)
+    t_aud = serve_block_times(chans, d_block, lib, S, blocks=a.audio_blocks, tune_fixed=tune, audio=48000,
+                              infos=aud)
+    k_base = percentile([k - x for k, x in base], 50)
+    k_aud = percentile([k - x for k, x in aud], 50)
+    f64, f32 = audio_flops(chans, 48000)
+    stage_ms = k_aud - k_base
+    print("[bench]   capacity with the audio stage", file=sys.stderr, flush=True)
+    start = max(256, n_rt // 2 // 256 * 256)
+    n_aud, sweep = capacity_sweep(
+        lambda n: serve_block_times(make_channels(n), d_block, lib, S, tune_fixed=tune, audio=48000),
+        start, P99_BOUND_MS, a.sweep_cap, 256)
+    return {"channels": n_rt, "audio_rate": 48000, "deemphasis_us": DEEMPH_US,
+            "p50_block_ms_off": round(percentile(t_base, 50), 4), "p99_block_ms_off": round(percentile(t_base, 99), 4),
+            "p50_block_ms_on": round(percentile(t_aud, 50), 4), "p99_block_ms_on": round(percentile(t_aud, 99), 4),
+            "p50_post_extraction_ms_off": round(k_base, 4), "p50_post_extraction_ms_on": round(k_aud, 4),
+            "audio_stage_ms": round(stage_ms, 4),
+            "resampler_fp64_gflop": round(f64 / 1e9, 3), "lowpass_fp32_gflop": round(f32 / 1e9, 3),
+            "real_time_channels_with_audio": n_aud,
+            "sweep": {str(k): {"p50_ms": round(v[0], 3), "p99_ms": round(v[1], 3)} for k, v in sorted(sweep.items())}}
 
 
 def capacity_sweep(times_fn, start, bound_ms, cap, grain):
@@ -465,6 +518,10 @@ def main() -> int:
                     help="skip surveillance, the 2.048 Msps case and the stereo multiplex case")
     ap.add_argument("--calibrate-copy-mb", type=int, default=0,
                     help="run one device-to-device copy of this many MiB (counter calibration)")
+    ap.add_argument("--audio", type=int, default=0,
+                    help="turn every channel's audio stage on at this rate in the timed serving steps")
+    ap.add_argument("--audio-blocks", type=int, default=100,
+                    help="blocks per point of the audio-stage figure (0: skip it)")
     ap.add_argument("--tune", default="",
                     help="LDS_ENTRIES,NARROW_OUTPUTS,GRID: use this tile/launch shape instead of autotuning")
     a = ap.parse_args()
@@ -523,6 +580,10 @@ def main() -> int:
     achieved = b_alg / (kern_ms * 1e-3) / 1e9
     valu = f_alg / (kern_ms * 1e-3) / 1e12
     traffic, traffic_prof = load_traffic(n_rt)
+    audio_stage = None
+    if a.audio_blocks > 0 and not a.no_extras:
+        print("[bench] audio stage (low-pass, resampler to 48 kHz, de-emphasis)", file=sys.stderr, flush=True)
+        audio_stage = bench_audio_stage(a, n_rt, d0.value, L, S)
     L.gpud_free(d0.value)
 
     extras = {}
@@ -589,6 +650,10 @@ def main() -> int:
                      "runner_batch_kernel_ms": round(runner_ms, 4)},
         "cpu_baseline": cpu,
     }
+    if a.audio:
+        out["config"]["audio_stage"] = {"rate": a.audio, "deemphasis_us": DEEMPH_US}
+    if audio_stage:
+        out["audio_stage_48k"] = audio_stage
     out.update(extras)
     print(json.dumps(out), flush=True)
     return 0
