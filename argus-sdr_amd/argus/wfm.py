@@ -75,6 +75,94 @@ class RDSDecoder:
         return out
 
 
+register("gpud_rds_batch_create", I, [C.POINTER(P), I])
+register("gpud_rds_batch_destroy", I, [P])
+register("gpud_rds_batch_reset", I, [P, I])
+register("gpud_rds_batch_decode_device", I, [P, I, P, P, P, P, P, P])
+register("gpud_rds_batch_state", I, [P, I, C.POINTER(C.c_uint16), C.c_char_p, C.c_char_p])
+register("gpud_rds_batch_groups", I, [P, I, P, I])
+
+
+class RDSBatchDecoder:
+    """n_stations rds.Decoder instances whose state lives on the device.
+
+        b = RDSBatchDecoder(20)
+        b.Decode([(station, samples, rate), ...])   # one Decode call each
+        b.PI(station), b.PS(station), b.RT(station), b.Groups(station)
+
+    DecodeDevice takes baseband that is already in device memory."""
+
+    def __init__(self, n_stations: int):
+        h = C.c_void_p()
+        check("gpud_rds_batch_create", lib().gpud_rds_batch_create(C.byref(h), int(n_stations)))
+        self._h = h
+        self.n_stations = int(n_stations)
+
+    def Close(self):
+        if self._h:
+            lib().gpud_rds_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.Close()
+        except Exception:
+            pass
+
+    def Reset(self, station: int):
+        check("gpud_rds_batch_reset", lib().gpud_rds_batch_reset(self._h, int(station)))
+
+    def DecodeDevice(self, stations, samples_dev: int, offsets, rates, stream=None):
+        st = np.ascontiguousarray(stations, np.int32)
+        offs = np.ascontiguousarray(offsets, np.int64)
+        rt = np.ascontiguousarray(rates, np.int32)
+        if len(offs) != len(st) + 1 or len(rt) != len(st):
+            raise ValueError("offsets needs one entry more than stations, rates one per station")
+        found = np.zeros(len(st), np.int32)
+        check("gpud_rds_batch_decode_device",
+              lib().gpud_rds_batch_decode_device(self._h, len(st), st.ctypes.data, samples_dev, offs.ctypes.data,
+                                                 rt.ctypes.data, found.ctypes.data, stream))
+        return [int(v) for v in found]
+
+    def Decode(self, calls):
+        """calls: [(station, complex samples, sample rate), ...], one Decode
+        call per listed station; returns the groups found in each."""
+        from .surveillance import DeviceBuffer
+        xs = [np.ascontiguousarray(x, np.complex64) for _, x, _ in calls]
+        offs = np.zeros(len(xs) + 1, np.int64)
+        offs[1:] = np.cumsum([len(x) for x in xs])
+        total = int(offs[-1])
+        buf = DeviceBuffer(8 * max(total, 1))
+        if total:
+            buf.upload(np.concatenate(xs))
+        return self.DecodeDevice([s for s, _, _ in calls], buf.ptr, offs, [r for _, _, r in calls])
+
+    def _state(self, station):
+        pi = C.c_uint16()
+        ps = C.create_string_buffer(9)
+        rt = C.create_string_buffer(65)
+        check("gpud_rds_batch_state", lib().gpud_rds_batch_state(self._h, int(station), C.byref(pi), ps, rt))
+        return pi.value, ps.value.decode("latin-1"), rt.value.decode("latin-1")
+
+    def PI(self, station: int) -> int:
+        return self._state(station)[0]
+
+    def PS(self, station: int) -> str:
+        return self._state(station)[1]
+
+    def RT(self, station: int) -> str:
+        return self._state(station)[2]
+
+    def Groups(self, station: int) -> np.ndarray:
+        total = lib().gpud_rds_batch_groups(self._h, int(station), None, 0)
+        if total < 0:
+            check("gpud_rds_batch_groups", total)
+        out = np.zeros((total, 4), np.uint16)
+        if total > 0:
+            lib().gpud_rds_batch_groups(self._h, int(station), out.ctypes.data, total)
+        return out
+
+
 register("gpud_wfm_stereo_batch", I, [P, P, P, I, I, P, P, P])
 
 

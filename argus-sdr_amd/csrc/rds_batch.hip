@@ -1,0 +1,532 @@
+// rds_batch.hip -- device-side RDS group decoder: one decoder per station,
+// many stations per call (gpud_rds_batch_*).  Same semantics as the host
+// decoder in rds.hip, which follows internal/rds/rds.go Decoder.Decode
+// (78-264), tryDecode (365-430), decodeGroupAt (432-473), applyGroup
+// (475-502), decodeBlockExpected (549-579) and crcSyndrome (590-599).
+//
+// Three launches per call, all on the caller's stream:
+//
+//   argus_rds_symbols_kernel  one lane per call.  Mueller & Muller timing
+//       with linear interpolation, the Costas loop on each symbol as it is
+//       produced, sign(I) hard bits and the differential bits (carried across
+//       calls).  The loops are serial recurrences, so a lane per station is
+//       the parallelism there is.  Arithmetic follows the reference's
+//       float32/float64 mix operation by operation, with contraction off, so
+//       the bits equal the host decoder's.  The Costas state and the last
+//       hard bit are committed only when the call produced at least 104
+//       symbols, as the reference returns before its Costas loop otherwise.
+//   argus_rds_match_kernel    one thread per (call, bit position).  Whether a
+//       whole group (blocks A, B, C/C', D) starts at that position, in the
+//       direct and in the inverted bit stream, with the unique-single-flip
+//       correction.  The CRC is linear over GF(2), so the syndrome of a word
+//       with bit i flipped is syndrome(word) ^ syndrome(1 << i): one division
+//       per block and 26 table lookups.
+//   argus_rds_scan_kernel     one lane per call.  The serial block-sync scan
+//       with the +-3-bit flywheel over the match flags (direct stream, then
+//       the inverted one if the direct found nothing); each group it takes is
+//       decoded again at its position and applied to the station's PI, PS and
+//       RT on the device, and written to the call's group list.
+//
+// Deviation: the symbol loop of a call stops after 2 n / sps + 16 symbols
+// (the reference has no bound; a symbol step shorter than half a symbol
+// period needs baseband amplitudes in the hundreds), and a step that takes
+// the timing position below -1 ends the call's symbol loop, where the
+// reference would index before the window and panic.
+#include "common.hpp"
+
+#include <algorithm>
+#include <cstring>
+#include <string>
+#include <vector>
+
+namespace {
+
+constexpr int kBlockBits = 26;
+constexpr int kGroupBits = 4 * kBlockBits;
+constexpr int kFlywheel = 3;
+constexpr int kThreads = 256;
+
+// Offset words (rds.go:509-515) and the allowed set per block slot.
+constexpr uint32_t kOffA = 0x0FC, kOffB = 0x198, kOffC = 0x168, kOffCp = 0x350, kOffD = 0x1B4;
+
+struct RdsState {              // per station, device resident
+    double costas_phase, costas_freq, mu;
+    float prev_re, prev_im, prev_dec;
+    int32_t have_prev, last_rate, last_hard, have_last_hard;
+    uint32_t pi;
+    char ps[8];
+    char rt[64];
+};
+
+struct RdsCall {               // per call, uploaded
+    const float2* x;
+    uint8_t* bits;
+    uint8_t* flags;
+    uint16_t* groups;          // 4 words per group
+    int32_t n, rate, station, bit_cap, group_cap, pad;
+};
+
+struct RdsOut {                // per call, downloaded
+    int32_t nb;                // differential bits (-1: no scan this call)
+    int32_t found;             // groups found (and applied)
+    int32_t written;           // groups written to the call's list
+    int32_t pad;
+};
+
+__host__ __device__ inline uint32_t rds_syndrome(uint32_t word)
+{
+    for (int bit = 25; bit >= 10; --bit)
+        if (word & (1u << bit)) word ^= 0x1B9u << (bit - 10);
+    return word & 0x3FFu;
+}
+
+// Syndromes of the 26 single-bit errors, index i = bit (25 - i) of the word.
+struct FlipTable {
+    uint16_t e[kBlockBits];
+};
+
+FlipTable make_flip_table()
+{
+    FlipTable t;
+    for (int i = 0; i < kBlockBits; i++) t.e[i] = (uint16_t)rds_syndrome(1u << (25 - i));
+    return t;
+}
+
+__device__ __forceinline__ bool rds_allowed(uint32_t s, int slot)
+{
+    switch (slot) {
+    case 0: return s == kOffA;
+    case 1: return s == kOffB;
+    case 2: return s == kOffC || s == kOffCp;
+    default: return s == kOffD;
+    }
+}
+
+// rds.go:549-579 with the FEC on: the block's data if its syndrome is an
+// allowed offset word, or if exactly one single-bit flip makes it one.
+__device__ bool rds_block(uint32_t word, int slot, const FlipTable& ft, uint16_t* data)
+{
+    const uint32_t s = rds_syndrome(word);
+    if (rds_allowed(s, slot)) {
+        *data = (uint16_t)(word >> 10);
+        return true;
+    }
+    int found = 0, at = 0;
+    for (int i = 0; i < kBlockBits; i++) {
+        if (rds_allowed(s ^ ft.e[i], slot)) {
+            found++;
+            at = i;
+        }
+    }
+    if (found != 1) return false;
+    *data = (uint16_t)((word ^ (1u << (25 - at))) >> 10);
+    return true;
+}
+
+__device__ uint32_t rds_word(const uint8_t* b, uint32_t inv)
+{
+    uint32_t w = 0;
+#pragma unroll
+    for (int i = 0; i < kBlockBits; i++) w = (w << 1) | (b[i] & 1u);
+    return w ^ inv;
+}
+
+__device__ bool rds_group(const uint8_t* b, uint32_t inv, const FlipTable& ft, uint16_t g[4])
+{
+#pragma unroll 1
+    for (int j = 0; j < 4; j++)
+        if (!rds_block(rds_word(b + j * kBlockBits, inv), j, ft, &g[j])) return false;
+    return true;
+}
+
+__device__ char rds_printable(unsigned v) { return (v < 32 || v > 126) ? ' ' : (char)v; }
+
+// ---------------------------------------------------------------- kernels
+
+__global__ __launch_bounds__(64) void argus_rds_symbols_kernel(const RdsCall* __restrict__ calls,
+                                                               RdsState* __restrict__ states,
+                                                               RdsOut* __restrict__ outs, int n_calls)
+{
+#pragma clang fp contract(off)
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= n_calls) return;
+    const RdsCall call = calls[c];
+    RdsOut& out = outs[c];
+    out.found = 0;
+    out.written = 0;
+    out.nb = -1;
+    const int n = call.n, rate = call.rate;
+    if (n < kGroupBits || rate <= 0) return;  // rds.go:79-81
+    RdsState& st = states[call.station];
+    if (st.last_rate != 0 && st.last_rate != rate) {  // rds.go:84-94
+        st.costas_phase = st.costas_freq = 0;
+        st.mu = 0;
+        st.prev_re = st.prev_im = st.prev_dec = 0;
+        st.have_prev = 0;
+        st.last_hard = 0;
+        st.have_last_hard = 0;
+    }
+    st.last_rate = rate;
+    const double sps = (double)rate / 1187.5;
+    double m = st.mu;
+    if (m < 0 || m >= sps || isnan(m) || isinf(m)) m = sps / 2;
+    float prev_re = st.prev_re, prev_im = st.prev_im, prev_dec = st.prev_dec;
+    bool have_prev = st.have_prev != 0;
+    double phase = st.costas_phase, freq = st.costas_freq;
+    const double alpha = 0.132, beta = alpha * alpha / 4.0;
+    const bool carry = st.have_last_hard != 0;
+    const int carried = st.last_hard;
+    const float2* x = call.x;
+    uint8_t* bits = call.bits;
+    int ns = 0, nb = 0, prev_hard = 0;
+    while (m < (double)(n - 1)) {
+        const int k = (int)m;
+        const double f = m - (double)k;
+        if (k + 1 >= n || k < 0 || ns >= call.bit_cap) break;
+        const float2 a = x[k], b = x[k + 1];
+        const float re = (float)((double)a.x * (1 - f) + (double)b.x * f);
+        const float im = (float)((double)a.y * (1 - f) + (double)b.y * f);
+        const float dec = re >= 0 ? 1.0f : -1.0f;
+        if (have_prev) {
+            const double e = (double)(dec - prev_dec) * (double)prev_re - (double)(re - prev_re) * (double)prev_dec;
+            m += sps + 0.01 * e;
+        } else {
+            m += sps;
+            have_prev = true;
+        }
+        prev_dec = dec;
+        prev_re = re;
+        prev_im = im;
+        // Costas de-rotation of this symbol (rds.go:157-183)
+        const float cp = (float)cos(phase), sp = (float)sin(phase);
+        const float sre = re * cp + im * sp;
+        const float sim = im * cp - re * sp;
+        const double err = sre >= 0 ? (double)sim : -(double)sim;
+        freq += beta * err;
+        phase += freq + alpha * err;
+        while (phase > M_PI) phase -= 2 * M_PI;
+        while (phase < -M_PI) phase += 2 * M_PI;
+        const int hard = sre >= 0 ? 1 : 0;
+        // differential bits, carried across calls (rds.go:214-233)
+        if (ns == 0) {
+            if (carry) bits[nb++] = (uint8_t)(hard ^ carried);
+        } else {
+            bits[nb++] = (uint8_t)(hard ^ prev_hard);
+        }
+        prev_hard = hard;
+        ns++;
+    }
+    double res = m - (double)n;
+    while (res < 0) res += sps;
+    while (res >= sps) res -= sps;
+    st.mu = res;
+    st.prev_re = prev_re;
+    st.prev_im = prev_im;
+    st.prev_dec = prev_dec;
+    st.have_prev = have_prev ? 1 : 0;
+    if (ns < kGroupBits) return;  // rds.go:143-146: Costas and bit carry untouched
+    st.costas_phase = phase;
+    st.costas_freq = freq;
+    st.last_hard = prev_hard;
+    st.have_last_hard = 1;
+    out.nb = nb;
+}
+
+__global__ __launch_bounds__(kThreads) void argus_rds_match_kernel(const RdsCall* __restrict__ calls,
+                                                                   const RdsOut* __restrict__ outs, FlipTable ft)
+{
+    const int c = blockIdx.y;
+    const int nb = outs[c].nb;
+    const int p = blockIdx.x * kThreads + threadIdx.x;
+    if (nb < kGroupBits || p + kGroupBits > nb) return;
+    const RdsCall& call = calls[c];
+    uint16_t g[4];
+    const uint8_t* b = call.bits + p;
+    const uint8_t f = (uint8_t)(rds_group(b, 0u, ft, g) ? 1 : 0) | (uint8_t)(rds_group(b, 0x3FFFFFFu, ft, g) ? 2 : 0);
+    call.flags[p] = f;
+}
+
+struct ScanCtx {
+    const uint8_t* bits;
+    const uint8_t* flags;
+    int nb;
+    uint32_t inv;
+    int pol;
+};
+
+__device__ __forceinline__ bool scan_hit(const ScanCtx& s, int p)
+{
+    return p >= 0 && p + kGroupBits <= s.nb && ((s.flags[p] >> s.pol) & 1);
+}
+
+// rds.go:475-502 applyGroup, plus the call's group list.
+__device__ void scan_apply(const ScanCtx& s, int p, const FlipTable& ft, RdsState& st, const RdsCall& call, RdsOut& out)
+{
+    uint16_t g[4] = {0, 0, 0, 0};
+    rds_group(s.bits + p, s.inv, ft, g);
+    if (g[0] != 0) st.pi = g[0];
+    const int type = (g[1] >> 12) & 0xF;
+    const bool version_a = ((g[1] >> 11) & 1) == 0;
+    if (type == 0 && version_a) {
+        const int at = (g[1] & 3) * 2;
+        st.ps[at] = rds_printable(g[3] >> 8);
+        st.ps[at + 1] = rds_printable(g[3] & 0xFF);
+    }
+    if (type == 2 && version_a) {
+        const int at = (g[1] & 0xF) * 4;
+        const unsigned ch[4] = {(unsigned)(g[2] >> 8), (unsigned)(g[2] & 0xFF), (unsigned)(g[3] >> 8),
+                                (unsigned)(g[3] & 0xFF)};
+        for (int j = 0; j < 4 && at + j < 64; j++) st.rt[at + j] = rds_printable(ch[j]);
+    }
+    if (out.written < call.group_cap) {
+        uint16_t* dst = call.groups + 4 * out.written;
+        dst[0] = g[0];
+        dst[1] = g[1];
+        dst[2] = g[2];
+        dst[3] = g[3];
+        out.written++;
+    }
+}
+
+// rds.go:365-430 tryDecode over the precomputed group flags.
+__device__ int scan_stream(const ScanCtx& s, const FlipTable& ft, RdsState& st, const RdsCall& call, RdsOut& out)
+{
+    int groups = 0;
+    for (int i = 0; i + kGroupBits <= s.nb;) {
+        if (!scan_hit(s, i)) {
+            i++;
+            continue;
+        }
+        groups++;
+        scan_apply(s, i, ft, st, call, out);
+        int next = i + kGroupBits;
+        bool locked = true;
+        while (locked && next + kGroupBits <= s.nb) {
+            if (scan_hit(s, next)) {
+                scan_apply(s, next, ft, st, call, out);
+                groups++;
+                next += kGroupBits;
+                continue;
+            }
+            bool hit = false;
+            for (int dl = 1; dl <= kFlywheel && !hit; dl++) {
+                if (scan_hit(s, next - dl)) {
+                    scan_apply(s, next - dl, ft, st, call, out);
+                    groups++;
+                    next = next - dl + kGroupBits;
+                    hit = true;
+                } else if (scan_hit(s, next + dl)) {
+                    scan_apply(s, next + dl, ft, st, call, out);
+                    groups++;
+                    next = next + dl + kGroupBits;
+                    hit = true;
+                }
+            }
+            locked = hit;
+        }
+        i = next - kFlywheel > i ? next - kFlywheel : i + 1;
+    }
+    return groups;
+}
+
+__global__ __launch_bounds__(64) void argus_rds_scan_kernel(const RdsCall* __restrict__ calls,
+                                                            RdsState* __restrict__ states, RdsOut* __restrict__ outs,
+                                                            int n_calls, FlipTable ft)
+{
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= n_calls) return;
+    const RdsCall call = calls[c];
+    RdsOut& out = outs[c];
+    if (out.nb < kGroupBits) return;
+    RdsState& st = states[call.station];
+    ScanCtx s{call.bits, call.flags, out.nb, 0u, 0};
+    int found = scan_stream(s, ft, st, call, out);
+    if (found == 0) {  // rds.go:242-250: the inverted stream
+        s.inv = 0x3FFFFFFu;
+        s.pol = 1;
+        found = scan_stream(s, ft, st, call, out);
+    }
+    out.found = found;
+}
+
+size_t align256(size_t v) { return (v + 255) & ~(size_t)255; }
+
+}  // namespace
+
+struct gpud_rds_batch {
+    int n_stations = 0;
+    RdsState* d_states = nullptr;
+    std::vector<RdsState> h_states;          // mirror after each call
+    std::vector<std::vector<uint16_t>> log;  // applied groups per station
+    FlipTable ft = make_flip_table();
+};
+
+extern "C" {
+
+GPUD_API int gpud_rds_batch_create(gpud_rds_batch** out, int n_stations)
+{
+    if (!out || n_stations <= 0) return -1;
+    auto* b = new gpud_rds_batch();
+    b->n_stations = n_stations;
+    const size_t bytes = (size_t)n_stations * sizeof(RdsState);
+    hipError_t e = hipMalloc((void**)&b->d_states, bytes);
+    if (e == hipSuccess) e = hipMemset(b->d_states, 0, bytes);
+    if (e != hipSuccess) {
+        if (b->d_states) (void)hipFree(b->d_states);
+        delete b;
+        return (int)e;
+    }
+    b->h_states.assign((size_t)n_stations, RdsState{});
+    std::memset(b->h_states.data(), 0, bytes);
+    b->log.resize((size_t)n_stations);
+    *out = b;
+    return 0;
+}
+
+GPUD_API int gpud_rds_batch_destroy(gpud_rds_batch* b)
+{
+    if (!b) return 0;
+    if (b->d_states) (void)hipFree(b->d_states);
+    delete b;
+    return 0;
+}
+
+GPUD_API int gpud_rds_batch_reset(gpud_rds_batch* b, int station)
+{
+    if (!b || station < 0 || station >= b->n_stations) return -1;
+    ARGUS_CHECK(hipMemset(b->d_states + station, 0, sizeof(RdsState)));
+    std::memset(&b->h_states[(size_t)station], 0, sizeof(RdsState));
+    b->log[(size_t)station].clear();
+    return 0;
+}
+
+GPUD_API int gpud_rds_batch_decode_device(gpud_rds_batch* b, int n_calls, const int32_t* stations,
+                                          const gpud_float2* samples_dev, const int64_t* offsets,
+                                          const int32_t* rates, int32_t* found, gpud_stream_handle stream)
+{
+    if (!b || n_calls < 0 || n_calls > 65535) return -1;
+    if (n_calls == 0) return 0;
+    if (!stations || !offsets || !rates || !found) return -1;
+    std::vector<char> seen((size_t)b->n_stations, 0);
+    if (offsets[0] < 0) return -1;
+    for (int c = 0; c < n_calls; c++) {
+        const int s = stations[c];
+        if (s < 0 || s >= b->n_stations || seen[(size_t)s]) return -1;  // one call per station
+        seen[(size_t)s] = 1;
+        const int64_t n = offsets[c + 1] - offsets[c];
+        if (n < 0 || n > (int64_t)0x3fffffff) return -1;
+        found[c] = 0;
+    }
+    if (offsets[n_calls] > offsets[0] && !samples_dev) return -1;
+    hipStream_t st = (hipStream_t)stream;
+
+    // Per-call workspace: [calls | outs | bits | flags | groups].
+    std::vector<RdsCall> calls((size_t)n_calls);
+    size_t bit_total = 0, group_total = 0;
+    int max_cap = 0;
+    for (int c = 0; c < n_calls; c++) {
+        const int n = (int)(offsets[c + 1] - offsets[c]);
+        const int rate = rates[c];
+        RdsCall& k = calls[(size_t)c];
+        k.n = n;
+        k.rate = rate;
+        k.station = stations[c];
+        k.bit_cap = 0;
+        k.group_cap = 0;
+        if (n >= kGroupBits && rate > 0) {
+            const double sps = (double)rate / 1187.5;
+            const double cap = 2.0 * (double)n / sps + 16.0;
+            k.bit_cap = cap > (double)0x3fffffff ? 0x3fffffff : (int)cap;
+            k.group_cap = k.bit_cap / (kGroupBits - kFlywheel) + 2;
+        }
+        bit_total += align256((size_t)k.bit_cap);
+        group_total += (size_t)k.group_cap * 4;
+        max_cap = std::max(max_cap, k.bit_cap);
+    }
+    const size_t off_outs = align256((size_t)n_calls * sizeof(RdsCall));
+    const size_t off_bits = off_outs + align256((size_t)n_calls * sizeof(RdsOut));
+    const size_t off_flags = off_bits + bit_total;
+    const size_t off_groups = off_flags + bit_total;
+    const size_t bytes = off_groups + group_total * sizeof(uint16_t);
+    char* d = nullptr;
+    ARGUS_CHECK(hipMallocAsync((void**)&d, bytes, st));
+    size_t bo = 0, go = 0;
+    for (int c = 0; c < n_calls; c++) {
+        RdsCall& k = calls[(size_t)c];
+        k.x = (const float2*)(samples_dev + offsets[c]);
+        k.bits = (uint8_t*)(d + off_bits + bo);
+        k.flags = (uint8_t*)(d + off_flags + bo);
+        k.groups = (uint16_t*)(d + off_groups) + go;
+        bo += align256((size_t)k.bit_cap);
+        go += (size_t)k.group_cap * 4;
+    }
+    std::vector<RdsOut> outs((size_t)n_calls);
+    std::vector<uint16_t> groups(group_total);
+    int err = (int)hipMemcpyAsync(d, calls.data(), (size_t)n_calls * sizeof(RdsCall), hipMemcpyHostToDevice, st);
+    const RdsCall* d_calls = (const RdsCall*)d;
+    RdsOut* d_outs = (RdsOut*)(d + off_outs);
+    if (!err) {
+        argus_clear_error();
+        const int blocks = (n_calls + 63) / 64;
+        argus_rds_symbols_kernel<<<blocks, 64, 0, st>>>(d_calls, b->d_states, d_outs, n_calls);
+        err = (int)hipGetLastError();
+    }
+    if (!err && max_cap >= kGroupBits) {
+        dim3 grid((unsigned)((max_cap + kThreads - 1) / kThreads), (unsigned)n_calls);
+        argus_rds_match_kernel<<<grid, kThreads, 0, st>>>(d_calls, d_outs, b->ft);
+        err = (int)hipGetLastError();
+    }
+    if (!err) {
+        argus_rds_scan_kernel<<<(n_calls + 63) / 64, 64, 0, st>>>(d_calls, b->d_states, d_outs, n_calls, b->ft);
+        err = (int)hipGetLastError();
+    }
+    if (!err)
+        err = (int)hipMemcpyAsync(outs.data(), d_outs, (size_t)n_calls * sizeof(RdsOut), hipMemcpyDeviceToHost, st);
+    if (!err && group_total)
+        err = (int)hipMemcpyAsync(groups.data(), d + off_groups, group_total * sizeof(uint16_t),
+                                  hipMemcpyDeviceToHost, st);
+    if (!err)
+        err = (int)hipMemcpyAsync(b->h_states.data(), b->d_states, (size_t)b->n_stations * sizeof(RdsState),
+                                  hipMemcpyDeviceToHost, st);
+    hipFreeAsync(d, st);
+    const hipError_t se = hipStreamSynchronize(st);
+    if (!err) err = (int)se;
+    if (err) return err;
+    go = 0;
+    for (int c = 0; c < n_calls; c++) {
+        const RdsOut& o = outs[(size_t)c];
+        found[c] = o.found;
+        auto& lg = b->log[(size_t)stations[c]];
+        lg.insert(lg.end(), groups.begin() + (ptrdiff_t)go, groups.begin() + (ptrdiff_t)(go + 4 * (size_t)o.written));
+        go += (size_t)calls[(size_t)c].group_cap * 4;
+    }
+    return 0;
+}
+
+GPUD_API int gpud_rds_batch_state(gpud_rds_batch* b, int station, uint16_t* pi, char* ps9, char* rt65)
+{
+    if (!b || station < 0 || station >= b->n_stations) return -1;
+    const RdsState& s = b->h_states[(size_t)station];
+    if (pi) *pi = (uint16_t)s.pi;
+    auto text = [](const char* src, int len, char* dst) {  // rds.go:604-636
+        int end = len;
+        while (end > 0 && (src[end - 1] == ' ' || src[end - 1] == 0)) end--;
+        for (int i = 0; i < end; i++) dst[i] = src[i] ? src[i] : ' ';
+        dst[end] = 0;
+    };
+    if (ps9) text(s.ps, 8, ps9);
+    if (rt65) text(s.rt, 64, rt65);
+    return 0;
+}
+
+GPUD_API int gpud_rds_batch_groups(gpud_rds_batch* b, int station, uint16_t* groups4, int cap)
+{
+    if (!b || station < 0 || station >= b->n_stations || cap < 0 || (cap > 0 && !groups4)) return -1;
+    const auto& lg = b->log[(size_t)station];
+    const int total = (int)(lg.size() / 4);
+    const int m = total < cap ? total : cap;
+    if (m > 0) std::memcpy(groups4, lg.data(), (size_t)m * 4 * sizeof(uint16_t));
+    return total;
+}
+
+}  // extern "C"

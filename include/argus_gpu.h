@@ -350,6 +350,33 @@ GPUD_API int gpud_rds_decoder_state(gpud_rds_decoder* d, uint16_t* pi, char* ps9
 GPUD_API int gpud_rds_decoder_groups(gpud_rds_decoder* d, uint16_t* groups4, int cap);
 
 /* ------------------------------------------------------------------ */
+/* RDS group decoder on the device: n_stations decoders with the         */
+/* semantics of rds.Decoder.Decode (rds.go:78-264, tryDecode 365-430,    */
+/* applyGroup 475-502, decodeBlockExpected 549-579), their state held in */
+/* device memory, decoding many stations' windows in one call.  Where    */
+/* pipeline_runtime.go:1132-1137 decodes each station's 4 s window on    */
+/* its own goroutine, one call here takes every station due that tick.   */
+/* ------------------------------------------------------------------ */
+typedef struct gpud_rds_batch gpud_rds_batch;
+GPUD_API int gpud_rds_batch_create(gpud_rds_batch** out, int n_stations);
+GPUD_API int gpud_rds_batch_destroy(gpud_rds_batch* b);
+/* Back to a fresh decoder (state zero, group log empty). */
+GPUD_API int gpud_rds_batch_reset(gpud_rds_batch* b, int station);
+/* One Decode call for each of n_calls stations (at most 65535, each at
+ * most once): station stations[i] decodes the complex baseband
+ * samples_dev[offsets[i] .. offsets[i+1]) (device memory) at rates[i];
+ * found[i] = groups found in that call.  Kernels run on `stream` after
+ * the work already queued there; returns when they have finished and the
+ * groups and state are on the host.  -1 bad arguments. */
+GPUD_API int gpud_rds_batch_decode_device(gpud_rds_batch* b, int n_calls, const int32_t* stations,
+                                          const gpud_float2* samples_dev, const int64_t* offsets,
+                                          const int32_t* rates, int32_t* found, gpud_stream_handle stream);
+/* PI, PS and RT of a station, as gpud_rds_decoder_state reports them. */
+GPUD_API int gpud_rds_batch_state(gpud_rds_batch* b, int station, uint16_t* pi, char* ps9, char* rt65);
+/* Up to cap of the station's applied groups in order; returns the total. */
+GPUD_API int gpud_rds_batch_groups(gpud_rds_batch* b, int station, uint16_t* groups4, int cap);
+
+/* ------------------------------------------------------------------ */
 /* Batch WFM stereo: wfmStereoDecodeBatch (internal/recorder/           */
 /* wfm_batch_stereo.go:76-157) over n_stations whole recordings at once. */
 /* offsets holds n_stations + 1 sample offsets: station i's             */
