@@ -60,6 +60,9 @@ register("gpud_serve_channel_wfm", I, [P, C.c_int64, C.c_int64, C.POINTER(P), C.
 register("gpud_serve_set_audio", I, [P, C.c_int64, I, C.c_double])
 register("gpud_serve_channel_audio", I, [P, C.c_int64, C.c_int64, C.POINTER(P), C.POINTER(I), C.POINTER(I),
                                          C.POINTER(I)])
+register("gpud_serve_set_rds", I, [P, C.c_int64, I])
+register("gpud_serve_channel_rds", I, [P, C.c_int64, C.c_int64, C.POINTER(I), C.POINTER(I), C.POINTER(C.c_uint16),
+                                       C.c_char_p, C.c_char_p, C.POINTER(P), C.POINTER(I)])
 
 
 @dataclass
@@ -188,6 +191,33 @@ class Serve:
         cnt = n.value * ch.value
         a = np.ctypeslib.as_array((C.c_float * cnt).from_address(p.value)).copy() if cnt else np.zeros(0, np.float32)
         return a.reshape(n.value, ch.value), rate.value
+
+    def SetRDS(self, channel_id: int, window_samples: int) -> bool:
+        """Decode a stereo channel's RDS on the device in windows of
+        window_samples baseband samples (0 turns it off).  False if the HBM
+        budget refuses it."""
+        rc = lib().gpud_serve_set_rds(self._h, channel_id, window_samples)
+        if rc == ERR_BUDGET:
+            return False
+        check("gpud_serve_set_rds", rc)
+        return True
+
+    def OutputRDS(self, ticket: int, channel_id: int):
+        """(decoded, found, groups (n, 4) uint16, PI, PS, RT) of a channel's
+        RDS decoding in a completed block; None if the channel was not in the
+        block or its decoding was off."""
+        dec, found, n = C.c_int(), C.c_int(), C.c_int()
+        pi = C.c_uint16()
+        ps, rt = C.create_string_buffer(9), C.create_string_buffer(65)
+        p = C.c_void_p()
+        rc = lib().gpud_serve_channel_rds(self._h, ticket, channel_id, C.byref(dec), C.byref(found), C.byref(pi),
+                                          ps, rt, C.byref(p), C.byref(n))
+        if rc in (-1, -3):
+            return None
+        check("gpud_serve_channel_rds", rc)
+        g = np.ctypeslib.as_array((C.c_uint16 * (4 * n.value)).from_address(p.value)).copy().reshape(-1, 4) \
+            if n.value else np.zeros((0, 4), np.uint16)
+        return (bool(dec.value), found.value, g, pi.value, ps.value.decode("latin-1"), rt.value.decode("latin-1"))
 
     def SetGraph(self, enable: bool) -> None:
         check("gpud_serve_set_graph", lib().gpud_serve_set_graph(self._h, 1 if enable else 0))

@@ -32,7 +32,7 @@
 // period needs baseband amplitudes in the hundreds), and a step that takes
 // the timing position below -1 ends the call's symbol loop, where the
 // reference would index before the window and panic.
-#include "common.hpp"
+#include "rds.hpp"
 
 #include <algorithm>
 #include <cstring>
@@ -48,30 +48,6 @@ constexpr int kThreads = 256;
 
 // Offset words (rds.go:509-515) and the allowed set per block slot.
 constexpr uint32_t kOffA = 0x0FC, kOffB = 0x198, kOffC = 0x168, kOffCp = 0x350, kOffD = 0x1B4;
-
-struct RdsState {              // per station, device resident
-    double costas_phase, costas_freq, mu;
-    float prev_re, prev_im, prev_dec;
-    int32_t have_prev, last_rate, last_hard, have_last_hard;
-    uint32_t pi;
-    char ps[8];
-    char rt[64];
-};
-
-struct RdsCall {               // per call, uploaded
-    const float2* x;
-    uint8_t* bits;
-    uint8_t* flags;
-    uint16_t* groups;          // 4 words per group
-    int32_t n, rate, station, bit_cap, group_cap, pad;
-};
-
-struct RdsOut {                // per call, downloaded
-    int32_t nb;                // differential bits (-1: no scan this call)
-    int32_t found;             // groups found (and applied)
-    int32_t written;           // groups written to the call's list
-    int32_t pad;
-};
 
 __host__ __device__ inline uint32_t rds_syndrome(uint32_t word)
 {
@@ -143,103 +119,107 @@ __device__ char rds_printable(unsigned v) { return (v < 32 || v > 126) ? ' ' : (
 
 // ---------------------------------------------------------------- kernels
 
-__global__ __launch_bounds__(64) void argus_rds_symbols_kernel(const RdsCall* __restrict__ calls,
-                                                               RdsState* __restrict__ states,
-                                                               RdsOut* __restrict__ outs, int n_calls)
+__global__ __launch_bounds__(64) void argus_rds_symbols_kernel(const ArgusRdsCall* __restrict__ calls, int n_calls)
 {
 #pragma clang fp contract(off)
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= n_calls) return;
-    const RdsCall call = calls[c];
-    RdsOut& out = outs[c];
+    const ArgusRdsCall call = calls[c];
+    ArgusRdsOut& out = *call.out;
     out.found = 0;
     out.written = 0;
     out.nb = -1;
     const int n = call.n, rate = call.rate;
-    if (n < kGroupBits || rate <= 0) return;  // rds.go:79-81
-    RdsState& st = states[call.station];
-    if (st.last_rate != 0 && st.last_rate != rate) {  // rds.go:84-94
-        st.costas_phase = st.costas_freq = 0;
-        st.mu = 0;
-        st.prev_re = st.prev_im = st.prev_dec = 0;
-        st.have_prev = 0;
-        st.last_hard = 0;
-        st.have_last_hard = 0;
-    }
-    st.last_rate = rate;
-    const double sps = (double)rate / 1187.5;
-    double m = st.mu;
-    if (m < 0 || m >= sps || isnan(m) || isinf(m)) m = sps / 2;
-    float prev_re = st.prev_re, prev_im = st.prev_im, prev_dec = st.prev_dec;
-    bool have_prev = st.have_prev != 0;
-    double phase = st.costas_phase, freq = st.costas_freq;
-    const double alpha = 0.132, beta = alpha * alpha / 4.0;
-    const bool carry = st.have_last_hard != 0;
-    const int carried = st.last_hard;
-    const float2* x = call.x;
-    uint8_t* bits = call.bits;
-    int ns = 0, nb = 0, prev_hard = 0;
-    while (m < (double)(n - 1)) {
-        const int k = (int)m;
-        const double f = m - (double)k;
-        if (k + 1 >= n || k < 0 || ns >= call.bit_cap) break;
-        const float2 a = x[k], b = x[k + 1];
-        const float re = (float)((double)a.x * (1 - f) + (double)b.x * f);
-        const float im = (float)((double)a.y * (1 - f) + (double)b.y * f);
-        const float dec = re >= 0 ? 1.0f : -1.0f;
-        if (have_prev) {
-            const double e = (double)(dec - prev_dec) * (double)prev_re - (double)(re - prev_re) * (double)prev_dec;
-            m += sps + 0.01 * e;
-        } else {
-            m += sps;
-            have_prev = true;
+    ArgusRdsState& st = *call.st;
+    if (n >= kGroupBits && rate > 0) {  // rds.go:79-81
+        if (st.last_rate != 0 && st.last_rate != rate) {  // rds.go:84-94
+            st.costas_phase = st.costas_freq = 0;
+            st.mu = 0;
+            st.prev_re = st.prev_im = st.prev_dec = 0;
+            st.have_prev = 0;
+            st.last_hard = 0;
+            st.have_last_hard = 0;
         }
-        prev_dec = dec;
-        prev_re = re;
-        prev_im = im;
-        // Costas de-rotation of this symbol (rds.go:157-183)
-        const float cp = (float)cos(phase), sp = (float)sin(phase);
-        const float sre = re * cp + im * sp;
-        const float sim = im * cp - re * sp;
-        const double err = sre >= 0 ? (double)sim : -(double)sim;
-        freq += beta * err;
-        phase += freq + alpha * err;
-        while (phase > M_PI) phase -= 2 * M_PI;
-        while (phase < -M_PI) phase += 2 * M_PI;
-        const int hard = sre >= 0 ? 1 : 0;
-        // differential bits, carried across calls (rds.go:214-233)
-        if (ns == 0) {
-            if (carry) bits[nb++] = (uint8_t)(hard ^ carried);
-        } else {
-            bits[nb++] = (uint8_t)(hard ^ prev_hard);
+        st.last_rate = rate;
+        const double sps = (double)rate / 1187.5;
+        double m = st.mu;
+        if (m < 0 || m >= sps || isnan(m) || isinf(m)) m = sps / 2;
+        float prev_re = st.prev_re, prev_im = st.prev_im, prev_dec = st.prev_dec;
+        bool have_prev = st.have_prev != 0;
+        double phase = st.costas_phase, freq = st.costas_freq;
+        const double alpha = 0.132, beta = alpha * alpha / 4.0;
+        const bool carry = st.have_last_hard != 0;
+        const int carried = st.last_hard;
+        const float2* x = call.x;
+        uint8_t* bits = call.bits;
+        int ns = 0, nb = 0, prev_hard = 0;
+        while (m < (double)(n - 1)) {
+            const int k = (int)m;
+            const double f = m - (double)k;
+            if (k + 1 >= n || k < 0 || ns >= call.bit_cap) break;
+            const float2 a = x[k], b = x[k + 1];
+            const float re = (float)((double)a.x * (1 - f) + (double)b.x * f);
+            const float im = (float)((double)a.y * (1 - f) + (double)b.y * f);
+            const float dec = re >= 0 ? 1.0f : -1.0f;
+            if (have_prev) {
+                const double e =
+                    (double)(dec - prev_dec) * (double)prev_re - (double)(re - prev_re) * (double)prev_dec;
+                m += sps + 0.01 * e;
+            } else {
+                m += sps;
+                have_prev = true;
+            }
+            prev_dec = dec;
+            prev_re = re;
+            prev_im = im;
+            // Costas de-rotation of this symbol (rds.go:157-183)
+            const float cp = (float)cos(phase), sp = (float)sin(phase);
+            const float sre = re * cp + im * sp;
+            const float sim = im * cp - re * sp;
+            const double err = sre >= 0 ? (double)sim : -(double)sim;
+            freq += beta * err;
+            phase += freq + alpha * err;
+            while (phase > M_PI) phase -= 2 * M_PI;
+            while (phase < -M_PI) phase += 2 * M_PI;
+            const int hard = sre >= 0 ? 1 : 0;
+            // differential bits, carried across calls (rds.go:214-233)
+            if (ns == 0) {
+                if (carry) bits[nb++] = (uint8_t)(hard ^ carried);
+            } else {
+                bits[nb++] = (uint8_t)(hard ^ prev_hard);
+            }
+            prev_hard = hard;
+            ns++;
         }
-        prev_hard = hard;
-        ns++;
+        double res = m - (double)n;
+        while (res < 0) res += sps;
+        while (res >= sps) res -= sps;
+        st.mu = res;
+        st.prev_re = prev_re;
+        st.prev_im = prev_im;
+        st.prev_dec = prev_dec;
+        st.have_prev = have_prev ? 1 : 0;
+        if (ns >= kGroupBits) {  // else rds.go:143-146: Costas and bit carry untouched
+            st.costas_phase = phase;
+            st.costas_freq = freq;
+            st.last_hard = prev_hard;
+            st.have_last_hard = 1;
+            out.nb = nb;
+        }
     }
-    double res = m - (double)n;
-    while (res < 0) res += sps;
-    while (res >= sps) res -= sps;
-    st.mu = res;
-    st.prev_re = prev_re;
-    st.prev_im = prev_im;
-    st.prev_dec = prev_dec;
-    st.have_prev = have_prev ? 1 : 0;
-    if (ns < kGroupBits) return;  // rds.go:143-146: Costas and bit carry untouched
-    st.costas_phase = phase;
-    st.costas_freq = freq;
-    st.last_hard = prev_hard;
-    st.have_last_hard = 1;
-    out.nb = nb;
+    // the decoder's text as it stands (the scan kernel updates it if it runs)
+    out.pi = st.pi;
+    for (int i = 0; i < 8; i++) out.ps[i] = st.ps[i];
+    for (int i = 0; i < 64; i++) out.rt[i] = st.rt[i];
 }
 
-__global__ __launch_bounds__(kThreads) void argus_rds_match_kernel(const RdsCall* __restrict__ calls,
-                                                                   const RdsOut* __restrict__ outs, FlipTable ft)
+__global__ __launch_bounds__(kThreads) void argus_rds_match_kernel(const ArgusRdsCall* __restrict__ calls,
+                                                                   FlipTable ft)
 {
-    const int c = blockIdx.y;
-    const int nb = outs[c].nb;
+    const ArgusRdsCall& call = calls[blockIdx.y];
+    const int nb = call.out->nb;
     const int p = blockIdx.x * kThreads + threadIdx.x;
     if (nb < kGroupBits || p + kGroupBits > nb) return;
-    const RdsCall& call = calls[c];
     uint16_t g[4];
     const uint8_t* b = call.bits + p;
     const uint8_t f = (uint8_t)(rds_group(b, 0u, ft, g) ? 1 : 0) | (uint8_t)(rds_group(b, 0x3FFFFFFu, ft, g) ? 2 : 0);
@@ -260,7 +240,8 @@ __device__ __forceinline__ bool scan_hit(const ScanCtx& s, int p)
 }
 
 // rds.go:475-502 applyGroup, plus the call's group list.
-__device__ void scan_apply(const ScanCtx& s, int p, const FlipTable& ft, RdsState& st, const RdsCall& call, RdsOut& out)
+__device__ void scan_apply(const ScanCtx& s, int p, const FlipTable& ft, ArgusRdsState& st, const ArgusRdsCall& call,
+                           ArgusRdsOut& out)
 {
     uint16_t g[4] = {0, 0, 0, 0};
     rds_group(s.bits + p, s.inv, ft, g);
@@ -289,7 +270,8 @@ __device__ void scan_apply(const ScanCtx& s, int p, const FlipTable& ft, RdsStat
 }
 
 // rds.go:365-430 tryDecode over the precomputed group flags.
-__device__ int scan_stream(const ScanCtx& s, const FlipTable& ft, RdsState& st, const RdsCall& call, RdsOut& out)
+__device__ int scan_stream(const ScanCtx& s, const FlipTable& ft, ArgusRdsState& st, const ArgusRdsCall& call,
+                           ArgusRdsOut& out)
 {
     int groups = 0;
     for (int i = 0; i + kGroupBits <= s.nb;) {
@@ -329,16 +311,15 @@ __device__ int scan_stream(const ScanCtx& s, const FlipTable& ft, RdsState& st, 
     return groups;
 }
 
-__global__ __launch_bounds__(64) void argus_rds_scan_kernel(const RdsCall* __restrict__ calls,
-                                                            RdsState* __restrict__ states, RdsOut* __restrict__ outs,
-                                                            int n_calls, FlipTable ft)
+__global__ __launch_bounds__(64) void argus_rds_scan_kernel(const ArgusRdsCall* __restrict__ calls, int n_calls,
+                                                            FlipTable ft)
 {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= n_calls) return;
-    const RdsCall call = calls[c];
-    RdsOut& out = outs[c];
+    const ArgusRdsCall call = calls[c];
+    ArgusRdsOut& out = *call.out;
     if (out.nb < kGroupBits) return;
-    RdsState& st = states[call.station];
+    ArgusRdsState& st = *call.st;
     ScanCtx s{call.bits, call.flags, out.nb, 0u, 0};
     int found = scan_stream(s, ft, st, call, out);
     if (found == 0) {  // rds.go:242-250: the inverted stream
@@ -347,18 +328,79 @@ __global__ __launch_bounds__(64) void argus_rds_scan_kernel(const RdsCall* __res
         found = scan_stream(s, ft, st, call, out);
     }
     out.found = found;
+    out.pi = st.pi;
+    for (int i = 0; i < 8; i++) out.ps[i] = st.ps[i];
+    for (int i = 0; i < 64; i++) out.rt[i] = st.rt[i];
+}
+
+// One workgroup per station: a block's RDS baseband into its decode windows.
+__global__ __launch_bounds__(kThreads) void argus_rds_append_kernel(const ArgusRdsAppend* __restrict__ ap)
+{
+    const ArgusRdsAppend a = ap[blockIdx.x];
+    for (int i = threadIdx.x; i < a.n0 + a.n1; i += kThreads) {
+        const float2 v = a.src[i];
+        if (i < a.n0)
+            a.dst0[i] = v;
+        else
+            a.dst1[i - a.n0] = v;
+    }
 }
 
 size_t align256(size_t v) { return (v + 255) & ~(size_t)255; }
 
+const FlipTable kFlips = make_flip_table();
+
 }  // namespace
+
+void argus_rds_caps(int n, int rate, int* bit_cap, int* group_cap)
+{
+    int bc = 0, gc = 0;
+    if (n >= kGroupBits && rate > 0) {
+        const double cap = 2.0 * (double)n / ((double)rate / 1187.5) + 16.0;
+        bc = cap > (double)0x3fffffff ? 0x3fffffff : (int)cap;
+        gc = bc / (kGroupBits - kFlywheel) + 2;  // groups start >= 101 bits apart
+    }
+    *bit_cap = bc;
+    *group_cap = gc;
+}
+
+int argus_launch_rds(const ArgusRdsCall* d_calls, int n_calls, int max_bit_cap, hipStream_t stream)
+{
+    if (n_calls <= 0) return 0;
+    if (n_calls > 65535) return -1;
+    argus_clear_error();
+    argus_rds_symbols_kernel<<<(n_calls + 63) / 64, 64, 0, stream>>>(d_calls, n_calls);
+    ARGUS_CHECK(hipGetLastError());
+    if (max_bit_cap >= kGroupBits) {
+        dim3 grid((unsigned)((max_bit_cap + kThreads - 1) / kThreads), (unsigned)n_calls);
+        argus_rds_match_kernel<<<grid, kThreads, 0, stream>>>(d_calls, kFlips);
+        ARGUS_CHECK(hipGetLastError());
+    }
+    argus_rds_scan_kernel<<<(n_calls + 63) / 64, 64, 0, stream>>>(d_calls, n_calls, kFlips);
+    return (int)hipGetLastError();
+}
+
+int argus_launch_rds_append(const ArgusRdsAppend* d_appends, int n, hipStream_t stream)
+{
+    if (n <= 0) return 0;
+    argus_clear_error();
+    argus_rds_append_kernel<<<n, kThreads, 0, stream>>>(d_appends);
+    return (int)hipGetLastError();
+}
+
+void argus_rds_text(const char* src, int len, char* dst)  // rds.go:604-636
+{
+    int end = len;
+    while (end > 0 && (src[end - 1] == ' ' || src[end - 1] == 0)) end--;
+    for (int i = 0; i < end; i++) dst[i] = src[i] ? src[i] : ' ';
+    dst[end] = 0;
+}
 
 struct gpud_rds_batch {
     int n_stations = 0;
-    RdsState* d_states = nullptr;
-    std::vector<RdsState> h_states;          // mirror after each call
+    ArgusRdsState* d_states = nullptr;
+    std::vector<ArgusRdsOut> last;           // each station's state after its last call
     std::vector<std::vector<uint16_t>> log;  // applied groups per station
-    FlipTable ft = make_flip_table();
 };
 
 extern "C" {
@@ -368,7 +410,7 @@ GPUD_API int gpud_rds_batch_create(gpud_rds_batch** out, int n_stations)
     if (!out || n_stations <= 0) return -1;
     auto* b = new gpud_rds_batch();
     b->n_stations = n_stations;
-    const size_t bytes = (size_t)n_stations * sizeof(RdsState);
+    const size_t bytes = (size_t)n_stations * sizeof(ArgusRdsState);
     hipError_t e = hipMalloc((void**)&b->d_states, bytes);
     if (e == hipSuccess) e = hipMemset(b->d_states, 0, bytes);
     if (e != hipSuccess) {
@@ -376,8 +418,8 @@ GPUD_API int gpud_rds_batch_create(gpud_rds_batch** out, int n_stations)
         delete b;
         return (int)e;
     }
-    b->h_states.assign((size_t)n_stations, RdsState{});
-    std::memset(b->h_states.data(), 0, bytes);
+    b->last.assign((size_t)n_stations, ArgusRdsOut{});
+    std::memset(b->last.data(), 0, b->last.size() * sizeof(ArgusRdsOut));
     b->log.resize((size_t)n_stations);
     *out = b;
     return 0;
@@ -394,8 +436,8 @@ GPUD_API int gpud_rds_batch_destroy(gpud_rds_batch* b)
 GPUD_API int gpud_rds_batch_reset(gpud_rds_batch* b, int station)
 {
     if (!b || station < 0 || station >= b->n_stations) return -1;
-    ARGUS_CHECK(hipMemset(b->d_states + station, 0, sizeof(RdsState)));
-    std::memset(&b->h_states[(size_t)station], 0, sizeof(RdsState));
+    ARGUS_CHECK(hipMemset(b->d_states + station, 0, sizeof(ArgusRdsState)));
+    std::memset(&b->last[(size_t)station], 0, sizeof(ArgusRdsOut));
     b->log[(size_t)station].clear();
     return 0;
 }
@@ -421,72 +463,47 @@ GPUD_API int gpud_rds_batch_decode_device(gpud_rds_batch* b, int n_calls, const 
     hipStream_t st = (hipStream_t)stream;
 
     // Per-call workspace: [calls | outs | bits | flags | groups].
-    std::vector<RdsCall> calls((size_t)n_calls);
+    std::vector<ArgusRdsCall> calls((size_t)n_calls);
     size_t bit_total = 0, group_total = 0;
     int max_cap = 0;
     for (int c = 0; c < n_calls; c++) {
-        const int n = (int)(offsets[c + 1] - offsets[c]);
-        const int rate = rates[c];
-        RdsCall& k = calls[(size_t)c];
-        k.n = n;
-        k.rate = rate;
-        k.station = stations[c];
-        k.bit_cap = 0;
-        k.group_cap = 0;
-        if (n >= kGroupBits && rate > 0) {
-            const double sps = (double)rate / 1187.5;
-            const double cap = 2.0 * (double)n / sps + 16.0;
-            k.bit_cap = cap > (double)0x3fffffff ? 0x3fffffff : (int)cap;
-            k.group_cap = k.bit_cap / (kGroupBits - kFlywheel) + 2;
-        }
+        ArgusRdsCall& k = calls[(size_t)c];
+        k.n = (int)(offsets[c + 1] - offsets[c]);
+        k.rate = rates[c];
+        argus_rds_caps(k.n, k.rate, &k.bit_cap, &k.group_cap);
         bit_total += align256((size_t)k.bit_cap);
         group_total += (size_t)k.group_cap * 4;
         max_cap = std::max(max_cap, k.bit_cap);
     }
-    const size_t off_outs = align256((size_t)n_calls * sizeof(RdsCall));
-    const size_t off_bits = off_outs + align256((size_t)n_calls * sizeof(RdsOut));
+    const size_t off_outs = align256((size_t)n_calls * sizeof(ArgusRdsCall));
+    const size_t off_bits = off_outs + align256((size_t)n_calls * sizeof(ArgusRdsOut));
     const size_t off_flags = off_bits + bit_total;
     const size_t off_groups = off_flags + bit_total;
     const size_t bytes = off_groups + group_total * sizeof(uint16_t);
     char* d = nullptr;
     ARGUS_CHECK(hipMallocAsync((void**)&d, bytes, st));
+    ArgusRdsOut* d_outs = (ArgusRdsOut*)(d + off_outs);
     size_t bo = 0, go = 0;
     for (int c = 0; c < n_calls; c++) {
-        RdsCall& k = calls[(size_t)c];
+        ArgusRdsCall& k = calls[(size_t)c];
         k.x = (const float2*)(samples_dev + offsets[c]);
         k.bits = (uint8_t*)(d + off_bits + bo);
         k.flags = (uint8_t*)(d + off_flags + bo);
         k.groups = (uint16_t*)(d + off_groups) + go;
+        k.st = b->d_states + stations[c];
+        k.out = d_outs + c;
         bo += align256((size_t)k.bit_cap);
         go += (size_t)k.group_cap * 4;
     }
-    std::vector<RdsOut> outs((size_t)n_calls);
+    std::vector<ArgusRdsOut> outs((size_t)n_calls);
     std::vector<uint16_t> groups(group_total);
-    int err = (int)hipMemcpyAsync(d, calls.data(), (size_t)n_calls * sizeof(RdsCall), hipMemcpyHostToDevice, st);
-    const RdsCall* d_calls = (const RdsCall*)d;
-    RdsOut* d_outs = (RdsOut*)(d + off_outs);
-    if (!err) {
-        argus_clear_error();
-        const int blocks = (n_calls + 63) / 64;
-        argus_rds_symbols_kernel<<<blocks, 64, 0, st>>>(d_calls, b->d_states, d_outs, n_calls);
-        err = (int)hipGetLastError();
-    }
-    if (!err && max_cap >= kGroupBits) {
-        dim3 grid((unsigned)((max_cap + kThreads - 1) / kThreads), (unsigned)n_calls);
-        argus_rds_match_kernel<<<grid, kThreads, 0, st>>>(d_calls, d_outs, b->ft);
-        err = (int)hipGetLastError();
-    }
-    if (!err) {
-        argus_rds_scan_kernel<<<(n_calls + 63) / 64, 64, 0, st>>>(d_calls, b->d_states, d_outs, n_calls, b->ft);
-        err = (int)hipGetLastError();
-    }
+    int err = (int)hipMemcpyAsync(d, calls.data(), (size_t)n_calls * sizeof(ArgusRdsCall), hipMemcpyHostToDevice, st);
+    if (!err) err = argus_launch_rds((const ArgusRdsCall*)d, n_calls, max_cap, st);
     if (!err)
-        err = (int)hipMemcpyAsync(outs.data(), d_outs, (size_t)n_calls * sizeof(RdsOut), hipMemcpyDeviceToHost, st);
+        err = (int)hipMemcpyAsync(outs.data(), d_outs, (size_t)n_calls * sizeof(ArgusRdsOut), hipMemcpyDeviceToHost,
+                                  st);
     if (!err && group_total)
         err = (int)hipMemcpyAsync(groups.data(), d + off_groups, group_total * sizeof(uint16_t),
-                                  hipMemcpyDeviceToHost, st);
-    if (!err)
-        err = (int)hipMemcpyAsync(b->h_states.data(), b->d_states, (size_t)b->n_stations * sizeof(RdsState),
                                   hipMemcpyDeviceToHost, st);
     hipFreeAsync(d, st);
     const hipError_t se = hipStreamSynchronize(st);
@@ -494,8 +511,9 @@ GPUD_API int gpud_rds_batch_decode_device(gpud_rds_batch* b, int n_calls, const 
     if (err) return err;
     go = 0;
     for (int c = 0; c < n_calls; c++) {
-        const RdsOut& o = outs[(size_t)c];
+        const ArgusRdsOut& o = outs[(size_t)c];
         found[c] = o.found;
+        b->last[(size_t)stations[c]] = o;
         auto& lg = b->log[(size_t)stations[c]];
         lg.insert(lg.end(), groups.begin() + (ptrdiff_t)go, groups.begin() + (ptrdiff_t)(go + 4 * (size_t)o.written));
         go += (size_t)calls[(size_t)c].group_cap * 4;
@@ -506,16 +524,10 @@ GPUD_API int gpud_rds_batch_decode_device(gpud_rds_batch* b, int n_calls, const 
 GPUD_API int gpud_rds_batch_state(gpud_rds_batch* b, int station, uint16_t* pi, char* ps9, char* rt65)
 {
     if (!b || station < 0 || station >= b->n_stations) return -1;
-    const RdsState& s = b->h_states[(size_t)station];
+    const ArgusRdsOut& s = b->last[(size_t)station];
     if (pi) *pi = (uint16_t)s.pi;
-    auto text = [](const char* src, int len, char* dst) {  // rds.go:604-636
-        int end = len;
-        while (end > 0 && (src[end - 1] == ' ' || src[end - 1] == 0)) end--;
-        for (int i = 0; i < end; i++) dst[i] = src[i] ? src[i] : ' ';
-        dst[end] = 0;
-    };
-    if (ps9) text(s.ps, 8, ps9);
-    if (rt65) text(s.rt, 64, rt65);
+    if (ps9) argus_rds_text(s.ps, 8, ps9);
+    if (rt65) argus_rds_text(s.rt, 64, rt65);
     return 0;
 }
 

@@ -118,7 +118,9 @@ static void release_channel(ServeChannel& c)
     if (c.d_carry) hipFree(c.d_carry);
     if (c.d_wfm) hipFree(c.d_wfm);
     if (c.d_aud) hipFree(c.d_aud);
+    if (c.d_rdec) hipFree(c.d_rdec);
     c.d_aud = nullptr;
+    c.d_rdec = nullptr;
     c.d_taps = nullptr;
     c.d_ctaps = nullptr;
     c.d_hist[0] = c.d_hist[1] = nullptr;
@@ -179,6 +181,14 @@ static void slot_release(ServeSlot& sl)
     hipHostFree(sl.h_aout);
     sl.d_aout = sl.h_aout = nullptr;
     sl.aout_cap = sl.h_aout_cap = 0;
+    hipHostFree(sl.h_rtab);
+    hipFree(sl.d_rtab);
+    hipFree(sl.d_rwork);
+    hipHostFree(sl.h_rgroups);
+    sl.h_rtab = sl.d_rtab = sl.d_rwork = nullptr;
+    sl.h_rgroups = nullptr;
+    sl.rtab_cap = sl.rwork_cap = sl.h_rgroups_cap = 0;
+    sl.n_rdec = 0;
     for (auto& ge : sl.gexec)
         if (ge) hipGraphExecDestroy(ge);
     sl.d_iq = nullptr;
@@ -199,11 +209,24 @@ static void slot_release(ServeSlot& sl)
     sl.tab_cap = 0;
 }
 
+// The RDS decodes of a completed block become their channels' latest state.
+static void rds_collect(gpud_serve* s, const ServeSlot& sl)
+{
+    const ArgusRdsOut* outs = (const ArgusRdsOut*)(sl.h_rtab + sl.off_rout);
+    for (int i = 0; i < sl.n_rdec; i++) {
+        auto it = s->live.find(sl.rdec_channel[(size_t)i]);
+        if (it == s->live.end() || it->second.r_last_ticket >= sl.ticket) continue;
+        it->second.r_last = outs[i];
+        it->second.r_last_ticket = sl.ticket;
+    }
+}
+
 static int slot_wait(gpud_serve* s, ServeSlot& sl)
 {
     if (!sl.busy) return 0;
     ARGUS_CHECK(hipEventSynchronize(sl.ev_done));
     sl.busy = false;
+    rds_collect(s, sl);
     if (sl.ticket > s->completed_ticket) s->completed_ticket = sl.ticket;
     reap(s);
     return 0;
@@ -455,6 +478,8 @@ static void layout_block(const gpud_serve* s, int n, BlockLayout& L)
         o.a_tnext = c.at0;
         o.a_rate = c.audio_rate;
         o.aout_off = 0;
+        o.r_on = c.rds_win > 0 ? 1 : 0;
+        o.r_index = -1;
         if (c.stereo) {
             o.n_mono = argus_demod_count(ARGUS_DEMOD_WFM, o.n_iq, c.have_carry);
             o.n_audio = 2 * o.n_mono;
@@ -782,6 +807,85 @@ static int launch_block(gpud_serve* s, ServeSlot& sl)
     return (int)hipGraphLaunch(sl.gexec[1], s->compute);
 }
 
+// Device-side RDS decoding of a launched block, on the compute stream after
+// its graphs: the block's RDS baseband of every decoding channel goes into
+// the channel's decode windows (one append launch), and the windows this
+// block fills are decoded (gpud_rds_batch's kernels).  Advances the
+// channels' window state, as commit_block does for the rest.
+static int rds_stage(gpud_serve* s, ServeSlot& sl)
+{
+    sl.n_rdec = 0;
+    sl.rdec_channel.clear();
+    sl.rdec_groups.clear();
+    auto& ap = s->rapp;
+    auto& calls = s->rcalls;
+    ap.clear();
+    calls.clear();
+    size_t bits_total = 0, group_words = 0;
+    int max_cap = 0;
+    int ci = 0;
+    for (auto& kv : s->live) {
+        ServeChannel& c = kv.second;
+        ServeOut& o = sl.outs[(size_t)ci++];
+        o.r_index = -1;
+        if (!o.r_on || c.rds_win <= 0 || o.n_rds <= 0) continue;
+        const int W = c.rds_win;  // >= the most RDS samples of one block
+        float2* win0 = (float2*)(c.d_rdec + sizeof(ArgusRdsState));
+        float2* cur = win0 + (size_t)c.rwin_sel * W;
+        float2* nxt = win0 + (size_t)(c.rwin_sel ^ 1) * W;
+        const int n0 = std::min(o.n_rds, W - c.rwin_fill);
+        ap.push_back({sl.d_rds + o.rds_off, cur + c.rwin_fill, nxt, n0, o.n_rds - n0});
+        c.rwin_fill += n0;
+        if (c.rwin_fill < W) continue;
+        ArgusRdsCall k{};
+        k.x = cur;
+        k.st = (ArgusRdsState*)c.d_rdec;
+        k.n = W;
+        k.rate = o.rds_rate;
+        argus_rds_caps(W, o.rds_rate, &k.bit_cap, &k.group_cap);
+        o.r_index = (int)calls.size();
+        calls.push_back(k);
+        sl.rdec_channel.push_back(kv.first);
+        sl.rdec_groups.push_back(group_words);
+        bits_total += align_up((size_t)k.bit_cap);
+        group_words += 4 * (size_t)k.group_cap;
+        max_cap = std::max(max_cap, k.bit_cap);
+        c.rwin_sel ^= 1;
+        c.rwin_fill = o.n_rds - n0;
+    }
+    if (ap.empty()) return 0;
+    const size_t off_calls = align_up(ap.size() * sizeof(ArgusRdsAppend));
+    const size_t off_outs = off_calls + align_up(calls.size() * sizeof(ArgusRdsCall));
+    const size_t tab = off_outs + align_up(std::max<size_t>(calls.size(), 1) * sizeof(ArgusRdsOut));
+    const size_t work = 2 * bits_total + group_words * sizeof(uint16_t);
+    ARGUS_CHECK((hipError_t)grow_dev(&sl.d_rtab, &sl.rtab_cap, tab));
+    ARGUS_CHECK((hipError_t)grow_host(&sl.h_rtab, &sl.h_rtab_cap, tab));
+    ARGUS_CHECK((hipError_t)grow_dev(&sl.d_rwork, &sl.rwork_cap, std::max<size_t>(work, 1)));
+    ARGUS_CHECK((hipError_t)grow_host(&sl.h_rgroups, &sl.h_rgroups_cap, std::max<size_t>(group_words, 1)));
+    ArgusRdsOut* d_outs = (ArgusRdsOut*)(sl.d_rtab + off_outs);
+    size_t bo = 0;
+    for (size_t i = 0; i < calls.size(); i++) {
+        ArgusRdsCall& k = calls[i];
+        k.bits = (uint8_t*)(sl.d_rwork + bo);
+        k.flags = (uint8_t*)(sl.d_rwork + bits_total + bo);
+        k.groups = (uint16_t*)(sl.d_rwork + 2 * bits_total) + sl.rdec_groups[i];
+        k.out = d_outs + i;
+        bo += align_up((size_t)k.bit_cap);
+    }
+    std::memcpy(sl.h_rtab, ap.data(), ap.size() * sizeof(ArgusRdsAppend));
+    if (!calls.empty()) std::memcpy(sl.h_rtab + off_calls, calls.data(), calls.size() * sizeof(ArgusRdsCall));
+    ARGUS_CHECK(hipMemcpyAsync(sl.d_rtab, sl.h_rtab, off_outs, hipMemcpyHostToDevice, s->compute));
+    int err = argus_launch_rds_append((const ArgusRdsAppend*)sl.d_rtab, (int)ap.size(), s->compute);
+    if (err) return err;
+    err = argus_launch_rds((const ArgusRdsCall*)(sl.d_rtab + off_calls), (int)calls.size(), max_cap, s->compute);
+    if (err) return err;
+    sl.n_rdec = (int)calls.size();
+    sl.off_rout = off_outs;
+    sl.rgroups_bytes = group_words * sizeof(uint16_t);
+    sl.rgroups_dev_off = 2 * bits_total;
+    return 0;
+}
+
 // Enqueue one block: upload (unless device-resident), extract, demodulate,
 // download.  Returns immediately; *ticket identifies the block.
 // Build and upload the next block's tables ahead, into the next slot, while
@@ -850,6 +954,8 @@ static int serve_submit(gpud_serve* s, const gpud_float2* iq, int n, bool on_dev
     ARGUS_CHECK(hipEventRecord(sl.ev_kstart, s->compute));
     err = launch_block(s, sl);
     if (err) return err;
+    err = rds_stage(s, sl);
+    if (err) return err;
     ARGUS_CHECK(hipEventRecord(sl.ev_kern, s->compute));
     commit_block(s, sl.outs, n, sl.ticket);
     // ---- copy stream: download ------------------------------------------
@@ -870,6 +976,12 @@ static int serve_submit(gpud_serve* s, const gpud_float2* iq, int n, bool on_dev
         if (sl.aout_total > 0)
             ARGUS_CHECK(hipMemcpyAsync(sl.h_aout, sl.d_aout, (size_t)sl.aout_total * sizeof(float),
                                        hipMemcpyDeviceToHost, s->copy));
+    }
+    if (sl.n_rdec > 0) {  // decoded RDS groups come back whether or not outputs do
+        ARGUS_CHECK(hipMemcpyAsync(sl.h_rtab + sl.off_rout, sl.d_rtab + sl.off_rout,
+                                   (size_t)sl.n_rdec * sizeof(ArgusRdsOut), hipMemcpyDeviceToHost, s->copy));
+        ARGUS_CHECK(hipMemcpyAsync(sl.h_rgroups, sl.d_rwork + sl.rgroups_dev_off, sl.rgroups_bytes,
+                                   hipMemcpyDeviceToHost, s->copy));
     }
     ARGUS_CHECK(hipEventRecord(sl.ev_done, s->copy));
     sl.busy = true;
@@ -1038,6 +1150,84 @@ GPUD_API int gpud_serve_set_audio(gpud_serve* s, int64_t channel_id, int audio_r
     s->bytes_in_use += fp - old_fp;
     s->epoch++;
     return 0;
+}
+
+// Device bytes of a channel's RDS decoding: state, two windows and its share
+// of the per-slot decode tables and work area.
+static size_t rds_footprint(const ServeChannel& c, int W)
+{
+    int bit_cap = 0, group_cap = 0;
+    argus_rds_caps(W, c.spec.out_rate / c.rds_decim, &bit_cap, &group_cap);
+    const size_t per_slot = 2 * align_up((size_t)bit_cap) + 8 * (size_t)group_cap + sizeof(ArgusRdsAppend) +
+                            sizeof(ArgusRdsCall) + sizeof(ArgusRdsOut);
+    return sizeof(ArgusRdsState) + 2 * (size_t)W * sizeof(float2) + per_slot * ARGUS_SERVE_SLOTS;
+}
+
+GPUD_API int gpud_serve_set_rds(gpud_serve* s, int64_t channel_id, int window_samples)
+{
+    if (!s || window_samples < 0) return -1;
+    auto it = s->live.find(channel_id);
+    if (it == s->live.end() || !it->second.stereo) return -1;
+    ServeChannel& c = it->second;
+    // the most RDS samples one block can carry, and a window that holds a group
+    const int per_block = (s->block_samples / c.decim + 2) / c.rds_decim + 2;
+    if (window_samples > 0 && window_samples < std::max(per_block, 104)) return -1;
+    if ((int64_t)window_samples * 2 * (int64_t)sizeof(float2) > ((int64_t)1 << 31)) return -1;
+    for (auto& sl : s->slots) {  // blocks in flight read and write the decoder's state
+        int err = slot_wait(s, sl);
+        if (err) return err;
+    }
+    const size_t old_fp = c.rds_win > 0 ? rds_footprint(c, c.rds_win) : 0;
+    const size_t fp = window_samples > 0 ? rds_footprint(c, window_samples) : 0;
+    if (s->bytes_in_use - old_fp + fp > s->budget) return GPUD_ERR_BUDGET;
+    if (c.d_rdec && window_samples != c.rds_win) {
+        hipFree(c.d_rdec);
+        c.d_rdec = nullptr;
+    }
+    if (window_samples > 0) {
+        if (!c.d_rdec)
+            ARGUS_CHECK(dev_alloc(&c.d_rdec, sizeof(ArgusRdsState) + 2 * (size_t)window_samples * sizeof(float2)));
+        const ArgusRdsState zero{};
+        ARGUS_CHECK(hipMemcpy(c.d_rdec, &zero, sizeof zero, hipMemcpyHostToDevice));
+    }
+    c.rds_win = window_samples;
+    c.rwin_sel = 0;
+    c.rwin_fill = 0;
+    c.r_last = ArgusRdsOut{};
+    c.r_last_ticket = s->last_ticket;
+    c.footprint += fp - old_fp;
+    s->bytes_in_use += fp - old_fp;
+    s->epoch++;
+    return 0;
+}
+
+GPUD_API int gpud_serve_channel_rds(gpud_serve* s, int64_t ticket, int64_t channel_id, int* decoded, int* found,
+                                    uint16_t* pi, char* ps9, char* rt65, const uint16_t** groups, int* n_groups)
+{
+    if (!s) return -1;
+    ServeSlot* sl = find_slot(s, ticket);
+    if (!sl || sl->busy) return -2;
+    for (const ServeOut& o : sl->outs)
+        if (o.channel_id == channel_id) {
+            if (!o.r_on) return -3;
+            ArgusRdsOut r{};
+            const bool dec = o.r_index >= 0;
+            if (dec) {
+                r = ((const ArgusRdsOut*)(sl->h_rtab + sl->off_rout))[o.r_index];
+            } else {
+                auto it = s->live.find(channel_id);
+                if (it != s->live.end()) r = it->second.r_last;
+            }
+            if (decoded) *decoded = dec ? 1 : 0;
+            if (found) *found = dec ? r.found : 0;
+            if (pi) *pi = (uint16_t)r.pi;
+            if (ps9) argus_rds_text(r.ps, 8, ps9);
+            if (rt65) argus_rds_text(r.rt, 64, rt65);
+            if (groups) *groups = dec ? sl->h_rgroups + sl->rdec_groups[(size_t)o.r_index] : nullptr;
+            if (n_groups) *n_groups = dec ? r.written : 0;
+            return 0;
+        }
+    return -1;
 }
 
 GPUD_API int gpud_serve_set_graph(gpud_serve* s, int enable)

@@ -4,6 +4,7 @@
 #include "demod.hpp"
 #include "wfm.hpp"
 #include "audio.hpp"
+#include "rds.hpp"
 #include <map>
 #include <tuple>
 #include <vector>
@@ -42,6 +43,15 @@ struct ServeChannel {
     int a_sel = 0;           // delay-line half read by the next block
     const float* d_arms = nullptr;  // shared per filter (gpud_serve::filters)
     char* d_aud = nullptr;
+    // device-side RDS group decoding (gpud_serve_set_rds): one device block of
+    // ArgusRdsState | two decode windows of rds_win samples; the block's RDS
+    // baseband fills window rwin_sel, a full window is decoded and the
+    // other one fills next
+    int rds_win = 0;         // 0: off
+    char* d_rdec = nullptr;
+    int rwin_sel = 0, rwin_fill = 0;
+    ArgusRdsOut r_last{};    // state after the latest completed decode
+    int64_t r_last_ticket = 0;
 };
 
 struct ServeOut {
@@ -55,6 +65,8 @@ struct ServeOut {
     int n_aout, a_channels, a_tnext;  // audio frames, channels per frame, next resampler phase
     int a_rate;                       // audio rate (0: no audio stage)
     int64_t aout_off;                 // audio output offset (floats)
+    int r_on;                         // RDS decoding on for the channel in this block
+    int r_index;                      // index of its decode among the block's, or -1
 };
 
 struct ServeSlot {
@@ -121,6 +133,22 @@ struct ServeSlot {
     int64_t aout_total = 0;
     int a_mode = 0;          // the described block runs the audio stage (1) or not (0)
     int g_audio = -1;        // a_mode the stage graph was captured for
+    // device-side RDS decoding of the block (after its graphs): a pinned
+    // table mirrored on the device, [appends | calls | outs], the decode
+    // work area [bits | flags | groups] and the pinned groups
+    char* h_rtab = nullptr;
+    char* d_rtab = nullptr;
+    size_t rtab_cap = 0, h_rtab_cap = 0;
+    char* d_rwork = nullptr;
+    size_t rwork_cap = 0;
+    uint16_t* h_rgroups = nullptr;
+    size_t h_rgroups_cap = 0;
+    int n_rdec = 0;
+    size_t off_rout = 0;                 // outs in the table
+    size_t rgroups_bytes = 0;
+    size_t rgroups_dev_off = 0;          // groups in the work area
+    std::vector<int64_t> rdec_channel;   // channel id per decode
+    std::vector<size_t> rdec_groups;     // first group word per decode
     std::vector<ServeOut> outs;
 };
 
@@ -181,4 +209,6 @@ struct gpud_serve {
     std::vector<ArgusWfmDesc> wdescs;
     std::vector<ArgusDemodItem> ditems;
     std::vector<ArgusWfmItem> witems;
+    std::vector<ArgusRdsAppend> rapp;
+    std::vector<ArgusRdsCall> rcalls;
 };

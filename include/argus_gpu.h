@@ -329,6 +329,27 @@ GPUD_API int gpud_serve_set_audio(gpud_serve* s, int64_t channel_id, int audio_r
 GPUD_API int gpud_serve_channel_audio(gpud_serve* s, int64_t ticket, int64_t channel_id, const float** audio,
                                       int* n_frames, int* channels, int* rate);
 
+/* Device-side RDS group decoding of a live GPUD_DEMOD_WFM_STEREO channel
+ * (the decode of pipeline_runtime.go:1132-1142 with rds.Decoder.Decode,
+ * rds.go:78-264, on the GPU): the channel's streaming RDS baseband fills
+ * windows of window_samples samples on the device, and each full window is
+ * decoded there by one Decode call of the channel's decoder (the reference
+ * decodes the latest 4 s every 4 s: window_samples = 4 * rds_rate).
+ * window_samples 0 turns decoding off; else it must hold a group (>= 104)
+ * and a whole block's RDS samples.  Resets the decoder and waits for blocks
+ * in flight; applies from the next submitted block.  0, -1 bad arguments,
+ * unknown or non-stereo channel, GPUD_ERR_BUDGET if it does not fit. */
+GPUD_API int gpud_serve_set_rds(gpud_serve* s, int64_t channel_id, int window_samples);
+/* RDS decoding of a channel in a completed block: *decoded = 1 if a window
+ * completed in this block and was decoded, with *found groups found and the
+ * *n_groups groups it applied (4 x 16-bit blocks each, pinned host memory,
+ * valid as gpud_serve_channel_output's; whether or not downloads are on).
+ * PI, PS (8 + NUL) and RT (64 + NUL) are the decoder's after this block's
+ * decode, or after the latest completed one when *decoded = 0.  -1 if the
+ * channel was not part of the block, -3 if its decoding was off. */
+GPUD_API int gpud_serve_channel_rds(gpud_serve* s, int64_t ticket, int64_t channel_id, int* decoded, int* found,
+                                    uint16_t* pi, char* ps9, char* rt65, const uint16_t** groups, int* n_groups);
+
 /* ------------------------------------------------------------------ */
 /* RDS group decoder (host): successor of rds.Decoder.Decode             */
 /* (internal/rds/rds.go:78-264, tryDecode 365-430, decodeGroupAt 432-473, */

@@ -93,3 +93,73 @@ def test_stereo_and_rds_match_oracle(serve_mod):
         gg = dg.Groups()
         np.testing.assert_array_equal(gg, np.asarray(dw.Groups(), np.uint16).reshape(-1, 4))
         assert len(gg) >= 4 and dg.PI == pi, (cid, len(gg), hex(dg.PI))
+
+
+def test_rds_decoded_on_device_in_the_service(serve_mod):
+    """Device-side RDS decoding inside the service: each window the service
+    decodes equals one Decode call of the oracle decoder on the same window
+    of the service's own RDS baseband (groups found, group bits, PI/PS/RT),
+    in the block where the window completes; the groups also equal the
+    oracle decoder's on the oracle chain's baseband in the same windows."""
+    S = serve_mod
+    from argus._lib import GpudError
+    seconds = 2.0
+    sizes = [7, BLOCK - 7, 0, 1234, BLOCK - 1234] + [BLOCK] * (int(seconds * FS) // BLOCK - 2)
+    st = _stations(seconds)
+    rng = np.random.default_rng(10)
+    s = S.Serve(FS, BLOCK, 1 << 30)
+    wins = {1: 9000, 2: 4000}  # station 3 leaves decoding off
+    orc = {}
+    for cid, off, *_ in STATIONS:
+        assert s.Open(S.ChannelSpec(cid, off, 200_000.0, 250_000, 101, S.DEMOD_WFM_STEREO, 0))
+        orc[cid] = OracleStereoChannel(FS, off, 200_000.0, 250_000, 101, 0)
+    assert s.Open(S.ChannelSpec(9, 0.0, 12_500.0, 50_000, 101, S.DEMOD_NFM, 0))
+    with pytest.raises(GpudError):
+        s.SetRDS(9, 4000)       # not a stereo channel
+    with pytest.raises(GpudError):
+        s.SetRDS(1, 100)        # shorter than a group
+    for cid, w in wins.items():
+        assert s.SetRDS(cid, w)
+    s.SetDownload(True)
+    pend = {cid: np.zeros(0, np.complex64) for cid in wins}
+    pend_w = {cid: np.zeros(0, np.complex64) for cid in wins}
+    dec_g = {cid: O.RDSDecoder() for cid in wins}   # on the service's baseband
+    dec_w = {cid: O.RDSDecoder() for cid in wins}   # on the oracle chain's baseband
+    n_dec = {cid: 0 for cid in wins}
+    for k, n in enumerate(sizes):
+        print(f"block {k} n={n}", flush=True)
+        blk = _mix(st, n, rng)
+        t = s.Submit(blk)
+        s.Wait(t)
+        assert s.OutputRDS(t, 3) is None and s.OutputRDS(t, 9) is None
+        for cid, oc in orc.items():
+            _, _, w_rds, _, _ = oc.block(blk)
+            if cid not in wins:
+                continue
+            g_rds, rate, _, _ = s.OutputWFM(t, cid)
+            pend[cid] = np.concatenate([pend[cid], g_rds])
+            pend_w[cid] = np.concatenate([pend_w[cid], w_rds])
+            decoded, found, groups, pi, ps, rt = s.OutputRDS(t, cid)
+            W = wins[cid]
+            want_dec = len(pend[cid]) >= W
+            assert decoded == want_dec, (cid, k, decoded, len(pend[cid]))
+            if not decoded:
+                assert found == 0 and groups.shape == (0, 4)
+                continue
+            before = len(dec_g[cid].Groups())
+            f = dec_g[cid].Decode(pend[cid][:W], rate)
+            dec_w[cid].Decode(pend_w[cid][:W], rate)
+            pend[cid] = pend[cid][W:]
+            pend_w[cid] = pend_w[cid][W:]
+            want = np.asarray(dec_g[cid].Groups(), np.uint16).reshape(-1, 4)[before:]
+            assert found == f, (cid, k, found, f)
+            np.testing.assert_array_equal(groups, want)
+            assert (pi, ps, rt) == (dec_g[cid].PI, dec_g[cid].PS, dec_g[cid].RT), (cid, k)
+            n_dec[cid] += 1
+    s.Close()
+    for cid, off, pi, ps, *_ in STATIONS:
+        if cid not in wins:
+            continue
+        gg = np.asarray(dec_g[cid].Groups(), np.uint16).reshape(-1, 4)
+        np.testing.assert_array_equal(gg, np.asarray(dec_w[cid].Groups(), np.uint16).reshape(-1, 4))
+        assert n_dec[cid] >= 3 and len(gg) >= 4 and dec_g[cid].PI == pi, (cid, n_dec[cid], len(gg))
