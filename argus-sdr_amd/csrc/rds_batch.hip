@@ -1,31 +1,45 @@
 // rds_batch.hip -- device-side RDS group decoder: one decoder per station,
-// many stations per call (gpud_rds_batch_*).  Same semantics as the host
-// decoder in rds.hip, which follows internal/rds/rds.go Decoder.Decode
+// many stations per call (gpud_rds_batch_*), and the kernels the streaming
+// service runs on each block (gpud_serve_set_rds).  Same semantics as the
+// host decoder in rds.hip, which follows internal/rds/rds.go Decoder.Decode
 // (78-264), tryDecode (365-430), decodeGroupAt (432-473), applyGroup
 // (475-502), decodeBlockExpected (549-579) and crcSyndrome (590-599).
 //
-// Three launches per call, all on the caller's stream:
+// Launches, all on the caller's stream:
 //
-//   argus_rds_symbols_kernel  one lane per call.  Mueller & Muller timing
-//       with linear interpolation, the Costas loop on each symbol as it is
-//       produced, sign(I) hard bits and the differential bits (carried across
-//       calls).  The loops are serial recurrences, so a lane per station is
-//       the parallelism there is.  Arithmetic follows the reference's
+//   argus_rds_symbols_kernel  one wavefront per call.  Mueller & Muller
+//       timing with linear interpolation, the Costas loop on each symbol as
+//       it is produced, sign(I) hard bits and the differential bits (carried
+//       across calls).  The loop is a serial recurrence, run by lane 0 over
+//       samples the wavefront stages in LDS 1024 at a time (the next sample
+//       index depends on the previous symbol, so a global load per symbol
+//       would put HBM latency on the recurrence).  A call may cover a window
+//       whose samples arrive over several launches: the loop pauses where it
+//       needs a sample not yet there and resumes from its saved registers
+//       (ArgusRdsProgress), which gives exactly the symbols of one call over
+//       the whole window.  Arithmetic follows the reference's
 //       float32/float64 mix operation by operation, with contraction off, so
-//       the bits equal the host decoder's.  The Costas state and the last
+//       the bits equal the host decoder's.  The Costas oscillator's FP64
+//       sine and cosine (fp64_sincos.hpp) round to the same float32 as the C
+//       library's on the wrapped phase.  The Costas state and the last
 //       hard bit are committed only when the call produced at least 104
 //       symbols, as the reference returns before its Costas loop otherwise.
-//   argus_rds_match_kernel    one thread per (call, bit position).  Whether a
-//       whole group (blocks A, B, C/C', D) starts at that position, in the
+//   argus_rds_match_kernel    one thread per (final call, bit position).
+//       Whether a whole group (blocks A, B, C/C', D) starts there, in the
 //       direct and in the inverted bit stream, with the unique-single-flip
 //       correction.  The CRC is linear over GF(2), so the syndrome of a word
 //       with bit i flipped is syndrome(word) ^ syndrome(1 << i): one division
 //       per block and 26 table lookups.
-//   argus_rds_scan_kernel     one lane per call.  The serial block-sync scan
-//       with the +-3-bit flywheel over the match flags (direct stream, then
-//       the inverted one if the direct found nothing); each group it takes is
-//       decoded again at its position and applied to the station's PI, PS and
-//       RT on the device, and written to the call's group list.
+//   argus_rds_next_kernel     one workgroup per final call: for every
+//       position, the next position at or after it where a group starts
+//       (a suffix minimum), per stream.
+//   argus_rds_scan_kernel     one lane per final call.  The serial block-sync
+//       scan with the +-3-bit flywheel (direct stream, then the inverted one
+//       if the direct found nothing).  Where the reference steps one bit at
+//       a time while it searches, the scan jumps to the next position with a
+//       group, which visits the same groups in the same order.  Each group
+//       taken is decoded again at its position and applied to the station's
+//       PI, PS and RT on the device, and written to the call's group list.
 //
 // Deviation: the symbol loop of a call stops after 2 n / sps + 16 symbols
 // (the reference has no bound; a symbol step shorter than half a symbol
@@ -33,6 +47,7 @@
 // the timing position below -1 ends the call's symbol loop, where the
 // reference would index before the window and panic.
 #include "rds.hpp"
+#include "fp64_sincos.hpp"
 
 #include <algorithm>
 #include <cstring>
@@ -119,94 +134,166 @@ __device__ char rds_printable(unsigned v) { return (v < 32 || v > 126) ? ' ' : (
 
 // ---------------------------------------------------------------- kernels
 
-__global__ __launch_bounds__(64) void argus_rds_symbols_kernel(const ArgusRdsCall* __restrict__ calls, int n_calls)
+constexpr int kStage = 1024;  // samples staged in LDS per pass of the symbol loop
+
+__global__ __launch_bounds__(64) void argus_rds_symbols_kernel(const ArgusRdsCall* __restrict__ calls)
 {
 #pragma clang fp contract(off)
-    const int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= n_calls) return;
-    const ArgusRdsCall call = calls[c];
+    __shared__ float2 xs[kStage];
+    __shared__ int next_stage;  // first sample of the next pass, or -1
+    // by value: the loop's byte stores to the bits could alias a reference
+    // into the table and force a reload per symbol
+    const ArgusRdsCall call = calls[blockIdx.x];
+    const int lane = threadIdx.x;
+    const int n = call.n, rate = call.rate, avail = call.avail, bit_cap = call.bit_cap;
+    ArgusRdsProgress pr{};
+    double sps = 0.0;
+    if (lane == 0) {
+        pr = *call.pr;
+        ArgusRdsState& st = *call.st;
+        next_stage = -1;
+        if (n >= kGroupBits && rate > 0) {  // rds.go:79-81
+            sps = (double)rate / 1187.5;
+            if (!pr.active) {  // the call starts: rds.go:84-110
+                if (st.last_rate != 0 && st.last_rate != rate) {
+                    st.costas_phase = st.costas_freq = 0;
+                    st.mu = 0;
+                    st.prev_re = st.prev_im = st.prev_dec = 0;
+                    st.have_prev = 0;
+                    st.last_hard = 0;
+                    st.have_last_hard = 0;
+                }
+                st.last_rate = rate;
+                double m = st.mu;
+                if (m < 0 || m >= sps || isnan(m) || isinf(m)) m = sps / 2;
+                pr.m = m;
+                pr.prev_re = st.prev_re;
+                pr.prev_im = st.prev_im;
+                pr.prev_dec = st.prev_dec;
+                pr.have_prev = st.have_prev;
+                pr.phase = st.costas_phase;
+                pr.freq = st.costas_freq;
+                pr.ns = pr.nb = pr.prev_hard = 0;
+                pr.loop_done = 0;
+                pr.active = 1;
+            }
+            if (!pr.loop_done && pr.m < (double)(n - 1)) {
+                const int k = (int)pr.m;
+                if (k >= 0 && k + 1 < avail) next_stage = k;
+            }
+        }
+    }
+    __syncthreads();
+    const double alpha = 0.132, beta = alpha * alpha / 4.0;
+    while (next_stage >= 0) {
+        const int k0 = next_stage;
+        const int cnt = min(kStage, avail - k0);
+        for (int i = lane; i < cnt; i += 64) xs[i] = call.x[k0 + i];
+        __syncthreads();
+        if (lane == 0) {
+            const ArgusRdsState& st = *call.st;
+            const bool carry = st.have_last_hard != 0;
+            const int carried = st.last_hard;
+            uint8_t* bits = call.bits;
+            double m = pr.m, phase = pr.phase, freq = pr.freq;
+            float prev_re = pr.prev_re, prev_im = pr.prev_im, prev_dec = pr.prev_dec;
+            bool have_prev = pr.have_prev != 0;
+            int ns = pr.ns, nb = pr.nb, prev_hard = pr.prev_hard;
+            int restage = -1;
+            bool done = true;  // the loop condition ended the call
+            while (m < (double)(n - 1)) {
+                const int k = (int)m;
+                const double f = m - (double)k;
+                if (k + 1 >= n || k < 0 || ns >= bit_cap) break;
+                if (k + 1 >= avail) {  // wait for the window's next samples
+                    done = false;
+                    break;
+                }
+                if (k < k0 || k + 1 >= k0 + cnt) {  // outside this pass
+                    restage = k;
+                    done = false;
+                    break;
+                }
+                const float2 a = xs[k - k0], b = xs[k + 1 - k0];
+                const float re = (float)((double)a.x * (1 - f) + (double)b.x * f);
+                const float im = (float)((double)a.y * (1 - f) + (double)b.y * f);
+                const float dec = re >= 0 ? 1.0f : -1.0f;
+                if (have_prev) {
+                    const double e =
+                        (double)(dec - prev_dec) * (double)prev_re - (double)(re - prev_re) * (double)prev_dec;
+                    m += sps + 0.01 * e;
+                } else {
+                    m += sps;
+                    have_prev = true;
+                }
+                prev_dec = dec;
+                prev_re = re;
+                prev_im = im;
+                // Costas de-rotation of this symbol (rds.go:157-183)
+                double sd, cd;
+                argus_sincos_wrapped(phase, &sd, &cd);  // phase stays in [-pi, pi]
+                const float cp = (float)cd, sp = (float)sd;
+                const float sre = re * cp + im * sp;
+                const float sim = im * cp - re * sp;
+                const double err = sre >= 0 ? (double)sim : -(double)sim;
+                freq += beta * err;
+                phase += freq + alpha * err;
+                while (phase > M_PI) phase -= 2 * M_PI;
+                while (phase < -M_PI) phase += 2 * M_PI;
+                const int hard = sre >= 0 ? 1 : 0;
+                // differential bits, carried across calls (rds.go:214-233)
+                if (ns == 0) {
+                    if (carry) bits[nb++] = (uint8_t)(hard ^ carried);
+                } else {
+                    bits[nb++] = (uint8_t)(hard ^ prev_hard);
+                }
+                prev_hard = hard;
+                ns++;
+            }
+            pr.m = m;
+            pr.phase = phase;
+            pr.freq = freq;
+            pr.prev_re = prev_re;
+            pr.prev_im = prev_im;
+            pr.prev_dec = prev_dec;
+            pr.have_prev = have_prev ? 1 : 0;
+            pr.ns = ns;
+            pr.nb = nb;
+            pr.prev_hard = prev_hard;
+            if (done) pr.loop_done = 1;
+            next_stage = restage;
+        }
+        __syncthreads();
+    }
+    if (lane != 0) return;
+    ArgusRdsState& st = *call.st;
     ArgusRdsOut& out = *call.out;
+    if (!call.final) {  // more of the window comes with later launches
+        *call.pr = pr;
+        return;
+    }
     out.found = 0;
     out.written = 0;
     out.nb = -1;
-    const int n = call.n, rate = call.rate;
-    ArgusRdsState& st = *call.st;
-    if (n >= kGroupBits && rate > 0) {  // rds.go:79-81
-        if (st.last_rate != 0 && st.last_rate != rate) {  // rds.go:84-94
-            st.costas_phase = st.costas_freq = 0;
-            st.mu = 0;
-            st.prev_re = st.prev_im = st.prev_dec = 0;
-            st.have_prev = 0;
-            st.last_hard = 0;
-            st.have_last_hard = 0;
-        }
-        st.last_rate = rate;
-        const double sps = (double)rate / 1187.5;
-        double m = st.mu;
-        if (m < 0 || m >= sps || isnan(m) || isinf(m)) m = sps / 2;
-        float prev_re = st.prev_re, prev_im = st.prev_im, prev_dec = st.prev_dec;
-        bool have_prev = st.have_prev != 0;
-        double phase = st.costas_phase, freq = st.costas_freq;
-        const double alpha = 0.132, beta = alpha * alpha / 4.0;
-        const bool carry = st.have_last_hard != 0;
-        const int carried = st.last_hard;
-        const float2* x = call.x;
-        uint8_t* bits = call.bits;
-        int ns = 0, nb = 0, prev_hard = 0;
-        while (m < (double)(n - 1)) {
-            const int k = (int)m;
-            const double f = m - (double)k;
-            if (k + 1 >= n || k < 0 || ns >= call.bit_cap) break;
-            const float2 a = x[k], b = x[k + 1];
-            const float re = (float)((double)a.x * (1 - f) + (double)b.x * f);
-            const float im = (float)((double)a.y * (1 - f) + (double)b.y * f);
-            const float dec = re >= 0 ? 1.0f : -1.0f;
-            if (have_prev) {
-                const double e =
-                    (double)(dec - prev_dec) * (double)prev_re - (double)(re - prev_re) * (double)prev_dec;
-                m += sps + 0.01 * e;
-            } else {
-                m += sps;
-                have_prev = true;
-            }
-            prev_dec = dec;
-            prev_re = re;
-            prev_im = im;
-            // Costas de-rotation of this symbol (rds.go:157-183)
-            const float cp = (float)cos(phase), sp = (float)sin(phase);
-            const float sre = re * cp + im * sp;
-            const float sim = im * cp - re * sp;
-            const double err = sre >= 0 ? (double)sim : -(double)sim;
-            freq += beta * err;
-            phase += freq + alpha * err;
-            while (phase > M_PI) phase -= 2 * M_PI;
-            while (phase < -M_PI) phase += 2 * M_PI;
-            const int hard = sre >= 0 ? 1 : 0;
-            // differential bits, carried across calls (rds.go:214-233)
-            if (ns == 0) {
-                if (carry) bits[nb++] = (uint8_t)(hard ^ carried);
-            } else {
-                bits[nb++] = (uint8_t)(hard ^ prev_hard);
-            }
-            prev_hard = hard;
-            ns++;
-        }
-        double res = m - (double)n;
+    if (n >= kGroupBits && rate > 0) {  // rds.go:136-146: the call's end
+        double res = pr.m - (double)n;
         while (res < 0) res += sps;
         while (res >= sps) res -= sps;
         st.mu = res;
-        st.prev_re = prev_re;
-        st.prev_im = prev_im;
-        st.prev_dec = prev_dec;
-        st.have_prev = have_prev ? 1 : 0;
-        if (ns >= kGroupBits) {  // else rds.go:143-146: Costas and bit carry untouched
-            st.costas_phase = phase;
-            st.costas_freq = freq;
-            st.last_hard = prev_hard;
+        st.prev_re = pr.prev_re;
+        st.prev_im = pr.prev_im;
+        st.prev_dec = pr.prev_dec;
+        st.have_prev = pr.have_prev;
+        if (pr.ns >= kGroupBits) {  // else Costas and bit carry untouched
+            st.costas_phase = pr.phase;
+            st.costas_freq = pr.freq;
+            st.last_hard = pr.prev_hard;
             st.have_last_hard = 1;
-            out.nb = nb;
+            out.nb = pr.nb;
         }
     }
+    pr.active = 0;
+    *call.pr = pr;
     // the decoder's text as it stands (the scan kernel updates it if it runs)
     out.pi = st.pi;
     for (int i = 0; i < 8; i++) out.ps[i] = st.ps[i];
@@ -226,9 +313,52 @@ __global__ __launch_bounds__(kThreads) void argus_rds_match_kernel(const ArgusRd
     call.flags[p] = f;
 }
 
+// next[2p + pol] = the first q >= p with a group in stream pol, or nb.
+__global__ __launch_bounds__(kThreads) void argus_rds_next_kernel(const ArgusRdsCall* __restrict__ calls)
+{
+    __shared__ int first[2][kThreads];
+    const ArgusRdsCall& call = calls[blockIdx.x];
+    const int nb = call.out->nb;
+    if (nb < kGroupBits) return;
+    const int npos = nb - kGroupBits + 1;
+    const int seg = (npos + kThreads - 1) / kThreads;
+    const int t = threadIdx.x;
+    const int lo = min(npos, t * seg), hi = min(npos, lo + seg);
+    int nd = nb, ni = nb;
+    for (int p = hi - 1; p >= lo; p--) {
+        const uint8_t f = call.flags[p];
+        if (f & 1) nd = p;
+        if (f & 2) ni = p;
+    }
+    first[0][t] = nd;
+    first[1][t] = ni;
+    __syncthreads();
+    for (int d = 1; d < kThreads; d <<= 1) {  // suffix minimum over the segments
+        int a = nb, b = nb;
+        if (t + d < kThreads) {
+            a = first[0][t + d];
+            b = first[1][t + d];
+        }
+        __syncthreads();
+        first[0][t] = min(first[0][t], a);
+        first[1][t] = min(first[1][t], b);
+        __syncthreads();
+    }
+    nd = t + 1 < kThreads ? first[0][t + 1] : nb;
+    ni = t + 1 < kThreads ? first[1][t + 1] : nb;
+    for (int p = hi - 1; p >= lo; p--) {
+        const uint8_t f = call.flags[p];
+        if (f & 1) nd = p;
+        if (f & 2) ni = p;
+        call.next[2 * p] = nd;
+        call.next[2 * p + 1] = ni;
+    }
+}
+
 struct ScanCtx {
     const uint8_t* bits;
     const uint8_t* flags;
+    const int32_t* next;
     int nb;
     uint32_t inv;
     int pol;
@@ -275,8 +405,8 @@ __device__ int scan_stream(const ScanCtx& s, const FlipTable& ft, ArgusRdsState&
 {
     int groups = 0;
     for (int i = 0; i + kGroupBits <= s.nb;) {
-        if (!scan_hit(s, i)) {
-            i++;
+        if (!scan_hit(s, i)) {  // the reference steps i++ until a group starts
+            i = s.next[2 * i + s.pol];
             continue;
         }
         groups++;
@@ -320,7 +450,7 @@ __global__ __launch_bounds__(64) void argus_rds_scan_kernel(const ArgusRdsCall* 
     ArgusRdsOut& out = *call.out;
     if (out.nb < kGroupBits) return;
     ArgusRdsState& st = *call.st;
-    ScanCtx s{call.bits, call.flags, out.nb, 0u, 0};
+    ScanCtx s{call.bits, call.flags, call.next, out.nb, 0u, 0};
     int found = scan_stream(s, ft, st, call, out);
     if (found == 0) {  // rds.go:242-250: the inverted stream
         s.inv = 0x3FFFFFFu;
@@ -364,19 +494,20 @@ void argus_rds_caps(int n, int rate, int* bit_cap, int* group_cap)
     *group_cap = gc;
 }
 
-int argus_launch_rds(const ArgusRdsCall* d_calls, int n_calls, int max_bit_cap, hipStream_t stream)
+int argus_launch_rds(const ArgusRdsCall* d_calls, int n_calls, int n_final, int max_bit_cap, hipStream_t stream)
 {
     if (n_calls <= 0) return 0;
-    if (n_calls > 65535) return -1;
+    if (n_calls > 65535 || n_final < 0 || n_final > n_calls) return -1;
     argus_clear_error();
-    argus_rds_symbols_kernel<<<(n_calls + 63) / 64, 64, 0, stream>>>(d_calls, n_calls);
+    argus_rds_symbols_kernel<<<n_calls, 64, 0, stream>>>(d_calls);
     ARGUS_CHECK(hipGetLastError());
-    if (max_bit_cap >= kGroupBits) {
-        dim3 grid((unsigned)((max_bit_cap + kThreads - 1) / kThreads), (unsigned)n_calls);
-        argus_rds_match_kernel<<<grid, kThreads, 0, stream>>>(d_calls, kFlips);
-        ARGUS_CHECK(hipGetLastError());
-    }
-    argus_rds_scan_kernel<<<(n_calls + 63) / 64, 64, 0, stream>>>(d_calls, n_calls, kFlips);
+    if (n_final == 0 || max_bit_cap < kGroupBits) return 0;
+    dim3 grid((unsigned)((max_bit_cap + kThreads - 1) / kThreads), (unsigned)n_final);
+    argus_rds_match_kernel<<<grid, kThreads, 0, stream>>>(d_calls, kFlips);
+    ARGUS_CHECK(hipGetLastError());
+    argus_rds_next_kernel<<<n_final, kThreads, 0, stream>>>(d_calls);
+    ARGUS_CHECK(hipGetLastError());
+    argus_rds_scan_kernel<<<(n_final + 63) / 64, 64, 0, stream>>>(d_calls, n_final, kFlips);
     return (int)hipGetLastError();
 }
 
@@ -462,23 +593,28 @@ GPUD_API int gpud_rds_batch_decode_device(gpud_rds_batch* b, int n_calls, const 
     if (offsets[n_calls] > offsets[0] && !samples_dev) return -1;
     hipStream_t st = (hipStream_t)stream;
 
-    // Per-call workspace: [calls | outs | bits | flags | groups].
+    // Per-call workspace: [calls | progress | outs | bits | flags | next | groups];
+    // calls and (zero) progress are uploaded.
     std::vector<ArgusRdsCall> calls((size_t)n_calls);
     size_t bit_total = 0, group_total = 0;
     int max_cap = 0;
     for (int c = 0; c < n_calls; c++) {
         ArgusRdsCall& k = calls[(size_t)c];
         k.n = (int)(offsets[c + 1] - offsets[c]);
+        k.avail = k.n;
+        k.final = 1;
         k.rate = rates[c];
         argus_rds_caps(k.n, k.rate, &k.bit_cap, &k.group_cap);
         bit_total += align256((size_t)k.bit_cap);
         group_total += (size_t)k.group_cap * 4;
         max_cap = std::max(max_cap, k.bit_cap);
     }
-    const size_t off_outs = align256((size_t)n_calls * sizeof(ArgusRdsCall));
+    const size_t off_prog = align256((size_t)n_calls * sizeof(ArgusRdsCall));
+    const size_t off_outs = off_prog + align256((size_t)n_calls * sizeof(ArgusRdsProgress));
     const size_t off_bits = off_outs + align256((size_t)n_calls * sizeof(ArgusRdsOut));
     const size_t off_flags = off_bits + bit_total;
-    const size_t off_groups = off_flags + bit_total;
+    const size_t off_next = off_flags + bit_total;
+    const size_t off_groups = off_next + 8 * bit_total;
     const size_t bytes = off_groups + group_total * sizeof(uint16_t);
     char* d = nullptr;
     ARGUS_CHECK(hipMallocAsync((void**)&d, bytes, st));
@@ -489,16 +625,20 @@ GPUD_API int gpud_rds_batch_decode_device(gpud_rds_batch* b, int n_calls, const 
         k.x = (const float2*)(samples_dev + offsets[c]);
         k.bits = (uint8_t*)(d + off_bits + bo);
         k.flags = (uint8_t*)(d + off_flags + bo);
+        k.next = (int32_t*)(d + off_next + 8 * bo);
         k.groups = (uint16_t*)(d + off_groups) + go;
         k.st = b->d_states + stations[c];
+        k.pr = (ArgusRdsProgress*)(d + off_prog) + c;
         k.out = d_outs + c;
         bo += align256((size_t)k.bit_cap);
         go += (size_t)k.group_cap * 4;
     }
+    std::vector<char> head(off_outs, 0);
+    std::memcpy(head.data(), calls.data(), (size_t)n_calls * sizeof(ArgusRdsCall));
     std::vector<ArgusRdsOut> outs((size_t)n_calls);
     std::vector<uint16_t> groups(group_total);
-    int err = (int)hipMemcpyAsync(d, calls.data(), (size_t)n_calls * sizeof(ArgusRdsCall), hipMemcpyHostToDevice, st);
-    if (!err) err = argus_launch_rds((const ArgusRdsCall*)d, n_calls, max_cap, st);
+    int err = (int)hipMemcpyAsync(d, head.data(), off_outs, hipMemcpyHostToDevice, st);
+    if (!err) err = argus_launch_rds((const ArgusRdsCall*)d, n_calls, n_calls, max_cap, st);
     if (!err)
         err = (int)hipMemcpyAsync(outs.data(), d_outs, (size_t)n_calls * sizeof(ArgusRdsOut), hipMemcpyDeviceToHost,
                                   st);

@@ -183,11 +183,8 @@ static void slot_release(ServeSlot& sl)
     sl.aout_cap = sl.h_aout_cap = 0;
     hipHostFree(sl.h_rtab);
     hipFree(sl.d_rtab);
-    hipFree(sl.d_rwork);
-    hipHostFree(sl.h_rgroups);
-    sl.h_rtab = sl.d_rtab = sl.d_rwork = nullptr;
-    sl.h_rgroups = nullptr;
-    sl.rtab_cap = sl.rwork_cap = sl.h_rgroups_cap = 0;
+    sl.h_rtab = sl.d_rtab = nullptr;
+    sl.rtab_cap = sl.h_rtab_cap = 0;
     sl.n_rdec = 0;
     for (auto& ge : sl.gexec)
         if (ge) hipGraphExecDestroy(ge);
@@ -349,6 +346,7 @@ GPUD_API int gpud_serve_close(gpud_serve* s, int64_t channel_id)
     if (!s) return -1;
     auto it = s->live.find(channel_id);
     if (it == s->live.end()) return -1;
+    if (it->second.rds_win > 0) s->n_rds_on--;
     s->closed.push_back(it->second);
     s->live.erase(it);
     s->epoch++;
@@ -807,21 +805,46 @@ static int launch_block(gpud_serve* s, ServeSlot& sl)
     return (int)hipGraphLaunch(sl.gexec[1], s->compute);
 }
 
+// A channel's RDS decoding block on the device (ServeChannel::d_rdec):
+// decoder state | call progress | bits | match flags | next-hit table |
+// two windows of W baseband samples.
+struct RdsLayout {
+    int bit_cap, group_cap;
+    size_t off_pr, off_bits, off_flags, off_next, off_win, bytes;
+};
+
+static RdsLayout rds_layout(const ServeChannel& c, int W)
+{
+    RdsLayout l{};
+    argus_rds_caps(W, c.spec.out_rate / c.rds_decim, &l.bit_cap, &l.group_cap);
+    l.off_pr = align_up(sizeof(ArgusRdsState));
+    l.off_bits = l.off_pr + align_up(sizeof(ArgusRdsProgress));
+    l.off_flags = l.off_bits + align_up((size_t)l.bit_cap);
+    l.off_next = l.off_flags + align_up((size_t)l.bit_cap);
+    l.off_win = l.off_next + align_up(8 * (size_t)l.bit_cap);
+    l.bytes = l.off_win + 2 * (size_t)W * sizeof(float2);
+    return l;
+}
+
 // Device-side RDS decoding of a launched block, on the compute stream after
-// its graphs: the block's RDS baseband of every decoding channel goes into
-// the channel's decode windows (one append launch), and the windows this
-// block fills are decoded (gpud_rds_batch's kernels).  Advances the
-// channels' window state, as commit_block does for the rest.
+// its graphs.  The block's RDS baseband of every decoding channel goes into
+// the channel's current window (one append launch); the symbol loop of each
+// channel's Decode call advances over what its window holds (the calls run
+// across the blocks that fill the window), and the calls whose window this
+// block completes finish and run their group scan.  Advances the channels'
+// window state, as commit_block does for the rest.
 static int rds_stage(gpud_serve* s, ServeSlot& sl)
 {
     sl.n_rdec = 0;
     sl.rdec_channel.clear();
     sl.rdec_groups.clear();
+    if (s->n_rds_on == 0) return 0;
     auto& ap = s->rapp;
     auto& calls = s->rcalls;
     ap.clear();
     calls.clear();
-    size_t bits_total = 0, group_words = 0;
+    std::vector<ArgusRdsCall> streaming;
+    size_t group_words = 0;
     int max_cap = 0;
     int ci = 0;
     for (auto& kv : s->live) {
@@ -830,59 +853,69 @@ static int rds_stage(gpud_serve* s, ServeSlot& sl)
         o.r_index = -1;
         if (!o.r_on || c.rds_win <= 0 || o.n_rds <= 0) continue;
         const int W = c.rds_win;  // >= the most RDS samples of one block
-        float2* win0 = (float2*)(c.d_rdec + sizeof(ArgusRdsState));
+        const RdsLayout l = rds_layout(c, W);
+        float2* win0 = (float2*)(c.d_rdec + l.off_win);
         float2* cur = win0 + (size_t)c.rwin_sel * W;
         float2* nxt = win0 + (size_t)(c.rwin_sel ^ 1) * W;
         const int n0 = std::min(o.n_rds, W - c.rwin_fill);
         ap.push_back({sl.d_rds + o.rds_off, cur + c.rwin_fill, nxt, n0, o.n_rds - n0});
         c.rwin_fill += n0;
-        if (c.rwin_fill < W) continue;
         ArgusRdsCall k{};
         k.x = cur;
+        k.bits = (uint8_t*)(c.d_rdec + l.off_bits);
+        k.flags = (uint8_t*)(c.d_rdec + l.off_flags);
+        k.next = (int32_t*)(c.d_rdec + l.off_next);
         k.st = (ArgusRdsState*)c.d_rdec;
+        k.pr = (ArgusRdsProgress*)(c.d_rdec + l.off_pr);
         k.n = W;
+        k.avail = c.rwin_fill;
         k.rate = o.rds_rate;
-        argus_rds_caps(W, o.rds_rate, &k.bit_cap, &k.group_cap);
+        k.bit_cap = l.bit_cap;
+        k.group_cap = l.group_cap;
+        k.final = c.rwin_fill == W ? 1 : 0;
+        if (!k.final) {
+            streaming.push_back(k);
+            continue;
+        }
         o.r_index = (int)calls.size();
         calls.push_back(k);
         sl.rdec_channel.push_back(kv.first);
         sl.rdec_groups.push_back(group_words);
-        bits_total += align_up((size_t)k.bit_cap);
-        group_words += 4 * (size_t)k.group_cap;
-        max_cap = std::max(max_cap, k.bit_cap);
+        group_words += 4 * (size_t)l.group_cap;
+        max_cap = std::max(max_cap, l.bit_cap);
         c.rwin_sel ^= 1;
         c.rwin_fill = o.n_rds - n0;
     }
     if (ap.empty()) return 0;
+    const int n_final = (int)calls.size();
+    calls.insert(calls.end(), streaming.begin(), streaming.end());
+    // table: appends | calls (final ones first) | outs | groups
     const size_t off_calls = align_up(ap.size() * sizeof(ArgusRdsAppend));
     const size_t off_outs = off_calls + align_up(calls.size() * sizeof(ArgusRdsCall));
-    const size_t tab = off_outs + align_up(std::max<size_t>(calls.size(), 1) * sizeof(ArgusRdsOut));
-    const size_t work = 2 * bits_total + group_words * sizeof(uint16_t);
+    const size_t off_groups = off_outs + align_up(std::max(n_final, 1) * sizeof(ArgusRdsOut));
+    const size_t tab = off_groups + std::max<size_t>(group_words, 1) * sizeof(uint16_t);
     ARGUS_CHECK((hipError_t)grow_dev(&sl.d_rtab, &sl.rtab_cap, tab));
     ARGUS_CHECK((hipError_t)grow_host(&sl.h_rtab, &sl.h_rtab_cap, tab));
-    ARGUS_CHECK((hipError_t)grow_dev(&sl.d_rwork, &sl.rwork_cap, std::max<size_t>(work, 1)));
-    ARGUS_CHECK((hipError_t)grow_host(&sl.h_rgroups, &sl.h_rgroups_cap, std::max<size_t>(group_words, 1)));
     ArgusRdsOut* d_outs = (ArgusRdsOut*)(sl.d_rtab + off_outs);
-    size_t bo = 0;
+    uint16_t* d_groups = (uint16_t*)(sl.d_rtab + off_groups);
     for (size_t i = 0; i < calls.size(); i++) {
         ArgusRdsCall& k = calls[i];
-        k.bits = (uint8_t*)(sl.d_rwork + bo);
-        k.flags = (uint8_t*)(sl.d_rwork + bits_total + bo);
-        k.groups = (uint16_t*)(sl.d_rwork + 2 * bits_total) + sl.rdec_groups[i];
-        k.out = d_outs + i;
-        bo += align_up((size_t)k.bit_cap);
+        const bool fin = (int)i < n_final;
+        k.out = d_outs + (fin ? i : 0);  // a non-final call writes no result
+        k.groups = fin ? d_groups + sl.rdec_groups[i] : nullptr;
     }
     std::memcpy(sl.h_rtab, ap.data(), ap.size() * sizeof(ArgusRdsAppend));
-    if (!calls.empty()) std::memcpy(sl.h_rtab + off_calls, calls.data(), calls.size() * sizeof(ArgusRdsCall));
+    std::memcpy(sl.h_rtab + off_calls, calls.data(), calls.size() * sizeof(ArgusRdsCall));
     ARGUS_CHECK(hipMemcpyAsync(sl.d_rtab, sl.h_rtab, off_outs, hipMemcpyHostToDevice, s->compute));
     int err = argus_launch_rds_append((const ArgusRdsAppend*)sl.d_rtab, (int)ap.size(), s->compute);
     if (err) return err;
-    err = argus_launch_rds((const ArgusRdsCall*)(sl.d_rtab + off_calls), (int)calls.size(), max_cap, s->compute);
+    err = argus_launch_rds((const ArgusRdsCall*)(sl.d_rtab + off_calls), (int)calls.size(), n_final, max_cap,
+                           s->compute);
     if (err) return err;
-    sl.n_rdec = (int)calls.size();
+    sl.n_rdec = n_final;
     sl.off_rout = off_outs;
-    sl.rgroups_bytes = group_words * sizeof(uint16_t);
-    sl.rgroups_dev_off = 2 * bits_total;
+    sl.off_rgroups = off_groups;
+    sl.rgroups_bytes = tab - off_outs;  // outs and groups come back in one copy
     return 0;
 }
 
@@ -977,12 +1010,9 @@ static int serve_submit(gpud_serve* s, const gpud_float2* iq, int n, bool on_dev
             ARGUS_CHECK(hipMemcpyAsync(sl.h_aout, sl.d_aout, (size_t)sl.aout_total * sizeof(float),
                                        hipMemcpyDeviceToHost, s->copy));
     }
-    if (sl.n_rdec > 0) {  // decoded RDS groups come back whether or not outputs do
-        ARGUS_CHECK(hipMemcpyAsync(sl.h_rtab + sl.off_rout, sl.d_rtab + sl.off_rout,
-                                   (size_t)sl.n_rdec * sizeof(ArgusRdsOut), hipMemcpyDeviceToHost, s->copy));
-        ARGUS_CHECK(hipMemcpyAsync(sl.h_rgroups, sl.d_rwork + sl.rgroups_dev_off, sl.rgroups_bytes,
+    if (sl.n_rdec > 0)  // decoded RDS groups come back whether or not outputs do
+        ARGUS_CHECK(hipMemcpyAsync(sl.h_rtab + sl.off_rout, sl.d_rtab + sl.off_rout, sl.rgroups_bytes,
                                    hipMemcpyDeviceToHost, s->copy));
-    }
     ARGUS_CHECK(hipEventRecord(sl.ev_done, s->copy));
     sl.busy = true;
     if (ticket) *ticket = sl.ticket;
@@ -1152,15 +1182,14 @@ GPUD_API int gpud_serve_set_audio(gpud_serve* s, int64_t channel_id, int audio_r
     return 0;
 }
 
-// Device bytes of a channel's RDS decoding: state, two windows and its share
-// of the per-slot decode tables and work area.
+// Device bytes of a channel's RDS decoding: its block and its share of the
+// per-slot decode tables.
 static size_t rds_footprint(const ServeChannel& c, int W)
 {
-    int bit_cap = 0, group_cap = 0;
-    argus_rds_caps(W, c.spec.out_rate / c.rds_decim, &bit_cap, &group_cap);
-    const size_t per_slot = 2 * align_up((size_t)bit_cap) + 8 * (size_t)group_cap + sizeof(ArgusRdsAppend) +
-                            sizeof(ArgusRdsCall) + sizeof(ArgusRdsOut);
-    return sizeof(ArgusRdsState) + 2 * (size_t)W * sizeof(float2) + per_slot * ARGUS_SERVE_SLOTS;
+    const RdsLayout l = rds_layout(c, W);
+    const size_t per_slot =
+        sizeof(ArgusRdsAppend) + sizeof(ArgusRdsCall) + sizeof(ArgusRdsOut) + 8 * (size_t)l.group_cap;
+    return l.bytes + per_slot * ARGUS_SERVE_SLOTS;
 }
 
 GPUD_API int gpud_serve_set_rds(gpud_serve* s, int64_t channel_id, int window_samples)
@@ -1184,12 +1213,13 @@ GPUD_API int gpud_serve_set_rds(gpud_serve* s, int64_t channel_id, int window_sa
         hipFree(c.d_rdec);
         c.d_rdec = nullptr;
     }
-    if (window_samples > 0) {
-        if (!c.d_rdec)
-            ARGUS_CHECK(dev_alloc(&c.d_rdec, sizeof(ArgusRdsState) + 2 * (size_t)window_samples * sizeof(float2)));
-        const ArgusRdsState zero{};
-        ARGUS_CHECK(hipMemcpy(c.d_rdec, &zero, sizeof zero, hipMemcpyHostToDevice));
+    if (window_samples > 0) {  // a fresh decoder: zero state and progress
+        const RdsLayout l = rds_layout(c, window_samples);
+        if (!c.d_rdec) ARGUS_CHECK(dev_alloc(&c.d_rdec, l.bytes));
+        const std::vector<char> zero(l.off_bits, 0);
+        ARGUS_CHECK(hipMemcpy(c.d_rdec, zero.data(), zero.size(), hipMemcpyHostToDevice));
     }
+    s->n_rds_on += (window_samples > 0 ? 1 : 0) - (c.rds_win > 0 ? 1 : 0);
     c.rds_win = window_samples;
     c.rwin_sel = 0;
     c.rwin_fill = 0;
@@ -1223,7 +1253,9 @@ GPUD_API int gpud_serve_channel_rds(gpud_serve* s, int64_t ticket, int64_t chann
             if (pi) *pi = (uint16_t)r.pi;
             if (ps9) argus_rds_text(r.ps, 8, ps9);
             if (rt65) argus_rds_text(r.rt, 64, rt65);
-            if (groups) *groups = dec ? sl->h_rgroups + sl->rdec_groups[(size_t)o.r_index] : nullptr;
+            if (groups)
+                *groups = dec ? (const uint16_t*)(sl->h_rtab + sl->off_rgroups) + sl->rdec_groups[(size_t)o.r_index]
+                              : nullptr;
             if (n_groups) *n_groups = dec ? r.written : 0;
             return 0;
         }

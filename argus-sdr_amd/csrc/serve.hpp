@@ -134,19 +134,15 @@ struct ServeSlot {
     int a_mode = 0;          // the described block runs the audio stage (1) or not (0)
     int g_audio = -1;        // a_mode the stage graph was captured for
     // device-side RDS decoding of the block (after its graphs): a pinned
-    // table mirrored on the device, [appends | calls | outs], the decode
-    // work area [bits | flags | groups] and the pinned groups
+    // table mirrored on the device, [appends | calls | outs | groups], the
+    // outs and groups of the block's completed decodes coming back
     char* h_rtab = nullptr;
     char* d_rtab = nullptr;
     size_t rtab_cap = 0, h_rtab_cap = 0;
-    char* d_rwork = nullptr;
-    size_t rwork_cap = 0;
-    uint16_t* h_rgroups = nullptr;
-    size_t h_rgroups_cap = 0;
-    int n_rdec = 0;
+    int n_rdec = 0;                      // decodes completed in the block
     size_t off_rout = 0;                 // outs in the table
-    size_t rgroups_bytes = 0;
-    size_t rgroups_dev_off = 0;          // groups in the work area
+    size_t off_rgroups = 0;          // groups in the table
+    size_t rgroups_bytes = 0;            // outs + groups
     std::vector<int64_t> rdec_channel;   // channel id per decode
     std::vector<size_t> rdec_groups;     // first group word per decode
     std::vector<ServeOut> outs;
@@ -209,6 +205,7 @@ struct gpud_serve {
     std::vector<ArgusWfmDesc> wdescs;
     std::vector<ArgusDemodItem> ditems;
     std::vector<ArgusWfmItem> witems;
+    int n_rds_on = 0;  // live channels with RDS decoding on
     std::vector<ArgusRdsAppend> rapp;
     std::vector<ArgusRdsCall> rcalls;
 };

@@ -427,8 +427,10 @@ def bench_wfm_stereo_20m(a, lib, S):
     """20 Msps span, 20 stereo WFM stations with RDS (pilot 19 kHz, L-R at
     38 kHz, RDS at 57 kHz with seeded PI/PS/RT groups, SNR 15-45 dB); per
     station: shift, /80 to 250 ksps, FM discriminator, pilot PLL stereo
-    decoder, RDS baseband; RDS groups decoded on the host from the downloaded
-    baseband.  Blocks are generated on the fly and submitted pipelined."""
+    decoder, RDS baseband, and RDS group decoding on the device in 0.5 s
+    windows.  Blocks are generated on the fly and submitted pipelined.  The
+    host decoder, run on the downloaded baseband in the same windows, checks
+    the device's groups."""
     from argus.synth import StereoScene, rds_group_bits
     from argus.wfm import RDSDecoder
     rng = np.random.default_rng(5)
@@ -442,21 +444,32 @@ def bench_wfm_stereo_20m(a, lib, S):
     print(f"[bench]   generating {n_blocks} blocks", file=sys.stderr, flush=True)
     blocks = [scene.block(BLOCK) for _ in range(n_blocks)]
     s = S.Serve(FS, BLOCK, 1 << 30)
+    rds_win = (250_000 // 13) // 2
     for k in range(n_st):
         assert s.Open(S.ChannelSpec(k + 1, offs[k], 200_000.0, 250_000, 101, S.DEMOD_WFM_STEREO, 0))
+        assert s.SetRDS(k + 1, rds_win)
     s.SetDownload(True)
     rds = {k: [] for k in range(n_st)}
+    dev_groups = {k: [] for k in range(n_st)}
+    dev_pi = [0] * n_st
     stereo = np.zeros(n_st, np.int64)
-    gpu, kern = [], []
+    gpu, kern, decodes = [], [], []
 
     def collect(t):
         info = s.Wait(t)
         gpu.append(info.gpu_ms)
         kern.append(info.kernel_ms)
+        nd = 0
         for k in range(n_st):
             x, rate, _, st = s.OutputWFM(t, k + 1)
             rds[k].append(x)
             stereo[k] += st
+            dec, _, g, pi, _, _ = s.OutputRDS(t, k + 1)
+            if dec:
+                nd += 1
+                dev_groups[k].append(g)
+                dev_pi[k] = pi
+        decodes.append(nd)
         return rate
 
     pending = []
@@ -470,21 +483,94 @@ def bench_wfm_stereo_20m(a, lib, S):
         rate = collect(t)
     wall = time.perf_counter() - t0
     s.Close()
-    groups, pi_ok = 0, 0
+    groups, pi_ok, same = 0, 0, True
     for k in range(n_st):
         d = RDSDecoder()
         x = np.concatenate(rds[k])
-        for i in range(0, len(x), rate // 2):
-            d.Decode(x[i:i + rate // 2], rate)
-        groups += len(d.Groups())
-        pi_ok += int(d.PI == pis[k])
+        for i in range(0, len(x) - rds_win + 1, rds_win):  # the windows the device decoded
+            d.Decode(x[i:i + rds_win], rate)
+        g = np.concatenate(dev_groups[k]) if dev_groups[k] else np.zeros((0, 4), np.uint16)
+        same = same and np.array_equal(g, d.Groups())
+        groups += len(g)
+        pi_ok += int(dev_pi[k] == pis[k])
     ms_block = wall * 1e3 / n_blocks
+    kd = [kk for kk, nd in zip(kern, decodes) if nd > 0]
+    kn = [kk for kk, nd in zip(kern, decodes) if nd == 0]
     return {"stations": n_st, "blocks": n_blocks, "block_ms_host_to_host": round(ms_block, 3),
             "p50_gpu_ms": round(percentile(gpu, 50), 3), "p99_gpu_ms": round(percentile(gpu, 99), 3),
             "p50_kernels_ms": round(percentile(kern, 50), 3), "p99_kernels_ms": round(percentile(kern, 99), 3),
             "realtime_factor": round(BLOCK_S * 1e3 / ms_block, 1),
             "stereo_blocks_per_station": round(float(stereo.mean()), 1),
-            "rds_groups_decoded": groups, "rds_pi_correct_stations": pi_ok}
+            "rds_window_samples": rds_win, "rds_decode_blocks": len(kd),
+            "p50_kernels_ms_decode_blocks": round(percentile(kd, 50), 3) if kd else None,
+            "p50_kernels_ms_other_blocks": round(percentile(kn, 50), 3) if kn else None,
+            "rds_groups_decoded_device": groups, "rds_pi_correct_stations": pi_ok,
+            "rds_device_equals_host_decoder": bool(same)}
+
+
+def rds_windows(n_unique, rate, seconds, seed=7):
+    """Synthetic complex RDS baseband windows at `rate`: differentially
+    encoded NRZ of seeded groups at 1187.5 bps, a random carrier phase and a
+    0.3 Hz residual frequency offset, AWGN at 15 dB SNR."""
+    from argus.synth import rds_group_bits, rds_symbols
+    rng = np.random.default_rng(seed)
+    n = int(seconds * rate)
+    out = []
+    for k in range(n_unique):
+        bits = rds_group_bits(0x2000 + k, f"RDS{k:03d}  ", f"device decoded radiotext {k}",
+                              int(seconds * 1187.5 / 104) + 2)
+        sym = rds_symbols(bits)
+        t = np.arange(n) / rate
+        nrz = sym[np.minimum((t * 1187.5 + rng.uniform()).astype(np.int64), len(sym) - 1)]
+        ph = rng.uniform(0, 2 * np.pi) + 2 * np.pi * 0.3 * t
+        x = nrz * np.exp(1j * ph)
+        sigma = np.sqrt(10 ** (-15 / 10) / 2)
+        x = x + sigma * (rng.standard_normal(n) + 1j * rng.standard_normal(n))
+        out.append(x.astype(np.complex64))
+    return out
+
+
+def bench_rds_decode(a, lib):
+    """Device-side RDS group decoding of many stations per call (one 4 s
+    window each, as the reference decodes per station every 4 s) against the
+    host decoder of the same library on one core."""
+    from argus.surveillance import DeviceBuffer
+    from argus.wfm import RDSBatchDecoder, RDSDecoder
+    rate = 250_000 // 13
+    seconds = 4.0
+    uniq = rds_windows(8, rate, seconds)
+    n_st = a.rds_stations
+    n = len(uniq[0])
+    flat = np.concatenate([uniq[i % len(uniq)] for i in range(n_st)])
+    buf = DeviceBuffer(flat.nbytes).upload(flat)
+    offs = np.arange(n_st + 1, dtype=np.int64) * n
+    rates = np.full(n_st, rate, np.int32)
+    dev = RDSBatchDecoder(n_st)
+    found = dev.DecodeDevice(np.arange(n_st), buf.ptr, offs, rates)  # warm-up: first window of every station
+    times = []
+    for _ in range(3):
+        t0 = time.perf_counter()
+        dev.DecodeDevice(np.arange(n_st), buf.ptr, offs, rates)
+        times.append(time.perf_counter() - t0)
+    dev_ms = 1e3 * min(times)
+    # the host decoder on the first windows of the unique stations
+    host_found, t_host = [], 0.0
+    for x in uniq:
+        d = RDSDecoder()
+        t0 = time.perf_counter()
+        host_found.append(d.Decode(x, rate))
+        t_host += time.perf_counter() - t0
+    host_ms = 1e3 * t_host / len(uniq)
+    same = all(found[i] == host_found[i] for i in range(len(uniq)))
+    buf.free()
+    return {"stations_per_call": n_st, "window_s": seconds, "rds_rate": rate,
+            "device_ms_per_call": round(dev_ms, 3),
+            "device_station_windows_per_s": round(n_st / (dev_ms * 1e-3), 1),
+            "device_realtime_stations": int(n_st * seconds / (dev_ms * 1e-3)),
+            "host_ms_per_station_window_1core": round(host_ms, 3),
+            "host_realtime_stations_per_core": int(seconds / (host_ms * 1e-3)),
+            "groups_first_window": int(sum(found[:len(uniq)])),
+            "device_equals_host_on_first_windows": bool(same)}
 
 
 def cpu_baseline_extract(chans, block, threads):
@@ -512,7 +598,11 @@ def main() -> int:
     ap.add_argument("--plain-cap", type=int, default=512)
     ap.add_argument("--cpu-channels", type=int, default=32)
     ap.add_argument("--surv-frames", type=int, default=500)
-    ap.add_argument("--stereo-blocks", type=int, default=30)
+    ap.add_argument("--stereo-blocks", type=int, default=60)
+    ap.add_argument("--rds-stations", type=int, default=1024,
+                    help="stations per device RDS decode call in the RDS decoding figure (0: skip it)")
+    ap.add_argument("--rds-only", action="store_true",
+                    help="run only the device RDS decoding figure and print it")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extras", action="store_true",
                     help="skip surveillance, the 2.048 Msps case and the stereo multiplex case")
@@ -538,6 +628,10 @@ def main() -> int:
     if not G.Available():
         print("no HIP device", file=sys.stderr)
         return 1
+
+    if a.rds_only:
+        print(json.dumps({"rds_decode_device": bench_rds_decode(a, L)}), flush=True)
+        return 0
 
     blocks = [make_block(s) for s in range(4)]
     if a.calibrate_copy_mb > 0:
@@ -594,6 +688,10 @@ def main() -> int:
         extras["wfm_2048ksps_4stations"], _, _ = bench_wfm_2048k(a, L, S)
         print("[bench] 20 Msps stereo WFM + RDS, 20 stations", file=sys.stderr, flush=True)
         extras["wfm_stereo_rds_20msps_20stations"] = bench_wfm_stereo_20m(a, L, S)
+        if a.rds_stations > 0:
+            print(f"[bench] device RDS decoding, {a.rds_stations} stations x 4 s windows", file=sys.stderr,
+                  flush=True)
+            extras["rds_decode_device"] = bench_rds_decode(a, L)
 
     cpu = None
     if not a.no_cpu:
