@@ -351,6 +351,54 @@ GPUD_API int gpud_serve_channel_rds(gpud_serve* s, int64_t ticket, int64_t chann
                                     uint16_t* pi, char* ps9, char* rt65, const uint16_t** groups, int* n_groups);
 
 /* ------------------------------------------------------------------ */
+/* SDRIQ001 capture files: successors of iqfile.Write / Read / Source    */
+/* (internal/iqfile/iqfile.go:5-166).  Little-endian "SDRIQ001", int64   */
+/* sample rate, float64 centre Hz, int64 count, complex64 samples.       */
+/* ------------------------------------------------------------------ */
+typedef struct gpud_iqfile gpud_iqfile;
+/* iqfile.Write (iqfile.go:37-63).  -2 cannot create, -3 write failed. */
+GPUD_API int gpud_iqfile_write(const char* path, const gpud_float2* iq, int64_t n, int64_t sample_rate,
+                               double center_hz);
+/* Open a capture for replay and report its header (iqfile.Read's header,
+ * iqfile.go:66-90).  Samples are read from the file as they are asked for,
+ * never loaded whole.  -2 cannot open, -3 bad magic or header, -4 file
+ * shorter than its declared samples. */
+GPUD_API int gpud_iqfile_open(const char* path, gpud_iqfile** out, int64_t* sample_rate, double* center_hz,
+                              int64_t* n_samples);
+/* Source.ReadIQ (iqfile.go:121-142): n samples from the current position,
+ * looping to the start at the end; an empty capture gives zeros. */
+GPUD_API int gpud_iqfile_read(gpud_iqfile* f, gpud_float2* dst, int n);
+GPUD_API int gpud_iqfile_seek(gpud_iqfile* f, int64_t pos);
+GPUD_API int gpud_iqfile_close(gpud_iqfile* f);
+
+/* ------------------------------------------------------------------ */
+/* Ingest: a ring of depth pinned host blocks and their device copies,   */
+/* filled by a producer ahead of the consumer with asynchronous H2D      */
+/* copies on the ring's own stream (the step before the path:            */
+/* captureSpectrum's srcMgr.ReadIQ, pipeline_runtime.go:484-556).  No     */
+/* reference counterpart as an API: the reference hands host slices to   */
+/* each GPU call, which copies them synchronously.                       */
+/* ------------------------------------------------------------------ */
+typedef struct gpud_ingest gpud_ingest;
+/* block_samples per slot, 2 <= depth <= 64 slots, on the current device. */
+GPUD_API int gpud_ingest_create(int block_samples, int depth, gpud_ingest** out);
+GPUD_API int gpud_ingest_destroy(gpud_ingest* g);
+/* Replay a capture (looping, unpaced) from a producer thread; the file must
+ * stay open until the ingest is destroyed. */
+GPUD_API int gpud_ingest_start_file(gpud_ingest* g, gpud_iqfile* f);
+/* Push one block of n <= block_samples samples from the caller's source;
+ * waits while every slot is queued or acquired (so the caller must release
+ * what it acquires).  Not with a replaying capture. */
+GPUD_API int gpud_ingest_push(gpud_ingest* g, const gpud_float2* iq, int n);
+/* The next block in order, resident on the device (*dev valid until its
+ * release), with its length and sequence number; waits for it and for its
+ * copy.  timeout_ms < 0 waits without bound; returns 1 if it timed out. */
+GPUD_API int gpud_ingest_acquire(gpud_ingest* g, const gpud_float2** dev, int* n, int64_t* seq, int timeout_ms);
+/* Give a block's slot back once the work that reads it has completed. */
+GPUD_API int gpud_ingest_release(gpud_ingest* g, int64_t seq);
+GPUD_API int gpud_ingest_stats(gpud_ingest* g, int64_t* produced, int64_t* consumed, int* queued);
+
+/* ------------------------------------------------------------------ */
 /* RDS group decoder (host): successor of rds.Decoder.Decode             */
 /* (internal/rds/rds.go:78-264, tryDecode 365-430, decodeGroupAt 432-473, */
 /* applyGroup 475-502, decodeBlockExpected 549-579, crcSyndrome 590-599): */
