@@ -47,6 +47,7 @@ register("gpud_surv_fft_device", I, [P, P, L64, I, I, P])
 register("gpud_surv_spectrum_device", I, [P, P, L64, I, I, P])
 register("gpud_surv_cfar_runs_device", I, [P, P, I, C.POINTER(gpud_cfar_params), P, P, I])
 register("gpud_surv_finish_frame", I, [P, I, I, P, I, C.POINTER(gpud_detector_params), P, I])
+register("gpud_surv_finish_device", I, [P, P, I, P, P, I, C.POINTER(gpud_detector_params), P, I, P])
 register("gpud_surv_detect", I, [P, P, L64, I, I, I, C.POINTER(gpud_cfar_params), C.POINTER(gpud_detector_params),
                                  P, I, P, P])
 
@@ -186,6 +187,42 @@ class Surveillance:
         for f in range(nf):
             out.append([raw[f * max_runs + i] for i in range(min(int(counts[f]), max_runs))])
         return counts, out
+
+    def DetectSpectra(self, psd, cfar, det, cap=1024, max_runs=2048):
+        """CFAR runs and the device detection list (expansion, merge,
+        centroid) of given dB spectra [frames][n]: (lists, signal counts)."""
+        psd = np.ascontiguousarray(psd, np.float32).reshape(-1, self.n)
+        nf = psd.shape[0]
+        dpsd = DeviceBuffer(psd.nbytes).upload(psd)
+        druns = DeviceBuffer(C.sizeof(gpud_surv_run) * max_runs * nf)
+        dcnt = DeviceBuffer(4 * nf)
+        dsig = DeviceBuffer(C.sizeof(gpud_surv_signal) * cap * nf)
+        dout = DeviceBuffer(4 * nf)
+        check("gpud_surv_cfar_runs_device", lib().gpud_surv_cfar_runs_device(self._h, dpsd.ptr, nf, C.byref(cfar),
+                                                                             druns.ptr, dcnt.ptr, max_runs))
+        check("gpud_surv_finish_device", lib().gpud_surv_finish_device(self._h, dpsd.ptr, nf, druns.ptr, dcnt.ptr,
+                                                                       max_runs, C.byref(det), dsig.ptr, cap,
+                                                                       dout.ptr))
+        self.sync()
+        counts = dout.download(np.int32, nf)
+        raw = (gpud_surv_signal * (cap * nf))()
+        check("gpud_memcpy_d2h", lib().gpud_memcpy_d2h(C.addressof(raw), dsig.ptr, C.sizeof(raw)))
+        lists = [[Signal(x.first_bin, x.last_bin, x.peak_bin, x.center_hz, x.bw_hz, x.peak_db, x.snr_db, x.noise_db)
+                  for x in raw[f * cap:f * cap + min(int(counts[f]), cap)]] for f in range(nf)]
+        return lists, counts
+
+    def FinishFrameHost(self, psd_row, runs, det, cap=1024):
+        """The host detection list of one frame from its CFAR runs
+        (gpud_surv_finish_frame): (list, signal count)."""
+        row = np.ascontiguousarray(psd_row, np.float32)
+        arr = (gpud_surv_run * max(len(runs), 1))(*runs)
+        out = (gpud_surv_signal * cap)()
+        m = lib().gpud_surv_finish_frame(row.ctypes.data, len(row), self.sample_rate, arr, len(runs), C.byref(det),
+                                         out, cap)
+        if m < 0:
+            check("gpud_surv_finish_frame", m)
+        return [Signal(x.first_bin, x.last_bin, x.peak_bin, x.center_hz, x.bw_hz, x.peak_db, x.snr_db, x.noise_db)
+                for x in out[:min(m, cap)]], m
 
     def Detect(self, iq, n_segs, seg_hop, frame_step, n_frames, cfar, det, cap=1024, want_psd=True):
         iq = np.ascontiguousarray(iq, np.complex64)

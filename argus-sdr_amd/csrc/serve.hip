@@ -959,13 +959,17 @@ static int serve_submit(gpud_serve* s, const gpud_float2* iq, int n, bool on_dev
     sl.n = n;
     const float2* in = on_device ? (const float2*)iq : sl.d_in;
     // tables prepared while the previous block ran, if nothing they depend on
-    // has changed since; else build them now
+    // has changed since; else build them now.  Only the header holds the
+    // input pointer, so a block from another device buffer (a ring of
+    // ingest slots) keeps the prepared tables and re-sends the header.
     ServePrep& pp = s->prep;
     const bool prepared =
-        pp.valid && pp.slot == idx && pp.ticket == sl.ticket && pp.epoch == s->epoch && pp.n == n && pp.in == in;
+        pp.valid && pp.slot == idx && pp.ticket == sl.ticket && pp.epoch == s->epoch && pp.n == n;
+    const bool repoint = prepared && pp.in != in;
     pp.valid = false;
     if (prepared) {
         slot_adopt(sl, pp.layout);
+        if (repoint) ((ArgusServeHdr*)(sl.h_tab + sl.off_hdr))->x.in = in;
     } else {
         err = build_block(s, sl, n, in, true);
         if (err) return err;
@@ -983,6 +987,9 @@ static int serve_submit(gpud_serve* s, const gpud_float2* iq, int n, bool on_dev
     if (!prepared) {  // a prepared table was uploaded when it was built
         err = upload_table(s, sl);
         if (err) return err;
+    } else if (repoint) {
+        ARGUS_CHECK(hipMemcpyAsync(sl.d_tab + sl.off_hdr, sl.h_tab + sl.off_hdr, sizeof(ArgusServeHdr),
+                                   hipMemcpyHostToDevice, s->compute));
     }
     ARGUS_CHECK(hipEventRecord(sl.ev_kstart, s->compute));
     err = launch_block(s, sl);

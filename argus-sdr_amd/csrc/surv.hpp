@@ -2,6 +2,9 @@
 #pragma once
 #include "common.hpp"
 
+// Most CFAR runs per frame the detection list takes (device finish).
+#define ARGUS_DETECT_MAX_RUNS 2048
+
 struct gpud_surv {
     int n = 0;             // FFT size
     int sample_rate = 0;

@@ -566,10 +566,21 @@ GPUD_API int gpud_surv_cfar_runs_device(gpud_surv* s, const float* psd_dev, int 
  * count (<= cap) or -1. */
 GPUD_API int gpud_surv_finish_frame(const float* psd, int n_bins, int sample_rate, const gpud_surv_run* runs,
                                     int n_runs, const gpud_detector_params* params, gpud_surv_signal* out, int cap);
+/* The same for n_frames frames on the device (one workgroup per frame):
+ * psd_dev [n_frames][n_fft] and the CFAR runs (runs_dev [n_frames][max_runs],
+ * run_counts_dev) of gpud_surv_cfar_runs_device in; signals_dev
+ * [n_frames][cap_per_frame] and counts_dev[n_frames] (the signal count,
+ * which may exceed cap_per_frame) out.  max_runs <= 2048.  Queued on the
+ * engine's stream.  Bins identical to the host version; centres to the
+ * last bits of pow(). */
+GPUD_API int gpud_surv_finish_device(gpud_surv* s, const float* psd_dev, int n_frames, const gpud_surv_run* runs_dev,
+                                     const int* run_counts_dev, int max_runs, const gpud_detector_params* params,
+                                     gpud_surv_signal* signals_dev, int cap_per_frame, int* counts_dev);
 /* Whole detection list for n_frames frames of device-resident IQ: Welch,
- * CFAR, runs, expansion.  signals: n_frames * cap_per_frame records; counts
- * per frame.  psd_host (optional, n_frames * n_fft floats) receives the
- * spectra. */
+ * CFAR, runs, expansion, merge and centroids, all on the device; only the
+ * lists (and the spectra if asked) come back.  signals: n_frames *
+ * cap_per_frame records; counts per frame.  psd_host (optional, n_frames *
+ * n_fft floats) receives the spectra. */
 GPUD_API int gpud_surv_detect(gpud_surv* s, const gpud_float2* iq_dev, int64_t frame_step, int seg_hop, int n_segs,
                               int n_frames, const gpud_cfar_params* cfar, const gpud_detector_params* det,
                               gpud_surv_signal* signals, int cap_per_frame, int* counts, float* psd_host);

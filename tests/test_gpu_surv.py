@@ -137,3 +137,63 @@ def test_spectra_sanitise_non_finite_input(surv):
     psd = s.WelchPSD(iq2, 2, hop, 2 * hop, 1)
     assert np.all(np.isfinite(psd)) and np.all(psd[0] == -200.0)
     s.Close()
+
+
+def _compare_finish(s, psd, cfar, det, cap):
+    dev_lists, dev_counts = s.DetectSpectra(psd, cfar, det, cap=cap)
+    counts, runs = s.CFARRuns(psd, cfar)
+    for f in range(psd.shape[0]):
+        want, m = s.FinishFrameHost(psd[f], runs[f], det, cap=cap)
+        print(f"frame {f}: {len(runs[f])} runs -> device {dev_counts[f]}, host {m} signals", flush=True)
+        assert dev_counts[f] == m, (f, dev_counts[f], m)
+        assert len(dev_lists[f]) == len(want)
+        for g, w in zip(dev_lists[f], want):
+            assert (g.FirstBin, g.LastBin, g.PeakBin) == (w.FirstBin, w.LastBin, w.PeakBin), (f, g, w)
+            assert (g.PeakDb, g.SNRDb, g.NoiseDb, g.BWHz) == (w.PeakDb, w.SNRDb, w.NoiseDb, w.BWHz), (f, g, w)
+            assert abs(g.CenterHz - w.CenterHz) <= 1e-6, (f, g.CenterHz, w.CenterHz)
+
+
+def test_device_detection_list_equals_host_on_a_scene(surv):
+    """gpud_surv_finish_device against the host detection list
+    (gpud_surv_finish_frame) on the same Welch spectra and CFAR runs."""
+    from argus.synth import surveillance_scene
+    n, hop, segs, frames = 32768, 16384, 8, 3
+    total = (frames - 1) * segs * hop + (segs - 1) * hop + n
+    iq, _ = surveillance_scene(total, FS, n_carriers=40, seed=21)
+    s = surv.Surveillance(n, FS)
+    psd = s.WelchPSD(iq, segs, hop, segs * hop, frames)
+    for mode in ("CA", "OS"):
+        cfar = surv.cfar_params(mode, guard=4, train=16, scale_db=12.3748, wrap=True, hysteresis_db=3.0, rank=24)
+        det = surv.detector_params(cfar_scale_db=12.3748, center_hz=100e6)
+        _compare_finish(s, psd, cfar, det, cap=1024)
+    s.Close()
+
+
+def test_device_detection_list_edges_merges_and_cap(surv):
+    """Crafted spectra: runs within 50 bins of either end (the global-median
+    path: the mean of the two middle values of the 4096 bins), neighbours
+    closer than the merge gap, expansions that cross a neighbour, and more
+    signals than the cap."""
+    n = 4096
+    rng = np.random.default_rng(5)
+    s = surv.Surveillance(n, FS)
+    frames = []
+    for f in range(2):
+        spec = (-100.0 + rng.standard_normal(n) * 0.5).astype(np.float32)
+        spec[10:20] = -40 + f             # near the left end
+        spec[n - 30:n - 22] = -45         # near the right end
+        spec[1000:1010] = -50             # two plateaus closer than the merge gap
+        spec[1012:1020] = -52
+        spec[2000:2040] = -60             # a wide one with a dip and a shoulder
+        spec[2015:2018] = -70
+        spec[2040:2060] = -75
+        for k in range(40):               # many narrow carriers
+            c = 2500 + 35 * k
+            spec[c:c + 3] = -55 - (k % 5)
+        frames.append(spec)
+    psd = np.stack(frames)
+    cfar = surv.cfar_params("CA", guard=2, train=12, scale_db=10.0, wrap=False, hysteresis_db=3.0)
+    for gap_hz, cap in ((5000.0, 1024), (0.0, 8), (60_000.0, 1024)):
+        det = surv.detector_params(cfar_scale_db=10.0, center_hz=0.0, merge_gap_hz=gap_hz)
+        _compare_finish(s, psd, cfar, det, cap=cap)
+    s.Close()
