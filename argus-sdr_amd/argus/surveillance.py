@@ -48,6 +48,7 @@ register("gpud_surv_spectrum_device", I, [P, P, L64, I, I, P])
 register("gpud_surv_cfar_runs_device", I, [P, P, I, C.POINTER(gpud_cfar_params), P, P, I])
 register("gpud_surv_finish_frame", I, [P, I, I, P, I, C.POINTER(gpud_detector_params), P, I])
 register("gpud_surv_finish_device", I, [P, P, I, P, P, I, C.POINTER(gpud_detector_params), P, I, P])
+register("gpud_surv_level_spectrum_device", I, [P, P, L64, I, I, P])
 register("gpud_surv_detect", I, [P, P, L64, I, I, I, C.POINTER(gpud_cfar_params), C.POINTER(gpud_detector_params),
                                  P, I, P, P])
 
@@ -158,6 +159,20 @@ class Surveillance:
                                                                            1 if window else 0, dout.ptr))
         self.sync()
         return dout.download(np.float32, frames * self.n).reshape(frames, self.n)
+
+    def LevelSpectrumDB(self, iq, decim: int = 2, frame_step: int = 0, n_frames: int = 1) -> np.ndarray:
+        """The decimated surveillance level's dB spectra (the multi-resolution
+        derived level of pipeline_runtime.go:624-645): frame f is the n_fft
+        samples at f * frame_step, low-passed, decimated by decim and
+        transformed at n_fft / decim points."""
+        iq = np.ascontiguousarray(iq, np.complex64)
+        din = DeviceBuffer(iq.nbytes).upload(iq)
+        n_d = self.n // decim
+        dout = DeviceBuffer(4 * n_d * n_frames)
+        check("gpud_surv_level_spectrum_device",
+              lib().gpud_surv_level_spectrum_device(self._h, din.ptr, frame_step or self.n, n_frames, decim, dout.ptr))
+        self.sync()
+        return dout.download(np.float32, n_d * n_frames).reshape(n_frames, n_d)
 
     def WelchDevice(self, iq_dev: int, frame_step: int, seg_hop: int, n_segs: int, n_frames: int, psd_dev: int):
         check("gpud_surv_welch_device", lib().gpud_surv_welch_device(self._h, iq_dev, frame_step, seg_hop, n_segs,

@@ -22,6 +22,8 @@
 #include <cstring>
 #include <vector>
 
+int argus_lowpass_fir(double cutoff_hz, int sample_rate, int taps, double* out, bool hamming);
+
 namespace {
 
 template <int R>
@@ -243,19 +245,54 @@ __global__ __launch_bounds__(R* R) void argus_fft_kernel(const float2* __restric
 
 // Single-FFT spectrum in dB (fft.go:58-73 SpectrumFromFFT): bin i of the
 // output is FFT bin (i + n/2) mod n, 20 log10(|X| / n + 1e-12), with NaN /
-// Inf mapped to -200 dB (pipeline_runtime.go:391-397).
-__global__ __launch_bounds__(256) void argus_spectrum_db_kernel(const float2* __restrict__ X, int n, int64_t total,
-                                                                float* __restrict__ out)
+// Inf mapped to -200 dB (pipeline_runtime.go:391-397).  Frame f's n bins
+// are the first n of X + f * x_stride.
+__global__ __launch_bounds__(256) void argus_spectrum_db_kernel(const float2* __restrict__ X, int n, int64_t x_stride,
+                                                                int64_t total, float* __restrict__ out)
 {
     const double inv_n = 1.0 / (double)n;
     for (int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; g < total; g += (int64_t)gridDim.x * blockDim.x) {
         const int64_t f = g / n;
         const int i = (int)(g - f * n);
-        const float2 v = X[f * n + (i + n / 2) % n];
+        const float2 v = X[f * x_stride + (i + n / 2) % n];
         const double mag = hypot((double)v.x, (double)v.y) * inv_n;
         const double db = 20.0 * log10(mag + 1e-12);
         out[g] = isfinite(db) ? (float)db : -200.0f;
     }
+}
+
+// A decimated surveillance level (pipeline_runtime.go:399-426, 624-645):
+// output m of frame f is ApplyFIR's y[m * decim] over the frame's window
+// (real FP64 taps, FP64 accumulation from zero history, fir.go:30-53, then
+// Decimate, fir.go:84-95), Hann-windowed over the level's n_d samples.  It
+// is written at z[f * N + m * decim] with zeros between, N = n_d * decim:
+// the N-point transform of that zero-stuffed frame is the level's
+// n_d-point transform repeated decim times, so the engine's own N-point FFT
+// serves every level.
+__global__ __launch_bounds__(256) void argus_level_fir_kernel(const float2* __restrict__ in, int64_t frame_step,
+                                                              int decim, int n_d, const double* __restrict__ taps,
+                                                              int n_taps, const float* __restrict__ win,
+                                                              float2* __restrict__ z)
+{
+#pragma clang fp contract(off)
+    const int m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= n_d) return;
+    const int f = blockIdx.y;
+    const float2* x = in + (int64_t)f * frame_step;
+    const int i = m * decim;
+    double ar = 0.0, ai = 0.0;
+    for (int k = 0; k < n_taps; k++) {
+        const int idx = i - k;
+        if (idx < 0) break;
+        const float2 v = x[idx];
+        const double w = taps[k];
+        ar += (double)v.x * w;
+        ai += (double)v.y * w;
+    }
+    const float wd = win[m];
+    float2* o = z + (int64_t)f * n_d * decim + i;
+    o[0] = make_float2((float)ar * wd, (float)ai * wd);
+    for (int j = 1; j < decim; j++) o[j] = make_float2(0.0f, 0.0f);
 }
 
 // ---------------------------------------------------------------------------
@@ -500,7 +537,10 @@ extern "C" {
 
 GPUD_API int gpud_surv_create(int n_fft, int sample_rate, gpud_surv** out)
 {
-    if (!out || fft_radix(n_fft) == 0 || sample_rate <= 0) return -1;
+    // transforms at 4096 and 32768 points; any other power of two from 1024
+    // to 32768 makes a detection-only engine (CFAR and the detection list of
+    // a decimated level whose spectra another engine computes)
+    if (!out || sample_rate <= 0 || n_fft < 1024 || n_fft > 32768 || (n_fft & (n_fft - 1)) != 0) return -1;
     gpud_surv* s = new gpud_surv();
     s->n = n_fft;
     s->sample_rate = sample_rate;
@@ -520,6 +560,9 @@ GPUD_API int gpud_surv_destroy(gpud_surv* s)
     hipStreamSynchronize(s->stream);
     hipFree(s->d_win);
     if (s->d_scratch) hipFree(s->d_scratch);
+    if (s->d_zstuff) hipFree(s->d_zstuff);
+    if (s->d_ltaps) hipFree(s->d_ltaps);
+    if (s->d_lwin) hipFree(s->d_lwin);
     hipStreamDestroy(s->stream);
     delete s;
     return 0;
@@ -536,6 +579,7 @@ GPUD_API int gpud_surv_welch_device(gpud_surv* s, const gpud_float2* iq_dev, int
                                     int n_segs, int n_frames, float* psd_dev)
 {
     if (!s || !iq_dev || !psd_dev || n_segs <= 0 || seg_hop < 0 || frame_step < 0) return -1;
+    if (fft_radix(s->n) == 0) return -1;  // a detection-only engine
     if (n_frames <= 0) return 0;
     const int R = fft_radix(s->n);
     argus_clear_error();
@@ -554,7 +598,7 @@ GPUD_API int gpud_surv_welch_device(gpud_surv* s, const gpud_float2* iq_dev, int
 GPUD_API int gpud_surv_fft_device(gpud_surv* s, const gpud_float2* in_dev, int64_t frame_step, int n_frames,
                                   int window, gpud_float2* out_dev)
 {
-    if (!s || !in_dev || !out_dev) return -1;
+    if (!s || !in_dev || !out_dev || fft_radix(s->n) == 0) return -1;
     if (n_frames <= 0) return 0;
     const float* w = window ? s->d_win : nullptr;
     argus_clear_error();
@@ -590,7 +634,71 @@ GPUD_API int gpud_surv_spectrum_device(gpud_surv* s, const gpud_float2* in_dev, 
     const int grid = (int)std::min<int64_t>((total + 255) / 256, 4096);
     argus_clear_error();
     hipLaunchKernelGGL(argus_spectrum_db_kernel, dim3(grid), dim3(256), 0, s->stream, (const float2*)s->d_scratch,
-                       s->n, total, db_dev);
+                       s->n, (int64_t)s->n, total, db_dev);
+    return (int)hipGetLastError();
+}
+
+static int grow_scratch(float2** p, size_t* cap, size_t need)
+{
+    if (*cap >= need) return 0;
+    if (*p) hipFree(*p);
+    *p = nullptr;
+    *cap = 0;
+    ARGUS_CHECK(hipMalloc((void**)p, need * sizeof(float2)));
+    *cap = need;
+    return 0;
+}
+
+// The spectrum of a decimated surveillance level (the multi-resolution
+// derived level, pipeline_runtime.go:1274-1293 plan, 624-645 use): per frame
+// the n_fft samples at in_dev + f * frame_step are low-passed with
+// LowpassFIR((sample_rate / decim) * 0.5 * 0.8, sample_rate, 101)
+// (decimationTaps, 399-412), decimated by decim, Hann-windowed over the
+// level's n_fft / decim samples and transformed; dB, fftshifted, NaN/Inf
+// sanitised: float32 [n_frames][n_fft / decim].
+GPUD_API int gpud_surv_level_spectrum_device(gpud_surv* s, const gpud_float2* in_dev, int64_t frame_step,
+                                             int n_frames, int decim, float* db_dev)
+{
+    if (!s || !in_dev || !db_dev || frame_step < 0 || fft_radix(s->n) == 0) return -1;
+    if (decim < 2 || (decim & (decim - 1)) != 0 || s->n % decim != 0 || s->n / decim < 256) return -1;
+    if (n_frames <= 0) return 0;
+    const int n_d = s->n / decim;
+    if (s->level_decim != decim) {  // taps and window of this level, once
+        std::vector<double> taps(102);
+        const int nt = argus_lowpass_fir((double)(s->sample_rate / decim) * 0.5 * 0.8, s->sample_rate, 101,
+                                         taps.data(), false);
+        std::vector<float> w((size_t)n_d);
+        for (int i = 0; i < n_d; i++)
+            w[(size_t)i] = (float)(0.5 * (1 - std::cos(2 * M_PI * (double)i / (double)(n_d - 1))));
+        if (s->d_ltaps) hipFree(s->d_ltaps);
+        if (s->d_lwin) hipFree(s->d_lwin);
+        s->d_ltaps = nullptr;
+        s->d_lwin = nullptr;
+        s->level_decim = 0;
+        ARGUS_CHECK(hipMalloc((void**)&s->d_ltaps, sizeof(double) * (size_t)nt));
+        ARGUS_CHECK(hipMalloc((void**)&s->d_lwin, sizeof(float) * (size_t)n_d));
+        ARGUS_CHECK(hipMemcpy(s->d_ltaps, taps.data(), sizeof(double) * (size_t)nt, hipMemcpyHostToDevice));
+        ARGUS_CHECK(hipMemcpy(s->d_lwin, w.data(), sizeof(float) * (size_t)n_d, hipMemcpyHostToDevice));
+        s->level_taps = nt;
+        s->level_decim = decim;
+    }
+    const size_t need = (size_t)n_frames * (size_t)s->n;
+    int err = grow_scratch(&s->d_zstuff, &s->zstuff_cap, need);
+    if (!err) err = grow_scratch(&s->d_scratch, &s->scratch_cap, need);
+    if (err) return err;
+    argus_clear_error();
+    hipLaunchKernelGGL(argus_level_fir_kernel, dim3((n_d + 255) / 256, n_frames), dim3(256), 0, s->stream,
+                       (const float2*)in_dev, frame_step, decim, n_d, (const double*)s->d_ltaps, s->level_taps,
+                       (const float*)s->d_lwin, s->d_zstuff);
+    err = (int)hipGetLastError();
+    if (!err)
+        err = gpud_surv_fft_device(s, (const gpud_float2*)s->d_zstuff, s->n, n_frames, 0,
+                                   (gpud_float2*)s->d_scratch);
+    if (err) return err;
+    const int64_t total = (int64_t)n_frames * n_d;
+    const int grid = (int)std::min<int64_t>((total + 255) / 256, 4096);
+    hipLaunchKernelGGL(argus_spectrum_db_kernel, dim3(grid), dim3(256), 0, s->stream, (const float2*)s->d_scratch,
+                       n_d, (int64_t)s->n, total, db_dev);
     return (int)hipGetLastError();
 }
 

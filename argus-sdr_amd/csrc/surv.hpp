@@ -12,4 +12,11 @@ struct gpud_surv {
     float* d_win = nullptr;  // symmetric Hann, n floats
     float2* d_scratch = nullptr;  // complex spectra of gpud_surv_spectrum_device
     size_t scratch_cap = 0;       // complex samples
+    // decimated level (gpud_surv_level_spectrum_device): zero-stuffed frames,
+    // FP64 low-pass taps and the level's Hann window for level_decim
+    float2* d_zstuff = nullptr;
+    size_t zstuff_cap = 0;
+    double* d_ltaps = nullptr;
+    float* d_lwin = nullptr;
+    int level_decim = 0, level_taps = 0;
 };

@@ -462,6 +462,35 @@ def bench_surveillance(a, lib):
         welch_ms += m1.value / reps
         cfar_ms += m2.value / reps
         finish_ms += m3.value / reps
+    # the multi-resolution derived level (decim 2, 16384 points at 5 Msps):
+    # its spectrum from the base engine, its CFAR runs and detection list
+    # on a detection-only engine of the level's size
+    n_d = n // 2
+    lv = SV.Surveillance(n_d, fs // 2)
+    dlvl = SV.DeviceBuffer(4 * n_d * frames)
+    lst = lib.gpud_surv_stream(lv._h)
+    level_ms = level_det_ms = 0.0
+    for it in range(reps + 1):
+        lib.gpud_event_record(e[0], st)
+        check("level", lib.gpud_surv_level_spectrum_device(sv._h, din.ptr, segs * hop, frames, 2, dlvl.ptr))
+        lib.gpud_event_record(e[1], st)
+        sv.sync()
+        lib.gpud_event_record(e[2], lst)
+        check("cfar", lib.gpud_surv_cfar_runs_device(lv._h, dlvl.ptr, frames, C.byref(cfar), druns.ptr, dcnt.ptr,
+                                                     max_runs))
+        check("finish", lib.gpud_surv_finish_device(lv._h, dlvl.ptr, frames, druns.ptr, dcnt.ptr, max_runs,
+                                                    C.byref(det), dsig.ptr, cap, dout.ptr))
+        lib.gpud_event_record(e[3], lst)
+        lv.sync()
+        m1, m2 = C.c_float(), C.c_float()
+        check("el", lib.gpud_event_elapsed_ms(e[0], e[1], C.byref(m1)))
+        check("el", lib.gpud_event_elapsed_ms(e[2], e[3], C.byref(m2)))
+        if it == 0:
+            continue
+        level_ms += m1.value / reps
+        level_det_ms += m2.value / reps
+    lv.Close()
+    dlvl.free()
     # whole detection list (upload of the IQ, kernels, download of the
     # lists) once, timed
     t0 = time.perf_counter()
@@ -474,6 +503,7 @@ def bench_surveillance(a, lib):
     flops = 5 * n * np.log2(n) * segs * frames
     return {"frames": frames, "fft": n, "segments_per_frame": segs, "welch_ms": round(welch_ms, 4),
             "cfar_runs_ms": round(cfar_ms, 4), "detection_list_device_ms": round(finish_ms, 4),
+            "level_decim2_spectrum_ms": round(level_ms, 4), "level_decim2_cfar_and_list_ms": round(level_det_ms, 4),
             "detect_host_to_host_ms": round(detect_ms, 2),
             "frames_per_s": round(frames / ((welch_ms + cfar_ms + finish_ms) * 1e-3), 1),
             "msps_sustained": round(total / ((welch_ms + cfar_ms + finish_ms) * 1e-3) / 1e6, 1),

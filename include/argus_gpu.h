@@ -539,7 +539,12 @@ typedef struct gpud_detector_params {
     double center_hz;
 } gpud_detector_params;
 
-/* gpu.go:33-51 New(n): n_fft in {4096, 32768}.  -1 otherwise. */
+/* gpu.go:33-51 New(n): transforms at n_fft in {4096, 32768}.  Any other
+ * power of two from 1024 to 32768 makes a detection-only engine (CFAR runs
+ * and detection lists of spectra another engine computed, e.g. a decimated
+ * level's: detector.New(cfg, level.SampleRate, level.FFTSize),
+ * pipeline_runtime.go:791-817), whose transform entry points return -1.
+ * -1 otherwise. */
 GPUD_API int gpud_surv_create(int n_fft, int sample_rate, gpud_surv** out);
 GPUD_API int gpud_surv_destroy(gpud_surv* s);
 GPUD_API gpud_stream_handle gpud_surv_stream(gpud_surv* s);
@@ -557,6 +562,17 @@ GPUD_API int gpud_surv_fft_device(gpud_surv* s, const gpud_float2* in_dev, int64
  * same way.) */
 GPUD_API int gpud_surv_spectrum_device(gpud_surv* s, const gpud_float2* in_dev, int64_t frame_step, int n_frames,
                                        int window, float* db_dev);
+/* The spectrum of a decimated surveillance level (the multi-resolution
+ * derived level: plan pipeline_runtime.go:1274-1293, spectrum 624-645 with
+ * decimateSurveillanceIQ 399-426): per frame the n_fft samples at in_dev +
+ * f*frame_step are low-passed by LowpassFIR((sample_rate/decim)*0.5*0.8,
+ * sample_rate, 101) from zero history (ApplyFIR), decimated by decim,
+ * Hann-windowed over the level's n_fft/decim samples and transformed;
+ * dB, fftshifted, NaN/Inf -> -200: float32 [n_frames][n_fft/decim] on the
+ * device.  decim a power of two with n_fft/decim >= 256 (the reference
+ * derives decim 2).  Queued on the engine's stream. */
+GPUD_API int gpud_surv_level_spectrum_device(gpud_surv* s, const gpud_float2* in_dev, int64_t frame_step,
+                                             int n_frames, int decim, float* db_dev);
 /* ca.go / gosca.go / caso.go / os.go + detector.go:436-480 per frame. */
 GPUD_API int gpud_surv_cfar_runs_device(gpud_surv* s, const float* psd_dev, int n_frames,
                                         const gpud_cfar_params* params, gpud_surv_run* runs_dev,

@@ -465,6 +465,17 @@ def SpectrumFromFFT(X):
     return out
 
 
+def SurveillanceLevelSpectrum(iq, sample_rate, decim):
+    """pipeline_runtime.go:624-645 (derived level) with decimationTaps /
+    decimateSurveillanceIQ (399-426): ApplyFIR with LowpassFIR((sample_rate //
+    decim) * 0.5 * 0.8, sample_rate, 101), Decimate, then the level's
+    spectrum (spectrumForLevel, 379-389: Hann of the level's size, FP64 FFT,
+    dB, fftshift).  iq is the level's n_fft * decim input samples."""
+    taps = LowpassFIR(float(sample_rate // decim) * 0.5 * 0.8, sample_rate, 101)
+    d = Decimate(ApplyFIR(iq, taps), decim)
+    return Spectrum(d, Hann(len(d)))
+
+
 def WelchPSD(iq, seg, overlap, window=None):
     iq = c64(iq)
     w = np.ascontiguousarray(window if window is not None else np.zeros(0), np.float64)

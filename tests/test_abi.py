@@ -68,3 +68,13 @@ def test_pure_host_helpers_match_oracle():
     np.testing.assert_array_equal(G.LowpassFIR(7500, 250000, 101), O.LowpassFIR(7500, 250000, 101))
     base = np.arange(1, 8, dtype=np.float32)
     np.testing.assert_array_equal(G.BuildPolyphaseTapsPhaseMajor(base, 3), [1, 4, 7, 2, 5, 0, 3, 6, 0])
+
+
+def test_library_resolves_its_own_symbols():
+    """Every argus_/gpud_ symbol the library uses is defined in it (a C-linkage
+    declaration of a C++ helper would leave one undefined until load)."""
+    import subprocess
+    lib = ROOT / "argus-sdr_amd" / "libargus_hip.so"
+    out = subprocess.run(["nm", "-D", "--undefined-only", str(lib)], capture_output=True, text=True, check=True).stdout
+    own = [l for l in out.splitlines() if " argus_" in l or " gpud_" in l]
+    assert not own, own

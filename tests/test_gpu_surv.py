@@ -197,3 +197,60 @@ def test_device_detection_list_edges_merges_and_cap(surv):
         det = surv.detector_params(cfar_scale_db=10.0, center_hz=0.0, merge_gap_hz=gap_hz)
         _compare_finish(s, psd, cfar, det, cap=cap)
     s.Close()
+
+
+@pytest.mark.parametrize("n,decim", [(32768, 2), (4096, 2), (4096, 4)])
+def test_level_spectrum_matches_fp64_oracle(surv, n, decim):
+    """The multi-resolution derived level (gpud_surv_level_spectrum_device):
+    low-pass + decimate + n/decim-point spectrum per frame, against the
+    oracle's FP64 composition (ApplyFIR, Decimate, Spectrum), on two frames
+    with a tone inside the level, one beyond its band and noise."""
+    rng = np.random.default_rng(n + decim)
+    frames = 2
+    t = np.arange(frames * n)
+    fs = FS
+    iq = (0.5 * np.exp(2j * np.pi * 0.0371 * t) + 0.2 * np.exp(2j * np.pi * 0.46 * t)
+          + 0.01 * (rng.standard_normal(frames * n) + 1j * rng.standard_normal(frames * n))).astype(np.complex64)
+    s = surv.Surveillance(n, fs)
+    got = s.LevelSpectrumDB(iq, decim=decim, frame_step=n, n_frames=frames)
+    n_d = n // decim
+    assert got.shape == (frames, n_d)
+    for f in range(frames):
+        want = O.SurveillanceLevelSpectrum(iq[f * n:(f + 1) * n], fs, decim)
+        lg, lw = 10 ** (got[f] / 20.0), 10 ** (want / 20.0)
+        err = float(np.max(np.abs(lg - lw)) / lw.max())
+        print(f"frame {f}: max |mag err| / peak {err:.2e}", flush=True)
+        assert err <= 2e-6
+        near = want > want.max() - 40.0
+        assert np.max(np.abs(got[f][near] - want[near])) < 1e-3
+        assert int(np.argmax(got[f])) == int(np.argmax(want))
+    with pytest.raises(Exception):
+        s.LevelSpectrumDB(iq, decim=3)
+    s.Close()
+
+
+def test_level_detection_on_the_device(surv):
+    """Derived-level detection (detectDerivedCandidates,
+    pipeline_runtime.go:742-789): the level spectra of the base engine go to
+    a detection-only engine at the level's rate and size, whose CFAR runs and
+    device detection list equal the host list and the oracle's."""
+    from argus.synth import surveillance_scene
+    n, decim, frames = 32768, 2, 3
+    iq, _ = surveillance_scene(frames * n, FS, n_carriers=30, seed=8)
+    base = surv.Surveillance(n, FS)
+    lvl = base.LevelSpectrumDB(iq, decim=decim, frame_step=n, n_frames=frames)
+    det_engine = surv.Surveillance(n // decim, FS // decim)
+    with pytest.raises(Exception):
+        det_engine.SpectrumDB(iq[:n // decim])  # no transform at this size
+    cfar = surv.cfar_params("CA", guard=4, train=16, scale_db=12.3748, wrap=True, hysteresis_db=3.0)
+    det = surv.detector_params(cfar_scale_db=12.3748, center_hz=100e6)
+    _compare_finish(det_engine, lvl, cfar, det, cap=1024)
+    lists, _ = det_engine.DetectSpectra(lvl, cfar, det)
+    for f in range(frames):
+        spec = lvl[f].astype(np.float64)
+        thr = O.CFARThresholds("CA", spec, 4, 16, 12.3748, wrap=True)
+        want = O.DetectSignals(spec, thr, sample_rate=FS // decim, center_hz=100e6, cfar_scale_db=12.3748)
+        assert _same_lists(lists[f], want), f
+        assert len(want) > 5
+    base.Close()
+    det_engine.Close()

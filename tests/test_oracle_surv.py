@@ -163,3 +163,30 @@ def test_detection_list_hysteresis_keeps_run_open():
     th = np.full(1000, -51.0)
     sig = O.DetectSignals(s, th, sample_rate=1_000_000, hysteresis_db=3.0, max_signal_bw_hz=0.0)
     assert len(sig) == 1 and sig[0].FirstBin == 100 and sig[0].LastBin == 119
+
+
+def test_level_spectrum_composition_and_known_answers():
+    """The derived surveillance level (pipeline_runtime.go:399-426, 624-645)
+    as the oracle composes it: equal to an independent numpy restatement
+    (causal FP64 convolution from zero history, every decim-th sample, Hann,
+    FFT, dB, fftshift); a tone inside the level lands at bin n_d/2 + f/df
+    (the +f -> upper-half orientation of synth_test.go:16-21); a tone beyond
+    the level's band is suppressed by the decimation low-pass."""
+    fs, decim, n = 2_000_000, 2, 8192
+    n_d = n // decim
+    df = fs / decim / n_d
+    t = np.arange(n)
+    f_in = 40 * df
+    x = (np.exp(2j * np.pi * f_in * t / fs) + 1e-3 * np.exp(2j * np.pi * 0.45 * t)).astype(np.complex64)
+    got = O.SurveillanceLevelSpectrum(x, fs, decim)
+    taps = O.LowpassFIR(float(fs // decim) * 0.5 * 0.8, fs, 101)
+    y = np.convolve(x.astype(np.complex128), taps)[:n].astype(np.complex64)[::decim]
+    w = 0.5 * (1 - np.cos(2 * np.pi * np.arange(n_d) / (n_d - 1)))
+    X = np.fft.fft(y.astype(np.complex128) * w)
+    want = 20 * np.log10(np.abs(np.fft.fftshift(X)) / n_d + 1e-12)
+    assert len(got) == n_d
+    np.testing.assert_allclose(got, want, atol=1e-6)
+    assert int(np.argmax(got)) == n_d // 2 + 40
+    # 0.45 fs aliases to 0.45 fs - fs/2 = -0.05 fs in the level: far below the tone
+    alias = n_d // 2 + int(round((0.45 * fs - fs / 2) / df))
+    assert got[alias] < got.max() - 80
