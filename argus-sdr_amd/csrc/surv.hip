@@ -269,28 +269,50 @@ __global__ __launch_bounds__(256) void argus_spectrum_db_kernel(const float2* __
 // the N-point transform of that zero-stuffed frame is the level's
 // n_d-point transform repeated decim times, so the engine's own N-point FFT
 // serves every level.
-__global__ __launch_bounds__(256) void argus_level_fir_kernel(const float2* __restrict__ in, int64_t frame_step,
-                                                              int decim, int n_d, const double* __restrict__ taps,
-                                                              int n_taps, const float* __restrict__ win,
-                                                              float2* __restrict__ z)
+//
+// One workgroup per 256 outputs of a frame: the input span they read
+// (256 * decim + taps - 1 samples, zero before the window's start, which
+// adds nothing to an accumulation exactly as ApplyFIR's early break) and the
+// taps are staged in LDS, then each thread accumulates its output over the
+// taps in order.
+constexpr int kLevelOut = 256;
+constexpr int kLevelMaxTaps = 128;
+constexpr int kLevelMaxDecim = 16;
+
+__global__ __launch_bounds__(kLevelOut) void argus_level_fir_kernel(const float2* __restrict__ in, int64_t frame_step,
+                                                                    int decim, int n_d,
+                                                                    const double* __restrict__ taps, int n_taps,
+                                                                    const float* __restrict__ win,
+                                                                    float2* __restrict__ z)
 {
 #pragma clang fp contract(off)
-    const int m = blockIdx.x * blockDim.x + threadIdx.x;
-    if (m >= n_d) return;
+    __shared__ float2 xs[kLevelOut * kLevelMaxDecim + kLevelMaxTaps];
+    __shared__ double hs[kLevelMaxTaps];
     const int f = blockIdx.y;
+    const int m0 = blockIdx.x * kLevelOut;
     const float2* x = in + (int64_t)f * frame_step;
-    const int i = m * decim;
+    const int base = m0 * decim - (n_taps - 1);  // input index of xs[0]
+    const int span = kLevelOut * decim + n_taps - 1;
+    for (int j = threadIdx.x; j < span; j += kLevelOut) {
+        const int idx = base + j;
+        const int lim = n_d * decim;
+        xs[j] = (idx >= 0 && idx < lim) ? x[idx] : make_float2(0.0f, 0.0f);
+    }
+    for (int k = threadIdx.x; k < n_taps; k += kLevelOut) hs[k] = taps[k];
+    __syncthreads();
+    const int m = m0 + threadIdx.x;
+    if (m >= n_d) return;
+    // y[i] = sum_k h[k] x[i - k], i = m * decim -> xs index i - k - base
+    const float2* xp = xs + (int)threadIdx.x * decim + (n_taps - 1);
     double ar = 0.0, ai = 0.0;
     for (int k = 0; k < n_taps; k++) {
-        const int idx = i - k;
-        if (idx < 0) break;
-        const float2 v = x[idx];
-        const double w = taps[k];
+        const float2 v = xp[-k];
+        const double w = hs[k];
         ar += (double)v.x * w;
         ai += (double)v.y * w;
     }
     const float wd = win[m];
-    float2* o = z + (int64_t)f * n_d * decim + i;
+    float2* o = z + (int64_t)f * n_d * decim + (int64_t)m * decim;
     o[0] = make_float2((float)ar * wd, (float)ai * wd);
     for (int j = 1; j < decim; j++) o[j] = make_float2(0.0f, 0.0f);
 }
@@ -660,13 +682,15 @@ GPUD_API int gpud_surv_level_spectrum_device(gpud_surv* s, const gpud_float2* in
                                              int n_frames, int decim, float* db_dev)
 {
     if (!s || !in_dev || !db_dev || frame_step < 0 || fft_radix(s->n) == 0) return -1;
-    if (decim < 2 || (decim & (decim - 1)) != 0 || s->n % decim != 0 || s->n / decim < 256) return -1;
+    if (decim < 2 || decim > kLevelMaxDecim || (decim & (decim - 1)) != 0 || s->n % decim != 0 || s->n / decim < 256)
+        return -1;
     if (n_frames <= 0) return 0;
     const int n_d = s->n / decim;
     if (s->level_decim != decim) {  // taps and window of this level, once
         std::vector<double> taps(102);
         const int nt = argus_lowpass_fir((double)(s->sample_rate / decim) * 0.5 * 0.8, s->sample_rate, 101,
                                          taps.data(), false);
+        if (nt <= 0 || nt > kLevelMaxTaps) return -1;
         std::vector<float> w((size_t)n_d);
         for (int i = 0; i < n_d; i++)
             w[(size_t)i] = (float)(0.5 * (1 - std::cos(2 * M_PI * (double)i / (double)(n_d - 1))));
@@ -687,7 +711,8 @@ GPUD_API int gpud_surv_level_spectrum_device(gpud_surv* s, const gpud_float2* in
     if (!err) err = grow_scratch(&s->d_scratch, &s->scratch_cap, need);
     if (err) return err;
     argus_clear_error();
-    hipLaunchKernelGGL(argus_level_fir_kernel, dim3((n_d + 255) / 256, n_frames), dim3(256), 0, s->stream,
+    hipLaunchKernelGGL(argus_level_fir_kernel, dim3((n_d + kLevelOut - 1) / kLevelOut, n_frames), dim3(kLevelOut), 0,
+                       s->stream,
                        (const float2*)in_dev, frame_step, decim, n_d, (const double*)s->d_ltaps, s->level_taps,
                        (const float*)s->d_lwin, s->d_zstuff);
     err = (int)hipGetLastError();

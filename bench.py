@@ -86,17 +86,20 @@ def alg_figures(chans, n_in):
 
 def load_traffic(channels):
     """(bytes, profile): measured HBM bytes per fused launch (rocprofv3
-    FETCH_SIZE + WRITE_SIZE, scaled by a copy of known size) if the profile
-    summary was taken at this channel count, else None; and what the profile
+    FETCH_SIZE + WRITE_SIZE, scaled by a copy of known size) from the profile
+    taken at this channel count, else None; and what the nearest profile
     holds (its channel count, bytes, and their ratio to the algorithmic
     bytes), or None."""
     p = ROOT / "profiles" / "extract_traffic.json"
     if not p.exists():
         return None, None
     try:
-        d = json.loads(p.read_text())
-    except (ValueError, OSError):
+        profs = json.loads(p.read_text())["profiles"]
+    except (ValueError, OSError, KeyError):
         return None, None
+    if not profs:
+        return None, None
+    d = min(profs, key=lambda q: abs(q.get("channels", 0) - channels))
     prof = {"channels": d.get("channels"), "bytes_per_launch": d.get("bytes_per_launch"),
             "over_algorithmic": round(d["bytes_per_launch"] / d["algorithmic_bytes_per_launch"], 3)
             if d.get("algorithmic_bytes_per_launch") else None}
@@ -729,8 +732,8 @@ def main() -> int:
     ap.add_argument("--stereo-blocks", type=int, default=60)
     ap.add_argument("--rds-stations", type=int, default=1024,
                     help="stations per device RDS decode call in the RDS decoding figure (0: skip it)")
-    ap.add_argument("--rds-only", action="store_true",
-                    help="run only the device RDS decoding figure and print it")
+    ap.add_argument("--only", choices=["rds", "surveillance"], default="",
+                    help="run only this secondary figure (device RDS decoding, or surveillance) and print it")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extras", action="store_true",
                     help="skip surveillance, the 2.048 Msps case and the stereo multiplex case")
@@ -757,8 +760,11 @@ def main() -> int:
         print("no HIP device", file=sys.stderr)
         return 1
 
-    if a.rds_only:
+    if a.only == "rds":
         print(json.dumps({"rds_decode_device": bench_rds_decode(a, L)}), flush=True)
+        return 0
+    if a.only == "surveillance":
+        print(json.dumps({"surveillance_10msps_n32768": bench_surveillance(a, L)[0]}), flush=True)
         return 0
 
     blocks = [make_block(s) for s in range(4)]

@@ -234,9 +234,14 @@ def test_level_detection_on_the_device(surv):
     pipeline_runtime.go:742-789): the level spectra of the base engine go to
     a detection-only engine at the level's rate and size, whose CFAR runs and
     device detection list equal the host list and the oracle's."""
-    from argus.synth import surveillance_scene
     n, decim, frames = 32768, 2, 3
-    iq, _ = surveillance_scene(frames * n, FS, n_carriers=30, seed=8)
+    rng = np.random.default_rng(8)
+    t = np.arange(frames * n)
+    iq = 0.01 * (rng.standard_normal(frames * n) + 1j * rng.standard_normal(frames * n))
+    for k in range(12):  # carriers inside the level's +-2.5 MHz, one beyond it
+        f0 = -2.2e6 + k * 0.4e6 + rng.uniform(-20e3, 20e3)
+        iq = iq + 10 ** (-rng.uniform(0, 1.5)) * np.exp(2j * np.pi * (f0 / FS * t + rng.uniform()))
+    iq = (iq + 0.5 * np.exp(2j * np.pi * 4.0e6 / FS * t)).astype(np.complex64)
     base = surv.Surveillance(n, FS)
     lvl = base.LevelSpectrumDB(iq, decim=decim, frame_step=n, n_frames=frames)
     det_engine = surv.Surveillance(n // decim, FS // decim)
