@@ -61,6 +61,7 @@ register("gpud_serve_set_audio", I, [P, C.c_int64, I, C.c_double])
 register("gpud_serve_channel_audio", I, [P, C.c_int64, C.c_int64, C.POINTER(P), C.POINTER(I), C.POINTER(I),
                                          C.POINTER(I)])
 register("gpud_serve_set_rds", I, [P, C.c_int64, I])
+register("gpud_serve_set_fast", I, [P, I])
 register("gpud_serve_channel_rds", I, [P, C.c_int64, C.c_int64, C.POINTER(I), C.POINTER(I), C.POINTER(C.c_uint16),
                                        C.c_char_p, C.c_char_p, C.POINTER(P), C.POINTER(I)])
 
@@ -218,6 +219,10 @@ class Serve:
         g = np.ctypeslib.as_array((C.c_uint16 * (4 * n.value)).from_address(p.value)).copy().reshape(-1, 4) \
             if n.value else np.zeros((0, 4), np.uint16)
         return (bool(dec.value), found.value, g, pi.value, ps.value.decode("latin-1"), rt.value.decode("latin-1"))
+
+    def SetFast(self, enable: bool) -> None:
+        """The fast-convolution form for eligible channels (on by default)."""
+        check("gpud_serve_set_fast", lib().gpud_serve_set_fast(self._h, 1 if enable else 0))
 
     def SetGraph(self, enable: bool) -> None:
         check("gpud_serve_set_graph", lib().gpud_serve_set_graph(self._h, 1 if enable else 0))

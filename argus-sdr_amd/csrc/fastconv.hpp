@@ -1,0 +1,58 @@
+// fastconv.hpp -- overlap-save fast convolution for the dense channels of
+// the streaming service (layout and semantics in fastconv.hip).
+#pragma once
+#include "common.hpp"
+#include "extract.hpp"
+
+#define ARGUS_FAST_N 4096      // segment transform size
+#define ARGUS_FAST_TAIL 511    // input samples kept before each block (taps <= 512)
+#define ARGUS_FAST_HOP 3584    // new samples per segment: <= N - TAIL, a multiple of 64
+#define ARGUS_FAST_GRID 2048   // workgroups of the per-channel kernel (grid-stride)
+
+// One channel on the fast path for one block.
+struct ArgusFastDesc {
+    const float2* G;   // ARGUS_FAST_N bins: FFT of the modulated taps, scaled 1/N
+    float2* out;       // n_out outputs of this block
+    uint64_t phase0;   // NCO phase of block sample 0 (turns, 2^64 == 1)
+    uint64_t inc;      // NCO increment per sample
+    int D;             // decimation (8, 16, 32 or 64)
+    int e0;            // block sample of output 0
+    int n_out;         // outputs this block
+    int r;             // (e0 + ARGUS_FAST_TAIL) mod D
+};
+
+// What the fast-path kernels of one block read (device memory, so a
+// captured launch replays with new contents).
+struct ArgusFastHdr {
+    const ArgusFastDesc* descs;
+    float2* tail;      // the stream's last ARGUS_FAST_TAIL input samples
+    float2* u;         // staged input: tail ++ block ++ zeros
+    float2* X;         // segment spectra, S x ARGUS_FAST_N
+    int n_fast;        // channels on the fast path this block
+    int S;             // segments this block (ceil(n / ARGUS_FAST_HOP))
+};
+
+// Whether a channel of decimation D and T taps can take the fast path.
+static inline bool argus_fast_eligible(int D, int T)
+{
+    return (D == 8 || D == 16 || D == 32 || D == 64) && T >= 1 && T <= ARGUS_FAST_TAIL + 1;
+}
+static inline int argus_fast_segments(int n) { return n > 0 ? (n + ARGUS_FAST_HOP - 1) / ARGUS_FAST_HOP : 0; }
+static inline size_t argus_fast_u_len(int segments)
+{
+    return segments > 0 ? (size_t)(segments - 1) * ARGUS_FAST_HOP + ARGUS_FAST_N : 0;
+}
+
+// G from the natural-order taps h (T floats) and the NCO increment: the
+// N-point transform of g[k] = h[k] e^{-j 2pi inc k}, scaled by 1/N, in FP64,
+// rounded once.
+void argus_fast_spectrum(const float* h, int T, uint64_t inc, float2* G);
+// Segment spectra and the per-channel pass (when S > 0), reading the block
+// input and its length from *xh; u and X are the buffers d_hdr names (fixed
+// per service), passed again for the transform launch.
+int argus_launch_fast(const ArgusFastHdr* d_hdr, const ArgusExtractHdr* d_xh, int S, float2* u, float2* X,
+                      hipStream_t stream);
+// The stream's tail after the block (every block, fast channels or not).
+int argus_launch_fast_tail(const ArgusFastHdr* d_hdr, const ArgusExtractHdr* d_xh, hipStream_t stream);
+// surv.hip: forward 4096-point transforms of n_frames frames at in + f * frame_step.
+int argus_launch_fft4096(const float2* in, int64_t frame_step, int n_frames, float2* out, hipStream_t stream);

@@ -101,6 +101,7 @@ static size_t channel_footprint(const gpud_serve* s, int D, int T, bool stereo)
     const size_t n_out = (size_t)s->block_samples / D + 2;
     size_t per_block = n_out * (sizeof(float2) + sizeof(float));
     size_t fixed = taps + hist + carry;
+    if (argus_fast_eligible(D, T)) fixed += ARGUS_FAST_N * sizeof(float2);
     if (stereo) {
         // mono + 4 scratch floats + a second audio float + RDS samples (<= 1 per 2 outputs)
         per_block += n_out * (6 * sizeof(float)) + (n_out / 2 + 1) * sizeof(float2) + sizeof(ArgusWfmBlock);
@@ -119,8 +120,10 @@ static void release_channel(ServeChannel& c)
     if (c.d_wfm) hipFree(c.d_wfm);
     if (c.d_aud) hipFree(c.d_aud);
     if (c.d_rdec) hipFree(c.d_rdec);
+    if (c.d_fastG) hipFree(c.d_fastG);
     c.d_aud = nullptr;
     c.d_rdec = nullptr;
+    c.d_fastG = nullptr;
     c.d_taps = nullptr;
     c.d_ctaps = nullptr;
     c.d_hist[0] = c.d_hist[1] = nullptr;
@@ -253,6 +256,11 @@ GPUD_API int gpud_serve_create(int sample_rate, int block_samples, uint64_t hbm_
         ARGUS_CHECK(dev_alloc(&sl.d_in, (size_t)block_samples));
         ARGUS_CHECK(host_alloc(&sl.h_in, (size_t)block_samples));
     }
+    const int smax = argus_fast_segments(block_samples);
+    ARGUS_CHECK(dev_alloc(&s->d_ftail, (size_t)ARGUS_FAST_TAIL));
+    ARGUS_CHECK(hipMemset(s->d_ftail, 0, ARGUS_FAST_TAIL * sizeof(float2)));
+    ARGUS_CHECK(dev_alloc(&s->d_fu, argus_fast_u_len(smax)));
+    ARGUS_CHECK(dev_alloc(&s->d_fX, (size_t)smax * ARGUS_FAST_N));
     *out = s;
     return 0;
 }
@@ -276,6 +284,9 @@ GPUD_API int gpud_serve_destroy(gpud_serve* s)
         slot_release(sl);
     }
     for (auto& kv : s->filters) hipFree(kv.second);
+    hipFree(s->d_ftail);
+    hipFree(s->d_fu);
+    hipFree(s->d_fX);
     hipStreamDestroy(s->compute);
     hipStreamDestroy(s->copy);
     delete s;
@@ -300,6 +311,7 @@ GPUD_API int gpud_serve_open(gpud_serve* s, const gpud_channel_spec* spec)
 
     ServeChannel c;
     c.spec = *spec;
+    c.r_last_ticket = s->last_ticket;  // decodes of blocks before it are not its own
     c.decim = D;
     c.T = T;
     c.footprint = fp;
@@ -324,6 +336,17 @@ GPUD_API int gpud_serve_open(gpud_serve* s, const gpud_channel_spec* spec)
         dev_alloc(&c.d_carry, 2)) {
         release_channel(c);
         return (int)hipErrorOutOfMemory;
+    }
+    if (argus_fast_eligible(D, T)) {  // the fast path's transform of the modulated taps
+        std::vector<float> h((size_t)T);
+        for (int k = 0; k < T; k++) h[(size_t)k] = poly[(size_t)(k % D) * L + k / D];
+        std::vector<float2> G(ARGUS_FAST_N);
+        argus_fast_spectrum(h.data(), T, inc, G.data());
+        if (dev_alloc(&c.d_fastG, (size_t)ARGUS_FAST_N) != hipSuccess ||
+            hipMemcpy(c.d_fastG, G.data(), G.size() * sizeof(float2), hipMemcpyHostToDevice) != hipSuccess) {
+            release_channel(c);
+            return (int)hipErrorOutOfMemory;
+        }
     }
     hipError_t e = hipMemcpy(c.d_taps, poly.data(), poly.size() * sizeof(float), hipMemcpyHostToDevice);
     if (e == hipSuccess)
@@ -584,6 +607,7 @@ static int describe_block(gpud_serve* s, ServeSlot& sl, const std::vector<ServeO
     int n_audio = 0;
     for (const ServeOut& o : outs) n_audio += o.a_index >= 0 ? 1 : 0;
     s->adescs.resize((size_t)n_audio);
+    s->fdescs.clear();
     size_t lds = 0;
     int ci = 0;
     for (const auto& kv : s->live) {
@@ -596,10 +620,26 @@ static int describe_block(gpud_serve* s, ServeSlot& sl, const std::vector<ServeO
                             c.d_ctaps, sl.d_iq + o.iq_off, c.decim, c.T, c.phase_count, n, c.nco_phase, inc,
                             s->tune) != 0)
             return -1;
-        if ((c.decim - 1 + n) / c.decim > 0) lds = std::max(lds, argus_item_lds_bytes(ch));
+        // the fast path, once the channel's history is the stream's own
+        // last T-1 samples; its direct items are then only the history
+        const bool fast = s->fast && c.d_fastG && c.hist_len == c.T - 1 && n > 0;
+        if (fast) {
+            ArgusFastDesc fd;
+            fd.G = c.d_fastG;
+            fd.out = ch.out;
+            fd.phase0 = ch.phase0;
+            fd.inc = ch.inc;
+            fd.D = ch.D;
+            fd.e0 = ch.e0;
+            fd.n_out = ch.n_out;
+            fd.r = (ch.e0 + ARGUS_FAST_TAIL) % ch.D;
+            s->fdescs.push_back(fd);
+        } else if ((c.decim - 1 + n) / c.decim > 0) {
+            lds = std::max(lds, argus_item_lds_bytes(ch));
+        }
         if (!reuse) {
             ArgusChanDesc full = ch;
-            full.n_out = (c.decim - 1 + n) / c.decim;
+            full.n_out = fast ? 0 : (c.decim - 1 + n) / c.decim;
             argus_append_items(full, ci, n, s->items, s->item_pos);
         }
 
@@ -647,7 +687,9 @@ static int describe_block(gpud_serve* s, ServeSlot& sl, const std::vector<ServeO
     const size_t off_ditems = align_up(off_wdesc + (size_t)n_wfm * sizeof(ArgusWfmDesc));
     const size_t off_witems = align_up(off_ditems + (size_t)n_ditems * sizeof(ArgusDemodItem));
     const size_t off_adesc = align_up(off_witems + (size_t)n_witems * sizeof(ArgusWfmItem));
-    const size_t total = off_adesc + (size_t)n_audio * sizeof(ArgusAudioDesc);
+    const int n_fast = (int)s->fdescs.size();
+    const size_t off_fdesc = align_up(off_adesc + (size_t)n_audio * sizeof(ArgusAudioDesc));
+    const size_t total = off_fdesc + (size_t)n_fast * sizeof(ArgusFastDesc);
     const char* d_before = sl.d_tab;
     ARGUS_CHECK((hipError_t)grow_table(sl, total, reuse ? items_bytes : 0));
     if (sl.d_tab != d_before) sl.items_dirty = true;  // a new device table holds no items yet
@@ -666,12 +708,15 @@ static int describe_block(gpud_serve* s, ServeSlot& sl, const std::vector<ServeO
     if (n_ditems) std::memcpy(sl.h_tab + off_ditems, s->ditems.data(), (size_t)n_ditems * sizeof(ArgusDemodItem));
     if (n_witems) std::memcpy(sl.h_tab + off_witems, s->witems.data(), (size_t)n_witems * sizeof(ArgusWfmItem));
     if (n_audio) std::memcpy(sl.h_tab + off_adesc, s->adescs.data(), (size_t)n_audio * sizeof(ArgusAudioDesc));
+    if (n_fast) std::memcpy(sl.h_tab + off_fdesc, s->fdescs.data(), (size_t)n_fast * sizeof(ArgusFastDesc));
     ArgusServeHdr& hdr = *(ArgusServeHdr*)(sl.h_tab + off_hdr);
     const char* d = sl.d_tab;
     hdr.x = ArgusExtractHdr{(const ArgusChanDesc*)(d + off_chans), (const ArgusWorkItem*)d, in, n_items, n};
     hdr.d = ArgusDemodHdr{(const ArgusDemodDesc*)(d + off_ddesc), (const ArgusDemodItem*)(d + off_ditems), n_ditems, 0};
     hdr.w = ArgusWfmHdr{(const ArgusWfmDesc*)(d + off_wdesc), (const ArgusWfmItem*)(d + off_witems), n_wfm, n_witems};
     hdr.a = ArgusAudioHdr{(const ArgusAudioDesc*)(d + off_adesc), n_audio, 0};
+    sl.fast_S = n_fast > 0 ? argus_fast_segments(n) : 0;
+    hdr.f = ArgusFastHdr{(const ArgusFastDesc*)(d + off_fdesc), s->d_ftail, s->d_fu, s->d_fX, n_fast, sl.fast_S};
     sl.a_mode = n_audio > 0 ? 1 : 0;
     sl.tab_bytes = total;
     sl.off_hdr = off_hdr;
@@ -686,10 +731,12 @@ static int describe_block(gpud_serve* s, ServeSlot& sl, const std::vector<ServeO
 static void commit_block(gpud_serve* s, const std::vector<ServeOut>& outs, int n, int64_t ticket)
 {
     int ci = 0;
+    bool warmed = false;
     for (auto& kv : s->live) {
         ServeChannel& c = kv.second;
         const ServeOut& o = outs[(size_t)ci++];
         const int keep = std::min(c.hist_len + n, c.T - 1);
+        if (c.d_fastG && c.hist_len < c.T - 1 && keep == c.T - 1) warmed = true;
         const double inc = -2.0 * M_PI * c.spec.offset_hz / (double)s->sample_rate;
         const int mode = c.stereo ? ARGUS_DEMOD_WFM : c.spec.demod;
         const double bfo_inc = 2.0 * M_PI * (mode == ARGUS_DEMOD_LSB ? -700.0 : 700.0) / (double)c.spec.out_rate;
@@ -712,6 +759,7 @@ static void commit_block(gpud_serve* s, const std::vector<ServeOut>& outs, int n
             if ((c.stereo ? o.n_mono : o.n_audio) > 0) c.a_sel ^= 1;
         }
     }
+    if (warmed && s->fast) s->epoch++;  // those channels' work items change (fast path from the next block)
 }
 
 // Layout, buffers and tables of a block in slot sl, built at submit time.
@@ -744,7 +792,12 @@ static int upload_table(gpud_serve* s, ServeSlot& sl)
 static int launch_stage(gpud_serve* s, ServeSlot& sl, int stage, bool wfm)
 {
     const ArgusServeHdr* dh = (const ArgusServeHdr*)(sl.d_tab + sl.off_hdr);
-    if (stage == 0) return argus_launch_extract_hdr(&dh->x, sl.lds, s->tune.grid, s->compute);
+    if (stage == 0) {
+        int err = argus_launch_fast(&dh->f, &dh->x, sl.fast_S, s->d_fu, s->d_fX, s->compute);
+        if (!err) err = argus_launch_extract_hdr(&dh->x, sl.lds, s->tune.grid, s->compute);
+        if (!err) err = argus_launch_fast_tail(&dh->f, &dh->x, s->compute);
+        return err;
+    }
     int err = argus_launch_demod_hdr(&dh->d, s->compute);
     if (err) return err;
     if (wfm) {
@@ -770,7 +823,7 @@ static int launch_block(gpud_serve* s, ServeSlot& sl)
     }
     const char* hdr = sl.d_tab + sl.off_hdr;
     if (!sl.gexec[0] || sl.g_hdr != hdr || sl.g_lds != sl.lds || sl.g_grid != s->tune.grid ||
-        sl.g_wfm != (int)wfm || sl.g_audio != sl.a_mode) {
+        sl.g_wfm != (int)wfm || sl.g_audio != sl.a_mode || sl.g_fastS != sl.fast_S) {
         for (auto& ge : sl.gexec)
             if (ge) {
                 hipGraphExecDestroy(ge);
@@ -799,6 +852,7 @@ static int launch_block(gpud_serve* s, ServeSlot& sl)
         sl.g_grid = s->tune.grid;
         sl.g_wfm = (int)wfm;
         sl.g_audio = sl.a_mode;
+        sl.g_fastS = sl.fast_S;
     }
     ARGUS_CHECK(hipGraphLaunch(sl.gexec[0], s->compute));
     ARGUS_CHECK(hipEventRecord(sl.ev_xend, s->compute));
@@ -1267,6 +1321,16 @@ GPUD_API int gpud_serve_channel_rds(gpud_serve* s, int64_t ticket, int64_t chann
             return 0;
         }
     return -1;
+}
+
+// The fast-convolution path for eligible channels (decimation 8, 16, 32 or
+// 64, at most 512 taps, full history): on by default.
+GPUD_API int gpud_serve_set_fast(gpud_serve* s, int enable)
+{
+    if (!s) return -1;
+    s->fast = enable != 0;
+    s->epoch++;
+    return 0;
 }
 
 GPUD_API int gpud_serve_set_graph(gpud_serve* s, int enable)

@@ -5,6 +5,7 @@
 #include "wfm.hpp"
 #include "audio.hpp"
 #include "rds.hpp"
+#include "fastconv.hpp"
 #include <map>
 #include <tuple>
 #include <vector>
@@ -47,6 +48,9 @@ struct ServeChannel {
     // ArgusRdsState | two decode windows of rds_win samples; the block's RDS
     // baseband fills window rwin_sel, a full window is decoded and the
     // other one fills next
+    // fast-convolution path (fastconv.hip): the transform of the channel's
+    // modulated taps, for channels whose decimation and length allow it
+    float2* d_fastG = nullptr;
     int rds_win = 0;         // 0: off
     char* d_rdec = nullptr;
     int rwin_sel = 0, rwin_fill = 0;
@@ -131,6 +135,8 @@ struct ServeSlot {
     size_t h_aout_cap = 0;
     int n_audio_chans = 0;
     int64_t aout_total = 0;
+    int fast_S = 0;          // segments of the fast path in the described block (0: none)
+    int g_fastS = -1;        // fast_S the extraction graph was captured for
     int a_mode = 0;          // the described block runs the audio stage (1) or not (0)
     int g_audio = -1;        // a_mode the stage graph was captured for
     // device-side RDS decoding of the block (after its graphs): a pinned
@@ -177,6 +183,7 @@ struct ArgusServeHdr {
     ArgusDemodHdr d;
     ArgusWfmHdr w;
     ArgusAudioHdr a;
+    ArgusFastHdr f;
 };
 
 struct gpud_serve {
@@ -206,6 +213,13 @@ struct gpud_serve {
     std::vector<ArgusDemodItem> ditems;
     std::vector<ArgusWfmItem> witems;
     int n_rds_on = 0;  // live channels with RDS decoding on
+    // fast-convolution path: on/off, the stream's last input samples, the
+    // staged segments and their transforms (sized for the largest block)
+    bool fast = true;
+    float2* d_ftail = nullptr;
+    float2* d_fu = nullptr;
+    float2* d_fX = nullptr;
+    std::vector<ArgusFastDesc> fdescs;
     std::vector<ArgusRdsAppend> rapp;
     std::vector<ArgusRdsCall> rcalls;
 };

@@ -15,7 +15,7 @@
 // HBM round trip.  Welch segments of one output frame are transformed back to
 // back by the same workgroup and their |X|^2 accumulates in registers.
 #include "common.hpp"
-#include "fft_twiddles.hpp"
+#include "fft_core.hpp"
 #include "surv.hpp"
 #include <algorithm>
 #include <cmath>
@@ -26,130 +26,8 @@ int argus_lowpass_fir(double cutoff_hz, int sample_rate, int taps, double* out, 
 
 namespace {
 
-template <int R>
-__host__ __device__ constexpr int brev(int i)
-{
-    int r = 0;
-    for (int b = 1; b < R; b <<= 1) {
-        r <<= 1;
-        if (i & b) r |= 1;
-    }
-    return r;
-}
+using namespace argus_fftc;
 
-// In-register forward DFT of R points (radix-2, decimation in time).
-template <int R>
-__device__ __forceinline__ void dft(float2 (&v)[R])
-{
-#pragma unroll
-    for (int i = 0; i < R; ++i) {
-        const int j = brev<R>(i);
-        if (i < j) {
-            const float2 t = v[i];
-            v[i] = v[j];
-            v[j] = t;
-        }
-    }
-#pragma unroll
-    for (int len = 2; len <= R; len <<= 1) {
-        const int half = len >> 1;
-#pragma unroll
-        for (int i = 0; i < R; i += len) {
-#pragma unroll
-            for (int k = 0; k < half; ++k) {
-                const float2 a = v[i + k], b = v[i + k + half];
-                float2 tb;
-                if (k == 0) {
-                    tb = b;
-                } else if (4 * k == len) {  // W = -i
-                    tb = make_float2(b.y, -b.x);
-                } else {
-                    const int wi = k * (32 / len);
-                    const float cr = argus_fft::W32_RE[wi], ci = argus_fft::W32_IM[wi];
-                    tb = make_float2(b.x * cr - b.y * ci, b.x * ci + b.y * cr);
-                }
-                v[i + k] = make_float2(a.x + tb.x, a.y + tb.y);
-                v[i + k + half] = make_float2(a.x - tb.x, a.y - tb.y);
-            }
-        }
-    }
-}
-
-// e^{-2 pi i e / M}, e in [0, M), M a power of two: exact turn argument.
-__device__ __forceinline__ float2 twiddle(int e, float inv_m)
-{
-    const float x = -(float)e * inv_m;
-    return make_float2(__builtin_amdgcn_cosf(x), __builtin_amdgcn_sinf(x));
-}
-
-__device__ __forceinline__ float2 cmul(float2 a, float2 w)
-{
-    return make_float2(a.x * w.x - a.y * w.y, a.x * w.y + a.y * w.x);
-}
-
-// Transpose through LDS: thread writes its R values to wr(k), barrier, reads
-// R values from rd(k); real and imaginary parts go in two rounds.
-template <int R, typename WR, typename RD>
-__device__ __forceinline__ void exchange(float2 (&v)[R], float* xs, WR wr, RD rd)
-{
-    // each half is read back into the registers it was written from, so the
-    // transpose needs no extra registers
-#pragma unroll
-    for (int k = 0; k < R; ++k) xs[wr(k)] = v[k].x;
-    __syncthreads();
-#pragma unroll
-    for (int k = 0; k < R; ++k) v[k].x = xs[rd(k)];
-    __syncthreads();
-#pragma unroll
-    for (int k = 0; k < R; ++k) xs[wr(k)] = v[k].y;
-    __syncthreads();
-#pragma unroll
-    for (int k = 0; k < R; ++k) v[k].y = xs[rd(k)];
-    __syncthreads();
-}
-
-// Full N = R^3 forward transform of the values each thread holds
-// (thread t = a + R b holds c = 0..R-1).  On return thread t = kb + R kc
-// holds X[kc + R kb + R^2 ka] in v[ka].
-template <int R>
-__device__ __forceinline__ void fft_r3(float2 (&v)[R], float* xs)
-{
-    constexpr int TH = R * R;
-    constexpr float invN = 1.0f / (float)(R * R * R);
-    constexpr float invRR = 1.0f / (float)(R * R);
-    int t = threadIdx.x;
-    // opaque to the optimiser: keeps the twiddles (a function of t only)
-    // from being hoisted out of a caller's segment loop into scratch memory
-    asm volatile("" : "+v"(t));
-    dft<R>(v);
-#pragma unroll
-    for (int k = 1; k < R; ++k) v[k] = cmul(v[k], twiddle(t * k, invN));
-    {
-        const int a = t % R, kc = t / R;
-        exchange<R>(v, xs, [&](int k) { return t + TH * k; }, [&](int b) { return a + R * b + TH * kc; });
-    }
-    dft<R>(v);
-    {
-        const int a = t % R, kc = t / R;
-#pragma unroll
-        for (int k = 1; k < R; ++k) v[k] = cmul(v[k], twiddle(a * k, invRR));
-        // padded layout a + (R+1) kb + R(R+1) kc: odd strides both ways, so
-        // neither the writes (lanes vary a) nor the reads (lanes vary kb)
-        // conflict, and every address is affine in the unrolled index
-        const int kb2 = t % R, kc2 = t / R;
-        exchange<R>(
-            v, xs, [&](int kb) { return a + (R + 1) * kb + R * (R + 1) * kc; },
-            [&](int aa) { return aa + (R + 1) * kb2 + R * (R + 1) * kc2; });
-    }
-    dft<R>(v);
-}
-
-// LDS floats the transforms need: the padded R x (R+1) x R image.
-template <int R>
-constexpr int fft_lds_floats()
-{
-    return R * R * (R + 1);
-}
 
 template <int R>
 __global__ __launch_bounds__(R* R) void argus_welch_kernel(const float2* __restrict__ in, int64_t frame_step,
@@ -176,7 +54,7 @@ __global__ __launch_bounds__(R* R) void argus_welch_kernel(const float2* __restr
             const float w = win[idx];
             v[c] = make_float2(s.x * w, s.y * w);
         }
-        fft_r3<R>(v, xs);
+        fft_r3<R>(v, xs, (int)threadIdx.x);
 #pragma unroll
         for (int k = 0; k < R; ++k) {
             const float mr = v[k].x * invN, mi = v[k].y * invN;
@@ -224,7 +102,7 @@ __global__ __launch_bounds__(R* R) void argus_fft_kernel(const float2* __restric
         }
         v[c] = s;
     }
-    fft_r3<R>(v, xs);
+    fft_r3<R>(v, xs, (int)threadIdx.x);
     const int kb = t % R, kc = t / R;
     float2* o = out + (int64_t)blockIdx.x * N;
     const int jc = t % R, jb = t / R;
@@ -632,6 +510,19 @@ GPUD_API int gpud_surv_fft_device(gpud_surv* s, const gpud_float2* in_dev, int64
                            (float2*)out_dev, frame_step);
     return (int)hipGetLastError();
 }
+
+}  // extern "C"
+
+int argus_launch_fft4096(const float2* in, int64_t frame_step, int n_frames, float2* out, hipStream_t stream)
+{
+    if (n_frames <= 0) return 0;
+    argus_clear_error();
+    hipLaunchKernelGGL(argus_fft_kernel<16>, dim3(n_frames), dim3(256), 0, stream, in, (const float*)nullptr, out,
+                       frame_step);
+    return (int)hipGetLastError();
+}
+
+extern "C" {
 
 // Single-FFT dB spectra of n_frames frames (frame f at f*frame_step): the
 // GPU path of spectrumFromIQWithPlan (pipeline_runtime.go:315-341) --

@@ -340,6 +340,13 @@ GPUD_API int gpud_serve_channel_audio(gpud_serve* s, int64_t ticket, int64_t cha
  * in flight; applies from the next submitted block.  0, -1 bad arguments,
  * unknown or non-stereo channel, GPUD_ERR_BUDGET if it does not fit. */
 GPUD_API int gpud_serve_set_rds(gpud_serve* s, int64_t channel_id, int window_samples);
+/* The fast-convolution form of the extraction (on by default): a channel of
+ * decimation 8, 16, 32 or 64 with at most 512 taps, once its history is
+ * full, is filtered through one shared 4096-point transform of each input
+ * segment (overlap-save, decimation by spectral folding) instead of the
+ * direct polyphase form; same filter, outputs within FP32 rounding of the
+ * direct form.  Applies from the next submitted block. */
+GPUD_API int gpud_serve_set_fast(gpud_serve* s, int enable);
 /* RDS decoding of a channel in a completed block: *decoded = 1 if a window
  * completed in this block and was decoded, with *found groups found and the
  * *n_groups groups it applied (4 x 16-bit blocks each, pinned host memory,
