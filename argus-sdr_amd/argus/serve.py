@@ -62,6 +62,15 @@ register("gpud_serve_channel_audio", I, [P, C.c_int64, C.c_int64, C.POINTER(P), 
                                          C.POINTER(I)])
 register("gpud_serve_set_rds", I, [P, C.c_int64, I])
 register("gpud_serve_set_fast", I, [P, I])
+register("gpud_fast_spectrum", I, [P, I, C.c_double, P])
+
+
+def FastSpectrum(taps, phase_inc: float) -> np.ndarray:
+    """The fast form's per-channel spectrum: FFT_4096(taps * e^{-j phase_inc k}) / 4096."""
+    h = np.ascontiguousarray(taps, np.float32)
+    out = np.zeros(4096, np.complex64)
+    check("gpud_fast_spectrum", lib().gpud_fast_spectrum(h.ctypes.data, len(h), phase_inc, out.ctypes.data))
+    return out
 register("gpud_serve_channel_rds", I, [P, C.c_int64, C.c_int64, C.POINTER(I), C.POINTER(I), C.POINTER(C.c_uint16),
                                        C.c_char_p, C.c_char_p, C.POINTER(P), C.POINTER(I)])
 

@@ -14,12 +14,15 @@
 //           (one launch of the surveillance 4096-point kernel for all s)
 //   channel G = FFT_4096(g) / 4096 (host, FP64, once per channel), times the
 //           decimation-phase ramp e^{j 2pi k r / 4096} (a shift by r samples);
-//           per segment, decimation by D in the frequency domain (folding),
-//           F[j] = sum_{i < D} X_s[j + i J] G'[j + i J],  J = 4096 / D,
+//           per segment, decimation by Dp (the power of two dividing D,
+//           at most 64) in the frequency domain (folding),
+//           F[j] = sum_{i < Dp} X_s[j + i J] G'[j + i J],  J = 4096 / Dp,
 //           and an inverse transform of J points gives the segment's
-//           convolution at u-positions q D + r, q < J.  Positions
-//           [511, 511 + 3584) are valid for T <= 512 (overlap-save), so each
-//           block sample belongs to exactly one segment.
+//           convolution at u-positions q Dp + r, q < J; the outputs are the
+//           positions among them that are emission points (every D/Dp-th).
+//           Positions [511, 511 + 3584) are valid for T <= 512
+//           (overlap-save), so each block sample belongs to exactly one
+//           segment.
 //   tail    the stream's last 511 samples after the block.
 //
 // The inverse transform is the 512-point FFT (8^3) of the conjugated folded
@@ -72,19 +75,19 @@ __global__ __launch_bounds__(kThreads) void argus_fast_stage_kernel(const ArgusF
     }
 }
 
-// F[j] = sum_{i < D} X[j + i J] G'[j + i J] for this lane's j = t + 64 c,
+// F[j] = sum_{i < DP} X[j + i J] G'[j + i J] for this lane's j = t + 64 c,
 // c < J / 64: every lane folds the same number of bins, all loads of one
 // i issued together.
-template <int D>
+template <int DP>
 __device__ __forceinline__ void fast_fold(const float2* __restrict__ xseg, const float2* gs, int t,
-                                          float2 (&f)[ARGUS_FAST_N / D / 64])
+                                          float2 (&f)[ARGUS_FAST_N / DP / 64])
 {
-    constexpr int J = ARGUS_FAST_N / D, PER = J / 64;
+    constexpr int J = ARGUS_FAST_N / DP, PER = J / 64;
     float ar[PER], ai[PER];
 #pragma unroll
     for (int c = 0; c < PER; c++) ar[c] = ai[c] = 0.0f;
 #pragma unroll
-    for (int i = 0; i < D; i++) {
+    for (int i = 0; i < DP; i++) {
         float2 xv[PER];
 #pragma unroll
         for (int c = 0; c < PER; c++) xv[c] = xseg[i * J + t + 64 * c];
@@ -103,19 +106,19 @@ __device__ __forceinline__ void fast_fold(const float2* __restrict__ xseg, const
 // outputs.  All four wavefronts of the workgroup call this the same number
 // of times (the transform's barriers are workgroup-wide); act = 0 does no
 // work but takes the barriers.
-template <int D>
+template <int DP>
 __device__ __forceinline__ void fast_segment(const ArgusFastDesc& d, const float2* __restrict__ X, const float2* gs,
                                              float* xs, int s, bool act, int t)
 {
-    constexpr int J = ARGUS_FAST_N / D, PER = J / 64, Z = kJ / J;  // Z: zero-stuffing of the 512-point input
+    constexpr int J = ARGUS_FAST_N / DP, PER = J / 64, Z = kJ / J;  // Z: zero-stuffing of the 512-point input
     float2 v[8];
     if constexpr (Z == 1) {
-        fast_fold<D>(X + (size_t)(act ? s : 0) * kN, gs, t, v);
+        fast_fold<DP>(X + (size_t)(act ? s : 0) * kN, gs, t, v);
 #pragma unroll
         for (int c = 0; c < 8; c++) v[c].y = -v[c].y;  // conjugate: the forward transform inverts
     } else {
         float2 f[PER];
-        fast_fold<D>(X + (size_t)(act ? s : 0) * kN, gs, t, f);
+        fast_fold<DP>(X + (size_t)(act ? s : 0) * kN, gs, t, f);
         float2* fs = (float2*)xs;  // J <= 256 bins fit the transform's scratch
 #pragma unroll
         for (int c = 0; c < PER; c++) fs[t + 64 * c] = f[c];
@@ -135,11 +138,13 @@ __device__ __forceinline__ void fast_segment(const ArgusFastDesc& d, const float
     for (int ka = 0; ka < 8; ka++) {
         const int q = kc + 8 * kb + 64 * ka;
         if (!act || q >= J) continue;
-        const int iseg = q * D + d.r;
+        const int iseg = q * DP + d.r;
         if (iseg < kTail || iseg >= kTail + kHop) continue;
-        const int e = s * kHop + iseg - kTail;  // block sample, = e0 mod D
-        const int m = (e - d.e0) / D;
-        if (m < 0 || m >= d.n_out) continue;
+        const int e = s * kHop + iseg - kTail;  // block sample, = e0 mod Dp
+        const int de = e - d.e0;
+        if (de < 0 || de % d.D != 0) continue;  // not an emission point (D > DP)
+        const int m = de / d.D;
+        if (m >= d.n_out) continue;
         const float2 y = make_float2(v[ka].x, -v[ka].y);
         float sn, cs;
         argus_nco(d.phase0 + d.inc * (uint64_t)(int64_t)e, sn, cs);
@@ -147,11 +152,11 @@ __device__ __forceinline__ void fast_segment(const ArgusFastDesc& d, const float
     }
 }
 
-template <int D>
+template <int DP>
 __device__ __forceinline__ void fast_channel(const ArgusFastDesc& d, const float2* __restrict__ X, const float2* gs,
                                              float* xs, int S, int w, int t)
 {
-    for (int s0 = 0; s0 < S; s0 += 4) fast_segment<D>(d, X, gs, xs, s0 + w, s0 + w < S, t);
+    for (int s0 = 0; s0 < S; s0 += 4) fast_segment<DP>(d, X, gs, xs, s0 + w, s0 + w < S, t);
 }
 
 __global__ __launch_bounds__(kThreads) void argus_fast_chan_kernel(const ArgusFastHdr* __restrict__ hdr)
@@ -172,7 +177,7 @@ __global__ __launch_bounds__(kThreads) void argus_fast_chan_kernel(const ArgusFa
             gs[k] = make_float2(g.x * cs - g.y * sn, g.x * sn + g.y * cs);
         }
         __syncthreads();
-        switch (d.D) {  // uniform over the workgroup (one channel)
+        switch (d.Dp) {  // uniform over the workgroup (one channel)
         case 8: fast_channel<8>(d, X, gs, xs[w], S, w, t); break;
         case 16: fast_channel<16>(d, X, gs, xs[w], S, w, t); break;
         case 32: fast_channel<32>(d, X, gs, xs[w], S, w, t); break;
@@ -195,10 +200,19 @@ __global__ __launch_bounds__(512) void argus_fast_tail_kernel(const ArgusFastHdr
     if (i < kTail) tail[i] = v;
 }
 
-// In-place radix-2 FFT in FP64 (host), forward, N a power of two.
+// In-place radix-2 FFT of ARGUS_FAST_N points in FP64 (host), forward; the
+// twiddles e^{-2 pi i k / N} are computed once.
 void fft64(std::vector<std::complex<double>>& a)
 {
-    const int n = (int)a.size();
+    static const std::vector<std::complex<double>> tw = [] {
+        std::vector<std::complex<double>> w((size_t)kN / 2);
+        for (int k = 0; k < kN / 2; k++) {
+            const double ang = -2.0 * M_PI * (double)k / (double)kN;
+            w[(size_t)k] = std::complex<double>(std::cos(ang), std::sin(ang));
+        }
+        return w;
+    }();
+    const int n = kN;
     for (int i = 1, j = 0; i < n; i++) {
         int bit = n >> 1;
         for (; j & bit; bit >>= 1) j ^= bit;
@@ -206,11 +220,10 @@ void fft64(std::vector<std::complex<double>>& a)
         if (i < j) std::swap(a[(size_t)i], a[(size_t)j]);
     }
     for (int len = 2; len <= n; len <<= 1) {
+        const int step = n / len;
         for (int i = 0; i < n; i += len) {
             for (int k = 0; k < len / 2; k++) {
-                const double ang = -2.0 * M_PI * (double)k / (double)len;
-                const std::complex<double> wk(std::cos(ang), std::sin(ang));
-                const std::complex<double> u = a[(size_t)(i + k)], v = a[(size_t)(i + k + len / 2)] * wk;
+                const std::complex<double> u = a[(size_t)(i + k)], v = a[(size_t)(i + k + len / 2)] * tw[(size_t)(k * step)];
                 a[(size_t)(i + k)] = u + v;
                 a[(size_t)(i + k + len / 2)] = u - v;
             }
@@ -255,4 +268,13 @@ int argus_launch_fast_tail(const ArgusFastHdr* d_hdr, const ArgusExtractHdr* d_x
     argus_clear_error();
     argus_fast_tail_kernel<<<1, 512, 0, stream>>>(d_hdr, d_xh);
     return (int)hipGetLastError();
+}
+
+// Host helper: argus_fast_spectrum for a phase increment in radians per
+// sample, converted as the service converts a channel's offset.
+extern "C" GPUD_API int gpud_fast_spectrum(const float* taps, int num_taps, double phase_inc, gpud_float2* G)
+{
+    if (!taps || !G || num_taps <= 0 || num_taps > ARGUS_FAST_TAIL + 1) return -1;
+    argus_fast_spectrum(taps, num_taps, argus_rad_to_fix(phase_inc), (float2*)G);
+    return 0;
 }

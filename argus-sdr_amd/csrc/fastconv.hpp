@@ -15,10 +15,12 @@ struct ArgusFastDesc {
     float2* out;       // n_out outputs of this block
     uint64_t phase0;   // NCO phase of block sample 0 (turns, 2^64 == 1)
     uint64_t inc;      // NCO increment per sample
-    int D;             // decimation (8, 16, 32 or 64)
+    int D;             // decimation, a multiple of 8
+    int Dp;            // folding factor: the power of two dividing D, at most 64
     int e0;            // block sample of output 0
     int n_out;         // outputs this block
-    int r;             // (e0 + ARGUS_FAST_TAIL) mod D
+    int r;             // (e0 + ARGUS_FAST_TAIL) mod Dp
+    int pad;
 };
 
 // What the fast-path kernels of one block read (device memory, so a
@@ -32,10 +34,19 @@ struct ArgusFastHdr {
     int S;             // segments this block (ceil(n / ARGUS_FAST_HOP))
 };
 
-// Whether a channel of decimation D and T taps can take the fast path.
+// Whether a channel of decimation D and T taps takes the fast path: a
+// multiple of 8 (so the fold factor, D's power-of-two part capped at 64,
+// leaves at most 512 bins), at most 512 taps, and overlapping windows
+// (D < T: the channels the direct form stages; where D >= T the direct
+// form reads each input once and is cheaper).
 static inline bool argus_fast_eligible(int D, int T)
 {
-    return (D == 8 || D == 16 || D == 32 || D == 64) && T >= 1 && T <= ARGUS_FAST_TAIL + 1;
+    return D >= 8 && D % 8 == 0 && D < T && T <= ARGUS_FAST_TAIL + 1;
+}
+static inline int argus_fast_fold(int D)
+{
+    const int p = D & -D;
+    return p > 64 ? 64 : p;
 }
 static inline int argus_fast_segments(int n) { return n > 0 ? (n + ARGUS_FAST_HOP - 1) / ARGUS_FAST_HOP : 0; }
 static inline size_t argus_fast_u_len(int segments)

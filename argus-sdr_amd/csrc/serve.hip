@@ -630,9 +630,11 @@ static int describe_block(gpud_serve* s, ServeSlot& sl, const std::vector<ServeO
             fd.phase0 = ch.phase0;
             fd.inc = ch.inc;
             fd.D = ch.D;
+            fd.Dp = argus_fast_fold(ch.D);
             fd.e0 = ch.e0;
             fd.n_out = ch.n_out;
-            fd.r = (ch.e0 + ARGUS_FAST_TAIL) % ch.D;
+            fd.r = (ch.e0 + ARGUS_FAST_TAIL) % fd.Dp;
+            fd.pad = 0;
             s->fdescs.push_back(fd);
         } else if ((c.decim - 1 + n) / c.decim > 0) {
             lds = std::max(lds, argus_item_lds_bytes(ch));

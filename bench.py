@@ -725,7 +725,7 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--channels", type=int, default=0,
                     help="fixed channel count (0: the p99 <= 5 ms capacity found by the sweep)")
-    ap.add_argument("--sweep-cap", type=int, default=32768)
+    ap.add_argument("--sweep-cap", type=int, default=65536)
     ap.add_argument("--plain-cap", type=int, default=512)
     ap.add_argument("--cpu-channels", type=int, default=32)
     ap.add_argument("--surv-frames", type=int, default=500)
@@ -876,7 +876,9 @@ def main() -> int:
         # flop/byte against a machine balance of ~20): its roof is FP32
         # arithmetic, computed on the vector ALUs (no matrix cores), against
         # MI355X's published FP32 peak.  The HBM figures sit beside it.
-        "roofline": {"kernel": "argus_extract_hdr_kernel", "bound": "mfma", "compute": "fp32 VALU",
+        "roofline": {"kernel": "extraction stage (argus_fast_chan_kernel + argus_extract_hdr_kernel + "
+                               "argus_fft_kernel<16> + argus_fast_stage/tail_kernel)",
+                     "bound": "mfma", "compute": "fp32 VALU",
                      "achieved": round(valu, 2), "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": round(valu / FP32_PEAK_TFLOPS, 4), "traffic": traffic, "traffic_profile": traffic_prof,
                      "kernel_ms": round(kern_ms, 4), "alg_flops": f_alg, "alg_bytes": b_alg,

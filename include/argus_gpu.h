@@ -347,6 +347,10 @@ GPUD_API int gpud_serve_set_rds(gpud_serve* s, int64_t channel_id, int window_sa
  * direct polyphase form; same filter, outputs within FP32 rounding of the
  * direct form.  Applies from the next submitted block. */
 GPUD_API int gpud_serve_set_fast(gpud_serve* s, int enable);
+/* Host helper of the fast form: the 4096 bins of FFT(g) / 4096 for
+ * g[k] = taps[k] e^{-j phase_inc k} (k < num_taps <= 512, natural order;
+ * FP64, rounded once), the spectrum the service keeps per channel. */
+GPUD_API int gpud_fast_spectrum(const float* taps, int num_taps, double phase_inc, gpud_float2* G);
 /* RDS decoding of a channel in a completed block: *decoded = 1 if a window
  * completed in this block and was decoded, with *found groups found and the
  * *n_groups groups it applied (4 x 16-bit blocks each, pinned host memory,

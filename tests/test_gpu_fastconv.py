@@ -1,6 +1,6 @@
 """The fast-convolution form of the service's extraction (fastconv.hip)
 against the oracle and against the direct form: channels of decimation 8,
-16, 32 and 64 with up to 511 taps over ragged blocks (shorter than the
+16, 32, 64, 40 and 80 with up to 511 taps over ragged blocks (shorter than the
 511-sample tail, one segment exactly, segment boundaries, empty), a channel
 opened mid-stream (direct form until its history is full, then fast), every
 demod mode, and ineligible channels beside them.  Baseband and audio within
@@ -25,7 +25,9 @@ SPECS = [  # id, kind, offset, bw, out_rate, taps, demod, window
     (4, "ssb", -150_000.0, 3_000, 40_000, 255, USB, 0),     # D 64
     (5, "fm", 700_000.0, 12_500, 256_000, 127, NFM, 0),     # D 10: direct form
     (6, "wfm", -800_000.0, 200_000, 320_000, 601, WFM, 0),  # 601 taps: direct form
-    (7, "tone", 950_000.0, 5_000, 160_000, 3, NONE, 0),     # three taps
+    (7, "tone", 950_000.0, 5_000, 160_000, 3, NONE, 0),     # D 16 >= T: direct form
+    (9, "fm", -250_000.0, 12_500, 64_000, 255, NFM, 0),     # D 40: fold by 8, every 5th
+    (10, "am", 450_000.0, 10_000, 32_000, 301, AM, 1),      # D 80: fold by 16, every 5th
 ]
 LATE = (8, "fm", 500_000.0, 12_500, 80_000, 201, NFM, 0)    # opened at block 3
 
@@ -87,4 +89,4 @@ def test_fast_form_matches_oracle_and_direct_form(serve_mod):
         d_iq = direct[cid][0]
         assert rel_rms(g_iq, d_iq) <= 1e-5, cid
         differ += int(not np.array_equal(g_iq, d_iq))
-    assert differ >= 4  # the eligible channels took the other form
+    assert differ >= 6  # the eligible channels took the other form
