@@ -7,6 +7,8 @@ stream_state.go, streaming_gpu_stub.go, polyphase.go, cpu_oracle.go):
     r = NewBatchRunner(max_samples, sample_rate)
     results = r.StreamingExtractGPU(iq_new, jobs)    # list[StreamingExtractResult]
     r.ResetSignalState(signal_id); r.ResetAllSignalStates(); r.Close()
+    outs, rates = r.ShiftFilterDecimateBatch(iq, extract_jobs)   # legacy block path
+    e = New(max_samples, sample_rate); e.Demod(iq, offset, bw, DemodNFM)
 
 All work runs in libargus_hip.so; this module only marshals arguments.
 """
@@ -17,12 +19,56 @@ from dataclasses import dataclass, field
 
 import numpy as np
 
-from ._lib import (ErrUnavailable, GpudError, check, gpud_streaming_job, gpud_streaming_result, lib, ptr)
+from ._lib import (ErrUnavailable, GpudError, check, gpud_streaming_job, gpud_streaming_result, lib, ptr, register)
 
 EXEC_BATCHED = 0
 EXEC_PER_SIGNAL = 1
 WINDOW_HANN = 0
 WINDOW_HAMMING = 1
+
+
+# gpudemod.go:81-90 DemodType
+DemodNFM, DemodWFM, DemodAM, DemodUSB, DemodLSB, DemodCW = range(6)
+
+
+class gpud_extract_job(C.Structure):
+    _fields_ = [("offset_hz", C.c_double), ("bw", C.c_double), ("out_rate", C.c_int32), ("pad", C.c_int32),
+                ("phase_start", C.c_double)]
+
+
+class gpud_extract_result(C.Structure):
+    _fields_ = [("rate", C.c_int32), ("n_out", C.c_int32), ("out_offset", C.c_int64), ("phase_end", C.c_double)]
+
+
+_P, _I, _D = C.c_void_p, C.c_int, C.c_double
+register("gpud_runner_shift_filter_decimate_batch", _I,
+         [_P, _P, _I, C.POINTER(gpud_extract_job), _I, _P, C.c_int64, C.POINTER(gpud_extract_result)])
+register("gpud_engine_create", _I, [_I, _I, C.POINTER(_P)])
+register("gpud_engine_destroy", _I, [_P])
+register("gpud_engine_set_fir", _I, [_P, _P, _I])
+register("gpud_engine_state", _I, [_P, C.POINTER(_D), C.POINTER(_D), C.POINTER(_I)])
+register("gpud_engine_shift_filter_decimate", _I, [_P, _P, _I, _D, _D, _I, _P, _I, C.POINTER(_I), C.POINTER(_I)])
+register("gpud_engine_demod", _I, [_P, _P, _I, _D, _D, _I, _I, _P, _I, C.POINTER(_I), C.POINTER(_I)])
+
+_LEGACY_ERRORS = {-2: "sample count exceeds engine capacity", -3: "invalid output sample rate",
+                  -4: "not enough output samples after decimation", -5: "unsupported demod type"}
+
+
+@dataclass
+class ExtractJob:
+    """batch.go:5-10"""
+    OffsetHz: float
+    BW: float
+    OutRate: int
+    PhaseStart: float = 0.0
+
+
+@dataclass
+class ExtractResult:
+    """batch.go:14-18"""
+    IQ: np.ndarray
+    Rate: int
+    PhaseEnd: float
 
 
 @dataclass
@@ -215,6 +261,105 @@ class BatchRunner:
         """stream_state.go:18-24"""
         check("gpud_runner_reset_all_signal_states", lib().gpud_runner_reset_all_signal_states(self._h))
 
+    # -- legacy block path ------------------------------------------------
+    def _sfd_batch(self, iq, jobs):
+        iq = np.ascontiguousarray(iq, np.complex64)
+        n = len(iq)
+        arr = (gpud_extract_job * max(1, len(jobs)))()
+        for i, j in enumerate(jobs):
+            arr[i] = gpud_extract_job(float(j.OffsetHz), float(j.BW), int(j.OutRate), 0, float(j.PhaseStart))
+        res = (gpud_extract_result * max(1, len(jobs)))()
+        out = np.zeros(max(1, n * max(1, len(jobs))), np.complex64)
+        rc = lib().gpud_runner_shift_filter_decimate_batch(self._h, ptr(iq) if n else None, n, arr, len(jobs),
+                                                           ptr(out), len(out), res)
+        if rc == -3:
+            raise ValueError("streaming polyphase extractor requires integer decimation")
+        if rc in _LEGACY_ERRORS:
+            raise ValueError(_LEGACY_ERRORS[rc])
+        check("gpud_runner_shift_filter_decimate_batch", rc)
+        return [(out[r.out_offset:r.out_offset + r.n_out].copy(), r.rate, r.phase_end) for r in res[:len(jobs)]]
+
+    def ShiftFilterDecimateBatch(self, iq, jobs):
+        """batch_runner.go:61-67: (outputs, rates)."""
+        got = self._sfd_batch(iq, jobs)
+        return [g[0] for g in got], [g[1] for g in got]
+
+    def ShiftFilterDecimateBatchWithPhase(self, iq, jobs) -> list[ExtractResult]:
+        """batch_runner.go:69-98"""
+        return [ExtractResult(*g) for g in self._sfd_batch(iq, jobs)]
+
+
+class Engine:
+    """gpudemod_windows.go Engine: carried shift and BFO phases, a FIR tap
+    set that sticks once designed or set, ShiftFilterDecimate, Demod and
+    DemodFused."""
+
+    def __init__(self, max_samples: int, sample_rate: int):
+        h = C.c_void_p()
+        rc = lib().gpud_engine_create(max_samples, sample_rate, C.byref(h))
+        if rc != 0:
+            raise ErrUnavailable(f"gpud_engine_create failed ({rc})")
+        self._h = h
+        self.max_samples = max_samples
+        self.sample_rate = sample_rate
+
+    def Close(self) -> None:
+        if self._h:
+            lib().gpud_engine_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.Close()
+        except Exception:
+            pass
+
+    def SetFIR(self, taps) -> None:
+        taps = np.ascontiguousarray(taps if taps is not None else np.zeros(0), np.float32)
+        check("gpud_engine_set_fir", lib().gpud_engine_set_fir(self._h, ptr(taps) if len(taps) else None, len(taps)))
+
+    def State(self):
+        """(shift phase, BFO phase, tap count)"""
+        ph, bfo, nt = C.c_double(), C.c_double(), C.c_int()
+        check("gpud_engine_state", lib().gpud_engine_state(self._h, C.byref(ph), C.byref(bfo), C.byref(nt)))
+        return ph.value, bfo.value, nt.value
+
+    def ShiftFilterDecimate(self, iq, offset_hz: float, bw: float, out_rate: int):
+        """gpudemod_windows.go:320-393: (IQ, rate)."""
+        iq = np.ascontiguousarray(iq, np.complex64)
+        out = np.zeros(max(1, len(iq)), np.complex64)
+        n_out, rate = C.c_int(), C.c_int()
+        rc = lib().gpud_engine_shift_filter_decimate(self._h, ptr(iq) if len(iq) else None, len(iq), offset_hz, bw,
+                                                     out_rate, ptr(out), len(out), C.byref(n_out), C.byref(rate))
+        if rc in _LEGACY_ERRORS:
+            raise ValueError(_LEGACY_ERRORS[rc])
+        check("gpud_engine_shift_filter_decimate", rc)
+        return out[:n_out.value].copy(), rate.value
+
+    def _demod(self, iq, offset_hz, bw, mode, fused):
+        iq = np.ascontiguousarray(iq, np.complex64)
+        out = np.zeros(max(1, len(iq)), np.float32)
+        n_out, rate = C.c_int(), C.c_int()
+        rc = lib().gpud_engine_demod(self._h, ptr(iq) if len(iq) else None, len(iq), offset_hz, bw, mode,
+                                     1 if fused else 0, ptr(out), len(out), C.byref(n_out), C.byref(rate))
+        if rc in _LEGACY_ERRORS:
+            raise ValueError(_LEGACY_ERRORS[rc])
+        check("gpud_engine_demod", rc)
+        return out[:n_out.value].copy(), rate.value
+
+    def Demod(self, iq, offset_hz: float, bw: float, mode: int):
+        """gpudemod_windows.go:523-654: (audio, rate)."""
+        return self._demod(iq, offset_hz, bw, mode, False)
+
+    def DemodFused(self, iq, offset_hz: float, bw: float, mode: int):
+        """gpudemod_windows.go:395-521: (audio, rate)."""
+        return self._demod(iq, offset_hz, bw, mode, True)
+
+
+def New(max_samples: int, sample_rate: int) -> Engine:
+    """gpudemod_windows.go:112-162"""
+    return Engine(max_samples, sample_rate)
+
 
 def NewBatchRunner(max_samples: int, sample_rate: int, **kw) -> BatchRunner:
     """batch_runner.go:23-34"""
@@ -225,5 +370,6 @@ __all__ = [
     "Available", "BatchRunner", "NewBatchRunner", "StreamingExtractJob", "StreamingExtractResult",
     "ExtractStreamState", "StreamingConfigHash", "ExactIntegerDecimation", "BuildPolyphaseTapsPhaseMajor",
     "PolyphasePhaseLen", "LowpassFIR", "ErrUnavailable", "GpudError", "EXEC_BATCHED", "EXEC_PER_SIGNAL",
-    "WINDOW_HANN", "WINDOW_HAMMING",
+    "WINDOW_HANN", "WINDOW_HAMMING", "ExtractJob", "ExtractResult", "Engine", "New", "DemodNFM", "DemodWFM",
+    "DemodAM", "DemodUSB", "DemodLSB", "DemodCW",
 ]

@@ -35,6 +35,14 @@ P, I = C.c_void_p, C.c_int
 register("gpud_serve_create", I, [I, I, C.c_uint64, C.POINTER(P)])
 register("gpud_serve_destroy", I, [P])
 register("gpud_serve_open", I, [P, C.POINTER(gpud_channel_spec)])
+
+
+class gpud_channel_opts(C.Structure):
+    _fields_ = [("demod_rate", C.c_int32), ("pad", C.c_int32 * 7)]
+
+
+register("gpud_serve_open_ex", I, [P, C.POINTER(gpud_channel_spec), C.POINTER(gpud_channel_opts)])
+register("gpud_serve_channel_predemod", I, [P, C.c_int64, C.c_int64, C.POINTER(P), C.POINTER(I), C.POINTER(I)])
 register("gpud_serve_close", I, [P, C.c_int64])
 register("gpud_serve_info", I, [P, C.POINTER(I), C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)])
 register("gpud_serve_submit", I, [P, P, I, C.POINTER(C.c_int64)])
@@ -109,10 +117,15 @@ class Serve:
         except Exception:
             pass
 
-    def Open(self, spec: ChannelSpec) -> bool:
-        """True if admitted, False if the HBM budget refuses it."""
+    def Open(self, spec: ChannelSpec, demod_rate: int = 0) -> bool:
+        """True if admitted, False if the HBM budget refuses it.  demod_rate > 0
+        adds the pre-demod decimating FIR to that rate (streamer.go:1098-1161)."""
         cs = spec.c()
-        rc = lib().gpud_serve_open(self._h, C.byref(cs))
+        if demod_rate:
+            op = gpud_channel_opts(int(demod_rate))
+            rc = lib().gpud_serve_open_ex(self._h, C.byref(cs), C.byref(op))
+        else:
+            rc = lib().gpud_serve_open(self._h, C.byref(cs))
         if rc == ERR_BUDGET:
             return False
         check("gpud_serve_open", rc)
@@ -162,6 +175,18 @@ class Serve:
         au = np.ctypeslib.as_array((C.c_float * n_a.value).from_address(aud_p.value)).copy() \
             if n_a.value else np.zeros(0, np.float32)
         return iq, au, rate.value
+
+    def OutputPreDemod(self, ticket: int, channel_id: int):
+        """(pre-demod stage output complex64 copy, its rate) of one channel,
+        or None when the channel has no such stage."""
+        p, n, rate = C.c_void_p(), C.c_int(), C.c_int()
+        rc = lib().gpud_serve_channel_predemod(self._h, ticket, channel_id, C.byref(p), C.byref(n), C.byref(rate))
+        if rc == -3:
+            return None
+        check("gpud_serve_channel_predemod", rc)
+        x = np.ctypeslib.as_array((C.c_float * (2 * n.value)).from_address(p.value)).view(np.complex64).copy() \
+            if n.value else np.zeros(0, np.complex64)
+        return x, rate.value
 
     def OutputWFM(self, ticket: int, channel_id: int):
         """(rds complex64 copy, rds_rate, locked, stereo) of a WFM stereo

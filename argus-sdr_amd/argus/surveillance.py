@@ -53,6 +53,19 @@ register("gpud_surv_detect", I, [P, P, L64, I, I, I, C.POINTER(gpud_cfar_params)
                                  P, I, P, P])
 
 
+class gpud_detector_config(C.Structure):
+    _fields_ = [("threshold_db", C.c_double), ("ema_alpha", C.c_double), ("hysteresis_db", C.c_double),
+                ("cfar_guard_hz", C.c_double), ("cfar_train_hz", C.c_double), ("cfar_scale_db", C.c_double),
+                ("edge_margin_db", C.c_double), ("max_signal_bw_hz", C.c_double), ("merge_gap_hz", C.c_double),
+                ("cfar_mode", C.c_int32), ("cfar_rank", C.c_int32), ("cfar_wrap", C.c_int32), ("pad", C.c_int32)]
+
+
+register("gpud_detector_settings", I, [C.POINTER(gpud_detector_config), I, I, C.POINTER(gpud_cfar_params),
+                                       C.POINTER(gpud_detector_params), C.POINTER(C.c_double), C.POINTER(I)])
+register("gpud_detector_adaptive_alpha", C.c_double, [C.c_double, C.c_double])
+register("gpud_surv_smooth_device", I, [P, P, I, P, I, P, P])
+
+
 @dataclass
 class Signal:
     FirstBin: int
@@ -254,3 +267,90 @@ class Surveillance:
             lists.append([Signal(s.first_bin, s.last_bin, s.peak_bin, s.center_hz, s.bw_hz, s.peak_db, s.snr_db,
                                  s.noise_db) for s in sig[f * cap:f * cap + min(int(counts[f]), cap)]])
         return lists, (psd.reshape(n_frames, self.n) if psd is not None else None)
+
+
+@dataclass
+class DetectorConfig:
+    """config.DetectorConfig (config.go:30-48): the fields the CFAR detection
+    path reads.  Defaults are the reference's config defaults
+    (config.go:468-481)."""
+    ThresholdDb: float = -20.0
+    EmaAlpha: float = 0.2
+    HysteresisDb: float = 3.0
+    CFARMode: str = "GOSCA"
+    CFARGuardHz: float = 500.0
+    CFARTrainHz: float = 5000.0
+    CFARRank: int = 36
+    CFARScaleDb: float = 6.0
+    CFARWrapAround: bool = True
+    EdgeMarginDb: float = 3.0
+    MaxSignalBwHz: float = 150000.0
+    MergeGapHz: float = 5000.0
+
+
+def DetectorSettings(cfg: DetectorConfig, sample_rate: int, fft_size: int):
+    """detector.New's mapping (detector.go:101-261): (cfar params, detector
+    params, EMA alpha, FFT bins)."""
+    c = gpud_detector_config(cfg.ThresholdDb, cfg.EmaAlpha, cfg.HysteresisDb, cfg.CFARGuardHz, cfg.CFARTrainHz,
+                             cfg.CFARScaleDb, cfg.EdgeMarginDb, cfg.MaxSignalBwHz, cfg.MergeGapHz,
+                             _MODES[cfg.CFARMode], cfg.CFARRank, 1 if cfg.CFARWrapAround else 0, 0)
+    cf, dp = gpud_cfar_params(), gpud_detector_params()
+    a, nb = C.c_double(), C.c_int()
+    check("gpud_detector_settings", lib().gpud_detector_settings(C.byref(c), sample_rate, fft_size, C.byref(cf),
+                                                                 C.byref(dp), C.byref(a), C.byref(nb)))
+    return cf, dp, a.value, nb.value
+
+
+def AdaptiveAlpha(ema_alpha: float, dt_seconds: float) -> float:
+    """detector.go:263-278"""
+    return float(lib().gpud_detector_adaptive_alpha(ema_alpha, dt_seconds))
+
+
+class Detector:
+    """The CFAR branch of detector.Process on the device for frames of one
+    size: EMA smoothing with the frame-rate-adaptive alpha (smoothSpectrum,
+    detector.go:806-817, state kept on the device), CFAR runs and the
+    detection list (detectSignals, detector.go:408-481) per frame."""
+
+    def __init__(self, cfg: DetectorConfig, sample_rate: int, fft_size: int, center_hz: float = 0.0):
+        self.cfar, self.det, self.alpha, n = DetectorSettings(cfg, sample_rate, fft_size)
+        self.det.center_hz = center_hz
+        self.surv = Surveillance(n, sample_rate)
+        self.n = n
+        self.ema = DeviceBuffer(8 * n)
+        self.first = True
+        self.last_t = None
+
+    def Smooth(self, psd, times=None) -> np.ndarray:
+        """Smoothed frames of psd [frames][n] (dB, host), frame f taken at
+        times[f] seconds (None: the reference's 1/15 s)."""
+        psd = np.ascontiguousarray(psd, np.float32).reshape(-1, self.n)
+        nf = psd.shape[0]
+        alphas = np.zeros(nf, np.float64)
+        for f in range(nf):
+            t = None if times is None else float(times[f])
+            dt = 0.0 if (t is None or self.last_t is None) else t - self.last_t
+            alphas[f] = AdaptiveAlpha(self.alpha, dt)
+            self.last_t = t
+        dpsd = DeviceBuffer(psd.nbytes).upload(psd)
+        dout = DeviceBuffer(psd.nbytes)
+        check("gpud_surv_smooth_device", lib().gpud_surv_smooth_device(self.surv._h, dpsd.ptr, nf, alphas.ctypes.data,
+                                                                       1 if self.first else 0, self.ema.ptr,
+                                                                       dout.ptr))
+        self.first = False
+        self.surv.sync()
+        return dout.download(np.float32, nf * self.n).reshape(nf, self.n)
+
+    def Process(self, psd, times=None, cap=1024):
+        """Smooth, then the detection list of each smoothed frame:
+        (lists, smoothed frames)."""
+        sm = self.Smooth(psd, times)
+        lists, _ = self.surv.DetectSpectra(sm, self.cfar, self.det, cap=cap)
+        return lists, sm
+
+    def EMA(self) -> np.ndarray:
+        return self.ema.download(np.float64, self.n)
+
+    def Close(self):
+        self.surv.Close()
+        self.ema.free()

@@ -100,6 +100,34 @@ int launch_fir(bool const_taps, const float2* in, float2* out, const float* taps
 
 }  // namespace
 
+// Stream-ordered launchers of the demodulation primitives, shared by the C
+// entry points below (default stream) and the legacy engine (its own stream).
+int argus_launch_fm_discrim(const float2* in, float* out, int n, hipStream_t st)
+{
+    if (n <= 1) return 0;
+    argus_clear_error();
+    hipLaunchKernelGGL(k_fm_discrim, dim3(grid_for(n)), dim3(kBlock), 0, st, in, out, n);
+    return (int)hipGetLastError();
+}
+
+int argus_launch_am_envelope(const float2* in, float* out, int n, hipStream_t st)
+{
+    if (n <= 0) return 0;
+    argus_clear_error();
+    hipLaunchKernelGGL(k_am_envelope, dim3(grid_for(n)), dim3(kBlock), 0, st, in, out, n);
+    return (int)hipGetLastError();
+}
+
+int argus_launch_ssb_product(const float2* in, float* out, int n, double phase_inc, double phase_start,
+                             hipStream_t st)
+{
+    if (n <= 0) return 0;
+    argus_clear_error();
+    hipLaunchKernelGGL(k_ssb_product, dim3(grid_for(n)), dim3(kBlock), 0, st, in, out, n,
+                       argus_rad_to_fix(phase_inc), argus_rad_to_fix(phase_start));
+    return (int)hipGetLastError();
+}
+
 extern "C" {
 
 GPUD_API const char* gpud_version(void) { return "argus-hip 0.1 (gfx950)"; }
@@ -201,10 +229,7 @@ GPUD_API int gpud_launch_freq_shift_hip(const gpud_float2* in, gpud_float2* out,
 }
 GPUD_API int gpud_launch_fm_discrim_hip(const gpud_float2* in, float* out, int n)
 {
-    if (n <= 1) return 0;
-    argus_clear_error();
-    hipLaunchKernelGGL(k_fm_discrim, dim3(grid_for(n)), dim3(kBlock), 0, 0, (const float2*)in, out, n);
-    return (int)hipGetLastError();
+    return argus_launch_fm_discrim((const float2*)in, out, n, 0);
 }
 GPUD_API int gpud_upload_fir_taps_hip(const float* taps, int n)
 {
@@ -241,19 +266,12 @@ GPUD_API int gpud_launch_decimate_hip(const gpud_float2* in, gpud_float2* out, i
 }
 GPUD_API int gpud_launch_am_envelope_hip(const gpud_float2* in, float* out, int n)
 {
-    if (n <= 0) return 0;
-    argus_clear_error();
-    hipLaunchKernelGGL(k_am_envelope, dim3(grid_for(n)), dim3(kBlock), 0, 0, (const float2*)in, out, n);
-    return (int)hipGetLastError();
+    return argus_launch_am_envelope((const float2*)in, out, n, 0);
 }
 GPUD_API int gpud_launch_ssb_product_hip(const gpud_float2* in, float* out, int n, double phase_inc,
                                          double phase_start)
 {
-    if (n <= 0) return 0;
-    argus_clear_error();
-    hipLaunchKernelGGL(k_ssb_product, dim3(grid_for(n)), dim3(kBlock), 0, 0, (const float2*)in, out, n,
-                       argus_rad_to_fix(phase_inc), argus_rad_to_fix(phase_start));
-    return (int)hipGetLastError();
+    return argus_launch_ssb_product((const float2*)in, out, n, phase_inc, phase_start, 0);
 }
 
 }  // extern "C"

@@ -311,6 +311,7 @@ GPUD_API int gpud_runner_destroy(gpud_runner* r)
         hipEventDestroy(r->stage_ev[i]);
     }
     for (hipEvent_t ev : r->tev) hipEventDestroy(ev);
+    argus_legacy_release(r->legacy);
     hipStreamDestroy(r->stream);
     delete r;
     return 0;

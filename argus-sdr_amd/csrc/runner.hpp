@@ -33,6 +33,9 @@ struct ArgusSignal {
     int out_cap = 0;
 };
 
+struct ArgusLegacyScratch;
+void argus_legacy_release(ArgusLegacyScratch* sc);
+
 struct gpud_runner {
     ArgusTune tune = argus_default_tune();
     int max_samples = 0;
@@ -66,6 +69,7 @@ struct gpud_runner {
     double t_ms = 0.0;
     int t_launches = 0;
     std::vector<gpud_float2> hist_host;
+    ArgusLegacyScratch* legacy = nullptr;  // staging of the legacy block path (engine.hip)
 };
 
 int argus_lowpass_fir(double cutoff_hz, int sample_rate, int taps, double* out, bool hamming);

@@ -90,9 +90,15 @@ uint64_t frac_to_fix(uint64_t num, uint64_t den)
 
 }  // namespace
 
+// Device block of a channel's pre-demod stage: FP64 taps, then two delay
+// lines of T-1 inputs.
+constexpr size_t kPreTaps = 0;
+constexpr size_t kPreHist = ARGUS_PRE_TAPS * sizeof(double);
+constexpr size_t kPreBytes = kPreHist + 2 * (ARGUS_PRE_TAPS - 1) * sizeof(float2);
+
 // Device bytes a channel holds while open (taps, modulated taps, ping-pong
 // history, FM carry) plus its share of the per-slot output buffers.
-static size_t channel_footprint(const gpud_serve* s, int D, int T, bool stereo)
+static size_t channel_footprint(const gpud_serve* s, int D, int T, bool stereo, int pre_factor)
 {
     const size_t L = (size_t)(T + D - 1) / D;
     const size_t taps = (size_t)D * L * sizeof(float) + (size_t)T * sizeof(float2);
@@ -102,6 +108,10 @@ static size_t channel_footprint(const gpud_serve* s, int D, int T, bool stereo)
     size_t per_block = n_out * (sizeof(float2) + sizeof(float));
     size_t fixed = taps + hist + carry;
     if (argus_fast_eligible(D, T)) fixed += ARGUS_FAST_N * sizeof(float2);
+    if (pre_factor > 1) {
+        fixed += kPreBytes;
+        per_block += (n_out / pre_factor + 1) * sizeof(float2);
+    }
     if (stereo) {
         // mono + 4 scratch floats + a second audio float + RDS samples (<= 1 per 2 outputs)
         per_block += n_out * (6 * sizeof(float)) + (n_out / 2 + 1) * sizeof(float2) + sizeof(ArgusWfmBlock);
@@ -121,6 +131,8 @@ static void release_channel(ServeChannel& c)
     if (c.d_aud) hipFree(c.d_aud);
     if (c.d_rdec) hipFree(c.d_rdec);
     if (c.d_fastG) hipFree(c.d_fastG);
+    if (c.d_pre) hipFree(c.d_pre);
+    c.d_pre = nullptr;
     c.d_aud = nullptr;
     c.d_rdec = nullptr;
     c.d_fastG = nullptr;
@@ -183,6 +195,10 @@ static void slot_release(ServeSlot& sl)
     hipFree(sl.d_aout);
     hipHostFree(sl.h_aout);
     sl.d_aout = sl.h_aout = nullptr;
+    hipFree(sl.d_pre);
+    hipHostFree(sl.h_pre);
+    sl.d_pre = sl.h_pre = nullptr;
+    sl.pre_cap = sl.h_pre_cap = 0;
     sl.aout_cap = sl.h_aout_cap = 0;
     hipHostFree(sl.h_rtab);
     hipFree(sl.d_rtab);
@@ -296,16 +312,26 @@ GPUD_API int gpud_serve_destroy(gpud_serve* s)
 // Admit a channel.  Returns 0, -1 on bad arguments or a non-integer
 // decimation (cpu_oracle.go:155-163), GPUD_ERR_DUPLICATE if the id is live,
 // GPUD_ERR_BUDGET if its device footprint does not fit the remaining budget.
-GPUD_API int gpud_serve_open(gpud_serve* s, const gpud_channel_spec* spec)
+GPUD_API int gpud_serve_open_ex(gpud_serve* s, const gpud_channel_spec* spec, const gpud_channel_opts* opts)
 {
     if (!s || !spec || !valid_mode(spec->demod) || (spec->window != 0 && spec->window != 1)) return -1;
     if (spec->out_rate <= 0 || s->sample_rate % spec->out_rate != 0) return -1;
+    if (opts && opts->demod_rate < 0) return -1;
     if (s->live.count(spec->channel_id)) return GPUD_ERR_DUPLICATE;
     const int D = s->sample_rate / spec->out_rate;
     const int T = spec->num_taps > 0 ? spec->num_taps : 101;
     if (T > ARGUS_MAX_TAPS) return -1;
     const bool stereo = spec->demod == GPUD_DEMOD_WFM_STEREO;
-    const size_t fp = channel_footprint(s, D, T, stereo);
+    // the pre-demod decimation (streamer.go:1098-1112): round(snipRate /
+    // demodRate), at least 1; WFM forced to 2 where that keeps >= 200 kHz
+    int pre = 1;
+    if (opts && opts->demod_rate > 0) {
+        pre = (int)std::lround((double)spec->out_rate / (double)opts->demod_rate);
+        if (pre < 1) pre = 1;
+        const bool wfm = spec->demod == GPUD_DEMOD_WFM || stereo;
+        if (wfm && pre > 2 && spec->out_rate / 2 >= 200000) pre = 2;
+    }
+    const size_t fp = channel_footprint(s, D, T, stereo, pre);
     reap(s);
     if (s->bytes_in_use + fp > s->budget) return GPUD_ERR_BUDGET;
 
@@ -316,6 +342,8 @@ GPUD_API int gpud_serve_open(gpud_serve* s, const gpud_channel_spec* spec)
     c.T = T;
     c.footprint = fp;
     c.stereo = stereo;
+    c.pre_factor = pre;
+    c.demod_rate = spec->out_rate / pre;
     double cutoff = spec->bandwidth / 2;
     if (cutoff < 200) cutoff = 200;
     std::vector<double> b64((size_t)T + 1);
@@ -351,7 +379,19 @@ GPUD_API int gpud_serve_open(gpud_serve* s, const gpud_channel_spec* spec)
     hipError_t e = hipMemcpy(c.d_taps, poly.data(), poly.size() * sizeof(float), hipMemcpyHostToDevice);
     if (e == hipSuccess)
         e = hipMemcpy(c.d_ctaps, ctaps.data(), ctaps.size() * sizeof(float2), hipMemcpyHostToDevice);
-    if (e == hipSuccess && stereo) e = wfm_channel_init(c, spec->out_rate);
+    if (e == hipSuccess && pre > 1) {
+        // LowpassFIR(cutoff, snipRate, 101) in FP64, cutoff 90 kHz for WFM,
+        // else 0.8 of the demod Nyquist (streamer.go:1114-1124); zero delay lines
+        const bool wfm = spec->demod == GPUD_DEMOD_WFM || stereo;
+        const double pc = wfm ? 90000.0 : (double)c.demod_rate / 2.0 * 0.8;
+        std::vector<char> blob(kPreBytes, 0);
+        std::vector<double> h(ARGUS_PRE_TAPS + 1);
+        argus_lowpass_fir(pc, spec->out_rate, ARGUS_PRE_TAPS, h.data(), false);
+        std::memcpy(blob.data() + kPreTaps, h.data(), ARGUS_PRE_TAPS * sizeof(double));
+        e = dev_alloc(&c.d_pre, kPreBytes);
+        if (e == hipSuccess) e = hipMemcpy(c.d_pre, blob.data(), kPreBytes, hipMemcpyHostToDevice);
+    }
+    if (e == hipSuccess && stereo) e = wfm_channel_init(c, c.demod_rate);
     if (e != hipSuccess) {
         release_channel(c);
         return (int)e;
@@ -360,6 +400,11 @@ GPUD_API int gpud_serve_open(gpud_serve* s, const gpud_channel_spec* spec)
     s->live.emplace(spec->channel_id, c);
     s->epoch++;
     return 0;
+}
+
+GPUD_API int gpud_serve_open(gpud_serve* s, const gpud_channel_spec* spec)
+{
+    return gpud_serve_open_ex(s, spec, nullptr);
 }
 
 // Retire a channel: it is absent from the next submitted block; its memory is
@@ -393,7 +438,7 @@ GPUD_API int gpud_serve_info(gpud_serve* s, int* live_channels, uint64_t* bytes_
 static void wfm_describe(const ServeChannel& c, const ServeOut& o, ServeSlot& sl, ArgusWfmDesc& wd,
                          std::vector<ArgusWfmItem>& items)
 {
-    const int rate = c.spec.out_rate;
+    const int rate = c.demod_rate;
     const int D = c.rds_decim;
     wd.mono = sl.d_mono + o.mono_off;
     wd.audio = sl.d_audio + o.audio_off;
@@ -480,15 +525,24 @@ static void layout_block(const gpud_serve* s, int n, BlockLayout& L)
 {
     L.outs.clear();
     L.outs.reserve(s->live.size());
-    L.iq_total = L.audio_total = L.mono_total = L.rds_total = L.aout_total = 0;
-    L.n_wfm = L.n_audio_chans = 0;
+    L.iq_total = L.audio_total = L.mono_total = L.rds_total = L.aout_total = L.pre_total = 0;
+    L.n_wfm = L.n_audio_chans = L.n_pre_chans = 0;
     for (const auto& kv : s->live) {
         const ServeChannel& c = kv.second;
         ServeOut o;
         o.channel_id = kv.first;
         o.n_iq = (c.phase_count + n) / c.decim;
         o.iq_off = L.iq_total;
-        o.n_audio = argus_demod_count(c.spec.demod, o.n_iq, c.have_carry);
+        // the demodulator's input: the pre-demod stage's outputs, or the baseband
+        o.n_pre = o.n_iq;
+        o.pre_off = -1;
+        if (c.pre_factor > 1) {
+            argus_pre_block(c.pre_phase, c.pre_factor, o.n_iq, &o.n_pre, nullptr);
+            o.pre_off = L.pre_total;
+            L.pre_total += o.n_pre;
+            L.n_pre_chans++;
+        }
+        o.n_audio = argus_demod_count(c.spec.demod, o.n_pre, c.have_carry);
         o.audio_off = L.audio_total;
         o.rate = c.spec.out_rate;
         o.wfm_index = -1;
@@ -502,14 +556,14 @@ static void layout_block(const gpud_serve* s, int n, BlockLayout& L)
         o.r_on = c.rds_win > 0 ? 1 : 0;
         o.r_index = -1;
         if (c.stereo) {
-            o.n_mono = argus_demod_count(ARGUS_DEMOD_WFM, o.n_iq, c.have_carry);
+            o.n_mono = argus_demod_count(ARGUS_DEMOD_WFM, o.n_pre, c.have_carry);
             o.n_audio = 2 * o.n_mono;
             o.wfm_index = L.n_wfm++;
             o.mono_off = L.mono_total;
             const int first = (int)((c.rds_decim - c.rds_count % c.rds_decim) % c.rds_decim);
             o.n_rds = o.n_mono > first ? (o.n_mono - 1 - first) / c.rds_decim + 1 : 0;
             o.rds_off = L.rds_total;
-            o.rds_rate = c.spec.out_rate / c.rds_decim;
+            o.rds_rate = c.demod_rate / c.rds_decim;
             L.mono_total += o.n_mono;
             L.rds_total += o.n_rds;
         }
@@ -541,6 +595,7 @@ static bool slot_fits(const ServeSlot& sl, const BlockLayout& L)
              fits(sl.h_wblk_cap, L.n_wfm);
     if (L.n_audio_chans > 0)
         ok = ok && fits(sl.aout_cap, L.aout_total) && fits(sl.h_aout_cap, L.aout_total);
+    if (L.n_pre_chans > 0) ok = ok && fits(sl.pre_cap, L.pre_total) && fits(sl.h_pre_cap, L.pre_total);
     return ok;
 }
 
@@ -564,6 +619,10 @@ static int slot_grow(ServeSlot& sl, const BlockLayout& L)
         ARGUS_CHECK((hipError_t)grow_dev(&sl.d_aout, &sl.aout_cap, (size_t)std::max<int64_t>(L.aout_total, 1)));
         ARGUS_CHECK((hipError_t)grow_host(&sl.h_aout, &sl.h_aout_cap, (size_t)std::max<int64_t>(L.aout_total, 1)));
     }
+    if (L.n_pre_chans > 0) {
+        ARGUS_CHECK((hipError_t)grow_dev(&sl.d_pre, &sl.pre_cap, (size_t)std::max<int64_t>(L.pre_total, 1)));
+        ARGUS_CHECK((hipError_t)grow_host(&sl.h_pre, &sl.h_pre_cap, (size_t)std::max<int64_t>(L.pre_total, 1)));
+    }
     return 0;
 }
 
@@ -577,6 +636,8 @@ static void slot_adopt(ServeSlot& sl, BlockLayout& L)
     sl.audio_total = L.audio_total;
     sl.n_audio_chans = L.n_audio_chans;
     sl.aout_total = L.aout_total;
+    sl.pre_total = L.pre_total;
+    sl.n_pre_chans = L.n_pre_chans;
 }
 
 // The block's tables for the live channels' current state, written into the
@@ -608,6 +669,8 @@ static int describe_block(gpud_serve* s, ServeSlot& sl, const std::vector<ServeO
     for (const ServeOut& o : outs) n_audio += o.a_index >= 0 ? 1 : 0;
     s->adescs.resize((size_t)n_audio);
     s->fdescs.clear();
+    s->pdescs.clear();
+    s->pitems.clear();
     size_t lds = 0;
     int ci = 0;
     for (const auto& kv : s->live) {
@@ -645,25 +708,46 @@ static int describe_block(gpud_serve* s, ServeSlot& sl, const std::vector<ServeO
             argus_append_items(full, ci, n, s->items, s->item_pos);
         }
 
+        if (c.pre_factor > 1) {  // pre-demod stage: baseband -> demodulator input
+            ArgusPreDesc pd;
+            double* taps = (double*)(c.d_pre + kPreTaps);
+            float2* hist = (float2*)(c.d_pre + kPreHist);
+            pd.in = sl.d_iq + o.iq_off;
+            pd.out = sl.d_pre + o.pre_off;
+            pd.taps = taps;
+            pd.hist_rd = hist + (size_t)c.pre_sel * (ARGUS_PRE_TAPS - 1);
+            pd.hist_wr = hist + (size_t)(c.pre_sel ^ 1) * (ARGUS_PRE_TAPS - 1);
+            pd.n_in = o.n_iq;
+            pd.n_out = o.n_pre;
+            pd.first = c.pre_phase;
+            pd.factor = c.pre_factor;
+            pd.T = ARGUS_PRE_TAPS;
+            pd.pad = 0;
+            const int pi = (int)s->pdescs.size();
+            s->pdescs.push_back(pd);
+            for (int st = 0; st < o.n_pre; st += ARGUS_PRE_TILE)
+                s->pitems.push_back({pi, st, std::min(ARGUS_PRE_TILE, o.n_pre - st), 0});
+            if (o.n_iq > 0) s->pitems.push_back({pi, 0, 0, 1});
+        }
         ArgusDemodDesc& dd = s->ddescs[(size_t)ci];
-        dd.iq = sl.d_iq + o.iq_off;
+        dd.iq = c.pre_factor > 1 ? sl.d_pre + o.pre_off : sl.d_iq + o.iq_off;
         dd.audio = c.stereo ? sl.d_mono + o.mono_off : sl.d_audio + o.audio_off;
         dd.carry_rd = c.d_carry + c.carry_sel;
         dd.carry_wr = c.d_carry + (c.carry_sel ^ 1);
-        dd.n = o.n_iq;
+        dd.n = o.n_pre;
         dd.mode = c.stereo ? ARGUS_DEMOD_WFM : c.spec.demod;
         dd.have_carry = c.have_carry ? 1 : 0;
         dd.skip = (dd.mode == ARGUS_DEMOD_NFM || dd.mode == ARGUS_DEMOD_WFM) && !c.have_carry ? 1 : 0;
         const double bfo_hz = dd.mode == ARGUS_DEMOD_LSB ? -700.0 : 700.0;
-        const double bfo_inc = 2.0 * M_PI * bfo_hz / (double)c.spec.out_rate;
+        const double bfo_inc = 2.0 * M_PI * bfo_hz / (double)c.demod_rate;
         dd.bfo0 = argus_rad_to_fix(c.bfo_phase + bfo_inc);
         dd.bfo_inc = argus_rad_to_fix(bfo_inc);
-        if (dd.mode != ARGUS_DEMOD_NONE && o.n_iq > 0) {
+        if (dd.mode != ARGUS_DEMOD_NONE && o.n_pre > 0) {
             if (dd.mode == ARGUS_DEMOD_AM) {
-                s->ditems.push_back({ci, 0, o.n_iq, 0});
+                s->ditems.push_back({ci, 0, o.n_pre, 0});
             } else {
-                for (int st = 0; st < o.n_iq; st += ARGUS_DEMOD_TILE)
-                    s->ditems.push_back({ci, st, std::min(ARGUS_DEMOD_TILE, o.n_iq - st), 0});
+                for (int st = 0; st < o.n_pre; st += ARGUS_DEMOD_TILE)
+                    s->ditems.push_back({ci, st, std::min(ARGUS_DEMOD_TILE, o.n_pre - st), 0});
             }
         }
         if (c.stereo) {
@@ -691,7 +775,10 @@ static int describe_block(gpud_serve* s, ServeSlot& sl, const std::vector<ServeO
     const size_t off_adesc = align_up(off_witems + (size_t)n_witems * sizeof(ArgusWfmItem));
     const int n_fast = (int)s->fdescs.size();
     const size_t off_fdesc = align_up(off_adesc + (size_t)n_audio * sizeof(ArgusAudioDesc));
-    const size_t total = off_fdesc + (size_t)n_fast * sizeof(ArgusFastDesc);
+    const int n_pre = (int)s->pdescs.size(), n_pitems = (int)s->pitems.size();
+    const size_t off_pdesc = align_up(off_fdesc + (size_t)n_fast * sizeof(ArgusFastDesc));
+    const size_t off_pitems = align_up(off_pdesc + (size_t)n_pre * sizeof(ArgusPreDesc));
+    const size_t total = off_pitems + (size_t)n_pitems * sizeof(ArgusPreItem);
     const char* d_before = sl.d_tab;
     ARGUS_CHECK((hipError_t)grow_table(sl, total, reuse ? items_bytes : 0));
     if (sl.d_tab != d_before) sl.items_dirty = true;  // a new device table holds no items yet
@@ -711,6 +798,8 @@ static int describe_block(gpud_serve* s, ServeSlot& sl, const std::vector<ServeO
     if (n_witems) std::memcpy(sl.h_tab + off_witems, s->witems.data(), (size_t)n_witems * sizeof(ArgusWfmItem));
     if (n_audio) std::memcpy(sl.h_tab + off_adesc, s->adescs.data(), (size_t)n_audio * sizeof(ArgusAudioDesc));
     if (n_fast) std::memcpy(sl.h_tab + off_fdesc, s->fdescs.data(), (size_t)n_fast * sizeof(ArgusFastDesc));
+    if (n_pre) std::memcpy(sl.h_tab + off_pdesc, s->pdescs.data(), (size_t)n_pre * sizeof(ArgusPreDesc));
+    if (n_pitems) std::memcpy(sl.h_tab + off_pitems, s->pitems.data(), (size_t)n_pitems * sizeof(ArgusPreItem));
     ArgusServeHdr& hdr = *(ArgusServeHdr*)(sl.h_tab + off_hdr);
     const char* d = sl.d_tab;
     hdr.x = ArgusExtractHdr{(const ArgusChanDesc*)(d + off_chans), (const ArgusWorkItem*)d, in, n_items, n};
@@ -720,6 +809,8 @@ static int describe_block(gpud_serve* s, ServeSlot& sl, const std::vector<ServeO
     sl.fast_S = n_fast > 0 ? argus_fast_segments(n) : 0;
     hdr.f = ArgusFastHdr{(const ArgusFastDesc*)(d + off_fdesc), s->d_ftail, s->d_fu, s->d_fX, n_fast, sl.fast_S};
     sl.a_mode = n_audio > 0 ? 1 : 0;
+    hdr.p = ArgusPreHdr{(const ArgusPreDesc*)(d + off_pdesc), (const ArgusPreItem*)(d + off_pitems), n_pitems, 0};
+    sl.p_mode = n_pitems > 0 ? 1 : 0;
     sl.tab_bytes = total;
     sl.off_hdr = off_hdr;
     sl.lds = lds;
@@ -741,16 +832,20 @@ static void commit_block(gpud_serve* s, const std::vector<ServeOut>& outs, int n
         if (c.d_fastG && c.hist_len < c.T - 1 && keep == c.T - 1) warmed = true;
         const double inc = -2.0 * M_PI * c.spec.offset_hz / (double)s->sample_rate;
         const int mode = c.stereo ? ARGUS_DEMOD_WFM : c.spec.demod;
-        const double bfo_inc = 2.0 * M_PI * (mode == ARGUS_DEMOD_LSB ? -700.0 : 700.0) / (double)c.spec.out_rate;
+        const double bfo_inc = 2.0 * M_PI * (mode == ARGUS_DEMOD_LSB ? -700.0 : 700.0) / (double)c.demod_rate;
         c.phase_count = (c.phase_count + n) % c.decim;
         c.nco_phase = argus_reduce_phase(c.nco_phase + inc * (double)n);
         if (keep > 0) c.hist_sel ^= 1;
         c.hist_len = keep;
-        if ((mode == ARGUS_DEMOD_NFM || mode == ARGUS_DEMOD_WFM) && o.n_iq > 0) {
+        if (c.pre_factor > 1) {
+            argus_pre_block(c.pre_phase, c.pre_factor, o.n_iq, nullptr, &c.pre_phase);
+            if (o.n_iq > 0) c.pre_sel ^= 1;
+        }
+        if ((mode == ARGUS_DEMOD_NFM || mode == ARGUS_DEMOD_WFM) && o.n_pre > 0) {
             c.carry_sel ^= 1;
             c.have_carry = true;
         }
-        c.bfo_phase = argus_reduce_phase(c.bfo_phase + bfo_inc * (double)o.n_iq);
+        c.bfo_phase = argus_reduce_phase(c.bfo_phase + bfo_inc * (double)o.n_pre);
         c.last_ticket = ticket;
         if (c.stereo) {
             if (o.n_mono > 0) c.wfm_sel ^= 1;
@@ -800,7 +895,9 @@ static int launch_stage(gpud_serve* s, ServeSlot& sl, int stage, bool wfm)
         if (!err) err = argus_launch_fast_tail(&dh->f, &dh->x, s->compute);
         return err;
     }
-    int err = argus_launch_demod_hdr(&dh->d, s->compute);
+    int err = sl.p_mode ? argus_launch_predemod_hdr(&dh->p, s->compute) : 0;
+    if (err) return err;
+    err = argus_launch_demod_hdr(&dh->d, s->compute);
     if (err) return err;
     if (wfm) {
         err = argus_launch_wfm_hdr(&dh->w, s->compute);
@@ -825,7 +922,7 @@ static int launch_block(gpud_serve* s, ServeSlot& sl)
     }
     const char* hdr = sl.d_tab + sl.off_hdr;
     if (!sl.gexec[0] || sl.g_hdr != hdr || sl.g_lds != sl.lds || sl.g_grid != s->tune.grid ||
-        sl.g_wfm != (int)wfm || sl.g_audio != sl.a_mode || sl.g_fastS != sl.fast_S) {
+        sl.g_wfm != (int)wfm || sl.g_audio != sl.a_mode || sl.g_fastS != sl.fast_S || sl.g_pre != sl.p_mode) {
         for (auto& ge : sl.gexec)
             if (ge) {
                 hipGraphExecDestroy(ge);
@@ -855,6 +952,7 @@ static int launch_block(gpud_serve* s, ServeSlot& sl)
         sl.g_wfm = (int)wfm;
         sl.g_audio = sl.a_mode;
         sl.g_fastS = sl.fast_S;
+        sl.g_pre = sl.p_mode;
     }
     ARGUS_CHECK(hipGraphLaunch(sl.gexec[0], s->compute));
     ARGUS_CHECK(hipEventRecord(sl.ev_xend, s->compute));
@@ -872,7 +970,7 @@ struct RdsLayout {
 static RdsLayout rds_layout(const ServeChannel& c, int W)
 {
     RdsLayout l{};
-    argus_rds_caps(W, c.spec.out_rate / c.rds_decim, &l.bit_cap, &l.group_cap);
+    argus_rds_caps(W, c.demod_rate / c.rds_decim, &l.bit_cap, &l.group_cap);
     l.off_pr = align_up(sizeof(ArgusRdsState));
     l.off_bits = l.off_pr + align_up(sizeof(ArgusRdsProgress));
     l.off_flags = l.off_bits + align_up((size_t)l.bit_cap);
@@ -1072,6 +1170,9 @@ static int serve_submit(gpud_serve* s, const gpud_float2* iq, int n, bool on_dev
         if (sl.aout_total > 0)
             ARGUS_CHECK(hipMemcpyAsync(sl.h_aout, sl.d_aout, (size_t)sl.aout_total * sizeof(float),
                                        hipMemcpyDeviceToHost, s->copy));
+        if (sl.pre_total > 0)
+            ARGUS_CHECK(hipMemcpyAsync(sl.h_pre, sl.d_pre, (size_t)sl.pre_total * sizeof(float2),
+                                       hipMemcpyDeviceToHost, s->copy));
     }
     if (sl.n_rdec > 0)  // decoded RDS groups come back whether or not outputs do
         ARGUS_CHECK(hipMemcpyAsync(sl.h_rtab + sl.off_rout, sl.d_rtab + sl.off_rout, sl.rgroups_bytes,
@@ -1156,6 +1257,24 @@ GPUD_API int gpud_serve_channel_output(gpud_serve* s, int64_t ticket, int64_t ch
     return -1;
 }
 
+GPUD_API int gpud_serve_channel_predemod(gpud_serve* s, int64_t ticket, int64_t channel_id, const gpud_float2** iq,
+                                         int* n, int* rate)
+{
+    if (!s) return -1;
+    ServeSlot* sl = find_slot(s, ticket);
+    if (!sl || sl->busy) return -2;
+    for (const ServeOut& o : sl->outs)
+        if (o.channel_id == channel_id) {
+            if (o.pre_off < 0) return -3;
+            auto it = s->live.find(channel_id);
+            if (iq) *iq = (const gpud_float2*)(sl->h_pre + o.pre_off);
+            if (n) *n = o.n_pre;
+            if (rate) *rate = it != s->live.end() ? it->second.demod_rate : 0;
+            return 0;
+        }
+    return -1;
+}
+
 GPUD_API int gpud_serve_channel_audio(gpud_serve* s, int64_t ticket, int64_t channel_id, const float** audio,
                                       int* n_frames, int* channels, int* rate)
 {
@@ -1208,14 +1327,14 @@ GPUD_API int gpud_serve_set_audio(gpud_serve* s, int64_t channel_id, int audio_r
     // stereo stage's frames already are
     const bool lowpass = c.spec.demod == GPUD_DEMOD_WFM;
     int L = 1, M = 1, T = 1;
-    if (argus_audio_filter(c.spec.out_rate, audio_rate, lowpass, &L, &M, &T, nullptr) != 0) return -1;
+    if (argus_audio_filter(c.demod_rate, audio_rate, lowpass, &L, &M, &T, nullptr) != 0) return -1;
     const size_t fp = audio_footprint(s, c, L, M);
     if (s->bytes_in_use - old_fp + fp > s->budget) return GPUD_ERR_BUDGET;
-    const auto key = std::make_tuple(c.spec.out_rate, audio_rate, lowpass);
+    const auto key = std::make_tuple(c.demod_rate, audio_rate, lowpass);
     auto fi = s->filters.find(key);
     if (fi == s->filters.end()) {
         std::vector<float> h((size_t)L * T);
-        if (argus_audio_filter(c.spec.out_rate, audio_rate, lowpass, nullptr, nullptr, nullptr, h.data()) != 0)
+        if (argus_audio_filter(c.demod_rate, audio_rate, lowpass, nullptr, nullptr, nullptr, h.data()) != 0)
             return -1;
         float* d = nullptr;
         ARGUS_CHECK(dev_alloc(&d, h.size()));
@@ -1262,7 +1381,7 @@ GPUD_API int gpud_serve_set_rds(gpud_serve* s, int64_t channel_id, int window_sa
     if (it == s->live.end() || !it->second.stereo) return -1;
     ServeChannel& c = it->second;
     // the most RDS samples one block can carry, and a window that holds a group
-    const int per_block = (s->block_samples / c.decim + 2) / c.rds_decim + 2;
+    const int per_block = (s->block_samples / c.decim / c.pre_factor + 2) / c.rds_decim + 2;
     if (window_samples > 0 && window_samples < std::max(per_block, 104)) return -1;
     if ((int64_t)window_samples * 2 * (int64_t)sizeof(float2) > ((int64_t)1 << 31)) return -1;
     for (auto& sl : s->slots) {  // blocks in flight read and write the decoder's state

@@ -6,6 +6,7 @@
 #include "audio.hpp"
 #include "rds.hpp"
 #include "fastconv.hpp"
+#include "predemod.hpp"
 #include <map>
 #include <tuple>
 #include <vector>
@@ -56,6 +57,14 @@ struct ServeChannel {
     int rwin_sel = 0, rwin_fill = 0;
     ArgusRdsOut r_last{};    // state after the latest completed decode
     int64_t r_last_ticket = 0;
+    // pre-demod decimating FIR (gpud_serve_open_ex with a demod rate): one
+    // device block of FP64 taps [ARGUS_PRE_TAPS] | delay lines [2][T-1]
+    // (ping-pong); pre_phase counts the inputs until its next output
+    int pre_factor = 1;      // 1: no stage
+    int pre_phase = 0;
+    int pre_sel = 0;
+    char* d_pre = nullptr;
+    int demod_rate = 0;      // rate the demodulator (and stereo, RDS, audio) runs at
 };
 
 struct ServeOut {
@@ -71,6 +80,8 @@ struct ServeOut {
     int64_t aout_off;                 // audio output offset (floats)
     int r_on;                         // RDS decoding on for the channel in this block
     int r_index;                      // index of its decode among the block's, or -1
+    int n_pre;                        // demodulator input samples (the pre-demod stage's outputs, or n_iq)
+    int64_t pre_off;                  // offset of the pre-demod outputs, -1 without the stage
 };
 
 struct ServeSlot {
@@ -135,6 +146,15 @@ struct ServeSlot {
     size_t h_aout_cap = 0;
     int n_audio_chans = 0;
     int64_t aout_total = 0;
+    // pre-demod stage outputs
+    float2* d_pre = nullptr;
+    size_t pre_cap = 0;
+    float2* h_pre = nullptr;
+    size_t h_pre_cap = 0;
+    int64_t pre_total = 0;
+    int n_pre_chans = 0;
+    int p_mode = 0;          // the described block runs the pre-demod stage (1) or not (0)
+    int g_pre = -1;          // p_mode the stage graph was captured for
     int fast_S = 0;          // segments of the fast path in the described block (0: none)
     int g_fastS = -1;        // fast_S the extraction graph was captured for
     int a_mode = 0;          // the described block runs the audio stage (1) or not (0)
@@ -159,7 +179,8 @@ struct BlockLayout {
     std::vector<ServeOut> outs;
     int n_wfm = 0;
     int n_audio_chans = 0;
-    int64_t iq_total = 0, audio_total = 0, mono_total = 0, rds_total = 0, aout_total = 0;
+    int n_pre_chans = 0;
+    int64_t iq_total = 0, audio_total = 0, mono_total = 0, rds_total = 0, aout_total = 0, pre_total = 0;
 };
 
 // A block prepared ahead: after a block is launched, the next block's
@@ -184,6 +205,7 @@ struct ArgusServeHdr {
     ArgusWfmHdr w;
     ArgusAudioHdr a;
     ArgusFastHdr f;
+    ArgusPreHdr p;
 };
 
 struct gpud_serve {
@@ -212,6 +234,8 @@ struct gpud_serve {
     std::vector<ArgusWfmDesc> wdescs;
     std::vector<ArgusDemodItem> ditems;
     std::vector<ArgusWfmItem> witems;
+    std::vector<ArgusPreDesc> pdescs;
+    std::vector<ArgusPreItem> pitems;
     int n_rds_on = 0;  // live channels with RDS decoding on
     // fast-convolution path: on/off, the stream's last input samples, the
     // staged segments and their transforms (sized for the largest block)

@@ -463,6 +463,8 @@ GPUD_API int gpud_surv_destroy(gpud_surv* s)
     if (s->d_zstuff) hipFree(s->d_zstuff);
     if (s->d_ltaps) hipFree(s->d_ltaps);
     if (s->d_lwin) hipFree(s->d_lwin);
+    if (s->d_alpha) hipFree(s->d_alpha);
+    if (s->h_alpha) hipHostFree(s->h_alpha);
     hipStreamDestroy(s->stream);
     delete s;
     return 0;

@@ -19,4 +19,8 @@ struct gpud_surv {
     double* d_ltaps = nullptr;
     float* d_lwin = nullptr;
     int level_decim = 0, level_taps = 0;
+    // per-frame EMA alphas of gpud_surv_smooth_device (device and pinned staging)
+    double* d_alpha = nullptr;
+    double* h_alpha = nullptr;
+    size_t alpha_cap = 0;
 };

@@ -216,6 +216,103 @@ GPUD_API int gpud_lowpass_fir(double cutoff_hz, int sample_rate, int taps, doubl
 GPUD_API int gpud_build_polyphase_taps(const float* base, int n, int decim, float* out);
 
 /* ------------------------------------------------------------------ */
+/* Job building of the DSP loop (cmd/sdrd): rate snapping and the job   */
+/* builders that turn detected signals into extraction jobs.  Host      */
+/* integer/FP64 work.                                                   */
+/* ------------------------------------------------------------------ */
+enum { GPUD_MOD_OTHER = 0, GPUD_MOD_WFM = 1, GPUD_MOD_WFM_STEREO = 2 };
+/* detector.Signal fields the builders read (detector.go:82-94): mod_type is
+ * Signal.Class.ModType ("WFM" / "WFM_STEREO", else other or unclassified). */
+typedef struct gpud_detected_signal {
+    int64_t id;
+    double  center_hz;
+    double  bw_hz;
+    int32_t mod_type;
+    int32_t pad;
+} gpud_detected_signal;
+/* extractionConfig (helpers.go:270-274).  snap_wfm = 0 keeps
+ * buildStreamingJobs as written (WFM out-rate 512000, which must divide the
+ * sample rate); 1 snaps it with snapOutRate, as the legacy builders do
+ * (helpers.go:194, 478). */
+typedef struct gpud_extraction_config {
+    int32_t fir_taps;  /* <= 0 -> 101 */
+    int32_t snap_wfm;
+    double  bw_mult;   /* <= 0 -> 1.0 */
+} gpud_extraction_config;
+/* batch.go:5-10 ExtractJob (legacy block path) */
+typedef struct gpud_extract_job {
+    double  offset_hz;
+    double  bw;
+    int32_t out_rate;
+    int32_t pad;
+    double  phase_start;
+} gpud_extract_job;
+/* batch.go:14-18 ExtractResult, plus where its IQ landed */
+typedef struct gpud_extract_result {
+    int32_t rate;
+    int32_t n_out;
+    int64_t out_offset;   /* complex-sample offset of the job's IQ in the output buffer */
+    double  phase_end;    /* phase_start + phase_inc * n (batch_runner.go:87-90) */
+} gpud_extract_result;
+/* helpers.go:292-310 snapOutRate: sample_rate / d for the divisor d <= 256 of
+ * sample_rate nearest sample_rate / target (first on ties). */
+GPUD_API int gpud_snap_out_rate(int sample_rate, int target);
+/* streaming_refactor.go:105-137 nearestExactDecimationRate */
+GPUD_API int gpud_nearest_exact_decimation_rate(int sample_rate, int target_rate, int min_rate);
+/* streaming_refactor.go:16-60 buildStreamingJobs: 0, -1 bad arguments, -2
+ * when a job's out-rate does not divide the sample rate (the reference's
+ * ExactIntegerDecimation error; *failed_index = that signal). */
+GPUD_API int gpud_build_streaming_jobs(int sample_rate, double center_hz, const gpud_detected_signal* signals,
+                                       int n_signals, const gpud_extraction_config* cfg, gpud_streaming_job* jobs,
+                                       int* failed_index);
+/* helpers.go:175-195 the ExtractJob list of extractSignalIQBatch (BW floors
+ * 250 kHz WFM / 20 kHz others, out-rate snapOutRate(512 k WFM, 200 k)). */
+GPUD_API int gpud_build_extract_jobs(int sample_rate, double center_hz, const gpud_detected_signal* signals,
+                                     int n_signals, gpud_extract_job* jobs);
+
+/* ------------------------------------------------------------------ */
+/* Legacy block path: successors of BatchRunner.ShiftFilterDecimateBatch */
+/* [WithPhase] (batch_runner.go:61-98, batch_runner_windows.go:87-182)   */
+/* and of the demod Engine (gpudemod_windows.go:94-654).  Shift from the */
+/* phase start, causal FIR from zero history, keep filtered samples 0,   */
+/* D, 2D, ...; only the kept outputs are computed (one fused launch).    */
+/* ------------------------------------------------------------------ */
+/* Per job: 101-tap LowpassFIR at max(bw/2, 200) Hz, decimation
+ * out_rate -> fs/out_rate (exact), n/D outputs at results[i].out_offset.
+ * n = 0 gives empty results with rate 0.  Errors: -1 bad args, -2 n >
+ * max_samples, -3 non-integer decimation, -4 fewer than D samples, -14
+ * out_cap too small.  Synchronous, like the reference. */
+GPUD_API int gpud_runner_shift_filter_decimate_batch(gpud_runner* r, const gpud_float2* iq, int n,
+                                                     const gpud_extract_job* jobs, int n_jobs, gpud_float2* out,
+                                                     int64_t out_cap, gpud_extract_result* results);
+
+typedef struct gpud_engine gpud_engine;
+/* gpudemod.go:81-90 DemodType */
+enum { GPUD_ENGINE_NFM = 0, GPUD_ENGINE_WFM = 1, GPUD_ENGINE_AM = 2, GPUD_ENGINE_USB = 3, GPUD_ENGINE_LSB = 4,
+       GPUD_ENGINE_CW = 5 };
+/* gpudemod_windows.go:112-162 New(maxSamples, sampleRate) */
+GPUD_API int gpud_engine_create(int max_samples, int sample_rate, gpud_engine** out);
+GPUD_API int gpud_engine_destroy(gpud_engine* e);
+/* gpudemod_windows.go:164-176 SetFIR: n <= 0 clears, longer than 256 is cut */
+GPUD_API int gpud_engine_set_fir(gpud_engine* e, const float* taps, int n);
+/* The carried shift and BFO phases and the current tap count. */
+GPUD_API int gpud_engine_state(gpud_engine* e, double* phase, double* bfo_phase, int* fir_taps);
+/* gpudemod_windows.go:320-393 ShiftFilterDecimate: taps = the engine's (or
+ * LowpassFIR(max(bw/2, 200), fs, 101), designed and kept), D =
+ * round(fs/out_rate), shift from the carried phase, n/D outputs, rate fs/D.
+ * Errors: -1, -2 n > max_samples, -3 out_rate <= 0, -4 n < D, -14. */
+GPUD_API int gpud_engine_shift_filter_decimate(gpud_engine* e, const gpud_float2* iq, int n, double offset_hz,
+                                               double bw, int out_rate, gpud_float2* out, int out_cap, int* n_out,
+                                               int* rate);
+/* gpudemod_windows.go:395-521 DemodFused (fused = 1) and 523-654 Demod
+ * (fused = 0): the chain to 48 kHz (192 kHz WFM), then the discriminator
+ * (n_out = m - 1), |x| (AM) or Re(x e^{j bfo}) with the carried BFO phase at
+ * +-700 Hz over the input rate (USB/CW +700, LSB -700).  -5 unsupported mode;
+ * -4 when the chain gives fewer than 2 (fused) or 1 (Demod) samples. */
+GPUD_API int gpud_engine_demod(gpud_engine* e, const gpud_float2* iq, int n, double offset_hz, double bw, int mode,
+                               int fused, float* out, int out_cap, int* n_out, int* rate);
+
+/* ------------------------------------------------------------------ */
 /* Streaming service (no reference counterpart: the reference drives one */
 /* runner per DSP tick from a goroutine, cmd/sdrd/dsp_loop.go:54-172).  */
 /* Channels are admitted against a fixed HBM budget and retired while   */
@@ -269,6 +366,19 @@ GPUD_API int gpud_serve_create(int sample_rate, int block_samples, uint64_t hbm_
 GPUD_API int gpud_serve_destroy(gpud_serve* s);
 /* 0, -1 bad args / non-integer decimation, GPUD_ERR_DUPLICATE, GPUD_ERR_BUDGET */
 GPUD_API int gpud_serve_open(gpud_serve* s, const gpud_channel_spec* spec);
+/* Per-channel options beyond the spec. */
+typedef struct gpud_channel_opts {
+    /* > 0: the live recorder's pre-demod stage (streamer.go:1098-1161): a
+     * stateful decimating FIR (dsp.StatefulDecimatingFIRComplex,
+     * decimating_fir.go:51-95; 101 FP64 taps, LowpassFIR at 90 kHz for WFM,
+     * else 0.8 of the demod Nyquist) by round(out_rate / demod_rate), forced
+     * to 2 for WFM when out_rate / 2 >= 200 kHz; the demodulator, stereo,
+     * RDS and audio stages then run at out_rate / factor.  0: none. */
+    int32_t demod_rate;
+    int32_t pad[7];
+} gpud_channel_opts;
+/* gpud_serve_open with options (opts may be NULL). */
+GPUD_API int gpud_serve_open_ex(gpud_serve* s, const gpud_channel_spec* spec, const gpud_channel_opts* opts);
 GPUD_API int gpud_serve_close(gpud_serve* s, int64_t channel_id);
 GPUD_API int gpud_serve_info(gpud_serve* s, int* live_channels, uint64_t* bytes_in_use, uint64_t* budget);
 /* Enqueue a block (host memory, copied before return) and return at once. */
@@ -284,6 +394,12 @@ GPUD_API int gpud_serve_wait(gpud_serve* s, int64_t ticket, gpud_serve_block_inf
  * its slot is reused, i.e. for the next two submits). */
 GPUD_API int gpud_serve_channel_output(gpud_serve* s, int64_t ticket, int64_t channel_id, const gpud_float2** iq,
                                        int* n_iq, const float** audio, int* n_audio, int* rate);
+/* The pre-demod stage's output of a channel in a completed block (the
+ * demodulator's input, at *rate; pinned host memory, valid as
+ * gpud_serve_channel_output's).  -1 if the channel was not part of the
+ * block, -3 if it has no pre-demod stage. */
+GPUD_API int gpud_serve_channel_predemod(gpud_serve* s, int64_t ticket, int64_t channel_id, const gpud_float2** iq,
+                                         int* n, int* rate);
 /* Replay (1, default) or not (0) a captured per-slot hipGraph of each
  * block's launches; waits for blocks in flight. */
 GPUD_API int gpud_serve_set_graph(gpud_serve* s, int enable);
@@ -550,6 +666,34 @@ typedef struct gpud_detector_params {
     double center_hz;
 } gpud_detector_params;
 
+/* config.DetectorConfig fields the detection path reads (config.go:30-48). */
+typedef struct gpud_detector_config {
+    double  threshold_db;
+    double  ema_alpha;        /* <= 0 or > 1 -> 0.2 */
+    double  hysteresis_db;    /* <= 0 -> 3 */
+    double  cfar_guard_hz;
+    double  cfar_train_hz;
+    double  cfar_scale_db;    /* <= 0 -> 6 */
+    double  edge_margin_db;   /* <= 0 -> 3 */
+    double  max_signal_bw_hz; /* <= 0 -> 150 kHz */
+    double  merge_gap_hz;     /* <= 0 -> 5 kHz */
+    int32_t cfar_mode;        /* GPUD_CFAR_* */
+    int32_t cfar_rank;        /* OS; outside 1..2*train -> round(0.75 * 2 * train) */
+    int32_t cfar_wrap;
+    int32_t pad;
+} gpud_detector_config;
+/* detector.go:101-261 New(cfg, sampleRate, fftSize), the detection settings:
+ * guard = max(0, ceil(guard_hz / binWidth)), train = max(1, ceil(train_hz /
+ * binWidth)), binWidth = sample_rate / fft (fft outside (0, 4 Mi] -> 65536,
+ * returned in *fft_bins), the rank and the defaults above.  Any output
+ * pointer may be NULL.  Host code; -1 bad arguments. */
+GPUD_API int gpud_detector_settings(const gpud_detector_config* cfg, int sample_rate, int fft_size,
+                                    gpud_cfar_params* cfar, gpud_detector_params* det, double* ema_alpha,
+                                    int* fft_bins);
+/* detector.go:263-278: alpha = 1 - (1 - ema_alpha)^(dt / (1/15 s)) clamped to
+ * [0.01, 0.99]; dt <= 0 or > 1 s (and the first frame: pass 0) -> 1/15 s. */
+GPUD_API double gpud_detector_adaptive_alpha(double ema_alpha, double dt_seconds);
+
 /* gpu.go:33-51 New(n): transforms at n_fft in {4096, 32768}.  Any other
  * power of two from 1024 to 32768 makes a detection-only engine (CFAR runs
  * and detection lists of spectra another engine computed, e.g. a decimated
@@ -584,6 +728,15 @@ GPUD_API int gpud_surv_spectrum_device(gpud_surv* s, const gpud_float2* in_dev, 
  * derives decim 2).  Queued on the engine's stream. */
 GPUD_API int gpud_surv_level_spectrum_device(gpud_surv* s, const gpud_float2* in_dev, int64_t frame_step,
                                              int n_frames, int decim, float* db_dev);
+/* detector.go:806-817 smoothSpectrum over n_frames consecutive frames (psd_dev
+ * [n_frames][n_fft] dB in, out_dev the smoothed frames, float32): per bin
+ * ema = alphas[f] * v + (1 - alphas[f]) * ema in FP64, the state ema_dev
+ * (n_fft doubles, device) carried across calls; first = 1 for a detector
+ * with no EMA yet (its first frame is copied, detector.go:807-810).  alphas
+ * is host memory (gpud_detector_adaptive_alpha per frame).  Queued on the
+ * engine's stream. */
+GPUD_API int gpud_surv_smooth_device(gpud_surv* s, const float* psd_dev, int n_frames, const double* alphas,
+                                     int first, double* ema_dev, float* out_dev);
 /* ca.go / gosca.go / caso.go / os.go + detector.go:436-480 per frame. */
 GPUD_API int gpud_surv_cfar_runs_device(gpud_surv* s, const float* psd_dev, int n_frames,
                                         const gpud_cfar_params* params, gpud_surv_run* runs_dev,

@@ -770,3 +770,187 @@ def Deemphasis(x, alpha, y, stride=1, offset=0):
         return x, y
     y = lib().orc_deemphasis(x[offset:].ctypes.data, (len(x) - offset + stride - 1) // stride, stride, alpha, y)
     return x, y
+
+
+# ---------------------------------------------------------------------------
+# Job building of the DSP loop (cmd/sdrd) and the legacy block path
+# ---------------------------------------------------------------------------
+WFM_STREAM_OUT_RATE = 512000   # helpers.go:279
+WFM_STREAM_MIN_BW = 250000.0   # helpers.go:280
+
+
+def snapOutRate(sample_rate, target):
+    """helpers.go:292-310: the divisor d <= 256 of sample_rate nearest
+    sample_rate / target (first on ties), as sample_rate / d."""
+    if sample_rate <= 0 or target <= 0:
+        return target
+    ideal = sample_rate / target
+    best, best_err = 0, float("inf")
+    for d in range(1, min(256, sample_rate) + 1):
+        if sample_rate % d:
+            continue
+        e = abs(d - ideal)
+        if e < best_err:
+            best, best_err = d, e
+    return target if best < 1 else sample_rate // best
+
+
+def nearestExactDecimationRate(sample_rate, target_rate, min_rate):
+    """streaming_refactor.go:105-137"""
+    if sample_rate <= 0 or target_rate <= 0:
+        return target_rate
+    if sample_rate % target_rate == 0:
+        return target_rate
+    td = sample_rate // target_rate
+    best_rate, best_dist = 0, sample_rate
+    for d in range(max(1, td - 2), td + 3):
+        rate = sample_rate // d
+        if rate < min_rate or rate <= 0 or sample_rate % rate:
+            continue
+        dist = abs(target_rate - rate)
+        if dist < best_dist:
+            best_dist, best_rate = dist, rate
+    return best_rate if best_rate > 0 else target_rate
+
+
+def isWFM(center_hz, mod_type=""):
+    """streaming_refactor.go:29-31: FM broadcast band or a WFM class."""
+    mhz = center_hz / 1e6
+    return (87.5 <= mhz <= 108.0) or mod_type in ("WFM", "WFM_STEREO")
+
+
+def buildStreamingJobs(sample_rate, center_hz, signals, fir_taps=0, bw_mult=0.0, snap_wfm=False):
+    """streaming_refactor.go:16-60.  signals: (id, center_hz, bw_hz, mod_type).
+    Returns (jobs, failed_index): jobs as (id, offset, bw, out_rate, taps,
+    hash), or (None, i) where the reference returns the
+    ExactIntegerDecimation error.  snap_wfm snaps the WFM out-rate with
+    snapOutRate."""
+    if bw_mult <= 0:
+        bw_mult = 1.0
+    if fir_taps <= 0:
+        fir_taps = 101
+    jobs = []
+    for i, (sid, cen, bw_hz, mod) in enumerate(signals):
+        bw = bw_hz * bw_mult
+        if isWFM(cen, mod):
+            out_rate = snapOutRate(sample_rate, WFM_STREAM_OUT_RATE) if snap_wfm else WFM_STREAM_OUT_RATE
+            if bw < WFM_STREAM_MIN_BW:
+                bw = WFM_STREAM_MIN_BW
+        else:
+            out_rate = nearestExactDecimationRate(sample_rate, 200000, 128000)
+            if bw < 20000:
+                bw = 20000.0
+        if lib().orc_exact_decim(sample_rate, out_rate) < 0:
+            return None, i
+        off = cen - center_hz
+        jobs.append((sid, off, bw, out_rate, fir_taps,
+                     StreamingConfigHash(sid, off, bw, out_rate, fir_taps, sample_rate)))
+    return jobs, -1
+
+
+def buildExtractJobs(sample_rate, center_hz, signals):
+    """helpers.go:175-195: ExtractJob (offset, bw, out_rate, phase_start 0)
+    of extractSignalIQBatch."""
+    out = []
+    for _, cen, bw, mod in signals:
+        rate = 200000
+        if isWFM(cen, mod):
+            rate = WFM_STREAM_OUT_RATE
+            if bw < WFM_STREAM_MIN_BW:
+                bw = WFM_STREAM_MIN_BW
+        elif bw < 20000:
+            bw = 20000.0
+        out.append((cen - center_hz, bw, snapOutRate(sample_rate, rate), 0.0))
+    return out
+
+
+def LegacyShiftFilterDecimate(iq, sample_rate, offset_hz, phase_start, taps, decim, n_out=None):
+    """The legacy GPU chain in FP64 (test reference for the block path):
+    gpud_freq_shift_kernel (exports.cu:37-60: sample i rotated by
+    phase_start + inc * i, inc = -2 pi offset / fs), gpud_fir_kernel
+    (exports.cu:127-161: causal FIR from zero history) and
+    gpud_decimate_kernel (exports.cu:114-124: keep indices 0, D, 2D, ...),
+    n / D outputs (batch_runner_windows.go:159-162).  Returns (IQ, phase_end)."""
+    x = np.asarray(iq, np.complex128)
+    n = len(x)
+    inc = -2.0 * np.pi * offset_hz / sample_rate
+    m = n // decim if n_out is None else n_out
+    idx = np.arange(m) * decim
+    s = x * np.exp(1j * (phase_start + inc * np.arange(n)))
+    h = np.asarray(taps, np.float64)
+    y = np.zeros(m, np.complex128)
+    for k in range(len(h)):
+        j = idx - k
+        ok = j >= 0
+        y[ok] += h[k] * s[j[ok]]
+    return y, phase_start + inc * n
+
+
+def LegacyDemod(dec, mode, sample_rate, bfo_phase=0.0):
+    """The engine's demodulators on the decimated chain output (FP64):
+    gpud_fm_discrim_kernel (exports.cu:87-112, n-1 outputs),
+    gpud_am_envelope_kernel (|x|, exports.cu:273-294), and
+    gpud_ssb_product_kernel (Re(x e^{j(bfo_phase + inc i)}),
+    exports.cu:296-326) with inc = 2 pi (+-700) / sample_rate, the engine's
+    input rate (gpudemod_windows.go:304, 508).  mode: DemodType order
+    NFM, WFM, AM, USB, LSB, CW.  Returns (audio, bfo_phase_end)."""
+    d = np.asarray(dec, np.complex128)
+    if mode in (0, 1):
+        return np.angle(np.conj(d[:-1]) * d[1:]), bfo_phase
+    if mode == 2:
+        return np.abs(d), bfo_phase
+    inc = 2.0 * np.pi * (-700.0 if mode == 4 else 700.0) / sample_rate
+    out = np.real(d * np.exp(1j * (bfo_phase + inc * np.arange(len(d)))))
+    return out, bfo_phase + inc * len(d)
+
+
+# ---------------------------------------------------------------------------
+# Detector settings and smoothing (detector.go:101-278, 806-817)
+# ---------------------------------------------------------------------------
+def go_round(x):
+    """math.Round: half away from zero."""
+    return int(np.floor(abs(x) + 0.5)) * (1 if x >= 0 else -1)
+
+
+def DetectorSettings(sample_rate, fft_size, guard_hz, train_hz, rank=0, ema_alpha=0.0, hysteresis_db=0.0,
+                     scale_db=0.0, edge_margin_db=0.0, max_signal_bw_hz=0.0, merge_gap_hz=0.0):
+    """detector.New (detector.go:101-230): dict of the mapped settings."""
+    n = fft_size if 0 < fft_size <= 4194304 else 65536
+    bw = sample_rate / n
+    guard = max(0, int(np.ceil(guard_hz / bw)))
+    train = max(1, int(np.ceil(train_hz / bw)))
+    if rank <= 0 or rank > 2 * train:
+        rank = go_round(0.75 * float(2 * train))
+        if rank <= 0:
+            rank = 1
+    return dict(fft=n, guard=guard, train=train, rank=rank,
+                ema_alpha=ema_alpha if 0 < ema_alpha <= 1 else 0.2,
+                hysteresis_db=hysteresis_db if hysteresis_db > 0 else 3.0,
+                scale_db=scale_db if scale_db > 0 else 6.0,
+                edge_margin_db=edge_margin_db if edge_margin_db > 0 else 3.0,
+                max_signal_bw_hz=max_signal_bw_hz if max_signal_bw_hz > 0 else 150000.0,
+                merge_gap_hz=merge_gap_hz if merge_gap_hz > 0 else 5000.0)
+
+
+def AdaptiveAlpha(ema_alpha, dt):
+    """detector.go:263-278 (dt <= 0 or > 1 s -> 1/15 s)."""
+    if not dt > 0 or dt > 1.0:
+        dt = 1.0 / 15.0
+    a = 1.0 - (1.0 - ema_alpha) ** (dt / (1.0 / 15.0))
+    return min(max(a, 0.01), 0.99)
+
+
+class SmoothSpectrum:
+    """detector.go:806-817: the EMA of a detector, FP64 per bin; the first
+    frame (or a size change) is copied."""
+
+    def __init__(self):
+        self.ema = None
+
+    def __call__(self, spectrum, alpha):
+        v = np.asarray(spectrum, np.float64)
+        if self.ema is None or len(self.ema) != len(v):
+            self.ema = v.copy()
+            return self.ema.copy()
+        self.ema = alpha * v + (1 - alpha) * self.ema
+        return self.ema.copy()

@@ -8,8 +8,25 @@ import oracle as O
 NONE, NFM, WFM, AM, USB, LSB, CW, WFM_STEREO = range(8)
 
 
+def pre_demod_stage(out_rate, demod_rate, wfm):
+    """The live recorder's pre-demod decimating FIR (streamer.go:1098-1124):
+    factor round(snipRate / demodRate), forced to 2 for WFM when that keeps
+    >= 200 kHz; LowpassFIR at 90 kHz (WFM) or 0.8 of the demod Nyquist, 101
+    taps; None when the factor is 1."""
+    if not demod_rate:
+        return None, 1
+    f = max(1, O.go_round(out_rate / demod_rate))
+    if wfm and f > 2 and out_rate // 2 >= 200000:
+        f = 2
+    if f <= 1:
+        return None, 1
+    cutoff = 90000.0 if wfm else (out_rate // f) / 2.0 * 0.8
+    return O.StatefulDecimatingFIRComplex(O.LowpassFIR(cutoff, out_rate, 101), f), f
+
+
 class OracleChannel:
-    def __init__(self, sample_rate, offset_hz, bandwidth, out_rate, num_taps=101, demod=NONE, window=0):
+    def __init__(self, sample_rate, offset_hz, bandwidth, out_rate, num_taps=101, demod=NONE, window=0,
+                 demod_rate=0):
         self.sr = sample_rate
         self.off = offset_hz
         self.decim = O.ExactIntegerDecimation(sample_rate, out_rate)
@@ -20,13 +37,19 @@ class OracleChannel:
         self.poly = O.BuildPolyphaseTapsPhaseMajor(base, self.decim)
         self.phase, self.pc, self.hist = 0.0, 0, np.zeros(0, np.complex64)
         self.demod = demod
-        self.out_rate = out_rate
+        self.pre, f = pre_demod_stage(out_rate, demod_rate, demod in (WFM, WFM_STEREO))
+        self.out_rate = out_rate // f   # the demodulator's rate
         self.fm = O.BridgedFMDiscrim()
         self.bfo = 0.0
+        self.last_pre = None
 
     def block(self, iq):
-        out, self.phase, self.pc, self.hist = O.runStreamingPolyphaseHostCore(
+        bb, self.phase, self.pc, self.hist = O.runStreamingPolyphaseHostCore(
             iq, self.sr, self.off, self.phase, self.pc, self.T, self.decim, self.hist, self.poly)
+        out = bb
+        if self.pre is not None:
+            out = self.pre.Process(bb) if len(bb) else np.zeros(0, np.complex64)
+            self.last_pre = out
         if self.demod in (NFM, WFM):
             audio = self.fm.Demod(out)
         elif self.demod == AM:
@@ -37,7 +60,7 @@ class OracleChannel:
             audio, self.bfo = O.CWDemod(out, self.out_rate, self.bfo)
         else:
             audio = np.zeros(0, np.float32)
-        return out, audio
+        return bb, audio
 
 
 class OracleStereoChannel:
@@ -46,11 +69,12 @@ class OracleStereoChannel:
     the mono fallback filter (streamer.go:1367-1395, 1511-1643), and the
     streaming RDS baseband of the discriminator output (fm.go:234-271)."""
 
-    def __init__(self, sample_rate, offset_hz, bandwidth, out_rate, num_taps=101, window=0):
-        self.ch = OracleChannel(sample_rate, offset_hz, bandwidth, out_rate, num_taps, WFM, window)
-        self.stereo = O.StereoDecoder(out_rate)
-        self.fallback = O.StatefulFIRReal(O.LowpassFIR(15000, out_rate, 101))
-        self.rds = O.RDSBasebandStream(out_rate)
+    def __init__(self, sample_rate, offset_hz, bandwidth, out_rate, num_taps=101, window=0, demod_rate=0):
+        self.ch = OracleChannel(sample_rate, offset_hz, bandwidth, out_rate, num_taps, WFM, window, demod_rate)
+        rate = self.ch.out_rate
+        self.stereo = O.StereoDecoder(rate)
+        self.fallback = O.StatefulFIRReal(O.LowpassFIR(15000, rate, 101))
+        self.rds = O.RDSBasebandStream(rate)
 
     def block(self, iq):
         """(baseband, audio L/R interleaved, rds baseband, locked, stereo)."""

@@ -41,7 +41,7 @@ def _late_bound():
     # modules that register their own entry points on import
     import importlib
     names = set()
-    for mod in ("argus.surveillance", "argus.serve", "argus.wfm", "argus.ingest"):
+    for mod in ("argus.surveillance", "argus.serve", "argus.wfm", "argus.ingest", "argus.jobs"):
         try:
             importlib.import_module(mod)
         except ModuleNotFoundError:
