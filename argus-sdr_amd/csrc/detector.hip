@@ -93,6 +93,8 @@ __global__ __launch_bounds__(kSmoothBlock) void argus_ema_kernel(const float* __
                                                                    int first, double* __restrict__ ema,
                                                                    float* __restrict__ out)
 {
+    // separate FP64 roundings in the reference's order: no fused multiply-add
+#pragma clang fp contract(off)
     const int b = blockIdx.x * kSmoothBlock + threadIdx.x;
     if (b >= n_bins) return;
     double e = first ? 0.0 : ema[b];

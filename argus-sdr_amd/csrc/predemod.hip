@@ -11,6 +11,8 @@ namespace {
 
 __global__ __launch_bounds__(ARGUS_PRE_TILE) void argus_predemod_hdr_kernel(const ArgusPreHdr* __restrict__ hdr)
 {
+    // separate FP64 roundings in the reference's order: no fused multiply-add
+#pragma clang fp contract(off)
     const ArgusPreHdr h = *hdr;
     for (int w = blockIdx.x; w < h.n_items; w += gridDim.x) {
         const ArgusPreItem it = h.items[w];
