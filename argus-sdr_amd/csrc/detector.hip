@@ -104,8 +104,8 @@ __global__ __launch_bounds__(kSmoothBlock) void argus_ema_kernel(const float* __
             e = v;
         } else {
             const double a = alphas[f];
-            // alpha*v + (1-alpha)*ema with separate roundings, as the reference evaluates it
-            e = __dadd_rn(__dmul_rn(a, v), __dmul_rn(__dadd_rn(1.0, -a), e));
+            // alpha*v + (1-alpha)*ema, each operation rounded (contraction is off here)
+            e = a * v + (1.0 - a) * e;
         }
         out[(size_t)f * n_bins + b] = (float)e;
     }

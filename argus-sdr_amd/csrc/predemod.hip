@@ -33,8 +33,8 @@ __global__ __launch_bounds__(ARGUS_PRE_TILE) void argus_predemod_hdr_kernel(cons
             const int e = i - k;
             const float2 v = e >= 0 ? d.in[e] : d.hist_rd[d.T - 1 + e];
             const double w8 = d.taps[k];
-            ar = __dadd_rn(ar, __dmul_rn((double)v.x, w8));
-            ai = __dadd_rn(ai, __dmul_rn((double)v.y, w8));
+            ar += (double)v.x * w8;
+            ai += (double)v.y * w8;
         }
         d.out[it.start + t] = make_float2((float)ar, (float)ai);
     }
