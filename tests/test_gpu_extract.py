@@ -99,12 +99,14 @@ def test_lifecycle_reset_and_capacity(gpu, mode):
 
 
 def test_uneven_mix_long_stream(gpu):
-    """20 Msps, uneven decimations 8..1600 and taps 63..511, many chunks."""
+    """The batched uneven extraction shape: 20 Msps, 64 concurrent signals of
+    the four modes with decimations 8..1600 and taps 63..511, 65 536-sample
+    chunks (ragged), one fused launch per chunk."""
     sr = 20_000_000
     rng = np.random.default_rng(0)
     modes = [(200_000, [8, 16, 40, 80]), (12_500, [400, 800]), (10_000, [500, 1000]), (3_000, [1250, 1600])]
     jobs = []
-    for i in range(24):
+    for i in range(64):
         bw, ds = modes[i % 4]
         d = int(rng.choice(ds))
         t = int(rng.integers(31, 256)) * 2 + 1
