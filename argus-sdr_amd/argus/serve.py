@@ -48,6 +48,10 @@ register("gpud_serve_info", I, [P, C.POINTER(I), C.POINTER(C.c_uint64), C.POINTE
 register("gpud_serve_submit", I, [P, P, I, C.POINTER(C.c_int64)])
 register("gpud_serve_submit_device", I, [P, P, I, C.POINTER(C.c_int64)])
 register("gpud_serve_set_download", I, [P, I])
+register("gpud_serve_set_download_parts", I, [P, I])
+register("gpud_serve_poll", I, [P, C.c_int64])
+register("gpud_serve_counters", I, [P, C.POINTER(C.c_int64), C.POINTER(C.c_uint64)])
+DL_IQ, DL_DEMOD, DL_WFM, DL_AUDIO, DL_PREDEMOD, DL_ALL = 1, 2, 4, 8, 16, 31
 register("gpud_serve_wait", I, [P, C.c_int64, C.POINTER(gpud_serve_block_info)])
 register("gpud_serve_channel_output", I, [P, C.c_int64, C.c_int64, C.POINTER(P), C.POINTER(I), C.POINTER(P),
                                           C.POINTER(I), C.POINTER(I)])
@@ -140,6 +144,23 @@ class Serve:
         bud = C.c_uint64()
         check("gpud_serve_info", lib().gpud_serve_info(self._h, C.byref(n), C.byref(used), C.byref(bud)))
         return n.value, used.value, bud.value
+
+    def SetDownloadParts(self, mask: int) -> None:
+        """Which outputs come to host memory (DL_* bits)."""
+        check("gpud_serve_set_download_parts", lib().gpud_serve_set_download_parts(self._h, mask))
+
+    def Poll(self, ticket: int) -> bool:
+        """True once the block's outputs are on the host (non-blocking)."""
+        rc = lib().gpud_serve_poll(self._h, ticket)
+        if rc < 0:
+            check("gpud_serve_poll", rc)
+        return rc == 1
+
+    def Counters(self):
+        """(graph instantiations so far, free bytes of the channel-state arena)"""
+        g, f = C.c_int64(), C.c_uint64()
+        check("gpud_serve_counters", lib().gpud_serve_counters(self._h, C.byref(g), C.byref(f)))
+        return g.value, f.value
 
     def SetDownload(self, enable: bool) -> None:
         check("gpud_serve_set_download", lib().gpud_serve_set_download(self._h, 1 if enable else 0))

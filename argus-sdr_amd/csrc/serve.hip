@@ -120,18 +120,95 @@ static size_t channel_footprint(const gpud_serve* s, int D, int T, bool stereo, 
     return fixed + per_block * ARGUS_SERVE_SLOTS;
 }
 
-static void release_channel(ServeChannel& c)
+static size_t align256(size_t v) { return (v + 255) & ~(size_t)255; }
+
+char* ServeArena::alloc(size_t bytes)
 {
-    if (c.d_taps) hipFree(c.d_taps);
-    if (c.d_ctaps) hipFree(c.d_ctaps);
-    if (c.d_hist[0]) hipFree(c.d_hist[0]);
-    if (c.d_hist[1]) hipFree(c.d_hist[1]);
-    if (c.d_carry) hipFree(c.d_carry);
-    if (c.d_wfm) hipFree(c.d_wfm);
-    if (c.d_aud) hipFree(c.d_aud);
-    if (c.d_rdec) hipFree(c.d_rdec);
-    if (c.d_fastG) hipFree(c.d_fastG);
-    if (c.d_pre) hipFree(c.d_pre);
+    bytes = align256(bytes ? bytes : 1);
+    for (auto it = free_list.begin(); it != free_list.end(); ++it) {
+        if (it->second < bytes) continue;
+        const size_t off = it->first, rest = it->second - bytes;
+        free_list.erase(it);
+        if (rest) free_list.emplace(off + bytes, rest);
+        return base + off;
+    }
+    return nullptr;
+}
+
+void ServeArena::release(char* p, size_t bytes)
+{
+    if (!p) return;
+    bytes = align256(bytes ? bytes : 1);
+    size_t off = (size_t)(p - base);
+    auto next = free_list.lower_bound(off);
+    if (next != free_list.end() && off + bytes == next->first) {  // merge with the block after
+        bytes += next->second;
+        next = free_list.erase(next);
+    }
+    if (next != free_list.begin()) {  // and with the block before
+        auto prev = std::prev(next);
+        if (prev->first + prev->second == off) {
+            prev->second += bytes;
+            return;
+        }
+    }
+    free_list.emplace(off, bytes);
+}
+
+// Copy `bytes` of host data to device memory `dst` through the pinned ring,
+// queued on the compute stream (ahead of every later block).  Waits only when
+// the ring wraps onto a region whose copy is still pending.
+static int stage_upload(gpud_serve* s, void* dst, const void* src, size_t bytes)
+{
+    ServeStage& st = s->stage;
+    if (!bytes) return 0;
+    if (bytes > st.size) return -1;
+    size_t at = align256(st.head);
+    if (at + bytes > st.size) at = 0;
+    // retire completed regions, then wait out any still overlapping [at, at + bytes)
+    while (!st.busy.empty() && hipEventQuery(st.busy.front().second) == hipSuccess) {
+        st.spare.push_back(st.busy.front().second);
+        st.busy.pop_front();
+    }
+    for (size_t i = 0; i < st.busy.size(); i++) {
+        const size_t o = st.busy[i].first.first, n = st.busy[i].first.second;
+        if (o < at + bytes && at < o + n) {
+            ARGUS_CHECK(hipEventSynchronize(st.busy[i].second));
+            for (size_t j = 0; j <= i; j++) {  // ring order: everything before it is done too
+                st.spare.push_back(st.busy.front().second);
+                st.busy.pop_front();
+            }
+            i = (size_t)-1;
+        }
+    }
+    std::memcpy(st.ring + at, src, bytes);
+    ARGUS_CHECK(hipMemcpyAsync(dst, st.ring + at, bytes, hipMemcpyHostToDevice, s->compute));
+    hipEvent_t ev = nullptr;
+    if (!st.spare.empty()) {
+        ev = st.spare.back();
+        st.spare.pop_back();
+    } else {
+        ARGUS_CHECK(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+    }
+    ARGUS_CHECK(hipEventRecord(ev, s->compute));
+    st.busy.push_back({{at, bytes}, ev});
+    st.head = at + bytes;
+    return 0;
+}
+
+static void release_channel(gpud_serve* s, ServeChannel& c)
+{
+    s->arena.release(c.d_block, c.block_bytes);
+    s->arena.release(c.d_aud, c.aud_bytes);
+    s->arena.release(c.d_rdec, c.rdec_bytes);
+    c.d_block = nullptr;
+    c.block_bytes = c.aud_bytes = c.rdec_bytes = 0;
+    c.d_taps = nullptr;
+    c.d_ctaps = nullptr;
+    c.d_hist[0] = c.d_hist[1] = nullptr;
+    c.d_carry = nullptr;
+    c.d_wfm = nullptr;
+    c.d_fastG = nullptr;
     c.d_pre = nullptr;
     c.d_aud = nullptr;
     c.d_rdec = nullptr;
@@ -150,7 +227,7 @@ static void reap(gpud_serve* s)
         ServeChannel& c = s->closed[i];
         if (c.last_ticket <= s->completed_ticket) {
             s->bytes_in_use -= c.footprint;
-            release_channel(c);
+            release_channel(s, c);
             s->closed[i] = s->closed.back();
             s->closed.pop_back();
         } else {
@@ -159,21 +236,18 @@ static void reap(gpud_serve* s)
     }
 }
 
-// Device block of a new stereo channel: PLL at 19 kHz (streamer.go:1532-1549),
-// zero delay lines, the tap sets at the channel's output rate.
-static hipError_t wfm_channel_init(ServeChannel& c, int rate)
+// Initial device block of a new stereo channel (kWfmBytes at blob): PLL at
+// 19 kHz (streamer.go:1532-1549), zero delay lines, the tap sets at the
+// channel's demod rate.
+static void wfm_channel_blob(ServeChannel& c, int rate, char* blob)
 {
-    std::vector<char> blob(kWfmBytes, 0);
     ArgusWfmState st{};
     st.freq = 2.0 * M_PI * 19000.0 / (double)rate;
-    std::memcpy(blob.data(), &st, sizeof st);
-    argus_wfm_design_taps(rate, (float*)(blob.data() + kWfmTaps));
+    std::memcpy(blob, &st, sizeof st);
+    argus_wfm_design_taps(rate, (float*)(blob + kWfmTaps));
     c.rds_decim = std::max(1, rate / 19000);
     c.rds_count = 0;
     c.wfm_sel = 0;
-    hipError_t e = dev_alloc(&c.d_wfm, kWfmBytes);
-    if (e != hipSuccess) return e;
-    return hipMemcpy(c.d_wfm, blob.data(), kWfmBytes, hipMemcpyHostToDevice);
 }
 
 // Free a slot's output buffers, tables and captured graph (not its input
@@ -272,6 +346,12 @@ GPUD_API int gpud_serve_create(int sample_rate, int block_samples, uint64_t hbm_
         ARGUS_CHECK(dev_alloc(&sl.d_in, (size_t)block_samples));
         ARGUS_CHECK(host_alloc(&sl.h_in, (size_t)block_samples));
     }
+    // channel state arena (bounded by the budget) and the upload ring
+    s->arena.size = align256((size_t)std::min<uint64_t>(hbm_budget_bytes, (uint64_t)16 << 30));
+    ARGUS_CHECK(hipMalloc((void**)&s->arena.base, s->arena.size));
+    s->arena.free_list.emplace(0, s->arena.size);
+    s->stage.size = (size_t)16 << 20;
+    ARGUS_CHECK(hipHostMalloc((void**)&s->stage.ring, s->stage.size, hipHostMallocDefault));
     const int smax = argus_fast_segments(block_samples);
     ARGUS_CHECK(dev_alloc(&s->d_ftail, (size_t)ARGUS_FAST_TAIL));
     ARGUS_CHECK(hipMemset(s->d_ftail, 0, ARGUS_FAST_TAIL * sizeof(float2)));
@@ -285,8 +365,12 @@ GPUD_API int gpud_serve_destroy(gpud_serve* s)
 {
     if (!s) return 0;
     hipDeviceSynchronize();
-    for (auto& kv : s->live) release_channel(kv.second);
-    for (auto& c : s->closed) release_channel(c);
+    for (auto& kv : s->live) release_channel(s, kv.second);
+    for (auto& c : s->closed) release_channel(s, c);
+    hipFree(s->arena.base);
+    hipHostFree(s->stage.ring);
+    for (auto& b : s->stage.busy) hipEventDestroy(b.second);
+    for (hipEvent_t ev : s->stage.spare) hipEventDestroy(ev);
     for (int i = 0; i < ARGUS_SERVE_SLOTS; i++) {
         ServeSlot& sl = s->slots[i];
         hipEventDestroy(sl.ev_in);
@@ -357,44 +441,51 @@ GPUD_API int gpud_serve_open_ex(gpud_serve* s, const gpud_channel_spec* spec, co
         }
     // the increment the block builder derives from the same offset
     const uint64_t inc = argus_rad_to_fix(-2.0 * M_PI * spec->offset_hz / (double)s->sample_rate);
-    std::vector<float2> ctaps((size_t)T);
-    argus_modulated_taps(poly.data(), D, T, inc, ctaps.data());
-    if (dev_alloc(&c.d_taps, poly.size()) != hipSuccess || dev_alloc(&c.d_ctaps, ctaps.size()) != hipSuccess ||
-        dev_alloc(&c.d_hist[0], (size_t)std::max(1, T - 1)) || dev_alloc(&c.d_hist[1], (size_t)std::max(1, T - 1)) ||
-        dev_alloc(&c.d_carry, 2)) {
-        release_channel(c);
-        return (int)hipErrorOutOfMemory;
-    }
-    if (argus_fast_eligible(D, T)) {  // the fast path's transform of the modulated taps
+    // one arena block: taps | modulated taps | history x2 | FM carry x2 |
+    // fast-path spectrum | stereo block | pre-demod block
+    const bool fast = argus_fast_eligible(D, T);
+    const size_t H = (size_t)std::max(1, T - 1);
+    const size_t o_ctaps = align256(poly.size() * sizeof(float));
+    const size_t o_hist0 = o_ctaps + align256((size_t)T * sizeof(float2));
+    const size_t o_hist1 = o_hist0 + align256(H * sizeof(float2));
+    const size_t o_carry = o_hist1 + align256(H * sizeof(float2));
+    const size_t o_fast = o_carry + align256(2 * sizeof(float2));
+    const size_t o_wfm = o_fast + (fast ? align256((size_t)ARGUS_FAST_N * sizeof(float2)) : 0);
+    const size_t o_pre = o_wfm + (stereo ? align256(kWfmBytes) : 0);
+    const size_t bytes = o_pre + (pre > 1 ? align256(kPreBytes) : 0);
+    std::vector<char> blob(bytes, 0);
+    std::memcpy(blob.data(), poly.data(), poly.size() * sizeof(float));
+    argus_modulated_taps(poly.data(), D, T, inc, (float2*)(blob.data() + o_ctaps));
+    if (fast) {  // the fast path's transform of the modulated taps
         std::vector<float> h((size_t)T);
         for (int k = 0; k < T; k++) h[(size_t)k] = poly[(size_t)(k % D) * L + k / D];
-        std::vector<float2> G(ARGUS_FAST_N);
-        argus_fast_spectrum(h.data(), T, inc, G.data());
-        if (dev_alloc(&c.d_fastG, (size_t)ARGUS_FAST_N) != hipSuccess ||
-            hipMemcpy(c.d_fastG, G.data(), G.size() * sizeof(float2), hipMemcpyHostToDevice) != hipSuccess) {
-            release_channel(c);
-            return (int)hipErrorOutOfMemory;
-        }
+        argus_fast_spectrum(h.data(), T, inc, (float2*)(blob.data() + o_fast));
     }
-    hipError_t e = hipMemcpy(c.d_taps, poly.data(), poly.size() * sizeof(float), hipMemcpyHostToDevice);
-    if (e == hipSuccess)
-        e = hipMemcpy(c.d_ctaps, ctaps.data(), ctaps.size() * sizeof(float2), hipMemcpyHostToDevice);
-    if (e == hipSuccess && pre > 1) {
+    if (stereo) wfm_channel_blob(c, c.demod_rate, blob.data() + o_wfm);
+    if (pre > 1) {
         // LowpassFIR(cutoff, snipRate, 101) in FP64, cutoff 90 kHz for WFM,
         // else 0.8 of the demod Nyquist (streamer.go:1114-1124); zero delay lines
         const bool wfm = spec->demod == GPUD_DEMOD_WFM || stereo;
         const double pc = wfm ? 90000.0 : (double)c.demod_rate / 2.0 * 0.8;
-        std::vector<char> blob(kPreBytes, 0);
         std::vector<double> h(ARGUS_PRE_TAPS + 1);
         argus_lowpass_fir(pc, spec->out_rate, ARGUS_PRE_TAPS, h.data(), false);
-        std::memcpy(blob.data() + kPreTaps, h.data(), ARGUS_PRE_TAPS * sizeof(double));
-        e = dev_alloc(&c.d_pre, kPreBytes);
-        if (e == hipSuccess) e = hipMemcpy(c.d_pre, blob.data(), kPreBytes, hipMemcpyHostToDevice);
+        std::memcpy(blob.data() + o_pre + kPreTaps, h.data(), ARGUS_PRE_TAPS * sizeof(double));
     }
-    if (e == hipSuccess && stereo) e = wfm_channel_init(c, c.demod_rate);
-    if (e != hipSuccess) {
-        release_channel(c);
-        return (int)e;
+    c.d_block = s->arena.alloc(bytes);
+    if (!c.d_block) return GPUD_ERR_BUDGET;  // the arena cannot place it
+    c.block_bytes = bytes;
+    c.d_taps = (float*)c.d_block;
+    c.d_ctaps = (float2*)(c.d_block + o_ctaps);
+    c.d_hist[0] = (float2*)(c.d_block + o_hist0);
+    c.d_hist[1] = (float2*)(c.d_block + o_hist1);
+    c.d_carry = (float2*)(c.d_block + o_carry);
+    if (fast) c.d_fastG = (float2*)(c.d_block + o_fast);
+    if (stereo) c.d_wfm = c.d_block + o_wfm;
+    if (pre > 1) c.d_pre = c.d_block + o_pre;
+    const int e = stage_upload(s, c.d_block, blob.data(), bytes);
+    if (e) {
+        release_channel(s, c);
+        return e;
     }
     s->bytes_in_use += fp;
     s->live.emplace(spec->channel_id, c);
@@ -813,7 +904,9 @@ static int describe_block(gpud_serve* s, ServeSlot& sl, const std::vector<ServeO
     sl.p_mode = n_pitems > 0 ? 1 : 0;
     sl.tab_bytes = total;
     sl.off_hdr = off_hdr;
-    sl.lds = lds;
+    // dynamic LDS in 8 KiB steps, so the captured graphs survive most
+    // changes of the channel set
+    sl.lds = lds ? std::min<size_t>((lds + 8191) & ~(size_t)8191, 160 * 1024) : 0;
     return 0;
 }
 
@@ -946,6 +1039,7 @@ static int launch_block(gpud_serve* s, ServeSlot& sl)
                 return (int)e;
             }
         }
+        s->graph_captures++;
         sl.g_hdr = hdr;
         sl.g_lds = sl.lds;
         sl.g_grid = s->tune.grid;
@@ -1154,26 +1248,25 @@ static int serve_submit(gpud_serve* s, const gpud_float2* iq, int n, bool on_dev
     commit_block(s, sl.outs, n, sl.ticket);
     // ---- copy stream: download ------------------------------------------
     ARGUS_CHECK(hipStreamWaitEvent(s->copy, sl.ev_kern, 0));
-    if (s->download) {
-        if (sl.iq_total > 0)
-            ARGUS_CHECK(hipMemcpyAsync(sl.h_iq, sl.d_iq, (size_t)sl.iq_total * sizeof(float2), hipMemcpyDeviceToHost,
-                                       s->copy));
-        if (sl.audio_total > 0)
-            ARGUS_CHECK(hipMemcpyAsync(sl.h_audio, sl.d_audio, (size_t)sl.audio_total * sizeof(float),
-                                       hipMemcpyDeviceToHost, s->copy));
-        if (sl.rds_total > 0)
-            ARGUS_CHECK(hipMemcpyAsync(sl.h_rds, sl.d_rds, (size_t)sl.rds_total * sizeof(float2),
-                                       hipMemcpyDeviceToHost, s->copy));
-        if (sl.n_wfm > 0)
-            ARGUS_CHECK(hipMemcpyAsync(sl.h_wblk, sl.d_wblk, (size_t)sl.n_wfm * sizeof(ArgusWfmBlock),
-                                       hipMemcpyDeviceToHost, s->copy));
-        if (sl.aout_total > 0)
-            ARGUS_CHECK(hipMemcpyAsync(sl.h_aout, sl.d_aout, (size_t)sl.aout_total * sizeof(float),
-                                       hipMemcpyDeviceToHost, s->copy));
-        if (sl.pre_total > 0)
-            ARGUS_CHECK(hipMemcpyAsync(sl.h_pre, sl.d_pre, (size_t)sl.pre_total * sizeof(float2),
-                                       hipMemcpyDeviceToHost, s->copy));
-    }
+    const int dl = s->dl_mask;
+    if ((dl & GPUD_DL_IQ) && sl.iq_total > 0)
+        ARGUS_CHECK(hipMemcpyAsync(sl.h_iq, sl.d_iq, (size_t)sl.iq_total * sizeof(float2), hipMemcpyDeviceToHost,
+                                   s->copy));
+    if ((dl & GPUD_DL_DEMOD) && sl.audio_total > 0)
+        ARGUS_CHECK(hipMemcpyAsync(sl.h_audio, sl.d_audio, (size_t)sl.audio_total * sizeof(float),
+                                   hipMemcpyDeviceToHost, s->copy));
+    if ((dl & GPUD_DL_WFM) && sl.rds_total > 0)
+        ARGUS_CHECK(hipMemcpyAsync(sl.h_rds, sl.d_rds, (size_t)sl.rds_total * sizeof(float2), hipMemcpyDeviceToHost,
+                                   s->copy));
+    if ((dl & GPUD_DL_WFM) && sl.n_wfm > 0)
+        ARGUS_CHECK(hipMemcpyAsync(sl.h_wblk, sl.d_wblk, (size_t)sl.n_wfm * sizeof(ArgusWfmBlock),
+                                   hipMemcpyDeviceToHost, s->copy));
+    if ((dl & GPUD_DL_AUDIO) && sl.aout_total > 0)
+        ARGUS_CHECK(hipMemcpyAsync(sl.h_aout, sl.d_aout, (size_t)sl.aout_total * sizeof(float),
+                                   hipMemcpyDeviceToHost, s->copy));
+    if ((dl & GPUD_DL_PREDEMOD) && sl.pre_total > 0)
+        ARGUS_CHECK(hipMemcpyAsync(sl.h_pre, sl.d_pre, (size_t)sl.pre_total * sizeof(float2), hipMemcpyDeviceToHost,
+                                   s->copy));
     if (sl.n_rdec > 0)  // decoded RDS groups come back whether or not outputs do
         ARGUS_CHECK(hipMemcpyAsync(sl.h_rtab + sl.off_rout, sl.d_rtab + sl.off_rout, sl.rgroups_bytes,
                                    hipMemcpyDeviceToHost, s->copy));
@@ -1206,7 +1299,37 @@ GPUD_API int gpud_serve_submit_device(gpud_serve* s, const gpud_float2* iq_dev, 
 GPUD_API int gpud_serve_set_download(gpud_serve* s, int enable)
 {
     if (!s) return -1;
-    s->download = enable != 0;
+    s->dl_mask = enable ? GPUD_DL_ALL : 0;
+    return 0;
+}
+
+GPUD_API int gpud_serve_set_download_parts(gpud_serve* s, int mask)
+{
+    if (!s || (mask & ~GPUD_DL_ALL)) return -1;
+    s->dl_mask = mask;
+    return 0;
+}
+
+GPUD_API int gpud_serve_poll(gpud_serve* s, int64_t ticket)
+{
+    if (!s) return -1;
+    ServeSlot* sl = find_slot(s, ticket);
+    if (!sl) return -2;
+    if (!sl->busy) return 1;
+    const hipError_t e = hipEventQuery(sl->ev_done);
+    if (e == hipSuccess) return 1;
+    return e == hipErrorNotReady ? 0 : (int)e;
+}
+
+GPUD_API int gpud_serve_counters(gpud_serve* s, int64_t* graph_captures, uint64_t* arena_free)
+{
+    if (!s) return -1;
+    if (graph_captures) *graph_captures = s->graph_captures;
+    if (arena_free) {
+        uint64_t f = 0;
+        for (const auto& kv : s->arena.free_list) f += kv.second;
+        *arena_free = f;
+    }
     return 0;
 }
 
@@ -1315,8 +1438,9 @@ GPUD_API int gpud_serve_set_audio(gpud_serve* s, int64_t channel_id, int audio_r
     }
     const size_t old_fp = c.audio_rate > 0 ? audio_footprint(s, c, c.aL, c.aM) : 0;
     if (audio_rate == 0) {
-        if (c.d_aud) hipFree(c.d_aud);
+        s->arena.release(c.d_aud, c.aud_bytes);
         c.d_aud = nullptr;
+        c.aud_bytes = 0;
         c.audio_rate = 0;
         c.footprint -= old_fp;
         s->bytes_in_use -= old_fp;
@@ -1346,9 +1470,14 @@ GPUD_API int gpud_serve_set_audio(gpud_serve* s, int64_t channel_id, int audio_r
         fi = s->filters.emplace(key, d).first;
     }
     // fresh state: zero delay lines and de-emphasis
-    if (!c.d_aud) ARGUS_CHECK(dev_alloc(&c.d_aud, kAudBytes));
+    if (!c.d_aud) {
+        c.d_aud = s->arena.alloc(kAudBytes);
+        if (!c.d_aud) return GPUD_ERR_BUDGET;
+        c.aud_bytes = kAudBytes;
+    }
     const std::vector<char> zero(kAudBytes, 0);
-    ARGUS_CHECK(hipMemcpy(c.d_aud, zero.data(), kAudBytes, hipMemcpyHostToDevice));
+    const int ue = stage_upload(s, c.d_aud, zero.data(), kAudBytes);
+    if (ue) return ue;
     c.audio_rate = audio_rate;
     c.deemph_us = deemphasis_us;
     c.aL = L;
@@ -1392,14 +1521,20 @@ GPUD_API int gpud_serve_set_rds(gpud_serve* s, int64_t channel_id, int window_sa
     const size_t fp = window_samples > 0 ? rds_footprint(c, window_samples) : 0;
     if (s->bytes_in_use - old_fp + fp > s->budget) return GPUD_ERR_BUDGET;
     if (c.d_rdec && window_samples != c.rds_win) {
-        hipFree(c.d_rdec);
+        s->arena.release(c.d_rdec, c.rdec_bytes);
         c.d_rdec = nullptr;
+        c.rdec_bytes = 0;
     }
     if (window_samples > 0) {  // a fresh decoder: zero state and progress
         const RdsLayout l = rds_layout(c, window_samples);
-        if (!c.d_rdec) ARGUS_CHECK(dev_alloc(&c.d_rdec, l.bytes));
+        if (!c.d_rdec) {
+            c.d_rdec = s->arena.alloc(l.bytes);
+            if (!c.d_rdec) return GPUD_ERR_BUDGET;
+            c.rdec_bytes = l.bytes;
+        }
         const std::vector<char> zero(l.off_bits, 0);
-        ARGUS_CHECK(hipMemcpy(c.d_rdec, zero.data(), zero.size(), hipMemcpyHostToDevice));
+        const int ue = stage_upload(s, c.d_rdec, zero.data(), zero.size());
+        if (ue) return ue;
     }
     s->n_rds_on += (window_samples > 0 ? 1 : 0) - (c.rds_win > 0 ? 1 : 0);
     c.rds_win = window_samples;

@@ -7,8 +7,10 @@
 #include "rds.hpp"
 #include "fastconv.hpp"
 #include "predemod.hpp"
+#include <deque>
 #include <map>
 #include <tuple>
+#include <utility>
 #include <vector>
 
 #define ARGUS_SERVE_SLOTS 3
@@ -17,6 +19,12 @@ struct ServeChannel {
     gpud_channel_spec spec{};
     int decim = 0, T = 0;
     size_t footprint = 0;
+    // the channel's state block in the service's device arena (taps,
+    // modulated taps, histories, carries and the optional stage blocks point
+    // into it), and the separate arena blocks of the audio and RDS stages
+    char* d_block = nullptr;
+    size_t block_bytes = 0;
+    size_t aud_bytes = 0, rdec_bytes = 0;
     float* d_taps = nullptr;
     float2* d_ctaps = nullptr;  // modulated taps for the channel's NCO increment (T)
     float2* d_hist[2] = {nullptr, nullptr};
@@ -208,15 +216,39 @@ struct ArgusServeHdr {
     ArgusPreHdr p;
 };
 
+// Device memory of channel state: one allocation carved into 256-byte aligned
+// blocks, first fit over a coalescing free list (offset -> bytes).  Opening
+// or retiring a channel is host bookkeeping only.
+struct ServeArena {
+    char* base = nullptr;
+    size_t size = 0;
+    std::map<size_t, size_t> free_list;
+    char* alloc(size_t bytes);
+    void release(char* p, size_t bytes);
+};
+
+// Pinned staging of channel-state uploads: regions taken in ring order, each
+// copied to the device on the compute stream (ahead of the channel's first
+// block) and reusable once the event recorded after its copy has completed.
+struct ServeStage {
+    char* ring = nullptr;
+    size_t size = 0, head = 0;
+    std::deque<std::pair<std::pair<size_t, size_t>, hipEvent_t>> busy;  // (offset, bytes), event
+    std::vector<hipEvent_t> spare;
+};
+
 struct gpud_serve {
     ArgusTune tune = argus_default_tune();
+    ServeArena arena;
+    ServeStage stage;
+    int64_t graph_captures = 0;  // instantiations of the per-slot graphs
     uint64_t epoch = 1;  // bumped when the channel set or the tune changes
     bool graph = true;  // replay a captured per-slot graph of the block's launches
     int sample_rate = 0;
     int block_samples = 0;
     uint64_t budget = 0;
     uint64_t bytes_in_use = 0;
-    bool download = true;
+    int dl_mask = GPUD_DL_ALL;  // parts of each block's outputs downloaded (GPUD_DL_*)
     hipStream_t compute = nullptr, copy = nullptr;
     std::map<int64_t, ServeChannel> live;  // ordered by id: a deterministic channel order per block
     std::vector<ServeChannel> closed;     // awaiting their last block

@@ -362,6 +362,12 @@ typedef struct gpud_serve_block_info {
     float   pad;
 } gpud_serve_block_info;
 
+/* hbm_budget_bytes bounds the channels admitted (their state plus their share
+ * of the pipelined output buffers); channel state lives in one arena of
+ * min(budget, 16 GiB) allocated here, so opening and retiring channels
+ * allocate and free nothing on the device (their initial state is copied
+ * asynchronously on the service's compute stream, ahead of their first
+ * block). */
 GPUD_API int gpud_serve_create(int sample_rate, int block_samples, uint64_t hbm_budget_bytes, gpud_serve** out);
 GPUD_API int gpud_serve_destroy(gpud_serve* s);
 /* 0, -1 bad args / non-integer decimation, GPUD_ERR_DUPLICATE, GPUD_ERR_BUDGET */
@@ -388,6 +394,23 @@ GPUD_API int gpud_serve_submit(gpud_serve* s, const gpud_float2* iq_host, int n,
 GPUD_API int gpud_serve_submit_device(gpud_serve* s, const gpud_float2* iq_dev, int n, int64_t* ticket);
 /* Disable (0) / enable (1, default) the download of outputs to host memory. */
 GPUD_API int gpud_serve_set_download(gpud_serve* s, int enable);
+/* Which outputs of each block come to host memory (the others stay on the
+ * device; their host views keep older contents). */
+enum {
+    GPUD_DL_IQ = 1,        /* baseband at out_rate */
+    GPUD_DL_DEMOD = 2,     /* demodulator output (L/R frames for stereo) */
+    GPUD_DL_WFM = 4,       /* RDS baseband and stereo lock flags */
+    GPUD_DL_AUDIO = 8,     /* audio-stage output */
+    GPUD_DL_PREDEMOD = 16, /* pre-demod stage output */
+    GPUD_DL_ALL = 31
+};
+GPUD_API int gpud_serve_set_download_parts(gpud_serve* s, int mask);
+/* Non-blocking completion test of a submitted block: 1 when its outputs
+ * (those downloaded) are on the host, 0 while it runs, -2 unknown ticket. */
+GPUD_API int gpud_serve_poll(gpud_serve* s, int64_t ticket);
+/* Graph instantiations so far (a block whose captured graphs still fit is
+ * replayed, not re-captured) and the free bytes of the channel-state arena. */
+GPUD_API int gpud_serve_counters(gpud_serve* s, int64_t* graph_captures, uint64_t* arena_free);
 /* Wait for a block.  -2 if the ticket is unknown or its slot was reused. */
 GPUD_API int gpud_serve_wait(gpud_serve* s, int64_t ticket, gpud_serve_block_info* info);
 /* A channel's outputs in a completed block (pinned host memory, valid until

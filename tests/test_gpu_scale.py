@@ -6,7 +6,10 @@
   persistent grid, so its grid-stride loop runs.  Every channel of the
   served form (fast form on) against the direct form (fast form off) over
   four blocks, and a sampled subset of every decimation class against the
-  oracle chain (streaming host core + demodulators).
+  oracle chain (streaming host core + demodulators).  The input is noise, so
+  an FM discriminator output near +-pi can come out on either side of the
+  branch cut for a last-bit difference of its inputs: FM outputs are
+  compared as phase steps, their difference taken modulo 2 pi.
 * WFM multiplex: 20 stereo stations with RDS at SNR 15-45 dB, 200 blocks
   (4 s), decimation 80 to 250 ksps, RDS decoded on the device in 0.5 s
   windows.  Against the oracle chain run on the same blocks (extraction,
@@ -29,6 +32,16 @@ pytestmark = pytest.mark.gpu
 FS = 20_000_000
 BLOCK = 400_000
 THREADS = min(16, os.cpu_count() or 4)
+
+
+def audio_rel(mode, got, want):
+    """Relative RMS of demodulator outputs; FM (NFM/WFM) phase steps modulo 2 pi."""
+    from argus import serve as S
+    if mode in (S.DEMOD_NFM, S.DEMOD_WFM):
+        d = np.angle(np.exp(1j * (np.asarray(got, np.float64) - np.asarray(want, np.float64))))
+        den = np.sqrt(np.mean(np.asarray(want, np.float64) ** 2))
+        return float(np.sqrt(np.mean(d ** 2)) / den) if len(d) else 0.0
+    return rel_rms(got, want)
 
 
 def _sample(chans):
@@ -69,7 +82,7 @@ def test_serving_mix_fast_and_direct_forms_and_oracle(gpu):
         for f, s in svc.items():
             s.Wait(t[f])
         worst = 0.0
-        for (cid, *_rest) in chans:
+        for (cid, _off, _bw, _rate, _taps, mode) in chans:
             a_iq, a_au, _ = svc[True].Output(t[True], cid)
             b_iq, b_au, _ = svc[False].Output(t[False], cid)
             assert len(a_iq) == len(b_iq) and len(a_au) == len(b_au), cid
@@ -77,7 +90,7 @@ def test_serving_mix_fast_and_direct_forms_and_oracle(gpu):
             worst = max(worst, e)
             assert e <= 1e-5, (cid, k, e)
             if len(a_au):
-                assert rel_rms(a_au, b_au) <= 1e-4, (cid, k)
+                assert audio_rel(mode, a_au, b_au) <= 1e-4, (cid, k, audio_rel(mode, a_au, b_au))
             differ += int(not np.array_equal(a_iq, b_iq))
             if cid in want:
                 w_iq, w_au = want[cid]
@@ -86,11 +99,13 @@ def test_serving_mix_fast_and_direct_forms_and_oracle(gpu):
                     acc[cid][i].append(v)
         print(f"  fast vs direct worst rel RMS {worst:.2e}", flush=True)
     assert differ >= fast_eligible  # the eligible channels took the fast form after their first block
+    modes = {c[0]: c[5] for c in chans}
     for cid, (gi, wi, ga, wa) in acc.items():
         e = rel_rms(np.concatenate(gi), np.concatenate(wi))
         assert e <= 1e-4, (cid, e)
         if sum(len(x) for x in wa):
-            assert rel_rms(np.concatenate(ga), np.concatenate(wa)) <= 1e-4, cid
+            ea = audio_rel(modes[cid], np.concatenate(ga), np.concatenate(wa))
+            assert ea <= 1e-4, (cid, ea)
     for s in svc.values():
         s.Close()
 
