@@ -429,15 +429,19 @@ GPUD_API int gpud_serve_set_graph(gpud_serve* s, int enable);
 GPUD_API int gpud_serve_get_tune(gpud_serve* s, gpud_tune* t);
 GPUD_API int gpud_serve_set_tune(gpud_serve* s, const gpud_tune* t);
 /* Autotune the tile and launch shape on the live channel set with a block
- * at iq_dev (device memory, n samples): times the fused extraction launch
- * (best of reps) over grid, LDS budget and narrow item size, one parameter at
- * a time, keeps the fastest and returns it.  Channel state does not advance. */
-/* Event-timed fused extraction launch of the live channel set at the
- * current tune on the block at iq_dev (n samples, device memory): reps
- * launches on a scratch slot after one warm-up (blocks in flight are
- * completed first; channel state does not advance); average and minimum
- * milliseconds (either pointer may be NULL).  No reference counterpart:
- * the measurement behind a kernel roofline. */
+ * at iq_dev (device memory, n samples): times the extraction stage (best of
+ * reps) over grid, LDS budget and narrow item size, one parameter at a
+ * time, keeps the fastest and returns it.  Channel state does not advance.
+ * Tune after the channels' first block, so channels that take the fast
+ * form once their history is full are timed in it. */
+/* Event-timed extraction stage of the live channel set at the current tune
+ * on the block at iq_dev (n samples, device memory): the fast form's
+ * transforms and per-channel pass, then the fused direct launch, as a block
+ * runs them, without the stream-tail update; reps launches on a scratch slot
+ * after one warm-up (blocks in flight are completed first; channel state
+ * does not advance); average and minimum milliseconds (either pointer may
+ * be NULL).  No reference counterpart: the measurement behind a kernel
+ * roofline. */
 GPUD_API int gpud_serve_extract_time(gpud_serve* s, const gpud_float2* iq_dev, int n, int reps, float* avg_ms,
                                      float* min_ms);
 GPUD_API int gpud_serve_autotune(gpud_serve* s, const gpud_float2* iq_dev, int n, int reps, gpud_tune* best);
