@@ -488,6 +488,7 @@ GPUD_API int gpud_serve_open_ex(gpud_serve* s, const gpud_channel_spec* spec, co
         return e;
     }
     s->bytes_in_use += fp;
+    if (c.d_fastG) s->n_fast_capable++;
     s->live.emplace(spec->channel_id, c);
     s->epoch++;
     return 0;
@@ -506,6 +507,7 @@ GPUD_API int gpud_serve_close(gpud_serve* s, int64_t channel_id)
     auto it = s->live.find(channel_id);
     if (it == s->live.end()) return -1;
     if (it->second.rds_win > 0) s->n_rds_on--;
+    if (it->second.d_fastG) s->n_fast_capable--;
     s->closed.push_back(it->second);
     s->live.erase(it);
     s->epoch++;
@@ -897,16 +899,21 @@ static int describe_block(gpud_serve* s, ServeSlot& sl, const std::vector<ServeO
     hdr.d = ArgusDemodHdr{(const ArgusDemodDesc*)(d + off_ddesc), (const ArgusDemodItem*)(d + off_ditems), n_ditems, 0};
     hdr.w = ArgusWfmHdr{(const ArgusWfmDesc*)(d + off_wdesc), (const ArgusWfmItem*)(d + off_witems), n_wfm, n_witems};
     hdr.a = ArgusAudioHdr{(const ArgusAudioDesc*)(d + off_adesc), n_audio, 0};
-    sl.fast_S = n_fast > 0 ? argus_fast_segments(n) : 0;
+    // the fast stage runs while any live channel can take the fast form
+    // (channels come and go without changing what the captured graph holds)
+    sl.fast_S = s->fast && s->n_fast_capable > 0 ? argus_fast_segments(n) : 0;
     hdr.f = ArgusFastHdr{(const ArgusFastDesc*)(d + off_fdesc), s->d_ftail, s->d_fu, s->d_fX, n_fast, sl.fast_S};
     sl.a_mode = n_audio > 0 ? 1 : 0;
     hdr.p = ArgusPreHdr{(const ArgusPreDesc*)(d + off_pdesc), (const ArgusPreItem*)(d + off_pitems), n_pitems, 0};
     sl.p_mode = n_pitems > 0 ? 1 : 0;
     sl.tab_bytes = total;
     sl.off_hdr = off_hdr;
-    // dynamic LDS in 8 KiB steps, so the captured graphs survive most
-    // changes of the channel set
-    sl.lds = lds ? std::min<size_t>((lds + 8191) & ~(size_t)8191, 160 * 1024) : 0;
+    // dynamic LDS: the largest any block has needed since the tune was set,
+    // in 8 KiB steps, so channels coming and going do not change what the
+    // captured graphs launch (it only grows, a few times at most)
+    const size_t lq = lds ? std::min<size_t>((lds + 8191) & ~(size_t)8191, 160 * 1024) : 0;
+    if (cache_items) s->lds_hw = std::max(s->lds_hw, lq);
+    sl.lds = cache_items ? s->lds_hw : lq;
     return 0;
 }
 
@@ -1614,6 +1621,7 @@ GPUD_API int gpud_serve_set_tune(gpud_serve* s, const gpud_tune* t)
 {
     if (!s || !t || !valid_tune(*t)) return -1;
     s->tune = ArgusTune{t->lds_entries, t->narrow_outputs, t->grid};
+    s->lds_hw = 0;
     s->epoch++;
     return 0;
 }
@@ -1724,6 +1732,7 @@ GPUD_API int gpud_serve_autotune(gpud_serve* s, const gpud_float2* iq_dev, int n
         }
     }
     s->tune = cur;
+    s->lds_hw = 0;
     s->epoch++;
     hipStreamSynchronize(s->compute);
     slot_release(scratch);

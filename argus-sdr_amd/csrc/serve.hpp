@@ -242,6 +242,8 @@ struct gpud_serve {
     ServeArena arena;
     ServeStage stage;
     int64_t graph_captures = 0;  // instantiations of the per-slot graphs
+    size_t lds_hw = 0;           // largest dynamic LDS of an extraction launch since the tune was set
+    int n_fast_capable = 0;      // live channels with a fast-form spectrum
     uint64_t epoch = 1;  // bumped when the channel set or the tune changes
     bool graph = true;  // replay a captured per-slot graph of the block's launches
     int sample_rate = 0;

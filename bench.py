@@ -134,7 +134,7 @@ def open_service(chans, d_block, S, tune=None, budget=256 << 30, audio=0):
 
 
 def host_block_times(chans, d_block, host_blocks, S, IG, parts, blocks=200, warmup=10, tune=None,
-                     pace_ms=PACE_MS, tune_out=None):
+                     pace_ms=PACE_MS, tune_out=None, gpu_out=None, audio=0):
     """Per-block latency from handing a block to the library to its outputs
     (the `parts`, argus.serve DL_* bits) in host memory.  Blocks are pushed
     into the ingest ring (copy into a pinned slot, asynchronous H2D on the
@@ -142,12 +142,13 @@ def host_block_times(chans, d_block, host_blocks, S, IG, parts, blocks=200, warm
     device, up to three in flight; completion is polled without blocking, so
     a block's end is seen within microseconds."""
     from collections import deque
-    s, info = open_service(chans, d_block, S, tune)
+    s, info = open_service(chans, d_block, S, tune, audio=audio)
     if tune_out is not None:
         tune_out.update(info)
     s.SetDownloadParts(parts)
     g = IG.Ingest(BLOCK, depth=6)
     lat = []
+    gpu = gpu_out if gpu_out is not None else []
     pending = deque()
     total = warmup + blocks
     i = 0
@@ -165,10 +166,11 @@ def host_block_times(chans, d_block, host_blocks, S, IG, parts, blocks=200, warm
         if pending and s.Poll(pending[0][0]):
             t_done = time.perf_counter()
             tk, seq, t_in, timed = pending.popleft()
-            s.Wait(tk)
+            info = s.Wait(tk)
             g.Release(seq)
             if timed:
                 lat.append((t_done - t_in) * 1e3)
+                gpu.append(info.gpu_ms)
             continue
         time.sleep(0)
     g.Close()
@@ -889,7 +891,7 @@ def main() -> int:
     ap.add_argument("--roof-reps", type=int, default=20)
     ap.add_argument("--rds-stations", type=int, default=1024,
                     help="stations per device RDS decode call in the RDS decoding figure (0: skip it)")
-    ap.add_argument("--only", choices=["rds", "surveillance", "roofline"], default="",
+    ap.add_argument("--only", choices=["rds", "surveillance", "roofline", "churn"], default="",
                     help="run only this secondary figure and print it")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extras", action="store_true",
@@ -943,18 +945,29 @@ def main() -> int:
     if a.only == "roofline":
         print(json.dumps({"roofline": bench_roofline(a, d0.value, S)}), flush=True)
         return 0
+    if a.only == "churn":
+        print(json.dumps({"churn_serving": bench_churn(a, d0.value, blocks, S, IG)}), flush=True)
+        return 0
 
     # ---- value: largest N with host-inclusive p99 <= 5 ms (demod outputs on host)
     sweep, sweep_iq = {}, {}
     tune_info = {}
+    gpu_at = {}
     if a.channels > 0:
         n_rt = a.channels
+        gpu_at[n_rt] = []
+        lat_n = host_block_times(make_channels(n_rt), d0.value, blocks, S, IG, S.DL_DEMOD, a.sweep_blocks,
+                                 tune=a.tune_fixed, gpu_out=gpu_at[n_rt])
+        sweep = {n_rt: (percentile(lat_n, 50), percentile(lat_n, 99))}
     else:
         print("[bench] capacity sweep: demodulated outputs on the host, p99 <= 5 ms", file=sys.stderr, flush=True)
-        n_rt, sweep = capacity_sweep(
-            lambda n: host_block_times(make_channels(n), d0.value, blocks, S, IG, S.DL_DEMOD, a.sweep_blocks,
-                                       tune=a.tune_fixed, tune_out=tune_info),
-            1024, P99_BOUND_MS, a.sweep_cap, 256)
+        gpu_at = {}
+
+        def point(n):
+            gpu_at[n] = []
+            return host_block_times(make_channels(n), d0.value, blocks, S, IG, S.DL_DEMOD, a.sweep_blocks,
+                                    tune=a.tune_fixed, tune_out=tune_info, gpu_out=gpu_at[n])
+        n_rt, sweep = capacity_sweep(point, 1024, P99_BOUND_MS, a.sweep_cap, 256)
     chans = make_channels(n_rt)
     print(f"[bench] timed steps: {n_rt} channels, block {BLOCK} samples @ {FS / 1e6:.0f} Msps", file=sys.stderr,
           flush=True)
@@ -989,6 +1002,15 @@ def main() -> int:
             "fused_over_plain_basis": "real-time channels with baseband IQ on the host, both forms host-inclusive"}
         print("[bench] serving under churn (Poisson arrivals, exponential lifetimes)", file=sys.stderr, flush=True)
         extras["churn_serving"] = bench_churn(a, d0.value, blocks, S, IG)
+        print("[bench] capacity sweep: 48 kHz audio-stage output on the host", file=sys.stderr, flush=True)
+        n_aud, sweep_aud = capacity_sweep(
+            lambda n: host_block_times(make_channels(n), d0.value, blocks, S, IG, S.DL_AUDIO, a.sweep_blocks,
+                                       tune=a.tune_fixed, audio=48000),
+            max(1024, n_rt // 1024 * 1024), P99_BOUND_MS, a.sweep_cap, 256)
+        extras["audio_on_host"] = {
+            "real_time_channels": n_aud, "sweep": sweep_table(sweep_aud),
+            "form": "the same path plus the audio stage (plain-WFM low-pass, resampling to 48 kHz, WFM "
+                    "de-emphasis), only its 48 kHz output downloaded"}
     print(f"[bench] roofline: extraction stage at {N_ROOF} channels", file=sys.stderr, flush=True)
     roof = bench_roofline(a, d0.value, S)
     if not a.no_extras:
@@ -1048,8 +1070,14 @@ def main() -> int:
                    "modes": "NFM/AM/USB/WFM (+ demod)", "decim": "8-1600", "taps": "63-511", "seed": 0,
                    "p99_bound_ms": P99_BOUND_MS, "latency": "block handed to the ingest ring -> demodulated "
                    "outputs in host memory; blocks every 10 ms (2x real time), up to 3 in flight"},
-        "p50_block_ms": round(percentile(lat, 50), 4),
-        "p99_block_ms": round(percentile(lat, 99), 4),
+        "p50_block_ms": round(sweep[n_rt][0], 4) if n_rt in sweep else None,
+        "p99_block_ms": round(sweep[n_rt][1], 4) if n_rt in sweep else None,
+        "p50_block_gpu_ms": round(percentile(gpu_at.get(n_rt, []), 50), 4) if gpu_at.get(n_rt) else None,
+        "p99_block_gpu_ms": round(percentile(gpu_at.get(n_rt, []), 99), 4) if gpu_at.get(n_rt) else None,
+        "timed_steps": {"form": "K blocks back to back through the same path, two in flight (throughput; "
+                                "latency here includes waiting behind the previous block)",
+                        "p50_block_ms": round(percentile(lat, 50), 4),
+                        "p99_block_ms": round(percentile(lat, 99), 4)},
         "p50_kernels_ms": round(percentile(kern, 50), 4),
         "p99_kernels_ms": round(percentile(kern, 99), 4),
         "p50_extract_ms": round(percentile(xk, 50), 4),
