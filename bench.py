@@ -852,7 +852,7 @@ def bench_roofline(a, d_block, S):
     fused direct launch) on the service's compute stream, averaged over
     a.roof_reps launches after autotuning on the steady state."""
     chans = make_channels(N_ROOF)
-    s, tune = open_service(chans, d_block, S)
+    s, tune = open_service(chans, d_block, S, a.tune_fixed)
     avg, mn = s.ExtractTime(d_block, BLOCK, a.roof_reps)
     s.Close()
     b_alg, f_alg = alg_figures(chans, BLOCK)
