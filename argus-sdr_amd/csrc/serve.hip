@@ -266,6 +266,10 @@ static void slot_release(ServeSlot& sl)
     hipHostFree(sl.h_wblk);
     hipHostFree(sl.h_tab);
     hipFree(sl.d_tab);
+    hipHostFree(sl.h_items);
+    hipFree(sl.d_items);
+    sl.h_items = sl.d_items = nullptr;
+    sl.h_items_cap = sl.d_items_cap = 0;
     hipFree(sl.d_aout);
     hipHostFree(sl.h_aout);
     sl.d_aout = sl.h_aout = nullptr;
@@ -590,8 +594,8 @@ static void audio_describe(const ServeChannel& c, const ServeOut& o, ServeSlot& 
 
 static size_t align_up(size_t v) { return (v + 255) & ~(size_t)255; }
 
-// Grow a slot's table to at least `need` bytes, keeping its first `keep` bytes.
-static int grow_table(ServeSlot& sl, size_t need, size_t keep)
+// Grow a slot's table to at least `need` bytes (its contents are rewritten by the caller).
+static int grow_table(ServeSlot& sl, size_t need)
 {
     if (sl.h_tab && sl.tab_cap >= need) return 0;
     const size_t cap = std::max<size_t>(need + need / 4, 1 << 16);
@@ -603,7 +607,6 @@ static int grow_table(ServeSlot& sl, size_t need, size_t keep)
         hipHostFree(h);
         return (int)e;
     }
-    if (keep && sl.h_tab) std::memcpy(h, sl.h_tab, keep);
     if (sl.h_tab) hipHostFree(sl.h_tab);
     if (sl.d_tab) hipFree(sl.d_tab);
     sl.h_tab = h;
@@ -859,7 +862,7 @@ static int describe_block(gpud_serve* s, ServeSlot& sl, const std::vector<ServeO
     }
     const int n_items = reuse ? sl.n_items : (int)s->items.size();
     const int n_ditems = (int)s->ditems.size(), n_witems = (int)s->witems.size();
-    const size_t off_hdr = align_up(items_bytes);
+    const size_t off_hdr = 0;
     const size_t off_chans = align_up(off_hdr + sizeof(ArgusServeHdr));
     const size_t off_ddesc = align_up(off_chans + (size_t)nch * sizeof(ArgusChanDesc));
     const size_t off_wdesc = align_up(off_ddesc + (size_t)nch * sizeof(ArgusDemodDesc));
@@ -872,11 +875,11 @@ static int describe_block(gpud_serve* s, ServeSlot& sl, const std::vector<ServeO
     const size_t off_pdesc = align_up(off_fdesc + (size_t)n_fast * sizeof(ArgusFastDesc));
     const size_t off_pitems = align_up(off_pdesc + (size_t)n_pre * sizeof(ArgusPreDesc));
     const size_t total = off_pitems + (size_t)n_pitems * sizeof(ArgusPreItem);
-    const char* d_before = sl.d_tab;
-    ARGUS_CHECK((hipError_t)grow_table(sl, total, reuse ? items_bytes : 0));
-    if (sl.d_tab != d_before) sl.items_dirty = true;  // a new device table holds no items yet
+    ARGUS_CHECK((hipError_t)grow_table(sl, total));
     if (!reuse) {
-        if (n_items) std::memcpy(sl.h_tab, s->items.data(), items_bytes);
+        ARGUS_CHECK((hipError_t)grow_dev(&sl.d_items, &sl.d_items_cap, (size_t)std::max(n_items, 1)));
+        ARGUS_CHECK((hipError_t)grow_host(&sl.h_items, &sl.h_items_cap, (size_t)std::max(n_items, 1)));
+        if (n_items) std::memcpy(sl.h_items, s->items.data(), items_bytes);
         sl.items_epoch = cache_items ? s->epoch : 0;
         sl.items_n = n;
         sl.n_items = n_items;
@@ -895,7 +898,7 @@ static int describe_block(gpud_serve* s, ServeSlot& sl, const std::vector<ServeO
     if (n_pitems) std::memcpy(sl.h_tab + off_pitems, s->pitems.data(), (size_t)n_pitems * sizeof(ArgusPreItem));
     ArgusServeHdr& hdr = *(ArgusServeHdr*)(sl.h_tab + off_hdr);
     const char* d = sl.d_tab;
-    hdr.x = ArgusExtractHdr{(const ArgusChanDesc*)(d + off_chans), (const ArgusWorkItem*)d, in, n_items, n};
+    hdr.x = ArgusExtractHdr{(const ArgusChanDesc*)(d + off_chans), sl.d_items, in, n_items, n};
     hdr.d = ArgusDemodHdr{(const ArgusDemodDesc*)(d + off_ddesc), (const ArgusDemodItem*)(d + off_ditems), n_ditems, 0};
     hdr.w = ArgusWfmHdr{(const ArgusWfmDesc*)(d + off_wdesc), (const ArgusWfmItem*)(d + off_witems), n_wfm, n_witems};
     hdr.a = ArgusAudioHdr{(const ArgusAudioDesc*)(d + off_adesc), n_audio, 0};
@@ -974,12 +977,12 @@ static int build_block(gpud_serve* s, ServeSlot& sl, int n, const float2* in, bo
 static int upload_table(gpud_serve* s, ServeSlot& sl)
 {
     if (sl.items_dirty) {
-        ARGUS_CHECK(hipMemcpyAsync(sl.d_tab, sl.h_tab, sl.tab_bytes, hipMemcpyHostToDevice, s->compute));
+        if (sl.n_items > 0)
+            ARGUS_CHECK(hipMemcpyAsync(sl.d_items, sl.h_items, (size_t)sl.n_items * sizeof(ArgusWorkItem),
+                                       hipMemcpyHostToDevice, s->compute));
         sl.items_dirty = false;
-    } else {
-        ARGUS_CHECK(hipMemcpyAsync(sl.d_tab + sl.off_hdr, sl.h_tab + sl.off_hdr, sl.tab_bytes - sl.off_hdr,
-                                   hipMemcpyHostToDevice, s->compute));
     }
+    ARGUS_CHECK(hipMemcpyAsync(sl.d_tab, sl.h_tab, sl.tab_bytes, hipMemcpyHostToDevice, s->compute));
     return 0;
 }
 
@@ -1644,7 +1647,8 @@ static int time_extract(gpud_serve* s, ServeSlot& scratch, const float2* iq_dev,
     if (rc) return rc;
     rc = argus_extract_allow_lds(scratch.lds);
     if (rc) return rc;
-    ARGUS_CHECK(hipMemcpyAsync(scratch.d_tab, scratch.h_tab, scratch.tab_bytes, hipMemcpyHostToDevice, s->compute));
+    rc = upload_table(s, scratch);
+    if (rc) return rc;
     const ArgusServeHdr* dh = (const ArgusServeHdr*)(scratch.d_tab + scratch.off_hdr);
     // the extraction stage as a block runs it (the fast path's transforms and
     // per-channel pass, then the direct kernel), without the tail update that

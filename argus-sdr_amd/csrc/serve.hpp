@@ -114,6 +114,12 @@ struct ServeSlot {
     // the tune, so they are rebuilt and uploaded only when those change
     // (items_epoch, items_n); everything from the header on is uploaded with
     // every block.
+    // the extraction work items, in buffers of their own (the header points
+    // at them, so rebuilding or moving them does not change what a captured
+    // graph launches; the header sits at the start of the table)
+    ArgusWorkItem* h_items = nullptr;
+    ArgusWorkItem* d_items = nullptr;
+    size_t h_items_cap = 0, d_items_cap = 0;
     char* h_tab = nullptr;
     char* d_tab = nullptr;
     size_t tab_cap = 0;
