@@ -198,6 +198,7 @@ def StereoDecodeBatch(monos, rates, mono_fallback: bool = False):
 
 
 register("gpud_stereo_pilot_present", I, [P, P, P, I, P, P, P, P, P])
+register("gpud_stereo_pilot_present_noalloc", I, [P, P, P, I, P, P, P, P, P])
 
 
 def StereoPilotPresent(iqs, rates):

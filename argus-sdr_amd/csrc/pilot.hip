@@ -19,6 +19,7 @@
 #include "common.hpp"
 
 #include <cstring>
+#include <mutex>
 #include <vector>
 
 #define ARGUS_PILOT_BINS 201
@@ -131,11 +132,39 @@ __global__ __launch_bounds__(256) void argus_pilot_reduce_kernel(const PilotSig*
     }
 }
 
-extern "C" {
+// Workspace of gpud_stereo_pilot_present_noalloc: a file-scope device
+// array, one per GPU (each device holds its own copy until the process
+// ends), shared by every call on that GPU.  Host code never passes the
+// array's name to a kernel or a copy: it uses the device address
+// hipGetSymbolAddress gives for the current device, taken once per device.
+#define ARGUS_PILOT_SCRATCH_BYTES (16u << 20)
+#define ARGUS_PILOT_MAX_DEVICES 64
+static __device__ __align__(256) char scratch[ARGUS_PILOT_SCRATCH_BYTES];
 
-GPUD_API int gpud_stereo_pilot_present(const gpud_float2* iq_dev, const int64_t* offsets, const int32_t* rates,
-                                       int n_signals, int32_t* present, double* ratio, double* offset_hz,
-                                       double* mags201, gpud_stream_handle stream)
+static std::mutex g_scratch_mu;
+static char* g_scratch_ptr[ARGUS_PILOT_MAX_DEVICES] = {};
+
+static int scratch_address(char** out)
+{
+    int dev = 0;
+    ARGUS_CHECK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= ARGUS_PILOT_MAX_DEVICES) return -1;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return (int)e;
+        g_scratch_ptr[dev] = (char*)p;
+    }
+    *out = g_scratch_ptr[dev];
+    return 0;
+}
+
+// The batch pilot test with its workspace either taken from the
+// stream-ordered pool (ws == nullptr) or the caller's ws of ws_cap bytes.
+static int pilot_present(const gpud_float2* iq_dev, const int64_t* offsets, const int32_t* rates, int n_signals,
+                         int32_t* present, double* ratio, double* offset_hz, double* mags201,
+                         gpud_stream_handle stream, char* ws, size_t ws_cap)
 {
     if (n_signals < 0 || (n_signals > 0 && (!offsets || !rates || !present))) return -1;
     if (n_signals == 0) return 0;
@@ -177,8 +206,12 @@ GPUD_API int gpud_stereo_pilot_present(const gpud_float2* iq_dev, const int64_t*
     std::vector<char> h(off_part, 0);
     std::memcpy(h.data(), items.data(), items.size() * sizeof(PilotItem));
     std::memcpy(h.data() + off_sigs, sigs.data(), sigs.size() * sizeof(PilotSig));
-    char* d = nullptr;
-    ARGUS_CHECK(hipMallocAsync((void**)&d, bytes, st));
+    char* d = ws;
+    if (ws) {
+        if (bytes > ws_cap) return -7;  // the workspace this call needs exceeds the scratch array
+    } else {
+        ARGUS_CHECK(hipMallocAsync((void**)&d, bytes, st));
+    }
     std::vector<double> mg((size_t)nl * ARGUS_PILOT_BINS);
     std::vector<PilotOut> out((size_t)nl);
     int err = (int)hipMemcpyAsync(d, h.data(), off_part, hipMemcpyHostToDevice, st);
@@ -198,7 +231,7 @@ GPUD_API int gpud_stereo_pilot_present(const gpud_float2* iq_dev, const int64_t*
         err = (int)hipMemcpyAsync(mg.data(), d + off_mags, mg.size() * sizeof(double), hipMemcpyDeviceToHost, st);
     if (!err)
         err = (int)hipMemcpyAsync(out.data(), d + off_out, out.size() * sizeof(PilotOut), hipMemcpyDeviceToHost, st);
-    hipFreeAsync(d, st);
+    if (!ws) hipFreeAsync(d, st);
     const hipError_t se = hipStreamSynchronize(st);
     if (!err) err = (int)se;
     if (err) return err;
@@ -212,6 +245,26 @@ GPUD_API int gpud_stereo_pilot_present(const gpud_float2* iq_dev, const int64_t*
                         ARGUS_PILOT_BINS * sizeof(double));
     }
     return 0;
+}
+
+extern "C" {
+
+GPUD_API int gpud_stereo_pilot_present(const gpud_float2* iq_dev, const int64_t* offsets, const int32_t* rates,
+                                       int n_signals, int32_t* present, double* ratio, double* offset_hz,
+                                       double* mags201, gpud_stream_handle stream)
+{
+    return pilot_present(iq_dev, offsets, rates, n_signals, present, ratio, offset_hz, mags201, stream, nullptr, 0);
+}
+
+GPUD_API int gpud_stereo_pilot_present_noalloc(const gpud_float2* iq_dev, const int64_t* offsets,
+                                               const int32_t* rates, int n_signals, int32_t* present, double* ratio,
+                                               double* offset_hz, double* mags201, gpud_stream_handle stream)
+{
+    char* ws = nullptr;
+    const int e = scratch_address(&ws);
+    if (e) return e;
+    return pilot_present(iq_dev, offsets, rates, n_signals, present, ratio, offset_hz, mags201, stream, ws,
+                         ARGUS_PILOT_SCRATCH_BYTES);
 }
 
 }  // extern "C"

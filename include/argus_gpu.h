@@ -637,6 +637,18 @@ GPUD_API int gpud_wfm_stereo_batch(const float* mono_dev, const int64_t* offsets
 GPUD_API int gpud_stereo_pilot_present(const gpud_float2* iq_dev, const int64_t* offsets, const int32_t* rates,
                                        int n_signals, int32_t* present, double* ratio, double* offset_hz,
                                        double* mags201, gpud_stream_handle stream);
+/* No reference counterpart: the same call for callers that cannot allocate
+ * device memory (tight serving loops, code that must not call hipMalloc).
+ * Results identical to gpud_stereo_pilot_present; its workspace (the chunk
+ * table, partial DFT sums and magnitudes: about 3.2 KiB per 1024
+ * discriminator samples analysed) lives in a file-scope device array of
+ * 16 MiB per GPU instead of the stream-ordered pool, so it allocates
+ * nothing; a batch needing more returns -7 before any launch.  Every call
+ * on one GPU shares that array: calls that overlap on one GPU (another
+ * thread, another stream) race, so serialise them. */
+GPUD_API int gpud_stereo_pilot_present_noalloc(const gpud_float2* iq_dev, const int64_t* offsets,
+                                               const int32_t* rates, int n_signals, int32_t* present, double* ratio,
+                                               double* offset_hz, double* mags201, gpud_stream_handle stream);
 
 /* ------------------------------------------------------------------ */
 /* Surveillance: Welch PSD, FFT, CFAR and the detection list.           */
