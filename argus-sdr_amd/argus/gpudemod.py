@@ -21,6 +21,85 @@ import numpy as np
 
 from ._lib import (ErrUnavailable, GpudError, check, gpud_streaming_job, gpud_streaming_result, lib, ptr, register)
 
+P_, I_, L_ = C.c_void_p, C.c_int, C.c_int64
+register("gpud_out_rings_create", I_, [C.POINTER(P_)])
+register("gpud_out_rings_destroy", I_, [P_])
+register("gpud_out_rings_next", I_, [P_, L_, L_, C.POINTER(P_)])
+register("gpud_out_rings_prune", I_, [P_, P_, I_])
+register("gpud_out_rings_pool_get", I_, [P_, L_, C.POINTER(P_), C.POINTER(L_)])
+register("gpud_out_rings_pool_put", I_, [P_, P_, L_])
+register("gpud_out_rings_stats", I_, [P_, C.POINTER(I_), C.POINTER(I_), C.POINTER(L_), C.POINTER(L_)])
+register("gpud_out_rings_slot", I_, [P_, L_, C.POINTER(I_), P_])
+register("gpud_runner_streaming_extract_ring", I_, [P_, P_, I_, P_, I_, P_, P_])
+register("gpud_runner_out_rings", P_, [P_])
+
+OUT_RING_DEPTH = 4  # streaming_out_ring.go:11 streamOutRingDepth
+
+
+class OutRings:
+    """Per-signal output rings and their shared free pool (streamOutRing /
+    outBufPool, streaming_out_ring.go:22-112).  Next(id, n) returns the host
+    address of the buffer that takes signal id's next n outputs; Prune(active)
+    returns the rings of other signals to the pool.  A runner's own rings
+    (BatchRunner.OutRings()) are borrowed and never destroyed here."""
+
+    def __init__(self, handle=None):
+        self._own = handle is None
+        if handle is None:
+            h = C.c_void_p()
+            check("gpud_out_rings_create", lib().gpud_out_rings_create(C.byref(h)))
+            handle = h.value
+        self._h = handle
+
+    def Close(self) -> None:
+        if self._h and self._own:
+            lib().gpud_out_rings_destroy(self._h)
+        self._h = None
+
+    def __del__(self):
+        try:
+            self.Close()
+        except Exception:
+            pass
+
+    def Next(self, signal_id: int, n: int) -> int:
+        b = C.c_void_p()
+        check("gpud_out_rings_next", lib().gpud_out_rings_next(self._h, int(signal_id), int(n), C.byref(b)))
+        return b.value or 0
+
+    def Prune(self, active) -> None:
+        a = np.ascontiguousarray(list(active), np.int64)
+        check("gpud_out_rings_prune", lib().gpud_out_rings_prune(self._h, a.ctypes.data if len(a) else None, len(a)))
+
+    def PoolGet(self, n: int) -> tuple[int, int]:
+        b, cap = C.c_void_p(), C.c_int64()
+        check("gpud_out_rings_pool_get", lib().gpud_out_rings_pool_get(self._h, int(n), C.byref(b), C.byref(cap)))
+        return b.value, cap.value
+
+    def PoolPut(self, buf: int, cap: int) -> None:
+        check("gpud_out_rings_pool_put", lib().gpud_out_rings_pool_put(self._h, buf, int(cap)))
+
+    def Stats(self) -> dict:
+        nr, nf, a, r = C.c_int(), C.c_int(), C.c_int64(), C.c_int64()
+        check("gpud_out_rings_stats", lib().gpud_out_rings_stats(self._h, C.byref(nr), C.byref(nf), C.byref(a),
+                                                                  C.byref(r)))
+        return {"rings": nr.value, "free": nf.value, "allocated": a.value, "released": r.value}
+
+    def Slot(self, signal_id: int):
+        """(next slot index, capacities of the ring's buffers) or None."""
+        idx = C.c_int()
+        caps = np.zeros(OUT_RING_DEPTH, np.int64)
+        rc = lib().gpud_out_rings_slot(self._h, int(signal_id), C.byref(idx), caps.ctypes.data)
+        return None if rc != 0 else (idx.value, [int(c) for c in caps])
+
+
+def ring_view(addr: int, n: int) -> np.ndarray:
+    """A complex64 view of n samples at a host address (no copy)."""
+    if not addr or n <= 0:
+        return np.zeros(0, np.complex64)
+    return np.ctypeslib.as_array((C.c_float * (2 * n)).from_address(addr)).view(np.complex64)
+
+
 EXEC_BATCHED = 0
 EXEC_PER_SIGNAL = 1
 WINDOW_HANN = 0
@@ -224,6 +303,33 @@ class BatchRunner:
 
     StreamingExtractGPUExec = StreamingExtractGPU
 
+    def StreamingExtractGPURing(self, iq_new, jobs) -> list[StreamingExtractResult]:
+        """StreamingExtractGPU with the reference's output ownership
+        (streaming_gpu_native_prepare.go:78-104): each result's IQ is a view
+        of its signal's ring buffer, valid until that signal's ring comes
+        round again (OUT_RING_DEPTH calls) or the signal is pruned."""
+        if not self._h:
+            raise ErrUnavailable("runner closed")
+        iq = np.ascontiguousarray(iq_new if iq_new is not None else np.zeros(0, np.complex64), np.complex64)
+        n = len(iq)
+        jarr = _jobs_array(jobs)
+        res = (gpud_streaming_result * max(1, len(jobs)))()
+        outs = (C.c_void_p * max(1, len(jobs)))()
+        rc = lib().gpud_runner_streaming_extract_ring(self._h, ptr(iq) if n else None, n, jarr, len(jobs), outs, res)
+        if rc == -1:
+            raise ValueError("streaming extract rejected the jobs (bad arguments or non-integer decimation)")
+        check("gpud_runner_streaming_extract_ring", rc)
+        results = []
+        for i in range(len(jobs)):
+            r = res[i]
+            results.append(StreamingExtractResult(r.signal_id, ring_view(outs[i] or 0, r.n_out), r.rate, r.n_out,
+                                                  r.phase_count, r.history_len, r.nco_phase))
+        return results
+
+    def OutRings(self) -> OutRings | None:
+        h = lib().gpud_runner_out_rings(self._h)
+        return OutRings(h) if h else None
+
     def StreamingExtractDevice(self, iq_dev_ptr: int, n_new: int, jobs):
         """Device-resident input; returns (device output pointer, raw results).
         Completion is ordered on self.stream()."""
@@ -371,5 +477,5 @@ __all__ = [
     "ExtractStreamState", "StreamingConfigHash", "ExactIntegerDecimation", "BuildPolyphaseTapsPhaseMajor",
     "PolyphasePhaseLen", "LowpassFIR", "ErrUnavailable", "GpudError", "EXEC_BATCHED", "EXEC_PER_SIGNAL",
     "WINDOW_HANN", "WINDOW_HAMMING", "ExtractJob", "ExtractResult", "Engine", "New", "DemodNFM", "DemodWFM",
-    "DemodAM", "DemodUSB", "DemodLSB", "DemodCW",
+    "DemodAM", "DemodUSB", "DemodLSB", "DemodCW", "OutRings", "OUT_RING_DEPTH", "ring_view",
 ]

@@ -150,17 +150,6 @@ GPUD_API int gpud_build_extract_jobs(int sample_rate, double center_hz, const gp
 // ---------------------------------------------------------------------------
 namespace {
 
-// One job of a legacy call: taps (natural order, float32), decimation, NCO.
-struct LegacyJob {
-    const float* taps;
-    int T;
-    int D;
-    double phase_start;
-    double phase_inc;
-    int n_out;      // outputs kept: n / D
-    float2* out;    // device
-};
-
 template <typename T>
 int grow_dev(T** p, size_t* cap, size_t need)
 {
@@ -192,6 +181,8 @@ int grow_host(T** p, size_t* cap, size_t need)
 size_t align16(size_t v) { return (v + 15) & ~(size_t)15; }
 
 }  // namespace
+
+using LegacyJob = ArgusLegacyJob;
 
 // Device and pinned staging of legacy calls: per job the phase-major and
 // modulated taps, then the channel descriptors (one region), and the work

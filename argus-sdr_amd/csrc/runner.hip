@@ -6,6 +6,7 @@
 // the reference's per-signal call pattern).
 #include "extract.hpp"
 #include "runner.hpp"
+#include "outring.hpp"
 #include <algorithm>
 #include <cstdio>
 #include <cstring>
@@ -312,6 +313,7 @@ GPUD_API int gpud_runner_destroy(gpud_runner* r)
     }
     for (hipEvent_t ev : r->tev) hipEventDestroy(ev);
     argus_legacy_release(r->legacy);
+    argus_out_rings_free(r->rings);
     hipStreamDestroy(r->stream);
     delete r;
     return 0;
@@ -470,6 +472,11 @@ static int prepare_jobs(gpud_runner* r, int n_new, const gpud_streaming_job* job
         if (err) return err;
     }
     argus_runner_prune(r);
+    if (r->rings) {  // syncNativeStreamingStates prunes the output rings too (streaming_gpu_native_prepare.go:211-223)
+        std::vector<int64_t> ids((size_t)n_jobs);
+        for (int i = 0; i < n_jobs; i++) ids[(size_t)i] = jobs[i].signal_id;
+        argus_out_rings_prune(r->rings, ids.data(), n_jobs);
+    }
     sig.assign((size_t)n_jobs, nullptr);
     int64_t off = 0;
     for (int i = 0; i < n_jobs; i++) {
@@ -560,6 +567,49 @@ GPUD_API int gpud_runner_streaming_extract(gpud_runner* r, const gpud_float2* iq
     if (total > 0) std::memcpy(out, r->h_out, (size_t)total * sizeof(float2));
     return 0;
 }
+
+GPUD_API int gpud_runner_streaming_extract_ring(gpud_runner* r, const gpud_float2* iq, int n_new,
+                                                const gpud_streaming_job* jobs, int n_jobs, gpud_float2** outs,
+                                                gpud_streaming_result* results)
+{
+    if (!r || (n_new > 0 && !iq) || (n_jobs > 0 && !outs)) return -1;
+    if (!r->rings) r->rings = new gpud_out_rings();
+    std::vector<ArgusSignal*> sig;
+    int err = prepare_jobs(r, n_new, jobs, n_jobs, sig, results);
+    if (err) return err;
+    int64_t total = 0;
+    for (int i = 0; i < n_jobs; i++) total += results[i].n_out;
+    ARGUS_CHECK((hipError_t)ensure_host(&r->h_out, &r->h_out_cap, (size_t)std::max<int64_t>(total, 1)));
+    if (r->exec_mode == GPUD_EXEC_PER_SIGNAL) {
+        err = run_per_signal(r, iq, n_new, jobs, n_jobs, sig, results, (gpud_float2*)r->h_out);
+        if (err) return err;
+    } else {
+        if (n_new > 0) {
+            ARGUS_CHECK(hipStreamSynchronize(r->stream));  // h_in reuse
+            std::memcpy(r->h_in, iq, (size_t)n_new * sizeof(float2));
+            ARGUS_CHECK(hipMemcpyAsync(r->d_in, r->h_in, (size_t)n_new * sizeof(float2), hipMemcpyHostToDevice,
+                                       r->stream));
+        }
+        err = run_batched(r, r->d_in, n_new, jobs, n_jobs, sig, results);
+        if (err) return err;
+        if (total > 0)
+            ARGUS_CHECK(hipMemcpyAsync(r->h_out, r->d_out, (size_t)total * sizeof(float2), hipMemcpyDeviceToHost,
+                                       r->stream));
+        ARGUS_CHECK(hipStreamSynchronize(r->stream));
+    }
+    // each signal's outputs into its ring's next buffer (streaming_gpu_native_prepare.go:82-88)
+    for (int i = 0; i < n_jobs; i++) {
+        gpud_float2* b = nullptr;
+        if (argus_out_ring_next(r->rings, jobs[i].signal_id, results[i].n_out, &b) != 0) return -3;
+        if (results[i].n_out > 0)
+            std::memcpy(b, (const gpud_float2*)r->h_out + results[i].out_offset,
+                        (size_t)results[i].n_out * sizeof(gpud_float2));
+        outs[i] = results[i].n_out > 0 ? b : nullptr;
+    }
+    return 0;
+}
+
+GPUD_API gpud_out_rings* gpud_runner_out_rings(gpud_runner* r) { return r ? r->rings : nullptr; }
 
 GPUD_API int gpud_runner_streaming_extract_device(gpud_runner* r, const gpud_float2* iq_dev, int n_new,
                                                   const gpud_streaming_job* jobs, int n_jobs,

@@ -33,8 +33,26 @@ struct ArgusSignal {
     int out_cap = 0;
 };
 
+// One job of a legacy block call (engine.hip): taps in natural order
+// (float32), decimation, NCO start and increment (radians), outputs kept
+// (input m*D feeds output m; at most ceil(n / D)), device output.
+struct ArgusLegacyJob {
+    const float* taps;
+    int T;
+    int D;
+    double phase_start;
+    double phase_inc;
+    int n_out;
+    float2* out;
+};
+
 struct ArgusLegacyScratch;
 void argus_legacy_release(ArgusLegacyScratch* sc);
+// Queue one fused launch of the jobs over d_in[0..n) on st (engine.hip).
+// The staging in *sc is reused: synchronise st before the next call on it.
+int argus_legacy_launch(ArgusLegacyScratch*& sc, hipStream_t st, const ArgusTune& tune, const float2* d_in, int n,
+                        const std::vector<ArgusLegacyJob>& jobs);
+struct gpud_out_rings;
 
 struct gpud_runner {
     ArgusTune tune = argus_default_tune();
@@ -70,6 +88,7 @@ struct gpud_runner {
     int t_launches = 0;
     std::vector<gpud_float2> hist_host;
     ArgusLegacyScratch* legacy = nullptr;  // staging of the legacy block path (engine.hip)
+    gpud_out_rings* rings = nullptr;       // per-signal output rings (outring.hip), on first ring call
 };
 
 int argus_lowpass_fir(double cutoff_hz, int sample_rate, int taps, double* out, bool hamming);

@@ -205,6 +205,48 @@ GPUD_API int gpud_runner_state_ids(gpud_runner* r, int64_t* ids, int cap);
 GPUD_API int gpud_runner_reset_signal_state(gpud_runner* r, int64_t signal_id);
 /* stream_state.go:18-24 ResetAllSignalStates */
 GPUD_API int gpud_runner_reset_all_signal_states(gpud_runner* r);
+/* ------------------------------------------------------------------ */
+/* Output rings: successor of streamOutRing / outBufPool                 */
+/* (internal/demod/gpudemod/streaming_out_ring.go:22-112).  Each signal  */
+/* owns a ring of GPUD_OUT_RING_DEPTH host buffers; a call hands its     */
+/* outputs out in the ring's next buffer, which stays untouched for the  */
+/* following DEPTH - 1 calls.  A slot too small for a call draws a buffer */
+/* from a free pool shared by all signal IDs (first free buffer that     */
+/* fits, else the largest, released if still too small, else a new one  */
+/* of exactly n samples); rings of pruned signals return their buffers   */
+/* to the pool, which keeps at most 4 x 32 of them.  Host memory only:   */
+/* these calls need no GPU.  Not thread-safe.                            */
+/* ------------------------------------------------------------------ */
+#define GPUD_OUT_RING_DEPTH 4
+typedef struct gpud_out_rings gpud_out_rings;
+GPUD_API int gpud_out_rings_create(gpud_out_rings** out);
+GPUD_API int gpud_out_rings_destroy(gpud_out_rings* o);
+/* streaming_out_ring.go:32-45 next (with outRingFor 91-101): the buffer
+ * for signal_id's next n samples.  -3 when host memory runs out. */
+GPUD_API int gpud_out_rings_next(gpud_out_rings* o, int64_t signal_id, int64_t n, gpud_float2** buf);
+/* streaming_out_ring.go:105-112 pruneOutRings: rings of signals not in
+ * active[0..n_active) return their buffers to the pool. */
+GPUD_API int gpud_out_rings_prune(gpud_out_rings* o, const int64_t* active, int n_active);
+/* streaming_out_ring.go:56-78 outBufPool.get and 81-89 put.  A buffer put
+ * must come from this object (get or next); its ownership moves to the pool. */
+GPUD_API int gpud_out_rings_pool_get(gpud_out_rings* o, int64_t n, gpud_float2** buf, int64_t* cap);
+GPUD_API int gpud_out_rings_pool_put(gpud_out_rings* o, gpud_float2* buf, int64_t cap);
+/* Live rings, pooled buffers, and buffers allocated / freed so far. */
+GPUD_API int gpud_out_rings_stats(const gpud_out_rings* o, int* n_rings, int* n_free, int64_t* allocated,
+                                  int64_t* released);
+/* A signal's ring: the slot its next call takes and the capacity of each
+ * of its GPUD_OUT_RING_DEPTH buffers (caps may be NULL).  -1 if it has none. */
+GPUD_API int gpud_out_rings_slot(const gpud_out_rings* o, int64_t signal_id, int* next_idx, int64_t* caps);
+/* StreamingExtractGPU with the reference's output ownership
+ * (streaming_gpu_native_prepare.go:78-104): as gpud_runner_streaming_extract,
+ * but job i's outputs land in outs[i], the next buffer of that signal's
+ * ring in the runner's own rings (NULL when it has no outputs).  Each call
+ * prunes the rings of signals absent from jobs together with their states. */
+GPUD_API int gpud_runner_streaming_extract_ring(gpud_runner* r, const gpud_float2* iq, int n_new,
+                                                const gpud_streaming_job* jobs, int n_jobs, gpud_float2** outs,
+                                                gpud_streaming_result* results);
+/* The runner's rings (created on its first ring call; NULL before). */
+GPUD_API gpud_out_rings* gpud_runner_out_rings(gpud_runner* r);
 /* streaming_types.go:58-72 StreamingConfigHash (FNV-1a-64) */
 GPUD_API uint64_t gpud_streaming_config_hash(int64_t signal_id, double offset_hz, double bandwidth, int out_rate,
                                              int num_taps, int sample_rate);
@@ -598,6 +640,54 @@ GPUD_API int gpud_rds_batch_decode_device(gpud_rds_batch* b, int n_calls, const 
 GPUD_API int gpud_rds_batch_state(gpud_rds_batch* b, int station, uint16_t* pi, char* ps9, char* rt65);
 /* Up to cap of the station's applied groups in order; returns the total. */
 GPUD_API int gpud_rds_batch_groups(gpud_rds_batch* b, int station, uint16_t* groups4, int cap);
+
+/* ------------------------------------------------------------------ */
+/* The daemon's RDS caller flow: successor of updateRDS                  */
+/* (cmd/sdrd/pipeline_runtime.go:1044-1156) with extractRDS              */
+/* (cmd/sdrd/helpers.go:117-160), its fallback chain (pipeline_runtime.go */
+/* 1104-1124) and the ID prune of maintenance (1158-1176).  One state per */
+/* tracked signal ID, each with a device RDS decoder; a signal decodes    */
+/* every 4 s of the caller's clock, a state unseen for more than 30 s is  */
+/* dropped.  Every signal due in one update call is decoded in the same   */
+/* batched device passes (extraction, stereo pilot, RDS baseband, decoder) */
+/* over the call's window of recent IQ, where the reference starts one    */
+/* goroutine per signal.  Not thread-safe.                                */
+/* ------------------------------------------------------------------ */
+typedef struct gpud_rds_runtime gpud_rds_runtime;
+/* max_stations: live states at once (decoder slots), at most 65535. */
+GPUD_API int gpud_rds_runtime_create(int max_stations, gpud_rds_runtime** out);
+GPUD_API int gpud_rds_runtime_destroy(gpud_rds_runtime* rt);
+/* updateRDS for n_signals signals in order at time now_s (seconds, the
+ * caller's monotonic clock).  window: the recorder ring's recent IQ
+ * (SliceRecentInto(4.0), recorder.go:354-370) -- n_window samples at
+ * window_rate, tuned to window_center_hz; host memory, or device memory
+ * when window_on_device is 1.  A window shorter than one second decodes
+ * nothing (the turns are still taken).  Per signal: stereo[i] = the 19 kHz
+ * pilot flag of its last decode; station9 + 9 i = its station name (the
+ * last decoded PS with spaces trimmed, "" if none yet); turn[i] = 1 when
+ * this call took its decode turn, 0 when not due, -1 for ID 0 (skipped).
+ * Outputs may be NULL.  Results are final on return.  -1 bad arguments,
+ * -8 more live states than max_stations. */
+GPUD_API int gpud_rds_runtime_update(gpud_rds_runtime* rt, double now_s, const int64_t* ids, const double* center_hz,
+                                     int n_signals, const gpud_float2* window, int64_t n_window, int window_on_device,
+                                     int window_rate, double window_center_hz, int32_t* stereo, char* station9,
+                                     int32_t* turn);
+/* maintenance: drop the states of IDs not in active_ids. */
+GPUD_API int gpud_rds_runtime_maintain(gpud_rds_runtime* rt, const int64_t* active_ids, int n_active);
+GPUD_API int gpud_rds_runtime_num_states(const gpud_rds_runtime* rt);
+/* A state's clock stamps, stereo flag and kept result (PI, PS, RT; empty
+ * until a decode reported a PS).  Returns 1 with a result, 0 without, -1
+ * if the ID has no state.  last_decode is -inf before the first turn. */
+GPUD_API int gpud_rds_runtime_state(const gpud_rds_runtime* rt, int64_t id, double* last_seen, double* last_decode,
+                                    int32_t* stereo, uint16_t* pi, char* ps9, char* rt65);
+/* The decimated baseband and RDS baseband the last decoding update made for
+ * the ID (up to the caps; counts and rates always).  -2 if that update did
+ * not decode it. */
+GPUD_API int gpud_rds_runtime_last(gpud_rds_runtime* rt, int64_t id, gpud_float2* base, int64_t base_cap,
+                                   int64_t* n_base, int32_t* base_rate, gpud_float2* rds, int64_t rds_cap,
+                                   int64_t* n_rds, int32_t* rds_rate);
+/* Up to cap of the ID's decoded groups in order; returns the total. */
+GPUD_API int gpud_rds_runtime_groups(gpud_rds_runtime* rt, int64_t id, uint16_t* groups4, int cap);
 
 /* ------------------------------------------------------------------ */
 /* Batch WFM stereo: wfmStereoDecodeBatch (internal/recorder/           */

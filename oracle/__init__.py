@@ -954,3 +954,156 @@ class SmoothSpectrum:
             return self.ema.copy()
         self.ema = alpha * v + (1 - alpha) * self.ema
         return self.ema.copy()
+
+
+# ---------------------------------------------------------------------------
+# Output rings (streaming_out_ring.go:22-112), pure Python
+# ---------------------------------------------------------------------------
+OUT_RING_DEPTH = 4          # streaming_out_ring.go:11
+OUT_POOL_MAX = 4 * 32       # streaming_out_ring.go:82
+
+
+class OutBuf:
+    """A host buffer of the model: an identity and a capacity."""
+    _next = 0
+
+    def __init__(self, cap):
+        OutBuf._next += 1
+        self.uid = OutBuf._next
+        self.cap = cap
+
+
+class OutBufPool:
+    """outBufPool (streaming_out_ring.go:51-89); `allocated` counts make()s."""
+
+    def __init__(self):
+        self.free = []
+        self.allocated = 0
+
+    def get(self, n):
+        best = -1
+        for i, b in enumerate(self.free):
+            if b.cap >= n:
+                best = i
+                break
+            if best < 0 or b.cap > self.free[best].cap:
+                best = i
+        if best >= 0:
+            b = self.free[best]
+            self.free[best] = self.free[-1]
+            self.free.pop()
+            if b.cap >= n:
+                return b
+        self.allocated += 1
+        return OutBuf(n)
+
+    def put(self, bufs):
+        for b in bufs:
+            if b is None or len(self.free) >= OUT_POOL_MAX:
+                continue
+            self.free.append(b)
+
+
+class OutRings:
+    """streamOutRing per signal ID with outRingFor / pruneOutRings
+    (streaming_out_ring.go:22-45, 91-112)."""
+
+    def __init__(self):
+        self.rings = {}
+        self.pool = OutBufPool()
+
+    def next(self, sid, n):
+        ring = self.rings.setdefault(sid, [[None] * OUT_RING_DEPTH, 0])
+        bufs, idx = ring
+        b = bufs[idx]
+        if b is None or b.cap < n:
+            if not (b is None and n == 0):  # cap(nil) < 0 is false: a nil slot serves n = 0
+                b = self.pool.get(n)
+        bufs[idx] = b
+        ring[1] = (idx + 1) % OUT_RING_DEPTH
+        return b
+
+    def prune(self, active):
+        # Go visits the map in random order; the product and this model use
+        # ascending IDs so the bounded pool's contents are reproducible
+        for sid in sorted(s for s in self.rings if s not in set(active)):
+            self.pool.put(self.rings[sid][0])
+            del self.rings[sid]
+
+
+# ---------------------------------------------------------------------------
+# The RDS caller flow (pipeline_runtime.go:1044-1176, helpers.go:117-160)
+# ---------------------------------------------------------------------------
+class RDSRuntime:
+    """updateRDS and maintenance's prune over the oracle chain.  Update takes
+    the signals of one tick in order; each due signal's goroutine body runs
+    right after the bookkeeping of the tick (its result visible on return).
+    `last[id]` keeps (baseband, rate, rds baseband, rds rate) of the last
+    decode, `dec[id]` the decoder."""
+
+    def __init__(self):
+        self.states = {}
+        self.last = {}
+
+    def _extract(self, iq, fs, offset):
+        """extractRDS (helpers.go:122-160) via ShiftFilterDecimateBatch, or
+        the fallback chain of pipeline_runtime.go:1104-1124."""
+        d = max(1, fs // 250000)
+        out_rate = fs // d
+        if out_rate > 0 and fs % out_rate == 0:
+            taps = LowpassFIR(max(200000 / 2, 200), fs, 101)
+            y, _ = LegacyShiftFilterDecimate(iq, fs, offset, 0.0, taps, d)
+            return y.astype(np.complex64), fs // d
+        shifted = FreqShift(iq, fs, offset)
+        d1 = max(1, fs // 1000000)
+        f1 = ApplyFIR(shifted, LowpassFIR(float(fs // d1) / 2.0 * 0.8, fs, 51))
+        x1 = Decimate(f1, d1)
+        rate1 = fs // d1
+        d2 = max(1, rate1 // 250000)
+        f2 = ApplyFIR(x1, LowpassFIR(float(rate1 // d2) / 2.0 * 0.8, rate1, 101))
+        return Decimate(f2, d2), rate1 // d2
+
+    def Update(self, now, signals, window, fs, center):
+        """-> [(stereo, station, turn)] per (id, center_hz)."""
+        due = []
+        turns = []
+        for sid, hz in signals:
+            if sid == 0:
+                turns.append(-1)
+                continue
+            st = self.states.get(sid)
+            if st is None:
+                st = self.states[sid] = {"last_seen": 0.0, "last_decode": -np.inf, "stereo": 0, "ps": None,
+                                         "pi": 0, "rt": "", "dec": RDSDecoder()}
+            st["last_seen"] = now
+            for other in [k for k, v in self.states.items() if now - v["last_seen"] > 30.0]:
+                del self.states[other]
+            if now - st["last_decode"] >= 4.0:
+                st["last_decode"] = now
+                due.append((sid, hz))
+                turns.append(1)
+            else:
+                turns.append(0)
+        if due and fs > 0 and len(window) >= fs:
+            for sid, hz in due:
+                st = self.states[sid]
+                base, rate = self._extract(window, fs, hz - center)
+                st["stereo"] = int(StereoPilotPresent(base, rate)[0])
+                rds, rrate = RDSBasebandComplex(base, rate)
+                self.last[sid] = (base, rate, rds, rrate)
+                if len(rds) == 0:
+                    continue
+                st["dec"].Decode(rds, rrate)
+                if st["dec"].PS:
+                    st["ps"], st["pi"], st["rt"] = st["dec"].PS, st["dec"].PI, st["dec"].RT
+        out = []
+        for (sid, hz), turn in zip(signals, turns):
+            st = self.states.get(sid) if sid else None
+            stereo = st["stereo"] if st else 0
+            station = st["ps"].strip(" ") if st and st["ps"] else ""
+            out.append((stereo, station, turn))
+        return out
+
+    def Maintain(self, active):
+        for sid in [s for s in self.states if s not in set(active)]:
+            del self.states[sid]
