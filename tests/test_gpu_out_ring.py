@@ -17,10 +17,10 @@ FS = 2_048_000
 
 
 def _jobs(ids):
-    from argus.gpudemod import StreamingExtractJob
+    from argus.gpudemod import StreamingConfigHash, StreamingExtractJob
     specs = {1: (-300e3, 20e3, 32000, 101), 2: (150e3, 12e3, 16000, 63), 3: (400e3, 150e3, 256000, 127),
              4: (-700e3, 5e3, 8000, 255), 5: (20e3, 25e3, 64000, 101), 6: (600e3, 5e3, 8000, 95)}
-    return [StreamingExtractJob(i, *specs[i]) for i in ids]
+    return [StreamingExtractJob(i, *specs[i], StreamingConfigHash(i, *specs[i], FS)) for i in ids]
 
 
 def test_ring_outputs_and_ownership(gpu):
