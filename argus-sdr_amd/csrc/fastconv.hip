@@ -26,10 +26,12 @@
 //   tail    the stream's last 511 samples after the block.
 //
 // The inverse transform is the 512-point FFT (8^3) of the conjugated folded
-// spectrum, zero-stuffed by 512 / J for D > 8 (stuffing the spectrum repeats
-// the signal, so its first J outputs are the J-point transform).  Each
-// workgroup takes one channel: G' in LDS (32 KiB), its four wavefronts take
-// four segments at a time.
+// spectrum, zero-stuffed by 512 / J for Dp > 8 (stuffing the spectrum
+// repeats the signal, so its first J outputs are the J-point transform).
+// A work item is one channel and a block of segments: G' in LDS (32 KiB),
+// each wavefront of the workgroup runs its own segments with transposes
+// synchronised within the wave.  An output's rotation is one NCO evaluation
+// per lane and segment, then a complex step per 64 fold positions.
 //
 // The last 511 samples of the stream are the channel's own history only once
 // the channel has seen T - 1 samples (a channel opened mid-stream starts
@@ -75,6 +77,94 @@ __global__ __launch_bounds__(kThreads) void argus_fast_stage_kernel(const ArgusF
     }
 }
 
+// ---------------------------------------------------------------------------
+// The per-channel pass.  Work items are (channel, block of kSegItem
+// segments); each of a workgroup's kWaves wavefronts takes every kWaves-th
+// segment of the block on its own: fold, a 512-point transform with its
+// LDS transposes synchronised within the wave only, and the outputs.  The
+// workgroup synchronises only to stage the channel's response G' in LDS.
+// Items are numbered channel-major, so with a grid that is a multiple of
+// the eight XCDs and two blocks per channel, each XCD works on one half of
+// the segments and its L2 holds that half of the segment spectra.
+// ---------------------------------------------------------------------------
+constexpr int kWaves = 8;                      // wavefronts per workgroup
+constexpr int kChanThreads = 64 * kWaves;
+constexpr int kSegWave = 7;                    // segments per wavefront per item
+constexpr int kSegItem = kWaves * kSegWave;    // 56: two items cover a 20 ms block at 20 Msps
+constexpr int kXStride = 72;                   // first transpose: 64 values + 8 pad per row (conflict-free reads)
+constexpr int kXsFloats = 8 * kXStride > 576 ? 8 * kXStride : 576;  // also the padded 8 x 9 x 8 image
+
+// The wave's LDS operations complete in issue order, so data a wave passes
+// between its own lanes through LDS needs no hardware barrier: only the
+// compiler must not move the accesses across this point.
+__device__ __forceinline__ void wave_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Per-lane twiddles of the 512-point transform (functions of the lane only):
+// pass 1 W_512^{t k}, pass 2 W_64^{(t mod 8) k}, k = 1..7.
+struct Tw512 {
+    float2 w1[7], w2[7];
+};
+
+__device__ __forceinline__ void tw512_init(Tw512& tw, int t)
+{
+#pragma unroll
+    for (int k = 1; k < 8; k++) {
+        tw.w1[k - 1] = twiddle((t * k) & 511, 1.0f / 512.0f);
+        tw.w2[k - 1] = twiddle(((t & 7) * k) & 63, 1.0f / 64.0f);
+    }
+}
+
+// Transpose through the wave's LDS region: write v[k] at wr(k), read v[k]
+// from rd(k); real parts then imaginary parts through the same region.
+template <typename WR, typename RD>
+__device__ __forceinline__ void wave_exchange(float2 (&v)[8], float* xs, WR wr, RD rd)
+{
+#pragma unroll
+    for (int k = 0; k < 8; ++k) xs[wr(k)] = v[k].x;
+    wave_sync();
+#pragma unroll
+    for (int k = 0; k < 8; ++k) v[k].x = xs[rd(k)];
+    wave_sync();
+#pragma unroll
+    for (int k = 0; k < 8; ++k) xs[wr(k)] = v[k].y;
+    wave_sync();
+#pragma unroll
+    for (int k = 0; k < 8; ++k) v[k].y = xs[rd(k)];
+    wave_sync();
+}
+
+// 512-point forward transform on one wavefront (8^3: lane t = a + 8 b holds
+// n = t + 64 c, c = 0..7).  On return lane t = kb + 8 kc holds
+// X[kc + 8 kb + 64 ka] in v[ka].
+__device__ __forceinline__ void wave_fft512(float2 (&v)[8], float* xs, int t, const Tw512& tw)
+{
+    dft<8>(v);
+#pragma unroll
+    for (int k = 1; k < 8; ++k) v[k] = cmul(v[k], tw.w1[k - 1]);
+    {
+        // element (t, kc) at t + 72 kc: the writes are consecutive per kc and
+        // the reads (lanes vary a and kc) fall on distinct banks per half-wave
+        const int a = t & 7, kc = t >> 3;
+        wave_exchange(v, xs, [&](int k) { return t + kXStride * k; },
+                      [&](int b) { return a + 8 * b + kXStride * kc; });
+    }
+    dft<8>(v);
+    {
+        const int a = t & 7, kc = t >> 3;
+#pragma unroll
+        for (int k = 1; k < 8; ++k) v[k] = cmul(v[k], tw.w2[k - 1]);
+        const int kb2 = t & 7, kc2 = t >> 3;
+        wave_exchange(v, xs, [&](int kb) { return a + 9 * kb + 72 * kc; },
+                      [&](int aa) { return aa + 9 * kb2 + 72 * kc2; });
+    }
+    dft<8>(v);
+}
+
 // F[j] = sum_{i < DP} X[j + i J] G'[j + i J] for this lane's j = t + 64 c,
 // c < J / 64: every lane folds the same number of bins, all loads of one
 // i issued together.
@@ -86,7 +176,7 @@ __device__ __forceinline__ void fast_fold(const float2* __restrict__ xseg, const
     float ar[PER], ai[PER];
 #pragma unroll
     for (int c = 0; c < PER; c++) ar[c] = ai[c] = 0.0f;
-#pragma unroll
+#pragma unroll 4
     for (int i = 0; i < DP; i++) {
         float2 xv[PER];
 #pragma unroll
@@ -102,88 +192,124 @@ __device__ __forceinline__ void fast_fold(const float2* __restrict__ xseg, const
     for (int c = 0; c < PER; c++) f[c] = make_float2(ar[c], ai[c]);
 }
 
-// One segment of one channel on one wavefront: fold, inverse transform,
-// outputs.  All four wavefronts of the workgroup call this the same number
-// of times (the transform's barriers are workgroup-wide); act = 0 does no
-// work but takes the barriers.
+// Per channel, the constants of the output stage.
+struct FastOut {
+    int E;          // D / DP: emission every E-th fold position
+    float invE;     // 1 / E (rounded up: floor by multiply, then corrected)
+    float2 step;    // rotation between a lane's consecutive values: e^{j 2pi inc 64 DP}
+};
+
+// One segment of one channel on one wavefront: fold, inverse transform
+// (the forward transform of the conjugate), outputs.
 template <int DP>
-__device__ __forceinline__ void fast_segment(const ArgusFastDesc& d, const float2* __restrict__ X, const float2* gs,
-                                             float* xs, int s, bool act, int t)
+__device__ __forceinline__ void fast_segment(const ArgusFastDesc& d, const FastOut& fo, const float2* __restrict__ X,
+                                             const float2* gs, float* xs, int s, int t, const Tw512& tw)
 {
     constexpr int J = ARGUS_FAST_N / DP, PER = J / 64, Z = kJ / J;  // Z: zero-stuffing of the 512-point input
+    // opaque to the optimiser: keeps the lane's LDS addresses (functions of
+    // t only) from being hoisted out of the segment loop into registers
+    asm volatile("" : "+v"(t));
     float2 v[8];
     if constexpr (Z == 1) {
-        fast_fold<DP>(X + (size_t)(act ? s : 0) * kN, gs, t, v);
+        fast_fold<DP>(X + (size_t)s * kN, gs, t, v);
 #pragma unroll
         for (int c = 0; c < 8; c++) v[c].y = -v[c].y;  // conjugate: the forward transform inverts
     } else {
         float2 f[PER];
-        fast_fold<DP>(X + (size_t)(act ? s : 0) * kN, gs, t, f);
-        float2* fs = (float2*)xs;  // J <= 256 bins fit the transform's scratch
+        fast_fold<DP>(X + (size_t)s * kN, gs, t, f);
+        float2* fs = (float2*)xs;  // J <= 256 bins fit the wave's transform scratch
 #pragma unroll
         for (int c = 0; c < PER; c++) fs[t + 64 * c] = f[c];
-        __syncthreads();
+        wave_sync();
 #pragma unroll
         for (int c = 0; c < 8; c++) {
             const int nn = t + 64 * c;
             const float2 y = (nn % Z) == 0 ? fs[nn / Z] : make_float2(0.0f, 0.0f);
             v[c] = make_float2(y.x, -y.y);
         }
-        __syncthreads();
+        wave_sync();
     }
-    fft_r3<8>(v, xs, t);
-    // thread t = kb + 8 kc holds Y[kc + 8 kb + 64 ka] in v[ka]
+    wave_fft512(v, xs, t, tw);
+    // lane t = kb + 8 kc holds Y[q], q = kc + 8 kb + 64 ka, in v[ka]: fold
+    // position q is block sample e = s hop + q DP + r - tail, and
+    // e - e0 = DP (q + cs) with cs uniform over the segment; it is an
+    // output when q + cs >= 0 is a multiple of E, output (q + cs) / E.
     const int kb = t & 7, kc = t >> 3;
+    const int q0 = kc + 8 * kb;
+    const int cs = (s * kHop + d.r - kTail - d.e0) / DP;  // exact: e - e0 is a multiple of DP
+    // rotation of the lane's first position, then one step per 64 positions
+    float sn, cn;
+    argus_nco(d.phase0 + d.inc * (uint64_t)(int64_t)(s * kHop + q0 * DP + d.r - kTail), sn, cn);
+    float2 rot = make_float2(cn, sn);
 #pragma unroll
     for (int ka = 0; ka < 8; ka++) {
-        const int q = kc + 8 * kb + 64 * ka;
-        if (!act || q >= J) continue;
+        const int q = q0 + 64 * ka;
         const int iseg = q * DP + d.r;
-        if (iseg < kTail || iseg >= kTail + kHop) continue;
-        const int e = s * kHop + iseg - kTail;  // block sample, = e0 mod Dp
-        const int de = e - d.e0;
-        if (de < 0 || de % d.D != 0) continue;  // not an emission point (D > DP)
-        const int m = de / d.D;
-        if (m >= d.n_out) continue;
-        const float2 y = make_float2(v[ka].x, -v[ka].y);
-        float sn, cs;
-        argus_nco(d.phase0 + d.inc * (uint64_t)(int64_t)e, sn, cs);
-        d.out[m] = argus_rot(y, sn, cs);
+        const int u = q + cs;
+        if (q < J && iseg >= kTail && iseg < kTail + kHop && u >= 0) {
+            int m = (int)((float)u * fo.invE);
+            int rem = u - m * fo.E;
+            if (rem < 0) {
+                m--;
+                rem += fo.E;
+            } else if (rem >= fo.E) {
+                m++;
+                rem -= fo.E;
+            }
+            if (rem == 0 && m < d.n_out) {
+                const float2 y = make_float2(v[ka].x, -v[ka].y);
+                d.out[m] = make_float2(y.x * rot.x - y.y * rot.y, y.x * rot.y + y.y * rot.x);
+            }
+        }
+        rot = make_float2(rot.x * fo.step.x - rot.y * fo.step.y, rot.x * fo.step.y + rot.y * fo.step.x);
     }
 }
 
 template <int DP>
-__device__ __forceinline__ void fast_channel(const ArgusFastDesc& d, const float2* __restrict__ X, const float2* gs,
-                                             float* xs, int S, int w, int t)
+__device__ __forceinline__ void fast_block(const ArgusFastDesc& d, const float2* __restrict__ X, const float2* gs,
+                                           float* xs, int s_begin, int s_end, int w, int t, const Tw512& tw)
 {
-    for (int s0 = 0; s0 < S; s0 += 4) fast_segment<DP>(d, X, gs, xs, s0 + w, s0 + w < S, t);
+    FastOut fo;
+    fo.E = d.D / DP;
+    fo.invE = __builtin_bit_cast(float, __builtin_bit_cast(uint32_t, 1.0f / (float)fo.E) + 1u);
+    float sn, cn;
+    argus_nco(d.inc * (uint64_t)(64 * DP), sn, cn);
+    fo.step = make_float2(cn, sn);
+    for (int s = s_begin + w; s < s_end; s += kWaves) fast_segment<DP>(d, fo, X, gs, xs, s, t, tw);
 }
 
-__global__ __launch_bounds__(kThreads) void argus_fast_chan_kernel(const ArgusFastHdr* __restrict__ hdr)
+__global__ __launch_bounds__(kChanThreads) void argus_fast_chan_kernel(const ArgusFastHdr* __restrict__ hdr)
 {
     __shared__ float2 gs[kN];
-    __shared__ float xs[4][fft_lds_floats<8>()];
-    static_assert(fft_lds_floats<8>() >= 2 * 256, "folded bins of D >= 16 fit the transform scratch");
+    __shared__ float xs[kWaves][kXsFloats];
+    static_assert(kXsFloats >= 2 * 256, "folded bins of D >= 16 fit the transform scratch");
+    static_assert(kXsFloats >= 9 * 8 * 8, "the padded second transpose fits the scratch");
     const int w = threadIdx.x >> 6, t = threadIdx.x & 63;
     const int S = hdr->S, n_fast = hdr->n_fast;
+    const int blocks = (S + kSegItem - 1) / kSegItem;
+    const int n_items = n_fast * blocks;
     const float2* X = hdr->X;
-    for (int c = blockIdx.x; c < n_fast; c += gridDim.x) {
+    Tw512 tw;
+    tw512_init(tw, t);
+    for (int it = blockIdx.x; it < n_items; it += gridDim.x) {
+        const int c = it / blocks, b = it - c * blocks;
         const ArgusFastDesc d = hdr->descs[c];
+        __syncthreads();  // the previous item's waves are done with gs
         // G with the ramp e^{j 2pi k r / N}: exact turn fractions
-        for (int k = threadIdx.x; k < kN; k += kThreads) {
+        for (int k = threadIdx.x; k < kN; k += kChanThreads) {
             const float2 g = d.G[k];
             const float x = (float)((k * d.r) & (kN - 1)) * (1.0f / kN);
             const float sn = __builtin_amdgcn_sinf(x), cs = __builtin_amdgcn_cosf(x);
             gs[k] = make_float2(g.x * cs - g.y * sn, g.x * sn + g.y * cs);
         }
         __syncthreads();
+        const int s0 = b * kSegItem, s1 = min(S, s0 + kSegItem);
         switch (d.Dp) {  // uniform over the workgroup (one channel)
-        case 8: fast_channel<8>(d, X, gs, xs[w], S, w, t); break;
-        case 16: fast_channel<16>(d, X, gs, xs[w], S, w, t); break;
-        case 32: fast_channel<32>(d, X, gs, xs[w], S, w, t); break;
-        default: fast_channel<64>(d, X, gs, xs[w], S, w, t); break;
+        case 8: fast_block<8>(d, X, gs, xs[w], s0, s1, w, t, tw); break;
+        case 16: fast_block<16>(d, X, gs, xs[w], s0, s1, w, t, tw); break;
+        case 32: fast_block<32>(d, X, gs, xs[w], s0, s1, w, t, tw); break;
+        default: fast_block<64>(d, X, gs, xs[w], s0, s1, w, t, tw); break;
         }
-        __syncthreads();  // gs is rewritten for the next channel
     }
 }
 
@@ -259,7 +385,7 @@ int argus_launch_fast(const ArgusFastHdr* d_hdr, const ArgusExtractHdr* d_xh, in
     ARGUS_CHECK(hipGetLastError());
     int err = argus_launch_fft4096(u, kHop, S, X, stream);
     if (err) return err;
-    argus_fast_chan_kernel<<<ARGUS_FAST_GRID, kThreads, 0, stream>>>(d_hdr);
+    argus_fast_chan_kernel<<<ARGUS_FAST_GRID, kChanThreads, 0, stream>>>(d_hdr);
     return (int)hipGetLastError();
 }
 

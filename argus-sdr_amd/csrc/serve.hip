@@ -1589,8 +1589,8 @@ GPUD_API int gpud_serve_channel_rds(gpud_serve* s, int64_t ticket, int64_t chann
     return -1;
 }
 
-// The fast-convolution path for eligible channels (decimation 8, 16, 32 or
-// 64, at most 512 taps, full history): on by default.
+// The fast-convolution path for eligible channels (decimation a multiple of
+// 8, 2 D <= taps <= 512, full history): on by default.
 GPUD_API int gpud_serve_set_fast(gpud_serve* s, int enable)
 {
     if (!s) return -1;

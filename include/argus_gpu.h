@@ -525,9 +525,10 @@ GPUD_API int gpud_serve_channel_audio(gpud_serve* s, int64_t ticket, int64_t cha
  * in flight; applies from the next submitted block.  0, -1 bad arguments,
  * unknown or non-stereo channel, GPUD_ERR_BUDGET if it does not fit. */
 GPUD_API int gpud_serve_set_rds(gpud_serve* s, int64_t channel_id, int window_samples);
-/* The fast-convolution form of the extraction (on by default): a channel of
- * decimation 8, 16, 32 or 64 with at most 512 taps, once its history is
- * full, is filtered through one shared 4096-point transform of each input
+/* The fast-convolution form of the extraction (on by default): a channel
+ * whose decimation D is a multiple of 8 and whose tap count T satisfies
+ * 2 D <= T <= 512 (where the fast form is the cheaper one), once its
+ * history is full, is filtered through one shared 4096-point transform of each input
  * segment (overlap-save, decimation by spectral folding) instead of the
  * direct polyphase form; same filter, outputs within FP32 rounding of the
  * direct form.  Applies from the next submitted block. */
