@@ -32,6 +32,7 @@ register("gpud_out_rings_stats", I_, [P_, C.POINTER(I_), C.POINTER(I_), C.POINTE
 register("gpud_out_rings_slot", I_, [P_, L_, C.POINTER(I_), P_])
 register("gpud_runner_streaming_extract_ring", I_, [P_, P_, I_, P_, I_, P_, P_])
 register("gpud_runner_out_rings", P_, [P_])
+register("gpud_runner_set_fast", I_, [P_, I_])
 
 OUT_RING_DEPTH = 4  # streaming_out_ring.go:11 streamOutRingDepth
 
@@ -276,6 +277,10 @@ class BatchRunner:
 
     def stream(self) -> int:
         return lib().gpud_runner_stream(self._h)
+
+    def SetFast(self, enable: bool) -> None:
+        """Fast-convolution form of the batched launch (on by default)."""
+        check("gpud_runner_set_fast", lib().gpud_runner_set_fast(self._h, 1 if enable else 0))
 
     # -- streaming extraction ----------------------------------------------
     def StreamingExtractGPU(self, iq_new, jobs) -> list[StreamingExtractResult]:

@@ -139,6 +139,9 @@ void argus_signal_release(ArgusSignal& s)
     if (s.d_hist[0]) hipFree(s.d_hist[0]);
     if (s.d_hist[1]) hipFree(s.d_hist[1]);
     if (s.d_out) hipFree(s.d_out);
+    if (s.d_fastG) hipFree(s.d_fastG);
+    s.d_fastG = nullptr;
+    s.fastG_valid = false;
     s.d_taps = nullptr;
     s.d_ctaps = nullptr;
     s.d_ctaps_len = 0;
@@ -233,6 +236,26 @@ int argus_runner_prepare_signal(gpud_runner* r, const gpud_streaming_job& job, A
         s->ctaps_inc = inc;
         s->ctaps_valid = true;
     }
+    // the fast form's response follows the taps and the increment too
+    if (argus_fast_eligible(decim, T)) {
+        if (s->taps_dirty || !s->fastG_valid || s->fastG_inc != inc) {
+            const int L = (T + decim - 1) / decim;
+            std::vector<float> h((size_t)T);
+            for (int k = 0; k < T; k++) h[(size_t)k] = s->poly[(size_t)(k % decim) * L + k / decim];
+            std::vector<float2> G((size_t)ARGUS_FAST_N);
+            argus_fast_spectrum(h.data(), T, inc, G.data());
+            if (!s->d_fastG) ARGUS_CHECK(hipMalloc((void**)&s->d_fastG, (size_t)ARGUS_FAST_N * sizeof(float2)));
+            ARGUS_CHECK(hipMemcpyAsync(s->d_fastG, G.data(), (size_t)ARGUS_FAST_N * sizeof(float2),
+                                       hipMemcpyHostToDevice, r->stream));
+            ARGUS_CHECK(hipStreamSynchronize(r->stream));  // G is pageable host memory
+            s->fastG_inc = inc;
+            s->fastG_valid = true;
+        }
+    } else if (s->d_fastG) {
+        hipFree(s->d_fastG);
+        s->d_fastG = nullptr;
+        s->fastG_valid = false;
+    }
     if (s->taps_dirty) {
         if (s->d_taps && s->d_taps_len != s->poly.size()) {
             hipFree(s->d_taps);
@@ -314,6 +337,15 @@ GPUD_API int gpud_runner_destroy(gpud_runner* r)
     for (hipEvent_t ev : r->tev) hipEventDestroy(ev);
     argus_legacy_release(r->legacy);
     argus_out_rings_free(r->rings);
+    hipFree(r->d_ftail);
+    hipFree(r->d_fu);
+    hipFree(r->d_fX);
+    for (int i = 0; i < 2; i++) {
+        hipHostFree(r->h_fdesc[i]);
+        hipFree(r->d_fdesc[i]);
+        hipHostFree(r->h_fhdr[i]);
+        hipFree(r->d_fhdr[i]);
+    }
     hipStreamDestroy(r->stream);
     delete r;
     return 0;
@@ -323,6 +355,16 @@ GPUD_API int gpud_runner_set_exec_mode(gpud_runner* r, int mode)
 {
     if (!r || (mode != GPUD_EXEC_BATCHED && mode != GPUD_EXEC_PER_SIGNAL)) return -1;
     r->exec_mode = mode;
+    return 0;
+}
+
+// The fast-convolution form in the batched launch (on by default): a signal
+// whose decimation D is a multiple of 8 with 2 D <= T <= 512 is filtered
+// through the shared segment transforms once its history is full.
+GPUD_API int gpud_runner_set_fast(gpud_runner* r, int enable)
+{
+    if (!r) return -1;
+    r->fast = enable != 0;
     return 0;
 }
 
@@ -413,6 +455,10 @@ static int run_batched(gpud_runner* r, const float2* d_in, int n_new, const gpud
 
     r->items.clear();
     r->item_pos.clear();
+    r->fdescs.clear();
+    // the fast form needs the stream's own last ARGUS_FAST_TAIL samples in
+    // the tail, which every batched call keeps
+    const bool tail_ok = r->tail_fill >= ARGUS_FAST_TAIL;
     size_t lds = 0;
     for (int i = 0; i < n_jobs; i++) {
         ArgusSignal* s = sig[(size_t)i];
@@ -424,8 +470,30 @@ static int run_batched(gpud_runner* r, const float2* d_in, int n_new, const gpud
                             s->d_ctaps, r->d_out + results[i].out_offset, D, T, s->phase_count, n_new, s->nco_phase,
                             inc, r->tune) != 0)
             return -1;
-        if (ch.n_out > 0) lds = std::max(lds, argus_item_lds_bytes(ch));
-        argus_append_items(ch, i, n_new, r->items, r->item_pos);
+        // once the signal's history is the stream's own last T-1 samples,
+        // its outputs come from the fast form and its direct items are only
+        // the history
+        const bool fast = r->fast && tail_ok && s->fastG_valid && s->hist_len == T - 1 && n_new > 0;
+        if (fast) {
+            ArgusFastDesc fd;
+            fd.G = s->d_fastG;
+            fd.out = ch.out;
+            fd.phase0 = ch.phase0;
+            fd.inc = ch.inc;
+            fd.D = ch.D;
+            fd.Dp = argus_fast_fold(ch.D);
+            fd.e0 = ch.e0;
+            fd.n_out = ch.n_out;
+            fd.r = (ch.e0 + ARGUS_FAST_TAIL) % fd.Dp;
+            fd.pad = 0;
+            r->fdescs.push_back(fd);
+            ArgusChanDesc hist_only = ch;
+            hist_only.n_out = 0;
+            argus_append_items(hist_only, i, n_new, r->items, r->item_pos);
+        } else {
+            if (ch.n_out > 0) lds = std::max(lds, argus_item_lds_bytes(ch));
+            argus_append_items(ch, i, n_new, r->items, r->item_pos);
+        }
     }
     argus_order_items(r->items, r->item_pos, n_new);
     const int n_items = (int)r->items.size();
@@ -436,13 +504,46 @@ static int run_batched(gpud_runner* r, const float2* d_in, int n_new, const gpud
                                hipMemcpyHostToDevice, r->stream));
     ARGUS_CHECK(hipMemcpyAsync(r->d_items[slot], r->h_items[slot], (size_t)n_items * sizeof(ArgusWorkItem),
                                hipMemcpyHostToDevice, r->stream));
+    // the fast form's descriptors and headers (the tail is kept every call)
+    const int n_fast = (int)r->fdescs.size();
+    const int S = n_fast > 0 ? argus_fast_segments(n_new) : 0;
+    if (!r->d_ftail) {
+        ARGUS_CHECK(hipMalloc((void**)&r->d_ftail, (size_t)ARGUS_FAST_TAIL * sizeof(float2)));
+        ARGUS_CHECK(hipMemset(r->d_ftail, 0, (size_t)ARGUS_FAST_TAIL * sizeof(float2)));
+    }
+    if (S > 0) {
+        ARGUS_CHECK((hipError_t)ensure_dev(&r->d_fu, &r->fu_cap, argus_fast_u_len(S)));
+        ARGUS_CHECK((hipError_t)ensure_dev(&r->d_fX, &r->fX_cap, (size_t)S * ARGUS_FAST_N));
+    }
+    ARGUS_CHECK((hipError_t)ensure_host(&r->h_fdesc[slot], &r->h_fdesc_cap[slot], (size_t)std::max(n_fast, 1)));
+    ARGUS_CHECK((hipError_t)ensure_dev(&r->d_fdesc[slot], &r->d_fdesc_cap[slot], (size_t)std::max(n_fast, 1)));
+    if (!r->h_fhdr[slot]) {
+        ARGUS_CHECK(hipHostMalloc((void**)&r->h_fhdr[slot], sizeof(gpud_runner::FastHdrs), hipHostMallocDefault));
+        ARGUS_CHECK(hipMalloc((void**)&r->d_fhdr[slot], sizeof(gpud_runner::FastHdrs)));
+    }
+    if (n_fast) std::memcpy(r->h_fdesc[slot], r->fdescs.data(), (size_t)n_fast * sizeof(ArgusFastDesc));
+    gpud_runner::FastHdrs* fh = r->h_fhdr[slot];
+    fh->f = ArgusFastHdr{r->d_fdesc[slot], r->d_ftail, r->d_fu, r->d_fX, n_fast, S};
+    fh->x = ArgusExtractHdr{r->d_chans[slot], r->d_items[slot], d_in, n_items, n_new};
+    ARGUS_CHECK(hipMemcpyAsync(r->d_fdesc[slot], r->h_fdesc[slot], (size_t)n_fast * sizeof(ArgusFastDesc),
+                               hipMemcpyHostToDevice, r->stream));
+    ARGUS_CHECK(hipMemcpyAsync(r->d_fhdr[slot], fh, sizeof(gpud_runner::FastHdrs), hipMemcpyHostToDevice,
+                               r->stream));
     ARGUS_CHECK(hipEventRecord(r->stage_ev[slot], r->stream));
     r->stage_recorded[slot] = true;
     if (r->timing) ARGUS_CHECK((hipError_t)argus_timing_begin(r));
-    int err = argus_launch_extract_batch(r->d_chans[slot], r->d_items[slot], n_items, lds, d_in, n_new, r->tune.grid,
+    const gpud_runner::FastHdrs* dfh = r->d_fhdr[slot];
+    int err = S > 0 ? argus_launch_fast(&dfh->f, &dfh->x, S, r->d_fu, r->d_fX, r->stream) : 0;
+    if (!err)
+        err = argus_launch_extract_batch(r->d_chans[slot], r->d_items[slot], n_items, lds, d_in, n_new, r->tune.grid,
                                          r->stream);
     if (err) return err;
     if (r->timing) ARGUS_CHECK(hipEventRecord(r->tev[(size_t)r->tev_used - 1], r->stream));
+    if (n_new > 0) {
+        err = argus_launch_fast_tail(&dfh->f, &dfh->x, r->stream);
+        if (err) return err;
+        r->tail_fill = std::min(ARGUS_FAST_TAIL, r->tail_fill + n_new);
+    }
     // carry state forward (the kernel computes the same quantities)
     for (int i = 0; i < n_jobs; i++) {
         ArgusSignal* s = sig[(size_t)i];
@@ -549,7 +650,10 @@ GPUD_API int gpud_runner_streaming_extract(gpud_runner* r, const gpud_float2* iq
     int64_t total = 0;
     for (int i = 0; i < n_jobs; i++) total += results[i].n_out;
     if (total > out_cap || (total > 0 && !out)) return -14;
-    if (r->exec_mode == GPUD_EXEC_PER_SIGNAL) return run_per_signal(r, iq, n_new, jobs, n_jobs, sig, results, out);
+    if (r->exec_mode == GPUD_EXEC_PER_SIGNAL) {
+        if (n_new > 0) r->tail_fill = 0;  // the per-signal form does not keep the fast form's tail
+        return run_per_signal(r, iq, n_new, jobs, n_jobs, sig, results, out);
+    }
     if (n_new > 0) {
         ARGUS_CHECK(hipStreamSynchronize(r->stream));  // h_in reuse
         std::memcpy(r->h_in, iq, (size_t)n_new * sizeof(float2));
@@ -581,6 +685,7 @@ GPUD_API int gpud_runner_streaming_extract_ring(gpud_runner* r, const gpud_float
     for (int i = 0; i < n_jobs; i++) total += results[i].n_out;
     ARGUS_CHECK((hipError_t)ensure_host(&r->h_out, &r->h_out_cap, (size_t)std::max<int64_t>(total, 1)));
     if (r->exec_mode == GPUD_EXEC_PER_SIGNAL) {
+        if (n_new > 0) r->tail_fill = 0;
         err = run_per_signal(r, iq, n_new, jobs, n_jobs, sig, results, (gpud_float2*)r->h_out);
         if (err) return err;
     } else {

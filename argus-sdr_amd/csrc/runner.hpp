@@ -1,6 +1,7 @@
 // runner.hpp -- host-side state of the streaming batch runner.
 #pragma once
 #include "extract.hpp"
+#include "fastconv.hpp"
 #include <vector>
 
 // Per-signal carried state (streaming_types.go:26-44 ExtractStreamState and
@@ -31,6 +32,9 @@ struct ArgusSignal {
     int hist_len = 0;
     float2* d_out = nullptr;   // per-signal output (plain per-signal form only)
     int out_cap = 0;
+    float2* d_fastG = nullptr; // fast form: FFT_4096 of the modulated taps / 4096 (eligible signals)
+    uint64_t fastG_inc = 0;
+    bool fastG_valid = false;
 };
 
 // One job of a legacy block call (engine.hip): taps in natural order
@@ -88,6 +92,27 @@ struct gpud_runner {
     int t_launches = 0;
     std::vector<gpud_float2> hist_host;
     ArgusLegacyScratch* legacy = nullptr;  // staging of the legacy block path (engine.hip)
+    // fast-convolution form of the batched launch (fastconv.hip): the
+    // stream's last ARGUS_FAST_TAIL raw samples (tail_fill of them known to
+    // be the stream's), the staged segments and their spectra, and per
+    // staging slot the fast descriptors and the two headers the kernels read
+    bool fast = true;
+    int tail_fill = 0;
+    float2* d_ftail = nullptr;
+    float2* d_fu = nullptr;
+    size_t fu_cap = 0;
+    float2* d_fX = nullptr;
+    size_t fX_cap = 0;
+    ArgusFastDesc* h_fdesc[2] = {nullptr, nullptr};
+    ArgusFastDesc* d_fdesc[2] = {nullptr, nullptr};
+    size_t h_fdesc_cap[2] = {0, 0}, d_fdesc_cap[2] = {0, 0};
+    struct FastHdrs {
+        ArgusFastHdr f;
+        ArgusExtractHdr x;
+    };
+    FastHdrs* h_fhdr[2] = {nullptr, nullptr};
+    FastHdrs* d_fhdr[2] = {nullptr, nullptr};
+    std::vector<ArgusFastDesc> fdescs;
     gpud_out_rings* rings = nullptr;       // per-signal output rings (outring.hip), on first ring call
 };
 

@@ -164,6 +164,14 @@ GPUD_API int gpud_runner_create(int max_samples, int sample_rate, gpud_runner** 
 /* batch_runner.go:36-50 Close */
 GPUD_API int gpud_runner_destroy(gpud_runner* r);
 GPUD_API int gpud_runner_set_exec_mode(gpud_runner* r, int mode);
+/* The fast-convolution form in the batched launch (on by default; no
+ * reference counterpart): a signal whose decimation D is a multiple of 8
+ * with 2 D <= T <= 512 is filtered through shared 4096-point transforms of
+ * the chunk (overlap-save, decimation by spectral folding) once its history
+ * is full -- the same filter, outputs within FP32 rounding of the direct
+ * form.  The runner keeps the stream's last 511 samples for it; a call in
+ * the per-signal mode restarts that count. */
+GPUD_API int gpud_runner_set_fast(gpud_runner* r, int enable);
 /* Tap window for the low-pass design of the runner's signals: 0 = Hann
  * (LowpassFIR, dsp/fir.go:6-28, the default), 1 = Hamming (same sinc). */
 GPUD_API int gpud_runner_set_window(gpud_runner* r, int window);
