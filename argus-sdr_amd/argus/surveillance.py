@@ -46,6 +46,7 @@ register("gpud_surv_welch_device", I, [P, P, L64, I, I, I, P])
 register("gpud_surv_fft_device", I, [P, P, L64, I, I, P])
 register("gpud_surv_spectrum_device", I, [P, P, L64, I, I, P])
 register("gpud_surv_cfar_runs_device", I, [P, P, I, C.POINTER(gpud_cfar_params), P, P, I])
+register("gpud_surv_cfar_runs_device_ex", I, [P, P, I, C.POINTER(gpud_cfar_params), P, P, I, P])
 register("gpud_surv_finish_frame", I, [P, I, I, P, I, C.POINTER(gpud_detector_params), P, I])
 register("gpud_surv_finish_device", I, [P, P, I, P, P, I, C.POINTER(gpud_detector_params), P, I, P])
 register("gpud_surv_level_spectrum_device", I, [P, P, L64, I, I, P])
@@ -199,14 +200,19 @@ class Surveillance:
         self.sync()
         return dpsd.download(np.float32, self.n * n_frames).reshape(n_frames, self.n)
 
-    def CFARRuns(self, psd, cfar: gpud_cfar_params, max_runs: int = 2048):
+    def CFARRuns(self, psd, cfar: gpud_cfar_params, max_runs: int = 2048, ambiguous: bool = False):
+        """(run counts, runs per frame), plus the per-frame count of bins whose
+        decision the reference's running sums could take the other way when
+        ambiguous is set."""
         psd = np.ascontiguousarray(psd, np.float32).reshape(-1, self.n)
         nf = psd.shape[0]
         dpsd = DeviceBuffer(psd.nbytes).upload(psd)
         druns = DeviceBuffer(C.sizeof(gpud_surv_run) * max_runs * nf)
         dcnt = DeviceBuffer(4 * nf)
-        check("gpud_surv_cfar_runs_device", lib().gpud_surv_cfar_runs_device(self._h, dpsd.ptr, nf, C.byref(cfar),
-                                                                             druns.ptr, dcnt.ptr, max_runs))
+        damb = DeviceBuffer(4 * nf) if ambiguous else None
+        check("gpud_surv_cfar_runs_device_ex",
+              lib().gpud_surv_cfar_runs_device_ex(self._h, dpsd.ptr, nf, C.byref(cfar), druns.ptr, dcnt.ptr, max_runs,
+                                                  damb.ptr if damb else None))
         self.sync()
         counts = dcnt.download(np.int32, nf)
         raw = (gpud_surv_run * (max_runs * nf))()
@@ -214,6 +220,8 @@ class Surveillance:
         out = []
         for f in range(nf):
             out.append([raw[f * max_runs + i] for i in range(min(int(counts[f]), max_runs))])
+        if ambiguous:
+            return counts, out, damb.download(np.int32, nf)
         return counts, out
 
     def DetectSpectra(self, psd, cfar, det, cap=1024, max_runs=2048):

@@ -207,12 +207,37 @@ __device__ __forceinline__ int wrap_bin(int i, int n, int wrap)
     return i < 0 ? 0 : (i >= n ? n - 1 : i);
 }
 
+// The noise estimate in the reference's operation order: the sums times the
+// reciprocal of the cell count (ca.go:57-58, gosca.go:55-57, caso.go).
 __device__ __forceinline__ double cfar_combine(int mode, double left, double right, int train)
 {
-    if (mode == GPUD_CFAR_CA) return (left + right) / (double)(2 * train);
-    const double lm = left / (double)train, rm = right / (double)train;
+    if (mode == GPUD_CFAR_CA) return (left + right) * (1.0 / (double)(2 * train));
+    const double inv = 1.0 / (double)train;
+    const double lm = left * inv, rm = right * inv;
     if (mode == GPUD_CFAR_GOSCA) return lm > rm ? lm : rm;
     return lm < rm ? lm : rm;  // CASO
+}
+
+// Near-tie test against the reference's running sums.  The reference slides
+// its window sums along the frame (ca.go:61-66: per bin one subtraction and
+// one addition per side), so at bin i each side carries at most 2 (i + t)
+// roundings of partial sums no larger than t * lmax (lmax the frame's largest
+// linear cell): |error| <= 2 u (i + t) t lmax per side, u = 2^-53.  Here the
+// sums are direct (t roundings).  A bin whose dB value lies within the
+// resulting threshold uncertainty of the threshold, or of the threshold less
+// the hysteresis, may be decided either way by the reference: it is counted
+// as ambiguous.  The bound adds the error of the direct sums and a slack for
+// last-bit differences of pow and log10.
+__device__ __forceinline__ double cfar_tie_db(int mode, double left, double right, int train, int i, double lmax)
+{
+    const double u = 1.1102230246251565e-16;
+    const double side = 2.0 * u * (double)(i + 2 * train + 2) * (double)train * lmax;
+    double ref;
+    if (mode == GPUD_CFAR_CA) ref = left + right;
+    else if (mode == GPUD_CFAR_GOSCA) ref = left > right ? left : right;
+    else ref = left < right ? left : right;
+    const double rel = (mode == GPUD_CFAR_CA ? 2.0 * side : side) / ref;
+    return 4.3429448190325182 * rel + 1e-12;  // 10 / ln 10 per unit relative error
 }
 
 // OS-CFAR (os.go:31-80): the rank-th smallest (0-based rk) of the 2*train
@@ -266,14 +291,33 @@ __device__ double cfar_threshold_at(const float* __restrict__ db, int n, int i, 
 __global__ __launch_bounds__(kCfarThreads) void argus_cfar_runs_kernel(const float* __restrict__ psd, int n,
                                                                        gpud_cfar_params p,
                                                                        gpud_surv_run* __restrict__ runs,
-                                                                       int* __restrict__ run_counts, int max_runs)
+                                                                       int* __restrict__ run_counts, int max_runs,
+                                                                       int* __restrict__ ambiguous)
 {
     extern __shared__ double lin[];  // kCfarTile + 2 (guard + train) doubles, reused below
+    __shared__ float red[kCfarThreads / 64];
+    __shared__ int n_tie;
     const int t = threadIdx.x;
     const float* db = psd + (int64_t)blockIdx.x * n;
     const int halo = p.guard + p.train;
     const int tile = n < kCfarTile ? n : kCfarTile;
     uint32_t* status = (uint32_t*)(lin + tile + 2 * halo);  // 2 bits per bin
+
+    // the frame's largest dB value (its linear power bounds every window sum)
+    double lmax = 0.0;
+    if (ambiguous && p.mode != GPUD_CFAR_OS) {
+        float m = -INFINITY;
+        for (int b = t; b < n; b += kCfarThreads) m = fmaxf(m, db[b]);
+        for (int o = 32; o > 0; o >>= 1) m = fmaxf(m, __shfl_xor(m, o));
+        if ((t & 63) == 0) red[t >> 6] = m;
+        if (t == 0) n_tie = 0;
+        __syncthreads();
+        m = red[0];
+        for (int w = 1; w < kCfarThreads / 64; w++) m = fmaxf(m, red[w]);
+        lmax = pow(10.0, (double)m / 10.0);
+    } else if (t == 0) {
+        n_tie = 0;
+    }
 
     // ---- CFAR status per bin: 2 = v >= thr, 0 = v < thr - hysteresis, 1 = between
     for (int w = t; w < n / 16; w += kCfarThreads) status[w] = 0u;
@@ -289,7 +333,8 @@ __global__ __launch_bounds__(kCfarThreads) void argus_cfar_runs_kernel(const flo
         // Window sums are direct (k = 1..train in order), the threshold of
         // ca.go:50-59 / gosca.go / caso.go without the running-sum drift.
         for (int j = t; j < tile; j += kCfarThreads) {
-            double thr;
+            double thr, tie = 0.0;
+            const int b = t0 + j;
             if (os) {
                 auto cell = [&](int m) {
                     const int k = m / 2 + 1;
@@ -303,9 +348,11 @@ __global__ __launch_bounds__(kCfarThreads) void argus_cfar_runs_kernel(const flo
                     right += lin[j + p.guard + k + halo];
                 }
                 thr = 10.0 * log10(cfar_combine(p.mode, left, right, p.train)) + p.scale_db;
+                if (ambiguous) tie = cfar_tie_db(p.mode, left, right, p.train, b, lmax);
             }
-            const int b = t0 + j;
             const double v = (double)db[b];
+            if (ambiguous && !os && (fabs(v - thr) <= tie || fabs(v - (thr - p.hysteresis_db)) <= tie))
+                atomicAdd(&n_tie, 1);
             const uint32_t code = v >= thr ? 2u : (v < thr - p.hysteresis_db ? 0u : 1u);
             if (code) atomicOr(&status[b >> 4], code << ((b & 15) * 2));
         }
@@ -409,7 +456,10 @@ __global__ __launch_bounds__(kCfarThreads) void argus_cfar_runs_kernel(const flo
         rr.noise_db = cfar_threshold_at(db, n, pk, p) - p.scale_db;
         out[r] = rr;
     }
-    if (t == 0) run_counts[blockIdx.x] = total;
+    if (t == 0) {
+        run_counts[blockIdx.x] = total;
+        if (ambiguous) ambiguous[blockIdx.x] = n_tie;
+    }
 }
 
 }  // namespace
@@ -630,6 +680,13 @@ GPUD_API int gpud_surv_cfar_runs_device(gpud_surv* s, const float* psd_dev, int 
                                         const gpud_cfar_params* params, gpud_surv_run* runs_dev,
                                         int* run_counts_dev, int max_runs)
 {
+    return gpud_surv_cfar_runs_device_ex(s, psd_dev, n_frames, params, runs_dev, run_counts_dev, max_runs, nullptr);
+}
+
+GPUD_API int gpud_surv_cfar_runs_device_ex(gpud_surv* s, const float* psd_dev, int n_frames,
+                                           const gpud_cfar_params* params, gpud_surv_run* runs_dev,
+                                           int* run_counts_dev, int max_runs, int* ambiguous_dev)
+{
     if (!s || !psd_dev || !params || !runs_dev || !run_counts_dev || max_runs <= 0) return -1;
     const gpud_cfar_params p = *params;
     if (p.train <= 0 || p.guard < 0 || p.guard + p.train > 2048) return -1;
@@ -653,7 +710,7 @@ GPUD_API int gpud_surv_cfar_runs_device(gpud_surv* s, const float* psd_dev, int 
                                         hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     argus_clear_error();
     hipLaunchKernelGGL(argus_cfar_runs_kernel, dim3(n_frames), dim3(kCfarThreads), lds, s->stream, psd_dev, s->n, pp,
-                       runs_dev, run_counts_dev, max_runs);
+                       runs_dev, run_counts_dev, max_runs, ambiguous_dev);
     return (int)hipGetLastError();
 }
 

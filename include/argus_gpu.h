@@ -879,6 +879,17 @@ GPUD_API int gpud_surv_smooth_device(gpud_surv* s, const float* psd_dev, int n_f
 GPUD_API int gpud_surv_cfar_runs_device(gpud_surv* s, const float* psd_dev, int n_frames,
                                         const gpud_cfar_params* params, gpud_surv_run* runs_dev,
                                         int* run_counts_dev, int max_runs);
+/* The same, with ambiguous_dev[f] = the number of frame f's bins whose CFAR
+ * decision the reference's running sums (ca.go:61-66, slid along the frame
+ * in FP64) could take the other way: bins whose dB value lies within the
+ * bound of that summation's rounding (2 u (i + t) t lmax per side at bin i,
+ * lmax the frame's largest linear cell) of the threshold or of the threshold
+ * less the hysteresis.  Zero means the frame's runs are those the reference
+ * computes from the same spectrum.  OS-CFAR ranks without arithmetic: always
+ * zero.  ambiguous_dev may be NULL. */
+GPUD_API int gpud_surv_cfar_runs_device_ex(gpud_surv* s, const float* psd_dev, int n_frames,
+                                           const gpud_cfar_params* params, gpud_surv_run* runs_dev,
+                                           int* run_counts_dev, int max_runs, int* ambiguous_dev);
 /* detector.go:483-639 expandSignalEdges + mergeOverlapping +
  * powerWeightedCenter on one frame's runs (host memory).  Returns the signal
  * count (<= cap) or -1. */

@@ -88,6 +88,49 @@ def test_detection_list_identical_on_same_spectrum(surv, mode):
     s.Close()
 
 
+@pytest.mark.parametrize("mode", ["CA", "GOSCA", "CASO", "OS"])
+def test_cfar_ambiguity_flags_ties(surv, mode):
+    """A flat spectrum with hysteresis equal to the CFAR scale puts every
+    bin on the threshold less the hysteresis (up to rounding): each bin's
+    decision is a tie the reference's running sums may break either way, so
+    every bin is ambiguous -- except for OS-CFAR, which ranks dB values
+    without arithmetic.  With any other hysteresis, no bin is ambiguous.  On
+    the 40-carrier scene no frame has an ambiguous bin, and the runs equal
+    the oracle's (running-sum) runs."""
+    from argus.synth import surveillance_scene
+    n = 4096
+    s = surv.Surveillance(n, FS)
+    flat = np.full((2, n), -87.25, np.float32)
+    for hyst, want in ((6.0, 0 if mode == "OS" else n), (3.0, 0)):
+        cfar = surv.cfar_params(mode, guard=4, train=16, scale_db=6.0, wrap=True, hysteresis_db=hyst, rank=24)
+        _, _, amb = s.CFARRuns(flat, cfar, ambiguous=True)
+        assert list(amb) == [want, want], (mode, hyst, list(amb))
+    s.Close()
+    n, hop, segs, frames = 32768, 16384, 8, 3
+    total = (frames - 1) * segs * hop + (segs - 1) * hop + n
+    iq, _ = surveillance_scene(total, FS, n_carriers=40, seed=11)
+    s = surv.Surveillance(n, FS)
+    psd = s.WelchPSD(iq, segs, hop, segs * hop, frames)
+    cfar = surv.cfar_params(mode, guard=4, train=16, scale_db=12.3748, wrap=True, hysteresis_db=3.0, rank=24)
+    counts, runs, amb = s.CFARRuns(psd, cfar, ambiguous=True)
+    assert list(amb) == [0] * frames
+    for f in range(frames):
+        spec = psd[f].astype(np.float64)
+        thr = O.CFARThresholds(mode, spec, 4, 16, 12.3748, wrap=True, rank=24)
+        want, on, start = [], False, 0  # threshold crossing with hysteresis (detector.go:436-480)
+        for b in range(n):
+            nxt = True if spec[b] >= thr[b] else (False if spec[b] < thr[b] - 3.0 else on)
+            if nxt and not on:
+                start = b
+            if on and not nxt:
+                want.append((start, b - 1))
+            on = nxt
+        if on:
+            want.append((start, n - 1))
+        assert [(r.first_bin, r.last_bin) for r in runs[f]] == want, f
+    s.Close()
+
+
 def test_cfar_runs_known_plateau(surv):
     n = 4096
     s = surv.Surveillance(n, FS)
