@@ -125,12 +125,26 @@ static size_t align256(size_t v) { return (v + 255) & ~(size_t)255; }
 char* ServeArena::alloc(size_t bytes)
 {
     bytes = align256(bytes ? bytes : 1);
-    for (auto it = free_list.begin(); it != free_list.end(); ++it) {
-        if (it->second < bytes) continue;
-        const size_t off = it->first, rest = it->second - bytes;
-        free_list.erase(it);
-        if (rest) free_list.emplace(off + bytes, rest);
-        return base + off;
+    for (int pass = 0; pass < 2; pass++) {
+        for (auto it = free_list.begin(); it != free_list.end(); ++it) {
+            if (it->second < bytes) continue;
+            char* p = it->first;
+            const size_t rest = it->second - bytes;
+            free_list.erase(it);
+            if (rest) free_list.emplace(p + bytes, rest);
+            return p;
+        }
+        if (pass || total >= cap) break;
+        // a new region: as large as everything so far (256 MiB first), at
+        // least the request, within the cap
+        size_t want = std::max<size_t>(total ? total : ((size_t)256 << 20), bytes);
+        want = std::min(want, cap - total);
+        if (want < bytes) break;
+        char* base = nullptr;
+        if (hipMalloc((void**)&base, want) != hipSuccess) break;
+        regions.emplace(base, want);
+        total += want;
+        free_list.emplace(base, want);
     }
     return nullptr;
 }
@@ -139,20 +153,30 @@ void ServeArena::release(char* p, size_t bytes)
 {
     if (!p) return;
     bytes = align256(bytes ? bytes : 1);
-    size_t off = (size_t)(p - base);
-    auto next = free_list.lower_bound(off);
-    if (next != free_list.end() && off + bytes == next->first) {  // merge with the block after
+    auto reg = std::prev(regions.upper_bound(p));  // the region holding p
+    char* lo = reg->first;
+    char* hi = reg->first + reg->second;
+    auto next = free_list.lower_bound(p);
+    if (next != free_list.end() && p + bytes == next->first && next->first < hi) {  // merge with the block after
         bytes += next->second;
         next = free_list.erase(next);
     }
     if (next != free_list.begin()) {  // and with the block before
         auto prev = std::prev(next);
-        if (prev->first + prev->second == off) {
+        if (prev->first >= lo && prev->first + prev->second == p) {
             prev->second += bytes;
             return;
         }
     }
-    free_list.emplace(off, bytes);
+    free_list.emplace(p, bytes);
+}
+
+void ServeArena::destroy()
+{
+    for (auto& kv : regions) hipFree(kv.first);
+    regions.clear();
+    free_list.clear();
+    total = 0;
 }
 
 // Copy `bytes` of host data to device memory `dst` through the pinned ring,
@@ -351,9 +375,7 @@ GPUD_API int gpud_serve_create(int sample_rate, int block_samples, uint64_t hbm_
         ARGUS_CHECK(host_alloc(&sl.h_in, (size_t)block_samples));
     }
     // channel state arena (bounded by the budget) and the upload ring
-    s->arena.size = align256((size_t)std::min<uint64_t>(hbm_budget_bytes, (uint64_t)16 << 30));
-    ARGUS_CHECK(hipMalloc((void**)&s->arena.base, s->arena.size));
-    s->arena.free_list.emplace(0, s->arena.size);
+    s->arena.cap = align256((size_t)std::min<uint64_t>(hbm_budget_bytes, (uint64_t)16 << 30));
     s->stage.size = (size_t)16 << 20;
     ARGUS_CHECK(hipHostMalloc((void**)&s->stage.ring, s->stage.size, hipHostMallocDefault));
     const int smax = argus_fast_segments(block_samples);
@@ -371,7 +393,7 @@ GPUD_API int gpud_serve_destroy(gpud_serve* s)
     hipDeviceSynchronize();
     for (auto& kv : s->live) release_channel(s, kv.second);
     for (auto& c : s->closed) release_channel(s, c);
-    hipFree(s->arena.base);
+    s->arena.destroy();
     hipHostFree(s->stage.ring);
     for (auto& b : s->stage.busy) hipEventDestroy(b.second);
     for (hipEvent_t ev : s->stage.spare) hipEventDestroy(ev);
@@ -696,13 +718,25 @@ static bool slot_fits(const ServeSlot& sl, const BlockLayout& L)
 }
 
 // Grow the slot's output buffers for a layout (invalidates the outputs of
-// the block the slot held).
-static int slot_grow(ServeSlot& sl, const BlockLayout& L)
+// the block the slot held).  Newly allocated download buffers are written
+// once, whole, by a copy from their device twins on the copy stream: the
+// first transfers into fresh pinned pages run at about half speed, so this
+// moves that cost to the block that grew the slot instead of the dozens of
+// blocks that follow.
+static int slot_grow(ServeSlot& sl, const BlockLayout& L, hipStream_t copy)
 {
+    float2* old_iq = sl.h_iq;
+    float* old_audio = sl.h_audio;
     ARGUS_CHECK((hipError_t)grow_dev(&sl.d_iq, &sl.iq_cap, (size_t)std::max<int64_t>(L.iq_total, 1)));
     ARGUS_CHECK((hipError_t)grow_dev(&sl.d_audio, &sl.audio_cap, (size_t)std::max<int64_t>(L.audio_total, 1)));
     ARGUS_CHECK((hipError_t)grow_host(&sl.h_iq, &sl.h_iq_cap, (size_t)std::max<int64_t>(L.iq_total, 1)));
     ARGUS_CHECK((hipError_t)grow_host(&sl.h_audio, &sl.h_audio_cap, (size_t)std::max<int64_t>(L.audio_total, 1)));
+    if (sl.h_iq != old_iq)
+        ARGUS_CHECK(hipMemcpyAsync(sl.h_iq, sl.d_iq, std::min(sl.h_iq_cap, sl.iq_cap) * sizeof(float2),
+                                   hipMemcpyDeviceToHost, copy));
+    if (sl.h_audio != old_audio)
+        ARGUS_CHECK(hipMemcpyAsync(sl.h_audio, sl.d_audio, std::min(sl.h_audio_cap, sl.audio_cap) * sizeof(float),
+                                   hipMemcpyDeviceToHost, copy));
     if (L.n_wfm > 0) {
         ARGUS_CHECK((hipError_t)grow_dev(&sl.d_mono, &sl.mono_cap, (size_t)std::max<int64_t>(L.mono_total, 1)));
         ARGUS_CHECK((hipError_t)grow_dev(&sl.d_wscratch, &sl.wscratch_cap, (size_t)std::max<int64_t>(4 * L.mono_total, 1)));
@@ -967,7 +1001,7 @@ static int build_block(gpud_serve* s, ServeSlot& sl, int n, const float2* in, bo
 {
     BlockLayout L;
     layout_block(s, n, L);
-    int err = slot_grow(sl, L);
+    int err = slot_grow(sl, L, s->copy);
     if (err) return err;
     slot_adopt(sl, L);
     return describe_block(s, sl, sl.outs, sl.n_wfm, n, in, cache_items);

@@ -225,12 +225,21 @@ struct ArgusServeHdr {
 // Device memory of channel state: one allocation carved into 256-byte aligned
 // blocks, first fit over a coalescing free list (offset -> bytes).  Opening
 // or retiring a channel is host bookkeeping only.
+// Device memory for channel state, taken from the budget in regions: the
+// first of 256 MiB (or the cap), each further one as large as everything
+// before it, up to `cap` in all.  Free space is a first-fit list of
+// (address, bytes) that coalesces within a region; a region is kept until
+// the service is destroyed.  Growing by regions keeps a service's footprint
+// near what its channels use: allocating (and later freeing) the whole cap
+// at once costs the driver a full clear of that memory.
 struct ServeArena {
-    char* base = nullptr;
-    size_t size = 0;
-    std::map<size_t, size_t> free_list;
+    size_t cap = 0;                              // bytes the regions may total
+    size_t total = 0;                            // bytes allocated in regions
+    std::map<char*, size_t> regions;             // base -> bytes
+    std::map<char*, size_t> free_list;           // address -> bytes
     char* alloc(size_t bytes);
     void release(char* p, size_t bytes);
+    void destroy();
 };
 
 // Pinned staging of channel-state uploads: regions taken in ring order, each

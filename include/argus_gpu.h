@@ -413,11 +413,13 @@ typedef struct gpud_serve_block_info {
 } gpud_serve_block_info;
 
 /* hbm_budget_bytes bounds the channels admitted (their state plus their share
- * of the pipelined output buffers); channel state lives in one arena of
- * min(budget, 16 GiB) allocated here, so opening and retiring channels
- * allocate and free nothing on the device (their initial state is copied
- * asynchronously on the service's compute stream, ahead of their first
- * block). */
+ * of the pipelined output buffers).  Channel state lives in an arena grown
+ * in regions -- 256 MiB first, each next one as large as all before, up to
+ * min(budget, 16 GiB) -- kept until the service is destroyed: opening a
+ * channel allocates on the device only when the arena grows, retiring one
+ * frees nothing (its block returns to the arena), and a channel's initial
+ * state is copied asynchronously on the service's compute stream, ahead of
+ * its first block. */
 GPUD_API int gpud_serve_create(int sample_rate, int block_samples, uint64_t hbm_budget_bytes, gpud_serve** out);
 GPUD_API int gpud_serve_destroy(gpud_serve* s);
 /* 0, -1 bad args / non-integer decimation, GPUD_ERR_DUPLICATE, GPUD_ERR_BUDGET */
