@@ -718,25 +718,13 @@ static bool slot_fits(const ServeSlot& sl, const BlockLayout& L)
 }
 
 // Grow the slot's output buffers for a layout (invalidates the outputs of
-// the block the slot held).  Newly allocated download buffers are written
-// once, whole, by a copy from their device twins on the copy stream: the
-// first transfers into fresh pinned pages run at about half speed, so this
-// moves that cost to the block that grew the slot instead of the dozens of
-// blocks that follow.
-static int slot_grow(ServeSlot& sl, const BlockLayout& L, hipStream_t copy)
+// the block the slot held).
+static int slot_grow(ServeSlot& sl, const BlockLayout& L)
 {
-    float2* old_iq = sl.h_iq;
-    float* old_audio = sl.h_audio;
     ARGUS_CHECK((hipError_t)grow_dev(&sl.d_iq, &sl.iq_cap, (size_t)std::max<int64_t>(L.iq_total, 1)));
     ARGUS_CHECK((hipError_t)grow_dev(&sl.d_audio, &sl.audio_cap, (size_t)std::max<int64_t>(L.audio_total, 1)));
     ARGUS_CHECK((hipError_t)grow_host(&sl.h_iq, &sl.h_iq_cap, (size_t)std::max<int64_t>(L.iq_total, 1)));
     ARGUS_CHECK((hipError_t)grow_host(&sl.h_audio, &sl.h_audio_cap, (size_t)std::max<int64_t>(L.audio_total, 1)));
-    if (sl.h_iq != old_iq)
-        ARGUS_CHECK(hipMemcpyAsync(sl.h_iq, sl.d_iq, std::min(sl.h_iq_cap, sl.iq_cap) * sizeof(float2),
-                                   hipMemcpyDeviceToHost, copy));
-    if (sl.h_audio != old_audio)
-        ARGUS_CHECK(hipMemcpyAsync(sl.h_audio, sl.d_audio, std::min(sl.h_audio_cap, sl.audio_cap) * sizeof(float),
-                                   hipMemcpyDeviceToHost, copy));
     if (L.n_wfm > 0) {
         ARGUS_CHECK((hipError_t)grow_dev(&sl.d_mono, &sl.mono_cap, (size_t)std::max<int64_t>(L.mono_total, 1)));
         ARGUS_CHECK((hipError_t)grow_dev(&sl.d_wscratch, &sl.wscratch_cap, (size_t)std::max<int64_t>(4 * L.mono_total, 1)));
@@ -1001,7 +989,7 @@ static int build_block(gpud_serve* s, ServeSlot& sl, int n, const float2* in, bo
 {
     BlockLayout L;
     layout_block(s, n, L);
-    int err = slot_grow(sl, L, s->copy);
+    int err = slot_grow(sl, L);
     if (err) return err;
     slot_adopt(sl, L);
     return describe_block(s, sl, sl.outs, sl.n_wfm, n, in, cache_items);
