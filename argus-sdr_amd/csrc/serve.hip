@@ -321,7 +321,7 @@ static void slot_release(ServeSlot& sl)
     sl.h_wblk = nullptr;
     sl.h_tab = nullptr;
     sl.d_tab = nullptr;
-    sl.gexec[0] = sl.gexec[1] = nullptr;
+    for (auto& ge : sl.gexec) ge = nullptr;
     sl.iq_cap = sl.audio_cap = sl.h_iq_cap = sl.h_audio_cap = 0;
     sl.mono_cap = sl.wscratch_cap = sl.rds_cap = sl.h_rds_cap = sl.wblk_cap = sl.h_wblk_cap = 0;
     sl.tab_cap = 0;
@@ -371,6 +371,7 @@ GPUD_API int gpud_serve_create(int sample_rate, int block_samples, uint64_t hbm_
         ARGUS_CHECK(hipEventCreate(&sl.ev_start));
         ARGUS_CHECK(hipEventCreate(&sl.ev_kstart));
         ARGUS_CHECK(hipEventCreate(&sl.ev_xend));  // timed: extract_ms (recorded inside the block's graph)
+        ARGUS_CHECK(hipEventCreateWithFlags(&sl.ev_early, hipEventDisableTiming));
         ARGUS_CHECK(dev_alloc(&sl.d_in, (size_t)block_samples));
         ARGUS_CHECK(host_alloc(&sl.h_in, (size_t)block_samples));
     }
@@ -405,6 +406,7 @@ GPUD_API int gpud_serve_destroy(gpud_serve* s)
         hipEventDestroy(sl.ev_start);
         hipEventDestroy(sl.ev_kstart);
         hipEventDestroy(sl.ev_xend);
+        hipEventDestroy(sl.ev_early);
         hipFree(sl.d_in);
         hipHostFree(sl.h_in);
         slot_release(sl);
@@ -639,18 +641,45 @@ static int grow_table(ServeSlot& sl, size_t need)
 
 // Output layout of one block of n samples over the live channels (in the
 // order of `live`): per-channel counts and offsets, and the totals.
+// Whether the channel takes the fast form this block (its history is the
+// stream's own last T-1 samples).
+static bool fast_now(const gpud_serve* s, const ServeChannel& c, int n)
+{
+    return s->fast && c.d_fastG && c.hist_len == c.T - 1 && n > 0;
+}
+
+// Whether its outputs are complete after the fast kernels and its own
+// demodulation (no stage after the demodulator, none before it).
+static bool early_now(const gpud_serve* s, const ServeChannel& c, int n)
+{
+    return fast_now(s, c, n) && c.pre_factor == 1 && !c.stereo && c.audio_rate == 0;
+}
+
 static void layout_block(const gpud_serve* s, int n, BlockLayout& L)
 {
     L.outs.clear();
     L.outs.reserve(s->live.size());
     L.iq_total = L.audio_total = L.mono_total = L.rds_total = L.aout_total = L.pre_total = 0;
     L.n_wfm = L.n_audio_chans = L.n_pre_chans = 0;
+    // the early channels' IQ and demodulated outputs first (in id order),
+    // then the others'
+    int64_t iq_e = 0, au_e = 0;
+    for (const auto& kv : s->live) {
+        const ServeChannel& c = kv.second;
+        if (!early_now(s, c, n)) continue;
+        iq_e += (c.phase_count + n) / c.decim;
+        au_e += argus_demod_count(c.spec.demod, (c.phase_count + n) / c.decim, c.have_carry);
+    }
+    L.iq_early = iq_e;
+    L.audio_early = au_e;
+    int64_t iq_at_e = 0, au_at_e = 0, iq_at = iq_e, au_at = au_e;
     for (const auto& kv : s->live) {
         const ServeChannel& c = kv.second;
         ServeOut o;
         o.channel_id = kv.first;
+        o.early = early_now(s, c, n) ? 1 : 0;
         o.n_iq = (c.phase_count + n) / c.decim;
-        o.iq_off = L.iq_total;
+        o.iq_off = o.early ? iq_at_e : iq_at;
         // the demodulator's input: the pre-demod stage's outputs, or the baseband
         o.n_pre = o.n_iq;
         o.pre_off = -1;
@@ -661,7 +690,7 @@ static void layout_block(const gpud_serve* s, int n, BlockLayout& L)
             L.n_pre_chans++;
         }
         o.n_audio = argus_demod_count(c.spec.demod, o.n_pre, c.have_carry);
-        o.audio_off = L.audio_total;
+        o.audio_off = o.early ? au_at_e : au_at;
         o.rate = c.spec.out_rate;
         o.wfm_index = -1;
         o.mono_off = o.rds_off = 0;
@@ -695,10 +724,12 @@ static void layout_block(const gpud_serve* s, int n, BlockLayout& L)
             o.aout_off = L.aout_total;
             L.aout_total += (int64_t)o.n_aout * o.a_channels;
         }
-        L.iq_total += o.n_iq;
-        L.audio_total += o.n_audio;
+        (o.early ? iq_at_e : iq_at) += o.n_iq;
+        (o.early ? au_at_e : au_at) += o.n_audio;
         L.outs.push_back(o);
     }
+    L.iq_total = iq_at;
+    L.audio_total = au_at;
 }
 
 // Whether the slot's output buffers already hold a block of this layout.
@@ -752,6 +783,8 @@ static void slot_adopt(ServeSlot& sl, BlockLayout& L)
     sl.rds_total = L.rds_total;
     sl.iq_total = L.iq_total;
     sl.audio_total = L.audio_total;
+    sl.iq_early = L.iq_early;
+    sl.audio_early = L.audio_early;
     sl.n_audio_chans = L.n_audio_chans;
     sl.aout_total = L.aout_total;
     sl.pre_total = L.pre_total;
@@ -782,6 +815,7 @@ static int describe_block(gpud_serve* s, ServeSlot& sl, const std::vector<ServeO
     s->ddescs.resize((size_t)nch);
     s->wdescs.resize((size_t)n_wfm);
     s->ditems.clear();
+    s->ditems_e.clear();
     s->witems.clear();
     int n_audio = 0;
     for (const ServeOut& o : outs) n_audio += o.a_index >= 0 ? 1 : 0;
@@ -803,7 +837,7 @@ static int describe_block(gpud_serve* s, ServeSlot& sl, const std::vector<ServeO
             return -1;
         // the fast path, once the channel's history is the stream's own
         // last T-1 samples; its direct items are then only the history
-        const bool fast = s->fast && c.d_fastG && c.hist_len == c.T - 1 && n > 0;
+        const bool fast = fast_now(s, c, n);
         if (fast) {
             ArgusFastDesc fd;
             fd.G = c.d_fastG;
@@ -861,11 +895,12 @@ static int describe_block(gpud_serve* s, ServeSlot& sl, const std::vector<ServeO
         dd.bfo0 = argus_rad_to_fix(c.bfo_phase + bfo_inc);
         dd.bfo_inc = argus_rad_to_fix(bfo_inc);
         if (dd.mode != ARGUS_DEMOD_NONE && o.n_pre > 0) {
+            std::vector<ArgusDemodItem>& di = o.early ? s->ditems_e : s->ditems;
             if (dd.mode == ARGUS_DEMOD_AM) {
-                s->ditems.push_back({ci, 0, o.n_pre, 0});
+                di.push_back({ci, 0, o.n_pre, 0});
             } else {
                 for (int st = 0; st < o.n_pre; st += ARGUS_DEMOD_TILE)
-                    s->ditems.push_back({ci, st, std::min(ARGUS_DEMOD_TILE, o.n_pre - st), 0});
+                    di.push_back({ci, st, std::min(ARGUS_DEMOD_TILE, o.n_pre - st), 0});
             }
         }
         if (c.stereo) {
@@ -883,6 +918,9 @@ static int describe_block(gpud_serve* s, ServeSlot& sl, const std::vector<ServeO
         items_bytes = s->items.size() * sizeof(ArgusWorkItem);
     }
     const int n_items = reuse ? sl.n_items : (int)s->items.size();
+    // the early channels' demod items first
+    const int n_ditems_e = (int)s->ditems_e.size();
+    s->ditems.insert(s->ditems.begin(), s->ditems_e.begin(), s->ditems_e.end());
     const int n_ditems = (int)s->ditems.size(), n_witems = (int)s->witems.size();
     const size_t off_hdr = 0;
     const size_t off_chans = align_up(off_hdr + sizeof(ArgusServeHdr));
@@ -921,7 +959,10 @@ static int describe_block(gpud_serve* s, ServeSlot& sl, const std::vector<ServeO
     ArgusServeHdr& hdr = *(ArgusServeHdr*)(sl.h_tab + off_hdr);
     const char* d = sl.d_tab;
     hdr.x = ArgusExtractHdr{(const ArgusChanDesc*)(d + off_chans), sl.d_items, in, n_items, n};
-    hdr.d = ArgusDemodHdr{(const ArgusDemodDesc*)(d + off_ddesc), (const ArgusDemodItem*)(d + off_ditems), n_ditems, 0};
+    hdr.de = ArgusDemodHdr{(const ArgusDemodDesc*)(d + off_ddesc), (const ArgusDemodItem*)(d + off_ditems), n_ditems_e,
+                           0};
+    hdr.d = ArgusDemodHdr{(const ArgusDemodDesc*)(d + off_ddesc), (const ArgusDemodItem*)(d + off_ditems) + n_ditems_e,
+                          n_ditems - n_ditems_e, 0};
     hdr.w = ArgusWfmHdr{(const ArgusWfmDesc*)(d + off_wdesc), (const ArgusWfmItem*)(d + off_witems), n_wfm, n_witems};
     hdr.a = ArgusAudioHdr{(const ArgusAudioDesc*)(d + off_adesc), n_audio, 0};
     // the fast stage runs while any live channel can take the fast form
@@ -1011,14 +1052,21 @@ static int upload_table(gpud_serve* s, ServeSlot& sl)
 // The block's launches in two stages, each reading its counts and pointers
 // from the table header on the device: the fused extraction, then the
 // demodulation and (with stereo channels) the stereo stage.
+// The block's launches in four stages, each reading its counts and pointers
+// from the table header on the device: the fast form's kernels; the early
+// channels' demodulation; the direct form's extraction and the tail; then
+// the other channels' pre-demod stage, demodulation, stereo and audio.
 static int launch_stage(gpud_serve* s, ServeSlot& sl, int stage, bool wfm)
 {
     const ArgusServeHdr* dh = (const ArgusServeHdr*)(sl.d_tab + sl.off_hdr);
-    if (stage == 0) {
-        int err = argus_launch_fast(&dh->f, &dh->x, sl.fast_S, s->d_fu, s->d_fX, s->compute);
-        if (!err) err = argus_launch_extract_hdr(&dh->x, sl.lds, s->tune.grid, s->compute);
-        if (!err) err = argus_launch_fast_tail(&dh->f, &dh->x, s->compute);
-        return err;
+    switch (stage) {
+    case 0: return argus_launch_fast(&dh->f, &dh->x, sl.fast_S, s->d_fu, s->d_fX, s->compute);
+    case 1: return argus_launch_demod_hdr(&dh->de, s->compute);
+    case 2: {
+        int err = argus_launch_extract_hdr(&dh->x, sl.lds, s->tune.grid, s->compute);
+        return err ? err : argus_launch_fast_tail(&dh->f, &dh->x, s->compute);
+    }
+    default: break;
     }
     int err = sl.p_mode ? argus_launch_predemod_hdr(&dh->p, s->compute) : 0;
     if (err) return err;
@@ -1031,19 +1079,23 @@ static int launch_stage(gpud_serve* s, ServeSlot& sl, int stage, bool wfm)
     return sl.a_mode ? argus_launch_audio_hdr(&dh->a, s->compute) : 0;
 }
 
-// Run the block's two stages, recording ev_xend between them (the
-// extraction's end, for extract_ms).  In graph mode each stage is the
-// replay of a captured per-slot graph, captured again when what it depends
-// on (table address, LDS size, grid, whether a stereo stage runs) has
-// changed; the event stays outside the graphs so its time can be read.
+// Run the block's four stages, recording ev_early after the second (the
+// early channels' outputs are complete: their download may start) and
+// ev_xend after the third (the extraction's end, for extract_ms).  In graph
+// mode each stage is the replay of a captured per-slot graph, captured again
+// when what it depends on (table address, LDS size, grid, which stages run)
+// has changed; the events stay outside the graphs.
 static int launch_block(gpud_serve* s, ServeSlot& sl)
 {
     const bool wfm = sl.n_wfm > 0;
     if (!s->graph) {
-        int err = launch_stage(s, sl, 0, wfm);
-        if (err) return err;
-        ARGUS_CHECK(hipEventRecord(sl.ev_xend, s->compute));
-        return launch_stage(s, sl, 1, wfm);
+        for (int stage = 0; stage < 4; stage++) {
+            int err = launch_stage(s, sl, stage, wfm);
+            if (err) return err;
+            if (stage == 1) ARGUS_CHECK(hipEventRecord(sl.ev_early, s->compute));
+            if (stage == 2) ARGUS_CHECK(hipEventRecord(sl.ev_xend, s->compute));
+        }
+        return 0;
     }
     const char* hdr = sl.d_tab + sl.off_hdr;
     if (!sl.gexec[0] || sl.g_hdr != hdr || sl.g_lds != sl.lds || sl.g_grid != s->tune.grid ||
@@ -1055,7 +1107,7 @@ static int launch_block(gpud_serve* s, ServeSlot& sl)
             }
         int lerr = argus_extract_allow_lds(sl.lds);  // outside capture
         if (lerr) return lerr;
-        for (int stage = 0; stage < 2; stage++) {
+        for (int stage = 0; stage < 4; stage++) {
             ARGUS_CHECK(hipStreamBeginCapture(s->compute, hipStreamCaptureModeThreadLocal));
             int err = launch_stage(s, sl, stage, wfm);
             hipGraph_t g = nullptr;
@@ -1080,9 +1132,12 @@ static int launch_block(gpud_serve* s, ServeSlot& sl)
         sl.g_fastS = sl.fast_S;
         sl.g_pre = sl.p_mode;
     }
-    ARGUS_CHECK(hipGraphLaunch(sl.gexec[0], s->compute));
-    ARGUS_CHECK(hipEventRecord(sl.ev_xend, s->compute));
-    return (int)hipGraphLaunch(sl.gexec[1], s->compute);
+    for (int stage = 0; stage < 4; stage++) {
+        ARGUS_CHECK(hipGraphLaunch(sl.gexec[stage], s->compute));
+        if (stage == 1) ARGUS_CHECK(hipEventRecord(sl.ev_early, s->compute));
+        if (stage == 2) ARGUS_CHECK(hipEventRecord(sl.ev_xend, s->compute));
+    }
+    return 0;
 }
 
 // A channel's RDS decoding block on the device (ServeChannel::d_rdec):
@@ -1279,14 +1334,28 @@ static int serve_submit(gpud_serve* s, const gpud_float2* iq, int n, bool on_dev
     ARGUS_CHECK(hipEventRecord(sl.ev_kern, s->compute));
     commit_block(s, sl.outs, n, sl.ticket);
     // ---- copy stream: download ------------------------------------------
-    ARGUS_CHECK(hipStreamWaitEvent(s->copy, sl.ev_kern, 0));
+    // the early channels' outputs as soon as they are complete, while the
+    // direct form runs; then the rest
     const int dl = s->dl_mask;
-    if ((dl & GPUD_DL_IQ) && sl.iq_total > 0)
-        ARGUS_CHECK(hipMemcpyAsync(sl.h_iq, sl.d_iq, (size_t)sl.iq_total * sizeof(float2), hipMemcpyDeviceToHost,
+    const bool early = ((dl & GPUD_DL_IQ) && sl.iq_early > 0) || ((dl & GPUD_DL_DEMOD) && sl.audio_early > 0);
+    if (early) {
+        ARGUS_CHECK(hipStreamWaitEvent(s->copy, sl.ev_early, 0));
+        if ((dl & GPUD_DL_IQ) && sl.iq_early > 0)
+            ARGUS_CHECK(hipMemcpyAsync(sl.h_iq, sl.d_iq, (size_t)sl.iq_early * sizeof(float2),
+                                       hipMemcpyDeviceToHost, s->copy));
+        if ((dl & GPUD_DL_DEMOD) && sl.audio_early > 0)
+            ARGUS_CHECK(hipMemcpyAsync(sl.h_audio, sl.d_audio, (size_t)sl.audio_early * sizeof(float),
+                                       hipMemcpyDeviceToHost, s->copy));
+    }
+    ARGUS_CHECK(hipStreamWaitEvent(s->copy, sl.ev_kern, 0));
+    if ((dl & GPUD_DL_IQ) && sl.iq_total > sl.iq_early)
+        ARGUS_CHECK(hipMemcpyAsync(sl.h_iq + sl.iq_early, sl.d_iq + sl.iq_early,
+                                   (size_t)(sl.iq_total - sl.iq_early) * sizeof(float2), hipMemcpyDeviceToHost,
                                    s->copy));
-    if ((dl & GPUD_DL_DEMOD) && sl.audio_total > 0)
-        ARGUS_CHECK(hipMemcpyAsync(sl.h_audio, sl.d_audio, (size_t)sl.audio_total * sizeof(float),
-                                   hipMemcpyDeviceToHost, s->copy));
+    if ((dl & GPUD_DL_DEMOD) && sl.audio_total > sl.audio_early)
+        ARGUS_CHECK(hipMemcpyAsync(sl.h_audio + sl.audio_early, sl.d_audio + sl.audio_early,
+                                   (size_t)(sl.audio_total - sl.audio_early) * sizeof(float), hipMemcpyDeviceToHost,
+                                   s->copy));
     if ((dl & GPUD_DL_WFM) && sl.rds_total > 0)
         ARGUS_CHECK(hipMemcpyAsync(sl.h_rds, sl.d_rds, (size_t)sl.rds_total * sizeof(float2), hipMemcpyDeviceToHost,
                                    s->copy));

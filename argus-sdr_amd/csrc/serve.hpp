@@ -90,6 +90,7 @@ struct ServeOut {
     int r_index;                      // index of its decode among the block's, or -1
     int n_pre;                        // demodulator input samples (the pre-demod stage's outputs, or n_iq)
     int64_t pre_off;                  // offset of the pre-demod outputs, -1 without the stage
+    int early;                        // outputs complete after the fast kernels (see BlockLayout)
 };
 
 struct ServeSlot {
@@ -98,6 +99,7 @@ struct ServeSlot {
     int n = 0;
     hipEvent_t ev_start = nullptr, ev_in = nullptr, ev_kstart = nullptr, ev_kern = nullptr, ev_done = nullptr;
     hipEvent_t ev_xend = nullptr;  // after the extraction launch
+    hipEvent_t ev_early = nullptr; // after the fast kernels and the early channels' demodulation
     float2* d_in = nullptr;
     float2* h_in = nullptr;
     float2* d_iq = nullptr;
@@ -132,7 +134,7 @@ struct ServeSlot {
     bool items_dirty = true;
     // captured launches of this slot (extraction; demodulation + stereo
     // stage) and what they were captured for
-    hipGraphExec_t gexec[2] = {nullptr, nullptr};
+    hipGraphExec_t gexec[4] = {nullptr, nullptr, nullptr, nullptr};
     const char* g_hdr = nullptr;
     size_t g_lds = 0;
     int g_grid = 0;
@@ -153,6 +155,7 @@ struct ServeSlot {
     int n_wfm = 0;
     int64_t rds_total = 0;
     int64_t iq_total = 0, audio_total = 0;
+    int64_t iq_early = 0, audio_early = 0;  // the early channels' share (BlockLayout)
     // post-demodulation audio stage
     float* d_aout = nullptr;
     size_t aout_cap = 0;
@@ -189,12 +192,18 @@ struct ServeSlot {
 };
 
 // Output layout of one block over the live channels.
+// A block's output layout.  Channels whose outputs are complete once the
+// fast-convolution kernels and their demodulation have run (fast form this
+// block, plain demodulation: no pre-demod, stereo or audio stage) come first
+// in the IQ and demodulated buffers, [0, iq_early) and [0, audio_early), so
+// their download starts while the direct form still runs.
 struct BlockLayout {
     std::vector<ServeOut> outs;
     int n_wfm = 0;
     int n_audio_chans = 0;
     int n_pre_chans = 0;
     int64_t iq_total = 0, audio_total = 0, mono_total = 0, rds_total = 0, aout_total = 0, pre_total = 0;
+    int64_t iq_early = 0, audio_early = 0;
 };
 
 // A block prepared ahead: after a block is launched, the next block's
@@ -215,16 +224,14 @@ struct ServePrep {
 // What the graph-mode kernels of one block read (first bytes of the table).
 struct ArgusServeHdr {
     ArgusExtractHdr x;
-    ArgusDemodHdr d;
+    ArgusDemodHdr d;   // demodulation of the channels that are not early
+    ArgusDemodHdr de;  // demodulation of the early channels
     ArgusWfmHdr w;
     ArgusAudioHdr a;
     ArgusFastHdr f;
     ArgusPreHdr p;
 };
 
-// Device memory of channel state: one allocation carved into 256-byte aligned
-// blocks, first fit over a coalescing free list (offset -> bytes).  Opening
-// or retiring a channel is host bookkeeping only.
 // Device memory for channel state, taken from the budget in regions: the
 // first of 256 MiB (or the cap), each further one as large as everything
 // before it, up to `cap` in all.  Free space is a first-fit list of
@@ -282,6 +289,7 @@ struct gpud_serve {
     std::vector<ArgusDemodDesc> ddescs;
     std::vector<ArgusWfmDesc> wdescs;
     std::vector<ArgusDemodItem> ditems;
+    std::vector<ArgusDemodItem> ditems_e;  // the early channels' (serve.hip: layout_block)
     std::vector<ArgusWfmItem> witems;
     std::vector<ArgusPreDesc> pdescs;
     std::vector<ArgusPreItem> pitems;

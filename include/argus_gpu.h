@@ -408,7 +408,8 @@ typedef struct gpud_serve_block_info {
     int64_t audio_total;
     float   gpu_ms;       /* event time: upload start -> download end */
     float   kernel_ms;    /* event time: extraction + demodulation kernels */
-    float   extract_ms;   /* event time: the fused extraction launch alone */
+    float   extract_ms;   /* event time: the extraction launches (fast form, the demodulation of the
+                           * channels it completes, direct form) */
     float   pad;
 } gpud_serve_block_info;
 
