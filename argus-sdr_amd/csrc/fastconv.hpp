@@ -36,14 +36,14 @@ struct ArgusFastHdr {
 
 // Whether a channel of decimation D and T taps takes the fast path: a
 // multiple of 8 (so the fold factor, D's power-of-two part capped at 64,
-// leaves at most 512 bins), at most 512 taps, and T >= 2 D.  Per segment the
-// fast form costs about 56 kflop (the 4096-bin fold and the 512-point
-// inverse) whatever D is; the direct form costs 8 T flop per output, 3584 / D
-// outputs per segment: about 28.7 T / D kflop.  Below T = 2 D the direct
-// form is the cheaper one.
+// leaves at most 512 bins), at most 512 taps, and overlapping windows
+// (D < T).  Measured at 8192 channels of the serving mix, the fast form
+// costs about 0.28 us of kernel time per channel and 20 ms block whatever D
+// is, the direct form about 0.68 ps per tap and output: the two meet near
+// T = D.
 static inline bool argus_fast_eligible(int D, int T)
 {
-    return D >= 8 && D % 8 == 0 && 2 * D <= T && T <= ARGUS_FAST_TAIL + 1;
+    return D >= 8 && D % 8 == 0 && D < T && T <= ARGUS_FAST_TAIL + 1;
 }
 static inline int argus_fast_fold(int D)
 {

@@ -359,7 +359,7 @@ GPUD_API int gpud_runner_set_exec_mode(gpud_runner* r, int mode)
 }
 
 // The fast-convolution form in the batched launch (on by default): a signal
-// whose decimation D is a multiple of 8 with 2 D <= T <= 512 is filtered
+// whose decimation D is a multiple of 8 with D < T <= 512 is filtered
 // through the shared segment transforms once its history is full.
 GPUD_API int gpud_runner_set_fast(gpud_runner* r, int enable)
 {

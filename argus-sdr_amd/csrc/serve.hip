@@ -1681,7 +1681,7 @@ GPUD_API int gpud_serve_channel_rds(gpud_serve* s, int64_t ticket, int64_t chann
 }
 
 // The fast-convolution path for eligible channels (decimation a multiple of
-// 8, 2 D <= taps <= 512, full history): on by default.
+// 8, D < taps <= 512, full history): on by default.
 GPUD_API int gpud_serve_set_fast(gpud_serve* s, int enable)
 {
     if (!s) return -1;

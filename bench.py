@@ -837,7 +837,7 @@ def fast_flops(chans, n_in):
     for (_, _, _, rate, taps, _) in chans:
         d = FS // rate
         n_out = n_in // d
-        if d % 8 == 0 and 2 * d <= taps <= 512:
+        if d % 8 == 0 and d < taps <= 512:
             dp = min(d & -d, 64)
             m = 4096 // dp
             total += S_ * (6 * 4096 + 2 * (4096 - m) + 5 * m * int(np.log2(m))) + 6 * n_out

@@ -166,7 +166,7 @@ GPUD_API int gpud_runner_destroy(gpud_runner* r);
 GPUD_API int gpud_runner_set_exec_mode(gpud_runner* r, int mode);
 /* The fast-convolution form in the batched launch (on by default; no
  * reference counterpart): a signal whose decimation D is a multiple of 8
- * with 2 D <= T <= 512 is filtered through shared 4096-point transforms of
+ * with D < T <= 512 is filtered through shared 4096-point transforms of
  * the chunk (overlap-save, decimation by spectral folding) once its history
  * is full -- the same filter, outputs within FP32 rounding of the direct
  * form.  The runner keeps the stream's last 511 samples for it; a call in
@@ -538,7 +538,7 @@ GPUD_API int gpud_serve_channel_audio(gpud_serve* s, int64_t ticket, int64_t cha
 GPUD_API int gpud_serve_set_rds(gpud_serve* s, int64_t channel_id, int window_samples);
 /* The fast-convolution form of the extraction (on by default): a channel
  * whose decimation D is a multiple of 8 and whose tap count T satisfies
- * 2 D <= T <= 512 (where the fast form is the cheaper one), once its
+ * D < T <= 512, once its
  * history is full, is filtered through one shared 4096-point transform of each input
  * segment (overlap-save, decimation by spectral folding) instead of the
  * direct polyphase form; same filter, outputs within FP32 rounding of the

@@ -21,11 +21,11 @@ BLOCK = 51_200  # 20 ms: 15 segments of 3584
 SPECS = [  # id, kind, offset, bw, out_rate, taps, demod, window
     (1, "wfm", 300_000.0, 200_000, 320_000, 511, WFM, 0),   # D 8, longest taps
     (2, "fm", -400_000.0, 25_000, 160_000, 127, NFM, 1),    # D 16
-    (3, "am", 100_000.0, 10_000, 80_000, 65, AM, 0),        # D 32, T = 2 D + 1
+    (3, "am", 100_000.0, 10_000, 80_000, 65, AM, 0),        # D 32
     (4, "ssb", -150_000.0, 3_000, 40_000, 255, USB, 0),     # D 64
     (5, "fm", 700_000.0, 12_500, 256_000, 127, NFM, 0),     # D 10: direct form
     (6, "wfm", -800_000.0, 200_000, 320_000, 601, WFM, 0),  # 601 taps: direct form
-    (7, "tone", 950_000.0, 5_000, 160_000, 3, NONE, 0),     # T < 2 D: direct form
+    (7, "tone", 950_000.0, 5_000, 160_000, 3, NONE, 0),     # D 16 >= T: direct form
     (9, "fm", -250_000.0, 12_500, 64_000, 255, NFM, 0),     # D 40: fold by 8, every 5th
     (10, "am", 450_000.0, 10_000, 32_000, 301, AM, 1),      # D 80: fold by 16, every 5th
 ]

@@ -2,7 +2,7 @@
 streaming_gpu_stub.go:21-26, with eligible signals filtered through shared
 segment transforms) against the oracle runner and the runner's own direct
 form on the same chunk sequence: ragged chunks (below the 511-sample tail,
-empty, a single sample), signals of D 8/16/32/40/64 with 2 D <= T and
+empty, a single sample), signals of D 8/16/32/40/64 with D < T and
 ineligible ones beside them, a signal added mid-stream, a configuration
 change (state reset, direct form until the history refills), a signal that
 leaves and returns.  Samples within 1e-4 of the oracle (the reference's
@@ -28,7 +28,7 @@ def _job(sid, off, bw, rate, taps, hash_salt=0):
 BASE = [  # id, offset, bw, out rate, taps
     (1, 300_000.0, 200_000.0, 320_000, 101),   # D 8
     (2, -400_000.0, 25_000.0, 160_000, 127),   # D 16
-    (3, 100_000.0, 10_000.0, 80_000, 65),      # D 32, T = 2 D + 1
+    (3, 100_000.0, 10_000.0, 80_000, 65),      # D 32
     (4, -150_000.0, 3_000.0, 40_000, 255),     # D 64
     (5, 250_000.0, 12_500.0, 64_000, 255),     # D 40: fold by 8, every 5th
     (6, 700_000.0, 12_500.0, 256_000, 127),    # D 10: direct form
