@@ -56,7 +56,7 @@ register("gpud_serve_wait", I, [P, C.c_int64, C.POINTER(gpud_serve_block_info)])
 register("gpud_serve_channel_output", I, [P, C.c_int64, C.c_int64, C.POINTER(P), C.POINTER(I), C.POINTER(P),
                                           C.POINTER(I), C.POINTER(I)])
 class gpud_tune(C.Structure):
-    _fields_ = [("lds_entries", C.c_int32), ("narrow_outputs", C.c_int32), ("grid", C.c_int32), ("pad", C.c_int32),
+    _fields_ = [("lds_entries", C.c_int32), ("narrow_outputs", C.c_int32), ("grid", C.c_int32), ("fast_grid", C.c_int32),
                 ("kernel_ms", C.c_float), ("pad2", C.c_float)]
 
 
@@ -287,9 +287,15 @@ class Serve:
         check("gpud_serve_get_tune", lib().gpud_serve_get_tune(self._h, C.byref(t)))
         return t
 
-    def SetTune(self, lds_entries: int, narrow_outputs: int, grid: int) -> None:
-        t = gpud_tune(lds_entries, narrow_outputs, grid, 0, 0.0, 0.0)
+    def SetTune(self, lds_entries: int, narrow_outputs: int, grid: int, fast_grid: int = 0) -> None:
+        """fast_grid 0 keeps the current fast-form grid."""
+        t = gpud_tune(lds_entries, narrow_outputs, grid, fast_grid, 0.0, 0.0)
         check("gpud_serve_set_tune", lib().gpud_serve_set_tune(self._h, C.byref(t)))
+
+    def GetTune(self) -> gpud_tune:
+        t = gpud_tune()
+        check("gpud_serve_get_tune", lib().gpud_serve_get_tune(self._h, C.byref(t)))
+        return t
 
     def Autotune(self, dev_ptr: int, n: int, reps: int = 3) -> gpud_tune:
         t = gpud_tune()

@@ -97,9 +97,15 @@ struct ArgusTune {
     int lds_entries;     // complex samples of LDS a dense tile may stage
     int narrow_outputs;  // outputs per narrow item
     int grid;            // persistent workgroups (multiple of ARGUS_XCDS)
+    int fast_grid;       // workgroups of the fast form's per-channel kernel (multiple of ARGUS_XCDS)
 };
 
-static inline ArgusTune argus_default_tune() { return ArgusTune{ARGUS_LDS_ENTRIES, ARGUS_NARROW_OUTPUTS, ARGUS_GRID}; }
+#define ARGUS_FAST_GRID_DEFAULT 2048
+
+static inline ArgusTune argus_default_tune()
+{
+    return ArgusTune{ARGUS_LDS_ENTRIES, ARGUS_NARROW_OUTPUTS, ARGUS_GRID, ARGUS_FAST_GRID_DEFAULT};
+}
 
 #define ARGUS_DENSE_R 4             // outputs per thread in the dense FIR
 

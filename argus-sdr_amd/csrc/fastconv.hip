@@ -376,7 +376,7 @@ void argus_fast_spectrum(const float* h, int T, uint64_t inc, float2* G)
 }
 
 int argus_launch_fast(const ArgusFastHdr* d_hdr, const ArgusExtractHdr* d_xh, int S, float2* u, float2* X,
-                      hipStream_t stream)
+                      int grid, hipStream_t stream)
 {
     if (S <= 0) return 0;
     const int len = (S - 1) * kHop + kN;
@@ -385,7 +385,7 @@ int argus_launch_fast(const ArgusFastHdr* d_hdr, const ArgusExtractHdr* d_xh, in
     ARGUS_CHECK(hipGetLastError());
     int err = argus_launch_fft4096(u, kHop, S, X, stream);
     if (err) return err;
-    argus_fast_chan_kernel<<<ARGUS_FAST_GRID, kChanThreads, 0, stream>>>(d_hdr);
+    argus_fast_chan_kernel<<<grid > 0 ? grid : ARGUS_FAST_GRID_DEFAULT, kChanThreads, 0, stream>>>(d_hdr);
     return (int)hipGetLastError();
 }
 

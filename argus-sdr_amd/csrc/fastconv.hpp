@@ -7,7 +7,6 @@
 #define ARGUS_FAST_N 4096      // segment transform size
 #define ARGUS_FAST_TAIL 511    // input samples kept before each block (taps <= 512)
 #define ARGUS_FAST_HOP 3584    // new samples per segment: <= N - TAIL, a multiple of 64
-#define ARGUS_FAST_GRID 2048   // workgroups of the per-channel kernel (grid-stride)
 
 // One channel on the fast path for one block.
 struct ArgusFastDesc {
@@ -64,7 +63,7 @@ void argus_fast_spectrum(const float* h, int T, uint64_t inc, float2* G);
 // input and its length from *xh; u and X are the buffers d_hdr names (fixed
 // per service), passed again for the transform launch.
 int argus_launch_fast(const ArgusFastHdr* d_hdr, const ArgusExtractHdr* d_xh, int S, float2* u, float2* X,
-                      hipStream_t stream);
+                      int grid, hipStream_t stream);
 // The stream's tail after the block (every block, fast channels or not).
 int argus_launch_fast_tail(const ArgusFastHdr* d_hdr, const ArgusExtractHdr* d_xh, hipStream_t stream);
 // surv.hip: forward 4096-point transforms of n_frames frames at in + f * frame_step.

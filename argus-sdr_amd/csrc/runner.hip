@@ -50,16 +50,17 @@ static int build_poly(const float* base, int n, int decim, float* out)
 extern "C" GPUD_API int gpud_runner_get_tune(gpud_runner* r, gpud_tune* t)
 {
     if (!r || !t) return -1;
-    *t = gpud_tune{r->tune.lds_entries, r->tune.narrow_outputs, r->tune.grid, 0, 0.0f, 0.0f};
+    *t = gpud_tune{r->tune.lds_entries, r->tune.narrow_outputs, r->tune.grid, r->tune.fast_grid, 0.0f, 0.0f};
     return 0;
 }
 
 extern "C" GPUD_API int gpud_runner_set_tune(gpud_runner* r, const gpud_tune* t)
 {
     if (!r || !t || t->lds_entries < 1024 || t->lds_entries > ARGUS_MAX_LDS_ENTRIES || t->narrow_outputs < 1 ||
-        t->narrow_outputs > 1024 || t->grid < ARGUS_XCDS || t->grid > 65536 || t->grid % ARGUS_XCDS != 0)
+        t->narrow_outputs > 1024 || t->grid < ARGUS_XCDS || t->grid > 65536 || t->grid % ARGUS_XCDS != 0 ||
+        t->fast_grid < 0 || t->fast_grid > 65536 || t->fast_grid % ARGUS_XCDS != 0)
         return -1;
-    r->tune = ArgusTune{t->lds_entries, t->narrow_outputs, t->grid};
+    r->tune = ArgusTune{t->lds_entries, t->narrow_outputs, t->grid, t->fast_grid ? t->fast_grid : r->tune.fast_grid};
     return 0;
 }
 
@@ -533,7 +534,7 @@ static int run_batched(gpud_runner* r, const float2* d_in, int n_new, const gpud
     r->stage_recorded[slot] = true;
     if (r->timing) ARGUS_CHECK((hipError_t)argus_timing_begin(r));
     const gpud_runner::FastHdrs* dfh = r->d_fhdr[slot];
-    int err = S > 0 ? argus_launch_fast(&dfh->f, &dfh->x, S, r->d_fu, r->d_fX, r->stream) : 0;
+    int err = S > 0 ? argus_launch_fast(&dfh->f, &dfh->x, S, r->d_fu, r->d_fX, r->tune.fast_grid, r->stream) : 0;
     if (!err)
         err = argus_launch_extract_batch(r->d_chans[slot], r->d_items[slot], n_items, lds, d_in, n_new, r->tune.grid,
                                          r->stream);

@@ -1060,7 +1060,7 @@ static int launch_stage(gpud_serve* s, ServeSlot& sl, int stage, bool wfm)
 {
     const ArgusServeHdr* dh = (const ArgusServeHdr*)(sl.d_tab + sl.off_hdr);
     switch (stage) {
-    case 0: return argus_launch_fast(&dh->f, &dh->x, sl.fast_S, s->d_fu, s->d_fX, s->compute);
+    case 0: return argus_launch_fast(&dh->f, &dh->x, sl.fast_S, s->d_fu, s->d_fX, s->tune.fast_grid, s->compute);
     case 1: return argus_launch_demod_hdr(&dh->de, s->compute);
     case 2: {
         int err = argus_launch_extract_hdr(&dh->x, sl.lds, s->tune.grid, s->compute);
@@ -1099,6 +1099,7 @@ static int launch_block(gpud_serve* s, ServeSlot& sl)
     }
     const char* hdr = sl.d_tab + sl.off_hdr;
     if (!sl.gexec[0] || sl.g_hdr != hdr || sl.g_lds != sl.lds || sl.g_grid != s->tune.grid ||
+        sl.g_fgrid != s->tune.fast_grid ||
         sl.g_wfm != (int)wfm || sl.g_audio != sl.a_mode || sl.g_fastS != sl.fast_S || sl.g_pre != sl.p_mode) {
         for (auto& ge : sl.gexec)
             if (ge) {
@@ -1127,6 +1128,7 @@ static int launch_block(gpud_serve* s, ServeSlot& sl)
         sl.g_hdr = hdr;
         sl.g_lds = sl.lds;
         sl.g_grid = s->tune.grid;
+        sl.g_fgrid = s->tune.fast_grid;
         sl.g_wfm = (int)wfm;
         sl.g_audio = sl.a_mode;
         sl.g_fastS = sl.fast_S;
@@ -1701,20 +1703,21 @@ GPUD_API int gpud_serve_set_graph(gpud_serve* s, int enable)
 GPUD_API int gpud_serve_get_tune(gpud_serve* s, gpud_tune* t)
 {
     if (!s || !t) return -1;
-    *t = gpud_tune{s->tune.lds_entries, s->tune.narrow_outputs, s->tune.grid, 0, 0.0f, 0.0f};
+    *t = gpud_tune{s->tune.lds_entries, s->tune.narrow_outputs, s->tune.grid, s->tune.fast_grid, 0.0f, 0.0f};
     return 0;
 }
 
 static bool valid_tune(const gpud_tune& t)
 {
     return t.lds_entries >= 1024 && t.lds_entries <= ARGUS_MAX_LDS_ENTRIES && t.narrow_outputs >= 1 &&
-           t.narrow_outputs <= 1024 && t.grid >= ARGUS_XCDS && t.grid <= 65536 && t.grid % ARGUS_XCDS == 0;
+           t.narrow_outputs <= 1024 && t.grid >= ARGUS_XCDS && t.grid <= 65536 && t.grid % ARGUS_XCDS == 0 &&
+           t.fast_grid >= 0 && t.fast_grid <= 65536 && t.fast_grid % ARGUS_XCDS == 0;
 }
 
 GPUD_API int gpud_serve_set_tune(gpud_serve* s, const gpud_tune* t)
 {
     if (!s || !t || !valid_tune(*t)) return -1;
-    s->tune = ArgusTune{t->lds_entries, t->narrow_outputs, t->grid};
+    s->tune = ArgusTune{t->lds_entries, t->narrow_outputs, t->grid, t->fast_grid ? t->fast_grid : s->tune.fast_grid};
     s->lds_hw = 0;
     s->epoch++;
     return 0;
@@ -1745,7 +1748,7 @@ static int time_extract(gpud_serve* s, ServeSlot& scratch, const float2* iq_dev,
     // per-channel pass, then the direct kernel), without the tail update that
     // would advance the stream's state
     auto stage = [&]() -> int {
-        int e = argus_launch_fast(&dh->f, &dh->x, scratch.fast_S, s->d_fu, s->d_fX, s->compute);
+        int e = argus_launch_fast(&dh->f, &dh->x, scratch.fast_S, s->d_fu, s->d_fX, s->tune.fast_grid, s->compute);
         return e ? e : argus_launch_extract_hdr(&dh->x, scratch.lds, s->tune.grid, s->compute);
     };
     rc = stage();  // warm-up
@@ -1806,16 +1809,18 @@ GPUD_API int gpud_serve_autotune(gpud_serve* s, const gpud_float2* iq_dev, int n
     float cur_ms = 0.0f;
     if (!err) err = eval(cur, &cur_ms);
     // coordinate descent, twice over (grid and LDS budget interact: smaller
-    // tiles mean more workgroups resident per CU)
+    // tiles mean more workgroups resident per CU); the fast form's grid last
     const int grids[] = {1024, 2048, 4096, 6144, 8192, 12288};
     const int ldss[] = {2048, 3072, 4096, 6144, 8192, 12288};
     const int narrows[] = {16, 32, 64};
-    for (int pass = 0; pass < 6 && !err; pass++) {
-        const int* vals = pass % 3 == 0 ? grids : pass % 3 == 1 ? ldss : narrows;
-        const int nv = pass % 3 == 2 ? 3 : 6;
+    const int fgrids[] = {512, 1024, 2048, 4096};
+    for (int pass = 0; pass < 8 && !err; pass++) {
+        const int axis = pass % 4;
+        const int* vals = axis == 0 ? grids : axis == 1 ? ldss : axis == 2 ? narrows : fgrids;
+        const int nv = axis == 0 || axis == 1 ? 6 : axis == 2 ? 3 : 4;
         for (int i = 0; i < nv && !err; i++) {
             ArgusTune t = cur;
-            int& field = pass % 3 == 0 ? t.grid : pass % 3 == 1 ? t.lds_entries : t.narrow_outputs;
+            int& field = axis == 0 ? t.grid : axis == 1 ? t.lds_entries : axis == 2 ? t.narrow_outputs : t.fast_grid;
             if (field == vals[i]) continue;
             field = vals[i];
             float ms = 0.0f;
@@ -1834,7 +1839,7 @@ GPUD_API int gpud_serve_autotune(gpud_serve* s, const gpud_float2* iq_dev, int n
     if (e0) hipEventDestroy(e0);
     if (e1) hipEventDestroy(e1);
     if (err) return err;
-    if (best) *best = gpud_tune{cur.lds_entries, cur.narrow_outputs, cur.grid, 0, cur_ms, 0.0f};
+    if (best) *best = gpud_tune{cur.lds_entries, cur.narrow_outputs, cur.grid, cur.fast_grid, cur_ms, 0.0f};
     return 0;
 }
 

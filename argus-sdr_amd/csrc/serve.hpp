@@ -138,6 +138,7 @@ struct ServeSlot {
     const char* g_hdr = nullptr;
     size_t g_lds = 0;
     int g_grid = 0;
+    int g_fgrid = 0;
     int g_wfm = -1;
     // WFM stereo stage
     float* d_mono = nullptr;

@@ -115,7 +115,7 @@ def percentile(v, q):
 def open_service(chans, d_block, S, tune=None, budget=256 << 30, audio=0):
     """A service with the channels open, one block run so every channel has
     its history (and the eligible ones take the fast form), then autotuned on
-    that steady state -- or set to `tune` (lds_entries, narrow_outputs, grid)."""
+    that steady state -- or set to `tune` (lds_entries, narrow_outputs, grid[, fast_grid])."""
     s = S.Serve(FS, BLOCK, budget)
     for (cid, off, bw, rate, taps, dm) in chans:
         assert s.Open(S.ChannelSpec(cid, off, bw, rate, taps, dm, 0))
@@ -125,10 +125,11 @@ def open_service(chans, d_block, S, tune=None, budget=256 << 30, audio=0):
     s.Wait(s.SubmitDevice(d_block, BLOCK))
     if tune:
         s.SetTune(*tune)
-        info = {"lds_entries": tune[0], "narrow_outputs": tune[1], "grid": tune[2], "autotuned": False}
+        info = {"lds_entries": tune[0], "narrow_outputs": tune[1], "grid": tune[2],
+                "fast_grid": tune[3] if len(tune) > 3 else s.GetTune().fast_grid, "autotuned": False}
     else:
         t = s.Autotune(d_block, BLOCK, reps=2)
-        info = {"lds_entries": t.lds_entries, "narrow_outputs": t.narrow_outputs, "grid": t.grid,
+        info = {"lds_entries": t.lds_entries, "narrow_outputs": t.narrow_outputs, "grid": t.grid, "fast_grid": t.fast_grid,
                 "extract_ms_at_tune": round(t.kernel_ms, 4), "autotuned": True}
     return s, info
 
@@ -252,7 +253,7 @@ def bench_capture_replay(a, chans, blocks, lib, S):
         for (cid, off, bw, rate, taps, dm) in chans:
             assert s.Open(S.ChannelSpec(cid, off, bw, rate, taps, dm, 0))
         if a.tune_fixed or isinstance(a.tune, dict):
-            t = a.tune_fixed or (a.tune["lds_entries"], a.tune["narrow_outputs"], a.tune["grid"])
+            t = a.tune_fixed or (a.tune["lds_entries"], a.tune["narrow_outputs"], a.tune["grid"], a.tune["fast_grid"])
             s.SetTune(*t)
         s.SetDownload(False)
         src = IG.IQFile(path)
@@ -411,7 +412,7 @@ def bench_extract_kernel(a, chans, blocks, lib, G):
     from argus.serve import gpud_tune
     jobs = [G.StreamingExtractJob(c[0], c[1], c[2], c[3], c[4], 0x9E3779B97F4A7C15 ^ c[0]) for c in chans]
     r = G.NewBatchRunner(BLOCK, FS)
-    t = gpud_tune(a.tune["lds_entries"], a.tune["narrow_outputs"], a.tune["grid"], 0, 0.0, 0.0)
+    t = gpud_tune(a.tune["lds_entries"], a.tune["narrow_outputs"], a.tune["grid"], a.tune["fast_grid"], 0.0, 0.0)
     check("gpud_runner_set_tune", lib.gpud_runner_set_tune(r._h, C.byref(t)))
     p = C.c_void_p()
     check("gpud_malloc", lib.gpud_malloc(C.byref(p), blocks[0].nbytes))
@@ -901,7 +902,7 @@ def main() -> int:
     ap.add_argument("--audio-blocks", type=int, default=100,
                     help="blocks per point of the audio-stage figure (0: skip it)")
     ap.add_argument("--tune", default="",
-                    help="LDS_ENTRIES,NARROW_OUTPUTS,GRID: use this tile/launch shape instead of autotuning")
+                    help="LDS_ENTRIES,NARROW_OUTPUTS,GRID[,FAST_GRID]: use this tile/launch shape instead of autotuning")
     a = ap.parse_args()
     a.tune_fixed = tuple(int(x) for x in a.tune.split(",")) if a.tune else None
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -973,7 +974,7 @@ def main() -> int:
           flush=True)
     s_t, tune0 = open_service(chans, d0.value, S, a.tune_fixed)  # the tune the steps run with
     s_t.Close()
-    tune_t = (tune0["lds_entries"], tune0["narrow_outputs"], tune0["grid"])
+    tune_t = (tune0["lds_entries"], tune0["narrow_outputs"], tune0["grid"], tune0["fast_grid"])
     ms_per_step, lat, kern, xk, out_bytes = timed_steps(a, chans, d0.value, blocks, S, IG, S.DL_DEMOD, tune_t)
 
     extras = {}

@@ -155,7 +155,8 @@ typedef struct gpud_tune {
     int32_t lds_entries;     /* complex samples of LDS one dense tile may stage (1024..16384; default 6144) */
     int32_t narrow_outputs;  /* outputs per narrow work item (default 32) */
     int32_t grid;            /* persistent workgroups, a multiple of 8 (default 2048) */
-    int32_t pad;
+    int32_t fast_grid;       /* workgroups of the fast form's per-channel kernel, a multiple of 8
+                              * (default 2048; 0 in a set call keeps the current value) */
     float   kernel_ms;       /* autotune: best launch time found */
     float   pad2;
 } gpud_tune;
