@@ -367,17 +367,24 @@ __device__ __forceinline__ void argus_narrow_group(const ArgusChanDesc& ch, cons
 }
 
 // Narrow group with modulated taps: four outputs per wavefront, one per
-// 16-lane row; lane r of a row takes taps k = r + 16 i.  Waves whose padded
-// windows (T rounded up to 64, zero taps beyond T) lie wholly in the new
-// input run a branch-free loop with four loads in flight per lane; otherwise
-// each tap checks its sample, and samples before the chunk (the history,
-// stored already shifted) are filtered with the real taps and added after
-// the rotation.  The row sum lands in every lane of the row; lane 15 writes.
+// 16-lane row.  Waves whose padded windows (T rounded up to 128, zero taps
+// beyond T) lie wholly in the new input run a branch-free loop in which lane
+// r of a row takes the tap pairs (k, k + 1), k = 2 r + 32 i: each load
+// brings two consecutive samples (16 bytes), which halves the vector memory
+// instructions per tap -- the address unit is what bounds these items.
+// Otherwise each tap (k = r + 16 i) checks its sample, and samples before
+// the chunk (the history, stored already shifted) are filtered with the real
+// taps and added after the rotation.  The row sum lands in every lane of the
+// row; lane 15 writes.
+struct __attribute__((aligned(8))) ArgusPair {
+    float2 a, b;  // two consecutive complex samples (taps)
+};
+
 __device__ __forceinline__ void argus_narrow_mod(const ArgusChanDesc& ch, const float2* __restrict__ in, int m0,
                                                  float2* lds)
 {
     const int T = ch.T, D = ch.D, L = ch.L;
-    const int TP = (T + 63) & ~63;
+    const int TP = (T + 127) & ~127;
     float2* gs = lds;
     float* hs = (float*)(lds + TP);
     const bool hist = ch.e0 + m0 * D - (T - 1) < 0;  // a window of this item reaches into the history
@@ -389,24 +396,28 @@ __device__ __forceinline__ void argus_narrow_mod(const ArgusChanDesc& ch, const 
     const int row = lane >> 4, rl = lane & 15;
     const int nw = blockDim.x >> 6;
     const int mend = min(m0 + ch.M, ch.n_out);
+#define ARGUS_CMAC(G, X)                                                        \
+    ar = fmaf(G.x, X.x, ar); ar = fmaf(-G.y, X.y, ar);                          \
+    ai = fmaf(G.x, X.y, ai); ai = fmaf(G.y, X.x, ai);
     for (int mb = m0 + 4 * (int)(threadIdx.x >> 6); mb < mend; mb += 4 * nw) {
         const int m = mb + row;
         const int e = ch.e0 + m * D;
         float ar = 0.0f, ai = 0.0f, br = 0.0f, bi = 0.0f;
         if (ch.e0 + mb * D - (TP - 1) >= 0) {  // same for the whole wave: its lowest output decides
             if (m < mend) {
-                const float2* src = in + (e - rl);
-                const float2* gp = gs + rl;
-                for (int k = 0; k < TP; k += 64) {
-                    const float2 x0 = src[-k], x1 = src[-k - 16], x2 = src[-k - 32], x3 = src[-k - 48];
-                    const float2 g0 = gp[k], g1 = gp[k + 16], g2 = gp[k + 32], g3 = gp[k + 48];
-#define ARGUS_CMAC(G, X)                                                        \
-    ar = fmaf(G.x, X.x, ar); ar = fmaf(-G.y, X.y, ar);                          \
-    ai = fmaf(G.x, X.y, ai); ai = fmaf(G.y, X.x, ai);
-                    ARGUS_CMAC(g0, x0)
-                    ARGUS_CMAC(g1, x1)
-                    ARGUS_CMAC(g2, x2)
-                    ARGUS_CMAC(g3, x3)
+                // pair u of this lane: taps k, k + 1 with k = k0 + 2 rl + 32 u,
+                // samples x[e - k - 1], x[e - k] in one 16-byte load
+                const ArgusPair* src = (const ArgusPair*)(in + (e - 2 * rl - 1));
+                const ArgusPair* gp = (const ArgusPair*)(gs + 2 * rl);
+                for (int k = 0; k < TP; k += 128) {
+                    const int q = k >> 1;  // in pairs
+                    const ArgusPair x0 = *(src - q), x1 = *(src - q - 16), x2 = *(src - q - 32),
+                                    x3 = *(src - q - 48);
+                    const ArgusPair g0 = gp[q], g1 = gp[q + 16], g2 = gp[q + 32], g3 = gp[q + 48];
+                    ARGUS_CMAC(g0.a, x0.b) ARGUS_CMAC(g0.b, x0.a)
+                    ARGUS_CMAC(g1.a, x1.b) ARGUS_CMAC(g1.b, x1.a)
+                    ARGUS_CMAC(g2.a, x2.b) ARGUS_CMAC(g2.b, x2.a)
+                    ARGUS_CMAC(g3.a, x3.b) ARGUS_CMAC(g3.b, x3.a)
                 }
             }
         } else if (m < mend) {
@@ -422,7 +433,6 @@ __device__ __forceinline__ void argus_narrow_mod(const ArgusChanDesc& ch, const 
                     bi = fmaf(hs[k], v.y, bi);
                 }
             }
-#undef ARGUS_CMAC
         }
         ar = argus_group_sum<16>(ar);
         ai = argus_group_sum<16>(ai);
@@ -437,6 +447,7 @@ __device__ __forceinline__ void argus_narrow_mod(const ArgusChanDesc& ch, const 
             ch.out[m] = make_float2(y.x + br, y.y + bi);
         }
     }
+#undef ARGUS_CMAC
     __syncthreads();
 }
 
@@ -520,7 +531,7 @@ __global__ __launch_bounds__(ARGUS_EXTRACT_THREADS) void argus_extract_one_kerne
 size_t argus_item_lds_bytes(const ArgusChanDesc& ch)
 {
     if (!ch.narrow) return argus_dense_lds_bytes(ch);
-    return ch.ctaps ? (size_t)((ch.T + 63) & ~63) * sizeof(float2) + (size_t)ch.T * sizeof(float)
+    return ch.ctaps ? (size_t)((ch.T + 127) & ~127) * sizeof(float2) + (size_t)ch.T * sizeof(float)
                     : (size_t)ch.T * sizeof(float);
 }
 
@@ -565,9 +576,9 @@ int argus_extract_allow_lds(size_t bytes) { return bytes > 64 * 1024 ? argus_all
 size_t argus_tune_max_lds(const ArgusTune& tune, int max_taps)
 {
     // dense: P*PS <= lds_entries words of samples plus P*L <= T + P - 1 < 2*T taps;
-    // narrow: T (+ 63 padding) complex and T real taps
+    // narrow: T (+ 127 padding) complex and T real taps
     const size_t dense = (size_t)tune.lds_entries * sizeof(float2) + (size_t)2 * max_taps * sizeof(float);
-    const size_t narrow = (size_t)(max_taps + 63) * sizeof(float2) + (size_t)max_taps * sizeof(float);
+    const size_t narrow = (size_t)(max_taps + 127) * sizeof(float2) + (size_t)max_taps * sizeof(float);
     return std::max(dense, narrow);
 }
 
