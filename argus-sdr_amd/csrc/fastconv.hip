@@ -92,7 +92,7 @@ constexpr int kChanThreads = 64 * kWaves;
 constexpr int kSegWave = 7;                    // segments per wavefront per item
 constexpr int kSegItem = kWaves * kSegWave;    // 56: two items cover a 20 ms block at 20 Msps
 constexpr int kXStride = 72;                   // first transpose: 64 values + 8 pad per row (conflict-free reads)
-constexpr int kXsFloats = 8 * kXStride > 576 ? 8 * kXStride : 576;  // also the padded 8 x 9 x 8 image
+constexpr int kXsFloats = 2 * 512;  // the 512 folded bins; also the transposes (8 x 72, 8 x 9 x 8 floats)
 
 // The wave's LDS operations complete in issue order, so data a wave passes
 // between its own lanes through LDS needs no hardware barrier: only the
@@ -192,6 +192,36 @@ __device__ __forceinline__ void fast_fold(const float2* __restrict__ xseg, const
     for (int c = 0; c < PER; c++) f[c] = make_float2(ar[c], ai[c]);
 }
 
+// The same fold with two adjacent bins per lane, j = 2 t + 128 c and j + 1
+// (c < J / 128), both read with one 16-byte load: half the vector memory
+// instructions, which bound this kernel.  f[c] holds F[j] in .xy and
+// F[j + 1] in .zw.
+template <int DP>
+__device__ __forceinline__ void fast_fold_pairs(const float2* __restrict__ xseg, const float2* gs, int t,
+                                                float4 (&f)[ARGUS_FAST_N / DP / 128])
+{
+    constexpr int J = ARGUS_FAST_N / DP, P = J / 128;
+    float ar[P], ai[P], br[P], bi[P];
+#pragma unroll
+    for (int c = 0; c < P; c++) ar[c] = ai[c] = br[c] = bi[c] = 0.0f;
+#pragma unroll 4
+    for (int i = 0; i < DP; i++) {
+        float4 xv[P];
+#pragma unroll
+        for (int c = 0; c < P; c++) xv[c] = *(const float4*)(xseg + i * J + 2 * t + 128 * c);
+#pragma unroll
+        for (int c = 0; c < P; c++) {
+            const float4 gv = *(const float4*)(gs + i * J + 2 * t + 128 * c);
+            ar[c] += xv[c].x * gv.x - xv[c].y * gv.y;
+            ai[c] += xv[c].x * gv.y + xv[c].y * gv.x;
+            br[c] += xv[c].z * gv.z - xv[c].w * gv.w;
+            bi[c] += xv[c].z * gv.w + xv[c].w * gv.z;
+        }
+    }
+#pragma unroll
+    for (int c = 0; c < P; c++) f[c] = make_float4(ar[c], ai[c], br[c], bi[c]);
+}
+
 // Per channel, the constants of the output stage.
 struct FastOut {
     int E;          // D / DP: emission every E-th fold position
@@ -210,17 +240,21 @@ __device__ __forceinline__ void fast_segment(const ArgusFastDesc& d, const FastO
     // t only) from being hoisted out of the segment loop into registers
     asm volatile("" : "+v"(t));
     float2 v[8];
-    if constexpr (Z == 1) {
-        fast_fold<DP>(X + (size_t)s * kN, gs, t, v);
+    float2* fs = (float2*)xs;  // the J folded bins pass through the wave's scratch
+    if constexpr (PER >= 2) {
+        // folded in bin pairs, then laid out as the transform takes them
+        float4 f[PER / 2];
+        fast_fold_pairs<DP>(X + (size_t)s * kN, gs, t, f);
 #pragma unroll
-        for (int c = 0; c < 8; c++) v[c].y = -v[c].y;  // conjugate: the forward transform inverts
+        for (int c = 0; c < PER / 2; c++) *(float4*)(fs + 2 * t + 128 * c) = f[c];
     } else {
         float2 f[PER];
         fast_fold<DP>(X + (size_t)s * kN, gs, t, f);
-        float2* fs = (float2*)xs;  // J <= 256 bins fit the wave's transform scratch
 #pragma unroll
         for (int c = 0; c < PER; c++) fs[t + 64 * c] = f[c];
-        wave_sync();
+    }
+    wave_sync();
+    {
 #pragma unroll
         for (int c = 0; c < 8; c++) {
             const int nn = t + 64 * c;
@@ -280,10 +314,10 @@ __device__ __forceinline__ void fast_block(const ArgusFastDesc& d, const float2*
 
 __global__ __launch_bounds__(kChanThreads) void argus_fast_chan_kernel(const ArgusFastHdr* __restrict__ hdr)
 {
-    __shared__ float2 gs[kN];
-    __shared__ float xs[kWaves][kXsFloats];
-    static_assert(kXsFloats >= 2 * 256, "folded bins of D >= 16 fit the transform scratch");
-    static_assert(kXsFloats >= 9 * 8 * 8, "the padded second transpose fits the scratch");
+    __shared__ __attribute__((aligned(16))) float2 gs[kN];
+    __shared__ __attribute__((aligned(16))) float xs[kWaves][kXsFloats];
+    static_assert(kXsFloats >= 2 * kJ, "the folded bins fit the transform scratch");
+    static_assert(kXsFloats >= 9 * 8 * 8 && kXsFloats >= 8 * kXStride, "the transposes fit the scratch");
     const int w = threadIdx.x >> 6, t = threadIdx.x & 63;
     const int S = hdr->S, n_fast = hdr->n_fast;
     const int blocks = (S + kSegItem - 1) / kSegItem;
