@@ -376,9 +376,18 @@ __device__ __forceinline__ void argus_narrow_group(const ArgusChanDesc& ch, cons
 // the chunk (the history, stored already shifted) are filtered with the real
 // taps and added after the rotation.  The row sum lands in every lane of the
 // row; lane 15 writes.
-struct __attribute__((aligned(8))) ArgusPair {
+struct ArgusPair {
     float2 a, b;  // two consecutive complex samples (taps)
 };
+// The same 16 bytes as one vector, aligned as a complex sample (8 bytes).
+typedef float ArgusPairV4 __attribute__((ext_vector_type(4)));
+typedef ArgusPairV4 ArgusPairV __attribute__((aligned(8)));
+
+template <typename V>
+__device__ __forceinline__ ArgusPair argus_pair(const V& v)
+{
+    return ArgusPair{make_float2(v.x, v.y), make_float2(v.z, v.w)};
+}
 
 __device__ __forceinline__ void argus_narrow_mod(const ArgusChanDesc& ch, const float2* __restrict__ in, int m0,
                                                  float2* lds)
@@ -406,14 +415,22 @@ __device__ __forceinline__ void argus_narrow_mod(const ArgusChanDesc& ch, const 
         if (ch.e0 + mb * D - (TP - 1) >= 0) {  // same for the whole wave: its lowest output decides
             if (m < mend) {
                 // pair u of this lane: taps k, k + 1 with k = k0 + 2 rl + 32 u,
-                // samples x[e - k - 1], x[e - k] in one 16-byte load
-                const ArgusPair* src = (const ArgusPair*)(in + (e - 2 * rl - 1));
-                const ArgusPair* gp = (const ArgusPair*)(gs + 2 * rl);
+                // samples x[e - k - 1], x[e - k] in one 16-byte load (the
+                // lowest pair is the base, so the others sit at immediate
+                // offsets above it); tap pairs as 16-byte LDS reads
+                // (global address space: loads with immediate offsets)
+                typedef __attribute__((address_space(1))) const ArgusPairV GlobalPair;
+                GlobalPair* src = (GlobalPair*)(in + (e - 2 * rl - 1)) - 48;
+                const float4* gp = (const float4*)(gs + 2 * rl);
                 for (int k = 0; k < TP; k += 128) {
                     const int q = k >> 1;  // in pairs
-                    const ArgusPair x0 = *(src - q), x1 = *(src - q - 16), x2 = *(src - q - 32),
-                                    x3 = *(src - q - 48);
-                    const ArgusPair g0 = gp[q], g1 = gp[q + 16], g2 = gp[q + 32], g3 = gp[q + 48];
+                    GlobalPair* b = src - q;
+                    const ArgusPairV v3 = b[0], v2 = b[16], v1 = b[32], v0 = b[48];
+                    const ArgusPair x0 = argus_pair(v0), x1 = argus_pair(v1), x2 = argus_pair(v2),
+                                    x3 = argus_pair(v3);
+                    const float4 f0 = gp[q], f1 = gp[q + 16], f2 = gp[q + 32], f3 = gp[q + 48];
+                    const ArgusPair g0 = argus_pair(f0), g1 = argus_pair(f1), g2 = argus_pair(f2),
+                                    g3 = argus_pair(f3);
                     ARGUS_CMAC(g0.a, x0.b) ARGUS_CMAC(g0.b, x0.a)
                     ARGUS_CMAC(g1.a, x1.b) ARGUS_CMAC(g1.b, x1.a)
                     ARGUS_CMAC(g2.a, x2.b) ARGUS_CMAC(g2.b, x2.a)
@@ -503,7 +520,7 @@ __global__ __launch_bounds__(ARGUS_EXTRACT_THREADS) void argus_extract_batch_ker
     const ArgusChanDesc* __restrict__ chans, const ArgusWorkItem* __restrict__ items, int n_items,
     const float2* __restrict__ in, int n_new)
 {
-    extern __shared__ float2 lds[];
+    extern __shared__ __attribute__((aligned(16))) float2 lds[];
     argus_extract_items(chans, items, n_items, in, n_new, lds);
 }
 
@@ -511,7 +528,7 @@ __global__ __launch_bounds__(ARGUS_EXTRACT_THREADS) void argus_extract_batch_ker
 // graph replays it unchanged while the block's tables change.
 __global__ __launch_bounds__(ARGUS_EXTRACT_THREADS) void argus_extract_hdr_kernel(const ArgusExtractHdr* __restrict__ hdr)
 {
-    extern __shared__ float2 lds[];
+    extern __shared__ __attribute__((aligned(16))) float2 lds[];
     const ArgusExtractHdr h = *hdr;
     argus_extract_items(h.chans, h.items, h.n_items, h.in, h.n_new, lds);
 }
@@ -522,7 +539,7 @@ __global__ __launch_bounds__(ARGUS_EXTRACT_THREADS) void argus_extract_one_kerne
                                                                                    const float2* __restrict__ in,
                                                                                    int n_new, int n_items)
 {
-    extern __shared__ float2 lds[];
+    extern __shared__ __attribute__((aligned(16))) float2 lds[];
     const int kind = (int)blockIdx.x < n_items ? (ch.narrow ? ARGUS_ITEM_NARROW : ARGUS_ITEM_DENSE)
                                                : ARGUS_ITEM_HISTORY;
     argus_run_item(ch, kind, blockIdx.x * ch.M, in, n_new, lds);
