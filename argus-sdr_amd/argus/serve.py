@@ -74,6 +74,7 @@ register("gpud_serve_channel_audio", I, [P, C.c_int64, C.c_int64, C.POINTER(P), 
                                          C.POINTER(I)])
 register("gpud_serve_set_rds", I, [P, C.c_int64, I])
 register("gpud_serve_set_fast", I, [P, I])
+register("gpud_serve_set_demod_fusion", I, [P, I])
 register("gpud_fast_spectrum", I, [P, I, C.c_double, P])
 
 
@@ -278,6 +279,12 @@ class Serve:
     def SetFast(self, enable: bool) -> None:
         """The fast-convolution form for eligible channels (on by default)."""
         check("gpud_serve_set_fast", lib().gpud_serve_set_fast(self._h, 1 if enable else 0))
+
+    def SetDemodFusion(self, enable: bool) -> None:
+        """Demodulation in the fast kernel's epilogue for the fast-form
+        channels that demodulate their baseband directly (on by default);
+        off, the separate demodulation pass gives the same outputs."""
+        check("gpud_serve_set_demod_fusion", lib().gpud_serve_set_demod_fusion(self._h, 1 if enable else 0))
 
     def SetGraph(self, enable: bool) -> None:
         check("gpud_serve_set_graph", lib().gpud_serve_set_graph(self._h, 1 if enable else 0))

@@ -7,13 +7,29 @@
 //   USB/LSB: internal/demod/ssb.go:19-39 Re(x e^{j phi}), phi pre-incremented
 //         by 2 pi (+-700 Hz)/fs, carried across blocks;
 //   CW  : internal/demod/cw.go:11-26 Re(x) cos(phi), +700 Hz.
+//
+// A channel whose baseband the fast-convolution kernel also demodulated
+// (FM, SSB, CW) needs here only its FM seams: the first output of each
+// 3584-sample segment, whose predecessor belongs to another wavefront's
+// segment (or, at output 0, is the carried sample), and the new carry.
 #include "demod.hpp"
+#include "fastconv.hpp"
 
 namespace {
 
-__device__ __forceinline__ float fm_step(float2 p, float2 c)
+// The FM outputs at the segment seams of a channel the fast kernel
+// demodulated: output m_s = ceil((s hop - e0) / D), the first of segment s.
+__device__ void fm_seams(const ArgusDemodDesc& d, int e0, int D)
 {
-    return atan2f(p.x * c.y - p.y * c.x, p.x * c.x + p.y * c.y);
+    for (int sg = threadIdx.x;; sg += blockDim.x) {
+        const int m = max(0, (sg * ARGUS_FAST_HOP - e0 + D - 1) / D);
+        if (m >= d.n) break;
+        if (m > 0 || d.have_carry) {
+            const float2 prev = m > 0 ? d.iq[m - 1] : *d.carry_rd;
+            d.audio[m - d.skip] = argus_fm_step(prev, d.iq[m]);
+        }
+    }
+    if (threadIdx.x == 0 && d.n > 0) *d.carry_wr = d.iq[d.n - 1];
 }
 
 __device__ void am_block(const ArgusDemodDesc& d)
@@ -48,6 +64,10 @@ __device__ __forceinline__ void argus_demod_items(const ArgusDemodDesc* __restri
     for (int w = blockIdx.x; w < n_items; w += gridDim.x) {
         const ArgusDemodItem it = items[w];
         const ArgusDemodDesc d = descs[it.chan];
+        if (it.kind == ARGUS_DEMOD_ITEM_SEAMS) {
+            fm_seams(d, it.start, it.count);
+            continue;
+        }
         const int end = it.start + it.count;
         switch (d.mode) {
         case ARGUS_DEMOD_NFM:
@@ -55,7 +75,7 @@ __device__ __forceinline__ void argus_demod_items(const ArgusDemodDesc* __restri
             for (int i = it.start + threadIdx.x; i < end; i += blockDim.x) {
                 if (i == 0 && !d.have_carry) continue;
                 const float2 prev = i > 0 ? d.iq[i - 1] : *d.carry_rd;
-                d.audio[i - d.skip] = fm_step(prev, d.iq[i]);
+                d.audio[i - d.skip] = argus_fm_step(prev, d.iq[i]);
                 if (i == d.n - 1) *d.carry_wr = d.iq[i];
             }
             break;
@@ -67,8 +87,7 @@ __device__ __forceinline__ void argus_demod_items(const ArgusDemodDesc* __restri
             for (int i = it.start + threadIdx.x; i < end; i += blockDim.x) {
                 float s, c;
                 argus_nco(d.bfo0 + d.bfo_inc * (uint64_t)(uint32_t)i, s, c);
-                const float2 v = d.iq[i];
-                d.audio[i] = v.x * c - v.y * s;
+                d.audio[i] = argus_ssb_out(d.iq[i], s, c);
             }
             break;
         case ARGUS_DEMOD_CW:

@@ -27,12 +27,37 @@ struct ArgusDemodDesc {
     int skip;             // outputs not produced at the head (FM without carry: 1)
 };
 
+// An item: outputs [start, start + count) of channel chan, or (kind
+// ARGUS_DEMOD_ITEM_SEAMS) only the FM outputs the fast-convolution kernel
+// leaves to this pass when it demodulates in its epilogue: the first output
+// of each segment (whose predecessor another wavefront produced), with the
+// channel's output grid e0 = start, decimation D = count, and the FM carry.
 struct ArgusDemodItem {
     int chan;
     int start;
     int count;
-    int pad;
+    int kind;
 };
+
+enum {
+    ARGUS_DEMOD_ITEM_OUT = 0,
+    ARGUS_DEMOD_ITEM_SEAMS = 1,
+};
+
+// Per-output arithmetic of the demodulators, shared by the demodulation pass
+// and the fast-convolution kernel's epilogue: separate roundings (no
+// contraction), so both give the same bits for the same baseband.
+__device__ __forceinline__ float argus_fm_step(float2 p, float2 c)
+{
+#pragma clang fp contract(off)
+    return atan2f(p.x * c.y - p.y * c.x, p.x * c.x + p.y * c.y);
+}
+
+__device__ __forceinline__ float argus_ssb_out(float2 v, float s, float c)
+{
+#pragma clang fp contract(off)
+    return v.x * c - v.y * s;
+}
 
 #define ARGUS_DEMOD_TILE 4096
 

@@ -222,12 +222,64 @@ __device__ __forceinline__ void fast_fold_pairs(const float2* __restrict__ xseg,
     for (int c = 0; c < P; c++) f[c] = make_float4(ar[c], ai[c], br[c], bi[c]);
 }
 
+// The segment's outputs m_lo + i, i < cnt, from os[i]: baseband, and the
+// demodulated value when the channel demodulates here.  FM takes its
+// predecessor from the same segment, so a segment's first output (i = 0) is
+// left to the demodulation pass.
+__device__ __forceinline__ void fast_emit(const ArgusFastDesc& d, const float2* os, int m_lo, int cnt, int t)
+{
+    for (int i = t; i < cnt; i += 64) {
+        const int m = m_lo + i;
+        const float2 y = os[i];
+        d.out[m] = y;
+        switch (d.demod) {  // uniform: one channel per workgroup
+        case ARGUS_DEMOD_NFM:
+        case ARGUS_DEMOD_WFM:
+            if (i > 0) d.audio[m - d.skip] = argus_fm_step(os[i - 1], y);
+            break;
+        case ARGUS_DEMOD_USB:
+        case ARGUS_DEMOD_LSB: {
+            float sn, cn;
+            argus_nco(d.bfo0 + d.bfo_inc * (uint64_t)(uint32_t)m, sn, cn);
+            d.audio[m] = argus_ssb_out(y, sn, cn);
+            break;
+        }
+        case ARGUS_DEMOD_CW: {
+            float sn, cn;
+            argus_nco(d.bfo0 + d.bfo_inc * (uint64_t)(uint32_t)m, sn, cn);
+            d.audio[m] = y.x * cn;
+            break;
+        }
+        default:
+            break;
+        }
+    }
+}
+
 // Per channel, the constants of the output stage.
 struct FastOut {
     int E;          // D / DP: emission every E-th fold position
     float invE;     // 1 / E (rounded up: floor by multiply, then corrected)
+    float invD;     // 1 / D, the same way
     float2 step;    // rotation between a lane's consecutive values: e^{j 2pi inc 64 DP}
 };
+
+// 1 / d rounded up (d >= 1): for 0 <= a < 2^22, (int)(a * recip) is
+// floor(a / d) or one off, which fast_div corrects.
+__device__ __forceinline__ float fast_recip(int d)
+{
+    return __builtin_bit_cast(float, __builtin_bit_cast(uint32_t, 1.0f / (float)d) + 1u);
+}
+
+// floor(a / d) for 0 <= a < 2^22 without an integer division.
+__device__ __forceinline__ int fast_div(int a, int d, float recip)
+{
+    int q = (int)((float)a * recip);
+    const int r = a - q * d;
+    if (r < 0) q--;
+    else if (r >= d) q++;
+    return q;
+}
 
 // One segment of one channel on one wavefront: fold, inverse transform
 // (the forward transform of the conjugate), outputs.
@@ -268,9 +320,17 @@ __device__ __forceinline__ void fast_segment(const ArgusFastDesc& d, const FastO
     // position q is block sample e = s hop + q DP + r - tail, and
     // e - e0 = DP (q + cs) with cs uniform over the segment; it is an
     // output when q + cs >= 0 is a multiple of E, output (q + cs) / E.
+    // The segment's outputs are m in [m_lo, m_hi) (block samples e0 + m D
+    // in [s hop, (s + 1) hop)), at most hop / 8 + 1 <= 449 of them; they
+    // are laid out in order in the wave's scratch, then written (and
+    // demodulated) by consecutive lanes.
     const int kb = t & 7, kc = t >> 3;
     const int q0 = kc + 8 * kb;
     const int cs = (s * kHop + d.r - kTail - d.e0) / DP;  // exact: e - e0 is a multiple of DP
+    // e0 < D: the numerators are >= 0 (and below 2^22: blocks are shorter)
+    const int m_lo = fast_div(s * kHop - d.e0 + d.D - 1, d.D, fo.invD);
+    const int m_hi = min(d.n_out, fast_div(s * kHop + kHop - d.e0 + d.D - 1, d.D, fo.invD));
+    float2* os = (float2*)xs;
     // rotation of the lane's first position, then one step per 64 positions
     float sn, cn;
     argus_nco(d.phase0 + d.inc * (uint64_t)(int64_t)(s * kHop + q0 * DP + d.r - kTail), sn, cn);
@@ -292,11 +352,14 @@ __device__ __forceinline__ void fast_segment(const ArgusFastDesc& d, const FastO
             }
             if (rem == 0 && m < d.n_out) {
                 const float2 y = make_float2(v[ka].x, -v[ka].y);
-                d.out[m] = make_float2(y.x * rot.x - y.y * rot.y, y.x * rot.y + y.y * rot.x);
+                os[m - m_lo] = make_float2(y.x * rot.x - y.y * rot.y, y.x * rot.y + y.y * rot.x);
             }
         }
         rot = make_float2(rot.x * fo.step.x - rot.y * fo.step.y, rot.x * fo.step.y + rot.y * fo.step.x);
     }
+    wave_sync();
+    fast_emit(d, os, m_lo, m_hi - m_lo, t);
+    wave_sync();  // the scratch is the next segment's fold buffer
 }
 
 template <int DP>
@@ -305,7 +368,8 @@ __device__ __forceinline__ void fast_block(const ArgusFastDesc& d, const float2*
 {
     FastOut fo;
     fo.E = d.D / DP;
-    fo.invE = __builtin_bit_cast(float, __builtin_bit_cast(uint32_t, 1.0f / (float)fo.E) + 1u);
+    fo.invE = fast_recip(fo.E);
+    fo.invD = fast_recip(d.D);
     float sn, cn;
     argus_nco(d.inc * (uint64_t)(64 * DP), sn, cn);
     fo.step = make_float2(cn, sn);

@@ -2,11 +2,15 @@
 // the streaming service (layout and semantics in fastconv.hip).
 #pragma once
 #include "common.hpp"
+#include "demod.hpp"
 #include "extract.hpp"
 
 #define ARGUS_FAST_N 4096      // segment transform size
 #define ARGUS_FAST_TAIL 511    // input samples kept before each block (taps <= 512)
 #define ARGUS_FAST_HOP 3584    // new samples per segment: <= N - TAIL, a multiple of 64
+// Longest block the fast path takes: its output indexing divides block
+// positions by float reciprocals, exact to one correction step below 2^22.
+#define ARGUS_FAST_MAX_BLOCK (1 << 22)
 
 // One channel on the fast path for one block.
 struct ArgusFastDesc {
@@ -19,6 +23,14 @@ struct ArgusFastDesc {
     int e0;            // block sample of output 0
     int n_out;         // outputs this block
     int r;             // (e0 + ARGUS_FAST_TAIL) mod Dp
+    // demodulation in the epilogue: ARGUS_DEMOD_NFM/WFM (every output but
+    // the first of each segment: the demodulation pass adds those), USB/LSB
+    // or CW (every output); ARGUS_DEMOD_NONE: baseband only
+    int demod;
+    float* audio;      // demodulated outputs (output m at audio[m - skip])
+    uint64_t bfo0;     // SSB/CW: BFO phase of output 0, pre-increment applied (turns)
+    uint64_t bfo_inc;  // SSB/CW: BFO increment per output (turns)
+    int skip;          // FM: 1 when output 0 has no predecessor (no carry)
     int pad;
 };
 
@@ -48,6 +60,24 @@ static inline int argus_fast_fold(int D)
 {
     const int p = D & -D;
     return p > 64 ? 64 : p;
+}
+// The fast-path descriptor of a channel described for the direct form
+// (same outputs, NCO and grid), baseband only.
+static inline ArgusFastDesc argus_fast_desc(const ArgusChanDesc& ch, const float2* G)
+{
+    ArgusFastDesc fd{};
+    fd.G = G;
+    fd.out = ch.out;
+    fd.phase0 = ch.phase0;
+    fd.inc = ch.inc;
+    fd.D = ch.D;
+    fd.Dp = argus_fast_fold(ch.D);
+    fd.e0 = ch.e0;
+    fd.n_out = ch.n_out;
+    fd.r = (ch.e0 + ARGUS_FAST_TAIL) % fd.Dp;
+    fd.demod = ARGUS_DEMOD_NONE;
+    fd.audio = nullptr;
+    return fd;
 }
 static inline int argus_fast_segments(int n) { return n > 0 ? (n + ARGUS_FAST_HOP - 1) / ARGUS_FAST_HOP : 0; }
 static inline size_t argus_fast_u_len(int segments)

@@ -474,20 +474,10 @@ static int run_batched(gpud_runner* r, const float2* d_in, int n_new, const gpud
         // once the signal's history is the stream's own last T-1 samples,
         // its outputs come from the fast form and its direct items are only
         // the history
-        const bool fast = r->fast && tail_ok && s->fastG_valid && s->hist_len == T - 1 && n_new > 0;
+        const bool fast = r->fast && tail_ok && s->fastG_valid && s->hist_len == T - 1 && n_new > 0 &&
+                          n_new <= ARGUS_FAST_MAX_BLOCK;
         if (fast) {
-            ArgusFastDesc fd;
-            fd.G = s->d_fastG;
-            fd.out = ch.out;
-            fd.phase0 = ch.phase0;
-            fd.inc = ch.inc;
-            fd.D = ch.D;
-            fd.Dp = argus_fast_fold(ch.D);
-            fd.e0 = ch.e0;
-            fd.n_out = ch.n_out;
-            fd.r = (ch.e0 + ARGUS_FAST_TAIL) % fd.Dp;
-            fd.pad = 0;
-            r->fdescs.push_back(fd);
+            r->fdescs.push_back(argus_fast_desc(ch, s->d_fastG));
             ArgusChanDesc hist_only = ch;
             hist_only.n_out = 0;
             argus_append_items(hist_only, i, n_new, r->items, r->item_pos);

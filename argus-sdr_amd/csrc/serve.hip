@@ -639,13 +639,11 @@ static int grow_table(ServeSlot& sl, size_t need)
     return 0;
 }
 
-// Output layout of one block of n samples over the live channels (in the
-// order of `live`): per-channel counts and offsets, and the totals.
 // Whether the channel takes the fast form this block (its history is the
 // stream's own last T-1 samples).
 static bool fast_now(const gpud_serve* s, const ServeChannel& c, int n)
 {
-    return s->fast && c.d_fastG && c.hist_len == c.T - 1 && n > 0;
+    return s->fast && c.d_fastG && c.hist_len == c.T - 1 && n > 0 && n <= ARGUS_FAST_MAX_BLOCK;
 }
 
 // Whether its outputs are complete after the fast kernels and its own
@@ -655,6 +653,9 @@ static bool early_now(const gpud_serve* s, const ServeChannel& c, int n)
     return fast_now(s, c, n) && c.pre_factor == 1 && !c.stereo && c.audio_rate == 0;
 }
 
+// Output layout of one block of n samples over the live channels (in the
+// order of `live`, the early channels' IQ and demodulated outputs ahead of
+// the others'): per-channel counts and offsets, and the totals.
 static void layout_block(const gpud_serve* s, int n, BlockLayout& L)
 {
     L.outs.clear();
@@ -838,19 +839,9 @@ static int describe_block(gpud_serve* s, ServeSlot& sl, const std::vector<ServeO
         // the fast path, once the channel's history is the stream's own
         // last T-1 samples; its direct items are then only the history
         const bool fast = fast_now(s, c, n);
+        const int fi = fast ? (int)s->fdescs.size() : -1;
         if (fast) {
-            ArgusFastDesc fd;
-            fd.G = c.d_fastG;
-            fd.out = ch.out;
-            fd.phase0 = ch.phase0;
-            fd.inc = ch.inc;
-            fd.D = ch.D;
-            fd.Dp = argus_fast_fold(ch.D);
-            fd.e0 = ch.e0;
-            fd.n_out = ch.n_out;
-            fd.r = (ch.e0 + ARGUS_FAST_TAIL) % fd.Dp;
-            fd.pad = 0;
-            s->fdescs.push_back(fd);
+            s->fdescs.push_back(argus_fast_desc(ch, c.d_fastG));
         } else if ((c.decim - 1 + n) / c.decim > 0) {
             lds = std::max(lds, argus_item_lds_bytes(ch));
         }
@@ -894,13 +885,30 @@ static int describe_block(gpud_serve* s, ServeSlot& sl, const std::vector<ServeO
         const double bfo_inc = 2.0 * M_PI * bfo_hz / (double)c.demod_rate;
         dd.bfo0 = argus_rad_to_fix(c.bfo_phase + bfo_inc);
         dd.bfo_inc = argus_rad_to_fix(bfo_inc);
+        // the fast kernel demodulates its channel's baseband in its epilogue
+        // when the demodulator reads that baseband directly
+        const bool fm = dd.mode == ARGUS_DEMOD_NFM || dd.mode == ARGUS_DEMOD_WFM;
+        const bool fused = fi >= 0 && s->fuse_demod && c.pre_factor == 1 && !c.stereo &&
+                           (fm || dd.mode == ARGUS_DEMOD_USB || dd.mode == ARGUS_DEMOD_LSB ||
+                            dd.mode == ARGUS_DEMOD_CW);
+        if (fused) {
+            ArgusFastDesc& fd = s->fdescs[(size_t)fi];
+            fd.demod = dd.mode;
+            fd.audio = dd.audio;
+            fd.bfo0 = dd.bfo0;
+            fd.bfo_inc = dd.bfo_inc;
+            fd.skip = dd.skip;
+        }
         if (dd.mode != ARGUS_DEMOD_NONE && o.n_pre > 0) {
             std::vector<ArgusDemodItem>& di = o.early ? s->ditems_e : s->ditems;
-            if (dd.mode == ARGUS_DEMOD_AM) {
-                di.push_back({ci, 0, o.n_pre, 0});
+            if (fused) {
+                // FM: the segments' first outputs and the carry; SSB/CW: done
+                if (fm) di.push_back({ci, ch.e0, ch.D, ARGUS_DEMOD_ITEM_SEAMS});
+            } else if (dd.mode == ARGUS_DEMOD_AM) {
+                di.push_back({ci, 0, o.n_pre, ARGUS_DEMOD_ITEM_OUT});
             } else {
                 for (int st = 0; st < o.n_pre; st += ARGUS_DEMOD_TILE)
-                    di.push_back({ci, st, std::min(ARGUS_DEMOD_TILE, o.n_pre - st), 0});
+                    di.push_back({ci, st, std::min(ARGUS_DEMOD_TILE, o.n_pre - st), ARGUS_DEMOD_ITEM_OUT});
             }
         }
         if (c.stereo) {
@@ -1688,6 +1696,18 @@ GPUD_API int gpud_serve_set_fast(gpud_serve* s, int enable)
 {
     if (!s) return -1;
     s->fast = enable != 0;
+    s->epoch++;
+    return 0;
+}
+
+// Demodulation in the fast kernel's epilogue for the fast-form channels
+// whose demodulator reads their baseband (FM, SSB, CW; no pre-demod or
+// stereo stage): on by default.  Off, those channels' baseband is
+// demodulated by the separate pass as every other channel's (same outputs).
+GPUD_API int gpud_serve_set_demod_fusion(gpud_serve* s, int enable)
+{
+    if (!s) return -1;
+    s->fuse_demod = enable != 0;
     s->epoch++;
     return 0;
 }

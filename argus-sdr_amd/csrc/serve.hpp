@@ -192,7 +192,6 @@ struct ServeSlot {
     std::vector<ServeOut> outs;
 };
 
-// Output layout of one block over the live channels.
 // A block's output layout.  Channels whose outputs are complete once the
 // fast-convolution kernels and their demodulation have run (fast form this
 // block, plain demodulation: no pre-demod, stereo or audio stage) come first
@@ -298,6 +297,7 @@ struct gpud_serve {
     // fast-convolution path: on/off, the stream's last input samples, the
     // staged segments and their transforms (sized for the largest block)
     bool fast = true;
+    bool fuse_demod = true;  // demodulation in the fast kernel's epilogue (gpud_serve_set_demod_fusion)
     float2* d_ftail = nullptr;
     float2* d_fu = nullptr;
     float2* d_fX = nullptr;

@@ -55,6 +55,7 @@ N_ROOF = 8192              # channels of the roofline measurement (traffic profi
 FP32_PEAK_TFLOPS = 157.3
 DEEMPH_US = 50.0           # de-emphasis time constant of the audio stage (Europe)
 METRIC = "real-time channels sustained @20 Msps span (1 GPU); p99 block latency; HBM GB/s"
+DEMOD_FUSION = [True]      # demodulation in the fast kernel's epilogue (--unfused-demod: the separate pass)
 
 
 def make_channels(n: int, seed: int = 0):
@@ -75,14 +76,29 @@ def make_block(seed: int) -> np.ndarray:
     return (rng.standard_normal(2 * BLOCK, dtype=np.float32) * np.float32(0.5)).view(np.complex64)
 
 
-def alg_figures(chans, n_in):
-    """Algorithmic bytes and flops of one fused extraction launch (unique bytes)."""
+DEMOD_FLOPS = {1: 6, 2: 6, 4: 3, 5: 3, 6: 1}  # per output: FM conjugate product (atan2 not counted), SSB, CW
+
+
+def fast_eligible(d, taps):
+    return d % 8 == 0 and d < taps <= 512
+
+
+def alg_figures(chans, n_in, fused_demod=True):
+    """Algorithmic bytes and flops of one extraction stage (unique bytes):
+    the input, every channel's outputs, taps and history, and the
+    shift-FIR-decimate flops; with fused_demod, the demodulation the fast
+    kernel does in its epilogue too (FM, SSB, CW on the fast form): 4 bytes
+    written and DEMOD_FLOPS flops per output."""
     b = 8 * n_in
     f = 0
-    for (_, _, _, rate, taps, _) in chans:
-        n_out = n_in // (FS // rate)
+    for (_, _, _, rate, taps, dm) in chans:
+        d = FS // rate
+        n_out = n_in // d
         b += n_out * 8 + 16 * (taps - 1) + 4 * taps
         f += 6 * n_in + 4 * taps * n_out
+        if fused_demod and dm in DEMOD_FLOPS and fast_eligible(d, taps):
+            b += 4 * n_out
+            f += DEMOD_FLOPS[dm] * n_out
     return b, f
 
 
@@ -117,6 +133,7 @@ def open_service(chans, d_block, S, tune=None, budget=256 << 30, audio=0):
     its history (and the eligible ones take the fast form), then autotuned on
     that steady state -- or set to `tune` (lds_entries, narrow_outputs, grid[, fast_grid])."""
     s = S.Serve(FS, BLOCK, budget)
+    s.SetDemodFusion(DEMOD_FUSION[0])
     for (cid, off, bw, rate, taps, dm) in chans:
         assert s.Open(S.ChannelSpec(cid, off, bw, rate, taps, dm, 0))
         if audio:
@@ -835,13 +852,14 @@ def fast_flops(chans, n_in):
     count for the rest."""
     S_ = -(-n_in // 3584)
     total = S_ * 5 * 4096 * 12
-    for (_, _, _, rate, taps, _) in chans:
+    for (_, _, _, rate, taps, dm) in chans:
         d = FS // rate
         n_out = n_in // d
-        if d % 8 == 0 and d < taps <= 512:
+        if fast_eligible(d, taps):
             dp = min(d & -d, 64)
             m = 4096 // dp
             total += S_ * (6 * 4096 + 2 * (4096 - m) + 5 * m * int(np.log2(m))) + 6 * n_out
+            total += DEMOD_FLOPS.get(dm, 0) * n_out
         else:
             total += 6 * n_in + 4 * taps * n_out
     return total
@@ -849,14 +867,18 @@ def fast_flops(chans, n_in):
 
 def bench_roofline(a, d_block, S):
     """The extraction stage at N_ROOF channels of the mix: HIP-event time of
-    the stage (the fast form's transforms and per-channel pass, then the
-    fused direct launch) on the service's compute stream, averaged over
-    a.roof_reps launches after autotuning on the steady state."""
+    the stage as served (the fast form's transforms and per-channel pass
+    with the demodulation fused into its epilogue, then the fused direct
+    launch) on the service's compute stream, averaged over a.roof_reps
+    launches after autotuning on the steady state.  Beside it, the same
+    stage with the demodulation left to its separate pass."""
     chans = make_channels(N_ROOF)
     s, tune = open_service(chans, d_block, S, a.tune_fixed)
     avg, mn = s.ExtractTime(d_block, BLOCK, a.roof_reps)
+    s.SetDemodFusion(False)
+    avg_unfused, _ = s.ExtractTime(d_block, BLOCK, a.roof_reps)
     s.Close()
-    b_alg, f_alg = alg_figures(chans, BLOCK)
+    b_alg, f_alg = alg_figures(chans, BLOCK, DEMOD_FUSION[0])
     f_exec = fast_flops(chans, BLOCK)
     traffic, prof = load_traffic(N_ROOF)
     valu = f_alg / (avg * 1e-3) / 1e12
@@ -866,6 +888,7 @@ def bench_roofline(a, d_block, S):
             "achieved": round(valu, 2), "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
             "frac": round(valu / FP32_PEAK_TFLOPS, 4), "traffic": traffic, "traffic_profile": prof,
             "channels": N_ROOF, "kernel_ms": round(avg, 4), "kernel_ms_min": round(mn, 4), "reps": a.roof_reps,
+            "kernel_ms_demod_unfused": round(avg_unfused, 4),
             "alg_flops": f_alg, "alg_bytes": b_alg,
             "executed_flops_estimate": f_exec,
             "executed_tflops": round(f_exec / (avg * 1e-3) / 1e12, 2),
@@ -901,10 +924,13 @@ def main() -> int:
                     help="run one device-to-device copy of this many MiB (counter calibration)")
     ap.add_argument("--audio-blocks", type=int, default=100,
                     help="blocks per point of the audio-stage figure (0: skip it)")
+    ap.add_argument("--unfused-demod", action="store_true",
+                    help="demodulate every channel in the separate pass (no demodulation in the fast kernel)")
     ap.add_argument("--tune", default="",
                     help="LDS_ENTRIES,NARROW_OUTPUTS,GRID[,FAST_GRID]: use this tile/launch shape instead of autotuning")
     a = ap.parse_args()
     a.tune_fixed = tuple(int(x) for x in a.tune.split(",")) if a.tune else None
+    DEMOD_FUSION[0] = not a.unfused_demod
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: the extraction path shards by channel and this build runs on a "
               "single MI355X; run with --gpus 1.", file=sys.stderr)
@@ -1070,7 +1096,8 @@ def main() -> int:
                    "sample_rate": FS, "block_samples": BLOCK, "channels": n_rt,
                    "modes": "NFM/AM/USB/WFM (+ demod)", "decim": "8-1600", "taps": "63-511", "seed": 0,
                    "p99_bound_ms": P99_BOUND_MS, "latency": "block handed to the ingest ring -> demodulated "
-                   "outputs in host memory; blocks every 10 ms (2x real time), up to 3 in flight"},
+                   "outputs in host memory; blocks every 10 ms (2x real time), up to 3 in flight",
+                   "demod_fused_into_fast_kernel": DEMOD_FUSION[0]},
         "p50_block_ms": round(sweep[n_rt][0], 4) if n_rt in sweep else None,
         "p99_block_ms": round(sweep[n_rt][1], 4) if n_rt in sweep else None,
         "p50_block_gpu_ms": round(percentile(gpu_at.get(n_rt, []), 50), 4) if gpu_at.get(n_rt) else None,

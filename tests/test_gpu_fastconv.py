@@ -5,12 +5,16 @@ against the oracle and against the direct form: channels of decimation 8,
 opened mid-stream (direct form until its history is full, then fast), every
 demod mode, and ineligible channels beside them.  Baseband and audio within
 1e-4 relative RMS of the oracle; fast and direct forms agree to FP32
-rounding but not bit for bit (so the fast form did run)."""
+rounding but not bit for bit (so the fast form did run).
+
+The demodulation fused into the fast kernel's epilogue (the default) gives
+the same bits as the separate demodulation pass over the same baseband:
+FM with its segment seams and carry, SSB, CW, on the same ragged blocks."""
 import numpy as np
 import pytest
 
 from helpers import rel_rms
-from serve_oracle import AM, NFM, NONE, USB, WFM, OracleChannel, synth_scene
+from serve_oracle import AM, CW, LSB, NFM, NONE, USB, WFM, OracleChannel, synth_scene
 
 pytestmark = pytest.mark.gpu
 
@@ -38,9 +42,10 @@ def serve_mod(gpu):
     return serve
 
 
-def _run(S, fast, x, sizes):
+def _run(S, fast, x, sizes, fusion=True):
     s = S.Serve(SR, BLOCK, 1 << 30)
     s.SetFast(fast)
+    s.SetDemodFusion(fusion)
     orc = {}
     specs = list(SPECS)
     for cid, _, off, bw, rate, taps, dm, win in specs:
@@ -90,3 +95,56 @@ def test_fast_form_matches_oracle_and_direct_form(serve_mod):
         assert rel_rms(g_iq, d_iq) <= 1e-5, cid
         differ += int(not np.array_equal(g_iq, d_iq))
     assert differ >= 6  # the eligible channels took the other form
+
+
+FUSED = [  # id, offset, bw, out_rate, taps, demod: every mode the epilogue demodulates, and AM beside them
+    (21, 300_000.0, 200_000, 320_000, 511, WFM),   # D 8: 448 outputs per segment
+    (22, -400_000.0, 25_000, 160_000, 127, NFM),   # D 16
+    (23, -150_000.0, 3_000, 40_000, 255, USB),     # D 64: fold by 64
+    (24, 250_000.0, 3_000, 80_000, 129, LSB),      # D 32
+    (25, 600_000.0, 1_000, 320_000, 63, CW),       # D 8
+    (26, -250_000.0, 12_500, 64_000, 255, NFM),    # D 40: every 5th fold position
+    (27, 100_000.0, 10_000, 80_000, 65, AM),       # D 32: the separate pass
+]
+
+
+def _run_fusion(S, fusion, x, sizes):
+    s = S.Serve(SR, BLOCK, 1 << 30)
+    s.SetDemodFusion(fusion)
+    for cid, off, bw, rate, taps, dm in FUSED:
+        assert s.Open(S.ChannelSpec(cid, off, bw, rate, taps, dm, 0))
+    out = []
+    pos = 0
+    for k, n in enumerate(sizes):
+        if k == 2:  # opened mid-stream: direct form first, then fast with the carry
+            assert s.Open(S.ChannelSpec(28, -700_000.0, 12_500, 160_000, 201, NFM, 0))
+        blk = x[pos:pos + n]
+        pos += n
+        print(f"fusion={fusion} block {k} n={n}", flush=True)
+        t = s.Submit(blk)
+        s.Wait(t)
+        cids = [c[0] for c in FUSED] + ([28] if k >= 2 else [])
+        out.append({cid: tuple(np.copy(a) for a in s.Output(t, cid)[:2]) for cid in cids})
+    s.Close()
+    return out
+
+
+def test_fused_demod_same_bits_as_separate_pass(serve_mod):
+    S = serve_mod
+    kinds = [("wfm", 300_000.0, 0.5), ("fm", -400_000.0, 0.5), ("ssb", -150_000.0, 0.5),
+             ("ssb", 250_000.0, 0.5), ("tone", 600_000.0, 0.5), ("fm", -250_000.0, 0.5),
+             ("am", 100_000.0, 0.5), ("fm", -700_000.0, 0.5)]
+    sizes = [BLOCK, BLOCK, 3584, 3585, 7, 0, 12_345, BLOCK, 7168, BLOCK]
+    x = synth_scene(sum(sizes), SR, kinds, seed=9)
+    fused = _run_fusion(S, True, x, sizes)
+    sep = _run_fusion(S, False, x, sizes)
+    n_audio = 0
+    for k, (bf, bs) in enumerate(zip(fused, sep)):
+        assert bf.keys() == bs.keys()
+        for cid in bf:
+            (fi, fa), (si, sa) = bf[cid], bs[cid]
+            assert np.array_equal(fi, si), (k, cid)
+            assert len(fa) == len(sa), (k, cid)
+            assert np.array_equal(fa, sa), (k, cid, np.flatnonzero(fa != sa)[:8])
+            n_audio += len(fa)
+    assert n_audio > 100_000
