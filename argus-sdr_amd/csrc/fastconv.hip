@@ -26,12 +26,14 @@
 //   tail    the stream's last 511 samples after the block.
 //
 // The inverse transform is the 512-point FFT (8^3) of the conjugated folded
-// spectrum, zero-stuffed by 512 / J for Dp > 8 (stuffing the spectrum
-// repeats the signal, so its first J outputs are the J-point transform).
-// A work item is one channel and a block of segments: G' in LDS (32 KiB),
-// each wavefront of the workgroup runs its own segments with transposes
-// synchronised within the wave.  An output's rotation is one NCO evaluation
-// per lane and segment, then a complex step per 64 fold positions.
+// spectrum: for Dp = 16 two segments' 256 bins packed into one transform
+// and separated after it, for Dp = 32 and 64 zero-stuffed by 512 / J
+// (stuffing the spectrum repeats the signal, so its first J outputs are the
+// J-point transform).  A work item is one channel and a block of segments:
+// G' in LDS (32 KiB), each wavefront of the workgroup runs its own segments
+// with transposes synchronised within the wave.  An output's rotation is
+// one NCO evaluation per lane and segment, then a complex step per 64 fold
+// positions.
 //
 // The last 511 samples of the stream are the channel's own history only once
 // the channel has seen T - 1 samples (a channel opened mid-stream starts
@@ -92,7 +94,8 @@ constexpr int kChanThreads = 64 * kWaves;
 constexpr int kSegWave = 7;                    // segments per wavefront per item
 constexpr int kSegItem = kWaves * kSegWave;    // 56: two items cover a 20 ms block at 20 Msps
 constexpr int kXStride = 72;                   // first transpose: 64 values + 8 pad per row (conflict-free reads)
-constexpr int kXsFloats = 2 * 512;  // the 512 folded bins; also the transposes (8 x 72, 8 x 9 x 8 floats)
+constexpr int kPadC = 32;                      // complex words of padding between a group's packed segments
+constexpr int kXsFloats = 2 * (512 + kPadC);   // the packed folded bins; also the transposes (8 x 72, 8 x 9 x 8 floats)
 
 // The wave's LDS operations complete in issue order, so data a wave passes
 // between its own lanes through LDS needs no hardware barrier: only the
@@ -222,6 +225,13 @@ __device__ __forceinline__ void fast_fold_pairs(const float2* __restrict__ xseg,
     for (int c = 0; c < P; c++) f[c] = make_float4(ar[c], ai[c], br[c], bi[c]);
 }
 
+// Segments per 512-point transform for a fold factor (see fast_group).
+template <int DP>
+__host__ __device__ constexpr int fast_pack()
+{
+    return DP == 16 ? 2 : 1;
+}
+
 // The segment's outputs m_lo + i, i < cnt, from os[i]: baseband, and the
 // demodulated value when the channel demodulates here.  FM takes its
 // predecessor from the same segment, so a segment's first output (i = 0) is
@@ -281,91 +291,159 @@ __device__ __forceinline__ int fast_div(int a, int d, float recip)
     return q;
 }
 
-// One segment of one channel on one wavefront: fold, inverse transform
-// (the forward transform of the conjugate), outputs.
+// One group of segments of one channel on one wavefront: fold each, one
+// 512-point transform for all of them, outputs.  A J-point inverse (J =
+// 4096 / DP) fills 512 points by zero-stuffing (ZS) or by packing ZP
+// segments (ZS ZP J = 512): DP = 8 takes one segment, DP = 16 packs two,
+// DP = 32 and 64 take one segment stuffed 4 or 8 times (packing more would
+// need more registers than four waves per SIMD leave).  The group's
+// segments are s_z = sb + kWaves z, z < ZP (those at or past s_end are
+// absent: zero input, no outputs).
+//
+// Input x[ZS (ZP n + z)] = conj(F_z[n]) (n < J), zero elsewhere.  Its
+// forward transform at k = q + J j (q < J, j < ZP) is
+// sum_z W_ZP^{zj} W_{ZP J}^{zq} Yc_z[q], Yc_z the J-point transform of
+// conj(F_z); the ZP values of one q sit in one lane (v[ka + KA j], KA =
+// J / 64 values of q per lane).  So with c_j = conj(V[q + J j]), the
+// segment's inverse transform out_z[q] = conj(Yc_z[q]) is
+// W_{ZP J}^{zq} (sum_j W_ZP^{zj} c_j) / ZP: a ZP-point DFT and a twiddle
+// per lane and value (ZP = 1: the forward transform of the conjugate).
+// Folded segment z sits at fs + z RS; RS = J + 32 / ZP keeps the
+// transform's strided reads of the regions in distinct banks (64 banks of
+// 4 bytes, 32 lanes per cycle).
 template <int DP>
-__device__ __forceinline__ void fast_segment(const ArgusFastDesc& d, const FastOut& fo, const float2* __restrict__ X,
-                                             const float2* gs, float* xs, int s, int t, const Tw512& tw)
+__device__ __forceinline__ void fast_group(const ArgusFastDesc& d, const FastOut& fo, const float2* __restrict__ X,
+                                           const float2* gs, float* xs, int sb, int s_end, int t, const Tw512& tw)
 {
-    constexpr int J = ARGUS_FAST_N / DP, PER = J / 64, Z = kJ / J;  // Z: zero-stuffing of the 512-point input
+    constexpr int J = ARGUS_FAST_N / DP, PER = J / 64, Z = fast_pack<DP>(), ZS = kJ / J / Z, KA = J / 64;
+    constexpr int RS = J + kPadC / Z;
+    static_assert(Z * RS * 2 <= kXsFloats, "the group's regions fit the wave's scratch");
+    static_assert(ZS * Z * J == kJ, "stuffing and packing fill the transform");
     // opaque to the optimiser: keeps the lane's LDS addresses (functions of
     // t only) from being hoisted out of the segment loop into registers
     asm volatile("" : "+v"(t));
-    float2 v[8];
-    float2* fs = (float2*)xs;  // the J folded bins pass through the wave's scratch
-    if constexpr (PER >= 2) {
-        // folded in bin pairs, then laid out as the transform takes them
-        float4 f[PER / 2];
-        fast_fold_pairs<DP>(X + (size_t)s * kN, gs, t, f);
+    float2* fs = (float2*)xs;
 #pragma unroll
-        for (int c = 0; c < PER / 2; c++) *(float4*)(fs + 2 * t + 128 * c) = f[c];
-    } else {
-        float2 f[PER];
-        fast_fold<DP>(X + (size_t)s * kN, gs, t, f);
+    for (int z = 0; z < Z; z++) {
+        const int sz = sb + kWaves * z;  // uniform over the wave
+        float2* fz = fs + z * RS;
+        if constexpr (PER >= 2) {
+            // folded in bin pairs, then laid out as the transform takes them
+            float4 f[PER / 2];
+            if (sz < s_end) {
+                fast_fold_pairs<DP>(X + (size_t)sz * kN, gs, t, f);
+            } else {
 #pragma unroll
-        for (int c = 0; c < PER; c++) fs[t + 64 * c] = f[c];
+                for (int c = 0; c < PER / 2; c++) f[c] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            }
+#pragma unroll
+            for (int c = 0; c < PER / 2; c++) *(float4*)(fz + 2 * t + 128 * c) = f[c];
+        } else {
+            float2 f[PER];
+            if (sz < s_end) {
+                fast_fold<DP>(X + (size_t)sz * kN, gs, t, f);
+            } else {
+#pragma unroll
+                for (int c = 0; c < PER; c++) f[c] = make_float2(0.0f, 0.0f);
+            }
+#pragma unroll
+            for (int c = 0; c < PER; c++) fz[t + 64 * c] = f[c];
+        }
     }
     wave_sync();
+    float2 v[8];
     {
 #pragma unroll
         for (int c = 0; c < 8; c++) {
             const int nn = t + 64 * c;
-            const float2 y = (nn % Z) == 0 ? fs[nn / Z] : make_float2(0.0f, 0.0f);
+            const int mm = nn / ZS;
+            const float2 y = (nn % ZS) == 0 ? fs[(mm % Z) * RS + mm / Z] : make_float2(0.0f, 0.0f);
             v[c] = make_float2(y.x, -y.y);
         }
         wave_sync();
     }
     wave_fft512(v, xs, t, tw);
-    // lane t = kb + 8 kc holds Y[q], q = kc + 8 kb + 64 ka, in v[ka]: fold
-    // position q is block sample e = s hop + q DP + r - tail, and
-    // e - e0 = DP (q + cs) with cs uniform over the segment; it is an
-    // output when q + cs >= 0 is a multiple of E, output (q + cs) / E.
-    // The segment's outputs are m in [m_lo, m_hi) (block samples e0 + m D
-    // in [s hop, (s + 1) hop)), at most hop / 8 + 1 <= 449 of them; they
-    // are laid out in order in the wave's scratch, then written (and
-    // demodulated) by consecutive lanes.
+    // lane t = kb + 8 kc holds V[q + J j], q = kc + 8 kb + 64 ka (ka < KA),
+    // in v[ka + KA j] (the rest, for ZS > 1, repeats)
     const int kb = t & 7, kc = t >> 3;
     const int q0 = kc + 8 * kb;
-    const int cs = (s * kHop + d.r - kTail - d.e0) / DP;  // exact: e - e0 is a multiple of DP
-    // e0 < D: the numerators are >= 0 (and below 2^22: blocks are shorter)
-    const int m_lo = fast_div(s * kHop - d.e0 + d.D - 1, d.D, fo.invD);
-    const int m_hi = min(d.n_out, fast_div(s * kHop + kHop - d.e0 + d.D - 1, d.D, fo.invD));
-    float2* os = (float2*)xs;
-    // rotation of the lane's first position, then one step per 64 positions
-    float sn, cn;
-    argus_nco(d.phase0 + d.inc * (uint64_t)(int64_t)(s * kHop + q0 * DP + d.r - kTail), sn, cn);
-    float2 rot = make_float2(cn, sn);
+    float2 y[Z][KA];  // out_z[q0 + 64 ka]
 #pragma unroll
-    for (int ka = 0; ka < 8; ka++) {
-        const int q = q0 + 64 * ka;
-        const int iseg = q * DP + d.r;
-        const int u = q + cs;
-        if (q < J && iseg >= kTail && iseg < kTail + kHop && u >= 0) {
-            int m = (int)((float)u * fo.invE);
-            int rem = u - m * fo.E;
-            if (rem < 0) {
-                m--;
-                rem += fo.E;
-            } else if (rem >= fo.E) {
-                m++;
-                rem -= fo.E;
+    for (int ka = 0; ka < KA; ka++) {
+        float2 c[Z];
+#pragma unroll
+        for (int j = 0; j < Z; j++) c[j] = make_float2(v[ka + KA * j].x, -v[ka + KA * j].y);
+        if constexpr (Z > 1) {
+            dft<Z>(c);
+            const int q = q0 + 64 * ka;
+            constexpr float inv = 1.0f / (float)Z;  // exact
+            y[0][ka] = make_float2(c[0].x * inv, c[0].y * inv);
+#pragma unroll
+            for (int z = 1; z < Z; z++) {
+                const float2 w = cmul(c[z], twiddle((ZS * z * q) & (kJ - 1), 1.0f / kJ));
+                y[z][ka] = make_float2(w.x * inv, w.y * inv);
             }
-            if (rem == 0 && m < d.n_out) {
-                const float2 y = make_float2(v[ka].x, -v[ka].y);
-                os[m - m_lo] = make_float2(y.x * rot.x - y.y * rot.y, y.x * rot.y + y.y * rot.x);
-            }
+        } else {
+            y[0][ka] = c[0];
         }
-        rot = make_float2(rot.x * fo.step.x - rot.y * fo.step.y, rot.x * fo.step.y + rot.y * fo.step.x);
+    }
+    // fold position q of segment s is block sample e = s hop + q DP + r -
+    // tail, and e - e0 = DP (q + cs) with cs uniform over the segment; it is
+    // an output when q + cs >= 0 is a multiple of E, output (q + cs) / E.
+    // Segment s's outputs are m in [m_lo, m_hi) (block samples e0 + m D in
+    // [s hop, (s + 1) hop)), at most hop / DP + 1 <= J of them; they are
+    // laid out in order in region z of the scratch (free after the
+    // transform), then written (and demodulated) by consecutive lanes.
+    float2* os = fs;
+    int m_lo[Z], m_hi[Z];
+#pragma unroll
+    for (int z = 0; z < Z; z++) {
+        const int sz = sb + kWaves * z;
+        m_lo[z] = m_hi[z] = 0;
+        if (sz >= s_end) continue;
+        const int cs = (sz * kHop + d.r - kTail - d.e0) / DP;  // exact: e - e0 is a multiple of DP
+        // e0 < D: the numerators are >= 0 (and below 2^22: blocks are shorter)
+        m_lo[z] = fast_div(sz * kHop - d.e0 + d.D - 1, d.D, fo.invD);
+        m_hi[z] = min(d.n_out, fast_div(sz * kHop + kHop - d.e0 + d.D - 1, d.D, fo.invD));
+        // rotation of the lane's first position, then one step per 64 positions
+        float sn, cn;
+        argus_nco(d.phase0 + d.inc * (uint64_t)(int64_t)(sz * kHop + q0 * DP + d.r - kTail), sn, cn);
+        float2 rot = make_float2(cn, sn);
+        float2* oz = os + z * RS;
+#pragma unroll
+        for (int ka = 0; ka < KA; ka++) {
+            const int q = q0 + 64 * ka;
+            const int iseg = q * DP + d.r;
+            const int u = q + cs;
+            if (iseg >= kTail && iseg < kTail + kHop && u >= 0) {
+                int m = (int)((float)u * fo.invE);
+                int rem = u - m * fo.E;
+                if (rem < 0) {
+                    m--;
+                    rem += fo.E;
+                } else if (rem >= fo.E) {
+                    m++;
+                    rem -= fo.E;
+                }
+                if (rem == 0 && m < d.n_out) {
+                    const float2 yy = y[z][ka];
+                    oz[m - m_lo[z]] = make_float2(yy.x * rot.x - yy.y * rot.y, yy.x * rot.y + yy.y * rot.x);
+                }
+            }
+            rot = make_float2(rot.x * fo.step.x - rot.y * fo.step.y, rot.x * fo.step.y + rot.y * fo.step.x);
+        }
     }
     wave_sync();
-    fast_emit(d, os, m_lo, m_hi - m_lo, t);
-    wave_sync();  // the scratch is the next segment's fold buffer
+#pragma unroll
+    for (int z = 0; z < Z; z++) fast_emit(d, os + z * RS, m_lo[z], m_hi[z] - m_lo[z], t);
+    wave_sync();  // the scratch holds the next group's folded bins
 }
 
 template <int DP>
 __device__ __forceinline__ void fast_block(const ArgusFastDesc& d, const float2* __restrict__ X, const float2* gs,
                                            float* xs, int s_begin, int s_end, int w, int t, const Tw512& tw)
 {
+    constexpr int Z = fast_pack<DP>();
     FastOut fo;
     fo.E = d.D / DP;
     fo.invE = fast_recip(fo.E);
@@ -373,7 +451,7 @@ __device__ __forceinline__ void fast_block(const ArgusFastDesc& d, const float2*
     float sn, cn;
     argus_nco(d.inc * (uint64_t)(64 * DP), sn, cn);
     fo.step = make_float2(cn, sn);
-    for (int s = s_begin + w; s < s_end; s += kWaves) fast_segment<DP>(d, fo, X, gs, xs, s, t, tw);
+    for (int sb = s_begin + w; sb < s_end; sb += kWaves * Z) fast_group<DP>(d, fo, X, gs, xs, sb, s_end, t, tw);
 }
 
 __global__ __launch_bounds__(kChanThreads) void argus_fast_chan_kernel(const ArgusFastHdr* __restrict__ hdr)
