@@ -196,10 +196,22 @@ __global__ __launch_bounds__(kLevelOut) void argus_level_fir_kernel(const float2
 }
 
 // ---------------------------------------------------------------------------
-// CFAR + run extraction, one workgroup of 1024 threads per frame.
+// CFAR in three launches over all frames of a call:
+//   argus_cfar_prep_kernel    per frame: its largest dB value (which bounds
+//                             the reference's running sums, for the near-tie
+//                             count) and the count cleared;
+//   argus_cfar_status_kernel  per (frame, tile of kCfarTile bins): the tile's
+//                             cells and halo in linear power, their FP64
+//                             prefix sum (wavefront scans), each bin's window
+//                             sums as two differences of it, and the bin's
+//                             decision as bits of 64-bin masks;
+//   argus_cfar_runs_kernel    per frame: the hysteresis state across the
+//                             masks (a scan of per-thread transfer functions),
+//                             the runs, their peaks and noise estimates.
 // ---------------------------------------------------------------------------
-constexpr int kCfarThreads = 1024;
-constexpr int kCfarTile = 16384;  // bins of linear power staged in LDS at a time
+constexpr int kCfarThreads = 256;
+constexpr int kCfarTile = 1024;  // bins per workgroup of the status kernel
+constexpr double kLn10Over10 = 0.23025850929940458;  // 10^(x/10) = exp(x ln10 / 10)
 
 __device__ __forceinline__ int wrap_bin(int i, int n, int wrap)
 {
@@ -222,22 +234,21 @@ __device__ __forceinline__ double cfar_combine(int mode, double left, double rig
 // its window sums along the frame (ca.go:61-66: per bin one subtraction and
 // one addition per side), so at bin i each side carries at most 2 (i + t)
 // roundings of partial sums no larger than t * lmax (lmax the frame's largest
-// linear cell): |error| <= 2 u (i + t) t lmax per side, u = 2^-53.  Here the
-// sums are direct (t roundings).  A bin whose dB value lies within the
-// resulting threshold uncertainty of the threshold, or of the threshold less
-// the hysteresis, may be decided either way by the reference: it is counted
-// as ambiguous.  The bound adds the error of the direct sums and a slack for
-// last-bit differences of pow and log10.
-__device__ __forceinline__ double cfar_tie_db(int mode, double left, double right, int train, int i, double lmax)
+// linear cell): |error| <= 2 u (i + t) t lmax per side, u = 2^-53.  Here each
+// side is a difference of two prefix sums, within `ours` of its exact value.
+// Returns the relative uncertainty of the noise estimate, plus a slack for
+// last-bit differences between the reference's pow / log10 / dB threshold
+// and this kernel's exp and comparison in linear power.
+__device__ __forceinline__ double cfar_tie_rel(int mode, double left, double right, int train, int i, double lmax,
+                                               double ours)
 {
     const double u = 1.1102230246251565e-16;
-    const double side = 2.0 * u * (double)(i + 2 * train + 2) * (double)train * lmax;
+    const double side = 2.0 * u * (double)(i + 2 * train + 2) * (double)train * lmax + ours;
     double ref;
     if (mode == GPUD_CFAR_CA) ref = left + right;
     else if (mode == GPUD_CFAR_GOSCA) ref = left > right ? left : right;
     else ref = left < right ? left : right;
-    const double rel = (mode == GPUD_CFAR_CA ? 2.0 * side : side) / ref;
-    return 4.3429448190325182 * rel + 1e-12;  // 10 / ln 10 per unit relative error
+    return (mode == GPUD_CFAR_CA ? 2.0 * side : side) / ref + 2.5e-13;
 }
 
 // OS-CFAR (os.go:31-80): the rank-th smallest (0-based rk) of the 2*train
@@ -288,165 +299,266 @@ __device__ double cfar_threshold_at(const float* __restrict__ db, int n, int i, 
     return 10.0 * log10(cfar_combine(p.mode, left, right, p.train)) + p.scale_db;
 }
 
+// Inclusive prefix sum over the 64 lanes of a wavefront.
+__device__ __forceinline__ double wave_scan_add(double x, int lane)
+{
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const double y = __shfl_up(x, o, 64);
+        if (lane >= o) x += y;
+    }
+    return x;
+}
+
+__device__ __forceinline__ int wave_scan_add(int x, int lane)
+{
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const int y = __shfl_up(x, o, 64);
+        if (lane >= o) x += y;
+    }
+    return x;
+}
+
+__global__ __launch_bounds__(256) void argus_cfar_prep_kernel(const float* __restrict__ psd, int n,
+                                                              float* __restrict__ frame_max,
+                                                              int* __restrict__ ambiguous)
+{
+    __shared__ float red[4];
+    const float* db = psd + (int64_t)blockIdx.x * n;
+    float m = -INFINITY;
+    for (int b = threadIdx.x; b < n; b += 256) m = fmaxf(m, db[b]);
+    for (int o = 32; o > 0; o >>= 1) m = fmaxf(m, __shfl_xor(m, o));
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        frame_max[blockIdx.x] = fmaxf(fmaxf(red[0], red[1]), fmaxf(red[2], red[3]));
+        ambiguous[blockIdx.x] = 0;
+    }
+}
+
+// Bin b's decision, as the reference takes it from its threshold thr_b =
+// 10 log10(noise_b) + scale (ca.go:57-66, gosca.go, caso.go) and the
+// crossing with hysteresis (detector.go:436-480): 2 if v >= thr_b, 0 if v <
+// thr_b - hysteresis, else 1.  In linear power these are 10^(v/10) >=
+// noise_b * k_on and 10^(v/10) < noise_b * k_off, k_on = 10^(scale/10),
+// k_off = 10^((scale - hysteresis)/10), so no bin needs a logarithm.  Masks:
+// for each 64-bin group (x: code 2, y: code 0).
+__global__ __launch_bounds__(kCfarThreads) void argus_cfar_status_kernel(const float* __restrict__ psd, int n,
+                                                                         gpud_cfar_params p, double k_on, double k_off,
+                                                                         const float* __restrict__ frame_max,
+                                                                         int* __restrict__ ambiguous,
+                                                                         ulonglong2* __restrict__ masks)
+{
+    extern __shared__ double cf_lds[];
+    __shared__ double wtot[kCfarThreads / 64];
+    __shared__ int wties[kCfarThreads / 64];
+    const int halo = p.guard + p.train;
+    const int E = kCfarTile + 2 * halo;  // cells: the tile and its halo on both sides
+    double* cell = cf_lds;               // E cells, linear power (dB for OS-CFAR)
+    double* pre = cf_lds + E;            // pre[e] = cell[0] + ... + cell[e - 1], e = 0..E
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    const int tiles = n / kCfarTile, f = blockIdx.x / tiles, t0 = (blockIdx.x % tiles) * kCfarTile;
+    const float* db = psd + (int64_t)f * n;
+    const bool os = p.mode == GPUD_CFAR_OS;  // OS ranks the dB values themselves
+    const bool ties = ambiguous != nullptr && !os;
+
+    for (int e = t; e < E; e += kCfarThreads) {  // cell e is bin t0 - halo + e
+        const double v = (double)db[wrap_bin(t0 - halo + e, n, p.wrap)];
+        cell[e] = os ? v : exp(v * kLn10Over10);
+    }
+    __syncthreads();
+    // Prefix sum: wave w scans its rows of 64 consecutive cells in order,
+    // carrying each row's total into the next; the waves' totals are added
+    // afterwards.  Every partial sum is of non-negative terms no larger than
+    // the tile's total, and each entry takes at most rpw + 11 roundings.
+    const int rows = (E + 63) >> 6, rpw = (rows + kCfarThreads / 64 - 1) / (kCfarThreads / 64);
+    const int r0 = wave * rpw, r1 = min(r0 + rpw, rows);
+    if (!os) {
+        double carry = 0.0;
+        for (int r = r0; r < r1; ++r) {
+            const int e = (r << 6) + lane;
+            const double v = wave_scan_add(e < E ? cell[e] : 0.0, lane) + carry;
+            if (e < E) pre[e + 1] = v;
+            carry = __shfl(v, 63, 64);
+        }
+        if (lane == 0) wtot[wave] = carry;
+        __syncthreads();
+        double off = 0.0;
+        for (int w = 0; w < wave; ++w) off += wtot[w];
+        for (int r = r0; r < r1 && wave > 0; ++r) {
+            const int e = (r << 6) + lane;
+            if (e < E) pre[e + 1] += off;
+        }
+        if (t == 0) pre[0] = 0.0;
+        __syncthreads();
+    }
+    const double u = 1.1102230246251565e-16;
+    const double lmax = ties ? exp((double)frame_max[f] * kLn10Over10) : 0.0;
+    // a side is pre[a] - pre[b]: at most 2 (rpw + 11) + 1 roundings of the total
+    const double ours = os ? 0.0 : (double)(2 * rpw + 24) * u * pre[E];
+    int nt = 0;
+#pragma unroll
+    for (int c = 0; c < kCfarTile / kCfarThreads; ++c) {
+        const int j = c * kCfarThreads + t, e = j + halo, b = t0 + j;
+        int code;
+        if (os) {
+            auto cellf = [&](int m) {
+                const int k = m / 2 + 1;
+                return cell[(m & 1) ? e + p.guard + k : e - p.guard - k];
+            };
+            const double thr = os_select(cellf, p.train, os_rank(p)) + p.scale_db;
+            const double v = cell[e];
+            code = v >= thr ? 2 : (v < thr - p.hysteresis_db ? 0 : 1);
+        } else {
+            const double left = pre[e - p.guard] - pre[e - p.guard - p.train];
+            const double right = pre[e + p.guard + p.train + 1] - pre[e + p.guard + 1];
+            const double noise = cfar_combine(p.mode, left, right, p.train);
+            const double on = noise * k_on, off = noise * k_off, v = cell[e];
+            code = v >= on ? 2 : (v < off ? 0 : 1);
+            if (ties) {
+                const double rel = cfar_tie_rel(p.mode, left, right, p.train, b, lmax, ours);
+                nt += (fabs(v - on) <= rel * on || fabs(v - off) <= rel * off) ? 1 : 0;
+            }
+        }
+        const uint64_t hi = __ballot(code == 2), lo = __ballot(code == 0);
+        if (lane == 0)
+            masks[((int64_t)f * n + t0 + c * kCfarThreads + (wave << 6)) >> 6] = make_ulonglong2(hi, lo);
+    }
+    if (ties) {
+        for (int o = 32; o > 0; o >>= 1) nt += __shfl_xor(nt, o);
+        if (lane == 0) wties[wave] = nt;
+        __syncthreads();
+        if (t == 0) {
+            int sum = 0;
+            for (int w = 0; w < kCfarThreads / 64; ++w) sum += wties[w];
+            if (sum) atomicAdd(&ambiguous[f], sum);
+        }
+    }
+}
+
+// Runs of one frame from its masks.  The "in a run" state changes to 1 at a
+// code-2 bin and to 0 at a code-0 bin; thread t owns a contiguous range of
+// 64-bin groups, whose effect on the state is a transfer function (keep, or
+// set to the last decisive bin's value).  An exclusive scan of the transfers
+// gives each thread its entering state; starts and ends are then found with
+// bit scans, counted, placed by a second scan and written in order.
 __global__ __launch_bounds__(kCfarThreads) void argus_cfar_runs_kernel(const float* __restrict__ psd, int n,
                                                                        gpud_cfar_params p,
-                                                                       gpud_surv_run* __restrict__ runs,
-                                                                       int* __restrict__ run_counts, int max_runs,
-                                                                       int* __restrict__ ambiguous)
+                                                                       const ulonglong2* __restrict__ masks_all,
+                                                                       gpud_surv_run* __restrict__ runs_all,
+                                                                       int* __restrict__ run_counts, int max_runs)
 {
-    extern __shared__ double lin[];  // kCfarTile + 2 (guard + train) doubles, reused below
-    __shared__ float red[kCfarThreads / 64];
-    __shared__ int n_tie;
-    const int t = threadIdx.x;
-    const float* db = psd + (int64_t)blockIdx.x * n;
-    const int halo = p.guard + p.train;
-    const int tile = n < kCfarTile ? n : kCfarTile;
-    uint32_t* status = (uint32_t*)(lin + tile + 2 * halo);  // 2 bits per bin
+    constexpr int NW = kCfarThreads / 64;
+    __shared__ int wsc[3][NW];
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    const int f = blockIdx.x;
+    const int G = n >> 6, gpt = (G + kCfarThreads - 1) / kCfarThreads;
+    const ulonglong2* masks = masks_all + (int64_t)f * G;
+    const float* db = psd + (int64_t)f * n;
+    gpud_surv_run* runs = runs_all + (int64_t)f * max_runs;
+    const int g0 = min(t * gpt, G), g1 = min(g0 + gpt, G);
 
-    // the frame's largest dB value (its linear power bounds every window sum)
-    double lmax = 0.0;
-    if (ambiguous && p.mode != GPUD_CFAR_OS) {
-        float m = -INFINITY;
-        for (int b = t; b < n; b += kCfarThreads) m = fmaxf(m, db[b]);
-        for (int o = 32; o > 0; o >>= 1) m = fmaxf(m, __shfl_xor(m, o));
-        if ((t & 63) == 0) red[t >> 6] = m;
-        if (t == 0) n_tie = 0;
-        __syncthreads();
-        m = red[0];
-        for (int w = 1; w < kCfarThreads / 64; w++) m = fmaxf(m, red[w]);
-        lmax = pow(10.0, (double)m / 10.0);
-    } else if (t == 0) {
-        n_tie = 0;
+    int tf = -1;  // -1 keeps the state, 0/1 sets it
+    for (int g = g0; g < g1; ++g) {
+        const ulonglong2 m = masks[g];
+        const uint64_t d = m.x | m.y;
+        if (d) tf = (int)((m.x >> (63 - __clzll((long long)d))) & 1ull);
     }
-
-    // ---- CFAR status per bin: 2 = v >= thr, 0 = v < thr - hysteresis, 1 = between
-    for (int w = t; w < n / 16; w += kCfarThreads) status[w] = 0u;
-    for (int t0 = 0; t0 < n; t0 += tile) {
-        __syncthreads();  // previous tile's readers are done with lin
-        const bool os = p.mode == GPUD_CFAR_OS;  // OS ranks the dB values themselves
-        for (int i = t; i < tile + 2 * halo; i += kCfarThreads) {
-            const int b = wrap_bin(t0 - halo + i, n, p.wrap);
-            lin[i] = os ? (double)db[b] : pow(10.0, (double)db[b] / 10.0);
-        }
-        __syncthreads();
-        // consecutive lanes take consecutive bins: conflict-free LDS reads.
-        // Window sums are direct (k = 1..train in order), the threshold of
-        // ca.go:50-59 / gosca.go / caso.go without the running-sum drift.
-        for (int j = t; j < tile; j += kCfarThreads) {
-            double thr, tie = 0.0;
-            const int b = t0 + j;
-            if (os) {
-                auto cell = [&](int m) {
-                    const int k = m / 2 + 1;
-                    return lin[((m & 1) ? j + p.guard + k : j - p.guard - k) + halo];
-                };
-                thr = os_select(cell, p.train, os_rank(p)) + p.scale_db;
-            } else {
-                double left = 0.0, right = 0.0;
-                for (int k = 1; k <= p.train; ++k) {
-                    left += lin[j - p.guard - k + halo];
-                    right += lin[j + p.guard + k + halo];
-                }
-                thr = 10.0 * log10(cfar_combine(p.mode, left, right, p.train)) + p.scale_db;
-                if (ambiguous) tie = cfar_tie_db(p.mode, left, right, p.train, b, lmax);
-            }
-            const double v = (double)db[b];
-            if (ambiguous && !os && (fabs(v - thr) <= tie || fabs(v - (thr - p.hysteresis_db)) <= tie))
-                atomicAdd(&n_tie, 1);
-            const uint32_t code = v >= thr ? 2u : (v < thr - p.hysteresis_db ? 0u : 1u);
-            if (code) atomicOr(&status[b >> 4], code << ((b & 15) * 2));
-        }
+    int s = tf;  // inclusive scan, the later constant winning
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const int y = __shfl_up(s, o, 64);
+        if (lane >= o && s == -1) s = y;
     }
+    if (lane == 63) wsc[0][wave] = s;
+    int prev = __shfl_up(s, 1, 64);
     __syncthreads();
-
-    // ---- run extraction: scan of the "in a run" state over bins -----------
-    const int bpt = n / kCfarThreads;  // bins owned by each thread
-    const int first = t * bpt;
-    auto code_at = [&](int b) { return (int)((status[b >> 4] >> ((b & 15) * 2)) & 3u); };
-    int f = -1;  // transfer of this chunk: -1 identity, 0/1 constant
-    for (int j = 0; j < bpt; ++j) {
-        const int c = code_at(first + j);
-        if (c == 2) f = 1;
-        else if (c == 0) f = 0;
+    int before = -1, last = -1;
+    for (int w = 0; w < NW; ++w) {
+        if (w < wave && wsc[0][w] != -1) before = wsc[0][w];
+        if (wsc[0][w] != -1) last = wsc[0][w];
     }
-    int* scan = (int*)lin;  // kCfarThreads ints (the linear tile is no longer needed)
-    scan[t] = f;
-    __syncthreads();
-    for (int off = 1; off < kCfarThreads; off <<= 1) {
-        const int mine = scan[t];
-        const int prev = t >= off ? scan[t - off] : -1;
-        __syncthreads();
-        scan[t] = mine != -1 ? mine : prev;  // the last constant wins
-        __syncthreads();
-    }
-    int in_state = t > 0 ? scan[t - 1] : -1;
+    if (lane == 0) prev = -1;
+    int in_state = prev != -1 ? prev : before;
     if (in_state < 0) in_state = 0;
-    // events: starts and ends in this chunk
+    const bool open_at_end = last == 1;  // a run still open at the last bin ends there
+
     int ns = 0, ne = 0;
     {
         int st = in_state;
-        for (int j = 0; j < bpt; ++j) {
-            const int c = code_at(first + j);
-            const int nx = c == 2 ? 1 : (c == 0 ? 0 : st);
-            if (!st && nx) ns++;
-            if (st && !nx) ne++;
-            st = nx;
+        for (int g = g0; g < g1; ++g) {
+            const ulonglong2 m = masks[g];
+            int pos = 0;
+            while (pos < 64) {
+                const uint64_t want = (st ? m.y : m.x) & (~0ull << pos);
+                if (!want) break;
+                const int bit = __builtin_ctzll(want);
+                if (st) ne++;
+                else ns++;
+                st ^= 1;
+                pos = bit + 1;
+            }
         }
-        if (t == kCfarThreads - 1 && st) ne++;  // a run still open at the last bin ends there
+        if (t == kCfarThreads - 1 && open_at_end) ne++;
+    }
+    const int is = wave_scan_add(ns, lane), ie = wave_scan_add(ne, lane);
+    if (lane == 63) {
+        wsc[1][wave] = is;
+        wsc[2][wave] = ie;
     }
     __syncthreads();
-    int* cs = scan;                  // start counts
-    int* ce = scan + kCfarThreads;   // end counts
-    cs[t] = ns;
-    ce[t] = ne;
-    __syncthreads();
-    for (int off = 1; off < kCfarThreads; off <<= 1) {
-        const int a = cs[t], b = ce[t];
-        const int pa = t >= off ? cs[t - off] : 0, pb = t >= off ? ce[t - off] : 0;
-        __syncthreads();
-        cs[t] = a + pa;
-        ce[t] = b + pb;
-        __syncthreads();
+    int ps = is - ns, pe = ie - ne, total = 0;
+    for (int w = 0; w < NW; ++w) {
+        if (w < wave) {
+            ps += wsc[1][w];
+            pe += wsc[2][w];
+        }
+        total += wsc[1][w];
     }
-    const int total = cs[kCfarThreads - 1];
-    int* starts = scan + 2 * kCfarThreads;
-    int* ends = starts + max_runs;
     {
-        int ps = cs[t] - ns, pe = ce[t] - ne;
         int st = in_state;
-        for (int j = 0; j < bpt; ++j) {
-            const int b = first + j;
-            const int c = code_at(b);
-            const int nx = c == 2 ? 1 : (c == 0 ? 0 : st);
-            if (!st && nx) {
-                if (ps < max_runs) starts[ps] = b;
-                ps++;
+        for (int g = g0; g < g1; ++g) {
+            const ulonglong2 m = masks[g];
+            int pos = 0;
+            while (pos < 64) {
+                const uint64_t want = (st ? m.y : m.x) & (~0ull << pos);
+                if (!want) break;
+                const int bit = __builtin_ctzll(want);
+                const int b = (g << 6) + bit;
+                if (st) {
+                    if (pe < max_runs) runs[pe].last_bin = b - 1;
+                    pe++;
+                } else {
+                    if (ps < max_runs) runs[ps].first_bin = b;
+                    ps++;
+                }
+                st ^= 1;
+                pos = bit + 1;
             }
-            if (st && !nx) {
-                if (pe < max_runs) ends[pe] = b - 1;
-                pe++;
-            }
-            st = nx;
         }
-        if (t == kCfarThreads - 1 && st) {
-            if (pe < max_runs) ends[pe] = n - 1;
-        }
+        if (t == kCfarThreads - 1 && open_at_end && pe < max_runs) runs[pe].last_bin = n - 1;
     }
-    __syncthreads();
+    __syncthreads();  // every run's first and last bin written (workgroup-scope fence)
     const int nr = total < max_runs ? total : max_runs;
-    gpud_surv_run* out = runs + (int64_t)blockIdx.x * max_runs;
     for (int r = t; r < nr; r += kCfarThreads) {
-        const int s0 = starts[r], e0 = ends[r];
+        const int s0 = runs[r].first_bin, e0 = runs[r].last_bin;
         int pk = s0;
         double pv = -1e300;
-        for (int b = s0; b <= e0; ++b)
-            if (code_at(b) == 2) {
+        for (int g = s0 >> 6; g <= (e0 >> 6); ++g) {
+            uint64_t hi = masks[g].x;
+            if (g == (s0 >> 6)) hi &= ~0ull << (s0 & 63);
+            if (g == (e0 >> 6) && (e0 & 63) != 63) hi &= (1ull << ((e0 & 63) + 1)) - 1ull;
+            while (hi) {
+                const int b = (g << 6) + __builtin_ctzll(hi);
+                hi &= hi - 1ull;
                 const double v = (double)db[b];
                 if (v > pv) {
                     pv = v;
                     pk = b;
                 }
             }
+        }
         gpud_surv_run rr;
         rr.first_bin = s0;
         rr.last_bin = e0;
@@ -454,12 +566,9 @@ __global__ __launch_bounds__(kCfarThreads) void argus_cfar_runs_kernel(const flo
         rr.pad = 0;
         rr.peak_db = pv;
         rr.noise_db = cfar_threshold_at(db, n, pk, p) - p.scale_db;
-        out[r] = rr;
+        runs[r] = rr;
     }
-    if (t == 0) {
-        run_counts[blockIdx.x] = total;
-        if (ambiguous) ambiguous[blockIdx.x] = n_tie;
-    }
+    if (t == 0) run_counts[f] = total;
 }
 
 }  // namespace
@@ -474,13 +583,9 @@ static int fft_radix(int n)
     return 0;
 }
 
-size_t argus_cfar_lds_bytes(int n, const gpud_cfar_params& p)
+static size_t argus_cfar_lds_bytes(const gpud_cfar_params& p)
 {
-    const int tile = n < kCfarTile ? n : kCfarTile;
-    const size_t a = (size_t)(tile + 2 * (p.guard + p.train)) * sizeof(double) + (size_t)(n / 16) * 4;
-    const size_t b = (size_t)(2 * kCfarThreads) * 4 + 0;  // scans (runs come after, see below)
-    (void)b;
-    return a;
+    return (size_t)(2 * (kCfarTile + 2 * (p.guard + p.train)) + 1) * sizeof(double);
 }
 
 extern "C" {
@@ -510,6 +615,7 @@ GPUD_API int gpud_surv_destroy(gpud_surv* s)
     hipStreamSynchronize(s->stream);
     hipFree(s->d_win);
     if (s->d_scratch) hipFree(s->d_scratch);
+    if (s->d_cfar) hipFree(s->d_cfar);
     if (s->d_zstuff) hipFree(s->d_zstuff);
     if (s->d_ltaps) hipFree(s->d_ltaps);
     if (s->d_lwin) hipFree(s->d_lwin);
@@ -693,24 +799,36 @@ GPUD_API int gpud_surv_cfar_runs_device_ex(gpud_surv* s, const float* psd_dev, i
     if (p.mode != GPUD_CFAR_CA && p.mode != GPUD_CFAR_GOSCA && p.mode != GPUD_CFAR_CASO && p.mode != GPUD_CFAR_OS)
         return -1;
     if (p.mode == GPUD_CFAR_OS && p.train > 128) return -1;
-    if (s->n % kCfarThreads != 0) return -1;
+    if (s->n % kCfarTile != 0) return -1;
     if (n_frames <= 0) return 0;
-    size_t lds = argus_cfar_lds_bytes(s->n, p);
-    const size_t need_runs = (size_t)(2 * kCfarThreads + 2 * max_runs) * 4 + (size_t)(s->n / 16) * 4;
-    // status words live after the linear tile; the scan/run region reuses the
-    // tile, so it must fit below the status words
-    const int tile = s->n < kCfarTile ? s->n : kCfarTile;
-    if ((size_t)(2 * kCfarThreads + 2 * max_runs) * 4 > (size_t)(tile + 2 * (p.guard + p.train)) * 8) return -1;
-    (void)need_runs;
-    if (lds > 160 * 1024) return -1;
+    const size_t lds = argus_cfar_lds_bytes(p);  // at most 82 KB (guard + train <= 2048)
     gpud_cfar_params pp = p;
     if (pp.scale_db <= 0) pp.scale_db = 6;  // cfar.go:13-15
+    // per frame n/64 masks of 16 bytes, then one float (the frame's largest dB value)
+    const size_t mask_bytes = (size_t)n_frames * (size_t)(s->n / 64) * sizeof(ulonglong2);
+    const size_t need = mask_bytes + (size_t)n_frames * sizeof(float);
+    if (s->cfar_cap < need) {
+        if (s->d_cfar) hipFree(s->d_cfar);
+        s->d_cfar = nullptr;
+        s->cfar_cap = 0;
+        ARGUS_CHECK(hipMalloc(&s->d_cfar, need));
+        s->cfar_cap = need;
+    }
+    ulonglong2* masks = (ulonglong2*)s->d_cfar;
+    float* fmax = (float*)((char*)s->d_cfar + mask_bytes);
+    const double k_on = std::pow(10.0, pp.scale_db / 10.0);
+    const double k_off = std::pow(10.0, (pp.scale_db - pp.hysteresis_db) / 10.0);
     if (lds > 64 * 1024)
-        ARGUS_CHECK(hipFuncSetAttribute((const void*)argus_cfar_runs_kernel,
+        ARGUS_CHECK(hipFuncSetAttribute((const void*)argus_cfar_status_kernel,
                                         hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     argus_clear_error();
-    hipLaunchKernelGGL(argus_cfar_runs_kernel, dim3(n_frames), dim3(kCfarThreads), lds, s->stream, psd_dev, s->n, pp,
-                       runs_dev, run_counts_dev, max_runs, ambiguous_dev);
+    if (ambiguous_dev)
+        hipLaunchKernelGGL(argus_cfar_prep_kernel, dim3(n_frames), dim3(256), 0, s->stream, psd_dev, s->n, fmax,
+                           ambiguous_dev);
+    hipLaunchKernelGGL(argus_cfar_status_kernel, dim3((s->n / kCfarTile) * n_frames), dim3(kCfarThreads), lds, s->stream,
+                       psd_dev, s->n, pp, k_on, k_off, (const float*)fmax, ambiguous_dev, masks);
+    hipLaunchKernelGGL(argus_cfar_runs_kernel, dim3(n_frames), dim3(kCfarThreads), 0, s->stream, psd_dev, s->n, pp,
+                       (const ulonglong2*)masks, runs_dev, run_counts_dev, max_runs);
     return (int)hipGetLastError();
 }
 

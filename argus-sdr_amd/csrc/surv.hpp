@@ -12,6 +12,9 @@ struct gpud_surv {
     float* d_win = nullptr;  // symmetric Hann, n floats
     float2* d_scratch = nullptr;  // complex spectra of gpud_surv_spectrum_device
     size_t scratch_cap = 0;       // complex samples
+    // CFAR: per-frame decision masks and largest dB values (bytes)
+    void* d_cfar = nullptr;
+    size_t cfar_cap = 0;
     // decimated level (gpud_surv_level_spectrum_device): zero-stuffed frames,
     // FP64 low-pass taps and the level's Hann window for level_decim
     float2* d_zstuff = nullptr;

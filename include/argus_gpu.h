@@ -897,10 +897,12 @@ GPUD_API int gpud_surv_cfar_runs_device(gpud_surv* s, const float* psd_dev, int 
  * decision the reference's running sums (ca.go:61-66, slid along the frame
  * in FP64) could take the other way: bins whose dB value lies within the
  * bound of that summation's rounding (2 u (i + t) t lmax per side at bin i,
- * lmax the frame's largest linear cell) of the threshold or of the threshold
+ * lmax the frame's largest linear cell), plus the rounding of this
+ * library's prefix-sum window sums, of the threshold or of the threshold
  * less the hysteresis.  Zero means the frame's runs are those the reference
  * computes from the same spectrum.  OS-CFAR ranks without arithmetic: always
- * zero.  ambiguous_dev may be NULL. */
+ * zero.  ambiguous_dev may be NULL.  The engine keeps a workspace of
+ * n_frames * (n_fft / 4 + 4) bytes, grown on demand. */
 GPUD_API int gpud_surv_cfar_runs_device_ex(gpud_surv* s, const float* psd_dev, int n_frames,
                                            const gpud_cfar_params* params, gpud_surv_run* runs_dev,
                                            int* run_counts_dev, int max_runs, int* ambiguous_dev);

@@ -131,6 +131,73 @@ def test_cfar_ambiguity_flags_ties(surv, mode):
     s.Close()
 
 
+def _runs_with_hysteresis(spec, thr, hyst):
+    """Threshold crossing with hysteresis (detector.go:436-480) on the
+    oracle's running-sum thresholds: [(first, last)] in order."""
+    want, on, start = [], False, 0
+    for b in range(len(spec)):
+        nxt = True if spec[b] >= thr[b] else (False if spec[b] < thr[b] - hyst else on)
+        if nxt and not on:
+            start = b
+        if on and not nxt:
+            want.append((start, b - 1))
+        on = nxt
+    if on:
+        want.append((start, len(spec) - 1))
+    return want
+
+
+@pytest.mark.parametrize("n,guard,train,wrap", [(4096, 4, 16, True), (4096, 0, 3, False), (32768, 40, 260, True),
+                                                 (1024, 2, 8, False)])
+def test_cfar_tiles_halos_and_edges_against_oracle(surv, n, guard, train, wrap):
+    """The status kernel works on tiles of 1024 bins with a halo of guard +
+    train cells on each side: runs that cross tile boundaries, halos larger
+    than a wavefront's row, clamped and wrapped edges and runs open at both
+    ends of a frame give the oracle's runs, peaks and noise estimates; the
+    per-frame run count is the true count when max_runs truncates the list."""
+    rng = np.random.default_rng(n + guard + train)
+    frames = 5
+    psd = (-90.0 + 3.0 * rng.standard_normal((frames, n))).astype(np.float32)
+    for f in range(frames):
+        for _ in range(n // 256):
+            c, w = int(rng.integers(0, n)), int(rng.integers(1, 80))
+            psd[f, c:c + w] += np.float32(rng.uniform(6.0, 40.0))
+        # overlapping carriers stack up; capped, the frame's largest cell stays
+        # within 60 dB of the noise and the near-tie bound far below every
+        # bin's margin (at least 8 times, computed on the CPU)
+        np.minimum(psd[f], np.float32(-48.0), out=psd[f])
+        for b in range(1024, n, 1024):  # plateaus across every tile boundary
+            psd[f, b - 3:b + 4] = np.float32(-40.0 - f)
+    psd[1, :5] = -30.0
+    psd[1, -4:] = -30.0
+    s = surv.Surveillance(n, FS)
+    for mode in ("CA", "GOSCA", "CASO", "OS"):
+        if mode == "OS" and train > 128:
+            continue
+        cfar = surv.cfar_params(mode, guard=guard, train=train, scale_db=9.0, wrap=wrap, hysteresis_db=3.0,
+                                rank=train)
+        counts, runs, amb = s.CFARRuns(psd, cfar, ambiguous=True)
+        for f in range(frames):
+            spec = psd[f].astype(np.float64)
+            thr = O.CFARThresholds(mode, spec, guard, train, 9.0, wrap=wrap, rank=train)
+            want = _runs_with_hysteresis(spec, thr, 3.0)
+            assert amb[f] == 0, (mode, f, int(amb[f]))
+            got = [(r.first_bin, r.last_bin) for r in runs[f]]
+            assert int(counts[f]) == len(want) and got == want, (mode, f, got[:6], want[:6])
+            for r, (a, b) in zip(runs[f], want):
+                seg = np.arange(a, b + 1)
+                above = seg[spec[seg] >= thr[seg]]
+                pk = int(above[np.argmax(spec[above])])
+                assert r.peak_bin == pk and r.peak_db == spec[pk], (mode, f, r.peak_bin, pk)
+                assert abs(r.noise_db - (thr[pk] - 9.0)) < 1e-9, (mode, f, r.noise_db, thr[pk] - 9.0)
+        # truncated lists keep the true count and the first max_runs runs
+        c2, runs2 = s.CFARRuns(psd, cfar, max_runs=3)
+        assert list(c2) == list(counts)
+        for f in range(frames):
+            assert [(r.first_bin, r.last_bin) for r in runs2[f]] == [(r.first_bin, r.last_bin) for r in runs[f][:3]]
+    s.Close()
+
+
 def test_cfar_runs_known_plateau(surv):
     n = 4096
     s = surv.Surveillance(n, FS)
