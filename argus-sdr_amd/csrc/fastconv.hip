@@ -454,22 +454,27 @@ __device__ __forceinline__ void fast_block(const ArgusFastDesc& d, const float2*
     for (int sb = s_begin + w; sb < s_end; sb += kWaves * Z) fast_group<DP>(d, fo, X, gs, xs, sb, s_end, t, tw);
 }
 
-__global__ __launch_bounds__(kChanThreads) void argus_fast_chan_kernel(const ArgusFastHdr* __restrict__ hdr)
+__global__ __launch_bounds__(kChanThreads) void argus_fast_chan_kernel(const ArgusFastHdr* __restrict__ hdr, int part)
 {
     __shared__ __attribute__((aligned(16))) float2 gs[kN];
     __shared__ __attribute__((aligned(16))) float xs[kWaves][kXsFloats];
     static_assert(kXsFloats >= 2 * kJ, "the folded bins fit the transform scratch");
     static_assert(kXsFloats >= 9 * 8 * 8 && kXsFloats >= 8 * kXStride, "the transposes fit the scratch");
     const int w = threadIdx.x >> 6, t = threadIdx.x & 63;
-    const int S = hdr->S, n_fast = hdr->n_fast;
+    const int S = hdr->S;
+    int c_begin = 0, c_end = hdr->n_fast;  // the channels of this launch
+    if (part >= 0) {
+        c_begin = part > 0 ? hdr->part_end[part - 1] : 0;
+        c_end = hdr->part_end[part];
+    }
     const int blocks = (S + kSegItem - 1) / kSegItem;
-    const int n_items = n_fast * blocks;
+    const int n_items = (c_end - c_begin) * blocks;
     const float2* X = hdr->X;
     Tw512 tw;
     tw512_init(tw, t);
     for (int it = blockIdx.x; it < n_items; it += gridDim.x) {
-        const int c = it / blocks, b = it - c * blocks;
-        const ArgusFastDesc d = hdr->descs[c];
+        const int ci = it / blocks, b = it - ci * blocks;
+        const ArgusFastDesc d = hdr->descs[c_begin + ci];
         __syncthreads();  // the previous item's waves are done with gs
         // G with the ramp e^{j 2pi k r / N}: exact turn fractions
         for (int k = threadIdx.x; k < kN; k += kChanThreads) {
@@ -551,18 +556,31 @@ void argus_fast_spectrum(const float* h, int T, uint64_t inc, float2* G)
     }
 }
 
-int argus_launch_fast(const ArgusFastHdr* d_hdr, const ArgusExtractHdr* d_xh, int S, float2* u, float2* X,
-                      int grid, hipStream_t stream)
+int argus_launch_fast_input(const ArgusFastHdr* d_hdr, const ArgusExtractHdr* d_xh, int S, float2* u, float2* X,
+                            hipStream_t stream)
 {
     if (S <= 0) return 0;
     const int len = (S - 1) * kHop + kN;
     argus_clear_error();
     argus_fast_stage_kernel<<<(len + kThreads - 1) / kThreads, kThreads, 0, stream>>>(d_hdr, d_xh);
     ARGUS_CHECK(hipGetLastError());
-    int err = argus_launch_fft4096(u, kHop, S, X, stream);
-    if (err) return err;
-    argus_fast_chan_kernel<<<grid > 0 ? grid : ARGUS_FAST_GRID_DEFAULT, kChanThreads, 0, stream>>>(d_hdr);
+    return argus_launch_fft4096(u, kHop, S, X, stream);
+}
+
+int argus_launch_fast_chans(const ArgusFastHdr* d_hdr, int S, int grid, int part, hipStream_t stream)
+{
+    if (S <= 0) return 0;
+    if (part >= ARGUS_FAST_PARTS) return -1;
+    argus_clear_error();
+    argus_fast_chan_kernel<<<grid > 0 ? grid : ARGUS_FAST_GRID_DEFAULT, kChanThreads, 0, stream>>>(d_hdr, part);
     return (int)hipGetLastError();
+}
+
+int argus_launch_fast(const ArgusFastHdr* d_hdr, const ArgusExtractHdr* d_xh, int S, float2* u, float2* X,
+                      int grid, hipStream_t stream)
+{
+    const int err = argus_launch_fast_input(d_hdr, d_xh, S, u, X, stream);
+    return err ? err : argus_launch_fast_chans(d_hdr, S, grid, -1, stream);
 }
 
 int argus_launch_fast_tail(const ArgusFastHdr* d_hdr, const ArgusExtractHdr* d_xh, hipStream_t stream)

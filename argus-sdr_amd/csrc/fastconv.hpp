@@ -11,6 +11,10 @@
 // Longest block the fast path takes: its output indexing divides block
 // positions by float reciprocals, exact to one correction step below 2^22.
 #define ARGUS_FAST_MAX_BLOCK (1 << 22)
+// The streaming service runs the per-channel pass in this many launches over
+// consecutive ranges of its fast channels, so the first range's outputs can
+// start down to the host while the others are computed.
+#define ARGUS_FAST_PARTS 4
 
 // One channel on the fast path for one block.
 struct ArgusFastDesc {
@@ -43,6 +47,9 @@ struct ArgusFastHdr {
     float2* X;         // segment spectra, S x ARGUS_FAST_N
     int n_fast;        // channels on the fast path this block
     int S;             // segments this block (ceil(n / ARGUS_FAST_HOP))
+    // part p of the per-channel pass takes descs [part_end[p-1], part_end[p])
+    // (part_end[-1] = 0); non-decreasing, the last equal to n_fast
+    int part_end[ARGUS_FAST_PARTS];
 };
 
 // Whether a channel of decimation D and T taps takes the fast path: a
@@ -94,6 +101,12 @@ void argus_fast_spectrum(const float* h, int T, uint64_t inc, float2* G);
 // per service), passed again for the transform launch.
 int argus_launch_fast(const ArgusFastHdr* d_hdr, const ArgusExtractHdr* d_xh, int S, float2* u, float2* X,
                       int grid, hipStream_t stream);
+// The two halves of argus_launch_fast: the staged input and its segment
+// spectra; the per-channel pass over every fast channel (part -1) or over
+// part p's range of them (d_hdr->part_end).
+int argus_launch_fast_input(const ArgusFastHdr* d_hdr, const ArgusExtractHdr* d_xh, int S, float2* u, float2* X,
+                            hipStream_t stream);
+int argus_launch_fast_chans(const ArgusFastHdr* d_hdr, int S, int grid, int part, hipStream_t stream);
 // The stream's tail after the block (every block, fast channels or not).
 int argus_launch_fast_tail(const ArgusFastHdr* d_hdr, const ArgusExtractHdr* d_xh, hipStream_t stream);
 // surv.hip: forward 4096-point transforms of n_frames frames at in + f * frame_step.

@@ -371,7 +371,7 @@ GPUD_API int gpud_serve_create(int sample_rate, int block_samples, uint64_t hbm_
         ARGUS_CHECK(hipEventCreate(&sl.ev_start));
         ARGUS_CHECK(hipEventCreate(&sl.ev_kstart));
         ARGUS_CHECK(hipEventCreate(&sl.ev_xend));  // timed: extract_ms (recorded inside the block's graph)
-        ARGUS_CHECK(hipEventCreateWithFlags(&sl.ev_early, hipEventDisableTiming));
+        for (auto& ev : sl.ev_part) ARGUS_CHECK(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
         ARGUS_CHECK(dev_alloc(&sl.d_in, (size_t)block_samples));
         ARGUS_CHECK(host_alloc(&sl.h_in, (size_t)block_samples));
     }
@@ -406,7 +406,7 @@ GPUD_API int gpud_serve_destroy(gpud_serve* s)
         hipEventDestroy(sl.ev_start);
         hipEventDestroy(sl.ev_kstart);
         hipEventDestroy(sl.ev_xend);
-        hipEventDestroy(sl.ev_early);
+        for (hipEvent_t ev : sl.ev_part) hipEventDestroy(ev);
         hipFree(sl.d_in);
         hipHostFree(sl.h_in);
         slot_release(sl);
@@ -822,6 +822,9 @@ static int describe_block(gpud_serve* s, ServeSlot& sl, const std::vector<ServeO
     for (const ServeOut& o : outs) n_audio += o.a_index >= 0 ? 1 : 0;
     s->adescs.resize((size_t)n_audio);
     s->fdescs.clear();
+    s->f_ditems.clear();
+    s->f_iq_end.clear();
+    s->f_au_end.clear();
     s->pdescs.clear();
     s->pitems.clear();
     size_t lds = 0;
@@ -911,6 +914,13 @@ static int describe_block(gpud_serve* s, ServeSlot& sl, const std::vector<ServeO
                     di.push_back({ci, st, std::min(ARGUS_DEMOD_TILE, o.n_pre - st), ARGUS_DEMOD_ITEM_OUT});
             }
         }
+        if (fast) {  // early outputs are laid out in channel order from 0
+            const int64_t pi = s->f_iq_end.empty() ? 0 : s->f_iq_end.back();
+            const int64_t pa = s->f_au_end.empty() ? 0 : s->f_au_end.back();
+            s->f_ditems.push_back((int)s->ditems_e.size());
+            s->f_iq_end.push_back(o.early ? o.iq_off + o.n_iq : pi);
+            s->f_au_end.push_back(o.early ? o.audio_off + o.n_audio : pa);
+        }
         if (c.stereo) {
             if (o.wfm_index < 0 || o.wfm_index >= n_wfm) return -1;
             wfm_describe(c, o, sl, s->wdescs[(size_t)o.wfm_index], s->witems);
@@ -967,8 +977,21 @@ static int describe_block(gpud_serve* s, ServeSlot& sl, const std::vector<ServeO
     ArgusServeHdr& hdr = *(ArgusServeHdr*)(sl.h_tab + off_hdr);
     const char* d = sl.d_tab;
     hdr.x = ArgusExtractHdr{(const ArgusChanDesc*)(d + off_chans), sl.d_items, in, n_items, n};
-    hdr.de = ArgusDemodHdr{(const ArgusDemodDesc*)(d + off_ddesc), (const ArgusDemodItem*)(d + off_ditems), n_ditems_e,
-                           0};
+    // the fast channels in ARGUS_FAST_PARTS parts of equal counts; part p's
+    // early demodulation items and output ranges follow from the channel order
+    int part_end[ARGUS_FAST_PARTS];
+    int e_prev = 0;
+    for (int p = 0; p < ARGUS_FAST_PARTS; p++) {
+        const int k = (int)(((int64_t)n_fast * (p + 1)) / ARGUS_FAST_PARTS);
+        part_end[p] = k;
+        const int e_end = k ? s->f_ditems[(size_t)k - 1] : 0;
+        hdr.de[p] = ArgusDemodHdr{(const ArgusDemodDesc*)(d + off_ddesc), (const ArgusDemodItem*)(d + off_ditems) + e_prev,
+                                  e_end - e_prev, 0};
+        e_prev = e_end;
+        sl.iq_part_end[p] = k ? s->f_iq_end[(size_t)k - 1] : 0;
+        sl.audio_part_end[p] = k ? s->f_au_end[(size_t)k - 1] : 0;
+    }
+    if (e_prev != n_ditems_e) return -1;  // every early item belongs to a fast channel
     hdr.d = ArgusDemodHdr{(const ArgusDemodDesc*)(d + off_ddesc), (const ArgusDemodItem*)(d + off_ditems) + n_ditems_e,
                           n_ditems - n_ditems_e, 0};
     hdr.w = ArgusWfmHdr{(const ArgusWfmDesc*)(d + off_wdesc), (const ArgusWfmItem*)(d + off_witems), n_wfm, n_witems};
@@ -977,6 +1000,7 @@ static int describe_block(gpud_serve* s, ServeSlot& sl, const std::vector<ServeO
     // (channels come and go without changing what the captured graph holds)
     sl.fast_S = s->fast && s->n_fast_capable > 0 ? argus_fast_segments(n) : 0;
     hdr.f = ArgusFastHdr{(const ArgusFastDesc*)(d + off_fdesc), s->d_ftail, s->d_fu, s->d_fX, n_fast, sl.fast_S};
+    std::memcpy(hdr.f.part_end, part_end, sizeof(part_end));
     sl.a_mode = n_audio > 0 ? 1 : 0;
     hdr.p = ArgusPreHdr{(const ArgusPreDesc*)(d + off_pdesc), (const ArgusPreItem*)(d + off_pitems), n_pitems, 0};
     sl.p_mode = n_pitems > 0 ? 1 : 0;
@@ -1057,24 +1081,26 @@ static int upload_table(gpud_serve* s, ServeSlot& sl)
     return 0;
 }
 
-// The block's launches in two stages, each reading its counts and pointers
-// from the table header on the device: the fused extraction, then the
-// demodulation and (with stereo channels) the stereo stage.
-// The block's launches in four stages, each reading its counts and pointers
-// from the table header on the device: the fast form's kernels; the early
-// channels' demodulation; the direct form's extraction and the tail; then
-// the other channels' pre-demod stage, demodulation, stereo and audio.
+// The block's launches in ARGUS_FAST_PARTS + 2 stages, each reading its
+// counts and pointers from the table header on the device: parts 0..P-1 of
+// the fast form (part 0 also stages the input and transforms the segments),
+// each with its early channels' demodulation; the direct form's extraction
+// and the tail; then the other channels' pre-demod stage, demodulation,
+// stereo and audio.
+constexpr int kServeStages = ARGUS_FAST_PARTS + 2;
+constexpr int kStageDirect = ARGUS_FAST_PARTS;
+
 static int launch_stage(gpud_serve* s, ServeSlot& sl, int stage, bool wfm)
 {
     const ArgusServeHdr* dh = (const ArgusServeHdr*)(sl.d_tab + sl.off_hdr);
-    switch (stage) {
-    case 0: return argus_launch_fast(&dh->f, &dh->x, sl.fast_S, s->d_fu, s->d_fX, s->tune.fast_grid, s->compute);
-    case 1: return argus_launch_demod_hdr(&dh->de, s->compute);
-    case 2: {
+    if (stage < ARGUS_FAST_PARTS) {
+        int err = stage == 0 ? argus_launch_fast_input(&dh->f, &dh->x, sl.fast_S, s->d_fu, s->d_fX, s->compute) : 0;
+        if (!err) err = argus_launch_fast_chans(&dh->f, sl.fast_S, s->tune.fast_grid, stage, s->compute);
+        return err ? err : argus_launch_demod_hdr(&dh->de[stage], s->compute);
+    }
+    if (stage == kStageDirect) {
         int err = argus_launch_extract_hdr(&dh->x, sl.lds, s->tune.grid, s->compute);
         return err ? err : argus_launch_fast_tail(&dh->f, &dh->x, s->compute);
-    }
-    default: break;
     }
     int err = sl.p_mode ? argus_launch_predemod_hdr(&dh->p, s->compute) : 0;
     if (err) return err;
@@ -1087,9 +1113,9 @@ static int launch_stage(gpud_serve* s, ServeSlot& sl, int stage, bool wfm)
     return sl.a_mode ? argus_launch_audio_hdr(&dh->a, s->compute) : 0;
 }
 
-// Run the block's four stages, recording ev_early after the second (the
-// early channels' outputs are complete: their download may start) and
-// ev_xend after the third (the extraction's end, for extract_ms).  In graph
+// Run the block's stages, recording ev_part[p] after fast part p (its early
+// channels' outputs are complete: their download may start) and ev_xend
+// after the direct form (the extraction's end, for extract_ms).  In graph
 // mode each stage is the replay of a captured per-slot graph, captured again
 // when what it depends on (table address, LDS size, grid, which stages run)
 // has changed; the events stay outside the graphs.
@@ -1097,11 +1123,11 @@ static int launch_block(gpud_serve* s, ServeSlot& sl)
 {
     const bool wfm = sl.n_wfm > 0;
     if (!s->graph) {
-        for (int stage = 0; stage < 4; stage++) {
+        for (int stage = 0; stage < kServeStages; stage++) {
             int err = launch_stage(s, sl, stage, wfm);
             if (err) return err;
-            if (stage == 1) ARGUS_CHECK(hipEventRecord(sl.ev_early, s->compute));
-            if (stage == 2) ARGUS_CHECK(hipEventRecord(sl.ev_xend, s->compute));
+            if (stage < ARGUS_FAST_PARTS) ARGUS_CHECK(hipEventRecord(sl.ev_part[stage], s->compute));
+            if (stage == kStageDirect) ARGUS_CHECK(hipEventRecord(sl.ev_xend, s->compute));
         }
         return 0;
     }
@@ -1116,7 +1142,7 @@ static int launch_block(gpud_serve* s, ServeSlot& sl)
             }
         int lerr = argus_extract_allow_lds(sl.lds);  // outside capture
         if (lerr) return lerr;
-        for (int stage = 0; stage < 4; stage++) {
+        for (int stage = 0; stage < kServeStages; stage++) {
             ARGUS_CHECK(hipStreamBeginCapture(s->compute, hipStreamCaptureModeThreadLocal));
             int err = launch_stage(s, sl, stage, wfm);
             hipGraph_t g = nullptr;
@@ -1142,10 +1168,10 @@ static int launch_block(gpud_serve* s, ServeSlot& sl)
         sl.g_fastS = sl.fast_S;
         sl.g_pre = sl.p_mode;
     }
-    for (int stage = 0; stage < 4; stage++) {
+    for (int stage = 0; stage < kServeStages; stage++) {
         ARGUS_CHECK(hipGraphLaunch(sl.gexec[stage], s->compute));
-        if (stage == 1) ARGUS_CHECK(hipEventRecord(sl.ev_early, s->compute));
-        if (stage == 2) ARGUS_CHECK(hipEventRecord(sl.ev_xend, s->compute));
+        if (stage < ARGUS_FAST_PARTS) ARGUS_CHECK(hipEventRecord(sl.ev_part[stage], s->compute));
+        if (stage == kStageDirect) ARGUS_CHECK(hipEventRecord(sl.ev_xend, s->compute));
     }
     return 0;
 }
@@ -1344,18 +1370,23 @@ static int serve_submit(gpud_serve* s, const gpud_float2* iq, int n, bool on_dev
     ARGUS_CHECK(hipEventRecord(sl.ev_kern, s->compute));
     commit_block(s, sl.outs, n, sl.ticket);
     // ---- copy stream: download ------------------------------------------
-    // the early channels' outputs as soon as they are complete, while the
-    // direct form runs; then the rest
+    // each fast part's early outputs as soon as they are complete, while the
+    // later parts and the direct form run; then the rest
     const int dl = s->dl_mask;
-    const bool early = ((dl & GPUD_DL_IQ) && sl.iq_early > 0) || ((dl & GPUD_DL_DEMOD) && sl.audio_early > 0);
-    if (early) {
-        ARGUS_CHECK(hipStreamWaitEvent(s->copy, sl.ev_early, 0));
-        if ((dl & GPUD_DL_IQ) && sl.iq_early > 0)
-            ARGUS_CHECK(hipMemcpyAsync(sl.h_iq, sl.d_iq, (size_t)sl.iq_early * sizeof(float2),
+    int64_t iq_lo = 0, au_lo = 0;
+    for (int p = 0; p < ARGUS_FAST_PARTS; p++) {
+        const int64_t iq_hi = sl.iq_part_end[p], au_hi = sl.audio_part_end[p];
+        const bool iq_part = (dl & GPUD_DL_IQ) && iq_hi > iq_lo;
+        const bool au_part = (dl & GPUD_DL_DEMOD) && au_hi > au_lo;
+        if (iq_part || au_part) ARGUS_CHECK(hipStreamWaitEvent(s->copy, sl.ev_part[p], 0));
+        if (iq_part)
+            ARGUS_CHECK(hipMemcpyAsync(sl.h_iq + iq_lo, sl.d_iq + iq_lo, (size_t)(iq_hi - iq_lo) * sizeof(float2),
                                        hipMemcpyDeviceToHost, s->copy));
-        if ((dl & GPUD_DL_DEMOD) && sl.audio_early > 0)
-            ARGUS_CHECK(hipMemcpyAsync(sl.h_audio, sl.d_audio, (size_t)sl.audio_early * sizeof(float),
+        if (au_part)
+            ARGUS_CHECK(hipMemcpyAsync(sl.h_audio + au_lo, sl.d_audio + au_lo, (size_t)(au_hi - au_lo) * sizeof(float),
                                        hipMemcpyDeviceToHost, s->copy));
+        iq_lo = std::max(iq_lo, iq_hi);
+        au_lo = std::max(au_lo, au_hi);
     }
     ARGUS_CHECK(hipStreamWaitEvent(s->copy, sl.ev_kern, 0));
     if ((dl & GPUD_DL_IQ) && sl.iq_total > sl.iq_early)

@@ -99,7 +99,8 @@ struct ServeSlot {
     int n = 0;
     hipEvent_t ev_start = nullptr, ev_in = nullptr, ev_kstart = nullptr, ev_kern = nullptr, ev_done = nullptr;
     hipEvent_t ev_xend = nullptr;  // after the extraction launch
-    hipEvent_t ev_early = nullptr; // after the fast kernels and the early channels' demodulation
+    // after part p of the fast kernels and its early channels' demodulation
+    hipEvent_t ev_part[ARGUS_FAST_PARTS] = {};
     float2* d_in = nullptr;
     float2* h_in = nullptr;
     float2* d_iq = nullptr;
@@ -134,7 +135,7 @@ struct ServeSlot {
     bool items_dirty = true;
     // captured launches of this slot (extraction; demodulation + stereo
     // stage) and what they were captured for
-    hipGraphExec_t gexec[4] = {nullptr, nullptr, nullptr, nullptr};
+    hipGraphExec_t gexec[ARGUS_FAST_PARTS + 2] = {};
     const char* g_hdr = nullptr;
     size_t g_lds = 0;
     int g_grid = 0;
@@ -157,6 +158,8 @@ struct ServeSlot {
     int64_t rds_total = 0;
     int64_t iq_total = 0, audio_total = 0;
     int64_t iq_early = 0, audio_early = 0;  // the early channels' share (BlockLayout)
+    // end of the early outputs of fast-kernel parts 0..p (describe_block)
+    int64_t iq_part_end[ARGUS_FAST_PARTS] = {}, audio_part_end[ARGUS_FAST_PARTS] = {};
     // post-demodulation audio stage
     float* d_aout = nullptr;
     size_t aout_cap = 0;
@@ -195,8 +198,9 @@ struct ServeSlot {
 // A block's output layout.  Channels whose outputs are complete once the
 // fast-convolution kernels and their demodulation have run (fast form this
 // block, plain demodulation: no pre-demod, stereo or audio stage) come first
-// in the IQ and demodulated buffers, [0, iq_early) and [0, audio_early), so
-// their download starts while the direct form still runs.
+// in the IQ and demodulated buffers, [0, iq_early) and [0, audio_early), in
+// channel order, so each part of the fast kernels' outputs starts down while
+// the later parts and the direct form still run.
 struct BlockLayout {
     std::vector<ServeOut> outs;
     int n_wfm = 0;
@@ -225,7 +229,7 @@ struct ServePrep {
 struct ArgusServeHdr {
     ArgusExtractHdr x;
     ArgusDemodHdr d;   // demodulation of the channels that are not early
-    ArgusDemodHdr de;  // demodulation of the early channels
+    ArgusDemodHdr de[ARGUS_FAST_PARTS];  // demodulation of the early channels, per fast-kernel part
     ArgusWfmHdr w;
     ArgusAudioHdr a;
     ArgusFastHdr f;
@@ -290,6 +294,10 @@ struct gpud_serve {
     std::vector<ArgusWfmDesc> wdescs;
     std::vector<ArgusDemodItem> ditems;
     std::vector<ArgusDemodItem> ditems_e;  // the early channels' (serve.hip: layout_block)
+    // per fast channel of the block being described: early demod items and
+    // early output ends up to and including it
+    std::vector<int> f_ditems;
+    std::vector<int64_t> f_iq_end, f_au_end;
     std::vector<ArgusWfmItem> witems;
     std::vector<ArgusPreDesc> pdescs;
     std::vector<ArgusPreItem> pitems;
