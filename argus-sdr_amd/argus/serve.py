@@ -49,6 +49,7 @@ register("gpud_serve_submit", I, [P, P, I, C.POINTER(C.c_int64)])
 register("gpud_serve_submit_device", I, [P, P, I, C.POINTER(C.c_int64)])
 register("gpud_serve_set_download", I, [P, I])
 register("gpud_serve_set_download_parts", I, [P, I])
+register("gpud_serve_set_download_split", I, [P, I, C.c_int64])
 register("gpud_serve_poll", I, [P, C.c_int64])
 register("gpud_serve_counters", I, [P, C.POINTER(C.c_int64), C.POINTER(C.c_uint64)])
 DL_IQ, DL_DEMOD, DL_WFM, DL_AUDIO, DL_PREDEMOD, DL_ALL = 1, 2, 4, 8, 16, 31
@@ -149,6 +150,12 @@ class Serve:
     def SetDownloadParts(self, mask: int) -> None:
         """Which outputs come to host memory (DL_* bits)."""
         check("gpud_serve_set_download_parts", lib().gpud_serve_set_download_parts(self._h, mask))
+
+    def SetDownloadSplit(self, max_parts: int, min_part_bytes: int) -> None:
+        """The early fast-form outputs in up to max_parts launches, one per
+        min_part_bytes downloaded, each range downloaded as its launch ends."""
+        check("gpud_serve_set_download_split",
+              lib().gpud_serve_set_download_split(self._h, max_parts, min_part_bytes))
 
     def Poll(self, ticket: int) -> bool:
         """True once the block's outputs are on the host (non-blocking)."""

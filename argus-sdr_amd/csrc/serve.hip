@@ -977,12 +977,22 @@ static int describe_block(gpud_serve* s, ServeSlot& sl, const std::vector<ServeO
     ArgusServeHdr& hdr = *(ArgusServeHdr*)(sl.h_tab + off_hdr);
     const char* d = sl.d_tab;
     hdr.x = ArgusExtractHdr{(const ArgusChanDesc*)(d + off_chans), sl.d_items, in, n_items, n};
-    // the fast channels in ARGUS_FAST_PARTS parts of equal counts; part p's
+    // the fast channels in n_parts parts of equal counts, one per
+    // part_min_bytes of early outputs the block downloads (at most
+    // max_parts; parts after n_parts are empty and not launched); part p's
     // early demodulation items and output ranges follow from the channel order
+    {
+        const int64_t iq_e = s->f_iq_end.empty() ? 0 : s->f_iq_end.back();
+        const int64_t au_e = s->f_au_end.empty() ? 0 : s->f_au_end.back();
+        const int64_t dl_bytes = ((s->dl_mask & GPUD_DL_IQ) ? (int64_t)sizeof(float2) * iq_e : 0) +
+                                 ((s->dl_mask & GPUD_DL_DEMOD) ? (int64_t)sizeof(float) * au_e : 0);
+        const int64_t by_bytes = s->part_min_bytes > 0 ? dl_bytes / s->part_min_bytes : s->max_parts;
+        sl.n_parts = (int)std::max<int64_t>(1, std::min<int64_t>(s->max_parts, by_bytes));
+    }
     int part_end[ARGUS_FAST_PARTS];
     int e_prev = 0;
     for (int p = 0; p < ARGUS_FAST_PARTS; p++) {
-        const int k = (int)(((int64_t)n_fast * (p + 1)) / ARGUS_FAST_PARTS);
+        const int k = p < sl.n_parts ? (int)(((int64_t)n_fast * (p + 1)) / sl.n_parts) : n_fast;
         part_end[p] = k;
         const int e_end = k ? s->f_ditems[(size_t)k - 1] : 0;
         hdr.de[p] = ArgusDemodHdr{(const ArgusDemodDesc*)(d + off_ddesc), (const ArgusDemodItem*)(d + off_ditems) + e_prev,
@@ -1124,6 +1134,7 @@ static int launch_block(gpud_serve* s, ServeSlot& sl)
     const bool wfm = sl.n_wfm > 0;
     if (!s->graph) {
         for (int stage = 0; stage < kServeStages; stage++) {
+            if (stage >= sl.n_parts && stage < ARGUS_FAST_PARTS) continue;  // an empty part
             int err = launch_stage(s, sl, stage, wfm);
             if (err) return err;
             if (stage < ARGUS_FAST_PARTS) ARGUS_CHECK(hipEventRecord(sl.ev_part[stage], s->compute));
@@ -1169,6 +1180,7 @@ static int launch_block(gpud_serve* s, ServeSlot& sl)
         sl.g_pre = sl.p_mode;
     }
     for (int stage = 0; stage < kServeStages; stage++) {
+        if (stage >= sl.n_parts && stage < ARGUS_FAST_PARTS) continue;  // an empty part
         ARGUS_CHECK(hipGraphLaunch(sl.gexec[stage], s->compute));
         if (stage < ARGUS_FAST_PARTS) ARGUS_CHECK(hipEventRecord(sl.ev_part[stage], s->compute));
         if (stage == kStageDirect) ARGUS_CHECK(hipEventRecord(sl.ev_xend, s->compute));
@@ -1374,7 +1386,7 @@ static int serve_submit(gpud_serve* s, const gpud_float2* iq, int n, bool on_dev
     // later parts and the direct form run; then the rest
     const int dl = s->dl_mask;
     int64_t iq_lo = 0, au_lo = 0;
-    for (int p = 0; p < ARGUS_FAST_PARTS; p++) {
+    for (int p = 0; p < sl.n_parts; p++) {
         const int64_t iq_hi = sl.iq_part_end[p], au_hi = sl.audio_part_end[p];
         const bool iq_part = (dl & GPUD_DL_IQ) && iq_hi > iq_lo;
         const bool au_part = (dl & GPUD_DL_DEMOD) && au_hi > au_lo;
@@ -1449,6 +1461,14 @@ GPUD_API int gpud_serve_set_download_parts(gpud_serve* s, int mask)
 {
     if (!s || (mask & ~GPUD_DL_ALL)) return -1;
     s->dl_mask = mask;
+    return 0;
+}
+
+GPUD_API int gpud_serve_set_download_split(gpud_serve* s, int max_parts, int64_t min_part_bytes)
+{
+    if (!s || max_parts < 1 || max_parts > ARGUS_FAST_PARTS || min_part_bytes < 0) return -1;
+    s->max_parts = max_parts;
+    s->part_min_bytes = min_part_bytes;
     return 0;
 }
 

@@ -160,6 +160,7 @@ struct ServeSlot {
     int64_t iq_early = 0, audio_early = 0;  // the early channels' share (BlockLayout)
     // end of the early outputs of fast-kernel parts 0..p (describe_block)
     int64_t iq_part_end[ARGUS_FAST_PARTS] = {}, audio_part_end[ARGUS_FAST_PARTS] = {};
+    int n_parts = 1;  // fast-kernel parts launched for the described block (the rest are empty)
     // post-demodulation audio stage
     float* d_aout = nullptr;
     size_t aout_cap = 0;
@@ -277,6 +278,10 @@ struct gpud_serve {
     uint64_t budget = 0;
     uint64_t bytes_in_use = 0;
     int dl_mask = GPUD_DL_ALL;  // parts of each block's outputs downloaded (GPUD_DL_*)
+    // the fast form runs in up to max_parts launches, one per part_min_bytes
+    // of early outputs downloaded (gpud_serve_set_download_split)
+    int max_parts = ARGUS_FAST_PARTS;
+    int64_t part_min_bytes = (int64_t)16 << 20;
     hipStream_t compute = nullptr, copy = nullptr;
     std::map<int64_t, ServeChannel> live;  // ordered by id: a deterministic channel order per block
     std::vector<ServeChannel> closed;     // awaiting their last block

@@ -459,6 +459,15 @@ enum {
     GPUD_DL_ALL = 31
 };
 GPUD_API int gpud_serve_set_download_parts(gpud_serve* s, int mask);
+/* No counterpart in the reference (its extraction returns outputs per call).
+ * The fast-convolution channels whose outputs are complete after their own
+ * kernels (no pre-demod, stereo or audio stage) are computed in up to
+ * max_parts (1..4) launches over consecutive channel ranges, one launch per
+ * min_part_bytes of such outputs the block downloads (GPUD_DL_IQ and
+ * GPUD_DL_DEMOD), and each range's outputs start down to the host as soon as
+ * its launch ends.  Defaults 4 and 16 MiB; min_part_bytes 0 always uses
+ * max_parts.  Outputs are the same whatever the split.  -1 on bad args. */
+GPUD_API int gpud_serve_set_download_split(gpud_serve* s, int max_parts, int64_t min_part_bytes);
 /* Non-blocking completion test of a submitted block: 1 when its outputs
  * (those downloaded) are on the host, 0 while it runs, -2 unknown ticket. */
 GPUD_API int gpud_serve_poll(gpud_serve* s, int64_t ticket);

@@ -9,7 +9,9 @@ rounding but not bit for bit (so the fast form did run).
 
 The demodulation fused into the fast kernel's epilogue (the default) gives
 the same bits as the separate demodulation pass over the same baseband:
-FM with its segment seams and carry, SSB, CW, on the same ragged blocks."""
+FM with its segment seams and carry, SSB, CW, on the same ragged blocks.
+Split into 2-4 launches over channel ranges (each range downloaded as its
+launch ends), the outputs are the same bits as in one launch."""
 import numpy as np
 import pytest
 
@@ -108,9 +110,11 @@ FUSED = [  # id, offset, bw, out_rate, taps, demod: every mode the epilogue demo
 ]
 
 
-def _run_fusion(S, fusion, x, sizes):
+def _run_fusion(S, fusion, x, sizes, split=None):
     s = S.Serve(SR, BLOCK, 1 << 30)
     s.SetDemodFusion(fusion)
+    if split is not None:
+        s.SetDownloadSplit(*split)
     for cid, off, bw, rate, taps, dm in FUSED:
         assert s.Open(S.ChannelSpec(cid, off, bw, rate, taps, dm, 0))
     out = []
@@ -120,7 +124,7 @@ def _run_fusion(S, fusion, x, sizes):
             assert s.Open(S.ChannelSpec(28, -700_000.0, 12_500, 160_000, 201, NFM, 0))
         blk = x[pos:pos + n]
         pos += n
-        print(f"fusion={fusion} block {k} n={n}", flush=True)
+        print(f"fusion={fusion} split={split} block {k} n={n}", flush=True)
         t = s.Submit(blk)
         s.Wait(t)
         cids = [c[0] for c in FUSED] + ([28] if k >= 2 else [])
@@ -148,3 +152,27 @@ def test_fused_demod_same_bits_as_separate_pass(serve_mod):
             assert np.array_equal(fa, sa), (k, cid, np.flatnonzero(fa != sa)[:8])
             n_audio += len(fa)
     assert n_audio > 100_000
+
+
+@pytest.mark.parametrize("parts", [2, 3, 4])
+def test_download_split_same_outputs(serve_mod, parts):
+    """The early channels' outputs computed and downloaded in 2-4 parts
+    (consecutive channel ranges, each downloaded as its launch ends) are
+    the same bits as in one part, with and without the fused demodulation,
+    over ragged blocks and a channel opened mid-stream."""
+    S = serve_mod
+    kinds = [("wfm", 300_000.0, 0.5), ("fm", -400_000.0, 0.5), ("ssb", -150_000.0, 0.5),
+             ("ssb", 250_000.0, 0.5), ("tone", 600_000.0, 0.5), ("fm", -250_000.0, 0.5),
+             ("am", 100_000.0, 0.5), ("fm", -700_000.0, 0.5)]
+    sizes = [BLOCK, BLOCK, 3585, 0, 12_345, BLOCK]
+    x = synth_scene(sum(sizes), SR, kinds, seed=10 + parts)
+    for fusion in (True, False):
+        one = _run_fusion(S, fusion, x, sizes, split=(1, 0))
+        many = _run_fusion(S, fusion, x, sizes, split=(parts, 0))
+        for k, (b1, bm) in enumerate(zip(one, many)):
+            assert b1.keys() == bm.keys()
+            for cid in b1:
+                for a1, am in zip(b1[cid], bm[cid]):
+                    assert len(a1) == len(am) and np.array_equal(a1, am), (fusion, k, cid)
+
+
