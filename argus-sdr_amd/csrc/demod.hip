@@ -32,6 +32,21 @@ __device__ void fm_seams(const ArgusDemodDesc& d, int e0, int D)
     if (threadIdx.x == 0 && d.n > 0) *d.carry_wr = d.iq[d.n - 1];
 }
 
+// The FM outputs m = k * M (the first of each narrow item of the direct
+// form, which demodulated the rest) and the carry.
+__device__ void fm_seams_every(const ArgusDemodDesc& d, int M)
+{
+    for (int k = threadIdx.x;; k += blockDim.x) {
+        const int64_t m = (int64_t)k * M;
+        if (m >= d.n) break;
+        if (m > 0 || d.have_carry) {
+            const float2 prev = m > 0 ? d.iq[m - 1] : *d.carry_rd;
+            d.audio[m - d.skip] = argus_fm_step(prev, d.iq[m]);
+        }
+    }
+    if (threadIdx.x == 0 && d.n > 0) *d.carry_wr = d.iq[d.n - 1];
+}
+
 __device__ void am_block(const ArgusDemodDesc& d)
 {
     __shared__ double part[ARGUS_DEMOD_TILE / 64];
@@ -66,6 +81,10 @@ __device__ __forceinline__ void argus_demod_items(const ArgusDemodDesc* __restri
         const ArgusDemodDesc d = descs[it.chan];
         if (it.kind == ARGUS_DEMOD_ITEM_SEAMS) {
             fm_seams(d, it.start, it.count);
+            continue;
+        }
+        if (it.kind == ARGUS_DEMOD_ITEM_SEAMS_EVERY) {
+            fm_seams_every(d, it.count);
             continue;
         }
         const int end = it.start + it.count;

@@ -31,7 +31,9 @@ struct ArgusDemodDesc {
 // ARGUS_DEMOD_ITEM_SEAMS) only the FM outputs the fast-convolution kernel
 // leaves to this pass when it demodulates in its epilogue: the first output
 // of each segment (whose predecessor another wavefront produced), with the
-// channel's output grid e0 = start, decimation D = count, and the FM carry.
+// channel's output grid e0 = start, decimation D = count, and the FM carry;
+// or (kind ARGUS_DEMOD_ITEM_SEAMS_EVERY) those the direct form's narrow
+// items leave: every count-th output (each item's first) and the FM carry.
 struct ArgusDemodItem {
     int chan;
     int start;
@@ -42,6 +44,7 @@ struct ArgusDemodItem {
 enum {
     ARGUS_DEMOD_ITEM_OUT = 0,
     ARGUS_DEMOD_ITEM_SEAMS = 1,
+    ARGUS_DEMOD_ITEM_SEAMS_EVERY = 2,
 };
 
 // Per-output arithmetic of the demodulators, shared by the demodulation pass

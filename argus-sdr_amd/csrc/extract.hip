@@ -396,6 +396,7 @@ __device__ __forceinline__ void argus_narrow_mod(const ArgusChanDesc& ch, const 
     const int TP = (T + 127) & ~127;
     float2* gs = lds;
     float* hs = (float*)(lds + TP);
+    float2* ys = lds + TP + (T + 1) / 2;  // the item's outputs, for the demodulation epilogue
     const bool hist = ch.e0 + m0 * D - (T - 1) < 0;  // a window of this item reaches into the history
     for (int i = threadIdx.x; i < TP; i += blockDim.x) gs[i] = i < T ? ch.ctaps[i] : make_float2(0.0f, 0.0f);
     if (hist)
@@ -461,10 +462,30 @@ __device__ __forceinline__ void argus_narrow_mod(const ArgusChanDesc& ch, const 
             float sn, cs;
             argus_nco(ch.phase0 + ch.inc * (uint64_t)(int64_t)e, sn, cs);
             const float2 y = argus_rot(make_float2(ar, ai), sn, cs);
-            ch.out[m] = make_float2(y.x + br, y.y + bi);
+            const float2 o = make_float2(y.x + br, y.y + bi);
+            ch.out[m] = o;
+            if (ch.demod) ys[m - m0] = o;
         }
     }
 #undef ARGUS_CMAC
+    if (ch.demod) {  // uniform: one channel per workgroup
+        __syncthreads();
+        // the same arithmetic as the demodulation pass (demod.hip), so the
+        // same bits; an FM item's first output needs the previous item's
+        // last, and is left to that pass with the carry
+        const bool fm = ch.demod == ARGUS_DEMOD_NFM || ch.demod == ARGUS_DEMOD_WFM;
+        for (int j = threadIdx.x + (fm ? 1 : 0); j < mend - m0; j += blockDim.x) {
+            const int m = m0 + j;
+            const float2 v = ys[j];
+            if (fm) {
+                ch.audio[m - ch.skip] = argus_fm_step(ys[j - 1], v);
+            } else {
+                float s, c;
+                argus_nco(ch.bfo0 + ch.bfo_inc * (uint64_t)(uint32_t)m, s, c);
+                ch.audio[m] = ch.demod == ARGUS_DEMOD_CW ? v.x * c : argus_ssb_out(v, s, c);
+            }
+        }
+    }
     __syncthreads();
 }
 
@@ -548,7 +569,8 @@ __global__ __launch_bounds__(ARGUS_EXTRACT_THREADS) void argus_extract_one_kerne
 size_t argus_item_lds_bytes(const ArgusChanDesc& ch)
 {
     if (!ch.narrow) return argus_dense_lds_bytes(ch);
-    return ch.ctaps ? (size_t)((ch.T + 127) & ~127) * sizeof(float2) + (size_t)ch.T * sizeof(float)
+    // modulated taps (T rounded up to 128), real taps, the item's outputs
+    return ch.ctaps ? (size_t)(((ch.T + 127) & ~127) + (ch.T + 1) / 2 + ch.M) * sizeof(float2)
                     : (size_t)ch.T * sizeof(float);
 }
 
@@ -593,9 +615,9 @@ int argus_extract_allow_lds(size_t bytes) { return bytes > 64 * 1024 ? argus_all
 size_t argus_tune_max_lds(const ArgusTune& tune, int max_taps)
 {
     // dense: P*PS <= lds_entries words of samples plus P*L <= T + P - 1 < 2*T taps;
-    // narrow: T (+ 127 padding) complex and T real taps
+    // narrow: T (+ 127 padding) complex and T real taps, and its outputs
     const size_t dense = (size_t)tune.lds_entries * sizeof(float2) + (size_t)2 * max_taps * sizeof(float);
-    const size_t narrow = (size_t)(max_taps + 127) * sizeof(float2) + (size_t)max_taps * sizeof(float);
+    const size_t narrow = (size_t)(max_taps + 127 + (max_taps + 1) / 2 + tune.narrow_outputs) * sizeof(float2);
     return std::max(dense, narrow);
 }
 
@@ -720,6 +742,10 @@ int argus_fill_chan(ArgusChanDesc* ch, const float2* hist_rd, int H, float2* his
     ch->H = H;
     ch->e0 = D - phase_count - 1;
     ch->keep = keep;
+    ch->demod = 0;  // baseband only unless the caller asks for the epilogue
+    ch->skip = 0;
+    ch->audio = nullptr;
+    ch->bfo0 = ch->bfo_inc = 0;
     // the tile shape depends on the chunk length only (planned for the most
     // outputs a chunk of n_new samples can give), so a work list built for
     // one chunk stays valid for the next of the same length

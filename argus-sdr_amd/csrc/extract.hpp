@@ -38,6 +38,7 @@
 // pair, so no item reads history another item is overwriting.
 #pragma once
 #include "common.hpp"
+#include "demod.hpp"
 #include <vector>
 
 struct ArgusChanDesc {
@@ -59,6 +60,15 @@ struct ArgusChanDesc {
     int R;                  // outputs per thread in the dense FIR phase (4; 1 for narrow)
     int S;                  // lanes per output group in the dense FIR phase (power of 2)
     int narrow;             // 1 = narrow (wave-per-output) items
+    // demodulation in the epilogue of the narrow items with modulated taps
+    // (streaming service): ARGUS_DEMOD_NFM/WFM (every output but each item's
+    // first: the demodulation pass adds those and the carry), USB/LSB or CW
+    // (every output); ARGUS_DEMOD_NONE (0): baseband only
+    int demod;
+    int skip;               // FM: 1 when output 0 has no predecessor (no carry)
+    float* audio;           // demodulated outputs (output m at audio[m - skip])
+    uint64_t bfo0;          // SSB/CW: BFO phase of output 0, pre-increment applied (turns)
+    uint64_t bfo_inc;       // SSB/CW: BFO increment per output (turns)
 };
 
 // Everything the graph-mode extraction kernel reads for one block (lives in

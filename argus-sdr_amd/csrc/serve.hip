@@ -902,11 +902,26 @@ static int describe_block(gpud_serve* s, ServeSlot& sl, const std::vector<ServeO
             fd.bfo_inc = dd.bfo_inc;
             fd.skip = dd.skip;
         }
+        // the direct form's narrow items (modulated taps) likewise
+        const bool fused_direct = fi < 0 && s->fuse_demod && ch.narrow && ch.ctaps && c.pre_factor == 1 &&
+                                  !c.stereo &&
+                                  (fm || dd.mode == ARGUS_DEMOD_USB || dd.mode == ARGUS_DEMOD_LSB ||
+                                   dd.mode == ARGUS_DEMOD_CW);
+        if (fused_direct) {
+            ch.demod = dd.mode;
+            ch.audio = dd.audio;
+            ch.bfo0 = dd.bfo0;
+            ch.bfo_inc = dd.bfo_inc;
+            ch.skip = dd.skip;
+        }
         if (dd.mode != ARGUS_DEMOD_NONE && o.n_pre > 0) {
             std::vector<ArgusDemodItem>& di = o.early ? s->ditems_e : s->ditems;
             if (fused) {
                 // FM: the segments' first outputs and the carry; SSB/CW: done
                 if (fm) di.push_back({ci, ch.e0, ch.D, ARGUS_DEMOD_ITEM_SEAMS});
+            } else if (fused_direct) {
+                // FM: each narrow item's first output and the carry; SSB/CW: done
+                if (fm) di.push_back({ci, 0, ch.M, ARGUS_DEMOD_ITEM_SEAMS_EVERY});
             } else if (dd.mode == ARGUS_DEMOD_AM) {
                 di.push_back({ci, 0, o.n_pre, ARGUS_DEMOD_ITEM_OUT});
             } else {

@@ -55,7 +55,7 @@ N_ROOF = 8192              # channels of the roofline measurement (traffic profi
 FP32_PEAK_TFLOPS = 157.3
 DEEMPH_US = 50.0           # de-emphasis time constant of the audio stage (Europe)
 METRIC = "real-time channels sustained @20 Msps span (1 GPU); p99 block latency; HBM GB/s"
-DEMOD_FUSION = [True]      # demodulation in the fast kernel's epilogue (--unfused-demod: the separate pass)
+DEMOD_FUSION = [True]      # demodulation in the extraction kernels' epilogues (--unfused-demod: the separate pass)
 
 
 def make_channels(n: int, seed: int = 0):
@@ -83,12 +83,23 @@ def fast_eligible(d, taps):
     return d % 8 == 0 and d < taps <= 512
 
 
+def direct_narrow(d, taps):
+    """Whether the direct form runs a channel as narrow items (the service's
+    plan with modulated taps: windows that do not overlap, or a dense tile
+    that would cost more)."""
+    if d >= taps:
+        return True
+    lp = ((-(-taps // d)) + 3) & ~3
+    return 22 * lp * d + 100 * d >= 55 * ((taps + 63) & ~63)
+
+
 def alg_figures(chans, n_in, fused_demod=True):
     """Algorithmic bytes and flops of one extraction stage (unique bytes):
     the input, every channel's outputs, taps and history, and the
-    shift-FIR-decimate flops; with fused_demod, the demodulation the fast
-    kernel does in its epilogue too (FM, SSB, CW on the fast form): 4 bytes
-    written and DEMOD_FLOPS flops per output."""
+    shift-FIR-decimate flops; with fused_demod, the demodulation the
+    extraction kernels do in their epilogues too (FM, SSB, CW on the fast
+    form and on the direct form's narrow items): 4 bytes written and
+    DEMOD_FLOPS flops per output."""
     b = 8 * n_in
     f = 0
     for (_, _, _, rate, taps, dm) in chans:
@@ -96,7 +107,7 @@ def alg_figures(chans, n_in, fused_demod=True):
         n_out = n_in // d
         b += n_out * 8 + 16 * (taps - 1) + 4 * taps
         f += 6 * n_in + 4 * taps * n_out
-        if fused_demod and dm in DEMOD_FLOPS and fast_eligible(d, taps):
+        if fused_demod and dm in DEMOD_FLOPS and (fast_eligible(d, taps) or direct_narrow(d, taps)):
             b += 4 * n_out
             f += DEMOD_FLOPS[dm] * n_out
     return b, f
@@ -849,7 +860,7 @@ def fast_flops(chans, n_in):
     bin), the fold of Dp groups (2 per folded bin), a 4096/Dp-point inverse
     (5 M log2 M) and the output rotations (6 per output); the segment
     transforms shared by all of them (5 N log2 N each); the direct form's
-    count for the rest."""
+    count for the rest; the demodulation done in either kernel's epilogue."""
     S_ = -(-n_in // 3584)
     total = S_ * 5 * 4096 * 12
     for (_, _, _, rate, taps, dm) in chans:
@@ -862,6 +873,8 @@ def fast_flops(chans, n_in):
             total += DEMOD_FLOPS.get(dm, 0) * n_out
         else:
             total += 6 * n_in + 4 * taps * n_out
+            if direct_narrow(d, taps):
+                total += DEMOD_FLOPS.get(dm, 0) * n_out
     return total
 
 
@@ -1097,7 +1110,7 @@ def main() -> int:
                    "modes": "NFM/AM/USB/WFM (+ demod)", "decim": "8-1600", "taps": "63-511", "seed": 0,
                    "p99_bound_ms": P99_BOUND_MS, "latency": "block handed to the ingest ring -> demodulated "
                    "outputs in host memory; blocks every 10 ms (2x real time), up to 3 in flight",
-                   "demod_fused_into_fast_kernel": DEMOD_FUSION[0]},
+                   "demod_fused_into_extraction_kernels": DEMOD_FUSION[0]},
         "p50_block_ms": round(sweep[n_rt][0], 4) if n_rt in sweep else None,
         "p99_block_ms": round(sweep[n_rt][1], 4) if n_rt in sweep else None,
         "p50_block_gpu_ms": round(percentile(gpu_at.get(n_rt, []), 50), 4) if gpu_at.get(n_rt) else None,

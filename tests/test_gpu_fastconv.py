@@ -7,9 +7,10 @@ demod mode, and ineligible channels beside them.  Baseband and audio within
 1e-4 relative RMS of the oracle; fast and direct forms agree to FP32
 rounding but not bit for bit (so the fast form did run).
 
-The demodulation fused into the fast kernel's epilogue (the default) gives
-the same bits as the separate demodulation pass over the same baseband:
-FM with its segment seams and carry, SSB, CW, on the same ragged blocks.
+The demodulation fused into the fast kernel's epilogue and into the direct
+form's narrow items (the default) gives the same bits as the separate
+demodulation pass over the same baseband: FM with its segment and item
+seams and carry, SSB, CW, on the same ragged blocks.
 Split into 2-4 launches over channel ranges (each range downloaded as its
 launch ends), the outputs are the same bits as in one launch."""
 import numpy as np
@@ -107,6 +108,13 @@ FUSED = [  # id, offset, bw, out_rate, taps, demod: every mode the epilogue demo
     (25, 600_000.0, 1_000, 320_000, 63, CW),       # D 8
     (26, -250_000.0, 12_500, 64_000, 255, NFM),    # D 40: every 5th fold position
     (27, 100_000.0, 10_000, 80_000, 65, AM),       # D 32: the separate pass
+    # the direct form's narrow items (D >= T) demodulate in their epilogue too
+    (31, 420_000.0, 12_500, 80_000, 31, NFM),      # D 32, 31 taps
+    (32, -520_000.0, 3_000, 40_000, 63, USB),      # D 64
+    (33, 820_000.0, 3_000, 64_000, 39, LSB),       # D 40
+    (34, -820_000.0, 1_000, 80_000, 17, CW),       # D 32
+    (35, 150_000.0, 200_000, 320_000, 7, WFM),     # D 8, 7 taps
+    (36, -60_000.0, 12_500, 256_000, 127, NFM),    # D 10 < T: a dense tile, the separate pass
 ]
 
 
@@ -137,7 +145,9 @@ def test_fused_demod_same_bits_as_separate_pass(serve_mod):
     S = serve_mod
     kinds = [("wfm", 300_000.0, 0.5), ("fm", -400_000.0, 0.5), ("ssb", -150_000.0, 0.5),
              ("ssb", 250_000.0, 0.5), ("tone", 600_000.0, 0.5), ("fm", -250_000.0, 0.5),
-             ("am", 100_000.0, 0.5), ("fm", -700_000.0, 0.5)]
+             ("am", 100_000.0, 0.5), ("fm", -700_000.0, 0.5), ("fm", 420_000.0, 0.5),
+             ("ssb", -520_000.0, 0.5), ("ssb", 820_000.0, 0.5), ("tone", -820_000.0, 0.5),
+             ("wfm", 150_000.0, 0.5), ("fm", -60_000.0, 0.5)]
     sizes = [BLOCK, BLOCK, 3584, 3585, 7, 0, 12_345, BLOCK, 7168, BLOCK]
     x = synth_scene(sum(sizes), SR, kinds, seed=9)
     fused = _run_fusion(S, True, x, sizes)
