@@ -1004,10 +1004,15 @@ static int describe_block(gpud_serve* s, ServeSlot& sl, const std::vector<ServeO
         const int64_t by_bytes = s->part_min_bytes > 0 ? dl_bytes / s->part_min_bytes : s->max_parts;
         sl.n_parts = (int)std::max<int64_t>(1, std::min<int64_t>(s->max_parts, by_bytes));
     }
+    // With several parts the first is half the size of the others, so the
+    // first download starts sooner (part 1's kernels still end long before
+    // part 0's outputs are down): boundaries at (2p + 1) / (2 n_parts - 1).
     int part_end[ARGUS_FAST_PARTS];
     int e_prev = 0;
     for (int p = 0; p < ARGUS_FAST_PARTS; p++) {
-        const int k = p < sl.n_parts ? (int)(((int64_t)n_fast * (p + 1)) / sl.n_parts) : n_fast;
+        const int k = p < sl.n_parts
+                          ? (int)(((int64_t)n_fast * (2 * p + 1)) / (2 * sl.n_parts - 1))
+                          : n_fast;
         part_end[p] = k;
         const int e_end = k ? s->f_ditems[(size_t)k - 1] : 0;
         hdr.de[p] = ArgusDemodHdr{(const ArgusDemodDesc*)(d + off_ddesc), (const ArgusDemodItem*)(d + off_ditems) + e_prev,

@@ -29,7 +29,9 @@ on a bounded sample on one core and on all host cores.
 from __future__ import annotations
 
 import argparse
+import contextlib
 import ctypes as C
+import gc
 import json
 import os
 import sys
@@ -56,6 +58,20 @@ FP32_PEAK_TFLOPS = 157.3
 DEEMPH_US = 50.0           # de-emphasis time constant of the audio stage (Europe)
 METRIC = "real-time channels sustained @20 Msps span (1 GPU); p99 block latency; HBM GB/s"
 DEMOD_FUSION = [True]      # demodulation in the extraction kernels' epilogues (--unfused-demod: the separate pass)
+
+
+@contextlib.contextmanager
+def no_gc():
+    """A timed loop without the interpreter's cyclic garbage collector (its
+    pauses are the harness's, not the library's); collected before and
+    after."""
+    gc.collect()
+    gc.disable()
+    try:
+        yield
+    finally:
+        gc.enable()
+        gc.collect()
 
 
 def make_channels(n: int, seed: int = 0):
@@ -180,6 +196,14 @@ def host_block_times(chans, d_block, host_blocks, S, IG, parts, blocks=200, warm
     gpu = gpu_out if gpu_out is not None else []
     pending = deque()
     total = warmup + blocks
+    with no_gc():
+        _host_loop(s, g, host_blocks, pending, lat, gpu, total, warmup, pace_ms)
+    g.Close()
+    s.Close()
+    return lat
+
+
+def _host_loop(s, g, host_blocks, pending, lat, gpu, total, warmup, pace_ms):
     i = 0
     next_t = time.perf_counter()
     while i < total or pending:
@@ -202,9 +226,6 @@ def host_block_times(chans, d_block, host_blocks, S, IG, parts, blocks=200, warm
                 gpu.append(info.gpu_ms)
             continue
         time.sleep(0)
-    g.Close()
-    s.Close()
-    return lat
 
 
 def timed_steps(a, chans, d_block, host_blocks, S, IG, parts, tune):
@@ -241,11 +262,12 @@ def timed_steps(a, chans, d_block, host_blocks, S, IG, parts, tune):
                 g.Release(sq)
 
     run(a.warmup, False)
-    check("gpud_device_sync", lib().gpud_device_sync())
-    t0 = time.perf_counter()
-    run(a.steps, True)
-    check("gpud_device_sync", lib().gpud_device_sync())
-    wall = time.perf_counter() - t0
+    with no_gc():
+        check("gpud_device_sync", lib().gpud_device_sync())
+        t0 = time.perf_counter()
+        run(a.steps, True)
+        check("gpud_device_sync", lib().gpud_device_sync())
+        wall = time.perf_counter() - t0
     g.Close()
     s.Close()
     return wall * 1e3 / a.steps, lat, kern, xk, out_bytes
@@ -291,6 +313,8 @@ def bench_capture_replay(a, chans, blocks, lib, S):
         steps = max(a.steps, 50)
         warm = 10
         t0 = 0.0
+        gc.collect()
+        gc.disable()
         for i in range(steps + warm):
             if i == warm:
                 t0 = time.perf_counter()
@@ -308,6 +332,7 @@ def bench_capture_replay(a, chans, blocks, lib, S):
             if timed:
                 gpu.append(info.gpu_ms)
         wall = time.perf_counter() - t0
+        gc.enable()
         s.Close()
         g.Close()
         src.Close()
@@ -328,13 +353,14 @@ def serve_block_times(chans, d_block, lib, S, blocks=200, warmup=10, tune_fixed=
     extraction ms)."""
     s, _ = open_service(chans, d_block, S, tune_fixed, audio=audio)
     out = []
-    for i in range(warmup + blocks):
-        t1 = time.perf_counter()
-        info = s.Wait(s.SubmitDevice(d_block, BLOCK))
-        if i >= warmup:
-            out.append((time.perf_counter() - t1) * 1e3)
-            if infos is not None:
-                infos.append((info.kernel_ms, info.extract_ms))
+    with no_gc():
+        for i in range(warmup + blocks):
+            t1 = time.perf_counter()
+            info = s.Wait(s.SubmitDevice(d_block, BLOCK))
+            if i >= warmup:
+                out.append((time.perf_counter() - t1) * 1e3)
+                if infos is not None:
+                    infos.append((info.kernel_ms, info.extract_ms))
     s.Close()
     return out
 
@@ -816,6 +842,8 @@ def bench_churn(a, d_block, host_blocks, S, IG):
     pending = deque()
     lat, n_live = [], []
     captures0 = s.Counters()[0]
+    gc.collect()
+    gc.disable()
     t_run = time.perf_counter()
     for b in range(n_blocks):
         t_stream = b * BLOCK_S
@@ -838,6 +866,7 @@ def bench_churn(a, d_block, host_blocks, S, IG):
         if b == 50:
             captures0 = s.Counters()[0]  # after the warm-up blocks
     wall = time.perf_counter() - t_run
+    gc.enable()
     captures = s.Counters()[0] - captures0
     g.Close()
     s.Close()
