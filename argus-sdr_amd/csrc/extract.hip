@@ -696,10 +696,31 @@ void argus_order_items(std::vector<ArgusWorkItem>& items, const std::vector<int>
     int count[kKeys + 1] = {0};
     const size_t n = items.size();
     key.resize(n);
-    const long span = std::max(1, n_new);
+    // each channel's items on one XCD (its taps are then fetched into one L2,
+    // not eight): channels packed onto the XCDs heaviest first, each to the
+    // least loaded one
+    thread_local std::vector<long> ccost;
+    thread_local std::vector<int> corder, cbin;
+    int n_ch = 0;
+    for (size_t i = 0; i < n; i++) n_ch = std::max(n_ch, items[i].chan + 1);
+    ccost.assign((size_t)n_ch, 0);
+    for (size_t i = 0; i < n; i++) ccost[(size_t)items[i].chan] += std::max(items[i].pad, 1);
+    corder.resize((size_t)n_ch);
+    for (int c = 0; c < n_ch; c++) corder[(size_t)c] = c;
+    std::sort(corder.begin(), corder.end(), [&](int a, int b) { return ccost[(size_t)a] > ccost[(size_t)b]; });
+    cbin.assign((size_t)n_ch, 0);
+    long load[ARGUS_XCDS] = {0};
+    for (int c : corder) {
+        int best = 0;
+        for (int x = 1; x < ARGUS_XCDS; x++)
+            if (load[x] < load[best]) best = x;
+        cbin[(size_t)c] = best;
+        load[best] += ccost[(size_t)c];
+    }
+    (void)pos;
+    (void)n_new;
     for (size_t i = 0; i < n; i++) {
-        int b = (int)((long)pos[i] * ARGUS_XCDS / span);
-        b = std::min(std::max(b, 0), ARGUS_XCDS - 1);
+        const int b = cbin[(size_t)items[i].chan];
         const unsigned long c = (unsigned long)std::max(items[i].pad, 1);
         const int msb = 63 - __builtin_clzl(c);
         const int half = msb > 0 ? (int)((c >> (msb - 1)) & 1) : 0;
