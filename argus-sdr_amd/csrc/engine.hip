@@ -195,7 +195,6 @@ struct ArgusLegacyScratch {
     ArgusWorkItem* h_items = nullptr;
     size_t d_items_cap = 0, h_items_cap = 0;
     std::vector<ArgusWorkItem> items;
-    std::vector<int> pos;
     std::vector<float> poly;
     std::vector<float2> ctaps;
 };
@@ -234,7 +233,6 @@ int argus_legacy_launch(ArgusLegacyScratch*& sc, hipStream_t st, const ArgusTune
     ARGUS_CHECK((hipError_t)grow_host(&sc->h_data, &sc->h_cap, bytes));
     ArgusChanDesc* descs = (ArgusChanDesc*)(sc->h_data + off_desc);
     sc->items.clear();
-    sc->pos.clear();
     size_t lds = 0;
     for (int j = 0; j < nj; j++) {
         const LegacyJob& jb = jobs[j];
@@ -256,9 +254,9 @@ int argus_legacy_launch(ArgusLegacyScratch*& sc, hipStream_t st, const ArgusTune
             return -1;
         ch.n_out = jb.n_out;
         if (ch.n_out > 0) lds = std::max(lds, argus_item_lds_bytes(ch));
-        argus_append_items(ch, j, n, sc->items, sc->pos);
+        argus_append_items(ch, j, sc->items);
     }
-    argus_order_items(sc->items, sc->pos, n);
+    argus_order_items(sc->items);
     const int n_items = (int)sc->items.size();
     ARGUS_CHECK((hipError_t)grow_dev(&sc->d_items, &sc->d_items_cap, (size_t)std::max(n_items, 1)));
     ARGUS_CHECK((hipError_t)grow_host(&sc->h_items, &sc->h_items_cap, (size_t)std::max(n_items, 1)));

@@ -651,8 +651,7 @@ int argus_launch_extract_batch(const ArgusChanDesc* d_chans, const ArgusWorkItem
 
 // Append the work items of one described channel; pos receives each item's
 // first input sample (history reads count as sample 0).
-void argus_append_items(const ArgusChanDesc& ch, int chan_index, int n_new, std::vector<ArgusWorkItem>& items,
-                        std::vector<int>& pos)
+void argus_append_items(const ArgusChanDesc& ch, int chan_index, std::vector<ArgusWorkItem>& items)
 {
     const int tiles = ch.n_out > 0 ? (ch.n_out + ch.M - 1) / ch.M : 0;
     // cost in lane-operations: NCO rotations (~24 each) plus taps (~2 each,
@@ -667,27 +666,23 @@ void argus_append_items(const ArgusChanDesc& ch, int chan_index, int n_new, std:
         cost = (long)ch.M * ch.T * 26;
     }
     const int kind = ch.narrow ? ARGUS_ITEM_NARROW : ARGUS_ITEM_DENSE;
-    for (int t = 0; t < tiles; t++) {
-        items.push_back({chan_index, t * ch.M, kind, (int)std::min<long>(cost, 1L << 30)});
-        pos.push_back(std::max(0, ch.e0 + t * ch.M * ch.D - (ch.T - 1)));
-    }
-    if (ch.keep > 0) {
-        items.push_back({chan_index, 0, ARGUS_ITEM_HISTORY, ch.keep * 24});
-        pos.push_back(std::max(0, n_new - ch.keep));
-    }
+    for (int t = 0; t < tiles; t++) items.push_back({chan_index, t * ch.M, kind, (int)std::min<long>(cost, 1L << 30)});
+    if (ch.keep > 0) items.push_back({chan_index, 0, ARGUS_ITEM_HISTORY, ch.keep * 24});
 }
 
 // Order the items of one launch for the persistent grid.  Workgroups are
 // dealt to the XCDs round-robin, and the grid is a multiple of ARGUS_XCDS, so
-// item w always runs on XCD w % ARGUS_XCDS.  Items are bucketed by where
-// their input lies in the block (one eighth per XCD), heaviest first within a
-// bucket, and interleaved: each XCD's L2 then holds one eighth of the block
-// instead of all of it.  Short buckets are padded with no-op items.
+// item w always runs on XCD w % ARGUS_XCDS.  Each channel's items go to one
+// XCD (channels packed heaviest first onto the least loaded XCD), heaviest
+// items first within an XCD, and the XCDs' lists are interleaved: a channel's
+// taps are then fetched into one L2, not eight (the 3.2 MB block, which every
+// XCD reads, stays in each L2).  Short lists are padded with no-op items.
 //
 // "Heaviest first" is by cost class (half-octaves of the cost estimate), a
-// stable counting sort: O(items) host time per block, which matters at
-// thousands of channels (a comparison sort of ~10^5 items costs milliseconds).
-void argus_order_items(std::vector<ArgusWorkItem>& items, const std::vector<int>& pos, int n_new)
+// stable counting sort: O(items) host time per work-list build, which
+// matters at thousands of channels (a comparison sort of ~10^5 items costs
+// milliseconds); the channel packing sorts channels, not items.
+void argus_order_items(std::vector<ArgusWorkItem>& items)
 {
     constexpr int kClasses = 64;
     constexpr int kKeys = ARGUS_XCDS * kClasses;
@@ -717,8 +712,6 @@ void argus_order_items(std::vector<ArgusWorkItem>& items, const std::vector<int>
         cbin[(size_t)c] = best;
         load[best] += ccost[(size_t)c];
     }
-    (void)pos;
-    (void)n_new;
     for (size_t i = 0; i < n; i++) {
         const int b = cbin[(size_t)items[i].chan];
         const unsigned long c = (unsigned long)std::max(items[i].pad, 1);

@@ -243,8 +243,7 @@ int argus_launch_extract_batch(const ArgusChanDesc* d_chans, const ArgusWorkItem
 int argus_extract_allow_lds(size_t bytes);
 // Graph-mode launch: counts and pointers come from *hdr (device memory).
 int argus_launch_extract_hdr(const ArgusExtractHdr* d_hdr, size_t lds_bytes, int grid, hipStream_t stream);
-void argus_append_items(const ArgusChanDesc& ch, int chan_index, int n_new, std::vector<ArgusWorkItem>& items,
-                        std::vector<int>& pos);
-void argus_order_items(std::vector<ArgusWorkItem>& items, const std::vector<int>& pos, int n_new);
+void argus_append_items(const ArgusChanDesc& ch, int chan_index, std::vector<ArgusWorkItem>& items);
+void argus_order_items(std::vector<ArgusWorkItem>& items);
 // LDS bytes of the largest dense tile or narrow item a tune allows for taps <= max_taps.
 size_t argus_tune_max_lds(const ArgusTune& tune, int max_taps);

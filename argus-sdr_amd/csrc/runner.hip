@@ -455,7 +455,6 @@ static int run_batched(gpud_runner* r, const float2* d_in, int n_new, const gpud
     ARGUS_CHECK((hipError_t)ensure_dev(&r->d_chans[slot], &r->d_chans_cap[slot], (size_t)std::max(n_jobs, 1)));
 
     r->items.clear();
-    r->item_pos.clear();
     r->fdescs.clear();
     // the fast form needs the stream's own last ARGUS_FAST_TAIL samples in
     // the tail, which every batched call keeps
@@ -480,13 +479,13 @@ static int run_batched(gpud_runner* r, const float2* d_in, int n_new, const gpud
             r->fdescs.push_back(argus_fast_desc(ch, s->d_fastG));
             ArgusChanDesc hist_only = ch;
             hist_only.n_out = 0;
-            argus_append_items(hist_only, i, n_new, r->items, r->item_pos);
+            argus_append_items(hist_only, i, r->items);
         } else {
             if (ch.n_out > 0) lds = std::max(lds, argus_item_lds_bytes(ch));
-            argus_append_items(ch, i, n_new, r->items, r->item_pos);
+            argus_append_items(ch, i, r->items);
         }
     }
-    argus_order_items(r->items, r->item_pos, n_new);
+    argus_order_items(r->items);
     const int n_items = (int)r->items.size();
     ARGUS_CHECK((hipError_t)ensure_host(&r->h_items[slot], &r->h_items_cap[slot], (size_t)std::max(n_items, 1)));
     ARGUS_CHECK((hipError_t)ensure_dev(&r->d_items[slot], &r->d_items_cap[slot], (size_t)std::max(n_items, 1)));

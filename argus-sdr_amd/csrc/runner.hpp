@@ -83,7 +83,6 @@ struct gpud_runner {
     ArgusWorkItem* d_items[2] = {nullptr, nullptr};
     size_t h_items_cap[2] = {0, 0}, d_items_cap[2] = {0, 0};
     std::vector<ArgusWorkItem> items;
-    std::vector<int> item_pos;
     // kernel timing (event pair per fused launch)
     bool timing = false;
     std::vector<hipEvent_t> tev;  // 2 per launch

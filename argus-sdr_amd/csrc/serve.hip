@@ -807,7 +807,6 @@ static int describe_block(gpud_serve* s, ServeSlot& sl, const std::vector<ServeO
     const bool reuse = sl.items_epoch == s->epoch && sl.items_n == n && sl.h_tab;
     if (!reuse) {
         s->items.clear();
-        s->item_pos.clear();
     }
     size_t items_bytes = reuse ? (size_t)sl.n_items * sizeof(ArgusWorkItem) : 0;
     // descriptors are written straight into the table, after the items
@@ -851,7 +850,7 @@ static int describe_block(gpud_serve* s, ServeSlot& sl, const std::vector<ServeO
         if (!reuse) {
             ArgusChanDesc full = ch;
             full.n_out = fast ? 0 : (c.decim - 1 + n) / c.decim;
-            argus_append_items(full, ci, n, s->items, s->item_pos);
+            argus_append_items(full, ci, s->items);
         }
 
         if (c.pre_factor > 1) {  // pre-demod stage: baseband -> demodulator input
@@ -947,7 +946,7 @@ static int describe_block(gpud_serve* s, ServeSlot& sl, const std::vector<ServeO
         ci++;
     }
     if (!reuse) {
-        argus_order_items(s->items, s->item_pos, n);
+        argus_order_items(s->items);
         items_bytes = s->items.size() * sizeof(ArgusWorkItem);
     }
     const int n_items = reuse ? sl.n_items : (int)s->items.size();

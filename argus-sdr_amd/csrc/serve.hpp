@@ -293,7 +293,6 @@ struct gpud_serve {
     std::map<std::tuple<int, int, bool>, float*> filters;  // audio filters per (in rate, out rate, low-pass)
     std::vector<ArgusAudioDesc> adescs;
     std::vector<ArgusWorkItem> items;
-    std::vector<int> item_pos;
     std::vector<ArgusChanDesc> chans;
     std::vector<ArgusDemodDesc> ddescs;
     std::vector<ArgusWfmDesc> wdescs;
