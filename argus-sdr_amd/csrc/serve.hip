@@ -1770,10 +1770,11 @@ GPUD_API int gpud_serve_set_fast(gpud_serve* s, int enable)
     return 0;
 }
 
-// Demodulation in the fast kernel's epilogue for the fast-form channels
-// whose demodulator reads their baseband (FM, SSB, CW; no pre-demod or
-// stereo stage): on by default.  Off, those channels' baseband is
-// demodulated by the separate pass as every other channel's (same outputs).
+// Demodulation in the epilogues of the fast kernel and of the direct form's
+// narrow items for the channels whose demodulator reads their baseband (FM,
+// SSB, CW; no pre-demod or stereo stage): on by default.  Off, those
+// channels' baseband is demodulated by the separate pass as every other
+// channel's (same outputs).
 GPUD_API int gpud_serve_set_demod_fusion(gpud_serve* s, int enable)
 {
     if (!s) return -1;

@@ -554,15 +554,16 @@ GPUD_API int gpud_serve_set_rds(gpud_serve* s, int64_t channel_id, int window_sa
  * direct polyphase form; same filter, outputs within FP32 rounding of the
  * direct form.  Applies from the next submitted block. */
 GPUD_API int gpud_serve_set_fast(gpud_serve* s, int enable);
-/* Demodulation fused into the fast form's kernel (no counterpart in the
+/* Demodulation fused into the extraction kernels (no counterpart in the
  * reference, whose demodulators run per signal after extraction,
- * streamer.go:1287-1302): on by default.  For a fast-form channel whose
- * demodulator reads its baseband (NFM/WFM, USB/LSB, CW; no pre-demod or
- * stereo stage), the kernel that writes the baseband writes the
- * demodulated outputs too, so the baseband is not read back; FM's first
- * output of each 3584-sample segment (and the carried sample) is added by a
- * short pass over those seams.  Off: the separate demodulation pass, the
- * same outputs bit for bit.  Applies from the next submitted block. */
+ * streamer.go:1287-1302): on by default.  For a channel whose demodulator
+ * reads its baseband (NFM/WFM, USB/LSB, CW; no pre-demod or stereo stage)
+ * and which runs on the fast form or as the direct form's narrow items, the
+ * kernel that writes the baseband writes the demodulated outputs too, so
+ * the baseband is not read back; FM's first output of each 3584-sample
+ * segment or narrow item (and the carried sample) is added by a short pass
+ * over those seams.  Off: the separate demodulation pass, the same outputs
+ * bit for bit.  Applies from the next submitted block. */
 GPUD_API int gpud_serve_set_demod_fusion(gpud_serve* s, int enable);
 /* Host helper of the fast form: the 4096 bins of FFT(g) / 4096 for
  * g[k] = taps[k] e^{-j phase_inc k} (k < num_taps <= 512, natural order;
