@@ -645,7 +645,11 @@ def bench_wfm_stereo_20m(a, lib, S):
     scene = StereoScene(FS, offs, snr, bits, seed=5, low_rate=1_000_000.0)
     n_blocks = a.stereo_blocks
     print(f"[bench]   generating {n_blocks} blocks", file=sys.stderr, flush=True)
-    blocks = [scene.block(BLOCK) for _ in range(n_blocks)]
+    blocks = []
+    for i in range(n_blocks):
+        blocks.append(scene.block(BLOCK))
+        if (i + 1) % 100 == 0:
+            print(f"[bench]   {i + 1} blocks generated", file=sys.stderr, flush=True)
     s = S.Serve(FS, BLOCK, 1 << 30)
     rds_win = (250_000 // 13) // 2
     for k in range(n_st):
@@ -957,7 +961,7 @@ def main() -> int:
     ap.add_argument("--roof-reps", type=int, default=20)
     ap.add_argument("--rds-stations", type=int, default=1024,
                     help="stations per device RDS decode call in the RDS decoding figure (0: skip it)")
-    ap.add_argument("--only", choices=["rds", "surveillance", "roofline", "churn"], default="",
+    ap.add_argument("--only", choices=["rds", "surveillance", "roofline", "churn", "stereo"], default="",
                     help="run only this secondary figure and print it")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extras", action="store_true",
@@ -989,6 +993,9 @@ def main() -> int:
 
     if a.only == "rds":
         print(json.dumps({"rds_decode_device": bench_rds_decode(a, L)}), flush=True)
+        return 0
+    if a.only == "stereo":
+        print(json.dumps({"wfm_stereo_rds_20msps_20stations": bench_wfm_stereo_20m(a, L, S)}), flush=True)
         return 0
     if a.only == "surveillance":
         print(json.dumps({"surveillance_10msps_n32768": bench_surveillance(a, L)[0]}), flush=True)
