@@ -649,8 +649,8 @@ int argus_launch_extract_batch(const ArgusChanDesc* d_chans, const ArgusWorkItem
     return (int)hipGetLastError();
 }
 
-// Append the work items of one described channel; pos receives each item's
-// first input sample (history reads count as sample 0).
+// Append the work items of one described channel, each with its cost
+// estimate (in `pad`) for argus_order_items.
 void argus_append_items(const ArgusChanDesc& ch, int chan_index, std::vector<ArgusWorkItem>& items)
 {
     const int tiles = ch.n_out > 0 ? (ch.n_out + ch.M - 1) / ch.M : 0;

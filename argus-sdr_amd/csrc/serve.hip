@@ -991,10 +991,10 @@ static int describe_block(gpud_serve* s, ServeSlot& sl, const std::vector<ServeO
     ArgusServeHdr& hdr = *(ArgusServeHdr*)(sl.h_tab + off_hdr);
     const char* d = sl.d_tab;
     hdr.x = ArgusExtractHdr{(const ArgusChanDesc*)(d + off_chans), sl.d_items, in, n_items, n};
-    // the fast channels in n_parts parts of equal counts, one per
-    // part_min_bytes of early outputs the block downloads (at most
-    // max_parts; parts after n_parts are empty and not launched); part p's
-    // early demodulation items and output ranges follow from the channel order
+    // the fast channels in n_parts parts, one per part_min_bytes of early
+    // outputs the block downloads (at most max_parts; parts after n_parts
+    // are empty and not launched); part p's early demodulation items and
+    // output ranges follow from the channel order
     {
         const int64_t iq_e = s->f_iq_end.empty() ? 0 : s->f_iq_end.back();
         const int64_t au_e = s->f_au_end.empty() ? 0 : s->f_au_end.back();
