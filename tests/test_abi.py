@@ -78,3 +78,18 @@ def test_library_resolves_its_own_symbols():
     out = subprocess.run(["nm", "-D", "--undefined-only", str(lib)], capture_output=True, text=True, check=True).stdout
     own = [l for l in out.splitlines() if " argus_" in l or " gpud_" in l]
     assert not own, own
+
+
+def test_entry_points_refuse_missing_handles():
+    """Entry points check their handle before any device work: a null
+    service or engine is refused with -1 (no GPU needed)."""
+    import ctypes as C
+    import argus
+    _late_bound()
+    lib = argus.lib()
+    assert lib.gpud_serve_set_download_split(None, 4, 0) == -1
+    assert lib.gpud_serve_set_download_parts(None, 1) == -1
+    assert lib.gpud_serve_set_demod_fusion(None, 1) == -1
+    from argus import surveillance as SV
+    cfar = SV.cfar_params("CA", guard=4, train=16)
+    assert lib.gpud_surv_cfar_runs_device_ex(None, None, 1, C.byref(cfar), None, None, 16, None) == -1
