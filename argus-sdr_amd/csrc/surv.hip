@@ -164,30 +164,59 @@ __global__ __launch_bounds__(kLevelOut) void argus_level_fir_kernel(const float2
                                                                     float2* __restrict__ z)
 {
 #pragma clang fp contract(off)
-    __shared__ float2 xs[kLevelOut * kLevelMaxDecim + kLevelMaxTaps];
+    // the span by polyphase residue: sample j (input base + j) at plane
+    // j mod decim, position j / decim, planes P positions apart
+    __shared__ float2 xs[kLevelOut * kLevelMaxDecim + kLevelMaxTaps + kLevelMaxDecim];
     __shared__ double hs[kLevelMaxTaps];
     const int f = blockIdx.y;
     const int m0 = blockIdx.x * kLevelOut;
     const float2* x = in + (int64_t)f * frame_step;
-    const int base = m0 * decim - (n_taps - 1);  // input index of xs[0]
+    const int base = m0 * decim - (n_taps - 1);  // input index of span sample 0
     const int span = kLevelOut * decim + n_taps - 1;
+    const int ld = __builtin_ctz(decim);  // decim is a power of two
+    const int P = (span + decim - 1) >> ld;
     for (int j = threadIdx.x; j < span; j += kLevelOut) {
         const int idx = base + j;
         const int lim = n_d * decim;
-        xs[j] = (idx >= 0 && idx < lim) ? x[idx] : make_float2(0.0f, 0.0f);
+        xs[(j & (decim - 1)) * P + (j >> ld)] = (idx >= 0 && idx < lim) ? x[idx] : make_float2(0.0f, 0.0f);
     }
-    for (int k = threadIdx.x; k < n_taps; k += kLevelOut) hs[k] = taps[k];
+    // the taps grouped the same way: residue rho's taps k = k0(rho) + i decim
+    // (k0 = (T - 1 - rho) mod decim) at hs[off(rho) + i], off the count of
+    // taps of the residues before it
+    for (int k = threadIdx.x; k < n_taps; k += kLevelOut) {
+        const int rho = (n_taps - 1 - k) & (decim - 1);
+        int off = 0;
+        for (int r = 0; r < rho; r++) {
+            const int k0 = (n_taps - 1 - r) & (decim - 1);
+            off += k0 < n_taps ? (n_taps - 1 - k0) / decim + 1 : 0;
+        }
+        hs[off + (k >> ld)] = taps[k];
+    }
     __syncthreads();
     const int m = m0 + threadIdx.x;
     if (m >= n_d) return;
-    // y[i] = sum_k h[k] x[i - k], i = m * decim -> xs index i - k - base
-    const float2* xp = xs + (int)threadIdx.x * decim + (n_taps - 1);
+    // y[i] = sum_k h[k] x[i - k], i = m * decim: span sample t decim + jj,
+    // jj = T - 1 - k.  The taps are summed grouped by the sample's residue
+    // rho = jj mod decim: those read plane rho at positions t + (jj - rho) /
+    // decim, consecutive over the lanes (no bank conflicts) and a fixed
+    // offset per tap, and their coefficients in a row.  FP64 throughout; the
+    // grouping changes only the last bits of the FP64 sum, far below the
+    // FP32 transform that follows.
     double ar = 0.0, ai = 0.0;
-    for (int k = 0; k < n_taps; k++) {
-        const float2 v = xp[-k];
-        const double w = hs[k];
-        ar += (double)v.x * w;
-        ai += (double)v.y * w;
+    const double* hr = hs;
+    for (int rho = 0; rho < decim; rho++) {
+        const int k0 = (n_taps - 1 - rho) & (decim - 1);  // first tap of this residue
+        if (k0 >= n_taps) continue;
+        const int cnt = (n_taps - 1 - k0) / decim + 1;
+        const float2* xp = xs + rho * P + (int)threadIdx.x + ((n_taps - 1 - k0 - rho) >> ld);
+#pragma unroll 4
+        for (int i = 0; i < cnt; i++) {
+            const float2 v = xp[-i];
+            const double w = hr[i];
+            ar += (double)v.x * w;
+            ai += (double)v.y * w;
+        }
+        hr += cnt;
     }
     const float wd = win[m];
     float2* o = z + (int64_t)f * n_d * decim + (int64_t)m * decim;
