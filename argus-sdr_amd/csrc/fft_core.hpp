@@ -133,6 +133,63 @@ __device__ __forceinline__ void fft_r3(float2 (&v)[R], float* xs, int t)
     dft<R>(v);
 }
 
+// Twiddle tables of the R^3-point transform, built once per workgroup in
+// LDS: t1[x][k] = W_N^{x k} and t2[x][k] = W_{R^2}^{x k} (x, k < R), rows
+// R + 1 apart so lanes reading different rows hit different banks.  Pass 1's
+// W_N^{(a + R b) k} is t1[a][k] t2[b][k]; pass 2's W_{R^2}^{a k} is t2[a][k].
+template <int R>
+constexpr int fft_tab_entries()
+{
+    return R * (R + 1);
+}
+
+template <int R>
+__device__ __forceinline__ void fft_tab_init(float2* t1, float2* t2, int tid, int nthreads)
+{
+    constexpr float invN = 1.0f / (float)(R * R * R);
+    constexpr float invRR = 1.0f / (float)(R * R);
+    for (int i = tid; i < R * R; i += nthreads) {
+        const int x = i / R, k = i - x * R;
+        t1[x * (R + 1) + k] = twiddle(x * k, invN);
+        t2[x * (R + 1) + k] = twiddle(x * k, invRR);
+    }
+}
+
+// fft_r3 with the twiddles read from the tables (fft_tab_init, after a
+// barrier) instead of evaluated: two LDS reads and a complex product for
+// pass 1, one read for pass 2, against a sine and a cosine (quarter-rate
+// instructions) per twiddle.
+template <int R>
+__device__ __forceinline__ void fft_r3_tab(float2 (&v)[R], float* xs, int t, const float2* t1, const float2* t2)
+{
+    constexpr int TH = R * R;
+    asm volatile("" : "+v"(t));  // keeps the table reads (functions of t) in the caller's loop
+    dft<R>(v);
+    {
+        const int a = t % R, b = t / R;
+        const float2* r1 = t1 + a * (R + 1);
+        const float2* r2 = t2 + b * (R + 1);
+#pragma unroll
+        for (int k = 1; k < R; ++k) v[k] = cmul(v[k], cmul(r1[k], r2[k]));
+    }
+    {
+        const int a = t % R, kc = t / R;
+        exchange<R>(v, xs, [&](int k) { return t + TH * k; }, [&](int b) { return a + R * b + TH * kc; });
+    }
+    dft<R>(v);
+    {
+        const int a = t % R, kc = t / R;
+        const float2* r2 = t2 + a * (R + 1);
+#pragma unroll
+        for (int k = 1; k < R; ++k) v[k] = cmul(v[k], r2[k]);
+        const int kb2 = t % R, kc2 = t / R;
+        exchange<R>(
+            v, xs, [&](int kb) { return a + (R + 1) * kb + R * (R + 1) * kc; },
+            [&](int aa) { return aa + (R + 1) * kb2 + R * (R + 1) * kc2; });
+    }
+    dft<R>(v);
+}
+
 // LDS floats the transforms need: the padded R x (R+1) x R image.
 template <int R>
 constexpr int fft_lds_floats()

@@ -36,9 +36,12 @@ __global__ __launch_bounds__(R* R) void argus_welch_kernel(const float2* __restr
 {
     constexpr int N = R * R * R, TH = R * R, PL = R * (R + 1);
     __shared__ float xs[fft_lds_floats<R>()];
+    __shared__ float2 tw1[fft_tab_entries<R>()], tw2[fft_tab_entries<R>()];
     const int t = threadIdx.x;
     const int64_t base = (int64_t)blockIdx.x * frame_step;
     const float invN = 1.0f / (float)N;
+    fft_tab_init<R>(tw1, tw2, t, TH);
+    __syncthreads();
     float acc[R];
 #pragma unroll
     for (int k = 0; k < R; ++k) acc[k] = 0.0f;
@@ -54,7 +57,7 @@ __global__ __launch_bounds__(R* R) void argus_welch_kernel(const float2* __restr
             const float w = win[idx];
             v[c] = make_float2(s.x * w, s.y * w);
         }
-        fft_r3<R>(v, xs, (int)threadIdx.x);
+        fft_r3_tab<R>(v, xs, (int)threadIdx.x, tw1, tw2);
 #pragma unroll
         for (int k = 0; k < R; ++k) {
             const float mr = v[k].x * invN, mi = v[k].y * invN;
@@ -89,8 +92,10 @@ __global__ __launch_bounds__(R* R) void argus_fft_kernel(const float2* __restric
 {
     constexpr int N = R * R * R, TH = R * R, PL = R * (R + 1);
     __shared__ float xs[fft_lds_floats<R>()];
+    __shared__ float2 tw1[fft_tab_entries<R>()], tw2[fft_tab_entries<R>()];
     const int t = threadIdx.x;
     const float2* x = in + (int64_t)blockIdx.x * frame_step;
+    fft_tab_init<R>(tw1, tw2, t, TH);
     float2 v[R];
 #pragma unroll
     for (int c = 0; c < R; ++c) {
@@ -102,7 +107,8 @@ __global__ __launch_bounds__(R* R) void argus_fft_kernel(const float2* __restric
         }
         v[c] = s;
     }
-    fft_r3<R>(v, xs, (int)threadIdx.x);
+    __syncthreads();  // the twiddle tables are complete
+    fft_r3_tab<R>(v, xs, (int)threadIdx.x, tw1, tw2);
     const int kb = t % R, kc = t / R;
     float2* o = out + (int64_t)blockIdx.x * N;
     const int jc = t % R, jb = t / R;
