@@ -37,7 +37,9 @@
 #define ARGUS_AUDIO_ARMCAP 3328     // arm coefficients held in LDS (L x T), else read from memory
 #define ARGUS_AUDIO_OUTCAP 2048     // output samples per chunk (all sides), LDS
 #define ARGUS_AUDIO_MAX_M (1 << 20) // phase arithmetic is 32-bit within a chunk
-#define ARGUS_AUDIO_GRID 2048
+// workgroups of the audio launch: one per channel up to this many (those past
+// the block's channel count exit at once), the hardware balancing their lengths
+#define ARGUS_AUDIO_GRID 8192
 
 struct ArgusAudioDesc {
     const float* in;         // block input: n_in mono samples or n_in L/R frames

@@ -984,6 +984,11 @@ static int describe_block(gpud_serve* s, ServeSlot& sl, const std::vector<ServeO
     if (n_wfm) std::memcpy(sl.h_tab + off_wdesc, s->wdescs.data(), (size_t)n_wfm * sizeof(ArgusWfmDesc));
     if (n_ditems) std::memcpy(sl.h_tab + off_ditems, s->ditems.data(), (size_t)n_ditems * sizeof(ArgusDemodItem));
     if (n_witems) std::memcpy(sl.h_tab + off_witems, s->witems.data(), (size_t)n_witems * sizeof(ArgusWfmItem));
+    // longest inputs first: workgroups are dispatched in order, so the
+    // high-rate channels (many chunks each) start first
+    std::stable_sort(s->adescs.begin(), s->adescs.end(), [](const ArgusAudioDesc& a, const ArgusAudioDesc& b) {
+        return (int64_t)a.n_in * a.channels > (int64_t)b.n_in * b.channels;
+    });
     if (n_audio) std::memcpy(sl.h_tab + off_adesc, s->adescs.data(), (size_t)n_audio * sizeof(ArgusAudioDesc));
     if (n_fast) std::memcpy(sl.h_tab + off_fdesc, s->fdescs.data(), (size_t)n_fast * sizeof(ArgusFastDesc));
     if (n_pre) std::memcpy(sl.h_tab + off_pdesc, s->pdescs.data(), (size_t)n_pre * sizeof(ArgusPreDesc));

@@ -961,7 +961,7 @@ def main() -> int:
     ap.add_argument("--roof-reps", type=int, default=20)
     ap.add_argument("--rds-stations", type=int, default=1024,
                     help="stations per device RDS decode call in the RDS decoding figure (0: skip it)")
-    ap.add_argument("--only", choices=["rds", "surveillance", "roofline", "churn", "stereo"], default="",
+    ap.add_argument("--only", choices=["rds", "surveillance", "roofline", "churn", "stereo", "audio"], default="",
                     help="run only this secondary figure and print it")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extras", action="store_true",
@@ -1020,6 +1020,10 @@ def main() -> int:
 
     if a.only == "roofline":
         print(json.dumps({"roofline": bench_roofline(a, d0.value, S)}), flush=True)
+        return 0
+    if a.only == "audio":
+        a.tune_dev = a.tune_fixed
+        print(json.dumps({"audio_stage_48k": bench_audio_stage(a, a.channels or 8192, d0.value, L, S)}), flush=True)
         return 0
     if a.only == "churn":
         print(json.dumps({"churn_serving": bench_churn(a, d0.value, blocks, S, IG)}), flush=True)
