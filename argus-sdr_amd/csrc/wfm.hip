@@ -50,6 +50,33 @@ __device__ __forceinline__ float fb_at(const ArgusWfmDesc& d, int fb_rd, int idx
     return idx >= 0 ? d.mono[idx] : d.fb_hist[fb_rd * ARGUS_WFM_HIST + ARGUS_WFM_HIST + idx];
 }
 
+// atan2 of the PLL's phase detector: the octant ratio a = min/max through
+// the hardware reciprocal (the denominator floored at the smallest normal,
+// so 0/0 gives a = 0), atan(a) on [0, 1] as a + a s P(s) (s = a^2, P of
+// degree 6 in Horner form, coefficients from a weighted least-squares fit
+// to atan made for this kernel: 1.2e-7 rad max error in FP32), then the
+// octant and quadrant reflections and the sign of y: 3.2e-7 rad max error
+// over all quadrants, and atan2(+-0, +0) = +-0, atan2(+-0, -0) = +-pi as in
+// IEEE atan2.
+__device__ __forceinline__ float pll_atan2f(float y, float x)
+{
+    const float ax = fabsf(x), ay = fabsf(y);
+    const float mx = fmaxf(fmaxf(ax, ay), 1.17549435e-38f), mn = fminf(ax, ay);
+    const float a = mn * __builtin_amdgcn_rcpf(mx);
+    const float s = a * a;
+    float q = -4.355480428785086e-3f;
+    q = fmaf(q, s, 2.3040393367409706e-2f);
+    q = fmaf(q, s, -5.7773932814598083e-2f);
+    q = fmaf(q, s, 9.794257581233978e-2f);
+    q = fmaf(q, s, -1.397659033536911e-1f);
+    q = fmaf(q, s, 1.9962705671787262e-1f);
+    q = fmaf(q, s, -3.333165943622589e-1f);
+    float r = fmaf(a * s, q, a);
+    r = ay > ax ? 1.5707963267948966f - r : r;
+    r = __builtin_signbit(x) ? 3.1415926535897932f - r : r;
+    return copysignf(r, y);
+}
+
 }  // namespace
 
 __device__ __forceinline__ void wfm_pre(const ArgusWfmDesc* __restrict__ descs, const ArgusWfmItem* __restrict__ items,
@@ -123,17 +150,19 @@ __device__ __forceinline__ void wfm_pre(const ArgusWfmDesc* __restrict__ descs, 
     }
 }
 
-// One wavefront per channel: the wave stages the pilot in LDS, lane 0 runs
-// the recurrence, the wave writes the per-sample oscillator phase back.
 __device__ __forceinline__ void wfm_pll_one(const ArgusWfmDesc& d)
 {
-    __shared__ float pl[ARGUS_WFM_TILE];
-    __shared__ float tt[ARGUS_WFM_TILE];
+    // One wavefront (the whole 64-thread workgroup) per station. Every lane
+    // runs the same serial recurrence: the pilot sample of step j comes from
+    // lane j of a 64-sample register tile (readlane, no LDS round trip), lane
+    // j keeps the phase after step j, and the lanes turn their phases into
+    // the L-R carrier argument together after each tile.
     __shared__ double tp[64];
     const int n = d.n;
+    const int lane = threadIdx.x;
     ArgusWfmState* sp = d.st;
     if (n <= 0) {  // no samples: state and lock counters unchanged (stereoDecodeStateful returns early)
-        if (threadIdx.x == 0) {
+        if (lane == 0) {
             ArgusWfmBlock b;
             b.locked = 0;
             b.stereo = sp->enabled;
@@ -145,52 +174,69 @@ __device__ __forceinline__ void wfm_pll_one(const ArgusWfmDesc& d)
     }
     double phase = sp->phase, freq = sp->freq, is = sp->i_state, qs = sp->q_state;
     const double alpha = d.alpha, beta = d.beta, lpa = d.lpa, min_f = d.min_f, max_f = d.max_f;
-    double pp = 0.0, es = 0.0;
+    double es = 0.0;    // sum of |err|, in sample order (every lane holds it)
+    double pp_l = 0.0;  // this lane's share of the pilot power
     if (d.batch) {  // total power of the discriminator output, for the batch lock test
         double acc = 0.0;
-        for (int j = threadIdx.x; j < n; j += blockDim.x) {
+        for (int j = lane; j < n; j += 64) {
             const double m = (double)d.mono[j];
             acc += m * m;
         }
-        tp[threadIdx.x] = acc;
+        tp[lane] = acc;
     }
-    const float* pilot = d.scratch + 2 * n;
+    typedef __attribute__((address_space(1))) const float GlobalF;  // global loads: no LDS counter to wait on
+    const GlobalF* pilot = (const GlobalF*)(d.scratch + 2 * n);
     float* t2 = d.scratch + 3 * n;
-    for (int i0 = 0; i0 < n; i0 += ARGUS_WFM_TILE) {
-        const int cnt = min(ARGUS_WFM_TILE, n - i0);
-        for (int j = threadIdx.x; j < cnt; j += blockDim.x) pl[j] = pilot[i0 + j];
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            for (int j = 0; j < cnt; j++) {
-                const double p = (double)pl[j];
-                const float ph_t = (float)(phase * kInvTwoPi);  // phase in [0, 2pi] -> [0, 1] turns
-                const double sn = (double)__builtin_amdgcn_sinf(ph_t);
-                const double cs = (double)__builtin_amdgcn_cosf(ph_t);
-                is += lpa * (p * cs - is);
-                qs += lpa * (p * -sn - qs);
-                const double err = (double)atan2f((float)qs, (float)is);
-                freq += beta * err;
-                freq = freq < min_f ? min_f : (freq > max_f ? max_f : freq);
-                phase += freq + alpha * err;
-                if (phase > kTwoPi)
-                    phase -= kTwoPi;
-                else if (phase < 0.0)
-                    phase += kTwoPi;
-                pp += p * p;
-                es += fabs(err);
-                double u = phase * kInvPi;  // sin(2 phase) = sin(2 pi frac(phase / pi))
-                u -= floor(u);
-                tt[j] = (float)u;
-            }
+    float pv = 0.0f;
+    double keep = 0.0;
+    // One PLL step (streamer.go:1588-1611) on the pilot sample in lane j of pv.
+    auto step = [&](int j) {
+        const double p = (double)__int_as_float(__builtin_amdgcn_readlane(__float_as_int(pv), j));
+        const float ph_t = (float)(phase * kInvTwoPi);  // phase in [0, 2pi] -> [0, 1] turns
+        const double sn = (double)__builtin_amdgcn_sinf(ph_t);
+        const double cs = (double)__builtin_amdgcn_cosf(ph_t);
+        is += lpa * (p * cs - is);
+        qs += lpa * (p * -sn - qs);
+        const double err = (double)pll_atan2f((float)qs, (float)is);
+        freq += beta * err;
+        // clamp to [min_f, max_f] with the bare max/min instructions (fmax and
+        // fmin would add canonicalising copies of the bounds to every step)
+        asm("v_max_f64 %0, %1, %2" : "=v"(freq) : "v"(freq), "v"(min_f));
+        asm("v_min_f64 %0, %1, %2" : "=v"(freq) : "v"(freq), "v"(max_f));
+        // phase += step, then -2pi if above 2pi, else +2pi if below 0: one
+        // FMA with k in {1, -1, 0} rounds as the subtraction or addition does
+        const double ph = phase + (freq + alpha * err);
+        const double k = ph > kTwoPi ? 1.0 : (ph < 0.0 ? -1.0 : 0.0);
+        phase = fma(-kTwoPi, k, ph);
+        es += fabs(err);
+        keep = lane == j ? phase : keep;
+    };
+    float pv_next = lane < n ? pilot[lane] : 0.0f;
+    // Land the first tile here: the inner loop then enters with only the
+    // next tile's prefetch in flight, which it does not read.
+    __builtin_amdgcn_s_waitcnt(0);
+    for (int i0 = 0; i0 < n; i0 += 64) {
+        const int cnt = min(64, n - i0);
+        pv = pv_next;  // 0 beyond the end
+        pv_next = i0 + 64 + lane < n ? pilot[i0 + 64 + lane] : 0.0f;
+        pp_l += (double)pv * (double)pv;
+        if (cnt == 64) {
+#pragma unroll 8
+            for (int j = 0; j < 64; j++) step(j);
+        } else {
+            for (int j = 0; j < cnt; j++) step(j);
         }
-        __syncthreads();
-        for (int j = threadIdx.x; j < cnt; j += blockDim.x) t2[i0 + j] = tt[j];
-        __syncthreads();
+        double u = keep * kInvPi;  // sin(2 phase) = sin(2 pi frac(phase / pi))
+        u -= floor(u);
+        if (lane < cnt) t2[i0 + lane] = (float)u;
     }
-    if (threadIdx.x == 0 && d.batch) {
+    double pp = pp_l;
+    for (int o = 32; o > 0; o >>= 1) pp += __shfl_xor(pp, o);
+    __syncthreads();  // tp
+    if (lane == 0 && d.batch) {
         // wfm_batch_stereo.go:139-145: one decision for the whole recording
         double total = 0.0;
-        for (int j = 0; j < (int)blockDim.x; j++) total += tp[j];
+        for (int j = 0; j < 64; j++) total += tp[j];
         const double ratio = total > 0.0 ? pp / total : 0.0;
         const double f_hz = freq * d.rate / kTwoPi;
         const int locked = ratio > 0.003 && fabs(f_hz - 19000.0) < 250.0 && es / (double)n < 0.35;
@@ -200,7 +246,7 @@ __device__ __forceinline__ void wfm_pll_one(const ArgusWfmDesc& d)
         b.fb_rd = 0;
         b.fb_wr = 0;
         *d.blk = b;
-    } else if (threadIdx.x == 0) {
+    } else if (lane == 0) {
         ArgusWfmState st = *sp;
         st.phase = phase;
         st.freq = freq;
