@@ -7,7 +7,9 @@
 //
 // Launches, all on the caller's stream:
 //
-//   argus_rds_symbols_kernel  one wavefront per call.  Mueller & Muller
+//   argus_rds_symbols_kernel  one wavefront per call; one call per
+//       workgroup up to 256 calls, then up to four per workgroup (one per
+//       SIMD of a CU).  Mueller & Muller
 //       timing with linear interpolation, the Costas loop on each symbol as
 //       it is produced, sign(I) hard bits and the differential bits (carried
 //       across calls).  The loop is a serial recurrence, run by lane 0 over
@@ -134,17 +136,42 @@ __device__ char rds_printable(unsigned v) { return (v < 32 || v > 126) ? ' ' : (
 
 // ---------------------------------------------------------------- kernels
 
-constexpr int kStage = 1024;  // samples staged in LDS per pass of the symbol loop
+constexpr int kStage = 1024;     // samples staged in LDS per pass of the symbol loop
+constexpr int kSymWaves = 4;     // at most this many calls per workgroup, one wavefront each
+constexpr int kSymCUs = 256;     // MI355X compute units
 
-__global__ __launch_bounds__(64) void argus_rds_symbols_kernel(const ArgusRdsCall* __restrict__ calls)
+// The staging of a wave's own LDS region is read by the same wave only: a
+// wavefront-scope fence and a wave barrier order it (no workgroup barrier,
+// so the waves of a workgroup run their calls independently).
+__device__ __forceinline__ void rds_wave_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// blockDim.x / 64 calls per workgroup (1 to kSymWaves). The loop is bound
+// by its CU's instruction issue (much of it scalar), so up to one call per CU
+// the launch gives each call a workgroup of its own (the dispatcher spreads
+// them over the CUs). Beyond that it packs calls into workgroups of up to
+// four waves: one workgroup's waves go to the four SIMDs of one CU, which
+// keeps the CUs evenly loaded, where many single-wave workgroups stacked
+// several calls on some SIMDs.
+__global__ __launch_bounds__(64 * kSymWaves) void argus_rds_symbols_kernel(const ArgusRdsCall* __restrict__ calls,
+                                                                          int n_calls)
 {
 #pragma clang fp contract(off)
-    __shared__ float2 xs[kStage];
-    __shared__ int next_stage;  // first sample of the next pass, or -1
+    __shared__ float2 xs_all[kSymWaves][kStage];
+    __shared__ int next_all[kSymWaves];  // first sample of the wave's next pass, or -1
+    const int wave = threadIdx.x >> 6;
+    const int ci = blockIdx.x * (int)(blockDim.x >> 6) + wave;
+    if (ci >= n_calls) return;  // whole waves only; no workgroup barrier follows
+    float2* xs = xs_all[wave];
+    int& next_stage = next_all[wave];
     // by value: the loop's byte stores to the bits could alias a reference
     // into the table and force a reload per symbol
-    const ArgusRdsCall call = calls[blockIdx.x];
-    const int lane = threadIdx.x;
+    const ArgusRdsCall call = calls[ci];
+    const int lane = threadIdx.x & 63;
     const int n = call.n, rate = call.rate, avail = call.avail, bit_cap = call.bit_cap;
     ArgusRdsProgress pr{};
     double sps = 0.0;
@@ -183,18 +210,21 @@ __global__ __launch_bounds__(64) void argus_rds_symbols_kernel(const ArgusRdsCal
             }
         }
     }
-    __syncthreads();
+    rds_wave_sync();
     const double alpha = 0.132, beta = alpha * alpha / 4.0;
     while (next_stage >= 0) {
         const int k0 = next_stage;
         const int cnt = min(kStage, avail - k0);
         for (int i = lane; i < cnt; i += 64) xs[i] = call.x[k0 + i];
-        __syncthreads();
+        rds_wave_sync();
         if (lane == 0) {
             const ArgusRdsState& st = *call.st;
             const bool carry = st.have_last_hard != 0;
             const int carried = st.last_hard;
-            uint8_t* bits = call.bits;
+            // a global (not generic) pointer: a flat store would count on the LDS
+            // counter, and the next symbol's LDS read would wait for it to land
+            typedef __attribute__((address_space(1))) uint8_t GlobalU8;
+            GlobalU8* bits = (GlobalU8*)call.bits;
             double m = pr.m, phase = pr.phase, freq = pr.freq;
             float prev_re = pr.prev_re, prev_im = pr.prev_im, prev_dec = pr.prev_dec;
             bool have_prev = pr.have_prev != 0;
@@ -263,7 +293,7 @@ __global__ __launch_bounds__(64) void argus_rds_symbols_kernel(const ArgusRdsCal
             if (done) pr.loop_done = 1;
             next_stage = restage;
         }
-        __syncthreads();
+        rds_wave_sync();
     }
     if (lane != 0) return;
     ArgusRdsState& st = *call.st;
@@ -499,7 +529,8 @@ int argus_launch_rds(const ArgusRdsCall* d_calls, int n_calls, int n_final, int 
     if (n_calls <= 0) return 0;
     if (n_calls > 65535 || n_final < 0 || n_final > n_calls) return -1;
     argus_clear_error();
-    argus_rds_symbols_kernel<<<n_calls, 64, 0, stream>>>(d_calls);
+    const int waves = std::min(kSymWaves, (n_calls + kSymCUs - 1) / kSymCUs);
+    argus_rds_symbols_kernel<<<(n_calls + waves - 1) / waves, 64 * waves, 0, stream>>>(d_calls, n_calls);
     ARGUS_CHECK(hipGetLastError());
     if (n_final == 0 || max_bit_cap < kGroupBits) return 0;
     dim3 grid((unsigned)((max_bit_cap + kThreads - 1) / kThreads), (unsigned)n_final);
