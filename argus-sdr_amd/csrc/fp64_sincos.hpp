@@ -20,7 +20,30 @@
 #define ARGUS_SC_HD
 #endif
 
-ARGUS_SC_HD inline void argus_sincos_wrapped(double x, double* sn, double* cs)
+// The constants, as a value a caller may keep in registers (a device loop
+// pins them in VGPRs once rather than rebuilding them per call).
+struct ArgusSinCosK {
+    double inv_half_pi, p1, p2;
+    double s[8];  // sin: coefficients of z^7 .. z^0 in the series after r + r^3 (...)
+    double c[8];  // cos: coefficients of z^7 .. z^0 in the series after z^2 (...)
+};
+
+ARGUS_SC_HD constexpr ArgusSinCosK argus_sincos_k()
+{
+    return ArgusSinCosK{0.6366197723675814,
+                        1.5707963267948966,
+                        6.123233995736766e-17,
+                        {2.8114572543455206e-15, -7.647163731819816e-13, 1.6059043836821613e-10,
+                         -2.505210838544172e-08, 2.7557319223985893e-06, -0.0001984126984126984,
+                         0.008333333333333333, -0.16666666666666666},
+                        {-1.5619206968586225e-16, 4.779477332387385e-14, -1.1470745597729725e-11,
+                         2.08767569878681e-09, -2.755731922398589e-07, 2.48015873015873e-05,
+                         -0.001388888888888889, 0.041666666666666664}};
+}
+
+// Every step rounds separately (no contraction); adding a negative
+// coefficient rounds as subtracting its magnitude.
+ARGUS_SC_HD inline void argus_sincos_wrapped_k(double x, const ArgusSinCosK& k, double* sn, double* cs)
 {
 #pragma clang fp contract(off)
     if (!(fabs(x) <= 4.0)) {
@@ -28,34 +51,25 @@ ARGUS_SC_HD inline void argus_sincos_wrapped(double x, double* sn, double* cs)
         *cs = cos(x);
         return;
     }
-    const double P1 = 1.5707963267948966, P2 = 6.123233995736766e-17;
-    const double q = rint(x * 0.6366197723675814);
-    const double r = (x - q * P1) - q * P2;
+    const double q = rint(x * k.inv_half_pi);
+    const double r = (x - q * k.p1) - q * k.p2;
     const double z = r * r;
     // sin r = r + r^3 (-1/3! + z (1/5! - z (1/7! - ...)))
-    double ps = 2.8114572543455206e-15;
-    ps = ps * z - 7.647163731819816e-13;
-    ps = ps * z + 1.6059043836821613e-10;
-    ps = ps * z - 2.505210838544172e-08;
-    ps = ps * z + 2.7557319223985893e-06;
-    ps = ps * z - 0.0001984126984126984;
-    ps = ps * z + 0.008333333333333333;
-    ps = ps * z - 0.16666666666666666;
+    double ps = k.s[0];
+    for (int i = 1; i < 8; i++) ps = ps * z + k.s[i];
     const double s = r + (r * z) * ps;
     // cos r = 1 - z/2 + z^2 (1/4! - z (1/6! - ...))
-    double pc = -1.5619206968586225e-16;
-    pc = pc * z + 4.779477332387385e-14;
-    pc = pc * z - 1.1470745597729725e-11;
-    pc = pc * z + 2.08767569878681e-09;
-    pc = pc * z - 2.755731922398589e-07;
-    pc = pc * z + 2.48015873015873e-05;
-    pc = pc * z - 0.001388888888888889;
-    pc = pc * z + 0.041666666666666664;
+    double pc = k.c[0];
+    for (int i = 1; i < 8; i++) pc = pc * z + k.c[i];
     const double c = (1.0 - 0.5 * z) + (z * z) * pc;
-    switch (((int)q) & 3) {
-    case 0: *sn = s; *cs = c; break;
-    case 1: *sn = c; *cs = -s; break;
-    case 2: *sn = -s; *cs = -c; break;
-    default: *sn = -c; *cs = s; break;
-    }
+    // quadrant (q mod 4): (s, c), (c, -s), (-s, -c), (-c, s), as selects
+    const int qi = (int)q;
+    const double a = (qi & 1) ? c : s, b = (qi & 1) ? s : c;
+    *sn = (qi & 2) ? -a : a;
+    *cs = ((qi + 1) & 2) ? -b : b;
+}
+
+ARGUS_SC_HD inline void argus_sincos_wrapped(double x, double* sn, double* cs)
+{
+    argus_sincos_wrapped_k(x, argus_sincos_k(), sn, cs);
 }

@@ -12,8 +12,9 @@
 //       SIMD of a CU).  Mueller & Muller
 //       timing with linear interpolation, the Costas loop on each symbol as
 //       it is produced, sign(I) hard bits and the differential bits (carried
-//       across calls).  The loop is a serial recurrence, run by lane 0 over
-//       samples the wavefront stages in LDS 1024 at a time (the next sample
+//       across calls).  The loop is a serial recurrence, run by every lane
+//       alike (uniform values, scalar branches) over samples the wavefront
+//       stages in LDS 1024 at a time (the next sample
 //       index depends on the previous symbol, so a global load per symbol
 //       would put HBM latency on the recurrence).  A call may cover a window
 //       whose samples arrive over several launches: the loop pauses where it
@@ -150,6 +151,16 @@ __device__ __forceinline__ void rds_wave_sync()
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// Lane 0's value of a double that every lane holds alike, as a uniform
+// value (SGPRs), so that what follows from it branches on scalars.
+__device__ __forceinline__ double rds_uniform(double v)
+{
+    const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+    const unsigned lo = (unsigned)__builtin_amdgcn_readfirstlane((int)(unsigned)b);
+    const unsigned hi = (unsigned)__builtin_amdgcn_readfirstlane((int)(unsigned)(b >> 32));
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
 // blockDim.x / 64 calls per workgroup (1 to kSymWaves). The loop is bound
 // by its CU's instruction issue (much of it scalar), so up to one call per CU
 // the launch gives each call a workgroup of its own (the dispatcher spreads
@@ -161,24 +172,31 @@ __global__ __launch_bounds__(64 * kSymWaves) void argus_rds_symbols_kernel(const
                                                                           int n_calls)
 {
 #pragma clang fp contract(off)
+    // What lane 0 works out at the call's start, for the whole wave.
+    struct Start {
+        ArgusRdsProgress pr;
+        double sps;
+        int next_stage;  // first sample of the first pass, or -1
+        int carry, carried;
+    };
     __shared__ float2 xs_all[kSymWaves][kStage];
-    __shared__ int next_all[kSymWaves];  // first sample of the wave's next pass, or -1
-    const int wave = threadIdx.x >> 6;
+    __shared__ Start start_all[kSymWaves];
+    // wave-uniform (readfirstlane): the call, its LDS and the whole
+    // recurrence are then uniform values and the loop's branches scalar
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int ci = blockIdx.x * (int)(blockDim.x >> 6) + wave;
     if (ci >= n_calls) return;  // whole waves only; no workgroup barrier follows
     float2* xs = xs_all[wave];
-    int& next_stage = next_all[wave];
     // by value: the loop's byte stores to the bits could alias a reference
     // into the table and force a reload per symbol
     const ArgusRdsCall call = calls[ci];
     const int lane = threadIdx.x & 63;
     const int n = call.n, rate = call.rate, avail = call.avail, bit_cap = call.bit_cap;
-    ArgusRdsProgress pr{};
-    double sps = 0.0;
     if (lane == 0) {
-        pr = *call.pr;
+        ArgusRdsProgress pr = *call.pr;
         ArgusRdsState& st = *call.st;
-        next_stage = -1;
+        double sps = 0.0;
+        int next_stage = -1;
         if (n >= kGroupBits && rate > 0) {  // rds.go:79-81
             sps = (double)rate / 1187.5;
             if (!pr.active) {  // the call starts: rds.go:84-110
@@ -209,91 +227,105 @@ __global__ __launch_bounds__(64 * kSymWaves) void argus_rds_symbols_kernel(const
                 if (k >= 0 && k + 1 < avail) next_stage = k;
             }
         }
+        // the bit carried into this call (unchanged until the call ends)
+        start_all[wave] = Start{pr, sps, next_stage, st.have_last_hard != 0, st.last_hard};
     }
     rds_wave_sync();
+    const Start& s0 = start_all[wave];
+    ArgusRdsProgress pr = s0.pr;
+    const double sps = s0.sps;
+    int next_stage = s0.next_stage;
+    const bool carry = s0.carry != 0;
+    const int carried = s0.carried;
     const double alpha = 0.132, beta = alpha * alpha / 4.0;
+    // The sine/cosine constants held in VGPRs for the whole loop (an empty asm
+    // hides their values, so the compiler cannot rebuild them in SGPR pairs
+    // at every symbol).
+    ArgusSinCosK sck = argus_sincos_k();
+    asm volatile("" : "+v"(sck.inv_half_pi), "+v"(sck.p1), "+v"(sck.p2));
+#pragma unroll
+    for (int i = 0; i < 8; i++) asm volatile("" : "+v"(sck.s[i]), "+v"(sck.c[i]));
+    // a global (not generic) pointer: a flat store would count on the LDS
+    // counter, and the next symbol's LDS read would wait for it to land
+    typedef __attribute__((address_space(1))) uint8_t GlobalU8;
+    GlobalU8* bits = (GlobalU8*)call.bits;
+    // Every lane runs the same recurrence on the same values; the bit stores
+    // are the same byte from every lane.
     while (next_stage >= 0) {
         const int k0 = next_stage;
         const int cnt = min(kStage, avail - k0);
+        rds_wave_sync();  // the previous pass is done reading xs
         for (int i = lane; i < cnt; i += 64) xs[i] = call.x[k0 + i];
         rds_wave_sync();
-        if (lane == 0) {
-            const ArgusRdsState& st = *call.st;
-            const bool carry = st.have_last_hard != 0;
-            const int carried = st.last_hard;
-            // a global (not generic) pointer: a flat store would count on the LDS
-            // counter, and the next symbol's LDS read would wait for it to land
-            typedef __attribute__((address_space(1))) uint8_t GlobalU8;
-            GlobalU8* bits = (GlobalU8*)call.bits;
-            double m = pr.m, phase = pr.phase, freq = pr.freq;
-            float prev_re = pr.prev_re, prev_im = pr.prev_im, prev_dec = pr.prev_dec;
-            bool have_prev = pr.have_prev != 0;
-            int ns = pr.ns, nb = pr.nb, prev_hard = pr.prev_hard;
-            int restage = -1;
-            bool done = true;  // the loop condition ended the call
-            while (m < (double)(n - 1)) {
-                const int k = (int)m;
-                const double f = m - (double)k;
-                if (k + 1 >= n || k < 0 || ns >= bit_cap) break;
-                if (k + 1 >= avail) {  // wait for the window's next samples
-                    done = false;
-                    break;
-                }
-                if (k < k0 || k + 1 >= k0 + cnt) {  // outside this pass
-                    restage = k;
-                    done = false;
-                    break;
-                }
-                const float2 a = xs[k - k0], b = xs[k + 1 - k0];
-                const float re = (float)((double)a.x * (1 - f) + (double)b.x * f);
-                const float im = (float)((double)a.y * (1 - f) + (double)b.y * f);
-                const float dec = re >= 0 ? 1.0f : -1.0f;
-                if (have_prev) {
-                    const double e =
-                        (double)(dec - prev_dec) * (double)prev_re - (double)(re - prev_re) * (double)prev_dec;
-                    m += sps + 0.01 * e;
-                } else {
-                    m += sps;
-                    have_prev = true;
-                }
-                prev_dec = dec;
-                prev_re = re;
-                prev_im = im;
-                // Costas de-rotation of this symbol (rds.go:157-183)
-                double sd, cd;
-                argus_sincos_wrapped(phase, &sd, &cd);  // phase stays in [-pi, pi]
-                const float cp = (float)cd, sp = (float)sd;
-                const float sre = re * cp + im * sp;
-                const float sim = im * cp - re * sp;
-                const double err = sre >= 0 ? (double)sim : -(double)sim;
-                freq += beta * err;
-                phase += freq + alpha * err;
-                while (phase > M_PI) phase -= 2 * M_PI;
-                while (phase < -M_PI) phase += 2 * M_PI;
-                const int hard = sre >= 0 ? 1 : 0;
-                // differential bits, carried across calls (rds.go:214-233)
-                if (ns == 0) {
-                    if (carry) bits[nb++] = (uint8_t)(hard ^ carried);
-                } else {
-                    bits[nb++] = (uint8_t)(hard ^ prev_hard);
-                }
-                prev_hard = hard;
-                ns++;
+        double m = pr.m, phase = pr.phase, freq = pr.freq;
+        float prev_re = pr.prev_re, prev_im = pr.prev_im, prev_dec = pr.prev_dec;
+        bool have_prev = pr.have_prev != 0;
+        int ns = pr.ns, nb = pr.nb, prev_hard = pr.prev_hard;
+        int restage = -1;
+        bool done = true;  // the loop condition ended the call
+        while (m < (double)(n - 1)) {
+            const int k = (int)m;
+            const double f = m - (double)k;
+            if (k + 1 >= n || k < 0 || ns >= bit_cap) break;
+            if (k + 1 >= avail) {  // wait for the window's next samples
+                done = false;
+                break;
             }
-            pr.m = m;
-            pr.phase = phase;
-            pr.freq = freq;
-            pr.prev_re = prev_re;
-            pr.prev_im = prev_im;
-            pr.prev_dec = prev_dec;
-            pr.have_prev = have_prev ? 1 : 0;
-            pr.ns = ns;
-            pr.nb = nb;
-            pr.prev_hard = prev_hard;
-            if (done) pr.loop_done = 1;
-            next_stage = restage;
+            if (k < k0 || k + 1 >= k0 + cnt) {  // outside this pass
+                restage = k;
+                done = false;
+                break;
+            }
+            const float2 a = xs[k - k0], b = xs[k + 1 - k0];
+            const float re = (float)((double)a.x * (1 - f) + (double)b.x * f);
+            const float im = (float)((double)a.y * (1 - f) + (double)b.y * f);
+            const float dec = re >= 0 ? 1.0f : -1.0f;
+            if (have_prev) {
+                const double e =
+                    (double)(dec - prev_dec) * (double)prev_re - (double)(re - prev_re) * (double)prev_dec;
+                m += sps + 0.01 * e;
+            } else {
+                m += sps;
+                have_prev = true;
+            }
+            prev_dec = dec;
+            prev_re = re;
+            prev_im = im;
+            // Costas de-rotation of this symbol (rds.go:157-183)
+            double sd, cd;
+            argus_sincos_wrapped_k(phase, sck, &sd, &cd);  // phase stays in [-pi, pi]
+            sd = rds_uniform(sd);  // computed from the pinned constants: uniform again
+            cd = rds_uniform(cd);
+            const float cp = (float)cd, sp = (float)sd;
+            const float sre = re * cp + im * sp;
+            const float sim = im * cp - re * sp;
+            const double err = sre >= 0 ? (double)sim : -(double)sim;
+            freq += beta * err;
+            phase += freq + alpha * err;
+            while (phase > M_PI) phase -= 2 * M_PI;
+            while (phase < -M_PI) phase += 2 * M_PI;
+            const int hard = sre >= 0 ? 1 : 0;
+            // differential bits, carried across calls (rds.go:214-233)
+            if (ns == 0) {
+                if (carry) bits[nb++] = (uint8_t)(hard ^ carried);
+            } else {
+                bits[nb++] = (uint8_t)(hard ^ prev_hard);
+            }
+            prev_hard = hard;
+            ns++;
         }
-        rds_wave_sync();
+        pr.m = m;
+        pr.phase = phase;
+        pr.freq = freq;
+        pr.prev_re = prev_re;
+        pr.prev_im = prev_im;
+        pr.prev_dec = prev_dec;
+        pr.have_prev = have_prev ? 1 : 0;
+        pr.ns = ns;
+        pr.nb = nb;
+        pr.prev_hard = prev_hard;
+        if (done) pr.loop_done = 1;
+        next_stage = restage;
     }
     if (lane != 0) return;
     ArgusRdsState& st = *call.st;
