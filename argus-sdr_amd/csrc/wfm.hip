@@ -22,6 +22,7 @@
 // the PLL state and loop arithmetic are FP64 as in the reference, and the
 // oscillators use the hardware turn-based sine/cosine.
 #include "wfm.hpp"
+#include "pll_atan2.hpp"
 #include "runner.hpp"
 
 namespace {
@@ -50,31 +51,11 @@ __device__ __forceinline__ float fb_at(const ArgusWfmDesc& d, int fb_rd, int idx
     return idx >= 0 ? d.mono[idx] : d.fb_hist[fb_rd * ARGUS_WFM_HIST + ARGUS_WFM_HIST + idx];
 }
 
-// atan2 of the PLL's phase detector: the octant ratio a = min/max through
-// the hardware reciprocal (the denominator floored at the smallest normal,
-// so 0/0 gives a = 0), atan(a) on [0, 1] as a + a s P(s) (s = a^2, P of
-// degree 6 in Horner form, coefficients from a weighted least-squares fit
-// to atan made for this kernel: 1.2e-7 rad max error in FP32), then the
-// octant and quadrant reflections and the sign of y: 3.2e-7 rad max error
-// over all quadrants, and atan2(+-0, +0) = +-0, atan2(+-0, -0) = +-pi as in
-// IEEE atan2.
+// atan2 of the PLL's phase detector (pll_atan2.hpp) on the hardware
+// reciprocal.
 __device__ __forceinline__ float pll_atan2f(float y, float x)
 {
-    const float ax = fabsf(x), ay = fabsf(y);
-    const float mx = fmaxf(fmaxf(ax, ay), 1.17549435e-38f), mn = fminf(ax, ay);
-    const float a = mn * __builtin_amdgcn_rcpf(mx);
-    const float s = a * a;
-    float q = -4.355480428785086e-3f;
-    q = fmaf(q, s, 2.3040393367409706e-2f);
-    q = fmaf(q, s, -5.7773932814598083e-2f);
-    q = fmaf(q, s, 9.794257581233978e-2f);
-    q = fmaf(q, s, -1.397659033536911e-1f);
-    q = fmaf(q, s, 1.9962705671787262e-1f);
-    q = fmaf(q, s, -3.333165943622589e-1f);
-    float r = fmaf(a * s, q, a);
-    r = ay > ax ? 1.5707963267948966f - r : r;
-    r = __builtin_signbit(x) ? 3.1415926535897932f - r : r;
-    return copysignf(r, y);
+    return argus_pll_atan2f(y, x, [](float v) { return __builtin_amdgcn_rcpf(v); });
 }
 
 }  // namespace
