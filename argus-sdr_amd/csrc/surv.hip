@@ -156,12 +156,23 @@ __global__ __launch_bounds__(256) void argus_spectrum_db_kernel(const float2* __
 //
 // One workgroup per 256 outputs of a frame: the input span they read
 // (256 * decim + taps - 1 samples, zero before the window's start, which
-// adds nothing to an accumulation exactly as ApplyFIR's early break) and the
-// taps are staged in LDS, then each thread accumulates its output over the
-// taps in order.
+// adds nothing to an accumulation exactly as ApplyFIR's early break),
+// converted to FP64 once, and the taps are staged in LDS (dynamic, sized to
+// the span: level_fir_lds_bytes), then each thread accumulates its output
+// over the taps.
 constexpr int kLevelOut = 256;
 constexpr int kLevelMaxTaps = 128;
 constexpr int kLevelMaxDecim = 16;
+
+// Positions per residue plane of the span, and the kernel's LDS bytes.
+static inline int level_fir_planes(int decim, int n_taps)
+{
+    return (kLevelOut * decim + n_taps - 1 + decim - 1) / decim;
+}
+static inline size_t level_fir_lds_bytes(int decim, int n_taps)
+{
+    return (size_t)level_fir_planes(decim, n_taps) * decim * sizeof(double2) + (size_t)n_taps * sizeof(double);
+}
 
 __global__ __launch_bounds__(kLevelOut) void argus_level_fir_kernel(const float2* __restrict__ in, int64_t frame_step,
                                                                     int decim, int n_d,
@@ -170,10 +181,10 @@ __global__ __launch_bounds__(kLevelOut) void argus_level_fir_kernel(const float2
                                                                     float2* __restrict__ z)
 {
 #pragma clang fp contract(off)
-    // the span by polyphase residue: sample j (input base + j) at plane
-    // j mod decim, position j / decim, planes P positions apart
-    __shared__ float2 xs[kLevelOut * kLevelMaxDecim + kLevelMaxTaps + kLevelMaxDecim];
-    __shared__ double hs[kLevelMaxTaps];
+    // the span by polyphase residue, in FP64: sample j (input base + j) at
+    // plane j mod decim, position j / decim, planes P positions apart; then
+    // the taps
+    extern __shared__ double2 level_lds[];
     const int f = blockIdx.y;
     const int m0 = blockIdx.x * kLevelOut;
     const float2* x = in + (int64_t)f * frame_step;
@@ -181,10 +192,13 @@ __global__ __launch_bounds__(kLevelOut) void argus_level_fir_kernel(const float2
     const int span = kLevelOut * decim + n_taps - 1;
     const int ld = __builtin_ctz(decim);  // decim is a power of two
     const int P = (span + decim - 1) >> ld;
+    double2* xs = level_lds;
+    double* hs = (double*)(level_lds + (size_t)P * decim);
     for (int j = threadIdx.x; j < span; j += kLevelOut) {
         const int idx = base + j;
         const int lim = n_d * decim;
-        xs[(j & (decim - 1)) * P + (j >> ld)] = (idx >= 0 && idx < lim) ? x[idx] : make_float2(0.0f, 0.0f);
+        const float2 v = (idx >= 0 && idx < lim) ? x[idx] : make_float2(0.0f, 0.0f);
+        xs[(j & (decim - 1)) * P + (j >> ld)] = make_double2((double)v.x, (double)v.y);
     }
     // the taps grouped the same way: residue rho's taps k = k0(rho) + i decim
     // (k0 = (T - 1 - rho) mod decim) at hs[off(rho) + i], off the count of
@@ -214,13 +228,13 @@ __global__ __launch_bounds__(kLevelOut) void argus_level_fir_kernel(const float2
         const int k0 = (n_taps - 1 - rho) & (decim - 1);  // first tap of this residue
         if (k0 >= n_taps) continue;
         const int cnt = (n_taps - 1 - k0) / decim + 1;
-        const float2* xp = xs + rho * P + (int)threadIdx.x + ((n_taps - 1 - k0 - rho) >> ld);
+        const double2* xp = xs + rho * P + (int)threadIdx.x + ((n_taps - 1 - k0 - rho) >> ld);
 #pragma unroll 4
         for (int i = 0; i < cnt; i++) {
-            const float2 v = xp[-i];
+            const double2 v = xp[-i];
             const double w = hr[i];
-            ar += (double)v.x * w;
-            ai += (double)v.y * w;
+            ar += v.x * w;
+            ai += v.y * w;
         }
         hr += cnt;
     }
@@ -795,8 +809,8 @@ GPUD_API int gpud_surv_level_spectrum_device(gpud_surv* s, const gpud_float2* in
     if (!err) err = grow_scratch(&s->d_scratch, &s->scratch_cap, need);
     if (err) return err;
     argus_clear_error();
-    hipLaunchKernelGGL(argus_level_fir_kernel, dim3((n_d + kLevelOut - 1) / kLevelOut, n_frames), dim3(kLevelOut), 0,
-                       s->stream,
+    hipLaunchKernelGGL(argus_level_fir_kernel, dim3((n_d + kLevelOut - 1) / kLevelOut, n_frames), dim3(kLevelOut),
+                       level_fir_lds_bytes(decim, s->level_taps), s->stream,
                        (const float2*)in_dev, frame_step, decim, n_d, (const double*)s->d_ltaps, s->level_taps,
                        (const float*)s->d_lwin, s->d_zstuff);
     err = (int)hipGetLastError();

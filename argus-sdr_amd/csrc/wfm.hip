@@ -19,8 +19,9 @@
 //   LPF7.5k (real taps on I and Q), keep stream samples c = 0 mod decim.
 //
 // Arithmetic: FIRs accumulate in FP32 (the reference accumulates in FP64),
-// the PLL state and loop arithmetic are FP64 as in the reference, and the
-// oscillators use the hardware turn-based sine/cosine.
+// the PLL state and loop arithmetic are FP64 as in the reference, the
+// oscillators use the hardware turn-based sine/cosine, and the phase
+// detector's atan2 is the FP32 polynomial of pll_atan2.hpp (3.2e-7 rad).
 #include "wfm.hpp"
 #include "pll_atan2.hpp"
 #include "runner.hpp"
