@@ -104,3 +104,30 @@ def test_batch_short_bad_and_reset(wfm_mod):
     assert dev.PI(2) == 0 and dev.Groups(2).shape == (0, 4) and dev.PS(2) == ""
     # a fresh decoder again: the same call finds the same groups
     assert dev.Decode([(2, x, rate)]) == [g]
+
+
+@pytest.mark.parametrize("n_st", [257, 301, 771])
+def test_batch_more_stations_than_cus(wfm_mod, n_st):
+    """More calls than compute units: the symbol kernel packs two to four
+    calls per workgroup, the last workgroup partly filled, and the waves of
+    one workgroup run windows of different lengths.  Every station's groups
+    equal the host decoder's, call by call."""
+    base = [_baseband(250_000, 1.0, seed, pi, ps, rt, 56, noise)
+            for pi, ps, rt, seed, noise, _ in STATIONS]
+    dev = wfm_mod.RDSBatchDecoder(n_st)
+    host = [wfm_mod.RDSDecoder() for _ in range(n_st)]
+    for step in range(2):
+        calls = []
+        for k in range(n_st):
+            x, rate = base[k % len(base)]
+            cut = len(x) // 2 - (k % 7) * 13
+            calls.append((k, x[:cut] if step == 0 else x[cut:], rate))
+        print(f"{n_st} stations, call {step}", flush=True)
+        got = dev.Decode(calls)
+        for (k, x, rate), g in zip(calls, got):
+            assert g == host[k].Decode(x, rate), (n_st, step, k)
+    for k in range(n_st):
+        want = np.asarray(host[k].Groups(), np.uint16).reshape(-1, 4)
+        np.testing.assert_array_equal(dev.Groups(k), want)
+        assert dev.PI(k) == host[k].PI, k
+    assert len(dev.Groups(0)) > 0
