@@ -107,6 +107,23 @@ __device__ __forceinline__ void wave_sync()
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// Global (not generic) pointers for the kernels' HBM accesses: a flat access
+// counts on the LDS counter too, so the waves' LDS waits (fold, transposes)
+// would also wait for segment-spectrum loads and output stores in flight.
+#if defined(__HIP_DEVICE_COMPILE__)
+typedef __attribute__((address_space(1))) const float2 GlobalCF2;
+typedef __attribute__((address_space(1))) const float4 GlobalCF4;
+typedef __attribute__((address_space(1))) float2 GlobalF2;
+typedef __attribute__((address_space(1))) float GlobalF;
+typedef const __attribute__((address_space(4))) ArgusFastDesc CFastDesc;
+#else  // the host pass only needs the kernels to compile
+typedef const float2 GlobalCF2;
+typedef const float4 GlobalCF4;
+typedef float2 GlobalF2;
+typedef float GlobalF;
+typedef const ArgusFastDesc CFastDesc;
+#endif
+
 // Per-lane twiddles of the 512-point transform (functions of the lane only):
 // pass 1 W_512^{t k}, pass 2 W_64^{(t mod 8) k}, k = 1..7.
 struct Tw512 {
@@ -172,7 +189,7 @@ __device__ __forceinline__ void wave_fft512(float2 (&v)[8], float* xs, int t, co
 // c < J / 64: every lane folds the same number of bins, all loads of one
 // i issued together.
 template <int DP>
-__device__ __forceinline__ void fast_fold(const float2* __restrict__ xseg, const float2* gs, int t,
+__device__ __forceinline__ void fast_fold(const GlobalCF2* __restrict__ xseg, const float2* gs, int t,
                                           float2 (&f)[ARGUS_FAST_N / DP / 64])
 {
     constexpr int J = ARGUS_FAST_N / DP, PER = J / 64;
@@ -200,7 +217,7 @@ __device__ __forceinline__ void fast_fold(const float2* __restrict__ xseg, const
 // instructions, which bound this kernel.  f[c] holds F[j] in .xy and
 // F[j + 1] in .zw.
 template <int DP>
-__device__ __forceinline__ void fast_fold_pairs(const float2* __restrict__ xseg, const float2* gs, int t,
+__device__ __forceinline__ void fast_fold_pairs(const GlobalCF2* __restrict__ xseg, const float2* gs, int t,
                                                 float4 (&f)[ARGUS_FAST_N / DP / 128])
 {
     constexpr int J = ARGUS_FAST_N / DP, P = J / 128;
@@ -211,7 +228,7 @@ __device__ __forceinline__ void fast_fold_pairs(const float2* __restrict__ xseg,
     for (int i = 0; i < DP; i++) {
         float4 xv[P];
 #pragma unroll
-        for (int c = 0; c < P; c++) xv[c] = *(const float4*)(xseg + i * J + 2 * t + 128 * c);
+        for (int c = 0; c < P; c++) xv[c] = *(const GlobalCF4*)(xseg + i * J + 2 * t + 128 * c);
 #pragma unroll
         for (int c = 0; c < P; c++) {
             const float4 gv = *(const float4*)(gs + i * J + 2 * t + 128 * c);
@@ -238,26 +255,28 @@ __host__ __device__ constexpr int fast_pack()
 // left to the demodulation pass.
 __device__ __forceinline__ void fast_emit(const ArgusFastDesc& d, const float2* os, int m_lo, int cnt, int t)
 {
+    GlobalF2* out = (GlobalF2*)d.out;
+    GlobalF* audio = (GlobalF*)d.audio;
     for (int i = t; i < cnt; i += 64) {
         const int m = m_lo + i;
         const float2 y = os[i];
-        d.out[m] = y;
+        out[m] = y;
         switch (d.demod) {  // uniform: one channel per workgroup
         case ARGUS_DEMOD_NFM:
         case ARGUS_DEMOD_WFM:
-            if (i > 0) d.audio[m - d.skip] = argus_fm_step(os[i - 1], y);
+            if (i > 0) audio[m - d.skip] = argus_fm_step(os[i - 1], y);
             break;
         case ARGUS_DEMOD_USB:
         case ARGUS_DEMOD_LSB: {
             float sn, cn;
             argus_nco(d.bfo0 + d.bfo_inc * (uint64_t)(uint32_t)m, sn, cn);
-            d.audio[m] = argus_ssb_out(y, sn, cn);
+            audio[m] = argus_ssb_out(y, sn, cn);
             break;
         }
         case ARGUS_DEMOD_CW: {
             float sn, cn;
             argus_nco(d.bfo0 + d.bfo_inc * (uint64_t)(uint32_t)m, sn, cn);
-            d.audio[m] = y.x * cn;
+            audio[m] = y.x * cn;
             break;
         }
         default:
@@ -312,7 +331,7 @@ __device__ __forceinline__ int fast_div(int a, int d, float recip)
 // transform's strided reads of the regions in distinct banks (64 banks of
 // 4 bytes, 32 lanes per cycle).
 template <int DP>
-__device__ __forceinline__ void fast_group(const ArgusFastDesc& d, const FastOut& fo, const float2* __restrict__ X,
+__device__ __forceinline__ void fast_group(const ArgusFastDesc& d, const FastOut& fo, const GlobalCF2* __restrict__ X,
                                            const float2* gs, float* xs, int sb, int s_end, int t, const Tw512& tw)
 {
     constexpr int J = ARGUS_FAST_N / DP, PER = J / 64, Z = fast_pack<DP>(), ZS = kJ / J / Z, KA = J / 64;
@@ -440,7 +459,7 @@ __device__ __forceinline__ void fast_group(const ArgusFastDesc& d, const FastOut
 }
 
 template <int DP>
-__device__ __forceinline__ void fast_block(const ArgusFastDesc& d, const float2* __restrict__ X, const float2* gs,
+__device__ __forceinline__ void fast_block(const ArgusFastDesc& d, const GlobalCF2* __restrict__ X, const float2* gs,
                                            float* xs, int s_begin, int s_end, int w, int t, const Tw512& tw)
 {
     constexpr int Z = fast_pack<DP>();
@@ -469,16 +488,18 @@ __global__ __launch_bounds__(kChanThreads) void argus_fast_chan_kernel(const Arg
     }
     const int blocks = (S + kSegItem - 1) / kSegItem;
     const int n_items = (c_end - c_begin) * blocks;
-    const float2* X = hdr->X;
+    const GlobalCF2* X = (const GlobalCF2*)hdr->X;
     Tw512 tw;
     tw512_init(tw, t);
     for (int it = blockIdx.x; it < n_items; it += gridDim.x) {
         const int ci = it / blocks, b = it - ci * blocks;
-        const ArgusFastDesc d = hdr->descs[c_begin + ci];
+        // scalar loads (constant address space): the descriptor is uniform and
+        // read-only here, and its fields then live in SGPRs
+        const ArgusFastDesc d = ((const CFastDesc*)hdr->descs)[c_begin + ci];
         __syncthreads();  // the previous item's waves are done with gs
         // G with the ramp e^{j 2pi k r / N}: exact turn fractions
         for (int k = threadIdx.x; k < kN; k += kChanThreads) {
-            const float2 g = d.G[k];
+            const float2 g = *(GlobalCF2*)(d.G + k);
             const float x = (float)((k * d.r) & (kN - 1)) * (1.0f / kN);
             const float sn = __builtin_amdgcn_sinf(x), cs = __builtin_amdgcn_cosf(x);
             gs[k] = make_float2(g.x * cs - g.y * sn, g.x * sn + g.y * cs);
